@@ -1,0 +1,269 @@
+/*
+ * vgpu.h -- the thin C ABI between vectra's C11 operator nodes and the
+ * hand-written gfx950 kernels (libvgpu.so).  Plain pointers and sizes only:
+ * every buffer argument documented "device" is device memory obtained from
+ * vgpu_malloc (or any HIP allocation on the current device); every other
+ * pointer is host memory.  All calls are synchronous on return unless they
+ * say otherwise, run on the library's own stream, and report failure as a
+ * negative status with the message in vgpu_last_error().
+ *
+ * Each entry point stands in for a reference loop (gcol33/vectra 0.2.2):
+ *   vgpu_filter ............ src/filter.c:13-39 apply_mask_sel over
+ *                            src/expr.c:91-146 + src/scalar_ops.c:11-185
+ *   vgpu_groupby_* ......... src/group_agg.c:155-281 (hash path) and
+ *                            466-585 (sorted path), src/hash.c:80-165,
+ *                            src/agg_ops.c:93-255
+ *   vgpu_join_build ........ src/join.c:156-220 (jht_create/jht_insert :23-61)
+ *   vgpu_join_probe ........ src/join.c:262-480 (jht_probe :115-135,
+ *                            jht_chain_next :137-150)
+ *   vgpu_gather ............ the per-value builder appends of
+ *                            src/join.c:385-448 (src/builder.c:110-138)
+ *   vgpu_groupby_feed_join . the fused form of Join -> Project('*') ->
+ *                            GroupAgg (src/join.c:262-480,
+ *                            src/project.c:51-252, src/group_agg.c)
+ */
+#ifndef VGPU_H
+#define VGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* column types use vectra's VecType numbering */
+#define VGPU_INT64  0
+#define VGPU_DOUBLE 1
+#define VGPU_BOOL   2
+
+/* A device column: data has 8 bytes per row (int64/double) or 1 (bool);
+ * validity is an LSB-first bitmap (bit i of byte i/8 = row i valid) or NULL
+ * for all-valid. */
+typedef struct {
+    int32_t        type;
+    int32_t        reserved;
+    const void    *data;
+    const uint8_t *validity;
+} vgpu_col;
+
+/* ---- runtime --------------------------------------------------------- */
+int         vgpu_init(int device);
+const char *vgpu_last_error(void);
+void       *vgpu_malloc(size_t bytes);
+int         vgpu_free(void *ptr);
+int         vgpu_memcpy_h2d(void *dst_device, const void *src_host, size_t bytes);
+int         vgpu_memcpy_d2h(void *dst_host, const void *src_device, size_t bytes);
+int         vgpu_memset(void *dst_device, int value, size_t bytes);
+int         vgpu_synchronize(void);
+/* pinned host staging (hipHostMalloc); never handed to vectra consumers */
+void       *vgpu_host_alloc(size_t bytes);
+int         vgpu_host_free(void *ptr);
+
+/* ---- expression programs ---------------------------------------------
+ * A straight-line program in SSA form: instruction i writes register i and
+ * reads registers a and b (indices < i).  Types are the VGPU_* column
+ * types.  vectra's coercion rules (src/coerce.c:7-19) are applied by the
+ * compiler that emits the program; the device never coerces implicitly. */
+enum {
+    VGPU_OP_COL = 1,    /* load column `col` (type = column type)          */
+    VGPU_OP_CONST,      /* immediate, valid                                 */
+    VGPU_OP_NA,         /* NA of `type`                                     */
+    VGPU_OP_CAST,       /* a -> type (bool->int64, bool/int64->double)      */
+    VGPU_OP_ADD, VGPU_OP_SUB, VGPU_OP_MUL, VGPU_OP_DIV, VGPU_OP_MOD,
+    VGPU_OP_EQ, VGPU_OP_NE, VGPU_OP_LT, VGPU_OP_LE, VGPU_OP_GT, VGPU_OP_GE,
+    VGPU_OP_AND, VGPU_OP_OR, VGPU_OP_NOT,
+    VGPU_OP_ISNA,
+    VGPU_OP_NEG
+};
+
+typedef struct {
+    uint8_t op;
+    uint8_t type;     /* result type for COL/CONST/NA/CAST/arith/NEG,
+                         operand type for comparisons */
+    uint8_t a, b;     /* operand registers */
+    int32_t col;      /* column slot for VGPU_OP_COL */
+    union { int64_t i; double d; } imm;
+} vgpu_insn;
+
+#define VGPU_MAX_INSN 32
+#define VGPU_MAX_COLS 8
+
+typedef struct {
+    int32_t   n_insn;   /* result is register n_insn - 1 */
+    int32_t   reserved;
+    vgpu_insn insn[VGPU_MAX_INSN];
+} vgpu_prog;
+
+/* ---- filter -> selection vector -------------------------------------
+ * Evaluates prog over the logical rows of a batch (physical rows
+ * 0..n_rows-1, or sel_in[0..n_sel_in-1] when sel_in is non-NULL) and
+ * writes to sel_out (device, capacity >= logical rows) the ascending
+ * physical indices whose result is valid and true; *n_out gets the count
+ * (host).  cols (host array of n_cols device columns) are the program's
+ * column slots. */
+int vgpu_filter(const vgpu_prog *prog, const vgpu_col *cols, int n_cols,
+                int64_t n_rows, const int32_t *sel_in, int64_t n_sel_in,
+                int32_t *sel_out, int64_t *n_out);
+
+/* ---- group-by --------------------------------------------------------- */
+enum { VGPU_AGG_COUNT = 0, VGPU_AGG_COUNT_STAR, VGPU_AGG_SUM, VGPU_AGG_MEAN,
+       VGPU_AGG_MIN, VGPU_AGG_MAX };   /* vectra's AggKind numbering */
+
+#define VGPU_KEY_NONE (-1)   /* ungrouped summarise: one group if any row */
+
+enum { VGPU_ORDER_FIRST_SEEN = 0,   /* hash path: first occurrence       */
+       VGPU_ORDER_KEY = 1 };        /* sorted path: ascending, NA last    */
+
+typedef struct {
+    int32_t kind;     /* VGPU_AGG_* */
+    int32_t input;    /* input slot, -1 for COUNT_STAR */
+    int32_t na_rm;
+    int32_t reserved;
+} vgpu_agg;
+
+#define VGPU_MAX_INPUTS 8
+#define VGPU_MAX_AGGS   16
+
+typedef struct vgpu_groupby vgpu_groupby;
+
+/* key_type: VGPU_INT64 / VGPU_DOUBLE / VGPU_BOOL / VGPU_KEY_NONE.
+ * input_types: n_inputs VGPU_* types of the aggregate input slots.
+ * expected_groups: sizing hint (0 = unknown). */
+vgpu_groupby *vgpu_groupby_create(int key_type, int n_inputs,
+                                  const int32_t *input_types, int n_aggs,
+                                  const vgpu_agg *aggs, int order,
+                                  int64_t expected_groups);
+
+/* Feeds the logical rows of one batch.  pos_base is the logical position
+ * of the batch's first logical row in the whole input (first-occurrence
+ * order and min/max tie rules count positions).  pred (optional, may be
+ * NULL) filters rows first, reading pred_cols. */
+int vgpu_groupby_feed(vgpu_groupby *g, const vgpu_col *key,
+                      const vgpu_col *inputs, int64_t n_rows,
+                      const int32_t *sel, int64_t n_sel, int64_t pos_base,
+                      const vgpu_prog *pred, const vgpu_col *pred_cols,
+                      int n_pred_cols);
+
+/* Finishes on the device and orders the groups; returns the group count. */
+int64_t vgpu_groupby_finish(vgpu_groupby *g);
+
+/* Copies the finished result into host buffers of n_groups entries:
+ * keys (8 bytes per group, or 1 for bool keys; may be NULL for KEY_NONE),
+ * key_validity (bitmap, (n_groups+7)/8 bytes, may be NULL), and per
+ * aggregate a double array plus a validity bitmap.  Returns the number of
+ * min/max results that had no value (the reference warns once for each),
+ * or a negative status. */
+int64_t vgpu_groupby_fetch(vgpu_groupby *g, void *keys, uint8_t *key_validity,
+                           double *const *agg_values,
+                           uint8_t *const *agg_validity);
+
+/* Device-side view of the finished result (valid until destroy): keys,
+ * key validity bitmap, and aggregate a's values / validity bitmap. */
+int vgpu_groupby_result_device(vgpu_groupby *g, const void **keys,
+                               const uint8_t **key_validity, int agg,
+                               const double **values,
+                               const uint8_t **validity);
+
+void vgpu_groupby_destroy(vgpu_groupby *g);
+
+/* ---- hash join -------------------------------------------------------- */
+enum { VGPU_JOIN_INNER = 0, VGPU_JOIN_LEFT, VGPU_JOIN_FULL, VGPU_JOIN_SEMI,
+       VGPU_JOIN_ANTI };   /* vectra's JoinKind numbering */
+
+typedef struct vgpu_jtable vgpu_jtable;
+
+/* Builds the device hash table over n_build rows of key (device column,
+ * already coerced to the common key type).  NA and NaN keys are kept out
+ * of the table: they never match (src/join.c:85). */
+vgpu_jtable *vgpu_join_build(const vgpu_col *key, int64_t n_build);
+
+/* Probes the logical rows of a batch.  For each probe row in logical order
+ * the output holds its matches in descending build row order; LEFT/FULL
+ * append the batch's unmatched probe rows after all matches; SEMI/ANTI
+ * emit matching / non-matching probe rows once.  first_match_only limits
+ * LEFT/FULL matches to the first one (the reference's behaviour when the
+ * probe key type differs from the build key type).  Results stay on the
+ * device: vgpu_join_pairs() returns them. */
+int vgpu_join_probe(vgpu_jtable *t, int kind, const vgpu_col *probe_key,
+                    int64_t n_rows, const int32_t *sel, int64_t n_sel,
+                    int first_match_only, int64_t *n_out);
+
+/* Device pointers to the last probe's output: probe physical rows and
+ * build rows (-1 = no build row). Valid until the next probe/destroy. */
+int vgpu_join_pairs(vgpu_jtable *t, const int32_t **probe_rows,
+                    const int64_t **build_rows);
+
+/* FULL join: after the last probe, lists build rows never matched, in
+ * ascending order, into the pair buffers (probe row -1). */
+int vgpu_join_unmatched_build(vgpu_jtable *t, int64_t *n_out);
+
+int64_t vgpu_join_build_rows(const vgpu_jtable *t);
+void    vgpu_join_destroy(vgpu_jtable *t);
+
+/* ---- cast --------------------------------------------------------------
+ * dst_data[i] = src[i] converted to dst_type (int64 -> double, bool ->
+ * int64 / double), vec_coerce's widenings (src/coerce.c:21-71).  Validity
+ * is unchanged, so the caller keeps using src's bitmap. */
+int vgpu_cast(const vgpu_col *src, int64_t n, int dst_type, void *dst_data);
+
+/* ---- gather (join output materialization) ----------------------------
+ * dst[i] = src[idx[i]] with validity, idx < 0 giving NA.  idx is int32 or
+ * int64 (idx_is_64).  dst_data / dst_validity are device buffers of n rows
+ * (validity: (n+7)/8 bytes, fully written). */
+int vgpu_gather(const vgpu_col *src, const void *idx, int idx_is_64,
+                int64_t n, void *dst_data, uint8_t *dst_validity);
+
+/* ---- fused join -> aggregate -----------------------------------------
+ * One pass over the probe rows: optional filter, probe of t, and for each
+ * output pair (kind INNER or LEFT) group-by on a key column taken from the
+ * probe or build side, with aggregate inputs that are a column of either
+ * side or the product of two such columns.  Equivalent to
+ * Join -> Project(tmp = a * b) -> GroupAgg on the same rows. */
+typedef struct {
+    int32_t op;        /* 0 = column a, 1 = a * b */
+    int32_t side_a;    /* 0 = probe, 1 = build */
+    int32_t col_a;
+    int32_t side_b;
+    int32_t col_b;
+    int32_t reserved;
+} vgpu_input;
+
+typedef struct {
+    int64_t          n_rows;
+    const int32_t   *sel;          /* device, optional */
+    int64_t          n_sel;
+    int64_t          pos_base;
+    const vgpu_prog *pred;         /* optional, over probe_cols */
+    vgpu_jtable     *join;         /* optional: NULL = no join */
+    int32_t          join_kind;    /* VGPU_JOIN_INNER or VGPU_JOIN_LEFT */
+    int32_t          key_side;     /* 0 = probe, 1 = build */
+    int32_t          key_col;
+    int32_t          probe_key_col;
+    int32_t          n_probe_cols;
+    int32_t          n_build_cols;
+    const vgpu_col  *probe_cols;   /* host array of device columns */
+    const vgpu_col  *build_cols;
+    int32_t          n_inputs;
+    int32_t          reserved;
+    const vgpu_input *inputs;      /* host array */
+} vgpu_feed_spec;
+
+int vgpu_groupby_feed_join(vgpu_groupby *g, const vgpu_feed_spec *spec);
+
+/* ---- device radix sort (exposed for tests) ----------------------------
+ * Stable ascending sort of n (key, value) pairs on key bits
+ * [begin_bit, end_bit), device buffers, in place. */
+int vgpu_sort_pairs_u64(uint64_t *keys, uint32_t *values, int64_t n,
+                        int begin_bit, int end_bit);
+
+/* ---- timing ---------------------------------------------------------
+ * Event timing of the library's own stream: elapsed milliseconds of all
+ * work issued between vgpu_timer_start() and vgpu_timer_stop(). */
+int    vgpu_timer_start(void);
+double vgpu_timer_stop(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,48 @@
+/*
+ * oracle.h -- CPU restatement of the reference engine's hot path, used ONLY
+ * as the checker in tests/, in __graft_entry__.smoke() and as bench.py's
+ * cpu_baseline leg.  Nothing in vectra_amd/ links, loads or calls it.
+ *
+ * Each node follows the reference algorithm step for step (citations in
+ * oracle.c), exposes the same VecNode vtable, and so can be stacked into the
+ * same trees as the GPU nodes for side-by-side comparison.
+ *
+ * Not restated: the SortNode's 1 GB spill and heap merge
+ * (src/sort.c:236-296, 359-472).  The oracle's key-sorted group-by always
+ * sorts in memory, so on inputs above the reference's 1 GB budget its double
+ * sums follow stable-sort order where the reference follows heap-merge order;
+ * both are checked against the exact (fsum) sum with a tolerance.
+ */
+#ifndef VECTRA_ORACLE_H
+#define VECTRA_ORACLE_H
+
+#include "vectra_abi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Expression evaluation over all physical rows of a batch
+ * (src/expr.c:91-146, src/scalar_ops.c, src/coerce.c). */
+VecArray *oracle_expr_eval(const VecExpr *expr, const VecBatch *batch);
+
+VecNode *oracle_filter_node_create(VecNode *child, VecExpr *predicate);
+VecNode *oracle_group_agg_node_create(VecNode *child, int n_keys,
+                                      char **key_names, int n_aggs,
+                                      AggSpec *agg_specs,
+                                      const char *temp_dir);
+VecNode *oracle_join_node_create(VecNode *left, VecNode *right,
+                                 JoinKind kind, int n_keys, JoinKey *keys,
+                                 const char *suffix_x, const char *suffix_y);
+
+/* FNV-1a of one key value exactly as src/hash.c:12-45 (NA -> offset^0xFF). */
+uint64_t oracle_hash_value(const VecArray *arr, int64_t row);
+
+/* Correctly rounded sum of n doubles (exact fixed-point accumulation, then
+ * one rounding); NaN/Inf inputs follow IEEE addition. */
+double oracle_exact_sum(const double *x, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,152 @@
+// vgpu_internal.h -- shared host/device helpers of libvgpu (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "vgpu.h"
+
+// ---------------------------------------------------------------------------
+// host side: error channel, stream, growable device buffers
+// ---------------------------------------------------------------------------
+
+void vg_set_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+hipStream_t vg_stream();
+int vg_ensure_init();
+
+#define VG_TRY(call)                                                          \
+    do {                                                                      \
+        hipError_t e_ = (call);                                               \
+        if (e_ != hipSuccess) {                                               \
+            vg_set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                         __FILE__, __LINE__);                                 \
+            return -1;                                                        \
+        }                                                                     \
+    } while (0)
+
+#define VG_TRY_NULL(call)                                                     \
+    do {                                                                      \
+        hipError_t e_ = (call);                                               \
+        if (e_ != hipSuccess) {                                               \
+            vg_set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                         __FILE__, __LINE__);                                 \
+            return nullptr;                                                   \
+        }                                                                     \
+    } while (0)
+
+#define VG_LAUNCH_CHECK()                                                     \
+    do {                                                                      \
+        hipError_t e_ = hipGetLastError();                                    \
+        if (e_ != hipSuccess) {                                               \
+            vg_set_error("kernel launch failed: %s (%s:%d)",                 \
+                         hipGetErrorString(e_), __FILE__, __LINE__);          \
+            return -1;                                                        \
+        }                                                                     \
+    } while (0)
+
+// A device buffer that only grows; reused across batches so the steady
+// state allocates nothing.
+struct DevBuf {
+    void  *ptr = nullptr;
+    size_t cap = 0;
+    int reserve(size_t bytes) {
+        if (bytes <= cap) return 0;
+        size_t n = cap ? cap : 4096;
+        while (n < bytes) n *= 2;
+        if (ptr) {
+            hipError_t e = hipFree(ptr);
+            (void)e;
+            ptr = nullptr;
+            cap = 0;
+        }
+        if (hipMalloc(&ptr, n) != hipSuccess) {
+            vg_set_error("hipMalloc(%zu) failed", n);
+            return -1;
+        }
+        cap = n;
+        return 0;
+    }
+    template <class T> T *as() const { return static_cast<T *>(ptr); }
+    void release() {
+        if (ptr) { hipError_t e = hipFree(ptr); (void)e; }
+        ptr = nullptr;
+        cap = 0;
+    }
+};
+
+// device-wide exclusive scan of n uint32 counts into uint64 offsets;
+// *total (device, may be null) gets the sum.  tmp is scratch.
+int vg_scan_u32_to_u64(const uint32_t *in, uint64_t *out, int64_t n,
+                       uint64_t *total, DevBuf &tmp);
+// device-wide exclusive scan of n uint64 values (in may equal out)
+int vg_scan_u64(const uint64_t *in, uint64_t *out, int64_t n, uint64_t *total,
+                DevBuf &tmp);
+// stable LSD radix sort of (key, value) pairs on bits [begin_bit, end_bit)
+int vg_sort_pairs(uint64_t *keys, uint32_t *vals, int64_t n, int begin_bit,
+                  int end_bit, DevBuf &tmp);
+
+static inline unsigned vg_blocks(int64_t n, int64_t per_block, int64_t cap = 1 << 20) {
+    int64_t b = (n + per_block - 1) / per_block;
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return (unsigned)b;
+}
+
+// ---------------------------------------------------------------------------
+// device side
+// ---------------------------------------------------------------------------
+
+#define VG_WAVE 64
+
+__device__ __forceinline__ int vg_lane() { return __lane_id(); }
+
+// number of set bits of mask strictly below this lane
+__device__ __forceinline__ int vg_lane_prefix(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ __forceinline__ uint64_t vg_ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ bool vg_valid(const uint8_t *validity, int64_t row) {
+    return validity == nullptr || ((validity[row >> 3] >> (row & 7)) & 1);
+}
+
+// raw 8-byte view of a row of an int64/double column, or the 0/1 byte of a
+// bool column widened to 8 bytes
+__device__ __forceinline__ uint64_t vg_load_bits(const vgpu_col &c, int64_t row) {
+    if (c.type == VGPU_BOOL) return (uint64_t)static_cast<const uint8_t *>(c.data)[row];
+    return static_cast<const uint64_t *>(c.data)[row];
+}
+
+__device__ __forceinline__ double vg_bits_to_f64(uint64_t b) { return __longlong_as_double((long long)b); }
+__device__ __forceinline__ uint64_t vg_f64_to_bits(double d) { return (uint64_t)__double_as_longlong(d); }
+
+// order-preserving map of a double's bits onto unsigned integers
+// (-0 maps just below +0; only used for non-NaN values)
+__device__ __forceinline__ uint64_t vg_f64_ordered(uint64_t bits) {
+    return (bits >> 63) ? ~bits : (bits | 0x8000000000000000ull);
+}
+__device__ __host__ __forceinline__ uint64_t vg_f64_unordered(uint64_t o) {
+    return (o >> 63) ? (o & 0x7FFFFFFFFFFFFFFFull) : ~o;
+}
+// signed int64 onto unsigned order
+__device__ __host__ __forceinline__ uint64_t vg_i64_ordered(uint64_t bits) {
+    return bits ^ 0x8000000000000000ull;
+}
+
+// 64-bit finalizer (Stafford variant 13 of the MurmurHash3 mixer)
+__device__ __host__ __forceinline__ uint64_t vg_mix64(uint64_t x) {
+    x ^= x >> 30;
+    x *= 0xbf58476d1ce4e5b9ull;
+    x ^= x >> 27;
+    x *= 0x94d049bb133111ebull;
+    x ^= x >> 31;
+    return x;
+}
+
+template <class T>
+__device__ __forceinline__ T vg_atomic_load(const T *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}

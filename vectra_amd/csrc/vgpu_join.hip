@@ -1,0 +1,406 @@
+// vgpu_join.hip -- hash-join build and probe, join output gather, and the
+// column cast the join uses to bring keys to a common type.
+//
+// Reference semantics (gcol33/vectra 0.2.2):
+//   build over the whole right side ............. src/join.c:156-220
+//   chain order: later build rows first ......... src/join.c:45-61, 115-150
+//   probe in logical order; left/full append the
+//   batch's unmatched probe rows after all matches src/join.c:262-480
+//   full join: unmatched build rows at the end .. src/join.c:487-551
+//   NA keys never match ......................... src/join.c:85
+//
+// Build: keys are claimed in an open-addressing table with 64-bit CAS while
+// a per-slot counter counts rows; a scan of the counters gives each key a
+// contiguous segment of rows[].  With unique keys (the common dimension
+// table) every segment has one row and is filled directly; with duplicates
+// a stable radix sort on (slot, ~row) lays each segment out in descending
+// build row order.
+// Probe: count pass (match count + segment offset per probe row), device
+// scans, write pass -- an ordered scatter, so the output order is the
+// reference's without any sort.
+#include "vgpu_join_table.h"
+
+#include <algorithm>
+
+#define JT_THREADS 256
+#define JT_NONE 0xFFFFFFFFu
+
+__global__ void k_jt_init(uint64_t *keys, int64_t n, uint64_t empty, uint32_t *cnt) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        keys[i] = empty;
+        cnt[i] = 0;
+    }
+}
+
+__global__ void k_jt_insert(vgpu_col key, int64_t n, uint64_t *keys, int64_t cap, uint32_t *cnt,
+                            uint32_t *slot_of, uint32_t *err) {
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    uint64_t kn;
+    int cls = vg_key_class(key, r, kn);
+    if (cls == VG_KEY_NA || cls == VG_KEY_NAN) {
+        slot_of[r] = JT_NONE;
+        return;
+    }
+    int64_t s = cap;
+    if (cls == VG_KEY_REGULAR) {
+        const uint64_t empty = key.type == VGPU_DOUBLE ? VG_EMPTY_F64 : VG_EMPTY_I64;
+        uint64_t mask = (uint64_t)cap - 1;
+        uint64_t h = vg_key_hash(kn) & mask;
+        s = -1;
+        for (int64_t probes = 0; probes < cap; probes++) {
+            uint64_t k = vg_atomic_load(&keys[h]);
+            if (k == kn) { s = (int64_t)h; break; }
+            if (k == empty) {
+                uint64_t old = atomicCAS((unsigned long long *)&keys[h], (unsigned long long)empty,
+                                         (unsigned long long)kn);
+                if (old == empty || old == kn) { s = (int64_t)h; break; }
+            }
+            h = (h + 1) & mask;
+        }
+        if (s < 0) {
+            atomicOr(err, 1u);
+            slot_of[r] = JT_NONE;
+            return;
+        }
+    }
+    atomicAdd(&cnt[s], 1u);
+    slot_of[r] = (uint32_t)s;
+}
+
+__global__ void k_jt_seg(const uint32_t *cnt, const uint64_t *off, int64_t n, uint64_t *seg, uint32_t *max_cnt) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    uint32_t c = cnt[s];
+    seg[s] = (off[s] << 32) | c;
+    if (c > 1) atomicMax(max_cnt, c);
+}
+
+__global__ void k_jt_fill_unique(const uint32_t *slot_of, int64_t n, const uint64_t *seg, uint32_t *rows) {
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    uint32_t s = slot_of[r];
+    if (s == JT_NONE) return;
+    rows[seg[s] >> 32] = (uint32_t)r;
+}
+
+__global__ void k_jt_sortkeys(const uint32_t *slot_of, int64_t n, uint64_t *skeys, uint32_t *svals) {
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    skeys[r] = ((uint64_t)slot_of[r] << 32) | (uint64_t)(0xFFFFFFFFu - (uint32_t)r);
+    svals[r] = (uint32_t)r;
+}
+
+extern "C" vgpu_jtable *vgpu_join_build(const vgpu_col *key, int64_t n_build) {
+    if (vg_ensure_init()) return nullptr;
+    if (key->type != VGPU_INT64 && key->type != VGPU_DOUBLE && key->type != VGPU_BOOL) {
+        vg_set_error("join_build: unsupported key type %d", key->type);
+        return nullptr;
+    }
+    if (n_build < 0 || n_build >= (int64_t)JT_NONE) {
+        vg_set_error("join_build: %lld build rows exceed the 32-bit row index", (long long)n_build);
+        return nullptr;
+    }
+    vgpu_jtable *t = new vgpu_jtable();
+    t->key_type = key->type;
+    t->n_build = n_build;
+    int64_t cap = 64;
+    while (cap < 2 * n_build) cap <<= 1;
+    t->cap = cap;
+    hipStream_t s = vg_stream();
+    int64_t slots = cap + 1;
+    DevBuf cnt, slot_of, off, misc;
+    auto fail = [&]() -> vgpu_jtable * {
+        cnt.release(); slot_of.release(); off.release(); misc.release();
+        t->keys.release(); t->seg.release(); t->rows.release(); t->sort_tmp.release(); t->scan_tmp.release();
+        delete t;
+        return nullptr;
+    };
+    if (t->keys.reserve((size_t)slots * 8) || t->seg.reserve((size_t)slots * 8) || cnt.reserve((size_t)slots * 4) ||
+        slot_of.reserve((size_t)std::max<int64_t>(n_build, 1) * 4) || off.reserve((size_t)slots * 8) ||
+        misc.reserve(64))
+        return fail();
+    uint64_t empty = t->key_type == VGPU_DOUBLE ? VG_EMPTY_F64 : VG_EMPTY_I64;
+    if (hipMemsetAsync(misc.ptr, 0, 64, s) != hipSuccess) { vg_set_error("join_build: memset failed"); return fail(); }
+    k_jt_init<<<vg_blocks(slots, 256, 8192), 256, 0, s>>>(t->keys.as<uint64_t>(), slots, empty, cnt.as<uint32_t>());
+    if (n_build > 0) {
+        vgpu_col k = *key;
+        k_jt_insert<<<vg_blocks(n_build, JT_THREADS), JT_THREADS, 0, s>>>(k, n_build, t->keys.as<uint64_t>(), cap,
+                                                                          cnt.as<uint32_t>(), slot_of.as<uint32_t>(),
+                                                                          misc.as<uint32_t>());
+    }
+    uint64_t *total = misc.as<uint64_t>() + 2;
+    if (vg_scan_u32_to_u64(cnt.as<uint32_t>(), off.as<uint64_t>(), slots, total, t->scan_tmp)) return fail();
+    k_jt_seg<<<vg_blocks(slots, 256), 256, 0, s>>>(cnt.as<uint32_t>(), off.as<uint64_t>(), slots,
+                                                   t->seg.as<uint64_t>(), misc.as<uint32_t>() + 1);
+    uint64_t hm[8];
+    if (hipMemcpyAsync(hm, misc.ptr, 64, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        vg_set_error("join_build: %s", hipGetErrorString(hipGetLastError()));
+        return fail();
+    }
+    uint32_t err = (uint32_t)hm[0], maxc = (uint32_t)(hm[0] >> 32);
+    if (err) {
+        vg_set_error("join_build: hash table overflow");
+        return fail();
+    }
+    t->n_valid = (int64_t)hm[2];
+    t->max_dup = (int32_t)std::max<uint32_t>(maxc, t->n_valid > 0 ? 1u : 0u);
+    if (t->rows.reserve((size_t)std::max<int64_t>(n_build, 1) * 4)) return fail();
+    if (n_build > 0) {
+        if (maxc <= 1) {
+            k_jt_fill_unique<<<vg_blocks(n_build, 256), 256, 0, s>>>(slot_of.as<uint32_t>(), n_build,
+                                                                      t->seg.as<uint64_t>(), t->rows.as<uint32_t>());
+        } else {
+            DevBuf skeys;
+            if (skeys.reserve((size_t)n_build * 8)) return fail();
+            k_jt_sortkeys<<<vg_blocks(n_build, 256), 256, 0, s>>>(slot_of.as<uint32_t>(), n_build,
+                                                                   skeys.as<uint64_t>(), t->rows.as<uint32_t>());
+            int rc = vg_sort_pairs(skeys.as<uint64_t>(), t->rows.as<uint32_t>(), n_build, 0, 64, t->sort_tmp);
+            hipError_t e = hipStreamSynchronize(s);
+            skeys.release();
+            if (rc || e != hipSuccess) {
+                if (!rc) vg_set_error("join_build: sort failed: %s", hipGetErrorString(e));
+                return fail();
+            }
+        }
+    }
+    if (hipStreamSynchronize(s) != hipSuccess) {
+        vg_set_error("join_build: %s", hipGetErrorString(hipGetLastError()));
+        return fail();
+    }
+    cnt.release();
+    slot_of.release();
+    off.release();
+    misc.release();
+    t->sort_tmp.release();
+    return t;
+}
+
+extern "C" int64_t vgpu_join_build_rows(const vgpu_jtable *t) { return t ? t->n_build : -1; }
+
+// ---------------------------------------------------------------------------
+// probe
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(JT_THREADS)
+k_jp_count(JView v, vgpu_col key, int64_t n_logical, const int32_t *sel, int kind, int first_only,
+           uint32_t *cnt_main, uint32_t *un, uint32_t *segoff) {
+    int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= n_logical) return;
+    int64_t pr = sel ? (int64_t)sel[li] : li;
+    uint32_t off;
+    uint32_t c = jt_lookup(v, key, pr, off);
+    uint32_t m;
+    switch (kind) {
+    case VGPU_JOIN_SEMI: m = c > 0; break;
+    case VGPU_JOIN_ANTI: m = c == 0; break;
+    default: m = (first_only && c > 1) ? 1u : c; break;
+    }
+    cnt_main[li] = m;
+    segoff[li] = off;
+    if (un) un[li] = c == 0;
+}
+
+__global__ void __launch_bounds__(JT_THREADS)
+k_jp_write(JView v, int64_t n_logical, const int32_t *sel, int kind, const uint32_t *cnt_main,
+           const uint64_t *off_main, const uint32_t *un, const uint64_t *off_un, const uint64_t *totals,
+           const uint32_t *segoff, int32_t *out_probe, int64_t *out_build, uint8_t *bmatched) {
+    int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= n_logical) return;
+    int32_t pr = sel ? sel[li] : (int32_t)li;
+    uint32_t m = cnt_main[li];
+    uint64_t o = off_main[li];
+    if (kind == VGPU_JOIN_SEMI || kind == VGPU_JOIN_ANTI) {
+        if (m) { out_probe[o] = pr; out_build[o] = -1; }
+        return;
+    }
+    uint32_t so = segoff[li];
+    for (uint32_t j = 0; j < m; j++) {
+        uint32_t br = v.rows[so + j];
+        out_probe[o + j] = pr;
+        out_build[o + j] = (int64_t)br;
+        if (bmatched) bmatched[br] = 1;
+    }
+    if (un && un[li]) {
+        uint64_t p = totals[0] + off_un[li];
+        out_probe[p] = pr;
+        out_build[p] = -1;
+    }
+}
+
+extern "C" int vgpu_join_probe(vgpu_jtable *t, int kind, const vgpu_col *probe_key, int64_t n_rows,
+                               const int32_t *sel, int64_t n_sel, int first_match_only, int64_t *n_out) {
+    if (kind < VGPU_JOIN_INNER || kind > VGPU_JOIN_ANTI) {
+        vg_set_error("join_probe: unknown join kind %d", kind);
+        return -1;
+    }
+    if (probe_key->type != t->key_type) {
+        vg_set_error("join_probe: probe key type %d differs from build key type %d", probe_key->type, t->key_type);
+        return -1;
+    }
+    hipStream_t s = vg_stream();
+    int64_t n = sel ? n_sel : n_rows;
+    *n_out = 0;
+    t->last_n_out = 0;
+    bool tail = kind == VGPU_JOIN_LEFT || kind == VGPU_JOIN_FULL;
+    if (kind == VGPU_JOIN_FULL && !t->bmatched.ptr) {
+        if (t->bmatched.reserve((size_t)std::max<int64_t>(t->n_build, 1))) return -1;
+        VG_TRY(hipMemsetAsync(t->bmatched.ptr, 0, (size_t)std::max<int64_t>(t->n_build, 1), s));
+    }
+    if (n <= 0) return 0;
+    if (t->cnt.reserve((size_t)n * 4) || t->un.reserve((size_t)n * 4 * 2) || t->off_main.reserve((size_t)n * 8) ||
+        t->off_un.reserve((size_t)n * 8) || t->totals.reserve(16))
+        return -1;
+    uint32_t *segoff = t->un.as<uint32_t>() + n;   // second half of `un`
+    JView v = t->view();
+    vgpu_col k = *probe_key;
+    unsigned grid = vg_blocks(n, JT_THREADS, 1 << 30);
+    k_jp_count<<<grid, JT_THREADS, 0, s>>>(v, k, n, sel, kind, first_match_only, t->cnt.as<uint32_t>(),
+                                           tail ? t->un.as<uint32_t>() : nullptr, segoff);
+    VG_LAUNCH_CHECK();
+    uint64_t *tot = t->totals.as<uint64_t>();
+    if (vg_scan_u32_to_u64(t->cnt.as<uint32_t>(), t->off_main.as<uint64_t>(), n, tot, t->scan_tmp)) return -1;
+    if (tail) {
+        if (vg_scan_u32_to_u64(t->un.as<uint32_t>(), t->off_un.as<uint64_t>(), n, tot + 1, t->scan_tmp)) return -1;
+    } else {
+        VG_TRY(hipMemsetAsync(tot + 1, 0, 8, s));
+    }
+    uint64_t h[2];
+    VG_TRY(hipMemcpyAsync(h, tot, 16, hipMemcpyDeviceToHost, s));
+    VG_TRY(hipStreamSynchronize(s));
+    int64_t total = (int64_t)(h[0] + h[1]);
+    if (t->out_probe.reserve((size_t)std::max<int64_t>(total, 1) * 4) ||
+        t->out_build.reserve((size_t)std::max<int64_t>(total, 1) * 8))
+        return -1;
+    k_jp_write<<<grid, JT_THREADS, 0, s>>>(v, n, sel, kind, t->cnt.as<uint32_t>(), t->off_main.as<uint64_t>(),
+                                           tail ? t->un.as<uint32_t>() : nullptr, t->off_un.as<uint64_t>(), tot, segoff,
+                                           t->out_probe.as<int32_t>(), t->out_build.as<int64_t>(),
+                                           kind == VGPU_JOIN_FULL ? t->bmatched.as<uint8_t>() : nullptr);
+    VG_LAUNCH_CHECK();
+    VG_TRY(hipStreamSynchronize(s));
+    *n_out = total;
+    t->last_n_out = total;
+    return 0;
+}
+
+extern "C" int vgpu_join_pairs(vgpu_jtable *t, const int32_t **probe_rows, const int64_t **build_rows) {
+    if (probe_rows) *probe_rows = t->out_probe.as<int32_t>();
+    if (build_rows) *build_rows = t->out_build.as<int64_t>();
+    return 0;
+}
+
+__global__ void k_unmatched_flags(const uint8_t *bmatched, int64_t n, uint32_t *flags) {
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) flags[r] = bmatched[r] == 0;
+}
+
+__global__ void k_unmatched_write(const uint32_t *flags, const uint64_t *off, int64_t n, int32_t *out_probe,
+                                  int64_t *out_build) {
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n && flags[r]) {
+        out_probe[off[r]] = -1;
+        out_build[off[r]] = r;
+    }
+}
+
+extern "C" int vgpu_join_unmatched_build(vgpu_jtable *t, int64_t *n_out) {
+    hipStream_t s = vg_stream();
+    int64_t n = t->n_build;
+    *n_out = 0;
+    t->last_n_out = 0;
+    if (n <= 0) return 0;
+    if (!t->bmatched.ptr) {
+        if (t->bmatched.reserve((size_t)n)) return -1;
+        VG_TRY(hipMemsetAsync(t->bmatched.ptr, 0, (size_t)n, s));
+    }
+    if (t->cnt.reserve((size_t)n * 4) || t->off_main.reserve((size_t)n * 8) || t->totals.reserve(16)) return -1;
+    k_unmatched_flags<<<vg_blocks(n, 256, 1 << 30), 256, 0, s>>>(t->bmatched.as<uint8_t>(), n, t->cnt.as<uint32_t>());
+    VG_LAUNCH_CHECK();
+    uint64_t *tot = t->totals.as<uint64_t>();
+    if (vg_scan_u32_to_u64(t->cnt.as<uint32_t>(), t->off_main.as<uint64_t>(), n, tot, t->scan_tmp)) return -1;
+    uint64_t h = 0;
+    VG_TRY(hipMemcpyAsync(&h, tot, 8, hipMemcpyDeviceToHost, s));
+    VG_TRY(hipStreamSynchronize(s));
+    if (t->out_probe.reserve((size_t)std::max<uint64_t>(h, 1) * 4) ||
+        t->out_build.reserve((size_t)std::max<uint64_t>(h, 1) * 8))
+        return -1;
+    k_unmatched_write<<<vg_blocks(n, 256, 1 << 30), 256, 0, s>>>(t->cnt.as<uint32_t>(), t->off_main.as<uint64_t>(), n,
+                                                                  t->out_probe.as<int32_t>(), t->out_build.as<int64_t>());
+    VG_LAUNCH_CHECK();
+    VG_TRY(hipStreamSynchronize(s));
+    *n_out = (int64_t)h;
+    t->last_n_out = (int64_t)h;
+    return 0;
+}
+
+extern "C" void vgpu_join_destroy(vgpu_jtable *t) {
+    if (!t) return;
+    hipError_t e = hipStreamSynchronize(vg_stream());
+    (void)e;
+    DevBuf *bufs[] = {&t->keys, &t->seg, &t->rows, &t->cnt, &t->un, &t->off_main, &t->off_un,
+                      &t->out_probe, &t->out_build, &t->totals, &t->scan_tmp, &t->sort_tmp, &t->bmatched};
+    for (DevBuf *b : bufs) b->release();
+    delete t;
+}
+
+// ---------------------------------------------------------------------------
+// gather and cast
+// ---------------------------------------------------------------------------
+
+__global__ void k_gather(vgpu_col src, const void *idx, int idx64, int64_t n, void *dst, uint64_t *dvalid) {
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool in = g < n;
+    int64_t r = -1;
+    if (in) r = idx64 ? static_cast<const int64_t *>(idx)[g] : (int64_t)static_cast<const int32_t *>(idx)[g];
+    bool ok = in && r >= 0 && vg_valid(src.validity, r);
+    if (in) {
+        if (src.type == VGPU_BOOL) static_cast<uint8_t *>(dst)[g] = ok ? static_cast<const uint8_t *>(src.data)[r] : 0;
+        else static_cast<uint64_t *>(dst)[g] = ok ? static_cast<const uint64_t *>(src.data)[r] : 0;
+    }
+    uint64_t w = vg_ballot(ok);
+    if (vg_lane() == 0 && in) dvalid[g >> 6] = w;
+}
+
+extern "C" int vgpu_gather(const vgpu_col *src, const void *idx, int idx_is_64, int64_t n, void *dst_data,
+                           uint8_t *dst_validity) {
+    if (vg_ensure_init()) return -1;
+    if (n <= 0) return 0;
+    if ((reinterpret_cast<uintptr_t>(dst_validity) & 7) != 0) {
+        vg_set_error("vgpu_gather: validity buffer must be 8-byte aligned");
+        return -1;
+    }
+    vgpu_col c = *src;
+    k_gather<<<vg_blocks(n, 256, 1 << 30), 256, 0, vg_stream()>>>(c, idx, idx_is_64, n, dst_data,
+                                                                   reinterpret_cast<uint64_t *>(dst_validity));
+    VG_LAUNCH_CHECK();
+    VG_TRY(hipStreamSynchronize(vg_stream()));
+    return 0;
+}
+
+__global__ void k_cast(vgpu_col src, int64_t n, int dst_type, uint64_t *dst) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t b = vg_load_bits(src, i);   // bool widens to 0/1
+    if (dst_type == VGPU_DOUBLE && src.type != VGPU_DOUBLE) b = vg_f64_to_bits((double)(int64_t)b);
+    dst[i] = b;
+}
+
+extern "C" int vgpu_cast(const vgpu_col *src, int64_t n, int dst_type, void *dst_data) {
+    if (vg_ensure_init()) return -1;
+    bool ok = (src->type == VGPU_INT64 && dst_type == VGPU_DOUBLE) ||
+              (src->type == VGPU_BOOL && (dst_type == VGPU_INT64 || dst_type == VGPU_DOUBLE)) ||
+              src->type == dst_type;
+    if (!ok || dst_type == VGPU_BOOL) {
+        vg_set_error("vgpu_cast: unsupported coercion %d -> %d", src->type, dst_type);
+        return -1;
+    }
+    if (n <= 0) return 0;
+    vgpu_col c = *src;
+    k_cast<<<vg_blocks(n, 256, 1 << 30), 256, 0, vg_stream()>>>(c, n, dst_type, static_cast<uint64_t *>(dst_data));
+    VG_LAUNCH_CHECK();
+    VG_TRY(hipStreamSynchronize(vg_stream()));
+    return 0;
+}

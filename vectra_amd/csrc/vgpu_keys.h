@@ -1,0 +1,34 @@
+// vgpu_keys.h -- key normalization shared by the group-by and join tables.
+//
+// Both tables are open-addressing arrays of 8-byte key bits claimed with a
+// 64-bit compare-and-swap against an EMPTY sentinel that no regular key can
+// take:
+//   int64 / bool keys : EMPTY = INT64_MIN's bits; a real INT64_MIN key goes
+//                       to a reserved slot instead of the hashed array;
+//   double keys       : EMPTY = all-ones (a NaN); NaN keys never reach the
+//                       hashed array (they never compare equal), and -0 is
+//                       folded onto +0 because -0 == +0 (src/hash.c:23-24,
+//                       src/hash.c:64).
+#pragma once
+
+#include "vgpu_internal.h"
+
+#define VG_EMPTY_I64 0x8000000000000000ull
+#define VG_EMPTY_F64 0xFFFFFFFFFFFFFFFFull
+
+enum { VG_KEY_REGULAR = 0, VG_KEY_NA = 1, VG_KEY_NAN = 2, VG_KEY_SPECIAL = 3 };
+
+__device__ __forceinline__ int vg_key_class(const vgpu_col &c, int64_t row, uint64_t &kn) {
+    if (!vg_valid(c.validity, row)) { kn = 0; return VG_KEY_NA; }
+    uint64_t b = vg_load_bits(c, row);
+    if (c.type == VGPU_DOUBLE) {
+        if ((b & 0x7FFFFFFFFFFFFFFFull) > 0x7FF0000000000000ull) { kn = b; return VG_KEY_NAN; }
+        if (b == 0x8000000000000000ull) b = 0;
+        kn = b;
+        return VG_KEY_REGULAR;
+    }
+    kn = b;
+    return b == VG_EMPTY_I64 ? VG_KEY_SPECIAL : VG_KEY_REGULAR;
+}
+
+__device__ __forceinline__ uint64_t vg_key_hash(uint64_t kn) { return vg_mix64(kn ^ 0x9e3779b97f4a7c15ull); }

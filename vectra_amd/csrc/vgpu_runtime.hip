@@ -1,0 +1,358 @@
+// vgpu_runtime.hip -- device setup, error channel, memory entry points,
+// device-wide scan and the stable LSD radix sort used for result ordering
+// and join builds.
+#include "vgpu_internal.h"
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <mutex>
+
+// ---------------------------------------------------------------------------
+// error channel / device state
+// ---------------------------------------------------------------------------
+
+static thread_local char g_err[1024];
+static hipStream_t g_stream = nullptr;
+static int g_device = -1;
+static std::mutex g_init_mu;
+static hipEvent_t g_t0 = nullptr, g_t1 = nullptr;
+
+void vg_set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+
+extern "C" const char *vgpu_last_error(void) { return g_err; }
+
+hipStream_t vg_stream() { return g_stream; }
+
+extern "C" int vgpu_init(int device) {
+    std::lock_guard<std::mutex> lk(g_init_mu);
+    if (g_device == device && g_stream) return 0;
+    int n = 0;
+    VG_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) {
+        vg_set_error("vgpu_init: device %d not present (%d devices)", device, n);
+        return -1;
+    }
+    VG_TRY(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    VG_TRY(hipGetDeviceProperties(&prop, device));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        vg_set_error("vgpu_init: libvgpu is built for gfx950, device %d is %s",
+                     device, prop.gcnArchName);
+        return -1;
+    }
+    if (!g_stream) VG_TRY(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    if (!g_t0) VG_TRY(hipEventCreate(&g_t0));
+    if (!g_t1) VG_TRY(hipEventCreate(&g_t1));
+    g_device = device;
+    return 0;
+}
+
+int vg_ensure_init() {
+    if (g_stream) return 0;
+    return vgpu_init(0);
+}
+
+extern "C" void *vgpu_malloc(size_t bytes) {
+    if (vg_ensure_init()) return nullptr;
+    void *p = nullptr;
+    VG_TRY_NULL(hipMalloc(&p, bytes ? bytes : 1));
+    return p;
+}
+
+extern "C" int vgpu_free(void *p) {
+    if (!p) return 0;
+    VG_TRY(hipFree(p));
+    return 0;
+}
+
+extern "C" int vgpu_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+    if (vg_ensure_init()) return -1;
+    if (!bytes) return 0;
+    VG_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream));
+    VG_TRY(hipStreamSynchronize(g_stream));
+    return 0;
+}
+
+extern "C" int vgpu_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+    if (vg_ensure_init()) return -1;
+    if (!bytes) return 0;
+    VG_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream));
+    VG_TRY(hipStreamSynchronize(g_stream));
+    return 0;
+}
+
+extern "C" int vgpu_memset(void *dst, int value, size_t bytes) {
+    if (vg_ensure_init()) return -1;
+    if (!bytes) return 0;
+    VG_TRY(hipMemsetAsync(dst, value, bytes, g_stream));
+    VG_TRY(hipStreamSynchronize(g_stream));
+    return 0;
+}
+
+extern "C" int vgpu_synchronize(void) {
+    if (vg_ensure_init()) return -1;
+    VG_TRY(hipStreamSynchronize(g_stream));
+    return 0;
+}
+
+extern "C" void *vgpu_host_alloc(size_t bytes) {
+    if (vg_ensure_init()) return nullptr;
+    void *p = nullptr;
+    VG_TRY_NULL(hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault));
+    return p;
+}
+
+extern "C" int vgpu_host_free(void *p) {
+    if (!p) return 0;
+    VG_TRY(hipHostFree(p));
+    return 0;
+}
+
+extern "C" int vgpu_timer_start(void) {
+    if (vg_ensure_init()) return -1;
+    VG_TRY(hipEventRecord(g_t0, g_stream));
+    return 0;
+}
+
+extern "C" double vgpu_timer_stop(void) {
+    if (hipEventRecord(g_t1, g_stream) != hipSuccess) return -1.0;
+    if (hipEventSynchronize(g_t1) != hipSuccess) return -1.0;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, g_t0, g_t1) != hipSuccess) return -1.0;
+    return (double)ms;
+}
+
+// ---------------------------------------------------------------------------
+// block-level scan helpers (256 threads = 4 waves)
+// ---------------------------------------------------------------------------
+
+#define SCAN_THREADS 256
+#define SCAN_ITEMS 16
+#define SCAN_TILE (SCAN_THREADS * SCAN_ITEMS)
+
+// exclusive scan of one value per thread across the block; *total gets
+// the block sum.  Uses 5 words of LDS.
+__device__ __forceinline__ uint64_t block_exclusive_scan(uint64_t v, uint64_t *lds, uint64_t &total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint64_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint64_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) lds[wave] = x;
+    __syncthreads();
+    uint64_t before = 0;
+    for (int w = 0; w < wave; w++) before += lds[w];
+    total = lds[0] + lds[1] + lds[2] + lds[3];
+    __syncthreads();
+    return before + x - v;
+}
+
+template <class TIn>
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_reduce(const TIn *in, int64_t n, uint64_t *partials) {
+    __shared__ uint64_t lds[4];
+    int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+    uint64_t s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        int64_t idx = base + (int64_t)i * SCAN_THREADS + threadIdx.x;
+        if (idx < n) s += (uint64_t)in[idx];
+    }
+    uint64_t total;
+    block_exclusive_scan(s, lds, total);
+    if (threadIdx.x == 0) partials[blockIdx.x] = total;
+}
+
+// each thread owns SCAN_ITEMS consecutive elements (blocked layout)
+template <class TIn>
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const TIn *in, uint64_t *out, int64_t n,
+                                                             const uint64_t *block_off, uint64_t *total_out) {
+    __shared__ uint64_t lds[4];
+    int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    uint64_t vals[SCAN_ITEMS];
+    uint64_t s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        int64_t idx = base + i;
+        vals[i] = idx < n ? (uint64_t)in[idx] : 0;
+        s += vals[i];
+    }
+    uint64_t total;
+    uint64_t run = block_exclusive_scan(s, lds, total) + (block_off ? block_off[blockIdx.x] : 0);
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        int64_t idx = base + i;
+        if (idx < n) out[idx] = run;
+        run += vals[i];
+    }
+    if (total_out && blockIdx.x == gridDim.x - 1 && threadIdx.x == SCAN_THREADS - 1)
+        *total_out = run;
+}
+
+template <class TIn>
+static int scan_impl(const TIn *in, uint64_t *out, int64_t n, uint64_t *total, DevBuf &tmp, size_t tmp_off) {
+    hipStream_t s = vg_stream();
+    if (n <= 0) {
+        if (total) VG_TRY(hipMemsetAsync(total, 0, 8, s));
+        return 0;
+    }
+    int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (nb == 1) {
+        k_scan_apply<TIn><<<1, SCAN_THREADS, 0, s>>>(in, out, n, nullptr, total);
+        VG_LAUNCH_CHECK();
+        return 0;
+    }
+    // partials live in tmp at tmp_off; recursion uses the space after them
+    size_t need = tmp_off + (size_t)nb * 8;
+    if (tmp.cap < need) {
+        vg_set_error("scan: scratch too small (%zu < %zu)", tmp.cap, need);
+        return -1;
+    }
+    uint64_t *partials = reinterpret_cast<uint64_t *>(static_cast<char *>(tmp.ptr) + tmp_off);
+    k_scan_reduce<TIn><<<(unsigned)nb, SCAN_THREADS, 0, s>>>(in, n, partials);
+    VG_LAUNCH_CHECK();
+    if (scan_impl<uint64_t>(partials, partials, nb, nullptr, tmp, need)) return -1;
+    k_scan_apply<TIn><<<(unsigned)nb, SCAN_THREADS, 0, s>>>(in, out, n, partials, total);
+    VG_LAUNCH_CHECK();
+    return 0;
+}
+
+static size_t scan_tmp_bytes(int64_t n) {
+    size_t b = 4096;
+    while (n > SCAN_TILE) {
+        n = (n + SCAN_TILE - 1) / SCAN_TILE;
+        b += (size_t)n * 8 + 64;
+    }
+    return b * 2 + 4096;
+}
+
+int vg_scan_u32_to_u64(const uint32_t *in, uint64_t *out, int64_t n, uint64_t *total, DevBuf &tmp) {
+    if (tmp.reserve(scan_tmp_bytes(n))) return -1;
+    return scan_impl<uint32_t>(in, out, n, total, tmp, 0);
+}
+
+int vg_scan_u64(const uint64_t *in, uint64_t *out, int64_t n, uint64_t *total, DevBuf &tmp) {
+    if (tmp.reserve(scan_tmp_bytes(n))) return -1;
+    return scan_impl<uint64_t>(in, out, n, total, tmp, 0);
+}
+
+// ---------------------------------------------------------------------------
+// stable LSD radix sort, 8-bit digits
+// ---------------------------------------------------------------------------
+
+#define RS_THREADS 256
+#define RS_ROUNDS 16
+#define RS_TILE (RS_THREADS * RS_ROUNDS)
+
+__global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const uint64_t *keys, int64_t n, int shift,
+                                                        uint32_t *hist, int64_t n_tiles) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ROUNDS; r++) {
+        int64_t idx = base + (int64_t)r * RS_THREADS + threadIdx.x;
+        if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & 0xFF], 1u);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * n_tiles + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const uint64_t *keys_in, const uint32_t *vals_in,
+                                                           uint64_t *keys_out, uint32_t *vals_out, int64_t n,
+                                                           int shift, const uint64_t *offsets, int64_t n_tiles) {
+    __shared__ uint64_t run[256];
+    __shared__ uint32_t wcnt[4][256];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    run[t] = offsets[(int64_t)t * n_tiles + blockIdx.x];
+    for (int w = 0; w < 4; w++) wcnt[w][t] = 0;
+    __syncthreads();
+    int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ROUNDS; r++) {
+        int64_t idx = base + (int64_t)r * RS_THREADS + t;
+        bool in = idx < n;
+        uint64_t k = in ? keys_in[idx] : 0;
+        uint32_t v = in ? vals_in[idx] : 0;
+        uint32_t d = (uint32_t)(k >> shift) & 0xFF;
+        uint64_t m = vg_ballot(in);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            uint64_t bb = vg_ballot((d >> b) & 1);
+            m &= ((d >> b) & 1) ? bb : ~bb;
+        }
+        int rank = vg_lane_prefix(m);
+        if (in && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(m);
+        __syncthreads();
+        if (in) {
+            uint64_t pos = run[d] + rank;
+            for (int w = 0; w < wave; w++) pos += wcnt[w][d];
+            keys_out[pos] = k;
+            vals_out[pos] = v;
+        }
+        (void)lane;
+        __syncthreads();
+        run[t] += (uint64_t)wcnt[0][t] + wcnt[1][t] + wcnt[2][t] + wcnt[3][t];
+        for (int w = 0; w < 4; w++) wcnt[w][t] = 0;
+        __syncthreads();
+    }
+}
+
+int vg_sort_pairs(uint64_t *keys, uint32_t *vals, int64_t n, int begin_bit, int end_bit, DevBuf &tmp) {
+    if (n <= 1 || begin_bit >= end_bit) return 0;
+    hipStream_t s = vg_stream();
+    int64_t n_tiles = (n + RS_TILE - 1) / RS_TILE;
+    int64_t n_hist = n_tiles * 256;
+    size_t off_keys = 0;
+    size_t off_vals = off_keys + (size_t)n * 8;
+    size_t off_hist = (off_vals + (size_t)n * 4 + 255) & ~(size_t)255;
+    size_t off_offs = (off_hist + (size_t)n_hist * 4 + 255) & ~(size_t)255;
+    size_t off_scan = (off_offs + (size_t)n_hist * 8 + 255) & ~(size_t)255;
+    size_t scan_bytes = scan_tmp_bytes(n_hist);
+    if (tmp.reserve(off_scan + scan_bytes)) return -1;
+    char *base = static_cast<char *>(tmp.ptr);
+    uint64_t *k_alt = reinterpret_cast<uint64_t *>(base + off_keys);
+    uint32_t *v_alt = reinterpret_cast<uint32_t *>(base + off_vals);
+    uint32_t *hist = reinterpret_cast<uint32_t *>(base + off_hist);
+    uint64_t *offs = reinterpret_cast<uint64_t *>(base + off_offs);
+    DevBuf scan_view;
+    scan_view.ptr = base + off_scan;
+    scan_view.cap = scan_bytes;
+    uint64_t *kin = keys, *kout = k_alt;
+    uint32_t *vin = vals, *vout = v_alt;
+    int passes = 0;
+    for (int shift = begin_bit; shift < end_bit; shift += 8) {
+        k_rs_hist<<<(unsigned)n_tiles, RS_THREADS, 0, s>>>(kin, n, shift, hist, n_tiles);
+        VG_LAUNCH_CHECK();
+        if (scan_impl<uint32_t>(hist, offs, n_hist, nullptr, scan_view, 0)) return -1;
+        k_rs_scatter<<<(unsigned)n_tiles, RS_THREADS, 0, s>>>(kin, vin, kout, vout, n, shift, offs, n_tiles);
+        VG_LAUNCH_CHECK();
+        uint64_t *tk = kin; kin = kout; kout = tk;
+        uint32_t *tv = vin; vin = vout; vout = tv;
+        passes++;
+    }
+    if (passes & 1) {
+        VG_TRY(hipMemcpyAsync(keys, kin, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
+        VG_TRY(hipMemcpyAsync(vals, vin, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+    }
+    scan_view.ptr = nullptr;   // borrowed view: nothing to free
+    scan_view.cap = 0;
+    return 0;
+}
+
+extern "C" int vgpu_sort_pairs_u64(uint64_t *keys, uint32_t *values, int64_t n, int begin_bit, int end_bit) {
+    if (vg_ensure_init()) return -1;
+    DevBuf tmp;
+    int rc = vg_sort_pairs(keys, values, n, begin_bit, end_bit, tmp);
+    if (!rc && hipStreamSynchronize(vg_stream()) != hipSuccess) {
+        vg_set_error("sort: stream sync failed");
+        rc = -1;
+    }
+    tmp.release();
+    return rc;
+}

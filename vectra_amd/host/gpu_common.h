@@ -1,0 +1,70 @@
+/*
+ * gpu_common.h -- helpers shared by the three GPU operator nodes: device
+ * copies of host columns, the VecExpr -> vgpu_prog compiler, and the
+ * mapping of libvgpu failures onto vectra_error().
+ */
+#ifndef VX_GPU_COMMON_H
+#define VX_GPU_COMMON_H
+
+#include "vectra_abi.h"
+#include "vgpu.h"
+#include "vx_core.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* raises vectra_error with libvgpu's message when status < 0 */
+#define VX_GPU(call)                                                   \
+    do {                                                               \
+        if ((call) < 0) vectra_error("GPU: %s", vgpu_last_error());   \
+    } while (0)
+
+void vx_gpu_init(void);
+
+/* A device copy of one host column; buffers grow and are reused. */
+typedef struct {
+    vgpu_col col;
+    void    *d_data;
+    size_t   cap_data;
+    uint8_t *d_valid;
+    size_t   cap_valid;
+} DevCol;
+
+/* uploads a[0..n) (int64/double/bool); validity only if some row is NA */
+void devcol_upload(DevCol *dc, const VecArray *a, int64_t n);
+/* device buffer of at least `bytes` inside dc (for kernel outputs) */
+void devcol_reserve(DevCol *dc, size_t data_bytes, size_t valid_bytes);
+void devcol_free(DevCol *dc);
+
+/* generic growable device buffer */
+typedef struct { void *ptr; size_t cap; } DevMem;
+void *devmem_reserve(DevMem *m, size_t bytes);
+void  devmem_free(DevMem *m);
+
+int vx_all_valid(const uint8_t *validity, int64_t n);
+VecType vx_common_type(VecType a, VecType b);   /* src/coerce.c:7-19 */
+const char *vx_type_name(VecType t);            /* src/types.h:14-22 */
+
+/* VecExpr -> straight-line program over the batch's columns.  Column slots
+ * index col_idx[] (batch column indices); *result_type gets the
+ * expression's type.  Raises vectra_error with the reference's messages
+ * for type errors and with an explicit message for shapes the GPU path
+ * does not evaluate (string operands). */
+typedef struct {
+    vgpu_prog prog;
+    int       n_cols;
+    int       col_idx[VGPU_MAX_COLS];
+    VecType   result_type;
+} CompiledExpr;
+
+void vx_compile_expr(const VecExpr *e, const VecBatch *batch, CompiledExpr *out);
+
+/* concatenates the logical rows of batches (consumed) into dense columns
+ * of the given schema; returns the row count */
+int64_t vx_concat_batches(VecNode *child, VecArray **cols_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

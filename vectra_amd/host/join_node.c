@@ -1,0 +1,382 @@
+/*
+ * join_node.c -- drop-in GPU JoinNode (src/join.h:45-78).
+ *
+ * Same constructor checks and messages, struct prefix, output schema (all
+ * left columns, then the right non-key columns, suffixed on name
+ * collisions; src/join.c:631-791) and batch protocol as the reference:
+ * BUILD pulls the whole right side once, PROBE emits one output batch per
+ * left batch (skipping empty ones), FULL then emits the never-matched
+ * build rows in batches of at most 65,536 (src/join.c:487-599).
+ *
+ * The build side lives on the device for the node's lifetime; each probe
+ * batch is copied up, probed (count / scan / ordered write), and the output
+ * columns are gathered on the device and copied back as malloc'd arrays.
+ * String payload columns are gathered on the host from the pair list.
+ */
+#include "gpu_common.h"
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define FINALIZE_ROWS 65536   /* src/join.c:487 */
+
+typedef struct {
+    int       built;
+    int       state;          /* 0 probe, 1 finalize, 2 done */
+    int       r_ncols;
+    VecArray *r_host;         /* materialized right side (host) */
+    int64_t   n_build;
+    DevCol   *r_dev;          /* right columns on the device (key coerced) */
+    VecType   build_key_type;
+    vgpu_jtable *jt;
+    int      *r_nonkey;
+    int       n_r_nonkey;
+    int       first_only;     /* see join_build_side */
+    DevCol    l_dev[64];
+    DevCol    pkey_cast;
+    DevMem    sel;
+    DevMem    out_data, out_valid;
+    int64_t  *h_build;        /* FULL finalize: unmatched build rows */
+    int64_t   n_unmatched, fin_cursor;
+} JoinGpu;
+
+static void join_build_side(JoinNode *jn) {
+    JoinGpu *g = (JoinGpu *)jn->gpu;
+    if (jn->n_keys != 1)
+        vectra_error("GPU join: %d key columns; the GPU path joins on one key", jn->n_keys);
+    const VecSchema *rs = &jn->right->output_schema;
+    const VecSchema *ls = &jn->left->output_schema;
+    g->r_ncols = rs->n_cols;
+    g->n_build = vx_concat_batches(jn->right, &g->r_host);
+    int rk = jn->keys[0].right_col;
+    VecType lt = ls->col_types[jn->keys[0].left_col];
+    VecType rt = rs->col_types[rk];
+    if (lt == VEC_STRING || rt == VEC_STRING)
+        vectra_error("GPU join: string keys are not joined on the GPU");
+    VecType common = vx_common_type(lt, rt);
+    g->build_key_type = common;
+    /* Left/full joins continue a chain walk by comparing the probe key in
+     * its ORIGINAL type against the build key (src/join.c:398-400,
+     * 413-415), so when the probe key had to be coerced only the first
+     * (highest build row) match survives. */
+    g->first_only = (jn->kind == JOIN_LEFT || jn->kind == JOIN_FULL) && lt != common;
+    g->r_dev = (DevCol *)calloc((size_t)(g->r_ncols > 0 ? g->r_ncols : 1), sizeof(DevCol));
+    for (int c = 0; c < g->r_ncols; c++) {
+        if (g->r_host[c].type == VEC_STRING) continue;   /* gathered on the host */
+        devcol_upload(&g->r_dev[c], &g->r_host[c], g->n_build);
+    }
+    vgpu_col bkey = g->r_dev[rk].col;
+    if (rt != common) {
+        /* coerce the build key on the device (src/join.c:190-205) */
+        DevCol *dc = &g->r_dev[rk];
+        void *dst = vgpu_malloc((size_t)(g->n_build > 0 ? g->n_build : 1) * 8);
+        if (!dst) vectra_error("GPU: %s", vgpu_last_error());
+        VX_GPU(vgpu_cast(&dc->col, g->n_build, (int)common, dst));
+        vgpu_free(dc->d_data);
+        dc->d_data = dst;
+        dc->cap_data = (size_t)(g->n_build > 0 ? g->n_build : 1) * 8;
+        dc->col.data = dst;
+        dc->col.type = (int32_t)common;
+        bkey = dc->col;
+    }
+    g->jt = vgpu_join_build(&bkey, g->n_build);
+    if (!g->jt) vectra_error("GPU: %s", vgpu_last_error());
+}
+
+/* copies one gathered device column back into a fresh VecArray */
+static VecArray fetch_gathered(JoinGpu *g, VecType t, int64_t n) {
+    VecArray a = vec_array_alloc(t, n);
+    if (n > 0) {
+        size_t w = t == VEC_BOOL ? 1 : 8;
+        VX_GPU(vgpu_memcpy_d2h(a.buf.i64, g->out_data.ptr, (size_t)n * w));
+        VX_GPU(vgpu_memcpy_d2h(a.validity, g->out_valid.ptr, (size_t)vec_validity_bytes(n)));
+    }
+    return a;
+}
+
+static VecArray gather_device(JoinGpu *g, const vgpu_col *src, VecType t, const void *idx, int idx64, int64_t n) {
+    devmem_reserve(&g->out_data, (size_t)(n > 0 ? n : 1) * 8);
+    devmem_reserve(&g->out_valid, (size_t)((n + 63) / 64 + 1) * 8);
+    if (n > 0) VX_GPU(vgpu_gather(src, idx, idx64, n, g->out_data.ptr, (uint8_t *)g->out_valid.ptr));
+    return fetch_gathered(g, t, n);
+}
+
+/* host gather of a string column through a row list (row < 0 -> NA) */
+static VecArray gather_strings(const VecArray *src, const int64_t *rows, int64_t n) {
+    VecArray d = vec_array_alloc(VEC_STRING, n);
+    int64_t total = 0;
+    for (int64_t i = 0; i < n; i++) {
+        int64_t r = rows[i];
+        if (r >= 0 && vec_array_is_valid(src, r)) total += src->buf.str.offsets[r + 1] - src->buf.str.offsets[r];
+    }
+    d.buf.str.data = (char *)malloc((size_t)(total > 0 ? total : 1));
+    d.buf.str.data_len = total;
+    int64_t off = 0;
+    for (int64_t i = 0; i < n; i++) {
+        int64_t r = rows[i];
+        d.buf.str.offsets[i] = off;
+        if (r >= 0 && vec_array_is_valid(src, r)) {
+            vec_array_set_valid(&d, i);
+            int64_t s = src->buf.str.offsets[r], l = src->buf.str.offsets[r + 1] - s;
+            memcpy(d.buf.str.data + off, src->buf.str.data + s, (size_t)l);
+            off += l;
+        }
+    }
+    d.buf.str.offsets[n] = off;
+    return d;
+}
+
+static VecBatch *make_output(JoinNode *jn, int64_t n) {
+    VecBatch *res = vec_batch_alloc(jn->base.output_schema.n_cols, n);
+    for (int c = 0; c < res->n_cols; c++) res->col_names[c] = vx_strdup(jn->base.output_schema.col_names[c]);
+    return res;
+}
+
+static VecBatch *probe_batch(JoinNode *jn, VecBatch *pb) {
+    JoinGpu *g = (JoinGpu *)jn->gpu;
+    int lnc = pb->n_cols;
+    if (lnc > 64) vectra_error("GPU join: more than 64 left columns");
+    int lk = jn->keys[0].left_col;
+    int64_t nl = vec_batch_logical_rows(pb);
+    if (nl == 0) return NULL;   /* empty probe batch: no output (src/join.c:451-459) */
+    /* probe key, coerced to the build key type (src/join.c:268-289) */
+    devcol_upload(&g->l_dev[lk], &pb->columns[lk], pb->n_rows);
+    vgpu_col pkey = g->l_dev[lk].col;
+    if ((VecType)pkey.type != g->build_key_type) {
+        devcol_reserve(&g->pkey_cast, (size_t)(pb->n_rows > 0 ? pb->n_rows : 1) * 8, 8);
+        VX_GPU(vgpu_cast(&pkey, pb->n_rows, (int)g->build_key_type, g->pkey_cast.d_data));
+        pkey.data = g->pkey_cast.d_data;
+        pkey.type = (int32_t)g->build_key_type;
+    }
+    const int32_t *d_sel = NULL;
+    if (pb->sel) {
+        d_sel = (const int32_t *)devmem_reserve(&g->sel, (size_t)(pb->sel_n > 0 ? pb->sel_n : 1) * 4);
+        VX_GPU(vgpu_memcpy_h2d((void *)d_sel, pb->sel, (size_t)pb->sel_n * 4));
+    }
+    int64_t n_out = 0;
+    VX_GPU(vgpu_join_probe(g->jt, (int)jn->kind, &pkey, pb->n_rows, d_sel, pb->sel ? pb->sel_n : 0,
+                           g->first_only, &n_out));
+    if (n_out == 0) return NULL;
+    const int32_t *d_prow;
+    const int64_t *d_brow;
+    vgpu_join_pairs(g->jt, &d_prow, &d_brow);
+    /* host copies of the pairs, needed only for string columns */
+    int64_t *h_p = NULL, *h_b = NULL;
+    int need_host = 0;
+    for (int c = 0; c < lnc; c++) need_host |= pb->columns[c].type == VEC_STRING;
+    if (jn->kind != JOIN_SEMI && jn->kind != JOIN_ANTI)
+        for (int j = 0; j < g->n_r_nonkey; j++) need_host |= g->r_host[g->r_nonkey[j]].type == VEC_STRING;
+    if (need_host) {
+        int32_t *tmp = (int32_t *)malloc((size_t)n_out * 4);
+        h_p = (int64_t *)malloc((size_t)n_out * 8);
+        h_b = (int64_t *)malloc((size_t)n_out * 8);
+        VX_GPU(vgpu_memcpy_d2h(tmp, d_prow, (size_t)n_out * 4));
+        VX_GPU(vgpu_memcpy_d2h(h_b, d_brow, (size_t)n_out * 8));
+        for (int64_t i = 0; i < n_out; i++) h_p[i] = tmp[i];
+        free(tmp);
+    }
+    VecBatch *res = make_output(jn, n_out);
+    for (int c = 0; c < lnc; c++) {
+        const VecArray *src = &pb->columns[c];
+        if (src->type == VEC_STRING) {
+            res->columns[c] = gather_strings(src, h_p, n_out);
+            continue;
+        }
+        if (c != lk) devcol_upload(&g->l_dev[c], src, pb->n_rows);
+        vgpu_col col = g->l_dev[c].col;
+        if (c == lk) col = g->l_dev[lk].col;   /* original (uncoerced) key values */
+        res->columns[c] = gather_device(g, &col, src->type, d_prow, 0, n_out);
+    }
+    if (jn->kind != JOIN_SEMI && jn->kind != JOIN_ANTI) {
+        for (int j = 0; j < g->n_r_nonkey; j++) {
+            int rc = g->r_nonkey[j];
+            if (g->r_host[rc].type == VEC_STRING) {
+                res->columns[lnc + j] = gather_strings(&g->r_host[rc], h_b, n_out);
+                continue;
+            }
+            res->columns[lnc + j] = gather_device(g, &g->r_dev[rc].col, g->r_host[rc].type, d_brow, 1, n_out);
+        }
+    }
+    free(h_p);
+    free(h_b);
+    return res;
+}
+
+/* FULL: one batch of never-matched build rows (src/join.c:487-551) */
+static VecBatch *finalize_batch(JoinNode *jn) {
+    JoinGpu *g = (JoinGpu *)jn->gpu;
+    if (!g->h_build) {
+        int64_t n = 0;
+        VX_GPU(vgpu_join_unmatched_build(g->jt, &n));
+        g->n_unmatched = n;
+        g->h_build = (int64_t *)malloc((size_t)(n > 0 ? n : 1) * 8);
+        if (n > 0) {
+            const int64_t *d_b;
+            vgpu_join_pairs(g->jt, NULL, &d_b);
+            VX_GPU(vgpu_memcpy_d2h(g->h_build, d_b, (size_t)n * 8));
+        }
+        g->fin_cursor = 0;
+    }
+    int64_t n = g->n_unmatched - g->fin_cursor;
+    if (n <= 0) return NULL;
+    if (n > FINALIZE_ROWS) n = FINALIZE_ROWS;
+    const int64_t *rows = g->h_build + g->fin_cursor;
+    g->fin_cursor += n;
+    DevMem idx = {0, 0};
+    devmem_reserve(&idx, (size_t)n * 8);
+    VX_GPU(vgpu_memcpy_h2d(idx.ptr, rows, (size_t)n * 8));
+    const VecSchema *ls = &jn->left->output_schema;
+    int lnc = ls->n_cols;
+    VecBatch *res = make_output(jn, n);
+    for (int c = 0; c < lnc; c++) {
+        int rsrc = (c == jn->keys[0].left_col) ? jn->keys[0].right_col : -1;
+        VecType t = ls->col_types[c];
+        if (rsrc < 0) {
+            res->columns[c] = vec_array_alloc(t, n);   /* all NA */
+        } else if (t == VEC_STRING || g->r_host[rsrc].type == VEC_STRING) {
+            res->columns[c] = gather_strings(&g->r_host[rsrc], rows, n);
+        } else {
+            /* the left-typed output takes the build key's bytes as they are
+             * (src/builder.c:110-138 writes through the builder's type) */
+            vgpu_col col = g->r_dev[rsrc].col;
+            if (t == VEC_BOOL && col.type != VGPU_BOOL) col.type = VGPU_BOOL;
+            else if (t != VEC_BOOL && col.type == VGPU_BOOL) col.type = VGPU_BOOL;
+            res->columns[c] = gather_device(g, &col, col.type == VGPU_BOOL ? VEC_BOOL : t, idx.ptr, 1, n);
+            res->columns[c].type = t;
+        }
+    }
+    for (int j = 0; j < g->n_r_nonkey; j++) {
+        int rc = g->r_nonkey[j];
+        if (g->r_host[rc].type == VEC_STRING)
+            res->columns[lnc + j] = gather_strings(&g->r_host[rc], rows, n);
+        else
+            res->columns[lnc + j] = gather_device(g, &g->r_dev[rc].col, g->r_host[rc].type, idx.ptr, 1, n);
+    }
+    devmem_free(&idx);
+    return res;
+}
+
+static VecBatch *gpu_join_next(VecNode *self) {
+    JoinNode *jn = (JoinNode *)self;
+    JoinGpu *g = (JoinGpu *)jn->gpu;
+    if (!g->built) {
+        join_build_side(jn);
+        g->built = 1;
+    }
+    while (g->state == 0) {
+        VecBatch *pb = jn->left->next_batch(jn->left);
+        if (!pb) {
+            g->state = jn->kind == JOIN_FULL ? 1 : 2;
+            break;
+        }
+        VecBatch *res = probe_batch(jn, pb);
+        vec_batch_free(pb);
+        if (res) return res;
+    }
+    if (g->state == 1) {
+        VecBatch *res = finalize_batch(jn);
+        if (res) return res;
+        g->state = 2;
+    }
+    return NULL;
+}
+
+static void gpu_join_free(VecNode *self) {
+    JoinNode *jn = (JoinNode *)self;
+    JoinGpu *g = (JoinGpu *)jn->gpu;
+    jn->left->free_node(jn->left);
+    jn->right->free_node(jn->right);
+    if (g) {
+        if (g->r_host) {
+            for (int c = 0; c < g->r_ncols; c++) vec_array_free(&g->r_host[c]);
+            free(g->r_host);
+        }
+        if (g->r_dev) {
+            for (int c = 0; c < g->r_ncols; c++) devcol_free(&g->r_dev[c]);
+            free(g->r_dev);
+        }
+        for (int c = 0; c < 64; c++) devcol_free(&g->l_dev[c]);
+        devcol_free(&g->pkey_cast);
+        devmem_free(&g->sel);
+        devmem_free(&g->out_data);
+        devmem_free(&g->out_valid);
+        if (g->jt) vgpu_join_destroy(g->jt);
+        free(g->r_nonkey);
+        free(g->h_build);
+        free(g);
+    }
+    free(jn->keys);
+    free(jn->suffix_x);
+    free(jn->suffix_y);
+    vec_schema_free(&jn->base.output_schema);
+    free(jn);
+}
+
+JoinNode *join_node_create(VecNode *left, VecNode *right, JoinKind kind, int n_keys, JoinKey *keys,
+                           const char *suffix_x, const char *suffix_y) {
+    static const char *kind_names[] = {"inner_join", "left_join", "full_join", "semi_join", "anti_join"};
+    vx_gpu_init();
+    const VecSchema *ls = &left->output_schema, *rs = &right->output_schema;
+    for (int k = 0; k < n_keys; k++) {
+        VecType lt = ls->col_types[keys[k].left_col], rt = rs->col_types[keys[k].right_col];
+        if (lt != rt && (lt == VEC_STRING || rt == VEC_STRING))
+            vectra_error("%s key type mismatch: x.%s (%s) vs y.%s (%s)", kind_names[kind],
+                         ls->col_names[keys[k].left_col], vx_type_name(lt), rs->col_names[keys[k].right_col],
+                         vx_type_name(rt));
+    }
+    JoinNode *jn = (JoinNode *)calloc(1, sizeof *jn);
+    JoinGpu *g = (JoinGpu *)calloc(1, sizeof *g);
+    if (!jn || !g) vectra_error("alloc failed for JoinNode");
+    jn->gpu = g;
+    jn->left = left;
+    jn->right = right;
+    jn->kind = kind;
+    jn->n_keys = n_keys;
+    jn->keys = keys;
+    jn->suffix_x = vx_strdup(suffix_x);
+    jn->suffix_y = vx_strdup(suffix_y);
+    int *r_is_key = (int *)calloc((size_t)(rs->n_cols + 1), sizeof(int));
+    for (int k = 0; k < n_keys; k++) r_is_key[keys[k].right_col] = 1;
+    g->r_nonkey = (int *)malloc(sizeof(int) * (size_t)(rs->n_cols + 1));
+    for (int c = 0; c < rs->n_cols; c++)
+        if (!r_is_key[c]) g->r_nonkey[g->n_r_nonkey++] = c;
+    int semi = kind == JOIN_SEMI || kind == JOIN_ANTI;
+    int out_n = semi ? ls->n_cols : ls->n_cols + g->n_r_nonkey;
+    char **names = (char **)calloc((size_t)(out_n + 1), sizeof(char *));
+    VecType *types = (VecType *)malloc(sizeof(VecType) * (size_t)(out_n + 1));
+    for (int i = 0; i < ls->n_cols; i++) {
+        names[i] = vx_strdup(ls->col_names[i]);
+        types[i] = ls->col_types[i];
+    }
+    if (!semi) {
+        for (int j = 0; j < g->n_r_nonkey; j++) {
+            int c = g->r_nonkey[j];
+            const char *rn = rs->col_names[c];
+            int hit = -1;
+            for (int i = 0; i < ls->n_cols; i++)
+                if (strcmp(ls->col_names[i], rn) == 0) { hit = i; break; }
+            if (hit >= 0) {
+                size_t L = strlen(rn) + strlen(suffix_y) + 1;
+                names[ls->n_cols + j] = (char *)malloc(L);
+                snprintf(names[ls->n_cols + j], L, "%s%s", rn, suffix_y);
+                size_t L2 = strlen(ls->col_names[hit]) + strlen(suffix_x) + 1;
+                free(names[hit]);
+                names[hit] = (char *)malloc(L2);
+                snprintf(names[hit], L2, "%s%s", ls->col_names[hit], suffix_x);
+            } else {
+                names[ls->n_cols + j] = vx_strdup(rn);
+            }
+            types[ls->n_cols + j] = rs->col_types[c];
+        }
+    }
+    jn->base.output_schema = vec_schema_create(out_n, names, types);
+    for (int i = 0; i < out_n; i++) free(names[i]);
+    free(names);
+    free(types);
+    free(r_is_key);
+    jn->base.next_batch = gpu_join_next;
+    jn->base.free_node = gpu_join_free;
+    jn->base.kind = "JoinNode";
+    return jn;
+}

@@ -1,0 +1,546 @@
+"""ctypes view of the native libraries: the layout-compatible engine types of
+include/vectra_abi.h, the kernel ABI of include/vgpu.h, and helpers that build
+node trees the way vectra's R bridge does (src/r_bridge.c:744-1093).
+
+This module is plumbing for tests and benchmarks; the product is the C11
+nodes and the HIP kernels.  Loading fails loudly when a library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+from typing import Any, Sequence
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB = PKG / "lib"
+
+VEC_INT64, VEC_DOUBLE, VEC_BOOL, VEC_STRING = 0, 1, 2, 3
+(EXPR_COL_REF, EXPR_LIT_INT64, EXPR_LIT_DOUBLE, EXPR_LIT_BOOL, EXPR_LIT_STRING, EXPR_LIT_NA, EXPR_ARITH,
+ EXPR_CMP, EXPR_BOOL, EXPR_IS_NA, EXPR_IF_ELSE, EXPR_CAST, EXPR_NEGATE) = range(13)
+AGG_COUNT, AGG_COUNT_STAR, AGG_SUM, AGG_MEAN, AGG_MIN, AGG_MAX = range(6)
+JOIN_INNER, JOIN_LEFT, JOIN_FULL, JOIN_SEMI, JOIN_ANTI = range(5)
+AGG_NAMES = {"count": AGG_COUNT, "n": AGG_COUNT_STAR, "count_star": AGG_COUNT_STAR, "sum": AGG_SUM,
+             "mean": AGG_MEAN, "min": AGG_MIN, "max": AGG_MAX}
+JOIN_NAMES = {"inner": JOIN_INNER, "left": JOIN_LEFT, "full": JOIN_FULL, "semi": JOIN_SEMI, "anti": JOIN_ANTI}
+
+
+# --------------------------------------------------------------------------
+# layout-compatible structs (include/vectra_abi.h)
+# --------------------------------------------------------------------------
+
+class _Str(C.Structure):
+    _fields_ = [("offsets", C.POINTER(C.c_int64)), ("data", C.c_char_p), ("data_len", C.c_int64)]
+
+
+class _Buf(C.Union):
+    _fields_ = [("i64", C.POINTER(C.c_int64)), ("dbl", C.POINTER(C.c_double)), ("bln", C.POINTER(C.c_uint8)),
+                ("str", _Str)]
+
+
+class VecArray(C.Structure):
+    _fields_ = [("type", C.c_int), ("length", C.c_int64), ("validity", C.POINTER(C.c_uint8)),
+                ("owns_data", C.c_uint8), ("buf", _Buf)]
+
+
+class VecBatch(C.Structure):
+    _fields_ = [("n_rows", C.c_int64), ("n_cols", C.c_int), ("col_names", C.POINTER(C.c_char_p)),
+                ("columns", C.POINTER(VecArray)), ("sel", C.POINTER(C.c_int32)), ("sel_n", C.c_int32)]
+
+
+class VecSchema(C.Structure):
+    _fields_ = [("n_cols", C.c_int), ("col_names", C.POINTER(C.c_char_p)), ("col_types", C.POINTER(C.c_int)),
+                ("col_annotations", C.POINTER(C.c_char_p))]
+
+
+class VecNode(C.Structure):
+    pass
+
+
+VecNode._fields_ = [("next_batch", C.c_void_p), ("free_node", C.c_void_p), ("output_schema", VecSchema),
+                    ("kind", C.c_char_p)]
+
+
+class VecExpr(C.Structure):
+    pass
+
+
+VecExpr._fields_ = [("kind", C.c_int), ("result_type", C.c_int), ("col_name", C.c_void_p), ("lit_i64", C.c_int64),
+                    ("lit_dbl", C.c_double), ("lit_bln", C.c_uint8), ("lit_str", C.c_void_p), ("op", C.c_char),
+                    ("op2", C.c_char), ("left", C.POINTER(VecExpr)), ("right", C.POINTER(VecExpr)),
+                    ("operand", C.POINTER(VecExpr)), ("cond", C.POINTER(VecExpr)),
+                    ("then_expr", C.POINTER(VecExpr)), ("else_expr", C.POINTER(VecExpr)), ("cast_to", C.c_int)]
+
+
+class AggSpec(C.Structure):
+    _fields_ = [("output_name", C.c_void_p), ("kind", C.c_int), ("input_col", C.c_void_p), ("na_rm", C.c_int)]
+
+
+class JoinKey(C.Structure):
+    _fields_ = [("left_col", C.c_int), ("right_col", C.c_int)]
+
+
+# --------------------------------------------------------------------------
+# kernel ABI structs (include/vgpu.h)
+# --------------------------------------------------------------------------
+
+class VgpuCol(C.Structure):
+    _fields_ = [("type", C.c_int32), ("reserved", C.c_int32), ("data", C.c_void_p), ("validity", C.c_void_p)]
+
+
+class _Imm(C.Union):
+    _fields_ = [("i", C.c_int64), ("d", C.c_double)]
+
+
+class VgpuInsn(C.Structure):
+    _fields_ = [("op", C.c_uint8), ("type", C.c_uint8), ("a", C.c_uint8), ("b", C.c_uint8), ("col", C.c_int32),
+                ("imm", _Imm)]
+
+
+VGPU_MAX_INSN = 32
+
+
+class VgpuProg(C.Structure):
+    _fields_ = [("n_insn", C.c_int32), ("reserved", C.c_int32), ("insn", VgpuInsn * VGPU_MAX_INSN)]
+
+
+class VgpuAgg(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("input", C.c_int32), ("na_rm", C.c_int32), ("reserved", C.c_int32)]
+
+
+class VgpuInput(C.Structure):
+    _fields_ = [("op", C.c_int32), ("side_a", C.c_int32), ("col_a", C.c_int32), ("side_b", C.c_int32),
+                ("col_b", C.c_int32), ("reserved", C.c_int32)]
+
+
+class VgpuFeedSpec(C.Structure):
+    _fields_ = [("n_rows", C.c_int64), ("sel", C.c_void_p), ("n_sel", C.c_int64), ("pos_base", C.c_int64),
+                ("pred", C.POINTER(VgpuProg)), ("join", C.c_void_p), ("join_kind", C.c_int32),
+                ("key_side", C.c_int32), ("key_col", C.c_int32), ("probe_key_col", C.c_int32),
+                ("n_probe_cols", C.c_int32), ("n_build_cols", C.c_int32), ("probe_cols", C.POINTER(VgpuCol)),
+                ("build_cols", C.POINTER(VgpuCol)), ("n_inputs", C.c_int32), ("reserved", C.c_int32),
+                ("inputs", C.POINTER(VgpuInput))]
+
+
+VGPU_OP = {name: i + 1 for i, name in enumerate(
+    ["COL", "CONST", "NA", "CAST", "ADD", "SUB", "MUL", "DIV", "MOD", "EQ", "NE", "LT", "LE", "GT", "GE",
+     "AND", "OR", "NOT", "ISNA", "NEG"])}
+VGPU_KEY_NONE = -1
+VGPU_ORDER_FIRST_SEEN, VGPU_ORDER_KEY = 0, 1
+
+
+# --------------------------------------------------------------------------
+# loading
+# --------------------------------------------------------------------------
+
+_libs: dict[str, C.CDLL] = {}
+_libc = C.CDLL(None)
+_libc.malloc.restype = C.c_void_p
+_libc.malloc.argtypes = [C.c_size_t]
+_libc.free.argtypes = [C.c_void_p]
+_libc.calloc.restype = C.c_void_p
+_libc.calloc.argtypes = [C.c_size_t, C.c_size_t]
+
+NODE_P = C.POINTER(VecNode)
+BATCH_P = C.POINTER(VecBatch)
+EXPR_P = C.POINTER(VecExpr)
+
+
+def _need(path: Path) -> Path:
+    if not path.exists():
+        raise RuntimeError(f"native library {path} is missing: build it first (python -m vectra_amd.build)")
+    return path
+
+
+def _proto(lib: C.CDLL, name: str, res, args) -> None:
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = args
+
+
+def core() -> C.CDLL:
+    if "core" not in _libs:
+        lib = C.CDLL(str(_need(LIB / "libvx_core.so")), mode=C.RTLD_GLOBAL)
+        _proto(lib, "vx_mem_scan_create", NODE_P, [C.c_int, C.POINTER(C.c_char_p), C.POINTER(VecArray), C.c_int64,
+                                                   C.c_int64])
+        _proto(lib, "vx_try_collect", C.c_int, [NODE_P, C.POINTER(BATCH_P)])
+        _proto(lib, "vx_try_next", C.c_int, [NODE_P, C.POINTER(BATCH_P)])
+        _proto(lib, "vx_node_free", None, [NODE_P])
+        _proto(lib, "vx_last_error", C.c_char_p, [])
+        _proto(lib, "vx_last_warning", C.c_char_p, [])
+        _proto(lib, "vx_warning_count", C.c_int64, [])
+        _proto(lib, "vx_clear_warnings", None, [])
+        _proto(lib, "vec_batch_free", None, [BATCH_P])
+        _proto(lib, "vec_expr_alloc", EXPR_P, [C.c_int])
+        _proto(lib, "vec_expr_free", None, [EXPR_P])
+        _proto(lib, "vx_strdup", C.c_void_p, [C.c_char_p])
+        _proto(lib, "vx_try_filter", C.c_int, [C.c_void_p, NODE_P, EXPR_P, C.POINTER(NODE_P)])
+        _proto(lib, "vx_try_group", C.c_int, [C.c_void_p, NODE_P, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                                              C.c_char_p, C.POINTER(NODE_P)])
+        _proto(lib, "vx_try_join", C.c_int, [C.c_void_p, NODE_P, NODE_P, C.c_int, C.c_int, C.c_void_p, C.c_char_p,
+                                             C.c_char_p, C.POINTER(NODE_P)])
+        _libs["core"] = lib
+    return _libs["core"]
+
+
+def vgpu() -> C.CDLL:
+    if "vgpu" not in _libs:
+        core()
+        lib = C.CDLL(str(_need(LIB / "libvgpu.so")), mode=C.RTLD_GLOBAL)
+        _proto(lib, "vgpu_init", C.c_int, [C.c_int])
+        _proto(lib, "vgpu_last_error", C.c_char_p, [])
+        _proto(lib, "vgpu_malloc", C.c_void_p, [C.c_size_t])
+        _proto(lib, "vgpu_free", C.c_int, [C.c_void_p])
+        _proto(lib, "vgpu_memcpy_h2d", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t])
+        _proto(lib, "vgpu_memcpy_d2h", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t])
+        _proto(lib, "vgpu_memset", C.c_int, [C.c_void_p, C.c_int, C.c_size_t])
+        _proto(lib, "vgpu_synchronize", C.c_int, [])
+        _proto(lib, "vgpu_filter", C.c_int, [C.POINTER(VgpuProg), C.POINTER(VgpuCol), C.c_int, C.c_int64, C.c_void_p,
+                                             C.c_int64, C.c_void_p, C.POINTER(C.c_int64)])
+        _proto(lib, "vgpu_groupby_create", C.c_void_p, [C.c_int, C.c_int, C.POINTER(C.c_int32), C.c_int,
+                                                        C.POINTER(VgpuAgg), C.c_int, C.c_int64])
+        _proto(lib, "vgpu_groupby_feed", C.c_int, [C.c_void_p, C.POINTER(VgpuCol), C.POINTER(VgpuCol), C.c_int64,
+                                                   C.c_void_p, C.c_int64, C.c_int64, C.POINTER(VgpuProg),
+                                                   C.POINTER(VgpuCol), C.c_int])
+        _proto(lib, "vgpu_groupby_feed_join", C.c_int, [C.c_void_p, C.POINTER(VgpuFeedSpec)])
+        _proto(lib, "vgpu_groupby_finish", C.c_int64, [C.c_void_p])
+        _proto(lib, "vgpu_groupby_fetch", C.c_int64, [C.c_void_p, C.c_void_p, C.c_void_p,
+                                                      C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)])
+        _proto(lib, "vgpu_groupby_destroy", None, [C.c_void_p])
+        _proto(lib, "vgpu_join_build", C.c_void_p, [C.POINTER(VgpuCol), C.c_int64])
+        _proto(lib, "vgpu_join_probe", C.c_int, [C.c_void_p, C.c_int, C.POINTER(VgpuCol), C.c_int64, C.c_void_p,
+                                                 C.c_int64, C.c_int, C.POINTER(C.c_int64)])
+        _proto(lib, "vgpu_join_pairs", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)])
+        _proto(lib, "vgpu_join_unmatched_build", C.c_int, [C.c_void_p, C.POINTER(C.c_int64)])
+        _proto(lib, "vgpu_join_build_rows", C.c_int64, [C.c_void_p])
+        _proto(lib, "vgpu_join_destroy", None, [C.c_void_p])
+        _proto(lib, "vgpu_gather", C.c_int, [C.POINTER(VgpuCol), C.c_void_p, C.c_int, C.c_int64, C.c_void_p,
+                                             C.c_void_p])
+        _proto(lib, "vgpu_cast", C.c_int, [C.POINTER(VgpuCol), C.c_int64, C.c_int, C.c_void_p])
+        _proto(lib, "vgpu_sort_pairs_u64", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_int])
+        _proto(lib, "vgpu_timer_start", C.c_int, [])
+        _proto(lib, "vgpu_timer_stop", C.c_double, [])
+        _libs["vgpu"] = lib
+    return _libs["vgpu"]
+
+
+def _node_protos(lib: C.CDLL, prefix: str) -> None:
+    _proto(lib, f"{prefix}filter_node_create", NODE_P, [NODE_P, EXPR_P])
+    _proto(lib, f"{prefix}group_agg_node_create", NODE_P, [NODE_P, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                                                           C.c_char_p])
+    _proto(lib, f"{prefix}join_node_create", NODE_P, [NODE_P, NODE_P, C.c_int, C.c_int, C.c_void_p, C.c_char_p,
+                                                      C.c_char_p])
+
+
+def nodes() -> C.CDLL:
+    if "nodes" not in _libs:
+        vgpu()
+        lib = C.CDLL(str(_need(LIB / "libvectra_gpu.so")))
+        _node_protos(lib, "")
+        _libs["nodes"] = lib
+    return _libs["nodes"]
+
+
+def load_oracle(path: Path) -> C.CDLL:
+    """Loads the oracle library (test infrastructure) sharing libvx_core."""
+    if "oracle" not in _libs:
+        core()
+        lib = C.CDLL(str(_need(path)))
+        _node_protos(lib, "oracle_")
+        _proto(lib, "oracle_exact_sum", C.c_double, [C.POINTER(C.c_double), C.c_int64])
+        _proto(lib, "oracle_hash_value", C.c_uint64, [C.POINTER(VecArray), C.c_int64])
+        _libs["oracle"] = lib
+    return _libs["oracle"]
+
+
+class VectraError(RuntimeError):
+    pass
+
+
+# --------------------------------------------------------------------------
+# host columns and tables
+# --------------------------------------------------------------------------
+
+class Column:
+    """A host column: numpy values (+ optional bool mask of valid rows) kept
+    alive for as long as a VecArray borrows them."""
+
+    def __init__(self, values: np.ndarray, valid: np.ndarray | None = None):
+        values = np.asarray(values)
+        if values.dtype == np.bool_:
+            self.type = VEC_BOOL
+            self.values = np.ascontiguousarray(values.astype(np.uint8))
+        elif np.issubdtype(values.dtype, np.integer):
+            self.type = VEC_INT64
+            self.values = np.ascontiguousarray(values.astype(np.int64))
+        elif np.issubdtype(values.dtype, np.floating):
+            self.type = VEC_DOUBLE
+            self.values = np.ascontiguousarray(values.astype(np.float64))
+        else:
+            raise TypeError(f"unsupported column dtype {values.dtype}")
+        n = len(self.values)
+        if valid is None:
+            self.bitmap = None
+        else:
+            valid = np.asarray(valid, dtype=bool)
+            self.bitmap = np.ascontiguousarray(np.packbits(valid, bitorder="little"))
+            if len(self.bitmap) == 0:
+                self.bitmap = np.zeros(1, np.uint8)
+        self.n = n
+
+    def as_vecarray(self) -> VecArray:
+        a = VecArray()
+        a.type = self.type
+        a.length = self.n
+        a.owns_data = 1
+        a.validity = self.bitmap.ctypes.data_as(C.POINTER(C.c_uint8)) if self.bitmap is not None else None
+        if self.type == VEC_BOOL:
+            a.buf.bln = self.values.ctypes.data_as(C.POINTER(C.c_uint8))
+        elif self.type == VEC_INT64:
+            a.buf.i64 = self.values.ctypes.data_as(C.POINTER(C.c_int64))
+        else:
+            a.buf.dbl = self.values.ctypes.data_as(C.POINTER(C.c_double))
+        return a
+
+
+class Table:
+    """Named host columns, emitted in row groups of rg_rows like a .vtr scan."""
+
+    def __init__(self, cols: dict[str, Any], rg_rows: int = 0):
+        self.cols = {k: (v if isinstance(v, Column) else Column(*v) if isinstance(v, tuple) else Column(v))
+                     for k, v in cols.items()}
+        lens = {c.n for c in self.cols.values()}
+        if len(lens) > 1:
+            raise ValueError(f"ragged table: {lens}")
+        self.n = lens.pop() if lens else 0
+        self.rg_rows = rg_rows
+        self._keep: list[Any] = []
+
+    def scan(self) -> Any:
+        names = list(self.cols)
+        arr = (VecArray * max(1, len(names)))()
+        for i, k in enumerate(names):
+            arr[i] = self.cols[k].as_vecarray()
+        cnames = (C.c_char_p * max(1, len(names)))(*[k.encode() for k in names])
+        self._keep.append((arr, cnames))
+        return core().vx_mem_scan_create(len(names), cnames, arr, self.n, self.rg_rows)
+
+
+# --------------------------------------------------------------------------
+# expressions (the R bridge's parse_expr, src/r_bridge.c:744-876)
+# --------------------------------------------------------------------------
+
+def _malloc_str(s: str) -> int:
+    return core().vx_strdup(s.encode())
+
+
+def expr(tree: Any) -> Any:
+    """Builds a malloc'd VecExpr from a nested tuple:
+    ("col", name) ("dbl", v) ("int", v) ("lgl", v) ("na",) ("cmp", op, l, r)
+    ("arith", op, l, r) ("and", l, r) ("or", l, r) ("not", x) ("is_na", x) ("neg", x)."""
+    lib = core()
+    kind = tree[0]
+    if kind == "col":
+        e = lib.vec_expr_alloc(EXPR_COL_REF)
+        e.contents.col_name = _malloc_str(tree[1])
+        return e
+    if kind == "dbl":
+        e = lib.vec_expr_alloc(EXPR_LIT_DOUBLE)
+        e.contents.lit_dbl = float(tree[1])
+        e.contents.result_type = VEC_DOUBLE
+        return e
+    if kind == "int":
+        e = lib.vec_expr_alloc(EXPR_LIT_INT64)
+        e.contents.lit_i64 = int(tree[1])
+        e.contents.result_type = VEC_INT64
+        return e
+    if kind == "lgl":
+        e = lib.vec_expr_alloc(EXPR_LIT_BOOL)
+        e.contents.lit_bln = 1 if tree[1] else 0
+        e.contents.result_type = VEC_BOOL
+        return e
+    if kind == "na":
+        e = lib.vec_expr_alloc(EXPR_LIT_NA)
+        e.contents.result_type = VEC_DOUBLE
+        return e
+    if kind in ("cmp", "arith"):
+        e = lib.vec_expr_alloc(EXPR_CMP if kind == "cmp" else EXPR_ARITH)
+        op = tree[1]
+        e.contents.op = op[0].encode()
+        e.contents.op2 = (op[1] if len(op) > 1 else " ").encode()
+        e.contents.left = expr(tree[2])
+        e.contents.right = expr(tree[3])
+        e.contents.result_type = VEC_BOOL if kind == "cmp" else VEC_DOUBLE
+        return e
+    if kind in ("and", "or"):
+        e = lib.vec_expr_alloc(EXPR_BOOL)
+        e.contents.op = b"&" if kind == "and" else b"|"
+        e.contents.left = expr(tree[1])
+        e.contents.right = expr(tree[2])
+        e.contents.result_type = VEC_BOOL
+        return e
+    if kind == "not":
+        e = lib.vec_expr_alloc(EXPR_BOOL)
+        e.contents.op = b"!"
+        e.contents.operand = expr(tree[1])
+        e.contents.result_type = VEC_BOOL
+        return e
+    if kind == "is_na":
+        e = lib.vec_expr_alloc(EXPR_IS_NA)
+        e.contents.operand = expr(tree[1])
+        e.contents.result_type = VEC_BOOL
+        return e
+    if kind == "neg":
+        e = lib.vec_expr_alloc(EXPR_NEGATE)
+        e.contents.operand = expr(tree[1])
+        return e
+    raise ValueError(f"unknown expression node {kind}")
+
+
+# --------------------------------------------------------------------------
+# node construction and execution
+# --------------------------------------------------------------------------
+
+def _lib_for(engine: str, oracle_path: Path | None = None) -> tuple[C.CDLL, str]:
+    if engine == "gpu":
+        return nodes(), ""
+    if engine == "oracle":
+        if "oracle" not in _libs:
+            if oracle_path is None:
+                raise RuntimeError("oracle library not loaded")
+            load_oracle(oracle_path)
+        return _libs["oracle"], "oracle_"
+    raise ValueError(engine)
+
+
+def _ctor(lib: C.CDLL, name: str) -> C.c_void_p:
+    return C.cast(getattr(lib, name), C.c_void_p)
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise VectraError(core().vx_last_error().decode())
+
+
+def filter_node(engine: str, child: Any, pred: Any) -> Any:
+    lib, p = _lib_for(engine)
+    out = NODE_P()
+    e = expr(pred) if isinstance(pred, tuple) else pred
+    _check(core().vx_try_filter(_ctor(lib, f"{p}filter_node_create"), child, e, C.byref(out)))
+    return out
+
+
+def group_agg_node(engine: str, child: Any, keys: Sequence[str], aggs: Sequence[tuple], temp_dir: str | None = "/tmp"
+                   ) -> Any:
+    """aggs: (output_name, kind, input_col or None, na_rm)."""
+    lib, p = _lib_for(engine)
+    nk = len(keys)
+    key_names = _libc.malloc(max(1, nk) * C.sizeof(C.c_void_p))
+    kn = C.cast(key_names, C.POINTER(C.c_void_p))
+    for i, k in enumerate(keys):
+        kn[i] = _malloc_str(k)
+    na = len(aggs)
+    specs = _libc.calloc(max(1, na), C.sizeof(AggSpec))
+    sp = C.cast(specs, C.POINTER(AggSpec))
+    for i, (name, kind, col, na_rm) in enumerate(aggs):
+        sp[i].output_name = _malloc_str(name)
+        sp[i].kind = AGG_NAMES[kind] if isinstance(kind, str) else kind
+        sp[i].input_col = _malloc_str(col) if col is not None else None
+        sp[i].na_rm = int(bool(na_rm))
+    td = temp_dir.encode() if temp_dir is not None else None
+    out = NODE_P()
+    _check(core().vx_try_group(_ctor(lib, f"{p}group_agg_node_create"), child, nk, key_names, na, specs, td,
+                               C.byref(out)))
+    return out
+
+
+def join_node(engine: str, left: Any, right: Any, kind: str, keys: Sequence[tuple[int, int]],
+              suffix_x: str = ".x", suffix_y: str = ".y") -> Any:
+    lib, p = _lib_for(engine)
+    jk = _libc.malloc(max(1, len(keys)) * C.sizeof(JoinKey))
+    kp = C.cast(jk, C.POINTER(JoinKey))
+    for i, (lc, rc) in enumerate(keys):
+        kp[i].left_col = lc
+        kp[i].right_col = rc
+    out = NODE_P()
+    _check(core().vx_try_join(_ctor(lib, f"{p}join_node_create"), left, right, JOIN_NAMES[kind], len(keys), jk,
+                              suffix_x.encode(), suffix_y.encode(), C.byref(out)))
+    return out
+
+
+def batch_to_numpy(b: Any) -> dict[str, tuple[np.ndarray, np.ndarray]]:
+    """Dense batch -> {name: (values, valid)}; strings come back as object arrays."""
+    out = {}
+    bt = b.contents
+    n = bt.n_rows
+    for c in range(bt.n_cols):
+        a = bt.columns[c]
+        name = bt.col_names[c].decode()
+        if a.validity and n > 0:
+            bits = np.ctypeslib.as_array(a.validity, shape=((n + 7) // 8,))
+            valid = np.unpackbits(bits, bitorder="little")[:n].astype(bool)
+        else:
+            valid = np.ones(n, dtype=bool)
+        if n == 0:
+            vals = np.zeros(0, dtype={VEC_INT64: np.int64, VEC_DOUBLE: np.float64, VEC_BOOL: np.uint8}.get(a.type, object))
+        elif a.type == VEC_INT64:
+            vals = np.ctypeslib.as_array(a.buf.i64, shape=(n,)).copy()
+        elif a.type == VEC_DOUBLE:
+            vals = np.ctypeslib.as_array(a.buf.dbl, shape=(n,)).copy()
+        elif a.type == VEC_BOOL:
+            vals = np.ctypeslib.as_array(a.buf.bln, shape=(n,)).copy()
+        else:
+            offs = np.ctypeslib.as_array(a.buf.str.offsets, shape=(n + 1,))
+            data = C.string_at(C.cast(a.buf.str.data, C.c_void_p), int(offs[-1])) if offs[-1] > 0 else b""
+            vals = np.array([data[offs[i]:offs[i + 1]].decode() for i in range(n)], dtype=object)
+        vals = np.where(valid, vals, 0) if a.type != VEC_STRING else vals
+        out[name] = (vals, valid)
+    return out
+
+
+def collect(root: Any, free: bool = True) -> dict[str, tuple[np.ndarray, np.ndarray]]:
+    """Pull loop of vec_collect (src/collect.c:188-206); raises VectraError."""
+    lib = core()
+    out = BATCH_P()
+    rc = lib.vx_try_collect(root, C.byref(out))
+    if rc != 0:
+        msg = lib.vx_last_error().decode()
+        if free:
+            lib.vx_node_free(root)
+        raise VectraError(msg)
+    try:
+        return batch_to_numpy(out)
+    finally:
+        lib.vec_batch_free(out)
+        if free:
+            lib.vx_node_free(root)
+
+
+def next_batches(root: Any, free: bool = True) -> list[dict[str, Any]]:
+    """Pulls every batch separately, keeping sel: [{name: (values, valid)}, 'sel': array|None, 'n_rows': n]."""
+    lib = core()
+    res = []
+    while True:
+        out = BATCH_P()
+        rc = lib.vx_try_next(root, C.byref(out))
+        if rc != 0:
+            msg = lib.vx_last_error().decode()
+            if free:
+                lib.vx_node_free(root)
+            raise VectraError(msg)
+        if not out:
+            break
+        bt = out.contents
+        sel = None
+        if bt.sel:
+            sel = np.ctypeslib.as_array(bt.sel, shape=(max(bt.sel_n, 1),))[:bt.sel_n].copy()
+        sel_ptr = bt.sel
+        bt.sel = None
+        d = batch_to_numpy(out)
+        bt.sel = sel_ptr
+        res.append({"cols": d, "sel": sel, "n_rows": bt.n_rows})
+        lib.vec_batch_free(out)
+    if free:
+        lib.vx_node_free(root)
+    return res
