@@ -86,7 +86,7 @@ typedef struct {
     union { int64_t i; double d; } imm;
 } vgpu_insn;
 
-#define VGPU_MAX_INSN 32
+#define VGPU_MAX_INSN 24
 #define VGPU_MAX_COLS 8
 
 typedef struct {

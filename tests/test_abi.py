@@ -1,0 +1,69 @@
+"""The C-ABI libraries load and export every function include/*.h declares;
+struct layouts seen from ctypes match the C compiler's (CPU only, no GPU
+calls)."""
+import ctypes as C
+import re
+import subprocess
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+DECL = re.compile(r"^[A-Za-z_][\w \*]*?\b(\w+)\s*\(", re.M)
+
+
+def declared(header):
+    text = (ROOT / "include" / header).read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"static inline[^{]*\{.*?\n\}", "", text, flags=re.S)
+    names = set()
+    for m in DECL.finditer(text):
+        name = m.group(1)
+        line = text[m.start():text.find("\n", m.start())]
+        if line.startswith(("typedef", "#", "struct")) or name in ("sizeof",):
+            continue
+        names.add(name)
+    return names
+
+
+def test_vgpu_exports(native):
+    lib = C.CDLL(str(ROOT / "vectra_amd" / "lib" / "libvgpu.so"))
+    names = declared("vgpu.h")
+    assert "vgpu_filter" in names and "vgpu_groupby_feed_join" in names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_node_exports(native):
+    lib = C.CDLL(str(ROOT / "vectra_amd" / "lib" / "libvectra_gpu.so"))
+    names = declared("vectra_abi.h")
+    assert {"filter_node_create", "group_agg_node_create", "join_node_create"} <= names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_struct_layouts(native, tmp_path):
+    """offsets of the fields the optimizer / explain read, compiled by gcc"""
+    src = tmp_path / "lay.c"
+    src.write_text(r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "vectra_abi.h"
+int main(void) {
+  printf("%zu %zu %zu %zu\n", sizeof(VecArray), sizeof(VecBatch), sizeof(VecNode), sizeof(VecExpr));
+  printf("%zu %zu %zu\n", offsetof(FilterNode, child), offsetof(FilterNode, predicate), sizeof(AggSpec));
+  printf("%zu %zu %zu %zu %zu\n", offsetof(GroupAggNode, n_keys), offsetof(GroupAggNode, key_names),
+         offsetof(GroupAggNode, n_aggs), offsetof(GroupAggNode, agg_specs), offsetof(GroupAggNode, use_sorted));
+  printf("%zu %zu %zu %zu\n", offsetof(JoinNode, left), offsetof(JoinNode, right), offsetof(JoinNode, kind),
+         offsetof(JoinNode, n_keys));
+  return 0;
+}''')
+    exe = tmp_path / "lay"
+    subprocess.run(["gcc", "-std=c11", f"-I{ROOT / 'include'}", str(src), "-o", str(exe)], check=True)
+    lines = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    nv = native
+    assert lines[0].split() == [str(C.sizeof(t)) for t in (nv.VecArray, nv.VecBatch, nv.VecNode, nv.VecExpr)]
+    # reference prefixes on LP64: VecNode is 2 pointers + schema (4 words) + kind = 56 bytes
+    assert lines[1].split()[:2] == ["56", "64"]
+    assert lines[2].split() == ["64", "72", "80", "88", "100"]
+    assert lines[3].split() == ["56", "64", "72", "76"]

@@ -1,0 +1,114 @@
+"""Device-resident kernel API: fused probe -> aggregate (the Join ->
+Project(value * weight) -> GroupAgg shape of the join-then-aggregate query)
+and the filtered group-by, against the oracle nodes on the same rows; plus
+the device radix sort against numpy's stable argsort."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from helpers import assert_tables_equal, rng
+
+pytestmark = pytest.mark.gpu
+
+
+def star_schema(seed, n_fact, n_dim, p_absent=0.1, dup=False):
+    r = rng(seed)
+    dim_key = (r.permutation(n_dim * 3)[:n_dim].astype(np.int64) * 13 - 7)
+    if dup:
+        dim_key[r.random(n_dim) < 0.2] = dim_key[0]
+    attr = r.integers(0, 100, n_dim)
+    weight = r.random(n_dim)
+    ranks = np.minimum(r.zipf(1.1, n_fact) - 1, n_dim - 1)
+    fkey = np.where(r.random(n_fact) < p_absent, -(r.integers(1, 10**9, n_fact)), dim_key[ranks])
+    value = r.lognormal(size=n_fact)
+    return fkey, value, dim_key, attr, weight
+
+
+def oracle_join_agg(nv, fkey, value, dim_key, attr, weight, kind, rg=1 << 20):
+    left = nv.Table({"fkey": fkey, "value": value}, rg_rows=rg)
+    right = nv.Table({"key": dim_key, "attr": attr, "weight": weight})
+    j = nv.collect(nv.join_node("oracle", left.scan(), right.scan(), kind, [(0, 0)]))
+    v = j["value"][0]
+    wv, wm = j["weight"]
+    prod = np.where(wm, v * wv, 0.0)
+    t = nv.Table({"attr": j["attr"], "s_in": (prod, wm), "ab": np.abs(prod)})
+    out = nv.collect(nv.group_agg_node("oracle", t.scan(), ["attr"],
+                                       [("s", "sum", "s_in", 0), ("n", "n", None, 0), ("sa", "sum", "ab", 0)],
+                                       "/tmp"))
+    return out
+
+
+@pytest.mark.parametrize("kind", ["inner", "left"])
+@pytest.mark.parametrize("dup", [False, True])
+def test_fused_join_aggregate(gpu, oracle, kind, dup):
+    nv = gpu
+    fkey, value, dim_key, attr, weight = star_schema(5, 200_000, 20_000, dup=dup)
+    exp = oracle_join_agg(nv, fkey, value, dim_key, attr, weight, kind)
+    sa = exp.pop("sa")[0]
+    cols = [nv.DeviceColumn(a) for a in (fkey, value, dim_key, attr, weight)]
+    fk, val, dk, at, wt = cols
+    jt = nv.JoinTable(dk)
+    gb = nv.GroupBy(nv.VEC_INT64, [nv.VEC_DOUBLE], [("sum", 0, 0), ("n", -1, 0)], nv.VGPU_ORDER_KEY, 101)
+    gb.feed_join(len(fkey), [fk, val], [dk, at, wt], [(0, 1, 1, 2)], key=(1, 1), join=jt,
+                 join_kind=nv.JOIN_NAMES[kind], probe_key_col=0)
+    gb.finish()
+    got = gb.fetch()
+    got = {"attr": got["key"], "s": got["a0"], "n": got["a1"]}
+    assert_tables_equal(got, exp, {"s": sa})
+    gb.free()
+    jt.free()
+    for c in cols:
+        c.free()
+
+
+def test_fused_filter_groupby(gpu, oracle):
+    nv = gpu
+    r = rng(9)
+    n = 300_000
+    key = r.integers(0, 5000, n)
+    x = r.normal(size=n)
+    dk, dx = nv.DeviceColumn(key), nv.DeviceColumn(x)
+    prog = nv.cmp_const_prog(0, nv.VEC_DOUBLE, "GT", 0.25)
+    for hint in (0, 5000):
+        gb = nv.GroupBy(nv.VEC_INT64, [nv.VEC_DOUBLE],
+                        [("sum", 0, 0), ("n", -1, 0), ("mean", 0, 0), ("min", 0, 0), ("max", 0, 0)],
+                        nv.VGPU_ORDER_FIRST_SEEN, hint)
+        gb.feed(dk, [dx], n, 0, prog, [dx])
+        gb.finish()
+        got = gb.fetch()
+        gb.free()
+        t = nv.Table({"key": key, "x": x, "ax": np.abs(x)}, rg_rows=65536)
+        exp = nv.collect(nv.group_agg_node("oracle", nv.filter_node("oracle", t.scan(),
+                                                                     ("cmp", ">", ("col", "x"), ("dbl", 0.25))),
+                                           ["key"], [("a0", "sum", "x", 0), ("a1", "n", None, 0), ("a2", "mean", "x", 0),
+                                                     ("a3", "min", "x", 0), ("a4", "max", "x", 0),
+                                                     ("sa", "sum", "ax", 0)], None))
+        sa = exp.pop("sa")[0]
+        assert_tables_equal({"key": got.pop("key"), **got}, exp, {"a0": sa, "a2": sa / exp["a1"][0]})
+    dk.free()
+    dx.free()
+
+
+@pytest.mark.parametrize("n,bits", [(1, 64), (1000, 64), (300_001, 40), (70_000, 13)])
+def test_radix_sort_pairs(gpu, n, bits):
+    nv = gpu
+    r = rng(n)
+    keys = r.integers(0, 2**63 - 1, n, dtype=np.int64).astype(np.uint64)
+    if bits < 64:
+        keys &= np.uint64((1 << bits) - 1)
+    keys[: n // 3] = keys[n // 2]          # ties: stability matters
+    vals = np.arange(n, dtype=np.uint32)
+    lib = nv.vgpu()
+    dk, dv = lib.vgpu_malloc(n * 8), lib.vgpu_malloc(n * 4)
+    assert lib.vgpu_memcpy_h2d(dk, keys.ctypes.data, n * 8) == 0
+    assert lib.vgpu_memcpy_h2d(dv, vals.ctypes.data, n * 4) == 0
+    assert lib.vgpu_sort_pairs_u64(dk, dv, n, 0, bits) == 0, lib.vgpu_last_error()
+    ok, ov = np.zeros(n, np.uint64), np.zeros(n, np.uint32)
+    lib.vgpu_memcpy_d2h(ok.ctypes.data, dk, n * 8)
+    lib.vgpu_memcpy_d2h(ov.ctypes.data, dv, n * 4)
+    lib.vgpu_free(dk)
+    lib.vgpu_free(dv)
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(ok, keys[order])
+    assert np.array_equal(ov, vals[order])

@@ -1,0 +1,188 @@
+"""GPU GroupAggNode vs the oracle (src/group_agg.c, src/agg_ops.c, src/hash.c).
+
+Keys, group order, counts, int64 sum/min/max and double min/max (including
+NaN-first and signed-zero cases) must be bit-exact; double sum and mean
+within 1e-12 * sum|x| of the group.
+"""
+import numpy as np
+import pytest
+
+from helpers import assert_tables_equal, dbl_with_specials, rng, with_na
+
+pytestmark = pytest.mark.gpu
+
+ALL_AGGS = [
+    ("n", "n", None, 0),
+    ("cx", "count", "x", 0),
+    ("sx", "sum", "x", 0), ("sx_rm", "sum", "x", 1),
+    ("mx", "mean", "x", 0), ("mx_rm", "mean", "x", 1),
+    ("lo", "min", "x", 0), ("lo_rm", "min", "x", 1),
+    ("hi", "max", "x", 0), ("hi_rm", "max", "x", 1),
+    ("sv", "sum", "v", 1), ("mv", "mean", "v", 1), ("lov", "min", "v", 1), ("hiv", "max", "v", 0),
+    ("sb", "sum", "b", 1), ("lob", "min", "b", 1),
+]
+CLOSE = {"sx", "sx_rm", "mx", "mx_rm"}
+
+
+def table(nv, n, seed, key_kind, card, rg, p_na=0.03, p_nan=0.02):
+    r = rng(seed)
+    if key_kind == "int":
+        kv = r.integers(-card, card, n)
+        kv[r.random(n) < 0.01] = np.iinfo(np.int64).min
+        key = (kv, with_na(r, n, p_na))
+    elif key_kind == "dbl":
+        kv = np.round(r.normal(size=n) * card) / 4.0
+        u = r.random(n)
+        kv[u < p_nan] = np.nan
+        z = (u > 0.5) & (u < 0.52)
+        kv[z] = np.where(r.random(z.sum()) < 0.5, 0.0, -0.0)
+        key = (kv, with_na(r, n, p_na))
+    elif key_kind == "bool":
+        key = (r.random(n) < 0.3, with_na(r, n, p_na))
+    else:
+        key = np.zeros(n, np.int64)
+    x = dbl_with_specials(r, n, p_nan, 0.03)
+    cols = {
+        "g": key,
+        "x": (x, with_na(r, n, p_na)),
+        "v": (r.integers(-10**12, 10**12, n), with_na(r, n, p_na)),
+        "b": (r.random(n) < 0.5, with_na(r, n, p_na)),
+        "ax": np.abs(np.nan_to_num(x, nan=0.0)),
+    }
+    return nv.Table(cols, rg_rows=rg)
+
+
+def run(nv, engine, t, keys, aggs, temp_dir, pred=None):
+    node = t.scan()
+    if pred is not None:
+        node = nv.filter_node(engine, node, pred)
+    return nv.collect(nv.group_agg_node(engine, node, keys, aggs, temp_dir))
+
+
+def check(nv, mk, keys, aggs, temp_dir, pred=None):
+    got = run(nv, "gpu", mk(), keys, aggs, temp_dir, pred)
+    extra = [("__sa", "sum", "ax", 1), ("__cnt", "count", "x", 0)]
+    exp = run(nv, "oracle", mk(), keys, list(aggs) + extra, temp_dir, pred)
+    sa = exp.pop("__sa")[0]
+    cnt = exp.pop("__cnt")[0]
+    close = {}
+    for name, kind, col, _ in aggs:
+        if name in CLOSE:
+            close[name] = sa if kind == "sum" else sa / np.maximum(cnt, 1)
+    assert_tables_equal(got, exp, close)
+    return got
+
+
+@pytest.mark.parametrize("temp_dir", [None, "/tmp"])
+@pytest.mark.parametrize("key_kind,card", [("int", 40), ("int", 30000), ("bool", 0), ("dbl", 8)])
+@pytest.mark.parametrize("rg", [0, 3001])
+def test_groupby_matches_oracle(gpu, oracle, temp_dir, key_kind, card, rg):
+    if key_kind == "dbl" and temp_dir is not None:
+        mk = lambda: table(gpu, 40000, 11, key_kind, card, rg, p_nan=0.0)
+    else:
+        mk = lambda: table(gpu, 40000, 11, key_kind, card, rg)
+    check(gpu, mk, ["g"], ALL_AGGS, temp_dir)
+
+
+@pytest.mark.parametrize("rg", [0, 5000])
+def test_ungrouped_summarise(gpu, oracle, rg):
+    mk = lambda: table(gpu, 30000, 5, "none", 0, rg)
+    check(gpu, mk, [], ALL_AGGS, "/tmp")
+
+
+def test_ungrouped_empty_input(gpu, oracle):
+    mk = lambda: gpu.Table({"x": np.zeros(0), "ax": np.zeros(0)})
+    got = check(gpu, mk, [], [("n", "n", None, 0), ("s", "sum", "x", 0)], None)
+    assert len(got["n"][0]) == 0
+
+
+def test_groupby_after_filter(gpu, oracle):
+    mk = lambda: table(gpu, 50000, 21, "int", 500, 4096)
+    pred = ("cmp", ">", ("col", "x"), ("dbl", 0.0))
+    check(gpu, mk, ["g"], ALL_AGGS, "/tmp", pred)
+    check(gpu, mk, ["g"], ALL_AGGS, None, pred)
+
+
+def test_groupby_nan_first_and_signed_zero(gpu, oracle):
+    """min/max order rules of src/agg_ops.c:118-150: a NaN fed first sticks,
+    later NaNs are ignored, and a zero result keeps the first zero's sign."""
+    x = np.array([np.nan, 1.0, -2.0, 3.0, 0.0, -0.0, 5.0, -0.0, 0.0, 2.0, np.nan, 7.0])
+    g = np.array([0, 0, 1, 1, 2, 2, 3, 3, 4, 4, 5, 5])
+    mk = lambda: gpu.Table({"g": g, "x": x, "ax": np.abs(np.nan_to_num(x))}, rg_rows=5)
+    aggs = [("lo", "min", "x", 0), ("hi", "max", "x", 0), ("n", "n", None, 0)]
+    for td in (None, "/tmp"):
+        got = check(gpu, mk, ["g"], aggs, td)
+        lo = got["lo"][0]
+        assert np.isnan(lo[0]) and lo[1] == -2.0
+        assert np.signbit(lo[2]) is np.False_ and np.signbit(lo[3]) is np.True_
+        hi = got["hi"][0]
+        assert hi[4] == 2.0 and np.signbit(hi[2]) is np.False_
+
+
+def test_groupby_config1(gpu, oracle):
+    """filter(x > 0) |> group_by(key) |> summarise(total = sum(value), n = n(), avg = mean(value)) on 1M rows / 10 RGs."""
+    n = 1_000_000
+    r = rng(7)
+    cols = {"id": np.arange(n), "key": r.integers(0, 1000, n), "x": r.normal(size=n), "value": r.uniform(0, 100, n)}
+    mk = lambda: gpu.Table(cols, rg_rows=100_000)
+    aggs = [("total", "sum", "value", 0), ("n", "n", None, 0), ("avg", "mean", "value", 0)]
+    pred = ("cmp", ">", ("col", "x"), ("dbl", 0.0))
+    got = run(gpu, "gpu", mk(), ["key"], aggs, "/tmp", pred)
+    exp = run(gpu, "oracle", mk(), ["key"], aggs + [("sa", "sum", "value", 0)], "/tmp", pred)
+    sa = exp.pop("sa")[0]
+    assert_tables_equal(got, exp, {"total": sa, "avg": sa / exp["n"][0]})
+    assert list(got["key"][0]) == list(range(1000))
+    # fsum oracle: exact group sums
+    import math
+    keep = cols["x"] > 0
+    for k in (0, 1, 499, 999):
+        sel = keep & (cols["key"] == k)
+        ex = math.fsum(cols["value"][sel])
+        assert abs(got["total"][0][k] - ex) <= 1e-12 * ex
+
+
+def test_groupby_known_answers(gpu):
+    """test-groupby.R:1-99 and test-agg.R:1-139, string keys recoded as
+    integers in the same first-seen / sorted order."""
+    nv = gpu
+
+    def q(cols, aggs, td=None, keys=("g",)):
+        return nv.collect(nv.group_agg_node("gpu", nv.Table(cols).scan(), list(keys), aggs, td))
+
+    out = q({"g": np.array([0, 1, 0, 1, 0]), "x": np.arange(1, 6)}, [("cnt", "n", None, 0)])
+    assert list(out["g"][0]) == [0, 1] and list(out["cnt"][0]) == [3, 2]
+    out = q({"g": np.array([0, 0, 1, 1]), "x": np.array([1., 2., 3., 4.])}, [("total", "sum", "x", 0)])
+    assert list(out["total"][0]) == [3, 7]
+    out = q({"g": np.array([0, 0, 1, 1]), "x": np.array([10., 20., 30., 40.])}, [("avg", "mean", "x", 0)])
+    assert list(out["avg"][0]) == [15, 35]
+    out = q({"g": np.array([0, 0, 1, 1]), "x": np.array([5., 1., 8., 3.])},
+            [("lo", "min", "x", 0), ("hi", "max", "x", 0)])
+    assert list(out["lo"][0]) == [1, 3] and list(out["hi"][0]) == [5, 8]
+    out = q({"g": (np.array([0, 0, 0, 0]), np.array([1, 0, 1, 0], bool)), "x": np.array([1., 2., 3., 4.])},
+            [("total", "sum", "x", 0)], td="/tmp")
+    assert list(out["g"][1]) == [True, False] and list(out["total"][0]) == [4, 6]
+    out = q({"g": np.array([0, 0, 1]), "x": (np.array([1., 0., 3.]), np.array([1, 0, 1], bool))},
+            [("total", "sum", "x", 1), ("s", "sum", "x", 0)])
+    assert list(out["total"][0]) == [1, 3] and list(out["s"][1]) == [False, True]
+    out = q({"g": np.array([0, 0]), "x": (np.zeros(2), np.zeros(2, bool))},
+            [("s", "sum", "x", 1), ("m", "mean", "x", 1), ("s0", "sum", "x", 0)])
+    assert out["s"][0][0] == 0 and np.isnan(out["m"][0][0]) and not out["s0"][1][0]
+    nv.core().vx_clear_warnings()
+    out = q({"g": np.array([0, 0]), "x": (np.zeros(2), np.zeros(2, bool))},
+            [("lo", "min", "x", 1), ("hi", "max", "x", 1)])
+    assert out["lo"][0][0] == np.inf and out["hi"][0][0] == -np.inf
+    assert nv.core().vx_warning_count() == 2
+    out = q({"g": np.array([0, 0, 0]), "x": (np.array([1., 0., 3.]), np.array([1, 0, 1], bool))},
+            [("cnt", "n", None, 0), ("m", "mean", "x", 1)])
+    assert out["cnt"][0][0] == 3 and out["m"][0][0] == 2
+    out = q({"g": np.array([0, 0, 1]), "x": np.array([100, 200, 50])}, [("s", "sum", "x", 0)])
+    assert list(out["s"][0]) == [300, 50]
+    with pytest.raises(nv.VectraError, match="summarise: column not found: zz"):
+        q({"g": np.array([0]), "x": np.array([1.])}, [("s", "sum", "zz", 0)])
+
+
+def test_groupby_high_cardinality_growth(gpu, oracle):
+    """many more groups than the initial table: exercises on-device rehash"""
+    mk = lambda: table(gpu, 300000, 3, "int", 10**9, 20000, p_na=0.001, p_nan=0.001)
+    check(gpu, mk, ["g"], [("n", "n", None, 0), ("sv", "sum", "v", 0), ("lo", "min", "x", 0),
+                           ("sx", "sum", "x", 0)], None)

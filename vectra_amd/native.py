@@ -98,7 +98,7 @@ class VgpuInsn(C.Structure):
                 ("imm", _Imm)]
 
 
-VGPU_MAX_INSN = 32
+VGPU_MAX_INSN = 24
 
 
 class VgpuProg(C.Structure):
@@ -324,7 +324,9 @@ class Table:
             arr[i] = self.cols[k].as_vecarray()
         cnames = (C.c_char_p * max(1, len(names)))(*[k.encode() for k in names])
         self._keep.append((arr, cnames))
-        return core().vx_mem_scan_create(len(names), cnames, arr, self.n, self.rg_rows)
+        node = core().vx_mem_scan_create(len(names), cnames, arr, self.n, self.rg_rows)
+        node._keep = self   # the scan borrows these buffers for its lifetime
+        return node
 
 
 # --------------------------------------------------------------------------
@@ -428,6 +430,7 @@ def filter_node(engine: str, child: Any, pred: Any) -> Any:
     out = NODE_P()
     e = expr(pred) if isinstance(pred, tuple) else pred
     _check(core().vx_try_filter(_ctor(lib, f"{p}filter_node_create"), child, e, C.byref(out)))
+    out._keep = child
     return out
 
 
@@ -452,6 +455,7 @@ def group_agg_node(engine: str, child: Any, keys: Sequence[str], aggs: Sequence[
     out = NODE_P()
     _check(core().vx_try_group(_ctor(lib, f"{p}group_agg_node_create"), child, nk, key_names, na, specs, td,
                                C.byref(out)))
+    out._keep = child
     return out
 
 
@@ -466,6 +470,7 @@ def join_node(engine: str, left: Any, right: Any, kind: str, keys: Sequence[tupl
     out = NODE_P()
     _check(core().vx_try_join(_ctor(lib, f"{p}join_node_create"), left, right, JOIN_NAMES[kind], len(keys), jk,
                               suffix_x.encode(), suffix_y.encode(), C.byref(out)))
+    out._keep = (left, right)
     return out
 
 
@@ -544,3 +549,184 @@ def next_batches(root: Any, free: bool = True) -> list[dict[str, Any]]:
     if free:
         lib.vx_node_free(root)
     return res
+
+
+# --------------------------------------------------------------------------
+# device-resident columns and the kernel-level API (include/vgpu.h)
+# --------------------------------------------------------------------------
+
+def _vcheck(rc: int) -> None:
+    if rc < 0:
+        raise VectraError(vgpu().vgpu_last_error().decode())
+
+
+class DeviceColumn:
+    """A column copied once into HBM (values + optional validity bitmap)."""
+
+    def __init__(self, values: np.ndarray, valid: np.ndarray | None = None):
+        host = Column(values, valid)
+        lib = vgpu()
+        nbytes = host.values.nbytes
+        self.n = host.n
+        self.type = host.type
+        self.data = lib.vgpu_malloc(max(nbytes, 8))
+        if not self.data:
+            raise VectraError(lib.vgpu_last_error().decode())
+        _vcheck(lib.vgpu_memcpy_h2d(self.data, host.values.ctypes.data, nbytes))
+        self.valid = None
+        if host.bitmap is not None:
+            self.valid = lib.vgpu_malloc(max(host.bitmap.nbytes, 8))
+            _vcheck(lib.vgpu_memcpy_h2d(self.valid, host.bitmap.ctypes.data, host.bitmap.nbytes))
+
+    @property
+    def col(self) -> VgpuCol:
+        c = VgpuCol()
+        c.type = self.type
+        c.data = self.data
+        c.validity = self.valid
+        return c
+
+    def free(self) -> None:
+        lib = vgpu()
+        if self.data:
+            lib.vgpu_free(self.data)
+        if self.valid:
+            lib.vgpu_free(self.valid)
+        self.data = self.valid = None
+
+
+def col_array(cols: Sequence[DeviceColumn | None]) -> Any:
+    arr = (VgpuCol * max(1, len(cols)))()
+    for i, c in enumerate(cols):
+        if c is not None:
+            arr[i] = c.col
+    return arr
+
+
+def compile_prog(insns: Sequence[tuple]) -> VgpuProg:
+    """(op, type, a, b, col, imm) tuples -> VgpuProg (op by name)."""
+    p = VgpuProg()
+    p.n_insn = len(insns)
+    for i, (op, typ, a, b, col, imm) in enumerate(insns):
+        ins = p.insn[i]
+        ins.op = VGPU_OP[op]
+        ins.type = typ
+        ins.a, ins.b, ins.col = a, b, col
+        if isinstance(imm, float):
+            ins.imm.d = imm
+        else:
+            ins.imm.i = int(imm)
+    return p
+
+
+def cmp_const_prog(col_slot: int, col_type: int, op: str, value: float | int) -> VgpuProg:
+    """col <op> constant, compared in the column's type (a double constant
+    against an int64 column widens the column, as vec_cmp does)."""
+    ctype = VEC_DOUBLE if (col_type == VEC_DOUBLE or isinstance(value, float)) else VEC_INT64
+    ins = [("COL", col_type, 0, 0, col_slot, 0)]
+    if ctype != col_type:
+        ins.append(("CAST", ctype, 0, 0, 0, 0))
+    a = len(ins) - 1
+    ins.append(("CONST", ctype, 0, 0, 0, float(value) if ctype == VEC_DOUBLE else int(value)))
+    ins.append((op, ctype, a, len(ins) - 1, 0, 0))
+    return compile_prog(ins)
+
+
+class JoinTable:
+    def __init__(self, key: DeviceColumn):
+        lib = vgpu()
+        self.h = lib.vgpu_join_build(C.byref(key.col), key.n)
+        if not self.h:
+            raise VectraError(lib.vgpu_last_error().decode())
+        self.key_type = key.type
+
+    def free(self) -> None:
+        if self.h:
+            vgpu().vgpu_join_destroy(self.h)
+            self.h = None
+
+
+class GroupBy:
+    """Kernel-level group-by: create / feed / feed_join / finish / fetch."""
+
+    def __init__(self, key_type: int, input_types: Sequence[int], aggs: Sequence[tuple], order: int,
+                 expected_groups: int = 0):
+        lib = vgpu()
+        self.n_aggs = len(aggs)
+        self.key_type = key_type
+        it = (C.c_int32 * max(1, len(input_types)))(*input_types)
+        ag = (VgpuAgg * max(1, len(aggs)))()
+        for i, (kind, inp, na_rm) in enumerate(aggs):
+            ag[i].kind = AGG_NAMES[kind] if isinstance(kind, str) else kind
+            ag[i].input = inp
+            ag[i].na_rm = na_rm
+        self.h = lib.vgpu_groupby_create(key_type, len(input_types), it, len(aggs), ag, order, expected_groups)
+        if not self.h:
+            raise VectraError(lib.vgpu_last_error().decode())
+
+    def feed(self, key: DeviceColumn | None, inputs: Sequence[DeviceColumn], n_rows: int, pos_base: int = 0,
+             pred: VgpuProg | None = None, pred_cols: Sequence[DeviceColumn] = ()) -> None:
+        ins = col_array(inputs)
+        pc = col_array(pred_cols)
+        kc = C.byref(key.col) if key is not None else None
+        _vcheck(vgpu().vgpu_groupby_feed(self.h, kc, ins, n_rows, None, 0, pos_base,
+                                         C.byref(pred) if pred is not None else None, pc, len(pred_cols)))
+
+    def feed_join(self, n_rows: int, probe_cols: Sequence[DeviceColumn], build_cols: Sequence[DeviceColumn],
+                  inputs: Sequence[tuple], key: tuple[int, int], join: JoinTable | None = None,
+                  join_kind: int = JOIN_INNER, probe_key_col: int = 0, pred: VgpuProg | None = None,
+                  pos_base: int = 0) -> None:
+        sp = VgpuFeedSpec()
+        sp.n_rows = n_rows
+        sp.pos_base = pos_base
+        sp.pred = C.pointer(pred) if pred is not None else None
+        sp.join = join.h if join is not None else None
+        sp.join_kind = join_kind
+        sp.key_side, sp.key_col = key
+        sp.probe_key_col = probe_key_col
+        pcs, bcs = col_array(probe_cols), col_array(build_cols)
+        sp.n_probe_cols, sp.n_build_cols = len(probe_cols), len(build_cols)
+        sp.probe_cols = C.cast(pcs, C.POINTER(VgpuCol))
+        sp.build_cols = C.cast(bcs, C.POINTER(VgpuCol))
+        inp = (VgpuInput * max(1, len(inputs)))()
+        for i, t in enumerate(inputs):
+            if len(t) == 2:
+                inp[i].op, inp[i].side_a, inp[i].col_a = 0, t[0], t[1]
+            else:
+                inp[i].op, inp[i].side_a, inp[i].col_a, inp[i].side_b, inp[i].col_b = 1, t[0], t[1], t[2], t[3]
+        sp.n_inputs = len(inputs)
+        sp.inputs = C.cast(inp, C.POINTER(VgpuInput))
+        self._keep = (pcs, bcs, inp, pred)
+        _vcheck(vgpu().vgpu_groupby_feed_join(self.h, C.byref(sp)))
+
+    def finish(self) -> int:
+        n = vgpu().vgpu_groupby_finish(self.h)
+        _vcheck(n)
+        self.n_groups = n
+        return n
+
+    def fetch(self) -> dict[str, tuple[np.ndarray, np.ndarray]]:
+        n = self.n_groups
+        keys = np.zeros(max(n, 1), np.int64)
+        kv = np.zeros(max((n + 7) // 8, 1), np.uint8)
+        vals = [np.zeros(max(n, 1), np.float64) for _ in range(self.n_aggs)]
+        valid = [np.zeros(max((n + 7) // 8, 1), np.uint8) for _ in range(self.n_aggs)]
+        vp = (C.c_void_p * max(1, self.n_aggs))(*[v.ctypes.data for v in vals])
+        mp = (C.c_void_p * max(1, self.n_aggs))(*[m.ctypes.data for m in valid])
+        w = vgpu().vgpu_groupby_fetch(self.h, keys.ctypes.data, kv.ctypes.data, vp, mp)
+        _vcheck(w)
+        self.warnings = w
+        unpack = lambda b: np.unpackbits(b, bitorder="little")[:n].astype(bool)
+        out = {}
+        if self.key_type == VEC_DOUBLE:
+            out["key"] = (keys[:n].view(np.float64), unpack(kv))
+        elif self.key_type != VGPU_KEY_NONE:
+            out["key"] = (keys[:n], unpack(kv))
+        for i in range(self.n_aggs):
+            out[f"a{i}"] = (vals[i][:n], unpack(valid[i]))
+        return out
+
+    def free(self) -> None:
+        if self.h:
+            vgpu().vgpu_groupby_destroy(self.h)
+            self.h = None
