@@ -106,6 +106,14 @@ int vgpu_filter(const vgpu_prog *prog, const vgpu_col *cols, int n_cols,
                 int64_t n_rows, const int32_t *sel_in, int64_t n_sel_in,
                 int32_t *sel_out, int64_t *n_out);
 
+/* Host execution of the same program interpreter and fast path, over HOST
+ * columns: a test hook that lets the expression compiler and the device
+ * evaluator's semantics be checked without a GPU.  No operator node calls
+ * it. */
+int vgpu_eval_host(const vgpu_prog *prog, const vgpu_col *cols, int n_cols,
+                   int64_t n_rows, const int32_t *sel_in, int64_t n_sel_in,
+                   int32_t *sel_out, int64_t *n_out);
+
 /* ---- group-by --------------------------------------------------------- */
 enum { VGPU_AGG_COUNT = 0, VGPU_AGG_COUNT_STAR, VGPU_AGG_SUM, VGPU_AGG_MEAN,
        VGPU_AGG_MIN, VGPU_AGG_MAX };   /* vectra's AggKind numbering */
@@ -262,6 +270,14 @@ int vgpu_sort_pairs_u64(uint64_t *keys, uint32_t *values, int64_t n,
  * work issued between vgpu_timer_start() and vgpu_timer_stop(). */
 int    vgpu_timer_start(void);
 double vgpu_timer_stop(void);
+
+/* Per-launch timing of the main kernel of vgpu_filter /
+ * vgpu_groupby_feed / vgpu_groupby_feed_join: when enabled, each of those
+ * calls brackets its kernel with events on the library stream and adds the
+ * elapsed time here.  Returns the accumulated milliseconds and launch count
+ * since the last reset. */
+int    vgpu_kernel_timing(int enable);
+double vgpu_kernel_time_ms(int64_t *launches, int reset);
 
 #ifdef __cplusplus
 }

@@ -1298,6 +1298,74 @@ VecNode *oracle_join_node_create(VecNode *left, VecNode *right, JoinKind kind,
 }
 
 /* ==================================================================== */
+/* hidden mutate (src/project.c:51-252, src/scalar_ops.c:11-66)          */
+/* ==================================================================== */
+
+typedef struct {
+    VecNode  base;
+    VecNode *child;
+    char    *a, *b;
+} OProject;
+
+static VecBatch *oproject_next(VecNode *self) {
+    OProject *p = (OProject *)self;
+    VecBatch *in = p->child->next_batch(p->child);
+    if (!in) return NULL;
+    /* gather the logical rows, then evaluate a * b over them */
+    int64_t n = vec_batch_logical_rows(in);
+    int nc = in->n_cols;
+    VecBatch *out = vec_batch_alloc(nc + 1, n);
+    for (int c = 0; c < nc; c++) {
+        const VecArray *s = &in->columns[c];
+        VecArray d = vec_array_alloc(s->type, n);
+        for (int64_t i = 0; i < n; i++) put_value(&d, i, s, vec_batch_physical_row(in, i));
+        out->columns[c] = d;
+        out->col_names[c] = vx_strdup(in->col_names[c]);
+    }
+    out->n_cols = nc;
+    VecArray *prod = arith(find_col(out, p->a), find_col(out, p->b), '*');
+    out->n_cols = nc + 1;
+    out->columns[nc] = *prod;
+    free(prod);
+    out->col_names[nc] = vx_strdup(p->base.output_schema.col_names[nc]);
+    vec_batch_free(in);
+    return out;
+}
+
+static void oproject_free(VecNode *self) {
+    OProject *p = (OProject *)self;
+    p->child->free_node(p->child);
+    free(p->a);
+    free(p->b);
+    vec_schema_free(&p->base.output_schema);
+    free(p);
+}
+
+VecNode *oracle_project_mul_node_create(VecNode *child, const char *out_name,
+                                        const char *a_name, const char *b_name) {
+    OProject *p = (OProject *)calloc(1, sizeof *p);
+    const VecSchema *cs = &child->output_schema;
+    int ia = resolve(cs, a_name, "mutate"), ib = resolve(cs, b_name, "mutate");
+    int nc = cs->n_cols;
+    char **names = (char **)malloc(sizeof(char *) * (size_t)(nc + 1));
+    VecType *types = (VecType *)malloc(sizeof(VecType) * (size_t)(nc + 1));
+    for (int c = 0; c < nc; c++) { names[c] = cs->col_names[c]; types[c] = cs->col_types[c]; }
+    names[nc] = (char *)out_name;
+    VecType t = common_type(cs->col_types[ia], cs->col_types[ib]);
+    types[nc] = t == VEC_BOOL ? VEC_INT64 : t;
+    p->base.output_schema = vec_schema_create(nc + 1, names, types);
+    free(names);
+    free(types);
+    p->child = child;
+    p->a = vx_strdup(a_name);
+    p->b = vx_strdup(b_name);
+    p->base.next_batch = oproject_next;
+    p->base.free_node = oproject_free;
+    p->base.kind = "ProjectNode";
+    return &p->base;
+}
+
+/* ==================================================================== */
 /* exact summation                                                        */
 /* ==================================================================== */
 

@@ -35,6 +35,13 @@ VecNode *oracle_join_node_create(VecNode *left, VecNode *right,
                                  JoinKind kind, int n_keys, JoinKey *keys,
                                  const char *suffix_x, const char *suffix_y);
 
+/* The hidden mutate summarise() inserts for a nested aggregate input such
+ * as sum(value * weight) (R/verbs.R:266-284): a ProjectNode keeping every
+ * child column and appending out_name = a * b (src/project.c:51-252 with
+ * src/scalar_ops.c:11-66). */
+VecNode *oracle_project_mul_node_create(VecNode *child, const char *out_name,
+                                        const char *a_name, const char *b_name);
+
 /* FNV-1a of one key value exactly as src/hash.c:12-45 (NA -> offset^0xFF). */
 uint64_t oracle_hash_value(const VecArray *arr, int64_t row);
 

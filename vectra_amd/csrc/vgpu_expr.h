@@ -31,7 +31,7 @@ struct PredInfo {
 
 PredInfo vg_pred_info(const vgpu_prog *p);
 
-__device__ __forceinline__ int vg_cmp_apply(int c, int op) {
+__host__ __device__ __forceinline__ int vg_cmp_apply(int c, int op) {
     switch (op) {
     case VGPU_OP_EQ: return c == 0;
     case VGPU_OP_NE: return c != 0;
@@ -42,11 +42,17 @@ __device__ __forceinline__ int vg_cmp_apply(int c, int op) {
     }
 }
 
-__device__ __forceinline__ int vg_cmp3_i64(int64_t a, int64_t b) { return (a > b) - (a < b); }
-__device__ __forceinline__ int vg_cmp3_f64(double a, double b) { return (a > b) - (a < b); }
+__host__ __device__ __forceinline__ int vg_cmp3_i64(int64_t a, int64_t b) { return (a > b) - (a < b); }
+__host__ __device__ __forceinline__ int vg_cmp3_f64(double a, double b) { return (a > b) - (a < b); }
+
+// type of the value an instruction produces (a comparison's type field
+// names its operands; its result, like the boolean operators', is bool)
+__host__ __device__ __forceinline__ int vg_result_type(const vgpu_insn &in) {
+    return (in.op >= VGPU_OP_EQ && in.op <= VGPU_OP_ISNA) ? VGPU_BOOL : in.type;
+}
 
 // value of a loaded column in the comparison type
-__device__ __forceinline__ bool vg_fast_pred(const PredInfo &pi, const ColSet &cs, int64_t row) {
+__host__ __device__ __forceinline__ bool vg_fast_pred(const PredInfo &pi, const ColSet &cs, int64_t row) {
     const vgpu_col &c = cs.c[pi.col];
     if (!vg_valid(c.validity, row)) return false;
     uint64_t bits = vg_load_bits(c, row);
@@ -61,10 +67,11 @@ __device__ __forceinline__ bool vg_fast_pred(const PredInfo &pi, const ColSet &c
 }
 
 // General VM.  regs/regv: LDS arrays [n_insn][stride], this thread's column
-// at index tid.  Returns the result register's (value, valid).
-__device__ inline void vg_vm_eval(const vgpu_prog &p, const ColSet &cs, int64_t row,
-                                  uint64_t *regs, uint8_t *regv, int stride, int tid,
-                                  uint64_t &out_val, bool &out_valid) {
+// at index tid.  Returns the result register's (value, valid).  Also built
+// for the host (vgpu_eval_host) so the compiler and VM are testable on CPU.
+__host__ __device__ inline void vg_vm_eval(const vgpu_prog &p, const ColSet &cs, int64_t row,
+                                           uint64_t *regs, uint8_t *regv, int stride, int tid,
+                                           uint64_t &out_val, bool &out_valid) {
     for (int i = 0; i < p.n_insn; i++) {
         const vgpu_insn in = p.insn[i];
         uint64_t v = 0;
@@ -90,7 +97,7 @@ __device__ inline void vg_vm_eval(const vgpu_prog &p, const ColSet &cs, int64_t 
         case VGPU_OP_NA: ok = false; break;
         case VGPU_OP_CAST: {
             ok = av;
-            uint8_t from = p.insn[in.a].type;
+            int from = vg_result_type(p.insn[in.a]);
             if (in.type == VGPU_DOUBLE && from != VGPU_DOUBLE)
                 v = vg_f64_to_bits((double)(int64_t)a);
             else
@@ -168,7 +175,7 @@ __device__ inline void vg_vm_eval(const vgpu_prog &p, const ColSet &cs, int64_t 
 }
 
 // predicate test of one row: valid && true
-__device__ __forceinline__ bool vg_pred_row(const PredInfo &pi, const vgpu_prog &p, const ColSet &cs,
+__host__ __device__ __forceinline__ bool vg_pred_row(const PredInfo &pi, const vgpu_prog &p, const ColSet &cs,
                                             int64_t row, uint64_t *regs, uint8_t *regv, int stride, int tid) {
     if (!pi.present) return true;
     if (pi.fast) return vg_fast_pred(pi, cs, row);

@@ -132,6 +132,27 @@ k_filter_write(int64_t n_logical, const int32_t *sel_in, const uint64_t *mask,
     }
 }
 
+extern "C" int vgpu_eval_host(const vgpu_prog *prog, const vgpu_col *cols, int n_cols, int64_t n_rows,
+                              const int32_t *sel_in, int64_t n_sel_in, int32_t *sel_out, int64_t *n_out) {
+    if (!prog || prog->n_insn <= 0 || prog->n_insn > VGPU_MAX_INSN || n_cols < 0 || n_cols > VGPU_MAX_COLS) {
+        vg_set_error("vgpu_eval_host: invalid program");
+        return -1;
+    }
+    ColSet cs;
+    memset(&cs, 0, sizeof cs);
+    for (int i = 0; i < n_cols; i++) cs.c[i] = cols[i];
+    PredInfo pi = vg_pred_info(prog);
+    uint64_t regs[VGPU_MAX_INSN];
+    uint8_t regv[VGPU_MAX_INSN];
+    int64_t n = sel_in ? n_sel_in : n_rows, k = 0;
+    for (int64_t li = 0; li < n; li++) {
+        int64_t row = sel_in ? sel_in[li] : li;
+        if (vg_pred_row(pi, *prog, cs, row, regs, regv, 1, 0)) sel_out[k++] = (int32_t)row;
+    }
+    *n_out = k;
+    return 0;
+}
+
 static std::mutex g_filter_mu;
 static DevBuf g_mask, g_counts, g_offs, g_scan, g_total;
 
@@ -172,9 +193,11 @@ extern "C" int vgpu_filter(const vgpu_prog *prog, const vgpu_col *cols, int n_co
         return -1;
     hipStream_t s = vg_stream();
     size_t lds = pi.fast ? 0 : vg_vm_lds_bytes(prog, FT_THREADS);
+    vg_kt_begin();
     k_filter_mask<<<(unsigned)n_tiles, FT_THREADS, lds, s>>>(pi, *prog, cs, n_logical, sel_in,
                                                               g_mask.as<uint64_t>(), g_counts.as<uint32_t>());
     VG_LAUNCH_CHECK();
+    vg_kt_end();
     if (vg_scan_u32_to_u64(g_counts.as<uint32_t>(), g_offs.as<uint64_t>(), n_tiles, g_total.as<uint64_t>(), g_scan))
         return -1;
     k_filter_write<<<(unsigned)n_tiles, FT_THREADS, 0, s>>>(n_logical, sel_in, g_mask.as<uint64_t>(),

@@ -849,8 +849,10 @@ extern "C" int vgpu_groupby_feed(vgpu_groupby *g, const vgpu_col *key, const vgp
     A.vm_lds = (int32_t)vm;
     A.lds_slots = gb_lds_slots(g, vm);
     size_t lds = vm + (size_t)A.lds_slots * (g->L.RH + g->L.RC) * 8;
+    vg_kt_begin();
     k_gb_feed<<<gb_grid(n_logical, A.lds_slots), GB_THREADS, lds, vg_stream()>>>(g->L, A, ins, pi, prog, pcs);
     VG_LAUNCH_CHECK();
+    vg_kt_end();
     g->any_fed = 1;
     return 0;
 }
@@ -955,8 +957,10 @@ extern "C" int vgpu_groupby_feed_join(vgpu_groupby *g, const vgpu_feed_spec *sp)
     A.vm_lds = (int32_t)vm;
     A.lds_slots = gb_lds_slots(g, vm);
     size_t lds = vm + (size_t)A.lds_slots * (g->L.RH + g->L.RC) * 8;
+    vg_kt_begin();
     k_gb_feed_join<<<gb_grid(n_logical, A.lds_slots), GB_THREADS, lds, vg_stream()>>>(g->L, A, cp, jv, pi, prog);
     VG_LAUNCH_CHECK();
+    vg_kt_end();
     g->any_fed = 1;
     return 0;
 }

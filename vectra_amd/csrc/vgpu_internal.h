@@ -14,6 +14,9 @@
 void vg_set_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
 hipStream_t vg_stream();
 int vg_ensure_init();
+// bracket a main kernel launch for vgpu_kernel_time_ms (no-ops unless enabled)
+void vg_kt_begin();
+void vg_kt_end();
 
 #define VG_TRY(call)                                                          \
     do {                                                                      \
@@ -109,19 +112,19 @@ __device__ __forceinline__ int vg_lane_prefix(uint64_t mask) {
 
 __device__ __forceinline__ uint64_t vg_ballot(bool p) { return __ballot(p); }
 
-__device__ __forceinline__ bool vg_valid(const uint8_t *validity, int64_t row) {
+__host__ __device__ __forceinline__ bool vg_valid(const uint8_t *validity, int64_t row) {
     return validity == nullptr || ((validity[row >> 3] >> (row & 7)) & 1);
 }
 
 // raw 8-byte view of a row of an int64/double column, or the 0/1 byte of a
 // bool column widened to 8 bytes
-__device__ __forceinline__ uint64_t vg_load_bits(const vgpu_col &c, int64_t row) {
+__host__ __device__ __forceinline__ uint64_t vg_load_bits(const vgpu_col &c, int64_t row) {
     if (c.type == VGPU_BOOL) return (uint64_t)static_cast<const uint8_t *>(c.data)[row];
     return static_cast<const uint64_t *>(c.data)[row];
 }
 
-__device__ __forceinline__ double vg_bits_to_f64(uint64_t b) { return __longlong_as_double((long long)b); }
-__device__ __forceinline__ uint64_t vg_f64_to_bits(double d) { return (uint64_t)__double_as_longlong(d); }
+__host__ __device__ __forceinline__ double vg_bits_to_f64(uint64_t b) { return __builtin_bit_cast(double, b); }
+__host__ __device__ __forceinline__ uint64_t vg_f64_to_bits(double d) { return __builtin_bit_cast(uint64_t, d); }
 
 // order-preserving map of a double's bits onto unsigned integers
 // (-0 maps just below +0; only used for non-NaN values)

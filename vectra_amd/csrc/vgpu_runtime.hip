@@ -127,6 +127,41 @@ extern "C" double vgpu_timer_stop(void) {
     return (double)ms;
 }
 
+// per-launch kernel timing (off by default: it synchronizes each launch)
+static int g_kt_on = 0;
+static hipEvent_t g_k0 = nullptr, g_k1 = nullptr;
+static double g_kt_ms = 0.0;
+static int64_t g_kt_n = 0;
+
+extern "C" int vgpu_kernel_timing(int enable) {
+    if (vg_ensure_init()) return -1;
+    if (!g_k0) VG_TRY(hipEventCreate(&g_k0));
+    if (!g_k1) VG_TRY(hipEventCreate(&g_k1));
+    g_kt_on = enable;
+    return 0;
+}
+
+extern "C" double vgpu_kernel_time_ms(int64_t *launches, int reset) {
+    double ms = g_kt_ms;
+    if (launches) *launches = g_kt_n;
+    if (reset) { g_kt_ms = 0.0; g_kt_n = 0; }
+    return ms;
+}
+
+void vg_kt_begin() {
+    if (g_kt_on) (void)hipEventRecord(g_k0, g_stream);
+}
+
+void vg_kt_end() {
+    if (!g_kt_on) return;
+    if (hipEventRecord(g_k1, g_stream) != hipSuccess || hipEventSynchronize(g_k1) != hipSuccess) return;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, g_k0, g_k1) == hipSuccess) {
+        g_kt_ms += ms;
+        g_kt_n++;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // block-level scan helpers (256 threads = 4 waves)
 // ---------------------------------------------------------------------------

@@ -235,13 +235,11 @@ static int cx_emit(Cx *c, const VecExpr *e) {
         l = cx_cast(c, l, t);
         r = cx_cast(c, r, t);
         uint8_t op = cmp_opcode(e->op, e->op2);
-        if (!op) {
-            /* FALSE wherever both sides are valid: (l != l) is never true
-             * for int64; NaN-safe for doubles since cmp is computed as
-             * (a>b)-(a<b) = 0 -> NE false */
-            return cx_push(c, VGPU_OP_NE, t, l, l);
-        }
-        return cx_push(c, op, t, l, r);
+        /* an unknown operator pair is FALSE wherever both sides are valid:
+         * l != l, decided on (a>b)-(a<b) = 0, is never true */
+        int k = op ? cx_push(c, op, t, l, r) : cx_push(c, VGPU_OP_NE, t, l, l);
+        c->rtype[k] = VEC_BOOL;   /* the instruction's type field is the operand type */
+        return k;
     }
     case EXPR_BOOL: {
         if (e->op == '!') {
@@ -288,6 +286,17 @@ void vx_compile_expr(const VecExpr *e, const VecBatch *batch, CompiledExpr *out)
         (void)k;
     }
     out->result_type = c.rtype[out->prog.n_insn - 1];
+}
+
+int vx_try_compile_expr(const VecExpr *e, const VecBatch *batch, CompiledExpr *out) {
+    vx_frame f;
+    vx_frame_push(&f);
+    if (setjmp(f.jb) == 0) {
+        vx_compile_expr(e, batch, out);
+        vx_frame_pop(&f);
+        return 0;
+    }
+    return -1;
 }
 
 /* ------------------------------------------------------------------ */

@@ -59,6 +59,8 @@ typedef struct {
 } CompiledExpr;
 
 void vx_compile_expr(const VecExpr *e, const VecBatch *batch, CompiledExpr *out);
+/* the same, returning -1 (message in vx_last_error) instead of raising */
+int vx_try_compile_expr(const VecExpr *e, const VecBatch *batch, CompiledExpr *out);
 
 /* concatenates the logical rows of batches (consumed) into dense columns
  * of the given schema; returns the row count */

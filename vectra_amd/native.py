@@ -198,6 +198,8 @@ def vgpu() -> C.CDLL:
         _proto(lib, "vgpu_synchronize", C.c_int, [])
         _proto(lib, "vgpu_filter", C.c_int, [C.POINTER(VgpuProg), C.POINTER(VgpuCol), C.c_int, C.c_int64, C.c_void_p,
                                              C.c_int64, C.c_void_p, C.POINTER(C.c_int64)])
+        _proto(lib, "vgpu_eval_host", C.c_int, [C.POINTER(VgpuProg), C.POINTER(VgpuCol), C.c_int, C.c_int64,
+                                                C.c_void_p, C.c_int64, C.c_void_p, C.POINTER(C.c_int64)])
         _proto(lib, "vgpu_groupby_create", C.c_void_p, [C.c_int, C.c_int, C.POINTER(C.c_int32), C.c_int,
                                                         C.POINTER(VgpuAgg), C.c_int, C.c_int64])
         _proto(lib, "vgpu_groupby_feed", C.c_int, [C.c_void_p, C.POINTER(VgpuCol), C.POINTER(VgpuCol), C.c_int64,
@@ -221,6 +223,8 @@ def vgpu() -> C.CDLL:
         _proto(lib, "vgpu_sort_pairs_u64", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_int])
         _proto(lib, "vgpu_timer_start", C.c_int, [])
         _proto(lib, "vgpu_timer_stop", C.c_double, [])
+        _proto(lib, "vgpu_kernel_timing", C.c_int, [C.c_int])
+        _proto(lib, "vgpu_kernel_time_ms", C.c_double, [C.POINTER(C.c_int64), C.c_int])
         _libs["vgpu"] = lib
     return _libs["vgpu"]
 
@@ -233,13 +237,62 @@ def _node_protos(lib: C.CDLL, prefix: str) -> None:
                                                       C.c_char_p])
 
 
+class CompiledExpr(C.Structure):
+    _fields_ = [("prog", VgpuProg), ("n_cols", C.c_int), ("col_idx", C.c_int * 8), ("result_type", C.c_int)]
+
+
 def nodes() -> C.CDLL:
     if "nodes" not in _libs:
         vgpu()
         lib = C.CDLL(str(_need(LIB / "libvectra_gpu.so")))
         _node_protos(lib, "")
+        _proto(lib, "vx_try_compile_expr", C.c_int, [EXPR_P, BATCH_P, C.POINTER(CompiledExpr)])
         _libs["nodes"] = lib
     return _libs["nodes"]
+
+
+def host_eval_batches(table: "Table", pred: tuple) -> list[np.ndarray]:
+    """Compiles pred per batch exactly as the GPU FilterNode does and runs the
+    program on the host interpreter (vgpu_eval_host): a CPU check of the
+    compiler and the evaluator.  Returns the non-empty selection vectors."""
+    lib, vg = nodes(), vgpu()
+    scan = table.scan()
+    out = []
+    while True:
+        b = BATCH_P()
+        if core().vx_try_next(scan, C.byref(b)) != 0:
+            raise VectraError(core().vx_last_error().decode())
+        if not b:
+            break
+        e = expr(pred)
+        cx = CompiledExpr()
+        rc = lib.vx_try_compile_expr(e, b, C.byref(cx))
+        core().vec_expr_free(e)
+        if rc != 0:
+            core().vec_batch_free(b)
+            raise VectraError(core().vx_last_error().decode())
+        if cx.result_type != VEC_BOOL:
+            core().vec_batch_free(b)
+            raise VectraError("filter predicate must evaluate to boolean")
+        bt = b.contents
+        cols = (VgpuCol * 8)()
+        for s in range(cx.n_cols):
+            a = bt.columns[cx.col_idx[s]]
+            cols[s].type = a.type
+            cols[s].data = C.cast(a.buf.i64, C.c_void_p)
+            cols[s].validity = C.cast(a.validity, C.c_void_p) if a.validity else None
+        n = bt.sel_n if bt.sel else bt.n_rows
+        sel = np.zeros(max(n, 1), np.int32)
+        k = C.c_int64()
+        if n > 0:
+            _vcheck(vg.vgpu_eval_host(C.byref(cx.prog), cols, cx.n_cols, bt.n_rows,
+                                      C.cast(bt.sel, C.c_void_p) if bt.sel else None, bt.sel_n, sel.ctypes.data,
+                                      C.byref(k)))
+        if k.value:
+            out.append(sel[:k.value].copy())
+        core().vec_batch_free(b)
+    core().vx_node_free(scan)
+    return out
 
 
 def load_oracle(path: Path) -> C.CDLL:
@@ -250,6 +303,7 @@ def load_oracle(path: Path) -> C.CDLL:
         _node_protos(lib, "oracle_")
         _proto(lib, "oracle_exact_sum", C.c_double, [C.POINTER(C.c_double), C.c_int64])
         _proto(lib, "oracle_hash_value", C.c_uint64, [C.POINTER(VecArray), C.c_int64])
+        _proto(lib, "oracle_project_mul_node_create", NODE_P, [NODE_P, C.c_char_p, C.c_char_p, C.c_char_p])
         _libs["oracle"] = lib
     return _libs["oracle"]
 
