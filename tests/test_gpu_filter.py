@@ -88,7 +88,8 @@ def test_filter_config1_size(gpu, oracle):
     n = 1_000_000
     r = rng(2024)
     x = r.normal(size=n)
-    t = lambda: gpu.Table({"id": np.arange(n), "key": r.integers(0, 1000, n), "x": x}, rg_rows=100_000)
+    key = r.integers(0, 1000, n)
+    t = lambda: gpu.Table({"id": np.arange(n), "key": key, "x": x}, rg_rows=100_000)
     p = ("cmp", ">", ("col", "x"), ("dbl", 0.0))
     got = run(gpu, "gpu", t(), p)
     exp = run(gpu, "oracle", t(), p)

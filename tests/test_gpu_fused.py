@@ -16,7 +16,9 @@ def star_schema(seed, n_fact, n_dim, p_absent=0.1, dup=False):
     r = rng(seed)
     dim_key = (r.permutation(n_dim * 3)[:n_dim].astype(np.int64) * 13 - 7)
     if dup:
-        dim_key[r.random(n_dim) < 0.2] = dim_key[0]
+        # duplicate build keys on low-frequency Zipf ranks: a few matches each
+        for k in range(200, 260):
+            dim_key[k * 7 + 1: k * 7 + 4] = dim_key[k * 7]
     attr = r.integers(0, 100, n_dim)
     weight = r.random(n_dim)
     ranks = np.minimum(r.zipf(1.1, n_fact) - 1, n_dim - 1)

@@ -89,41 +89,44 @@ struct InVal {
     bool valid;
 };
 
-__device__ __forceinline__ void gb_update(const GbLayout &L, uint64_t *hot, uint64_t *cold,
-                                          uint64_t pos, const InVal *iv) {
+// the per-row words: n and the first position
+__device__ __forceinline__ void gb_update_row(const GbLayout &L, uint64_t *hot, uint64_t pos) {
     gb_atomic_add(&hot[1], 1);
     if (L.w_fpos >= 0) gb_atomic_min(&hot[L.w_fpos], pos);
-    for (int i = 0; i < L.n_inputs; i++) {
-        if (!iv[i].valid) {
-            gb_atomic_add(&cold[L.c_na[i]], 1);
-            continue;
-        }
-        uint64_t b = iv[i].bits;
-        int t = L.in_type[i];
-        if (L.w_fvpos[i] >= 0) gb_atomic_min(&hot[L.w_fvpos[i]], pos);
-        if (L.w_sum[i] >= 0) {
-            if (t == VGPU_DOUBLE) gb_atomic_addf(&hot[L.w_sum[i]], vg_bits_to_f64(b));
-            else gb_atomic_add(&hot[L.w_sum[i]], b);   // int64 wraps; bool adds 0/1
-        }
-        if (t == VGPU_BOOL) continue;   // the reference never updates min/max for bool
-        if (L.w_min[i] < 0 && L.w_max[i] < 0) continue;
-        uint64_t o;
-        if (t == VGPU_DOUBLE) {
-            if ((b & 0x7FFFFFFFFFFFFFFFull) > 0x7FF0000000000000ull) {
-                gb_atomic_min(&cold[L.c_nan[i]], pos);
-                continue;
-            }
-            if ((b & 0x7FFFFFFFFFFFFFFFull) == 0) {
-                gb_atomic_min(&cold[(b >> 63) ? L.c_zneg[i] : L.c_zpos[i]], pos);
-                b = 0;
-            }
-            o = vg_f64_ordered(b);
-        } else {
-            o = vg_i64_ordered(b);
-        }
-        if (L.w_min[i] >= 0) gb_atomic_min(&hot[L.w_min[i]], o);
-        if (L.w_max[i] >= 0) gb_atomic_max(&hot[L.w_max[i]], o);
+}
+
+// the words of input i (no local arrays: everything stays in registers)
+__device__ __forceinline__ void gb_update_input(const GbLayout &L, uint64_t *hot, uint64_t *cold, int i,
+                                                uint64_t pos, InVal v) {
+    if (!v.valid) {
+        gb_atomic_add(&cold[L.c_na[i]], 1);
+        return;
     }
+    uint64_t b = v.bits;
+    int t = L.in_type[i];
+    if (L.w_fvpos[i] >= 0) gb_atomic_min(&hot[L.w_fvpos[i]], pos);
+    if (L.w_sum[i] >= 0) {
+        if (t == VGPU_DOUBLE) gb_atomic_addf(&hot[L.w_sum[i]], vg_bits_to_f64(b));
+        else gb_atomic_add(&hot[L.w_sum[i]], b);   // int64 wraps; bool adds 0/1
+    }
+    if (t == VGPU_BOOL) return;   // the reference never updates min/max for bool
+    if (L.w_min[i] < 0 && L.w_max[i] < 0) return;
+    uint64_t o;
+    if (t == VGPU_DOUBLE) {
+        if ((b & 0x7FFFFFFFFFFFFFFFull) > 0x7FF0000000000000ull) {
+            gb_atomic_min(&cold[L.c_nan[i]], pos);
+            return;
+        }
+        if ((b & 0x7FFFFFFFFFFFFFFFull) == 0) {
+            gb_atomic_min(&cold[(b >> 63) ? L.c_zneg[i] : L.c_zpos[i]], pos);
+            b = 0;
+        }
+        o = vg_f64_ordered(b);
+    } else {
+        o = vg_i64_ordered(b);
+    }
+    if (L.w_min[i] >= 0) gb_atomic_min(&hot[L.w_min[i]], o);
+    if (L.w_max[i] >= 0) gb_atomic_max(&hot[L.w_max[i]], o);
 }
 
 // merge a whole record (LDS partial -> HBM)
@@ -231,11 +234,58 @@ struct FeedArgs {
     int32_t vm_lds;        // bytes of VM registers at the start of dyn LDS
 };
 
-__device__ __forceinline__ void gb_load_inputs(const GbLayout &L, const InSet &ins, int64_t row, InVal *iv) {
+// The LDS-private table holds S hashed records plus two reserved ones (NA
+// key, EMPTY-valued int64 key), like the HBM table's slots cap / cap+1, so
+// that no row class funnels every block into one HBM record.
+__device__ __forceinline__ void gb_lds_init(const GbLayout &L, uint64_t *lhot, uint64_t *lcold, int S,
+                                            uint64_t empty) {
+    for (int s = threadIdx.x; s < S + 2; s += GB_THREADS) {
+        uint64_t *h = lhot + (size_t)s * L.RH;
+        h[0] = empty;
+        for (int w = 1; w < L.RH; w++) h[w] = ((L.hot_init_ones >> w) & 1) ? ~0ull : 0ull;
+        uint64_t *c = lcold + (size_t)s * L.RC;
+        for (int w = 0; w < L.RC; w++) c[w] = ((L.cold_init_ones >> w) & 1) ? ~0ull : 0ull;
+    }
+    __syncthreads();
+}
+
+// LDS record of a key: reserved records for NA / EMPTY-valued keys; -1 for
+// NaN keys (each its own group, kept in HBM) or an exhausted probe window
+__device__ __forceinline__ int64_t gb_lds_slot(uint64_t *lhot, int S, int RH, int cls, uint64_t kn, uint64_t empty) {
+    if (cls == VG_KEY_NA) return S;
+    if (cls == VG_KEY_SPECIAL) return S + 1;
+    if (cls == VG_KEY_NAN) return -1;
+    return gb_slot(lhot, S, RH, kn, empty, GB_LDS_PROBES, nullptr);
+}
+
+__device__ __forceinline__ void gb_lds_merge(const GbLayout &L, const uint64_t *lhot, const uint64_t *lcold, int S,
+                                             uint64_t empty, uint64_t *hot, uint64_t *cold, int64_t cap,
+                                             uint64_t *ctl) {
+    __syncthreads();
+    for (int s = threadIdx.x; s < S + 2; s += GB_THREADS) {
+        const uint64_t *h = lhot + (size_t)s * L.RH;
+        int64_t gs;
+        if (s < S) {
+            if (h[0] == empty) continue;
+            gs = gb_global_slot(L, hot, cap, VG_KEY_REGULAR, h[0], 0, ctl);
+        } else {
+            if (h[1] == 0) continue;
+            gs = cap + (s - S);
+        }
+        if (gs >= 0) gb_merge(L, hot + (size_t)gs * L.RH, cold + (size_t)gs * L.RC, h, lcold + (size_t)s * L.RC);
+    }
+}
+
+// all words of one row into one record (inlined separately for the LDS and
+// the HBM record so each call site keeps its address space)
+__device__ __forceinline__ void gb_apply(const GbLayout &L, const InSet &ins, int64_t row, uint64_t pos,
+                                         uint64_t *h, uint64_t *c) {
+    gb_update_row(L, h, pos);
     for (int i = 0; i < L.n_inputs; i++) {
-        const vgpu_col &c = ins.c[i];
-        iv[i].valid = vg_valid(c.validity, row);
-        iv[i].bits = iv[i].valid ? vg_load_bits(c, row) : 0;
+        InVal v;
+        v.valid = vg_valid(ins.c[i].validity, row);
+        v.bits = v.valid ? vg_load_bits(ins.c[i], row) : 0;
+        gb_update_input(L, h, c, i, pos, v);
     }
 }
 
@@ -246,53 +296,31 @@ k_gb_feed(GbLayout L, FeedArgs A, InSet ins, PredInfo pi, vgpu_prog prog, ColSet
     uint64_t *regs = reinterpret_cast<uint64_t *>(dyn);
     uint8_t *regv = dyn + (size_t)prog.n_insn * GB_THREADS * 8;
     uint64_t *lhot = reinterpret_cast<uint64_t *>(dyn + A.vm_lds);
-    uint64_t *lcold = lhot + (size_t)A.lds_slots * L.RH;
     const int S = A.lds_slots;
+    uint64_t *lcold = lhot + (size_t)(S + 2) * L.RH;
     const uint64_t empty = L.key_is_double ? VG_EMPTY_F64 : VG_EMPTY_I64;
-    if (S) {
-        for (int s = tid; s < S; s += GB_THREADS) {
-            uint64_t *h = lhot + (size_t)s * L.RH;
-            h[0] = empty;
-            for (int w = 1; w < L.RH; w++) h[w] = ((L.hot_init_ones >> w) & 1) ? ~0ull : 0ull;
-            uint64_t *c = lcold + (size_t)s * L.RC;
-            for (int w = 0; w < L.RC; w++) c[w] = ((L.cold_init_ones >> w) & 1) ? ~0ull : 0ull;
-        }
-        __syncthreads();
-    }
-    InVal iv[VGPU_MAX_INPUTS];
+    if (S) gb_lds_init(L, lhot, lcold, S, empty);
     const int64_t stride = (int64_t)gridDim.x * GB_THREADS;
     for (int64_t li = (int64_t)blockIdx.x * GB_THREADS + tid; li < A.n_logical; li += stride) {
         int64_t row = A.sel ? (int64_t)A.sel[li] : li;
         if (!vg_pred_row(pi, prog, pcs, row, regs, regv, GB_THREADS, tid)) continue;
         uint64_t pos = (uint64_t)(A.pos_base + li);
-        uint64_t kn = 0, raw = 0;
+        uint64_t kn = 0;
         int cls = VG_KEY_REGULAR;
         if (!L.key_none) {
             cls = vg_key_class(A.key, row, kn);
-            if (L.key_is_double && cls == VG_KEY_REGULAR && kn == 0) {
-                raw = vg_load_bits(A.key, row);
-                gb_note_zero_key(A.ctl, raw, pos);
-            }
+            if (L.key_is_double && cls == VG_KEY_REGULAR && kn == 0)
+                gb_note_zero_key(A.ctl, vg_load_bits(A.key, row), pos);
         }
-        gb_load_inputs(L, ins, row, iv);
-        int64_t ls = -1;
-        if (S && cls == VG_KEY_REGULAR) ls = gb_slot(lhot, S, L.RH, kn, empty, GB_LDS_PROBES, nullptr);
+        int64_t ls = S ? gb_lds_slot(lhot, S, L.RH, cls, kn, empty) : -1;
         if (ls >= 0) {
-            gb_update(L, lhot + (size_t)ls * L.RH, lcold + (size_t)ls * L.RC, pos, iv);
+            gb_apply(L, ins, row, pos, lhot + (size_t)ls * L.RH, lcold + (size_t)ls * L.RC);
         } else {
             int64_t gs = gb_global_slot(L, A.hot, A.cap, cls, kn, pos, A.ctl);
-            if (gs >= 0) gb_update(L, A.hot + (size_t)gs * L.RH, A.cold + (size_t)gs * L.RC, pos, iv);
+            if (gs >= 0) gb_apply(L, ins, row, pos, A.hot + (size_t)gs * L.RH, A.cold + (size_t)gs * L.RC);
         }
     }
-    if (S) {
-        __syncthreads();
-        for (int s = tid; s < S; s += GB_THREADS) {
-            const uint64_t *h = lhot + (size_t)s * L.RH;
-            if (h[0] == empty) continue;
-            int64_t gs = gb_global_slot(L, A.hot, A.cap, VG_KEY_REGULAR, h[0], 0, A.ctl);
-            if (gs >= 0) gb_merge(L, A.hot + (size_t)gs * L.RH, A.cold + (size_t)gs * L.RC, h, lcold + (size_t)s * L.RC);
-        }
-    }
+    if (S) gb_lds_merge(L, lhot, lcold, S, empty, A.hot, A.cold, A.cap, A.ctl);
 }
 
 // fused probe -> aggregate ----------------------------------------------------
@@ -317,22 +345,22 @@ struct JoinFeedArgs {
 };
 
 struct ColPair {
-    vgpu_col probe[VGPU_MAX_COLS];
-    vgpu_col build[VGPU_MAX_COLS];
+    ColSet probe;
+    ColSet build;
 };
 
 // value of (side, col) at the current pair; build side is NA when br < 0
 __device__ __forceinline__ InVal jf_col(const ColPair &cp, int side, int col, int64_t pr, int64_t br) {
     InVal v;
     if (side == 0) {
-        const vgpu_col &c = cp.probe[col];
+        const vgpu_col &c = cp.probe.c[col];
         v.valid = vg_valid(c.validity, pr);
         v.bits = v.valid ? vg_load_bits(c, pr) : 0;
     } else if (br < 0) {
         v.valid = false;
         v.bits = 0;
     } else {
-        const vgpu_col &c = cp.build[col];
+        const vgpu_col &c = cp.build.c[col];
         v.valid = vg_valid(c.validity, br);
         v.bits = v.valid ? vg_load_bits(c, br) : 0;
     }
@@ -340,7 +368,7 @@ __device__ __forceinline__ InVal jf_col(const ColPair &cp, int side, int col, in
 }
 
 __device__ __forceinline__ int jf_type(const ColPair &cp, int side, int col) {
-    return side == 0 ? cp.probe[col].type : cp.build[col].type;
+    return side == 0 ? cp.probe.c[col].type : cp.build.c[col].type;
 }
 
 // product in vectra's arithmetic: common type (bool -> int64), NA if either
@@ -360,24 +388,27 @@ __device__ __forceinline__ InVal jf_mul(InVal a, int ta, InVal b, int tb) {
     return r;
 }
 
+__device__ __forceinline__ void jf_apply(const GbLayout &L, const JoinFeedArgs &A, const ColPair &cp, int64_t pr,
+                                         int64_t br, uint64_t pos, uint64_t *h, uint64_t *c) {
+    gb_update_row(L, h, pos);
+    for (int i = 0; i < A.n_inputs; i++) {
+        const vgpu_input &in = A.in[i];
+        InVal v = jf_col(cp, in.side_a, in.col_a, pr, br);
+        if (in.op == 1) {
+            InVal b = jf_col(cp, in.side_b, in.col_b, pr, br);
+            v = jf_mul(v, jf_type(cp, in.side_a, in.col_a), b, jf_type(cp, in.side_b, in.col_b));
+        }
+        gb_update_input(L, h, c, i, pos, v);
+    }
+}
+
 __device__ __forceinline__ void jf_emit(const GbLayout &L, const JoinFeedArgs &A, const ColPair &cp,
                                         int64_t pr, int64_t br, uint64_t pos,
                                         uint64_t *lhot, uint64_t *lcold, int S, uint64_t empty) {
-    InVal iv[VGPU_MAX_INPUTS];
-    for (int i = 0; i < A.n_inputs; i++) {
-        const vgpu_input &in = A.in[i];
-        InVal a = jf_col(cp, in.side_a, in.col_a, pr, br);
-        if (in.op == 1) {
-            InVal b = jf_col(cp, in.side_b, in.col_b, pr, br);
-            iv[i] = jf_mul(a, jf_type(cp, in.side_a, in.col_a), b, jf_type(cp, in.side_b, in.col_b));
-        } else {
-            iv[i] = a;
-        }
-    }
     uint64_t kn = 0;
     int cls = VG_KEY_REGULAR;
     if (!L.key_none) {
-        const vgpu_col &kc = A.key_side == 0 ? cp.probe[A.key_col] : cp.build[A.key_col];
+        const vgpu_col &kc = A.key_side == 0 ? cp.probe.c[A.key_col] : cp.build.c[A.key_col];
         int64_t kr = A.key_side == 0 ? pr : br;
         if (kr < 0) {
             cls = VG_KEY_NA;
@@ -387,13 +418,12 @@ __device__ __forceinline__ void jf_emit(const GbLayout &L, const JoinFeedArgs &A
                 gb_note_zero_key(A.ctl, vg_load_bits(kc, kr), pos);
         }
     }
-    int64_t ls = -1;
-    if (S && cls == VG_KEY_REGULAR) ls = gb_slot(lhot, S, L.RH, kn, empty, GB_LDS_PROBES, nullptr);
+    int64_t ls = S ? gb_lds_slot(lhot, S, L.RH, cls, kn, empty) : -1;
     if (ls >= 0) {
-        gb_update(L, lhot + (size_t)ls * L.RH, lcold + (size_t)ls * L.RC, pos, iv);
+        jf_apply(L, A, cp, pr, br, pos, lhot + (size_t)ls * L.RH, lcold + (size_t)ls * L.RC);
     } else {
         int64_t gs = gb_global_slot(L, A.hot, A.cap, cls, kn, pos, A.ctl);
-        if (gs >= 0) gb_update(L, A.hot + (size_t)gs * L.RH, A.cold + (size_t)gs * L.RC, pos, iv);
+        if (gs >= 0) jf_apply(L, A, cp, pr, br, pos, A.hot + (size_t)gs * L.RH, A.cold + (size_t)gs * L.RC);
     }
 }
 
@@ -412,31 +442,20 @@ k_gb_feed_join(GbLayout L, JoinFeedArgs A, ColPair cp, JView jv, PredInfo pi, vg
     uint64_t *regs = reinterpret_cast<uint64_t *>(dyn);
     uint8_t *regv = dyn + (size_t)prog.n_insn * GB_THREADS * 8;
     uint64_t *lhot = reinterpret_cast<uint64_t *>(dyn + A.vm_lds);
-    uint64_t *lcold = lhot + (size_t)A.lds_slots * L.RH;
     const int S = A.lds_slots;
+    uint64_t *lcold = lhot + (size_t)(S + 2) * L.RH;
     const uint64_t empty = L.key_is_double ? VG_EMPTY_F64 : VG_EMPTY_I64;
-    ColSet pcs;
-    for (int i = 0; i < VGPU_MAX_COLS; i++) pcs.c[i] = cp.probe[i];
-    if (S) {
-        for (int s = tid; s < S; s += GB_THREADS) {
-            uint64_t *h = lhot + (size_t)s * L.RH;
-            h[0] = empty;
-            for (int w = 1; w < L.RH; w++) h[w] = ((L.hot_init_ones >> w) & 1) ? ~0ull : 0ull;
-            uint64_t *c = lcold + (size_t)s * L.RC;
-            for (int w = 0; w < L.RC; w++) c[w] = ((L.cold_init_ones >> w) & 1) ? ~0ull : 0ull;
-        }
-        __syncthreads();
-    }
+    if (S) gb_lds_init(L, lhot, lcold, S, empty);
     const int64_t stride = (int64_t)gridDim.x * GB_THREADS;
     for (int64_t li = (int64_t)blockIdx.x * GB_THREADS + tid; li < A.n_logical; li += stride) {
         int64_t pr = A.sel ? (int64_t)A.sel[li] : li;
-        if (!vg_pred_row(pi, prog, pcs, pr, regs, regv, GB_THREADS, tid)) continue;
+        if (!vg_pred_row(pi, prog, cp.probe, pr, regs, regv, GB_THREADS, tid)) continue;
         if (!A.has_join) {
             jf_emit(L, A, cp, pr, -1, (uint64_t)(A.pos_base + li), lhot, lcold, S, empty);
             continue;
         }
         uint32_t off;
-        uint32_t cnt = jt_lookup(jv, cp.probe[A.probe_key_col], pr, off);
+        uint32_t cnt = jt_lookup(jv, cp.probe.c[A.probe_key_col], pr, off);
         for (uint32_t j = 0; j < cnt; j++) {
             int64_t br = (int64_t)jv.rows[off + j];
             jf_emit(L, A, cp, pr, br, jf_pos((uint64_t)A.pos_base, li, j, false), lhot, lcold, S, empty);
@@ -444,15 +463,7 @@ k_gb_feed_join(GbLayout L, JoinFeedArgs A, ColPair cp, JView jv, PredInfo pi, vg
         if (cnt == 0 && A.join_kind == VGPU_JOIN_LEFT)
             jf_emit(L, A, cp, pr, -1, jf_pos((uint64_t)A.pos_base, li, 0, true), lhot, lcold, S, empty);
     }
-    if (S) {
-        __syncthreads();
-        for (int s = tid; s < S; s += GB_THREADS) {
-            const uint64_t *h = lhot + (size_t)s * L.RH;
-            if (h[0] == empty) continue;
-            int64_t gs = gb_global_slot(L, A.hot, A.cap, VG_KEY_REGULAR, h[0], 0, A.ctl);
-            if (gs >= 0) gb_merge(L, A.hot + (size_t)gs * L.RH, A.cold + (size_t)gs * L.RC, h, lcold + (size_t)s * L.RC);
-        }
-    }
+    if (S) gb_lds_merge(L, lhot, lcold, S, empty, A.hot, A.cold, A.cap, A.ctl);
 }
 
 // ---------------------------------------------------------------------------
@@ -780,7 +791,7 @@ static int gb_lds_slots(const vgpu_groupby *g, size_t vm_bytes) {
     size_t rec = (size_t)(g->L.RH + g->L.RC) * 8;
     size_t budget = GB_LDS_BUDGET > vm_bytes ? GB_LDS_BUDGET - vm_bytes : 0;
     int64_t want = pow2_at_least(2 * g->hint);
-    if (want > 4096 || (size_t)want * rec > budget) return 0;
+    if (want > 4096 || (size_t)(want + 2) * rec > budget) return 0;
     return (int)want;
 }
 
@@ -848,7 +859,7 @@ extern "C" int vgpu_groupby_feed(vgpu_groupby *g, const vgpu_col *key, const vgp
     vm = (vm + 15) & ~(size_t)15;
     A.vm_lds = (int32_t)vm;
     A.lds_slots = gb_lds_slots(g, vm);
-    size_t lds = vm + (size_t)A.lds_slots * (g->L.RH + g->L.RC) * 8;
+    size_t lds = vm + (A.lds_slots ? (size_t)(A.lds_slots + 2) * (g->L.RH + g->L.RC) * 8 : 0);
     vg_kt_begin();
     k_gb_feed<<<gb_grid(n_logical, A.lds_slots), GB_THREADS, lds, vg_stream()>>>(g->L, A, ins, pi, prog, pcs);
     VG_LAUNCH_CHECK();
@@ -930,14 +941,14 @@ extern "C" int vgpu_groupby_feed_join(vgpu_groupby *g, const vgpu_feed_spec *sp)
     A.cap = g->cap;
     ColPair cp;
     memset(&cp, 0, sizeof cp);
-    for (int i = 0; i < sp->n_probe_cols; i++) cp.probe[i] = sp->probe_cols[i];
-    for (int i = 0; i < sp->n_build_cols; i++) cp.build[i] = sp->build_cols[i];
+    for (int i = 0; i < sp->n_probe_cols; i++) cp.probe.c[i] = sp->probe_cols[i];
+    for (int i = 0; i < sp->n_build_cols; i++) cp.build.c[i] = sp->build_cols[i];
     for (int i = 0; i < sp->n_inputs; i++) {
         const vgpu_input &in = sp->inputs[i];
-        int ta = in.side_a == 0 ? cp.probe[in.col_a].type : cp.build[in.col_a].type;
+        int ta = in.side_a == 0 ? cp.probe.c[in.col_a].type : cp.build.c[in.col_a].type;
         int t = ta;
         if (in.op == 1) {
-            int tb = in.side_b == 0 ? cp.probe[in.col_b].type : cp.build[in.col_b].type;
+            int tb = in.side_b == 0 ? cp.probe.c[in.col_b].type : cp.build.c[in.col_b].type;
             t = (ta == VGPU_DOUBLE || tb == VGPU_DOUBLE) ? VGPU_DOUBLE : VGPU_INT64;
         }
         if (t != g->L.in_type[i]) {
@@ -956,7 +967,7 @@ extern "C" int vgpu_groupby_feed_join(vgpu_groupby *g, const vgpu_feed_spec *sp)
     vm = (vm + 15) & ~(size_t)15;
     A.vm_lds = (int32_t)vm;
     A.lds_slots = gb_lds_slots(g, vm);
-    size_t lds = vm + (size_t)A.lds_slots * (g->L.RH + g->L.RC) * 8;
+    size_t lds = vm + (A.lds_slots ? (size_t)(A.lds_slots + 2) * (g->L.RH + g->L.RC) * 8 : 0);
     vg_kt_begin();
     k_gb_feed_join<<<gb_grid(n_logical, A.lds_slots), GB_THREADS, lds, vg_stream()>>>(g->L, A, cp, jv, pi, prog);
     VG_LAUNCH_CHECK();
