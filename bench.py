@@ -92,7 +92,8 @@ class C2:
         return out
 
     def step(self):
-        jt = self.nv.JoinTable(self.dk)
+        # the dimension's attr and weight ride in the table entries
+        jt = self.nv.JoinTable(self.dk, [self.at, self.wt])
         res = {k: self.query(jt, k) for k in ("inner", "left")}
         jt.free()
         return res

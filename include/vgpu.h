@@ -186,6 +186,14 @@ typedef struct vgpu_jtable vgpu_jtable;
  * of the table: they never match (src/join.c:85). */
 vgpu_jtable *vgpu_join_build(const vgpu_col *key, int64_t n_build);
 
+/* The same, embedding up to 4 build-side payload columns in the table: each
+ * key's entry carries the payload of its first (highest) build row, so a
+ * fused probe -> aggregate over a unique-key table reads key, match count
+ * and payload in one load.  The key and payload columns must outlive the
+ * table (the fused feed matches build columns to payload by data pointer). */
+vgpu_jtable *vgpu_join_build_payload(const vgpu_col *key, int64_t n_build,
+                                     const vgpu_col *payload, int n_payload);
+
 /* Probes the logical rows of a batch.  For each probe row in logical order
  * the output holds its matches in descending build row order; LEFT/FULL
  * append the batch's unmatched probe rows after all matches; SEMI/ANTI

@@ -43,14 +43,17 @@ def oracle_join_agg(nv, fkey, value, dim_key, attr, weight, kind, rg=1 << 20):
 
 @pytest.mark.parametrize("kind", ["inner", "left"])
 @pytest.mark.parametrize("dup", [False, True])
-def test_fused_join_aggregate(gpu, oracle, kind, dup):
+@pytest.mark.parametrize("payload", [0, 1, 2])
+def test_fused_join_aggregate(gpu, oracle, kind, dup, payload):
+    """payload = build columns embedded in the join table: none, weight only
+    (attr read through rows[]), or attr and weight"""
     nv = gpu
     fkey, value, dim_key, attr, weight = star_schema(5, 200_000, 20_000, dup=dup)
     exp = oracle_join_agg(nv, fkey, value, dim_key, attr, weight, kind)
     sa = exp.pop("sa")[0]
     cols = [nv.DeviceColumn(a) for a in (fkey, value, dim_key, attr, weight)]
     fk, val, dk, at, wt = cols
-    jt = nv.JoinTable(dk)
+    jt = nv.JoinTable(dk, [[], [wt], [at, wt]][payload])
     gb = nv.GroupBy(nv.VEC_INT64, [nv.VEC_DOUBLE], [("sum", 0, 0), ("n", -1, 0)], nv.VGPU_ORDER_KEY, 101)
     gb.feed_join(len(fkey), [fk, val], [dk, at, wt], [(0, 1, 1, 2)], key=(1, 1), join=jt,
                  join_kind=nv.JOIN_NAMES[kind], probe_key_col=0)

@@ -335,6 +335,8 @@ struct JoinFeedArgs {
     int32_t key_col;
     int32_t probe_key_col;
     int32_t n_inputs;
+    int32_t need_rows;                 // some build column is read through rows[]
+    int8_t bpay[VGPU_MAX_COLS];        // payload slot of a build column, -1 = not embedded
     vgpu_input in[VGPU_MAX_INPUTS];
     uint64_t *hot;
     uint64_t *cold;
@@ -349,21 +351,45 @@ struct ColPair {
     ColSet build;
 };
 
-// value of (side, col) at the current pair; build side is NA when br < 0
-__device__ __forceinline__ InVal jf_col(const ColPair &cp, int side, int col, int64_t pr, int64_t br) {
+// the build side of one output pair: table entry (nullptr = unmatched),
+// segment offset, match index and build row (-1 when not needed)
+struct BRef {
+    const uint64_t *e;
+    uint64_t segw;
+    uint32_t off;
+    uint32_t j;
+    int64_t br;
+};
+
+// value of (side, col) at the current pair; build side is NA when unmatched
+__device__ __forceinline__ InVal jf_col(const JoinFeedArgs &A, const ColPair &cp, const JView &jv, int side,
+                                        int col, int64_t pr, const BRef &b) {
     InVal v;
     if (side == 0) {
         const vgpu_col &c = cp.probe.c[col];
         v.valid = vg_valid(c.validity, pr);
         v.bits = v.valid ? vg_load_bits(c, pr) : 0;
-    } else if (br < 0) {
+        return v;
+    }
+    if (!b.e) {
         v.valid = false;
         v.bits = 0;
-    } else {
-        const vgpu_col &c = cp.build.c[col];
-        v.valid = vg_valid(c.validity, br);
-        v.bits = v.valid ? vg_load_bits(c, br) : 0;
+        return v;
     }
+    int p = A.bpay[col];
+    if (p >= 0) {
+        if (b.j == 0) {
+            v.valid = (b.segw >> (24 + p)) & 1;
+            v.bits = b.e[2 + p];
+        } else {
+            v.valid = jv.cval[p][b.off + b.j];
+            v.bits = jv.cpay[p][b.off + b.j];
+        }
+        return v;
+    }
+    const vgpu_col &c = cp.build.c[col];
+    v.valid = vg_valid(c.validity, b.br);
+    v.bits = v.valid ? vg_load_bits(c, b.br) : 0;
     return v;
 }
 
@@ -388,42 +414,37 @@ __device__ __forceinline__ InVal jf_mul(InVal a, int ta, InVal b, int tb) {
     return r;
 }
 
-__device__ __forceinline__ void jf_apply(const GbLayout &L, const JoinFeedArgs &A, const ColPair &cp, int64_t pr,
-                                         int64_t br, uint64_t pos, uint64_t *h, uint64_t *c) {
+__device__ __forceinline__ void jf_apply(const GbLayout &L, const JoinFeedArgs &A, const ColPair &cp,
+                                         const JView &jv, int64_t pr, const BRef &b, uint64_t pos, uint64_t *h,
+                                         uint64_t *c) {
     gb_update_row(L, h, pos);
     for (int i = 0; i < A.n_inputs; i++) {
         const vgpu_input &in = A.in[i];
-        InVal v = jf_col(cp, in.side_a, in.col_a, pr, br);
+        InVal v = jf_col(A, cp, jv, in.side_a, in.col_a, pr, b);
         if (in.op == 1) {
-            InVal b = jf_col(cp, in.side_b, in.col_b, pr, br);
-            v = jf_mul(v, jf_type(cp, in.side_a, in.col_a), b, jf_type(cp, in.side_b, in.col_b));
+            InVal w = jf_col(A, cp, jv, in.side_b, in.col_b, pr, b);
+            v = jf_mul(v, jf_type(cp, in.side_a, in.col_a), w, jf_type(cp, in.side_b, in.col_b));
         }
         gb_update_input(L, h, c, i, pos, v);
     }
 }
 
 __device__ __forceinline__ void jf_emit(const GbLayout &L, const JoinFeedArgs &A, const ColPair &cp,
-                                        int64_t pr, int64_t br, uint64_t pos,
+                                        const JView &jv, int64_t pr, const BRef &b, uint64_t pos,
                                         uint64_t *lhot, uint64_t *lcold, int S, uint64_t empty) {
     uint64_t kn = 0;
     int cls = VG_KEY_REGULAR;
     if (!L.key_none) {
-        const vgpu_col &kc = A.key_side == 0 ? cp.probe.c[A.key_col] : cp.build.c[A.key_col];
-        int64_t kr = A.key_side == 0 ? pr : br;
-        if (kr < 0) {
-            cls = VG_KEY_NA;
-        } else {
-            cls = vg_key_class(kc, kr, kn);
-            if (L.key_is_double && cls == VG_KEY_REGULAR && kn == 0)
-                gb_note_zero_key(A.ctl, vg_load_bits(kc, kr), pos);
-        }
+        InVal kv = jf_col(A, cp, jv, A.key_side, A.key_col, pr, b);
+        cls = vg_key_class_bits(jf_type(cp, A.key_side, A.key_col), kv.valid, kv.bits, kn);
+        if (L.key_is_double && cls == VG_KEY_REGULAR && kn == 0) gb_note_zero_key(A.ctl, kv.bits, pos);
     }
     int64_t ls = S ? gb_lds_slot(lhot, S, L.RH, cls, kn, empty) : -1;
     if (ls >= 0) {
-        jf_apply(L, A, cp, pr, br, pos, lhot + (size_t)ls * L.RH, lcold + (size_t)ls * L.RC);
+        jf_apply(L, A, cp, jv, pr, b, pos, lhot + (size_t)ls * L.RH, lcold + (size_t)ls * L.RC);
     } else {
         int64_t gs = gb_global_slot(L, A.hot, A.cap, cls, kn, pos, A.ctl);
-        if (gs >= 0) jf_apply(L, A, cp, pr, br, pos, A.hot + (size_t)gs * L.RH, A.cold + (size_t)gs * L.RC);
+        if (gs >= 0) jf_apply(L, A, cp, jv, pr, b, pos, A.hot + (size_t)gs * L.RH, A.cold + (size_t)gs * L.RC);
     }
 }
 
@@ -450,18 +471,33 @@ k_gb_feed_join(GbLayout L, JoinFeedArgs A, ColPair cp, JView jv, PredInfo pi, vg
     for (int64_t li = (int64_t)blockIdx.x * GB_THREADS + tid; li < A.n_logical; li += stride) {
         int64_t pr = A.sel ? (int64_t)A.sel[li] : li;
         if (!vg_pred_row(pi, prog, cp.probe, pr, regs, regv, GB_THREADS, tid)) continue;
+        BRef b;
+        b.e = nullptr;
+        b.segw = 0;
+        b.off = 0;
+        b.j = 0;
+        b.br = -1;
         if (!A.has_join) {
-            jf_emit(L, A, cp, pr, -1, (uint64_t)(A.pos_base + li), lhot, lcold, S, empty);
+            jf_emit(L, A, cp, jv, pr, b, (uint64_t)(A.pos_base + li), lhot, lcold, S, empty);
             continue;
         }
-        uint32_t off;
-        uint32_t cnt = jt_lookup(jv, cp.probe.c[A.probe_key_col], pr, off);
-        for (uint32_t j = 0; j < cnt; j++) {
-            int64_t br = (int64_t)jv.rows[off + j];
-            jf_emit(L, A, cp, pr, br, jf_pos((uint64_t)A.pos_base, li, j, false), lhot, lcold, S, empty);
+        const uint64_t *e = jt_find(jv, cp.probe.c[A.probe_key_col], pr);
+        uint32_t cnt = 0;
+        if (e) {
+            b.segw = e[1];
+            cnt = jt_count(jv, b.segw);
+            b.off = (uint32_t)(b.segw >> 32);
         }
-        if (cnt == 0 && A.join_kind == VGPU_JOIN_LEFT)
-            jf_emit(L, A, cp, pr, -1, jf_pos((uint64_t)A.pos_base, li, 0, true), lhot, lcold, S, empty);
+        for (uint32_t j = 0; j < cnt; j++) {
+            b.e = e;
+            b.j = j;
+            b.br = A.need_rows ? (int64_t)jv.rows[b.off + j] : -1;
+            jf_emit(L, A, cp, jv, pr, b, jf_pos((uint64_t)A.pos_base, li, j, false), lhot, lcold, S, empty);
+        }
+        if (cnt == 0 && A.join_kind == VGPU_JOIN_LEFT) {
+            b.e = nullptr;
+            jf_emit(L, A, cp, jv, pr, b, jf_pos((uint64_t)A.pos_base, li, 0, true), lhot, lcold, S, empty);
+        }
     }
     if (S) gb_lds_merge(L, lhot, lcold, S, empty, A.hot, A.cold, A.cap, A.ctl);
 }
@@ -935,6 +971,26 @@ extern "C" int vgpu_groupby_feed_join(vgpu_groupby *g, const vgpu_feed_spec *sp)
     A.probe_key_col = sp->probe_key_col;
     A.n_inputs = sp->n_inputs;
     for (int i = 0; i < sp->n_inputs; i++) A.in[i] = sp->inputs[i];
+    // build columns embedded in the table are read from the probed entry
+    for (int c = 0; c < VGPU_MAX_COLS; c++) A.bpay[c] = -1;
+    if (sp->join) {
+        for (int c = 0; c < sp->n_build_cols; c++)
+            for (int p = 0; p < sp->join->n_payload; p++) {
+                const vgpu_col &q = sp->join->payload[p];
+                if (q.data == sp->build_cols[c].data && q.validity == sp->build_cols[c].validity &&
+                    q.type == sp->build_cols[c].type) {
+                    A.bpay[c] = (int8_t)p;
+                    break;
+                }
+            }
+        bool need = !g->L.key_none && sp->key_side == 1 && A.bpay[sp->key_col] < 0;
+        for (int i = 0; i < sp->n_inputs; i++) {
+            const vgpu_input &in = sp->inputs[i];
+            need |= in.side_a == 1 && A.bpay[in.col_a] < 0;
+            need |= in.op == 1 && in.side_b == 1 && A.bpay[in.col_b] < 0;
+        }
+        A.need_rows = need;
+    }
     A.hot = g->hot.as<uint64_t>();
     A.cold = g->cold.as<uint64_t>();
     A.ctl = g->ctl.as<uint64_t>();

@@ -25,16 +25,18 @@
 #define JT_THREADS 256
 #define JT_NONE 0xFFFFFFFFu
 
-__global__ void k_jt_init(uint64_t *keys, int64_t n, uint64_t empty, uint32_t *cnt) {
+__global__ void k_jt_init(uint64_t *ent, int64_t n, int E, uint64_t empty, uint32_t *cnt) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) {
-        keys[i] = empty;
+        uint64_t *e = ent + i * E;
+        e[0] = empty;
+        for (int w = 1; w < E; w++) e[w] = 0;
         cnt[i] = 0;
     }
 }
 
-__global__ void k_jt_insert(vgpu_col key, int64_t n, uint64_t *keys, int64_t cap, uint32_t *cnt,
+__global__ void k_jt_insert(vgpu_col key, int64_t n, uint64_t *ent, int E, int64_t cap, uint32_t *cnt,
                             uint32_t *slot_of, uint32_t *err) {
     int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
@@ -51,10 +53,11 @@ __global__ void k_jt_insert(vgpu_col key, int64_t n, uint64_t *keys, int64_t cap
         uint64_t h = vg_key_hash(kn) & mask;
         s = -1;
         for (int64_t probes = 0; probes < cap; probes++) {
-            uint64_t k = vg_atomic_load(&keys[h]);
+            uint64_t *kp = &ent[h * E];
+            uint64_t k = vg_atomic_load(kp);
             if (k == kn) { s = (int64_t)h; break; }
             if (k == empty) {
-                uint64_t old = atomicCAS((unsigned long long *)&keys[h], (unsigned long long)empty,
+                uint64_t old = atomicCAS((unsigned long long *)kp, (unsigned long long)empty,
                                          (unsigned long long)kn);
                 if (old == empty || old == kn) { s = (int64_t)h; break; }
             }
@@ -70,20 +73,21 @@ __global__ void k_jt_insert(vgpu_col key, int64_t n, uint64_t *keys, int64_t cap
     slot_of[r] = (uint32_t)s;
 }
 
-__global__ void k_jt_seg(const uint32_t *cnt, const uint64_t *off, int64_t n, uint64_t *seg, uint32_t *max_cnt) {
+__global__ void k_jt_seg(const uint32_t *cnt, const uint64_t *off, int64_t n, uint64_t *ent, int E,
+                         uint32_t *max_cnt) {
     int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     uint32_t c = cnt[s];
-    seg[s] = (off[s] << 32) | c;
+    ent[s * E + 1] = (off[s] << 32) | c;
     if (c > 1) atomicMax(max_cnt, c);
 }
 
-__global__ void k_jt_fill_unique(const uint32_t *slot_of, int64_t n, const uint64_t *seg, uint32_t *rows) {
+__global__ void k_jt_fill_unique(const uint32_t *slot_of, int64_t n, const uint64_t *ent, int E, uint32_t *rows) {
     int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     uint32_t s = slot_of[r];
     if (s == JT_NONE) return;
-    rows[seg[s] >> 32] = (uint32_t)r;
+    rows[ent[(int64_t)s * E + 1] >> 32] = (uint32_t)r;
 }
 
 __global__ void k_jt_sortkeys(const uint32_t *slot_of, int64_t n, uint64_t *skeys, uint32_t *svals) {
@@ -93,7 +97,38 @@ __global__ void k_jt_sortkeys(const uint32_t *slot_of, int64_t n, uint64_t *skey
     svals[r] = (uint32_t)r;
 }
 
-extern "C" vgpu_jtable *vgpu_join_build(const vgpu_col *key, int64_t n_build) {
+struct PaySet {
+    vgpu_col c[JT_MAX_PAYLOAD];
+};
+
+// embeds the payload of each key's first (highest) build row in its entry
+__global__ void k_jt_embed(uint64_t *ent, int E, int64_t n_slots, const uint32_t *rows, PaySet ps, int n_payload) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_slots) return;
+    uint64_t *e = ent + s * E;
+    uint64_t sg = e[1];
+    if ((uint32_t)sg == 0) return;
+    uint32_t r0 = rows[sg >> 32];
+    uint64_t vbits = 0;
+    for (int p = 0; p < n_payload; p++) {
+        bool ok = vg_valid(ps.c[p].validity, r0);
+        e[2 + p] = ok ? vg_load_bits(ps.c[p], r0) : 0;
+        if (ok) vbits |= 1ull << (24 + p);
+    }
+    e[1] = sg | vbits;
+}
+
+// payload in rows[] order, for keys with several build rows
+__global__ void k_jt_cluster(const uint32_t *rows, int64_t n_valid, vgpu_col c, uint64_t *pay, uint8_t *val) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_valid) return;
+    uint32_t r = rows[i];
+    bool ok = vg_valid(c.validity, r);
+    pay[i] = ok ? vg_load_bits(c, r) : 0;
+    val[i] = ok;
+}
+
+static vgpu_jtable *join_build_impl(const vgpu_col *key, int64_t n_build, const vgpu_col *payload, int n_payload) {
     if (vg_ensure_init()) return nullptr;
     if (key->type != VGPU_INT64 && key->type != VGPU_DOUBLE && key->type != VGPU_BOOL) {
         vg_set_error("join_build: unsupported key type %d", key->type);
@@ -103,38 +138,48 @@ extern "C" vgpu_jtable *vgpu_join_build(const vgpu_col *key, int64_t n_build) {
         vg_set_error("join_build: %lld build rows exceed the 32-bit row index", (long long)n_build);
         return nullptr;
     }
+    if (n_payload < 0 || n_payload > JT_MAX_PAYLOAD) {
+        vg_set_error("join_build: %d payload columns exceed %d", n_payload, JT_MAX_PAYLOAD);
+        return nullptr;
+    }
     vgpu_jtable *t = new vgpu_jtable();
     t->key_type = key->type;
     t->n_build = n_build;
+    t->n_payload = n_payload;
+    for (int p = 0; p < n_payload; p++) t->payload[p] = payload[p];
+    t->E = n_payload == 0 ? 2 : (n_payload <= 2 ? 4 : 6);
+    if (key->type != VGPU_BOOL) t->bkey = static_cast<const uint64_t *>(key->data);
     int64_t cap = 64;
     while (cap < 2 * n_build) cap <<= 1;
     t->cap = cap;
+    const int E = t->E;
     hipStream_t s = vg_stream();
     int64_t slots = cap + 1;
     DevBuf cnt, slot_of, off, misc;
     auto fail = [&]() -> vgpu_jtable * {
         cnt.release(); slot_of.release(); off.release(); misc.release();
-        t->keys.release(); t->seg.release(); t->rows.release(); t->sort_tmp.release(); t->scan_tmp.release();
+        t->ent.release(); t->rows.release(); t->sort_tmp.release(); t->scan_tmp.release();
+        for (int p = 0; p < JT_MAX_PAYLOAD; p++) { t->cpay[p].release(); t->cval[p].release(); }
         delete t;
         return nullptr;
     };
-    if (t->keys.reserve((size_t)slots * 8) || t->seg.reserve((size_t)slots * 8) || cnt.reserve((size_t)slots * 4) ||
+    if (t->ent.reserve((size_t)slots * E * 8) || cnt.reserve((size_t)slots * 4) ||
         slot_of.reserve((size_t)std::max<int64_t>(n_build, 1) * 4) || off.reserve((size_t)slots * 8) ||
         misc.reserve(64))
         return fail();
     uint64_t empty = t->key_type == VGPU_DOUBLE ? VG_EMPTY_F64 : VG_EMPTY_I64;
     if (hipMemsetAsync(misc.ptr, 0, 64, s) != hipSuccess) { vg_set_error("join_build: memset failed"); return fail(); }
-    k_jt_init<<<vg_blocks(slots, 256, 8192), 256, 0, s>>>(t->keys.as<uint64_t>(), slots, empty, cnt.as<uint32_t>());
+    k_jt_init<<<vg_blocks(slots, 256, 8192), 256, 0, s>>>(t->ent.as<uint64_t>(), slots, E, empty, cnt.as<uint32_t>());
     if (n_build > 0) {
         vgpu_col k = *key;
-        k_jt_insert<<<vg_blocks(n_build, JT_THREADS), JT_THREADS, 0, s>>>(k, n_build, t->keys.as<uint64_t>(), cap,
+        k_jt_insert<<<vg_blocks(n_build, JT_THREADS), JT_THREADS, 0, s>>>(k, n_build, t->ent.as<uint64_t>(), E, cap,
                                                                           cnt.as<uint32_t>(), slot_of.as<uint32_t>(),
                                                                           misc.as<uint32_t>());
     }
     uint64_t *total = misc.as<uint64_t>() + 2;
     if (vg_scan_u32_to_u64(cnt.as<uint32_t>(), off.as<uint64_t>(), slots, total, t->scan_tmp)) return fail();
     k_jt_seg<<<vg_blocks(slots, 256), 256, 0, s>>>(cnt.as<uint32_t>(), off.as<uint64_t>(), slots,
-                                                   t->seg.as<uint64_t>(), misc.as<uint32_t>() + 1);
+                                                   t->ent.as<uint64_t>(), E, misc.as<uint32_t>() + 1);
     uint64_t hm[8];
     if (hipMemcpyAsync(hm, misc.ptr, 64, hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
@@ -146,13 +191,17 @@ extern "C" vgpu_jtable *vgpu_join_build(const vgpu_col *key, int64_t n_build) {
         vg_set_error("join_build: hash table overflow");
         return fail();
     }
+    if (n_payload > 0 && maxc >= (1u << 24)) {
+        vg_set_error("join_build: a key repeats %u times; payload embedding needs fewer than 2^24", maxc);
+        return fail();
+    }
     t->n_valid = (int64_t)hm[2];
     t->max_dup = (int32_t)std::max<uint32_t>(maxc, t->n_valid > 0 ? 1u : 0u);
     if (t->rows.reserve((size_t)std::max<int64_t>(n_build, 1) * 4)) return fail();
     if (n_build > 0) {
         if (maxc <= 1) {
             k_jt_fill_unique<<<vg_blocks(n_build, 256), 256, 0, s>>>(slot_of.as<uint32_t>(), n_build,
-                                                                      t->seg.as<uint64_t>(), t->rows.as<uint32_t>());
+                                                                      t->ent.as<uint64_t>(), E, t->rows.as<uint32_t>());
         } else {
             DevBuf skeys;
             if (skeys.reserve((size_t)n_build * 8)) return fail();
@@ -164,6 +213,24 @@ extern "C" vgpu_jtable *vgpu_join_build(const vgpu_col *key, int64_t n_build) {
             if (rc || e != hipSuccess) {
                 if (!rc) vg_set_error("join_build: sort failed: %s", hipGetErrorString(e));
                 return fail();
+            }
+        }
+        if (n_payload > 0) {
+            PaySet ps;
+            memset(&ps, 0, sizeof ps);
+            for (int p = 0; p < n_payload; p++) ps.c[p] = payload[p];
+            k_jt_embed<<<vg_blocks(slots, 256), 256, 0, s>>>(t->ent.as<uint64_t>(), E, slots, t->rows.as<uint32_t>(),
+                                                             ps, n_payload);
+            if (maxc > 1) {
+                for (int p = 0; p < n_payload; p++) {
+                    if (t->cpay[p].reserve((size_t)std::max<int64_t>(t->n_valid, 1) * 8) ||
+                        t->cval[p].reserve((size_t)std::max<int64_t>(t->n_valid, 1)))
+                        return fail();
+                    if (t->n_valid > 0)
+                        k_jt_cluster<<<vg_blocks(t->n_valid, 256), 256, 0, s>>>(
+                            t->rows.as<uint32_t>(), t->n_valid, payload[p], t->cpay[p].as<uint64_t>(),
+                            t->cval[p].as<uint8_t>());
+                }
             }
         }
     }
@@ -179,6 +246,15 @@ extern "C" vgpu_jtable *vgpu_join_build(const vgpu_col *key, int64_t n_build) {
     return t;
 }
 
+extern "C" vgpu_jtable *vgpu_join_build(const vgpu_col *key, int64_t n_build) {
+    return join_build_impl(key, n_build, nullptr, 0);
+}
+
+extern "C" vgpu_jtable *vgpu_join_build_payload(const vgpu_col *key, int64_t n_build, const vgpu_col *payload,
+                                                int n_payload) {
+    return join_build_impl(key, n_build, payload, n_payload);
+}
+
 extern "C" int64_t vgpu_join_build_rows(const vgpu_jtable *t) { return t ? t->n_build : -1; }
 
 // ---------------------------------------------------------------------------
@@ -187,7 +263,7 @@ extern "C" int64_t vgpu_join_build_rows(const vgpu_jtable *t) { return t ? t->n_
 
 __global__ void __launch_bounds__(JT_THREADS)
 k_jp_count(JView v, vgpu_col key, int64_t n_logical, const int32_t *sel, int kind, int first_only,
-           uint32_t *cnt_main, uint32_t *un, uint32_t *segoff) {
+           uint32_t *cnt_main, uint32_t *un, uint32_t *segoff, uint32_t *segcnt) {
     int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (li >= n_logical) return;
     int64_t pr = sel ? (int64_t)sel[li] : li;
@@ -197,17 +273,28 @@ k_jp_count(JView v, vgpu_col key, int64_t n_logical, const int32_t *sel, int kin
     switch (kind) {
     case VGPU_JOIN_SEMI: m = c > 0; break;
     case VGPU_JOIN_ANTI: m = c == 0; break;
-    default: m = (first_only && c > 1) ? 1u : c; break;
+    default:
+        m = c;
+        if (first_only && c > 1) {
+            // the continued chain walk compares the int64 probe value with
+            // the build double's raw bits: only 0 against +0.0 still matches
+            m = 1;
+            if (vg_load_bits(key, pr) == 0 && v.bkey)
+                for (uint32_t j = 1; j < c; j++) m += v.bkey[v.rows[off + j]] == 0;
+        }
+        break;
     }
     cnt_main[li] = m;
     segoff[li] = off;
+    if (segcnt) segcnt[li] = c;
     if (un) un[li] = c == 0;
 }
 
 __global__ void __launch_bounds__(JT_THREADS)
 k_jp_write(JView v, int64_t n_logical, const int32_t *sel, int kind, const uint32_t *cnt_main,
            const uint64_t *off_main, const uint32_t *un, const uint64_t *off_un, const uint64_t *totals,
-           const uint32_t *segoff, int32_t *out_probe, int64_t *out_build, uint8_t *bmatched) {
+           const uint32_t *segoff, const uint32_t *segcnt, int32_t *out_probe, int64_t *out_build,
+           uint8_t *bmatched) {
     int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (li >= n_logical) return;
     int32_t pr = sel ? sel[li] : (int32_t)li;
@@ -218,10 +305,15 @@ k_jp_write(JView v, int64_t n_logical, const int32_t *sel, int kind, const uint3
         return;
     }
     uint32_t so = segoff[li];
-    for (uint32_t j = 0; j < m; j++) {
+    uint32_t c = segcnt[li];
+    for (uint32_t j = 0, k = 0; j < c && k < m; j++) {
         uint32_t br = v.rows[so + j];
-        out_probe[o + j] = pr;
-        out_build[o + j] = (int64_t)br;
+        // rows past the first are skipped exactly when the count pass
+        // did not count them (first-match mode, see k_jp_count)
+        if (m < c && j > 0 && !(v.bkey && v.bkey[br] == 0)) continue;
+        out_probe[o + k] = pr;
+        out_build[o + k] = (int64_t)br;
+        k++;
         if (bmatched) bmatched[br] = 1;
     }
     if (un && un[li]) {
@@ -251,15 +343,16 @@ extern "C" int vgpu_join_probe(vgpu_jtable *t, int kind, const vgpu_col *probe_k
         VG_TRY(hipMemsetAsync(t->bmatched.ptr, 0, (size_t)std::max<int64_t>(t->n_build, 1), s));
     }
     if (n <= 0) return 0;
-    if (t->cnt.reserve((size_t)n * 4) || t->un.reserve((size_t)n * 4 * 2) || t->off_main.reserve((size_t)n * 8) ||
+    if (t->cnt.reserve((size_t)n * 4) || t->un.reserve((size_t)n * 4 * 3) || t->off_main.reserve((size_t)n * 8) ||
         t->off_un.reserve((size_t)n * 8) || t->totals.reserve(16))
         return -1;
-    uint32_t *segoff = t->un.as<uint32_t>() + n;   // second half of `un`
+    uint32_t *segoff = t->un.as<uint32_t>() + n;       // second third of `un`
+    uint32_t *segcnt = t->un.as<uint32_t>() + 2 * n;   // last third
     JView v = t->view();
     vgpu_col k = *probe_key;
     unsigned grid = vg_blocks(n, JT_THREADS, 1 << 30);
     k_jp_count<<<grid, JT_THREADS, 0, s>>>(v, k, n, sel, kind, first_match_only, t->cnt.as<uint32_t>(),
-                                           tail ? t->un.as<uint32_t>() : nullptr, segoff);
+                                           tail ? t->un.as<uint32_t>() : nullptr, segoff, segcnt);
     VG_LAUNCH_CHECK();
     uint64_t *tot = t->totals.as<uint64_t>();
     if (vg_scan_u32_to_u64(t->cnt.as<uint32_t>(), t->off_main.as<uint64_t>(), n, tot, t->scan_tmp)) return -1;
@@ -277,7 +370,7 @@ extern "C" int vgpu_join_probe(vgpu_jtable *t, int kind, const vgpu_col *probe_k
         return -1;
     k_jp_write<<<grid, JT_THREADS, 0, s>>>(v, n, sel, kind, t->cnt.as<uint32_t>(), t->off_main.as<uint64_t>(),
                                            tail ? t->un.as<uint32_t>() : nullptr, t->off_un.as<uint64_t>(), tot, segoff,
-                                           t->out_probe.as<int32_t>(), t->out_build.as<int64_t>(),
+                                           segcnt, t->out_probe.as<int32_t>(), t->out_build.as<int64_t>(),
                                            kind == VGPU_JOIN_FULL ? t->bmatched.as<uint8_t>() : nullptr);
     VG_LAUNCH_CHECK();
     VG_TRY(hipStreamSynchronize(s));
@@ -340,9 +433,10 @@ extern "C" void vgpu_join_destroy(vgpu_jtable *t) {
     if (!t) return;
     hipError_t e = hipStreamSynchronize(vg_stream());
     (void)e;
-    DevBuf *bufs[] = {&t->keys, &t->seg, &t->rows, &t->cnt, &t->un, &t->off_main, &t->off_un,
+    DevBuf *bufs[] = {&t->ent, &t->rows, &t->cnt, &t->un, &t->off_main, &t->off_un,
                       &t->out_probe, &t->out_build, &t->totals, &t->scan_tmp, &t->sort_tmp, &t->bmatched};
     for (DevBuf *b : bufs) b->release();
+    for (int p = 0; p < JT_MAX_PAYLOAD; p++) { t->cpay[p].release(); t->cval[p].release(); }
     delete t;
 }
 

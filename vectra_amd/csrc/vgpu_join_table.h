@@ -1,21 +1,34 @@
 // vgpu_join_table.h -- layout of the device join table (shared by the join
 // probe kernels and the fused join -> aggregate kernel).
 //
-// keys[s]  : key bits of slot s (EMPTY = free); slot `cap` is reserved for
-//            the int64 key whose bits equal the EMPTY sentinel.
-// seg[s]   : (offset << 32) | count of the slot's build rows in rows[].
-// rows[]   : build row ids grouped by key, DESCENDING within a key -- the
-//            order in which the reference walks a head-inserted chain
-//            (src/join.c:54-57, 124-131).
+// ent[s * E + 0] : key bits of slot s (EMPTY = free); slot `cap` is reserved
+//                  for the int64 key whose bits equal the EMPTY sentinel.
+// ent[s * E + 1] : (offset << 32) | count of the slot's build rows in rows[];
+//                  with embedded payload the count keeps 24 bits and bits
+//                  24..31 hold the validity of the embedded values.
+// ent[s * E + 2 + p] : payload column p of the slot's FIRST build row, so a
+//                  probe of a unique-key table reads key, row count and the
+//                  build-side values it aggregates in one 16/32/48-byte load.
+// rows[]         : build row ids grouped by key, DESCENDING within a key --
+//                  the order in which the reference walks a head-inserted
+//                  chain (src/join.c:54-57, 124-131).
+// cpay[p][i]     : payload p of rows[i] (only built when some key repeats),
+//                  cval[p][i] its validity byte.
 #pragma once
 
 #include "vgpu_keys.h"
 
+#define JT_MAX_PAYLOAD 4
+
 struct JView {
-    const uint64_t *keys;
-    const uint64_t *seg;
+    const uint64_t *ent;
     const uint32_t *rows;
+    const uint64_t *bkey;    // raw build key bits per build row (8-byte keys)
+    const uint64_t *cpay[JT_MAX_PAYLOAD];
+    const uint8_t *cval[JT_MAX_PAYLOAD];
     int64_t cap;
+    int32_t E;               // entry stride in words
+    int32_t n_payload;
     int32_t key_is_double;
     int32_t pad;
 };
@@ -26,45 +39,64 @@ struct vgpu_jtable {
     int64_t n_valid = 0;
     int64_t cap = 0;
     int32_t max_dup = 0;
-    DevBuf keys, seg, rows;
+    int32_t E = 2;
+    int32_t n_payload = 0;
+    vgpu_col payload[JT_MAX_PAYLOAD];
+    const uint64_t *bkey = nullptr;   // caller's build key column (kept alive by the caller)
+    DevBuf ent, rows;
+    DevBuf cpay[JT_MAX_PAYLOAD], cval[JT_MAX_PAYLOAD];
     // probe scratch
     DevBuf cnt, un, off_main, off_un, out_probe, out_build, totals, scan_tmp, sort_tmp;
     DevBuf bmatched;       // FULL join: one byte per build row
     int64_t last_n_out = 0;
     JView view() const {
         JView v;
-        v.keys = keys.as<uint64_t>();
-        v.seg = seg.as<uint64_t>();
+        memset(&v, 0, sizeof v);
+        v.ent = ent.as<uint64_t>();
         v.rows = rows.as<uint32_t>();
+        v.bkey = bkey;
+        for (int p = 0; p < n_payload; p++) {
+            v.cpay[p] = cpay[p].as<uint64_t>();
+            v.cval[p] = cval[p].as<uint8_t>();
+        }
         v.cap = cap;
+        v.E = E;
+        v.n_payload = n_payload;
         v.key_is_double = key_type == VGPU_DOUBLE;
-        v.pad = 0;
         return v;
     }
 };
 
-// Looks up a probe key; returns the match count (0 = none) and the first
-// index into rows[].  NA and NaN keys never match (src/join.c:85, ==).
-__device__ __forceinline__ uint32_t jt_lookup(const JView &v, const vgpu_col &kc, int64_t row, uint32_t &off) {
+// Entry of a probe key, or nullptr: NA and NaN keys never match
+// (src/join.c:85, ==).
+__device__ __forceinline__ const uint64_t *jt_find(const JView &v, const vgpu_col &kc, int64_t row) {
     uint64_t kn;
     int cls = vg_key_class(kc, row, kn);
-    off = 0;
-    if (cls == VG_KEY_NA || cls == VG_KEY_NAN) return 0;
-    uint64_t s;
-    if (cls == VG_KEY_SPECIAL) {
-        s = (uint64_t)v.cap;
-    } else {
-        const uint64_t empty = v.key_is_double ? VG_EMPTY_F64 : VG_EMPTY_I64;
-        uint64_t mask = (uint64_t)v.cap - 1;
-        s = vg_key_hash(kn) & mask;
-        for (int64_t probes = 0;; probes++) {
-            uint64_t k = v.keys[s];
-            if (k == kn) break;
-            if (k == empty || probes >= v.cap) return 0;
-            s = (s + 1) & mask;
-        }
+    if (cls == VG_KEY_NA || cls == VG_KEY_NAN) return nullptr;
+    if (cls == VG_KEY_SPECIAL) return v.ent + (size_t)v.cap * v.E;
+    const uint64_t empty = v.key_is_double ? VG_EMPTY_F64 : VG_EMPTY_I64;
+    uint64_t mask = (uint64_t)v.cap - 1;
+    uint64_t s = vg_key_hash(kn) & mask;
+    for (int64_t probes = 0; probes < v.cap; probes++) {
+        const uint64_t *e = v.ent + s * v.E;
+        uint64_t k = e[0];
+        if (k == kn) return e;
+        if (k == empty) return nullptr;
+        s = (s + 1) & mask;
     }
-    uint64_t sg = v.seg[s];
+    return nullptr;
+}
+
+__device__ __forceinline__ uint32_t jt_count(const JView &v, uint64_t segw) {
+    return v.n_payload ? (uint32_t)(segw & 0xFFFFFFu) : (uint32_t)segw;
+}
+
+// match count and first rows[] index of a probe key (0 = no match)
+__device__ __forceinline__ uint32_t jt_lookup(const JView &v, const vgpu_col &kc, int64_t row, uint32_t &off) {
+    const uint64_t *e = jt_find(v, kc, row);
+    off = 0;
+    if (!e) return 0;
+    uint64_t sg = e[1];
     off = (uint32_t)(sg >> 32);
-    return (uint32_t)sg;
+    return jt_count(v, sg);
 }

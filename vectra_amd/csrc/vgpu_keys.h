@@ -18,10 +18,10 @@
 
 enum { VG_KEY_REGULAR = 0, VG_KEY_NA = 1, VG_KEY_NAN = 2, VG_KEY_SPECIAL = 3 };
 
-__device__ __forceinline__ int vg_key_class(const vgpu_col &c, int64_t row, uint64_t &kn) {
-    if (!vg_valid(c.validity, row)) { kn = 0; return VG_KEY_NA; }
-    uint64_t b = vg_load_bits(c, row);
-    if (c.type == VGPU_DOUBLE) {
+// class and normalized bits of a key value already loaded
+__device__ __forceinline__ int vg_key_class_bits(int type, bool valid, uint64_t b, uint64_t &kn) {
+    if (!valid) { kn = 0; return VG_KEY_NA; }
+    if (type == VGPU_DOUBLE) {
         if ((b & 0x7FFFFFFFFFFFFFFFull) > 0x7FF0000000000000ull) { kn = b; return VG_KEY_NAN; }
         if (b == 0x8000000000000000ull) b = 0;
         kn = b;
@@ -29,6 +29,11 @@ __device__ __forceinline__ int vg_key_class(const vgpu_col &c, int64_t row, uint
     }
     kn = b;
     return b == VG_EMPTY_I64 ? VG_KEY_SPECIAL : VG_KEY_REGULAR;
+}
+
+__device__ __forceinline__ int vg_key_class(const vgpu_col &c, int64_t row, uint64_t &kn) {
+    bool v = vg_valid(c.validity, row);
+    return vg_key_class_bits(c.type, v, v ? vg_load_bits(c, row) : 0, kn);
 }
 
 __device__ __forceinline__ uint64_t vg_key_hash(uint64_t kn) { return vg_mix64(kn ^ 0x9e3779b97f4a7c15ull); }

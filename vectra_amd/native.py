@@ -211,6 +211,8 @@ def vgpu() -> C.CDLL:
                                                       C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)])
         _proto(lib, "vgpu_groupby_destroy", None, [C.c_void_p])
         _proto(lib, "vgpu_join_build", C.c_void_p, [C.POINTER(VgpuCol), C.c_int64])
+        _proto(lib, "vgpu_join_build_payload", C.c_void_p, [C.POINTER(VgpuCol), C.c_int64, C.POINTER(VgpuCol),
+                                                            C.c_int])
         _proto(lib, "vgpu_join_probe", C.c_int, [C.c_void_p, C.c_int, C.POINTER(VgpuCol), C.c_int64, C.c_void_p,
                                                  C.c_int64, C.c_int, C.POINTER(C.c_int64)])
         _proto(lib, "vgpu_join_pairs", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)])
@@ -687,9 +689,13 @@ def cmp_const_prog(col_slot: int, col_type: int, op: str, value: float | int) ->
 
 
 class JoinTable:
-    def __init__(self, key: DeviceColumn):
+    def __init__(self, key: DeviceColumn, payload: Sequence[DeviceColumn] = ()):
         lib = vgpu()
-        self.h = lib.vgpu_join_build(C.byref(key.col), key.n)
+        if payload:
+            self._pay = col_array(payload)
+            self.h = lib.vgpu_join_build_payload(C.byref(key.col), key.n, self._pay, len(payload))
+        else:
+            self.h = lib.vgpu_join_build(C.byref(key.col), key.n)
         if not self.h:
             raise VectraError(lib.vgpu_last_error().decode())
         self.key_type = key.type
