@@ -273,6 +273,15 @@ int vgpu_groupby_feed_join(vgpu_groupby *g, const vgpu_feed_spec *spec);
 int vgpu_sort_pairs_u64(uint64_t *keys, uint32_t *values, int64_t n,
                         int begin_bit, int end_bit);
 
+/* ---- tuning -----------------------------------------------------------
+ * Launch-shape knobs the autotuner sets (defaults in parentheses):
+ *   "gb.lds.max_blocks" (2048)  "gb.lds.rows_per_block" (8192)
+ *   "gb.glb.max_blocks" (16384) "gb.glb.rows_per_block" (1024)
+ *   "gb.lds.max_slots" (4096)   "gb.lds.budget" (65536 bytes)
+ * Returns -1 for an unknown name. */
+int     vgpu_tune_set(const char *name, int64_t value);
+int64_t vgpu_tune_get(const char *name);
+
 /* ---- timing ---------------------------------------------------------
  * Event timing of the library's own stream: elapsed milliseconds of all
  * work issued between vgpu_timer_start() and vgpu_timer_stop(). */

@@ -518,6 +518,7 @@ __global__ void k_gb_compact(GbLayout L, const uint64_t *hot, int64_t cap, int o
         const uint64_t *h = hot + s * L.RH;
         if (s < cap) take = h[0] != empty;
         else take = h[1] != 0;
+        if (s == cap) ctl[8] = h[1];                                      // rows of the NA group
         if (take && order == VGPU_ORDER_KEY && s == cap) take = false;   // NA: appended last
         if (take) {
             if (order == VGPU_ORDER_FIRST_SEEN) sk = h[L.w_fpos];
@@ -738,20 +739,39 @@ __global__ void k_gb_rehash(GbLayout L, const uint64_t *ohot, const uint64_t *oc
     for (int w = 0; w < L.RC; w++) dc[w] = c[w];
 }
 
+#define GB_CTL_WORDS 16
+
 static int gb_read_ctl(vgpu_groupby *g, uint64_t *ctl_host) {
-    VG_TRY(hipMemcpyAsync(ctl_host, g->ctl.ptr, 8 * 8, hipMemcpyDeviceToHost, vg_stream()));
+    VG_TRY(hipMemcpyAsync(ctl_host, g->ctl.ptr, GB_CTL_WORDS * 8, hipMemcpyDeviceToHost, vg_stream()));
     VG_TRY(hipStreamSynchronize(vg_stream()));
     return 0;
 }
 
+// control block: [0] claimed slots, [1] overflow flag, [2]/[3] first
+// position of a -0 / +0 double key, [4] min/max warnings, [5] compaction
+// counter, [6]/[7] OR / AND of the sort keys, [8] rows of the NA group
+__global__ void k_gb_ctl_init(uint64_t *ctl, int finish_only) {
+    int i = threadIdx.x;
+    if (i >= GB_CTL_WORDS) return;
+    if (finish_only) {
+        if (i == 5 || i == 6 || i == 8) ctl[i] = 0;
+        if (i == 7) ctl[i] = ~0ull;
+        return;
+    }
+    ctl[i] = (i == 2 || i == 3 || i == 7) ? ~0ull : 0ull;
+}
+
 // grows the table so that `incoming` more groups keep the load <= 1/2
 static int gb_ensure_capacity(vgpu_groupby *g, int64_t incoming) {
-    uint64_t ctl[8];
+    uint64_t ctl[GB_CTL_WORDS];
     if (g->cap == 0) {
         int64_t want = pow2_at_least(2 * std::max<int64_t>(incoming, 32));
         if (g->hint > 0) want = std::max(want, pow2_at_least(2 * g->hint));
         return gb_alloc_table(g, want);
     }
+    // a table sized for the caller's group count is trusted until finish,
+    // which reports an overflow; no per-feed synchronization
+    if (g->hint > 0 && g->cap >= pow2_at_least(2 * g->hint)) return 0;
     if (gb_read_ctl(g, ctl)) return -1;
     if (ctl[1]) {
         vg_set_error("groupby: hash table overflow");
@@ -803,13 +823,12 @@ extern "C" vgpu_groupby *vgpu_groupby_create(int key_type, int n_inputs, const i
     g->hint = expected_groups;
     g->aggs.n = n_aggs;
     for (int a = 0; a < n_aggs; a++) g->aggs.a[a] = aggs[a];
-    if (gb_layout(g, key_type, n_inputs, input_types, n_aggs, aggs) || g->ctl.reserve(8 * 8)) {
+    if (gb_layout(g, key_type, n_inputs, input_types, n_aggs, aggs) || g->ctl.reserve(GB_CTL_WORDS * 8)) {
         delete g;
         return nullptr;
     }
-    uint64_t init[8] = {0, 0, GB_NOPOS, GB_NOPOS, 0, 0, 0, ~0ull};
-    if (hipMemcpyAsync(g->ctl.ptr, init, sizeof init, hipMemcpyHostToDevice, vg_stream()) != hipSuccess ||
-        hipStreamSynchronize(vg_stream()) != hipSuccess) {
+    k_gb_ctl_init<<<1, 64, 0, vg_stream()>>>(g->ctl.as<uint64_t>(), 0);
+    if (hipGetLastError() != hipSuccess) {
         vg_set_error("groupby: control block init failed");
         g->ctl.release();
         delete g;
@@ -825,16 +844,18 @@ extern "C" vgpu_groupby *vgpu_groupby_create(int key_type, int n_inputs, const i
 static int gb_lds_slots(const vgpu_groupby *g, size_t vm_bytes) {
     if (g->hint <= 0) return 0;
     size_t rec = (size_t)(g->L.RH + g->L.RC) * 8;
-    size_t budget = GB_LDS_BUDGET > vm_bytes ? GB_LDS_BUDGET - vm_bytes : 0;
+    size_t lim = (size_t)vg_tune(VG_TUNE_GB_LDS_BUDGET);
+    size_t budget = lim > vm_bytes ? lim - vm_bytes : 0;
     int64_t want = pow2_at_least(2 * g->hint);
-    if (want > 4096 || (size_t)(want + 2) * rec > budget) return 0;
+    if (want > vg_tune(VG_TUNE_GB_LDS_MAX_SLOTS) || (size_t)(want + 2) * rec > budget) return 0;
     return (int)want;
 }
 
 static unsigned gb_grid(int64_t n_logical, int lds_slots) {
     // privatized blocks amortize their merge over many rows
-    int64_t per_block = lds_slots ? 8192 : 1024;
-    return vg_blocks(n_logical, per_block, lds_slots ? 2048 : 16384);
+    if (lds_slots)
+        return vg_blocks(n_logical, vg_tune(VG_TUNE_GB_LDS_ROWS), vg_tune(VG_TUNE_GB_LDS_MAX_BLOCKS));
+    return vg_blocks(n_logical, vg_tune(VG_TUNE_GB_GLB_ROWS), vg_tune(VG_TUNE_GB_GLB_MAX_BLOCKS));
 }
 
 extern "C" int vgpu_groupby_feed(vgpu_groupby *g, const vgpu_col *key, const vgpu_col *inputs, int64_t n_rows,
@@ -1039,32 +1060,24 @@ extern "C" int64_t vgpu_groupby_finish(vgpu_groupby *g) {
     if (g->finished) return g->n_groups;
     hipStream_t s = vg_stream();
     if (g->cap == 0 && gb_alloc_table(g, 64)) return -1;
-    uint64_t ctl[8];
-    if (gb_read_ctl(g, ctl)) return -1;
-    if (ctl[1]) {
-        vg_set_error("groupby: hash table overflow");
-        return -1;
-    }
     int64_t slots = g->cap + 2;
     if (g->sort_keys.reserve((size_t)slots * 8) || g->sort_vals.reserve((size_t)slots * 4)) return -1;
-    uint64_t zero = 0, ones = ~0ull;
-    VG_TRY(hipMemcpyAsync(g->ctl.as<uint64_t>() + 5, &zero, 8, hipMemcpyHostToDevice, s));
-    VG_TRY(hipMemcpyAsync(g->ctl.as<uint64_t>() + 6, &zero, 8, hipMemcpyHostToDevice, s));
-    VG_TRY(hipMemcpyAsync(g->ctl.as<uint64_t>() + 7, &ones, 8, hipMemcpyHostToDevice, s));
+    k_gb_ctl_init<<<1, 64, 0, s>>>(g->ctl.as<uint64_t>(), 1);
     k_gb_compact<<<vg_blocks(slots, 256), 256, 0, s>>>(g->L, g->hot.as<uint64_t>(), g->cap, g->order,
                                                         g->ctl.as<uint64_t>(), g->sort_keys.as<uint64_t>(),
                                                         g->sort_vals.as<uint32_t>());
     VG_LAUNCH_CHECK();
+    // the one synchronization of finish: group count, sort-key bit range,
+    // overflow flag and the NA group's size
+    uint64_t ctl[GB_CTL_WORDS];
     if (gb_read_ctl(g, ctl)) return -1;
+    if (ctl[1]) {
+        vg_set_error("groupby: hash table overflow (more groups than the table was sized for)");
+        return -1;
+    }
     int64_t n_sorted = (int64_t)ctl[5];
     // NA group of the key-sorted order: appended after the sort
-    int na_last = 0;
-    if (g->order == VGPU_ORDER_KEY) {
-        uint64_t na_n = 0;
-        VG_TRY(hipMemcpyAsync(&na_n, g->hot.as<uint64_t>() + (size_t)g->cap * g->L.RH + 1, 8, hipMemcpyDeviceToHost, s));
-        VG_TRY(hipStreamSynchronize(s));
-        na_last = na_n != 0;
-    }
+    int na_last = g->order == VGPU_ORDER_KEY && ctl[8] != 0;
     uint64_t diff = ctl[6] ^ ctl[7];
     if (n_sorted > 1 && diff) {
         int lo = lowest_bit(diff), hi = highest_bit(diff) + 1;
@@ -1072,7 +1085,6 @@ extern "C" int64_t vgpu_groupby_finish(vgpu_groupby *g) {
             return -1;
     }
     int64_t n_out = n_sorted + na_last;
-    int key_width = g->L.key_none ? 8 : 8;
     size_t vwords = (size_t)((n_out + 63) / 64 + 1);
     if (g->out_keys.reserve((size_t)(n_out + 1) * 8) || g->out_kvalid.reserve(vwords * 8)) return -1;
     EmitOut O;
@@ -1087,11 +1099,9 @@ extern "C" int64_t vgpu_groupby_finish(vgpu_groupby *g) {
     if (n_out > 0) {
         k_gb_emit<<<vg_blocks(n_out, 256), 256, 0, s>>>(g->L, g->aggs, g->hot.as<uint64_t>(), g->cold.as<uint64_t>(),
                                                          g->cap, g->sort_vals.as<uint32_t>(), n_sorted, na_last, n_out,
-                                                         key_width, O, g->ctl.as<uint64_t>());
+                                                         8, O, g->ctl.as<uint64_t>());
         VG_LAUNCH_CHECK();
     }
-    if (gb_read_ctl(g, ctl)) return -1;
-    g->warn_count = (int64_t)ctl[4];
     g->n_groups = n_out;
     g->finished = 1;
     return n_out;
@@ -1106,13 +1116,11 @@ extern "C" int64_t vgpu_groupby_fetch(vgpu_groupby *g, void *keys, uint8_t *key_
     hipStream_t s = vg_stream();
     int64_t n = g->n_groups;
     size_t vb = (size_t)(n + 7) / 8;
+    uint64_t ctl[GB_CTL_WORDS];
     if (n > 0) {
-        if (keys && !g->L.key_none) {
-            if (g->L.key_is_double || true) {
-                // bool keys are emitted as 8-byte 0/1 words; narrow on the host
-                VG_TRY(hipMemcpyAsync(keys, g->out_keys.ptr, (size_t)n * 8, hipMemcpyDeviceToHost, s));
-            }
-        }
+        // bool keys are emitted as 8-byte 0/1 words; callers narrow them
+        if (keys && !g->L.key_none)
+            VG_TRY(hipMemcpyAsync(keys, g->out_keys.ptr, (size_t)n * 8, hipMemcpyDeviceToHost, s));
         if (key_validity) VG_TRY(hipMemcpyAsync(key_validity, g->out_kvalid.ptr, vb, hipMemcpyDeviceToHost, s));
         for (int a = 0; a < g->aggs.n; a++) {
             if (agg_values && agg_values[a])
@@ -1121,7 +1129,9 @@ extern "C" int64_t vgpu_groupby_fetch(vgpu_groupby *g, void *keys, uint8_t *key_
                 VG_TRY(hipMemcpyAsync(agg_validity[a], g->out_valid[a].ptr, vb, hipMemcpyDeviceToHost, s));
         }
     }
+    VG_TRY(hipMemcpyAsync(ctl, g->ctl.ptr, sizeof ctl, hipMemcpyDeviceToHost, s));
     VG_TRY(hipStreamSynchronize(s));
+    g->warn_count = (int64_t)ctl[4];
     return g->warn_count;
 }
 
@@ -1142,8 +1152,7 @@ extern "C" int vgpu_groupby_result_device(vgpu_groupby *g, const void **keys, co
 
 extern "C" void vgpu_groupby_destroy(vgpu_groupby *g) {
     if (!g) return;
-    hipError_t e = hipStreamSynchronize(vg_stream());
-    (void)e;
+    // buffers go back to the stream-ordered pool: no synchronization needed
     g->hot.release();
     g->cold.release();
     g->ctl.release();

@@ -18,6 +18,11 @@ int vg_ensure_init();
 void vg_kt_begin();
 void vg_kt_end();
 
+// launch-shape knobs (see vgpu_tune_set)
+enum { VG_TUNE_GB_LDS_MAX_BLOCKS, VG_TUNE_GB_LDS_ROWS, VG_TUNE_GB_GLB_MAX_BLOCKS, VG_TUNE_GB_GLB_ROWS,
+       VG_TUNE_GB_LDS_MAX_SLOTS, VG_TUNE_GB_LDS_BUDGET, VG_TUNE_COUNT };
+int64_t vg_tune(int knob);
+
 #define VG_TRY(call)                                                          \
     do {                                                                      \
         hipError_t e_ = (call);                                               \
@@ -48,6 +53,14 @@ void vg_kt_end();
         }                                                                     \
     } while (0)
 
+// Caching device allocator: blocks are power-of-two sized (>= 4 KiB) and
+// returned to per-size free lists instead of hipFree, so a steady stream of
+// queries allocates nothing and never takes hipFree's implicit device
+// synchronization.  vg_pool_trim() hands cached blocks back to the runtime.
+void *vg_pool_alloc(size_t bytes, size_t *granted);
+void vg_pool_free(void *ptr, size_t bytes);
+void vg_pool_trim();
+
 // A device buffer that only grows; reused across batches so the steady
 // state allocates nothing.
 struct DevBuf {
@@ -55,24 +68,23 @@ struct DevBuf {
     size_t cap = 0;
     int reserve(size_t bytes) {
         if (bytes <= cap) return 0;
-        size_t n = cap ? cap : 4096;
-        while (n < bytes) n *= 2;
         if (ptr) {
-            hipError_t e = hipFree(ptr);
-            (void)e;
+            vg_pool_free(ptr, cap);
             ptr = nullptr;
             cap = 0;
         }
-        if (hipMalloc(&ptr, n) != hipSuccess) {
-            vg_set_error("hipMalloc(%zu) failed", n);
+        size_t got = 0;
+        ptr = vg_pool_alloc(bytes, &got);
+        if (!ptr) {
+            vg_set_error("device allocation of %zu bytes failed", bytes);
             return -1;
         }
-        cap = n;
+        cap = got;
         return 0;
     }
     template <class T> T *as() const { return static_cast<T *>(ptr); }
     void release() {
-        if (ptr) { hipError_t e = hipFree(ptr); (void)e; }
+        if (ptr) vg_pool_free(ptr, cap);
         ptr = nullptr;
         cap = 0;
     }

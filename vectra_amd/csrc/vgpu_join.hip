@@ -90,10 +90,11 @@ __global__ void k_jt_fill_unique(const uint32_t *slot_of, int64_t n, const uint6
     rows[ent[(int64_t)s * E + 1] >> 32] = (uint32_t)r;
 }
 
-__global__ void k_jt_sortkeys(const uint32_t *slot_of, int64_t n, uint64_t *skeys, uint32_t *svals) {
+__global__ void k_jt_sortkeys(const uint32_t *slot_of, int64_t n, int64_t cap, uint64_t *skeys, uint32_t *svals) {
     int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
-    skeys[r] = ((uint64_t)slot_of[r] << 32) | (uint64_t)(0xFFFFFFFFu - (uint32_t)r);
+    uint64_t s = slot_of[r] == JT_NONE ? (uint64_t)cap + 1 : (uint64_t)slot_of[r];
+    skeys[r] = (s << 32) | (uint64_t)(0xFFFFFFFFu - (uint32_t)r);
     svals[r] = (uint32_t)r;
 }
 
@@ -205,15 +206,15 @@ static vgpu_jtable *join_build_impl(const vgpu_col *key, int64_t n_build, const 
         } else {
             DevBuf skeys;
             if (skeys.reserve((size_t)n_build * 8)) return fail();
-            k_jt_sortkeys<<<vg_blocks(n_build, 256), 256, 0, s>>>(slot_of.as<uint32_t>(), n_build,
+            k_jt_sortkeys<<<vg_blocks(n_build, 256), 256, 0, s>>>(slot_of.as<uint32_t>(), n_build, cap,
                                                                    skeys.as<uint64_t>(), t->rows.as<uint32_t>());
-            int rc = vg_sort_pairs(skeys.as<uint64_t>(), t->rows.as<uint32_t>(), n_build, 0, 64, t->sort_tmp);
-            hipError_t e = hipStreamSynchronize(s);
-            skeys.release();
-            if (rc || e != hipSuccess) {
-                if (!rc) vg_set_error("join_build: sort failed: %s", hipGetErrorString(e));
-                return fail();
-            }
+            // slots (cap + 1 for rows kept out of the table) sit above the 32
+            // bits of row complement: sort only the bits they occupy
+            int hi = 33;
+            while (hi < 64 && ((int64_t)1 << (hi - 32)) <= cap + 1) hi++;
+            int rc = vg_sort_pairs(skeys.as<uint64_t>(), t->rows.as<uint32_t>(), n_build, 0, hi, t->sort_tmp);
+            skeys.release();   // stream-ordered pool: safe before the sort has run
+            if (rc) return fail();
         }
         if (n_payload > 0) {
             PaySet ps;
@@ -234,10 +235,12 @@ static vgpu_jtable *join_build_impl(const vgpu_col *key, int64_t n_build, const 
             }
         }
     }
-    if (hipStreamSynchronize(s) != hipSuccess) {
-        vg_set_error("join_build: %s", hipGetErrorString(hipGetLastError()));
+    if (hipGetLastError() != hipSuccess) {
+        vg_set_error("join_build: kernel launch failed");
         return fail();
     }
+    // scratch returns to the stream-ordered pool; later work on the stream
+    // sees the finished table without a host synchronization here
     cnt.release();
     slot_of.release();
     off.release();
@@ -431,8 +434,6 @@ extern "C" int vgpu_join_unmatched_build(vgpu_jtable *t, int64_t *n_out) {
 
 extern "C" void vgpu_join_destroy(vgpu_jtable *t) {
     if (!t) return;
-    hipError_t e = hipStreamSynchronize(vg_stream());
-    (void)e;
     DevBuf *bufs[] = {&t->ent, &t->rows, &t->cnt, &t->un, &t->off_main, &t->off_un,
                       &t->out_probe, &t->out_build, &t->totals, &t->scan_tmp, &t->sort_tmp, &t->bmatched};
     for (DevBuf *b : bufs) b->release();

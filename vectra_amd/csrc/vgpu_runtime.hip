@@ -6,6 +6,7 @@
 #include <stdarg.h>
 #include <stdio.h>
 #include <mutex>
+#include <vector>
 
 // ---------------------------------------------------------------------------
 // error channel / device state
@@ -55,6 +56,61 @@ extern "C" int vgpu_init(int device) {
 int vg_ensure_init() {
     if (g_stream) return 0;
     return vgpu_init(0);
+}
+
+// ---------------------------------------------------------------------------
+// caching allocator
+// ---------------------------------------------------------------------------
+
+static std::mutex g_pool_mu;
+static std::vector<void *> g_pool[64];   // free blocks of 2^k bytes
+
+static int pool_class(size_t bytes, size_t *rounded) {
+    int k = 12;
+    while (((size_t)1 << k) < bytes) k++;
+    *rounded = (size_t)1 << k;
+    return k;
+}
+
+void *vg_pool_alloc(size_t bytes, size_t *granted) {
+    size_t r;
+    int k = pool_class(bytes ? bytes : 1, &r);
+    {
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        if (!g_pool[k].empty()) {
+            void *p = g_pool[k].back();
+            g_pool[k].pop_back();
+            *granted = r;
+            return p;
+        }
+    }
+    void *p = nullptr;
+    if (hipMalloc(&p, r) != hipSuccess) {
+        // give cached blocks back and retry once
+        vg_pool_trim();
+        if (hipMalloc(&p, r) != hipSuccess) return nullptr;
+    }
+    *granted = r;
+    return p;
+}
+
+void vg_pool_free(void *ptr, size_t bytes) {
+    if (!ptr) return;
+    size_t r;
+    int k = pool_class(bytes ? bytes : 1, &r);
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    g_pool[k].push_back(ptr);
+}
+
+void vg_pool_trim() {
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    for (auto &v : g_pool) {
+        for (void *p : v) {
+            hipError_t e = hipFree(p);
+            (void)e;
+        }
+        v.clear();
+    }
 }
 
 extern "C" void *vgpu_malloc(size_t bytes) {
@@ -125,6 +181,37 @@ extern "C" double vgpu_timer_stop(void) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, g_t0, g_t1) != hipSuccess) return -1.0;
     return (double)ms;
+}
+
+// launch-shape knobs
+static const char *g_tune_names[VG_TUNE_COUNT] = {"gb.lds.max_blocks", "gb.lds.rows_per_block", "gb.glb.max_blocks",
+                                                  "gb.glb.rows_per_block", "gb.lds.max_slots", "gb.lds.budget"};
+static int64_t g_tune[VG_TUNE_COUNT] = {2048, 8192, 16384, 1024, 4096, 65536};
+
+int64_t vg_tune(int knob) { return g_tune[knob]; }
+
+extern "C" int vgpu_tune_set(const char *name, int64_t value) {
+    for (int i = 0; i < VG_TUNE_COUNT; i++)
+        if (strcmp(name, g_tune_names[i]) == 0) {
+            if (value <= 0) {
+                vg_set_error("vgpu_tune_set: %s must be positive", name);
+                return -1;
+            }
+            if (i == VG_TUNE_GB_LDS_BUDGET && value > 65536) {
+                vg_set_error("vgpu_tune_set: %s is at most 65536 bytes", name);
+                return -1;
+            }
+            g_tune[i] = value;
+            return 0;
+        }
+    vg_set_error("vgpu_tune_set: unknown knob %s", name);
+    return -1;
+}
+
+extern "C" int64_t vgpu_tune_get(const char *name) {
+    for (int i = 0; i < VG_TUNE_COUNT; i++)
+        if (strcmp(name, g_tune_names[i]) == 0) return g_tune[i];
+    return -1;
 }
 
 // per-launch kernel timing (off by default: it synchronizes each launch)

@@ -225,6 +225,8 @@ def vgpu() -> C.CDLL:
         _proto(lib, "vgpu_sort_pairs_u64", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_int])
         _proto(lib, "vgpu_timer_start", C.c_int, [])
         _proto(lib, "vgpu_timer_stop", C.c_double, [])
+        _proto(lib, "vgpu_tune_set", C.c_int, [C.c_char_p, C.c_int64])
+        _proto(lib, "vgpu_tune_get", C.c_int64, [C.c_char_p])
         _proto(lib, "vgpu_kernel_timing", C.c_int, [C.c_int])
         _proto(lib, "vgpu_kernel_time_ms", C.c_double, [C.POINTER(C.c_int64), C.c_int])
         _libs["vgpu"] = lib
