@@ -503,6 +503,195 @@ k_gb_feed_join(GbLayout L, JoinFeedArgs A, ColPair cp, JView jv, PredInfo pi, vg
 }
 
 // ---------------------------------------------------------------------------
+// pipelined probe -> aggregate for the star shape
+// ---------------------------------------------------------------------------
+//
+// The common join-then-aggregate query reads, per probe row, one probe key,
+// at most two probe-side values and build-side values that ride in the
+// join table's entries (vgpu_join_build_payload).  The generic kernel walks
+// one dependent chain per row (key load -> entry load -> LDS update), which
+// leaves the wave waiting on memory; this one gives every thread ITEMS rows
+// at a time and issues all of their key, value and entry loads before it
+// consumes any, so ITEMS table probes are in flight per thread.
+
+struct StarArgs {
+    int64_t n_logical;
+    const int32_t *sel;
+    int64_t pos_base;
+    vgpu_col pkey;                 // probe key column
+    vgpu_col pval[2];              // probe-side value columns (n_pval of them)
+    int32_t n_pval;
+    int32_t join_kind;
+    int32_t key_src;               // 0 = probe column pval-free key (kcol), 1 = payload slot kslot, 2 = none
+    int32_t kslot;
+    vgpu_col kcol;                 // probe-side group key (key_src 0)
+    int32_t n_inputs;
+    // input i: a = probe value pa[i] (-1 none), b = payload slot pb[i] (-1 none); product when both
+    int32_t pa[VGPU_MAX_INPUTS];
+    int32_t pb[VGPU_MAX_INPUTS];
+    int32_t type[VGPU_MAX_INPUTS]; // result type of input i
+    int32_t ta[VGPU_MAX_INPUTS];   // type of the probe operand
+    int32_t tb[VGPU_MAX_INPUTS];   // type of the payload operand
+    int32_t ktype;                 // group key type
+    uint64_t *hot;
+    uint64_t *cold;
+    uint64_t *ctl;
+    int64_t cap;
+    int32_t lds_slots;
+    int32_t vm_lds;
+};
+
+template <int ITEMS>
+__global__ void __launch_bounds__(GB_THREADS)
+k_gb_star(GbLayout L, StarArgs A, JView jv, PredInfo pi, ColSet pcs) {
+    extern __shared__ uint8_t dyn[];
+    const int tid = threadIdx.x;
+    uint64_t *lhot = reinterpret_cast<uint64_t *>(dyn + A.vm_lds);
+    const int S = A.lds_slots;
+    uint64_t *lcold = lhot + (size_t)(S + 2) * L.RH;
+    const uint64_t empty_g = L.key_is_double ? VG_EMPTY_F64 : VG_EMPTY_I64;
+    const uint64_t empty_j = jv.key_is_double ? VG_EMPTY_F64 : VG_EMPTY_I64;
+    if (S) gb_lds_init(L, lhot, lcold, S, empty_g);
+    const uint64_t jmask = (uint64_t)jv.cap - 1;
+    const int64_t step = (int64_t)gridDim.x * GB_THREADS * ITEMS;
+    for (int64_t base = (int64_t)blockIdx.x * GB_THREADS * ITEMS; base < A.n_logical; base += step) {
+        int64_t li[ITEMS], pr[ITEMS];
+        bool live[ITEMS];
+        uint64_t kn[ITEMS], pv0[ITEMS], pv1[ITEMS];
+        bool pv0ok[ITEMS], pv1ok[ITEMS], kfound[ITEMS];
+        uint64_t e0[ITEMS], e1[ITEMS], e2[ITEMS], e3[ITEMS], slot[ITEMS];
+        uint64_t gk[ITEMS];
+        bool gkok[ITEMS];
+        // phase 1: probe-side loads
+#pragma unroll
+        for (int i = 0; i < ITEMS; i++) {
+            li[i] = base + (int64_t)i * GB_THREADS + tid;
+            live[i] = li[i] < A.n_logical;
+            pr[i] = live[i] ? (A.sel ? (int64_t)A.sel[li[i]] : li[i]) : 0;
+            if (live[i] && pi.present) live[i] = vg_fast_pred(pi, pcs, pr[i]);
+            int cls = VG_KEY_NA;
+            kn[i] = 0;
+            if (live[i]) cls = vg_key_class(A.pkey, pr[i], kn[i]);
+            kfound[i] = live[i] && (cls == VG_KEY_REGULAR || cls == VG_KEY_SPECIAL);
+            slot[i] = cls == VG_KEY_SPECIAL ? (uint64_t)jv.cap : (vg_key_hash(kn[i]) & jmask);
+            pv0ok[i] = pv1ok[i] = false;
+            pv0[i] = pv1[i] = 0;
+            if (live[i] && A.n_pval > 0) {
+                pv0ok[i] = vg_valid(A.pval[0].validity, pr[i]);
+                pv0[i] = pv0ok[i] ? vg_load_bits(A.pval[0], pr[i]) : 0;
+            }
+            if (live[i] && A.n_pval > 1) {
+                pv1ok[i] = vg_valid(A.pval[1].validity, pr[i]);
+                pv1[i] = pv1ok[i] ? vg_load_bits(A.pval[1], pr[i]) : 0;
+            }
+            gkok[i] = false;
+            gk[i] = 0;
+            if (live[i] && A.key_src == 0) {
+                gkok[i] = vg_valid(A.kcol.validity, pr[i]);
+                gk[i] = gkok[i] ? vg_load_bits(A.kcol, pr[i]) : 0;
+            }
+        }
+        // phase 2: first table probe of every row, all loads in flight
+#pragma unroll
+        for (int i = 0; i < ITEMS; i++) {
+            e0[i] = e1[i] = e2[i] = e3[i] = 0;
+            if (kfound[i]) {
+                const uint64_t *e = jv.ent + slot[i] * jv.E;
+                e0[i] = e[0];
+                e1[i] = e[1];
+                e2[i] = e[2];
+                e3[i] = jv.E > 3 ? e[3] : 0;
+            }
+        }
+        // phase 3: finish collisions (rare at load <= 1/2), then aggregate
+#pragma unroll
+        for (int i = 0; i < ITEMS; i++) {
+            if (!live[i]) continue;
+            uint32_t cnt = 0;
+            if (kfound[i]) {
+                if (slot[i] != (uint64_t)jv.cap) {
+                    int64_t probes = 0;
+                    while (e0[i] != kn[i] && e0[i] != empty_j && probes < jv.cap) {
+                        slot[i] = (slot[i] + 1) & jmask;
+                        const uint64_t *e = jv.ent + slot[i] * jv.E;
+                        e0[i] = e[0];
+                        e1[i] = e[1];
+                        e2[i] = e[2];
+                        e3[i] = jv.E > 3 ? e[3] : 0;
+                        probes++;
+                    }
+                    if (e0[i] == kn[i]) cnt = jt_count(jv, e1[i]);
+                } else {
+                    cnt = jt_count(jv, e1[i]);
+                }
+            }
+            uint32_t off = (uint32_t)(e1[i] >> 32);
+            for (uint32_t j = 0; j < cnt || (j == 0 && cnt == 0 && A.join_kind == VGPU_JOIN_LEFT); j++) {
+                bool matched = cnt > 0;
+                // payload values of this pair
+                uint64_t pay0 = 0, pay1 = 0;
+                bool pok0 = false, pok1 = false;
+                if (matched) {
+                    if (j == 0) {
+                        pay0 = e2[i]; pok0 = (e1[i] >> 24) & 1;
+                        pay1 = e3[i]; pok1 = (e1[i] >> 25) & 1;
+                    } else {
+                        pay0 = jv.cpay[0][off + j]; pok0 = jv.cval[0][off + j];
+                        if (jv.n_payload > 1) { pay1 = jv.cpay[1][off + j]; pok1 = jv.cval[1][off + j]; }
+                    }
+                }
+                uint64_t pos = jf_pos((uint64_t)A.pos_base, li[i], j, !matched);
+                // group key
+                uint64_t gkn = 0;
+                int gcls = VG_KEY_REGULAR;
+                if (A.key_src == 0) {
+                    gcls = vg_key_class_bits(A.ktype, gkok[i], gk[i], gkn);
+                } else if (A.key_src == 1) {
+                    bool ok = A.kslot == 0 ? pok0 : pok1;
+                    gcls = vg_key_class_bits(A.ktype, ok, A.kslot == 0 ? pay0 : pay1, gkn);
+                }
+                if (L.key_is_double && gcls == VG_KEY_REGULAR && gkn == 0) {
+                    uint64_t raw = A.key_src == 0 ? gk[i] : (A.kslot == 0 ? pay0 : pay1);
+                    gb_note_zero_key(A.ctl, raw, pos);
+                }
+                int64_t ls = S ? gb_lds_slot(lhot, S, L.RH, gcls, gkn, empty_g) : -1;
+                // inputs
+#define STAR_APPLY(HP, CP)                                                                          \
+                do {                                                                                \
+                    gb_update_row(L, HP, pos);                                                      \
+                    for (int q = 0; q < A.n_inputs; q++) {                                          \
+                        InVal a, b, v;                                                              \
+                        a.valid = A.pa[q] == 0 ? pv0ok[i] : pv1ok[i];                               \
+                        a.bits = A.pa[q] == 0 ? pv0[i] : pv1[i];                                    \
+                        b.valid = A.pb[q] == 0 ? pok0 : pok1;                                       \
+                        b.bits = A.pb[q] == 0 ? pay0 : pay1;                                        \
+                        if (A.pb[q] < 0) v = a;                                                     \
+                        else if (A.pa[q] < 0) v = b;                                                \
+                        else v = jf_mul(a, A.ta[q], b, A.tb[q]);                                    \
+                        if (A.pb[q] >= 0 && !matched) v.valid = false;                              \
+                        gb_update_input(L, HP, CP, q, pos, v);                                      \
+                    }                                                                               \
+                } while (0)
+                // one expansion per address space: LDS record or HBM record
+                if (ls >= 0) {
+                    uint64_t *h = lhot + (size_t)ls * L.RH, *c = lcold + (size_t)ls * L.RC;
+                    STAR_APPLY(h, c);
+                } else {
+                    int64_t gs = gb_global_slot(L, A.hot, A.cap, gcls, gkn, pos, A.ctl);
+                    if (gs >= 0) {
+                        uint64_t *h = A.hot + (size_t)gs * L.RH, *c = A.cold + (size_t)gs * L.RC;
+                        STAR_APPLY(h, c);
+                    }
+                }
+#undef STAR_APPLY
+                if (!matched) break;
+            }
+        }
+    }
+    if (S) gb_lds_merge(L, lhot, lcold, S, empty_g, A.hot, A.cold, A.cap, A.ctl);
+}
+
+// ---------------------------------------------------------------------------
 // finish: compact, order, emit
 // ---------------------------------------------------------------------------
 
@@ -925,6 +1114,77 @@ extern "C" int vgpu_groupby_feed(vgpu_groupby *g, const vgpu_col *key, const vgp
     return 0;
 }
 
+// Is this feed the star shape k_gb_star handles?  (join with one or two
+// embedded payload columns, no rows[] reads, a fast or absent predicate,
+// inputs that are a probe column, a payload column or the product of one
+// of each.)  Fills st when it is.
+static bool gb_star_shape(const vgpu_groupby *g, const vgpu_feed_spec *sp, const JoinFeedArgs &A, const PredInfo &pi,
+                          StarArgs *st) {
+    if (!sp->join || (sp->join_kind != VGPU_JOIN_INNER && sp->join_kind != VGPU_JOIN_LEFT)) return false;
+    if (sp->join->n_payload < 1 || sp->join->n_payload > 2 || sp->join->E != 4 || A.need_rows) return false;
+    if (pi.present && !pi.fast) return false;
+    memset(st, 0, sizeof *st);
+    st->n_logical = A.n_logical;
+    st->sel = A.sel;
+    st->pos_base = A.pos_base;
+    st->pkey = sp->probe_cols[sp->probe_key_col];
+    st->join_kind = sp->join_kind;
+    int pcol[2] = {-1, -1};
+    auto pval_slot = [&](int col) -> int {
+        for (int k = 0; k < st->n_pval; k++)
+            if (pcol[k] == col) return k;
+        if (st->n_pval == 2) return -1;
+        pcol[st->n_pval] = col;
+        st->pval[st->n_pval] = sp->probe_cols[col];
+        return st->n_pval++;
+    };
+    if (g->L.key_none) {
+        st->key_src = 2;
+    } else if (sp->key_side == 0) {
+        st->key_src = 0;
+        st->kcol = sp->probe_cols[sp->key_col];
+        st->ktype = st->kcol.type;
+    } else {
+        st->key_src = 1;
+        st->kslot = A.bpay[sp->key_col];
+        st->ktype = sp->build_cols[sp->key_col].type;
+    }
+    st->n_inputs = sp->n_inputs;
+    for (int i = 0; i < sp->n_inputs; i++) {
+        const vgpu_input &in = sp->inputs[i];
+        int sa = in.side_a, ca = in.col_a, sb = in.side_b, cb = in.col_b;
+        if (in.op == 1 && sa == 1 && sb == 0) {   // a * b == b * a: probe operand first
+            sa = 0; ca = in.col_b; sb = 1; cb = in.col_a;
+        }
+        st->pa[i] = st->pb[i] = -1;
+        if (in.op == 0) {
+            if (sa == 0) {
+                st->pa[i] = pval_slot(ca);
+                if (st->pa[i] < 0) return false;
+                st->ta[i] = sp->probe_cols[ca].type;
+                st->type[i] = st->ta[i];
+            } else {
+                st->pb[i] = A.bpay[ca];
+                st->tb[i] = sp->build_cols[ca].type;
+                st->type[i] = st->tb[i];
+            }
+            continue;
+        }
+        if (sa != 0 || sb != 1) return false;
+        st->pa[i] = pval_slot(ca);
+        if (st->pa[i] < 0) return false;
+        st->pb[i] = A.bpay[cb];
+        st->ta[i] = sp->probe_cols[ca].type;
+        st->tb[i] = sp->build_cols[cb].type;
+        st->type[i] = (st->ta[i] == VGPU_DOUBLE || st->tb[i] == VGPU_DOUBLE) ? VGPU_DOUBLE : VGPU_INT64;
+    }
+    st->hot = A.hot;
+    st->cold = A.cold;
+    st->ctl = A.ctl;
+    st->cap = A.cap;
+    return true;
+}
+
 extern "C" int vgpu_groupby_feed_join(vgpu_groupby *g, const vgpu_feed_spec *sp) {
     if (g->finished) {
         vg_set_error("groupby: feed after finish");
@@ -1045,6 +1305,26 @@ extern "C" int vgpu_groupby_feed_join(vgpu_groupby *g, const vgpu_feed_spec *sp)
     A.vm_lds = (int32_t)vm;
     A.lds_slots = gb_lds_slots(g, vm);
     size_t lds = vm + (A.lds_slots ? (size_t)(A.lds_slots + 2) * (g->L.RH + g->L.RC) * 8 : 0);
+    StarArgs st;
+    if (gb_star_shape(g, sp, A, pi, &st)) {
+        st.lds_slots = A.lds_slots;
+        st.vm_lds = 0;
+        size_t slds = A.lds_slots ? (size_t)(A.lds_slots + 2) * (g->L.RH + g->L.RC) * 8 : 0;
+        int items = (int)vg_tune(VG_TUNE_GB_STAR_ITEMS);
+        unsigned grid = vg_blocks(n_logical, vg_tune(A.lds_slots ? VG_TUNE_GB_LDS_ROWS : VG_TUNE_GB_GLB_ROWS),
+                                  vg_tune(A.lds_slots ? VG_TUNE_GB_LDS_MAX_BLOCKS : VG_TUNE_GB_GLB_MAX_BLOCKS));
+        vg_kt_begin();
+        switch (items) {
+        case 1: k_gb_star<1><<<grid, GB_THREADS, slds, vg_stream()>>>(g->L, st, jv, pi, cp.probe); break;
+        case 2: k_gb_star<2><<<grid, GB_THREADS, slds, vg_stream()>>>(g->L, st, jv, pi, cp.probe); break;
+        case 8: k_gb_star<8><<<grid, GB_THREADS, slds, vg_stream()>>>(g->L, st, jv, pi, cp.probe); break;
+        default: k_gb_star<4><<<grid, GB_THREADS, slds, vg_stream()>>>(g->L, st, jv, pi, cp.probe); break;
+        }
+        VG_LAUNCH_CHECK();
+        vg_kt_end();
+        g->any_fed = 1;
+        return 0;
+    }
     vg_kt_begin();
     k_gb_feed_join<<<gb_grid(n_logical, A.lds_slots), GB_THREADS, lds, vg_stream()>>>(g->L, A, cp, jv, pi, prog);
     VG_LAUNCH_CHECK();
