@@ -171,6 +171,10 @@ def run_c2(args, nv):
                "sample": f"first {sample:,} fact rows of the same tables, inner + left join -> group_by(attr) "
                          f"(oracle restatement of the reference engine, {secs:.1f} s)"}
     gbs = (2 * per_launch) / (ms / 1e3) / 1e9
+    traffic = None
+    tfile = ROOT / "profiles" / "k_gb_star_join_aggregate_50M.json"
+    if tfile.exists() and n_fact == 50_000_000 and n_dim == 1_000_000:
+        traffic = json.loads(tfile.read_text())["traffic_bytes_per_launch"]
     return {
         "metric": METRIC, "value": rows_s, "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
@@ -182,8 +186,11 @@ def run_c2(args, nv):
                    "fact_rows": n_fact, "dim_rows": n_dim, "queries_per_step": 2},
         "gb_per_s": gbs,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_gb_feed_join", "bytes_per_launch": per_launch, "avg_launch_ms": avg_ms},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/k_gb_star_join_aggregate_50M.md"
+                     if traffic else None,
+                     "kernel": "k_gb_star (fused probe -> aggregate)", "bytes_per_launch": per_launch,
+                     "avg_launch_ms": avg_ms},
         "cpu_baseline": cpu,
     }
 
@@ -198,7 +205,18 @@ def main():
     ap.add_argument("--dim-rows", type=int, default=1_000_000)
     ap.add_argument("--cpu-sample", type=int, default=10_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="copy 1 GiB three times (reference kernel for rocprofv3 FETCH_SIZE / WRITE_SIZE scaling)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": 1 << 30, "repeats": 3}))
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only "

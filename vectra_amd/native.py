@@ -8,6 +8,7 @@ nodes and the HIP kernels.  Loading fails loudly when a library is missing.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 from typing import Any, Sequence
 
@@ -15,6 +16,8 @@ import numpy as np
 
 PKG = Path(__file__).resolve().parent
 LIB = PKG / "lib"
+if os.environ.get("VECTRA_AMD_LIB_DIR"):
+    LIB = Path(os.environ["VECTRA_AMD_LIB_DIR"])
 
 VEC_INT64, VEC_DOUBLE, VEC_BOOL, VEC_STRING = 0, 1, 2, 3
 (EXPR_COL_REF, EXPR_LIT_INT64, EXPR_LIT_DOUBLE, EXPR_LIT_BOOL, EXPR_LIT_STRING, EXPR_LIT_NA, EXPR_ARITH,
