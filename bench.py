@@ -195,12 +195,188 @@ def run_c2(args, nv):
     }
 
 
+# ---------------------------------------------------------------- c1: drop-in nodes
+
+def run_c1(args, nv):
+    """filter(x > 0) |> group_by(key) |> summarise(total = sum(value), n = n(),
+    avg = mean(value)) over 1M rows in 10 row groups of 100,000, through the
+    drop-in FilterNode and GroupAggNode (host batches staged per row group)."""
+    n = 1_000_000
+    r = np.random.default_rng(0x7665637472610100)
+    cols = {"id": np.arange(n, dtype=np.int64), "key": r.integers(0, 1000, n), "x": r.normal(size=n),
+            "value": r.uniform(0, 100, n)}
+    t = nv.Table(cols, rg_rows=100_000)
+    aggs = [("total", "sum", "value", 0), ("n", "n", None, 0), ("avg", "mean", "value", 0)]
+    pred = ("cmp", ">", ("col", "x"), ("dbl", 0.0))
+
+    def query(engine):
+        node = nv.filter_node(engine, t.scan(), pred)
+        return nv.collect(nv.group_agg_node(engine, node, ["key"], aggs, "/tmp"))
+
+    lib = nv.vgpu()
+    for _ in range(args.warmup):
+        res = query("gpu")
+    keep = cols["x"] > 0
+    assert int(res["n"][0].sum()) == int(keep.sum()) and list(res["key"][0]) == list(range(1000))
+    lib.vgpu_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        query("gpu")
+    lib.vgpu_synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    import ctypes as C
+    lib.vgpu_kernel_timing(1)
+    lib.vgpu_kernel_time_ms(None, 1)
+    query("gpu")
+    nl = C.c_int64()
+    kms = lib.vgpu_kernel_time_ms(C.byref(nl), 1)
+    lib.vgpu_kernel_timing(0)
+    # algorithmic bytes of the query: x, key, value read once; 1,000 x 4 outputs
+    alg = n * 24 + 1000 * 4 * 8
+    achieved = alg / (kms / 1e3) / 1e9
+    log(f"[bench] c1: {ms:.3f} ms/query wall; {nl.value} timed kernels, {kms:.3f} ms total")
+    cpu = None
+    if not args.no_cpu:
+        nv.load_oracle(ROOT / "oracle" / "_build" / "liboracle.so")
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            query("oracle")
+        secs = (time.perf_counter() - t0) / reps
+        cpu = {"value": n / secs, "unit": "rows/s", "cores": 1, "kind": "port",
+               "sample": f"the full 1M-row query, {reps} runs (oracle restatement of the reference engine, "
+                         f"{secs * 1e3:.0f} ms per query)"}
+    return {
+        "metric": METRIC, "value": n / (ms / 1e3), "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded numpy generators)",
+        "config": {"workload": "filter(x > 0) |> group_by(key) |> summarise(total = sum(value), n = n(), "
+                               "avg = mean(value)); 1M rows as 10 row groups of 100,000 (key U{0..999}, x N(0,1), "
+                               "value U(0,100)); drop-in FilterNode + GroupAggNode, host batches staged over PCIe",
+                   "rows": n, "row_groups": 10},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_filter_mask/k_filter_write + k_gb_feed, all launches of one query",
+                     "bytes_per_launch": alg / max(nl.value, 1), "avg_launch_ms": kms / max(nl.value, 1)},
+        "cpu_baseline": cpu,
+    }
+
+
+# ---------------------------------------------------------------- c3: group-by over 200M rows
+
+C3_AGGS = [("n", -1, 0), ("sum", 0, 0), ("min", 0, 0), ("max", 0, 0), ("mean", 0, 0),
+           ("sum", 1, 0), ("min", 1, 0), ("max", 1, 0), ("mean", 1, 0)]
+
+
+def c3_data(n, card, seed=0x7665637472610300):
+    """key: U over `card` distinct int64 (card > 16: random distinct values;
+    else 0..card-1); v1 lognormal(0,1); v2 int64 U[-1e9, 1e9]."""
+    r = np.random.default_rng(seed + card)
+    idx = r.integers(0, card, n)
+    present = int(np.count_nonzero(np.bincount(idx, minlength=card)))
+    if card > 16:
+        ks = np.unique(r.integers(-2**62, 2**62, int(card * 1.01) + 1024))
+        r.shuffle(ks)
+        key = ks[:card][idx]
+    else:
+        key = idx
+    del idx
+    v1 = r.lognormal(0.0, 1.0, n)
+    v2 = r.integers(-10**9, 10**9, n, endpoint=True)
+    return {"key": key, "v1": v1, "v2": v2}, present
+
+
+def run_c3(args, nv, card):
+    n = args.c3_rows
+    log(f"[bench] generating c3 data: {n:,} rows, {card:,} key values")
+    t = time.perf_counter()
+    data, present = c3_data(n, card)
+    log(f"[bench] data ready in {time.perf_counter() - t:.1f}s ({present:,} keys present); uploading")
+    kc, c1, c2 = (nv.DeviceColumn(data[c]) for c in ("key", "v1", "v2"))
+    lib = nv.vgpu()
+
+    def step(fetch=False):
+        gb = nv.GroupBy(nv.VEC_INT64, [nv.VEC_DOUBLE, nv.VEC_INT64], C3_AGGS, nv.VGPU_ORDER_KEY, 0)
+        gb.feed(kc, [c1, c2], n)
+        ng = gb.finish()
+        out = gb.fetch() if fetch else None
+        gb.free()
+        return ng, out
+
+    rel_err = None
+    for i in range(max(args.warmup, 1)):
+        ng, out = step(fetch=(i == 0))
+        if i == 0:
+            # size-independent checks of the full result
+            k = out["key"][0]
+            assert ng == present, (ng, present)
+            assert np.all(k[1:] > k[:-1]), "keys not in ascending order"
+            assert int(out["a0"][0].sum()) == n
+            assert int(np.round(out["a5"][0]).astype(np.int64).sum()) == int(data["v2"].sum())
+            assert out["a2"][0].min() == data["v1"].min() and out["a3"][0].max() == data["v1"].max()
+            assert out["a6"][0].min() == data["v2"].min() and out["a7"][0].max() == data["v2"].max()
+            import math
+            tot = math.fsum(out["a1"][0])
+            ex = math.fsum(data["v1"])
+            rel_err = abs(tot - ex) / ex
+            log(f"[bench] sum of group sums of v1 vs fsum of v1: relative error {rel_err:.2e}")
+            del out
+    log("[bench] warmup done, result checked")
+    lib.vgpu_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    lib.vgpu_synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    import ctypes as C
+    lib.vgpu_kernel_timing(1)
+    lib.vgpu_kernel_time_ms(None, 1)
+    for _ in range(3):
+        step()
+    nl = C.c_int64()
+    kms = lib.vgpu_kernel_time_ms(C.byref(nl), 1)
+    lib.vgpu_kernel_timing(0)
+    avg_ms = kms / max(nl.value, 1)
+    per_launch = n * 24
+    achieved = per_launch / (avg_ms / 1e3) / 1e9
+    log(f"[bench] c3 card {card}: {ms:.3f} ms/step; k_gb_feed avg {avg_ms:.3f} ms, {achieved:.0f} GB/s")
+    cpu = None
+    if not args.no_cpu:
+        sample = min(args.cpu_sample, n)
+        nv.load_oracle(ROOT / "oracle" / "_build" / "liboracle.so")
+        tb = nv.Table({k: v[:sample] for k, v in data.items()}, rg_rows=1 << 20)
+        aggs = [("n", "n", None, 0)] + [(f"{f}_{c}", f, c, 0) for c in ("v1", "v2")
+                                        for f in ("sum", "min", "max", "mean")]
+        t0 = time.perf_counter()
+        nv.collect(nv.group_agg_node("oracle", tb.scan(), ["key"], aggs, "/tmp"))
+        secs = time.perf_counter() - t0
+        cpu = {"value": sample / secs, "unit": "rows/s", "cores": 1, "kind": "port",
+               "sample": f"first {sample:,} rows, sort-based group-by (oracle restatement of the reference "
+                         f"engine's GroupAgg over SortNode, {secs:.1f} s)"}
+    label = "high" if card > 16 else "low"
+    return {
+        "metric": METRIC, "value": n / (ms / 1e3), "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded numpy generators)",
+        "config": {"workload": f"group_by(key) |> summarise(n, sum/min/max/mean of v1 and of v2) over {n:,} rows, "
+                               f"{label} cardinality ({card:,} key values, {present:,} present); v1 lognormal(0,1), "
+                               "v2 int64 U[-1e9,1e9]; key order, result left in device memory; device-resident inputs",
+                   "rows": n, "key_values": card, "groups": present},
+        "result_check": {"groups": present, "sum_v1_rel_err_vs_fsum": rel_err},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_gb_feed", "bytes_per_launch": per_launch, "avg_launch_ms": avg_ms},
+        "cpu_baseline": cpu,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2", choices=["c2"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3-high", "c3-low"])
+    ap.add_argument("--c3-rows", type=int, default=200_000_000)
     ap.add_argument("--fact-rows", type=int, default=50_000_000)
     ap.add_argument("--dim-rows", type=int, default=1_000_000)
     ap.add_argument("--cpu-sample", type=int, default=10_000_000)
@@ -226,7 +402,14 @@ def main():
     lib = nv.vgpu()
     if lib.vgpu_init(0) != 0:
         raise SystemExit("bench: " + lib.vgpu_last_error().decode())
-    out = run_c2(args, nv)
+    if args.workload == "c1":
+        out = run_c1(args, nv)
+    elif args.workload == "c3-high":
+        out = run_c3(args, nv, 50_000_000)
+    elif args.workload == "c3-low":
+        out = run_c3(args, nv, 16)
+    else:
+        out = run_c2(args, nv)
     print(json.dumps(out), flush=True)
 
 

@@ -186,3 +186,12 @@ def test_groupby_high_cardinality_growth(gpu, oracle):
     mk = lambda: table(gpu, 300000, 3, "int", 10**9, 20000, p_na=0.001, p_nan=0.001)
     check(gpu, mk, ["g"], [("n", "n", None, 0), ("sv", "sum", "v", 0), ("lo", "min", "x", 0),
                            ("sx", "sum", "x", 0)], None)
+
+
+@pytest.mark.parametrize("key_kind,card,temp_dir", [("int", 8, "/tmp"), ("int", 10**9, "/tmp"),
+                                                    ("int", 3000, None), ("dbl", 64, None)])
+def test_groupby_large_single_feed(gpu, oracle, key_kind, card, temp_dir):
+    """one batch of more than 2^20 rows and no group-count hint: a key sketch
+    sizes the table and picks the LDS-private path"""
+    mk = lambda: table(gpu, 1_200_000, 11, key_kind, card, 0, p_na=0.01, p_nan=0.01)
+    check(gpu, mk, ["g"], ALL_AGGS, temp_dir)

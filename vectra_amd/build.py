@@ -27,7 +27,8 @@ CC = os.environ.get("CC", "gcc")
 
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
-    # packed FP32 instructions stay off
+    # packed FP32 instructions (v_pk_*_f32) stay off: some of their forms have
+    # given silently wrong results on this hardware
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     # FP64 atomic add as one hardware instruction instead of a CAS loop
     "-munsafe-fp-atomics",

@@ -27,6 +27,7 @@
 #include "vgpu_join_table.h"
 
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 #define GB_THREADS 256
@@ -61,9 +62,11 @@ struct vgpu_groupby {
     int64_t cap = 0;          // hashed slots (pow2); +2 reserved slots follow
     int64_t n_groups = 0;     // after finish
     int64_t hint = 0;
+    int64_t est_groups = 0;   // groups expected when no hint was given (sketch or groups seen); 0 = unknown
     int32_t finished = 0;
     int32_t any_fed = 0;
     DevBuf hot, cold;
+    DevBuf sketch;            // cardinality sketch registers (GB_SK_REGS u32) + NaN-key row count (u64)
     DevBuf ctl;               // [0] occupied count, [1] error flag, [2] zero-key -0 pos,
                               // [3] zero-key +0 pos, [4] warn count, [5] tmp counter,
                               // [6] or-reduce, [7] and-reduce
@@ -951,6 +954,80 @@ __global__ void k_gb_ctl_init(uint64_t *ctl, int finish_only) {
 }
 
 // grows the table so that `incoming` more groups keep the load <= 1/2
+// Cardinality sketch of one feed's keys (HyperLogLog, Flajolet et al. 2007,
+// with the linear-counting correction for small counts): sizes the table
+// and picks the LDS-private path for a large feed when the caller gave no
+// group count.  4096 registers give a relative error of about 1.6%; the
+// table is sized at twice a padded estimate, and a table that still fills
+// up is reported by finish as an overflow, never as a wrong result.
+#define GB_SK_BITS 12
+#define GB_SK_REGS (1 << GB_SK_BITS)
+#define GB_SK_MIN_ROWS (1 << 20)
+
+__global__ void __launch_bounds__(GB_THREADS)
+k_gb_sketch(vgpu_col key, const int32_t *sel, int64_t n_logical, uint32_t *regs, unsigned long long *nan_rows) {
+    __shared__ uint32_t r[GB_SK_REGS];
+    __shared__ unsigned long long nn;
+    for (int i = threadIdx.x; i < GB_SK_REGS; i += GB_THREADS) r[i] = 0;
+    if (threadIdx.x == 0) nn = 0;
+    __syncthreads();
+    unsigned long long my_nan = 0;
+    const int64_t stride = (int64_t)gridDim.x * GB_THREADS;
+    for (int64_t li = (int64_t)blockIdx.x * GB_THREADS + threadIdx.x; li < n_logical; li += stride) {
+        int64_t row = sel ? (int64_t)sel[li] : li;
+        uint64_t kn;
+        int cls = vg_key_class(key, row, kn);
+        if (cls == VG_KEY_NAN) { my_nan++; continue; }
+        if (cls != VG_KEY_REGULAR) continue;
+        uint64_t h = vg_mix64(kn);
+        uint32_t idx = (uint32_t)(h >> (64 - GB_SK_BITS));
+        uint64_t rest = (h << GB_SK_BITS) | (1ull << (GB_SK_BITS - 1));
+        uint32_t rank = (uint32_t)__builtin_clzll(rest) + 1;
+        // repeated keys find their register already set: no atomic
+        if (r[idx] < rank) atomicMax(&r[idx], rank);
+    }
+    if (my_nan) atomicAdd(&nn, my_nan);
+    __syncthreads();
+    for (int i = threadIdx.x; i < GB_SK_REGS; i += GB_THREADS)
+        if (r[i]) atomicMax(&regs[i], r[i]);
+    if (threadIdx.x == 0 && nn) atomicAdd(nan_rows, nn);
+}
+
+__global__ void k_gb_sketch_clear(uint32_t *regs, unsigned long long *nan_rows) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < GB_SK_REGS) regs[i] = 0;
+    if (i == 0) *nan_rows = 0;
+}
+
+// estimated groups of a feed: distinct regular keys + one per NaN row + the
+// two reserved groups (NA key, EMPTY-valued key)
+static int gb_sketch(vgpu_groupby *g, const vgpu_col *key, const int32_t *sel, int64_t n_logical, int64_t *est) {
+    size_t bytes = GB_SK_REGS * 4 + 8;
+    if (g->sketch.reserve(bytes)) return -1;
+    uint32_t *regs = g->sketch.as<uint32_t>();
+    auto *nan_rows = reinterpret_cast<unsigned long long *>(regs + GB_SK_REGS);
+    k_gb_sketch_clear<<<GB_SK_REGS / 256, 256, 0, vg_stream()>>>(regs, nan_rows);
+    VG_LAUNCH_CHECK();
+    k_gb_sketch<<<vg_blocks(n_logical, 65536, 2048), GB_THREADS, 0, vg_stream()>>>(*key, sel, n_logical, regs,
+                                                                                     nan_rows);
+    VG_LAUNCH_CHECK();
+    std::vector<uint32_t> h(GB_SK_REGS + 2);
+    VG_TRY(hipMemcpyAsync(h.data(), regs, bytes, hipMemcpyDeviceToHost, vg_stream()));
+    VG_TRY(hipStreamSynchronize(vg_stream()));
+    double m = GB_SK_REGS, z = 0;
+    int zeros = 0;
+    for (int i = 0; i < GB_SK_REGS; i++) {
+        z += std::ldexp(1.0, -(int)h[i]);
+        zeros += h[i] == 0;
+    }
+    double e = 0.7213 / (1.0 + 1.079 / m) * m * m / z;
+    if (e <= 2.5 * m && zeros > 0) e = m * std::log(m / zeros);
+    uint64_t nan;
+    memcpy(&nan, &h[GB_SK_REGS], 8);
+    *est = (int64_t)std::ceil(e) + (int64_t)nan + 2;
+    return 0;
+}
+
 static int gb_ensure_capacity(vgpu_groupby *g, int64_t incoming) {
     uint64_t ctl[GB_CTL_WORDS];
     if (g->cap == 0) {
@@ -966,8 +1043,9 @@ static int gb_ensure_capacity(vgpu_groupby *g, int64_t incoming) {
         vg_set_error("groupby: hash table overflow");
         return -1;
     }
-    // occupied slots are counted at finish only; bound them by rows fed
+    // new groups of this feed are bounded by its rows (or its sketch)
     int64_t occupied = (int64_t)ctl[0];
+    if (g->hint <= 0) g->est_groups = std::max(g->est_groups, occupied);
     int64_t need = pow2_at_least(2 * (occupied + incoming));
     if (need <= g->cap) return 0;
     DevBuf nhot, ncold;
@@ -1031,11 +1109,12 @@ extern "C" vgpu_groupby *vgpu_groupby_create(int key_type, int n_inputs, const i
 // LDS slots for privatization: the table must hold the expected groups at
 // load <= 1/2 within the LDS budget
 static int gb_lds_slots(const vgpu_groupby *g, size_t vm_bytes) {
-    if (g->hint <= 0) return 0;
+    int64_t groups = g->L.key_none ? 1 : g->hint > 0 ? g->hint : g->est_groups;
+    if (groups <= 0) return 0;
     size_t rec = (size_t)(g->L.RH + g->L.RC) * 8;
     size_t lim = (size_t)vg_tune(VG_TUNE_GB_LDS_BUDGET);
     size_t budget = lim > vm_bytes ? lim - vm_bytes : 0;
-    int64_t want = pow2_at_least(2 * g->hint);
+    int64_t want = pow2_at_least(2 * groups);
     if (want > vg_tune(VG_TUNE_GB_LDS_MAX_SLOTS) || (size_t)(want + 2) * rec > budget) return 0;
     return (int)want;
 }
@@ -1076,9 +1155,16 @@ extern "C" int vgpu_groupby_feed(vgpu_groupby *g, const vgpu_col *key, const vgp
             vg_set_error("groupby: input %d type %d differs from declared %d", i, inputs[i].type, g->L.in_type[i]);
             return -1;
         }
-    // groups created by this feed are bounded by its rows (and by the hint)
+    // groups created by this feed are bounded by its rows, by the hint, or
+    // (a large feed without a hint) by its sketched key count
     int64_t incoming = n_logical;
     if (g->hint > 0 && incoming > g->hint) incoming = g->hint;
+    if (g->hint <= 0 && !g->L.key_none && n_logical >= GB_SK_MIN_ROWS) {
+        int64_t est;
+        if (gb_sketch(g, key, sel, n_logical, &est)) return -1;
+        g->est_groups = std::max(g->est_groups, est);
+        incoming = std::min(incoming, est + est / 4 + 4096);
+    }
     if (gb_ensure_capacity(g, incoming)) return -1;
 
     FeedArgs A;
@@ -1435,6 +1521,7 @@ extern "C" void vgpu_groupby_destroy(vgpu_groupby *g) {
     // buffers go back to the stream-ordered pool: no synchronization needed
     g->hot.release();
     g->cold.release();
+    g->sketch.release();
     g->ctl.release();
     g->sort_keys.release();
     g->sort_vals.release();
