@@ -275,10 +275,13 @@ int vgpu_sort_pairs_u64(uint64_t *keys, uint32_t *values, int64_t n,
 
 /* ---- tuning -----------------------------------------------------------
  * Launch-shape knobs the autotuner sets (defaults in parentheses):
- *   "gb.lds.max_blocks" (2048)  "gb.lds.rows_per_block" (8192)
+ *   "gb.lds.max_blocks" (1024)  "gb.lds.rows_per_block" (8192)
  *   "gb.glb.max_blocks" (16384) "gb.glb.rows_per_block" (1024)
  *   "gb.lds.max_slots" (4096)   "gb.lds.budget" (65536 bytes)
- * Returns -1 for an unknown name. */
+ *   "gb.star.items" (4: rows in flight per thread of the fused star probe)
+ *   "gb.part.min_groups" (65536: groups from which a first feed of at
+ *                         least 2^20 rows is hash-partitioned)
+ * Returns -1 for an unknown name or a value <= 0. */
 int     vgpu_tune_set(const char *name, int64_t value);
 int64_t vgpu_tune_get(const char *name);
 

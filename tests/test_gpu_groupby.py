@@ -195,3 +195,56 @@ def test_groupby_large_single_feed(gpu, oracle, key_kind, card, temp_dir):
     sizes the table and picks the LDS-private path"""
     mk = lambda: table(gpu, 1_200_000, 11, key_kind, card, 0, p_na=0.01, p_nan=0.01)
     check(gpu, mk, ["g"], ALL_AGGS, temp_dir)
+
+
+@pytest.fixture
+def part_threshold(gpu):
+    """lowers the group count from which a large first feed is partitioned"""
+    lib = gpu.vgpu()
+    old = lib.vgpu_tune_get(b"gb.part.min_groups")
+    assert lib.vgpu_tune_set(b"gb.part.min_groups", 1000) == 0
+    yield
+    lib.vgpu_tune_set(b"gb.part.min_groups", old)
+
+
+@pytest.mark.parametrize("key_kind,card,temp_dir", [("int", 3000, "/tmp"), ("int", 3000, None),
+                                                    ("int", 10**9, None), ("dbl", 64, "/tmp"),
+                                                    ("dbl", 4000, None)])
+def test_groupby_partitioned(gpu, oracle, part_threshold, key_kind, card, temp_dir):
+    """one batch of 1.2M rows hash-partitioned into LDS-sized partitions
+    (NA keys, INT64_MIN keys, NaN keys, signed zeros, NA / NaN inputs)"""
+    # NaN keys only on the hash path: the reference's sorted path splits
+    # groups around NaN keys (its merge sort ranks NaN equal to every key)
+    p_nan = 0.0 if key_kind == "dbl" and temp_dir is not None else 0.01
+    mk = lambda: table(gpu, 1_200_000, 17, key_kind, card, 0, p_na=0.02, p_nan=p_nan)
+    check(gpu, mk, ["g"], ALL_AGGS, temp_dir)
+
+
+@pytest.mark.parametrize("temp_dir", [None, "/tmp"])
+def test_groupby_partitioned_hot_keys(gpu, oracle, part_threshold, temp_dir):
+    """partitions far above one item (a hot key, a 30% NA key): partial
+    records of several blocks are combined"""
+    n = 1_300_000
+    r = rng(23)
+    kv = r.integers(-10**12, 10**12, n)
+    u = r.random(n)
+    kv[u < 0.25] = 7
+    kv[(u >= 0.25) & (u < 0.27)] = np.iinfo(np.int64).min
+    x = dbl_with_specials(r, n, 0.01, 0.02)
+    cols = {"g": (kv, r.random(n) >= 0.3), "x": (x, with_na(r, n, 0.02)),
+            "v": (r.integers(-10**12, 10**12, n), with_na(r, n, 0.02)),
+            "b": (r.random(n) < 0.5, with_na(r, n, 0.02)), "ax": np.abs(np.nan_to_num(x, nan=0.0))}
+    check(gpu, lambda: gpu.Table(cols, rg_rows=0), ["g"], ALL_AGGS, temp_dir)
+
+
+@pytest.mark.parametrize("temp_dir", [None, "/tmp"])
+def test_groupby_partitioned_then_more_batches(gpu, oracle, part_threshold, temp_dir):
+    """a partitioned first batch folded into the hash table by later batches"""
+    mk = lambda: table(gpu, 1_600_000, 29, "int", 200_000, 1_100_000, p_na=0.01, p_nan=0.01)
+    check(gpu, mk, ["g"], ALL_AGGS, temp_dir)
+
+
+def test_groupby_partitioned_after_filter(gpu, oracle, part_threshold):
+    """a selection vector from the filter feeds the partitioned path"""
+    mk = lambda: table(gpu, 2_400_000, 31, "int", 50_000, 0, p_na=0.01, p_nan=0.01)
+    check(gpu, mk, ["g"], ALL_AGGS, "/tmp", pred=("cmp", ">", ("col", "ax"), ("dbl", 0.3)))

@@ -55,6 +55,56 @@ struct GbAggs {
     vgpu_agg a[VGPU_MAX_AGGS];
 };
 
+// ---------------------------------------------------------------------------
+// partitioned feed (one large feed of many groups)
+//
+// Per-row HBM atomics do not scale to tens of millions of groups: every
+// update is a device-scope atomic on a random record.  A large feed whose
+// sketch shows more groups than LDS tables can hold is instead
+// hash-partitioned in two passes (2^B1 x 2^B2 partitions of a few hundred
+// groups each) into staged rows, and every partition is then aggregated in
+// an LDS table by one block, which writes its finished records once,
+// contiguously.  Partitions with more than GBP_CH rows (hot keys) are split
+// into several items whose partial records one more block combines.  The
+// records stay as the group-by's "run" until finish, which sorts them
+// directly; any later feed first folds the run into the hash table.
+//
+// Staged row (W = n_inputs + 2 words): key bits (NaN keys already replaced
+// by their per-row synthetic key), input bits, and a meta word: bits 0-31
+// the feed-local row index, 32-33 the key class, 34.. input validity.
+// ---------------------------------------------------------------------------
+
+#define GBP_G1 2048                // first-pass blocks (tiles of the feed)
+#define GBP_T2 32768               // second-pass rows per block
+#define GBP_CH 4096                // rows per aggregation item
+#define GBP_MAX_BITS 10            // fan-out bits per pass
+
+struct GbpArgs {
+    int32_t W, RW, B1, B2, S, G1;
+    int64_t n_rows;                // logical rows of the feed (bounds every buffer)
+    int64_t T1;                    // first-pass rows per block
+    int64_t P;                     // partitions
+    int64_t pos_base;
+    uint64_t *S1, *S2;             // staged rows
+    uint32_t *H1;                  // first-pass histogram [2^B1][G1]
+    uint64_t *O1;                  // its exclusive scan (+ total at [2^B1 * G1])
+    uint64_t *blkbase2;            // second-pass blocks per bucket, scanned (2^B1 + 1)
+    uint32_t *H2;                  // second-pass histogram (bucket, sub-bucket, block)
+    uint64_t *O2;
+    int64_t NB2max;
+    uint64_t *pstart;              // partition row starts (P + 1)
+    uint32_t *nitems;              // items per partition (P)
+    uint64_t *itembase;            // scanned items (P + 1)
+    int64_t NImax;
+    uint32_t *icount;              // regular records per item
+    uint64_t *irec;                // first record index of an item
+    uint32_t *rflags;              // reserved partials of an item: 1 NA, 2 EMPTY-valued key
+    uint32_t *mlist;               // partitions with more than one item
+    uint64_t *rec;                 // records: (n_rows + 2) x RW words; n_rows / n_rows+1 = NA / EMPTY key
+    uint64_t *hot, *cold, *ctl;    // hash table (fallback for records that do not fit an LDS table)
+    int64_t cap;
+};
+
 struct vgpu_groupby {
     GbLayout L;
     GbAggs aggs;
@@ -71,7 +121,11 @@ struct vgpu_groupby {
                               // [3] zero-key +0 pos, [4] warn count, [5] tmp counter,
                               // [6] or-reduce, [7] and-reduce
     // finish buffers
-    DevBuf sort_keys, sort_vals, sort_tmp;
+    DevBuf sort_keys, sort_vals, sort_tmp, cmp;
+    // partitioned feed: staged rows, bookkeeping and the run's records
+    DevBuf gbp_s1, gbp_s2, gbp_aux, run_rec;
+    GbpArgs gbp;
+    int32_t run = 0;          // the records of a partitioned feed await finish
     DevBuf out_keys, out_kvalid;
     std::vector<DevBuf> out_vals, out_valid;
     int64_t warn_count = 0;
@@ -203,17 +257,25 @@ __device__ __forceinline__ void gb_note_zero_key(uint64_t *ctl, uint64_t raw_bit
 // table init
 // ---------------------------------------------------------------------------
 
+// The arrays are written as flat word streams (consecutive lanes store
+// consecutive words), each word's value set by its index within a record.
+__device__ __forceinline__ void gb_init_words(uint64_t *a, int64_t n_words, int R, uint64_t ones, bool key_word0,
+                                              uint64_t key) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int w = (int)(i % R);
+    const int step = (int)(stride % R);
+    for (; i < n_words; i += stride) {
+        a[i] = (key_word0 && w == 0) ? key : (((ones >> w) & 1) ? ~0ull : 0ull);
+        w += step;
+        if (w >= R) w -= R;
+    }
+}
+
 __global__ void k_gb_init(uint64_t *hot, uint64_t *cold, int64_t n_slots, int RH, int RC,
                           uint64_t hot_ones, uint64_t cold_ones, uint64_t empty) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (; i < n_slots; i += stride) {
-        uint64_t *h = hot + i * RH;
-        h[0] = empty;
-        for (int w = 1; w < RH; w++) h[w] = ((hot_ones >> w) & 1) ? ~0ull : 0ull;
-        uint64_t *c = cold + i * RC;
-        for (int w = 0; w < RC; w++) c[w] = ((cold_ones >> w) & 1) ? ~0ull : 0ull;
-    }
+    gb_init_words(hot, n_slots * RH, RH, hot_ones, true, empty);
+    gb_init_words(cold, n_slots * RC, RC, cold_ones, false, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -698,39 +760,149 @@ k_gb_star(GbLayout L, StarArgs A, JView jv, PredInfo pi, ColSet pcs) {
 // finish: compact, order, emit
 // ---------------------------------------------------------------------------
 
-// collects occupied slots with their sort keys; the NA group (slot cap) is
-// kept out of the ORDER_KEY sort and appended last by the host
-__global__ void k_gb_compact(GbLayout L, const uint64_t *hot, int64_t cap, int order,
-                             uint64_t *ctl, uint64_t *skeys, uint32_t *svals) {
-    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t empty = L.key_is_double ? VG_EMPTY_F64 : VG_EMPTY_I64;
-    bool take = false;
-    uint64_t sk = 0;
-    if (s < cap + 2) {
-        const uint64_t *h = hot + s * L.RH;
-        if (s < cap) take = h[0] != empty;
-        else take = h[1] != 0;
-        if (s == cap) ctl[8] = h[1];                                      // rows of the NA group
-        if (take && order == VGPU_ORDER_KEY && s == cap) take = false;   // NA: appended last
-        if (take) {
-            if (order == VGPU_ORDER_FIRST_SEEN) sk = h[L.w_fpos];
-            else if (s == cap + 1) sk = 0;                                // INT64_MIN: smallest
-            else if (L.key_is_double) sk = vg_f64_ordered(h[0]);          // NaN groups sort above +Inf by position
-            else sk = vg_i64_ordered(h[0]);
-        }
+// Where finish reads the group records from: the hash table (hot and cold
+// arrays, reserved NA / EMPTY-key records at slots cap and cap+1) or the
+// records of a partitioned feed (hot and cold words of a record side by
+// side, reserved records at indices n and n+1).
+struct RecSrc {
+    const uint64_t *hot;
+    const uint64_t *cold;
+    int64_t hstride, cstride;
+    int64_t na_slot, sp_slot;
+};
+
+// Sort key of record s in the finish order; false when the record takes no
+// part in the sort (an empty slot, an unused reserved record, or the NA
+// group of the key order, which is appended after the sort).
+__device__ __forceinline__ bool gb_sort_entry(const GbLayout &L, const RecSrc &R, int order, int64_t s,
+                                              uint64_t &sk) {
+    const uint64_t *h = R.hot + s * R.hstride;
+    if (s == R.na_slot || s == R.sp_slot) {
+        if (h[1] == 0) return false;
+    } else if (h[0] == (L.key_is_double ? VG_EMPTY_F64 : VG_EMPTY_I64)) {
+        return false;
     }
-    uint64_t m = vg_ballot(take);
-    if (!m) return;
-    uint64_t base = 0;
-    int leader = __ffsll((unsigned long long)m) - 1;
-    if (vg_lane() == leader) base = atomicAdd((unsigned long long *)&ctl[5], (unsigned long long)__popcll(m));
-    base = __shfl(base, leader, 64);
-    if (take) {
-        uint64_t idx = base + vg_lane_prefix(m);
-        skeys[idx] = sk;
-        svals[idx] = (uint32_t)s;
-        atomicOr((unsigned long long *)&ctl[6], (unsigned long long)sk);
-        atomicAnd((unsigned long long *)&ctl[7], (unsigned long long)sk);
+    if (s == R.na_slot && order == VGPU_ORDER_KEY) return false;
+    if (order == VGPU_ORDER_FIRST_SEEN) sk = h[L.w_fpos];
+    else if (s == R.sp_slot) sk = 0;                          // INT64_MIN: smallest
+    else if (L.key_is_double) sk = vg_f64_ordered(h[0]);      // NaN groups sort above +Inf by position
+    else sk = vg_i64_ordered(h[0]);
+    return true;
+}
+
+// exclusive prefix of v over the block; *total = the block's sum
+__device__ __forceinline__ uint32_t gb_block_prefix(uint32_t v, uint32_t *lds, uint32_t &total) {
+    const int lane = vg_lane(), wid = threadIdx.x >> 6;
+    uint32_t x = v;
+    for (int d = 1; d < VG_WAVE; d <<= 1) {
+        uint32_t y = __shfl_up(x, d, VG_WAVE);
+        if (lane >= d) x += y;
+    }
+    if (lane == VG_WAVE - 1) lds[wid] = x;
+    __syncthreads();
+    uint32_t base = 0, tot = 0;
+    for (int w = 0; w < GB_THREADS / VG_WAVE; w++) {
+        uint32_t t = lds[w];
+        if (w < wid) base += t;
+        tot += t;
+    }
+    __syncthreads();
+    total = tot;
+    return base + x - v;
+}
+
+// OR and AND of the block's values, valid in thread 0
+__device__ __forceinline__ void gb_block_or_and(uint64_t &o, uint64_t &a, uint64_t *lds) {
+    for (int d = VG_WAVE / 2; d >= 1; d >>= 1) {
+        o |= __shfl_xor(o, d, VG_WAVE);
+        a &= __shfl_xor(a, d, VG_WAVE);
+    }
+    const int wid = threadIdx.x >> 6;
+    if (vg_lane() == 0) {
+        lds[2 * wid] = o;
+        lds[2 * wid + 1] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int w = 1; w < GB_THREADS / VG_WAVE; w++) {
+            o |= lds[2 * w];
+            a &= lds[2 * w + 1];
+        }
+    __syncthreads();
+}
+
+// Table compaction in two passes over blocks of GB_CMP_SLOTS slots: count
+// the sorted records per block, scan, then write (sort key, slot) pairs at
+// the scanned offsets with per-block OR / AND partials of the sort keys
+// (no contended atomics: one word per block).
+#define GB_CMP_PER_THREAD 16
+#define GB_CMP_SLOTS (GB_THREADS * GB_CMP_PER_THREAD)
+
+__global__ void __launch_bounds__(GB_THREADS)
+k_gb_cmp_count(GbLayout L, RecSrc R, int64_t n_slots, int order, uint32_t *bcount) {
+    __shared__ uint32_t lds[GB_THREADS / VG_WAVE];
+    const int64_t base = (int64_t)blockIdx.x * GB_CMP_SLOTS;
+    uint32_t c = 0;
+#pragma unroll
+    for (int j = 0; j < GB_CMP_PER_THREAD; j++) {
+        int64_t s = base + j * GB_THREADS + threadIdx.x;
+        uint64_t sk;
+        if (s < n_slots && gb_sort_entry(L, R, order, s, sk)) c++;
+    }
+    uint32_t total;
+    gb_block_prefix(c, lds, total);
+    if (threadIdx.x == 0) bcount[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(GB_THREADS)
+k_gb_cmp_write(GbLayout L, RecSrc R, int64_t n_slots, int order, const uint64_t *boff, uint64_t *skeys,
+               uint32_t *svals, uint64_t *por, uint64_t *pand, uint64_t *ctl) {
+    __shared__ uint32_t lds[GB_THREADS / VG_WAVE];
+    __shared__ uint64_t lor[2 * GB_THREADS / VG_WAVE];
+    const int64_t base = (int64_t)blockIdx.x * GB_CMP_SLOTS;
+    uint64_t sk[GB_CMP_PER_THREAD];
+    uint32_t mask = 0;
+    uint64_t o = 0, a = ~0ull;
+#pragma unroll
+    for (int j = 0; j < GB_CMP_PER_THREAD; j++) {
+        int64_t s = base + j * GB_THREADS + threadIdx.x;
+        sk[j] = 0;
+        if (s < n_slots && gb_sort_entry(L, R, order, s, sk[j])) {
+            mask |= 1u << j;
+            o |= sk[j];
+            a &= sk[j];
+        }
+        if (s == R.na_slot && s < n_slots) ctl[8] = R.hot[s * R.hstride + 1];   // rows of the NA group
+    }
+    uint32_t total;
+    uint64_t off = boff[blockIdx.x] + gb_block_prefix(__builtin_popcount(mask), lds, total);
+#pragma unroll
+    for (int j = 0; j < GB_CMP_PER_THREAD; j++)
+        if (mask & (1u << j)) {
+            skeys[off] = sk[j];
+            svals[off] = (uint32_t)(base + j * GB_THREADS + threadIdx.x);
+            off++;
+        }
+    gb_block_or_and(o, a, lor);
+    if (threadIdx.x == 0) {
+        por[blockIdx.x] = o;
+        pand[blockIdx.x] = a;
+    }
+}
+
+// final OR / AND of the per-block partials -> ctl[6] / ctl[7]
+__global__ void __launch_bounds__(GB_THREADS)
+k_gb_or_and_final(const uint64_t *por, const uint64_t *pand, int64_t nb, uint64_t *ctl) {
+    __shared__ uint64_t lor[2 * GB_THREADS / VG_WAVE];
+    uint64_t o = 0, a = ~0ull;
+    for (int64_t i = threadIdx.x; i < nb; i += GB_THREADS) {
+        o |= por[i];
+        a &= pand[i];
+    }
+    gb_block_or_and(o, a, lor);
+    if (threadIdx.x == 0) {
+        ctl[6] = o;
+        ctl[7] = a;
     }
 }
 
@@ -781,20 +953,19 @@ __device__ __forceinline__ void gb_finish_agg(const GbLayout &L, const vgpu_agg 
     }
 }
 
-__global__ void k_gb_emit(GbLayout L, GbAggs AG, const uint64_t *hot, const uint64_t *cold, int64_t cap,
-                          const uint32_t *order_slots, int64_t n_sorted, int na_last, int64_t n_out,
-                          int key_width, EmitOut O, uint64_t *ctl) {
+__global__ void k_gb_emit(GbLayout L, GbAggs AG, RecSrc R, const uint32_t *order_slots, int64_t n_sorted,
+                          int na_last, int64_t n_out, int key_width, EmitOut O, uint64_t *ctl) {
     int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool in = g < n_out;
     int64_t s = 0;
-    if (in) s = (na_last && g == n_sorted) ? cap : (int64_t)order_slots[g];
-    const uint64_t *h = hot + s * L.RH;
-    const uint64_t *c = cold + s * L.RC;
-    bool kvalid = in && s != cap;
+    if (in) s = (na_last && g == n_sorted) ? R.na_slot : (int64_t)order_slots[g];
+    const uint64_t *h = R.hot + s * R.hstride;
+    const uint64_t *c = R.cold + s * R.cstride;
+    bool kvalid = in && s != R.na_slot;
     if (in && !L.key_none) {
         uint64_t kb = 0;
-        if (s == cap + 1) kb = VG_EMPTY_I64;
-        else if (s < cap) {
+        if (s == R.sp_slot) kb = VG_EMPTY_I64;
+        else if (s != R.na_slot) {
             kb = h[0];
             if (L.key_is_double) {
                 if ((kb & 0x7FFFFFFFFFFFFFFFull) > 0x7FF0000000000000ull) kb = 0x7FF8000000000000ull;
@@ -953,7 +1124,6 @@ __global__ void k_gb_ctl_init(uint64_t *ctl, int finish_only) {
     ctl[i] = (i == 2 || i == 3 || i == 7) ? ~0ull : 0ull;
 }
 
-// grows the table so that `incoming` more groups keep the load <= 1/2
 // Cardinality sketch of one feed's keys (HyperLogLog, Flajolet et al. 2007,
 // with the linear-counting correction for small counts): sizes the table
 // and picks the LDS-private path for a large feed when the caller gave no
@@ -1028,6 +1198,7 @@ static int gb_sketch(vgpu_groupby *g, const vgpu_col *key, const int32_t *sel, i
     return 0;
 }
 
+// grows the table so that `incoming` more groups keep the load <= 1/2
 static int gb_ensure_capacity(vgpu_groupby *g, int64_t incoming) {
     uint64_t ctl[GB_CTL_WORDS];
     if (g->cap == 0) {
@@ -1106,6 +1277,614 @@ extern "C" vgpu_groupby *vgpu_groupby_create(int key_type, int n_inputs, const i
     return g;
 }
 
+__device__ __forceinline__ uint64_t gbp_hash(int cls, uint64_t kn) {
+    if (cls == VG_KEY_NA) return 0x9e3779b97f4a7c15ull;       // every NA row in one partition
+    if (cls == VG_KEY_SPECIAL) return 0x7f4a7c159e3779b9ull;  // every EMPTY-valued key row in one partition
+    return vg_key_hash(kn);
+}
+
+// largest i in [0, n) with a[i] <= x (a ascending, a[0] <= x)
+__device__ __forceinline__ int64_t gbp_search(const uint64_t *a, int64_t n, uint64_t x) {
+    int64_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int64_t mid = (lo + hi + 1) >> 1;
+        if (a[mid] <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// key of logical row li of the feed; false when the predicate drops it
+__device__ __forceinline__ bool gbp_source_row(const FeedArgs &A, const PredInfo &pi, const vgpu_prog &prog,
+                                               const ColSet &pcs, uint64_t *regs, uint8_t *regv, int64_t li,
+                                               int64_t &row, int &cls, uint64_t &kn, uint64_t &raw) {
+    row = A.sel ? (int64_t)A.sel[li] : li;
+    if (!vg_pred_row(pi, prog, pcs, row, regs, regv, GB_THREADS, threadIdx.x)) return false;
+    bool v = vg_valid(A.key.validity, row);
+    raw = v ? vg_load_bits(A.key, row) : 0;
+    cls = vg_key_class_bits(A.key.type, v, raw, kn);
+    if (cls == VG_KEY_NAN) {
+        kn = 0x7FF8000000000000ull | ((uint64_t)(A.pos_base + li) & 0x0007FFFFFFFFFFFFull);
+        cls = VG_KEY_REGULAR;
+    }
+    return true;
+}
+
+__global__ void __launch_bounds__(GB_THREADS)
+k_gbp_hist1(FeedArgs A, GbpArgs G, PredInfo pi, vgpu_prog prog, ColSet pcs) {
+    extern __shared__ uint8_t dyn[];
+    uint64_t *regs = reinterpret_cast<uint64_t *>(dyn);
+    uint8_t *regv = dyn + (size_t)prog.n_insn * GB_THREADS * 8;
+    uint32_t *hist = reinterpret_cast<uint32_t *>(dyn + A.vm_lds);
+    const int NB = 1 << G.B1;
+    for (int b = threadIdx.x; b < NB; b += GB_THREADS) hist[b] = 0;
+    __syncthreads();
+    const int64_t r0 = (int64_t)blockIdx.x * G.T1;
+    const int64_t r1 = r0 + G.T1 < A.n_logical ? r0 + G.T1 : A.n_logical;
+    for (int64_t li = r0 + threadIdx.x; li < r1; li += GB_THREADS) {
+        int64_t row;
+        int cls;
+        uint64_t kn, raw;
+        if (!gbp_source_row(A, pi, prog, pcs, regs, regv, li, row, cls, kn, raw)) continue;
+        atomicAdd(&hist[gbp_hash(cls, kn) >> (64 - G.B1)], 1u);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < NB; b += GB_THREADS) G.H1[(int64_t)b * G.G1 + blockIdx.x] = hist[b];
+}
+
+__global__ void __launch_bounds__(GB_THREADS)
+k_gbp_scatter1(GbLayout L, FeedArgs A, GbpArgs G, InSet ins, PredInfo pi, vgpu_prog prog, ColSet pcs) {
+    extern __shared__ uint8_t dyn[];
+    uint64_t *regs = reinterpret_cast<uint64_t *>(dyn);
+    uint8_t *regv = dyn + (size_t)prog.n_insn * GB_THREADS * 8;
+    unsigned long long *cur = reinterpret_cast<unsigned long long *>(dyn + A.vm_lds);
+    const int NB = 1 << G.B1;
+    for (int b = threadIdx.x; b < NB; b += GB_THREADS) cur[b] = G.O1[(int64_t)b * G.G1 + blockIdx.x];
+    __syncthreads();
+    const int64_t r0 = (int64_t)blockIdx.x * G.T1;
+    const int64_t r1 = r0 + G.T1 < A.n_logical ? r0 + G.T1 : A.n_logical;
+    const int lane = vg_lane();
+    for (int64_t li = r0 + threadIdx.x; li < r1; li += GB_THREADS) {
+        int64_t row = 0;
+        int cls = VG_KEY_NA;
+        uint64_t kn = 0, raw = 0;
+        bool ok = gbp_source_row(A, pi, prog, pcs, regs, regv, li, row, cls, kn, raw);
+        if (L.key_is_double) {
+            // first -0 / +0 key positions decide the sign printed for the 0 key;
+            // lanes hold ascending rows, so the lowest flagged lane is the first
+            bool z = ok && cls == VG_KEY_REGULAR && kn == 0;
+            uint64_t mneg = vg_ballot(z && (raw >> 63)), mpos = vg_ballot(z && !(raw >> 63));
+            if (mneg && lane == __ffsll((unsigned long long)mneg) - 1)
+                gb_atomic_min(&A.ctl[2], (uint64_t)(A.pos_base + li));
+            if (mpos && lane == __ffsll((unsigned long long)mpos) - 1)
+                gb_atomic_min(&A.ctl[3], (uint64_t)(A.pos_base + li));
+        }
+        if (!ok) continue;
+        uint64_t dst = atomicAdd(&cur[gbp_hash(cls, kn) >> (64 - G.B1)], 1ull);
+        uint64_t *w = G.S1 + dst * G.W;
+        uint64_t meta = (uint64_t)(uint32_t)li | ((uint64_t)cls << 32);
+        w[0] = kn;
+        for (int i = 0; i < L.n_inputs; i++) {
+            bool v = vg_valid(ins.c[i].validity, row);
+            w[1 + i] = v ? vg_load_bits(ins.c[i], row) : 0;
+            meta |= (uint64_t)v << (34 + i);
+        }
+        w[G.W - 1] = meta;
+    }
+}
+
+// blocks of the second pass per first-pass bucket, scanned (one block)
+__global__ void __launch_bounds__(1024) k_gbp_plan2(GbpArgs G) {
+    __shared__ uint64_t part[1024];
+    const int NB = 1 << G.B1;
+    const int per = (NB + 1023) / 1024;
+    uint64_t local = 0;
+    for (int j = 0; j < per; j++) {
+        int b = threadIdx.x * per + j;
+        if (b < NB) {
+            uint64_t size = G.O1[(int64_t)(b + 1) * G.G1] - G.O1[(int64_t)b * G.G1];
+            local += (size + GBP_T2 - 1) / GBP_T2;
+        }
+    }
+    part[threadIdx.x] = local;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t run = 0;
+        for (int t = 0; t < 1024; t++) {
+            uint64_t v = part[t];
+            part[t] = run;
+            run += v;
+        }
+        G.blkbase2[NB] = run;
+    }
+    __syncthreads();
+    uint64_t run = part[threadIdx.x];
+    for (int j = 0; j < per; j++) {
+        int b = threadIdx.x * per + j;
+        if (b < NB) {
+            G.blkbase2[b] = run;
+            uint64_t size = G.O1[(int64_t)(b + 1) * G.G1] - G.O1[(int64_t)b * G.G1];
+            run += (size + GBP_T2 - 1) / GBP_T2;
+        }
+    }
+}
+
+// second-pass block -> (bucket, chunk, rows); false for blocks past the end
+__device__ __forceinline__ bool gbp_block2(const GbpArgs &G, int64_t blk, int64_t &b, int64_t &chunk, int64_t &nblk,
+                                           int64_t &r0, int64_t &r1) {
+    const int64_t NB = 1 << G.B1;
+    if ((uint64_t)blk >= G.blkbase2[NB]) return false;
+    b = gbp_search(G.blkbase2, NB + 1, (uint64_t)blk);
+    chunk = blk - (int64_t)G.blkbase2[b];
+    nblk = (int64_t)(G.blkbase2[b + 1] - G.blkbase2[b]);
+    int64_t bs = (int64_t)G.O1[b * G.G1], be = (int64_t)G.O1[(b + 1) * G.G1];
+    r0 = bs + chunk * GBP_T2;
+    r1 = r0 + GBP_T2 < be ? r0 + GBP_T2 : be;
+    return true;
+}
+
+__device__ __forceinline__ int64_t gbp_h2_index(const GbpArgs &G, int64_t b, int64_t sub, int64_t chunk, int64_t nblk) {
+    return (int64_t)G.blkbase2[b] * ((int64_t)1 << G.B2) + sub * nblk + chunk;
+}
+
+__device__ __forceinline__ uint32_t gbp_sub(const GbpArgs &G, const uint64_t *w) {
+    int cls = (int)((w[G.W - 1] >> 32) & 3);
+    return (uint32_t)((gbp_hash(cls, w[0]) >> (64 - G.B1 - G.B2)) & ((1u << G.B2) - 1));
+}
+
+__global__ void __launch_bounds__(GB_THREADS) k_gbp_hist2(GbpArgs G) {
+    __shared__ uint32_t hist[1 << GBP_MAX_BITS];
+    int64_t b, chunk, nblk, r0, r1;
+    if (!gbp_block2(G, blockIdx.x, b, chunk, nblk, r0, r1)) return;
+    const int NS = 1 << G.B2;
+    for (int i = threadIdx.x; i < NS; i += GB_THREADS) hist[i] = 0;
+    __syncthreads();
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += GB_THREADS) atomicAdd(&hist[gbp_sub(G, G.S1 + r * G.W)], 1u);
+    __syncthreads();
+    for (int i = threadIdx.x; i < NS; i += GB_THREADS) G.H2[gbp_h2_index(G, b, i, chunk, nblk)] = hist[i];
+}
+
+__global__ void __launch_bounds__(GB_THREADS) k_gbp_scatter2(GbpArgs G) {
+    __shared__ unsigned long long cur[1 << GBP_MAX_BITS];
+    int64_t b, chunk, nblk, r0, r1;
+    if (!gbp_block2(G, blockIdx.x, b, chunk, nblk, r0, r1)) return;
+    const int NS = 1 << G.B2;
+    for (int i = threadIdx.x; i < NS; i += GB_THREADS) cur[i] = G.O2[gbp_h2_index(G, b, i, chunk, nblk)];
+    __syncthreads();
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += GB_THREADS) {
+        const uint64_t *w = G.S1 + r * G.W;
+        uint64_t dst = atomicAdd(&cur[gbp_sub(G, w)], 1ull);
+        uint64_t *d = G.S2 + dst * G.W;
+        for (int k = 0; k < G.W; k++) d[k] = w[k];
+    }
+}
+
+// row start of every partition (P + 1 entries) and its item count
+__global__ void k_gbp_pstart(GbpArgs G) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > G.P) return;
+    const int64_t NB = 1 << G.B1;
+    uint64_t start;
+    if (p == G.P) {
+        start = G.O1[NB * G.G1];
+    } else {
+        int64_t b = p >> G.B2, sub = p & (((int64_t)1 << G.B2) - 1);
+        int64_t nblk = (int64_t)(G.blkbase2[b + 1] - G.blkbase2[b]);
+        start = nblk ? G.O2[gbp_h2_index(G, b, sub, 0, nblk)] : G.O1[b * G.G1];
+    }
+    G.pstart[p] = start;
+}
+
+__global__ void k_gbp_nitems(GbpArgs G) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= G.P) return;
+    uint64_t cnt = G.pstart[p + 1] - G.pstart[p];
+    G.nitems[p] = (uint32_t)((cnt + GBP_CH - 1) / GBP_CH);
+}
+
+// initial (empty) NA and EMPTY-key records, multi-item list counter
+__global__ void k_gbp_init_reserved(GbLayout L, GbpArgs G) {
+    int w = threadIdx.x;
+    const int64_t n = G.n_rows;
+    for (int j = 0; j < 2; j++) {
+        uint64_t *r = G.rec + (n + j) * G.RW;
+        if (w < L.RH) r[w] = w == 0 ? 0 : (((L.hot_init_ones >> w) & 1) ? ~0ull : 0ull);
+        else if (w < L.RH + L.RC) r[w] = ((L.cold_init_ones >> (w - L.RH)) & 1) ? ~0ull : 0ull;
+    }
+    if (w == 0) G.ctl[10] = 0;
+}
+
+// copies the occupied records of an LDS table (slots [0, S) regular) to
+// dst, compacted, with coalesced stores; returns the count in every thread
+__device__ __forceinline__ uint32_t gbp_write_records(const GbLayout &L, const uint64_t *lhot, const uint64_t *lcold,
+                                                      int S, uint64_t empty, uint64_t *dst, int RW, uint16_t *slist,
+                                                      uint32_t *scan_lds) {
+    const int per = (S + GB_THREADS - 1) / GB_THREADS;
+    uint32_t mine = 0;
+    for (int j = 0; j < per; j++) {
+        int s = threadIdx.x * per + j;
+        if (s < S && lhot[(size_t)s * L.RH] != empty) mine++;
+    }
+    uint32_t total;
+    uint32_t off = gb_block_prefix(mine, scan_lds, total);
+    for (int j = 0; j < per; j++) {
+        int s = threadIdx.x * per + j;
+        if (s < S && lhot[(size_t)s * L.RH] != empty) slist[off++] = (uint16_t)s;
+    }
+    __syncthreads();
+    for (int64_t q = threadIdx.x; q < (int64_t)total * RW; q += GB_THREADS) {
+        int k = (int)(q / RW), w = (int)(q % RW);
+        int s = slist[k];
+        dst[q] = w < L.RH ? lhot[(size_t)s * L.RH + w] : lcold[(size_t)s * L.RC + (w - L.RH)];
+    }
+    return total;
+}
+
+__device__ __forceinline__ void gbp_copy_record(const GbLayout &L, const uint64_t *h, const uint64_t *c, uint64_t *dst) {
+    for (int w = 0; w < L.RH; w++) dst[w] = h[w];
+    for (int w = 0; w < L.RC; w++) dst[L.RH + w] = c[w];
+}
+
+// LDS slot of a staged or partial record's key, or -1 (window exhausted)
+__device__ __forceinline__ int64_t gbp_lds_slot(uint64_t *lhot, int S, int RH, int cls, uint64_t kn, uint64_t empty) {
+    if (cls == VG_KEY_NA) return S;
+    if (cls == VG_KEY_SPECIAL) return S + 1;
+    return gb_slot(lhot, S, RH, kn, empty, GB_LDS_PROBES, nullptr);
+}
+
+__global__ void __launch_bounds__(GB_THREADS) k_gbp_agg(GbLayout L, GbpArgs G) {
+    extern __shared__ uint64_t dynw[];   // 8-byte aligned after the static arrays
+    __shared__ uint16_t slist[1024];
+    __shared__ uint32_t scan_lds[GB_THREADS / VG_WAVE];
+    const int64_t item = blockIdx.x;
+    const int64_t NI = (int64_t)G.itembase[G.P];
+    if (item >= NI) {
+        if (threadIdx.x == 0 && item < G.NImax) G.icount[item] = 0;
+        return;
+    }
+    const int S = G.S;
+    uint64_t *lhot = dynw;
+    uint64_t *lcold = lhot + (size_t)(S + 2) * L.RH;
+    const uint64_t empty = L.key_is_double ? VG_EMPTY_F64 : VG_EMPTY_I64;
+    gb_lds_init(L, lhot, lcold, S, empty);
+    const int64_t p = gbp_search(G.itembase, G.P + 1, (uint64_t)item);
+    const int64_t i0 = (int64_t)G.itembase[p], nit = (int64_t)G.itembase[p + 1] - i0, chunk = item - i0;
+    const int64_t pe = (int64_t)G.pstart[p + 1];
+    const int64_t r0 = (int64_t)G.pstart[p] + chunk * GBP_CH;
+    const int64_t r1 = r0 + GBP_CH < pe ? r0 + GBP_CH : pe;
+    bool spilled = false;
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += GB_THREADS) {
+        const uint64_t *w = G.S2 + r * G.W;
+        uint64_t kn = w[0], meta = w[G.W - 1];
+        int cls = (int)((meta >> 32) & 3);
+        uint64_t pos = (uint64_t)G.pos_base + (uint32_t)meta;
+        int64_t ls = gbp_lds_slot(lhot, S, L.RH, cls, kn, empty);
+        uint64_t *h, *c;
+        if (ls >= 0) {
+            h = lhot + (size_t)ls * L.RH;
+            c = lcold + (size_t)ls * L.RC;
+        } else {
+            int64_t gs = gb_global_slot(L, G.hot, G.cap, cls, kn, pos, G.ctl);
+            spilled = true;
+            if (gs < 0) continue;
+            h = G.hot + (size_t)gs * L.RH;
+            c = G.cold + (size_t)gs * L.RC;
+        }
+        gb_update_row(L, h, pos);
+        for (int i = 0; i < L.n_inputs; i++) {
+            InVal v;
+            v.valid = (meta >> (34 + i)) & 1;
+            v.bits = w[1 + i];
+            gb_update_input(L, h, c, i, pos, v);
+        }
+    }
+    if (spilled) G.ctl[9] = 1;
+    __syncthreads();
+    uint32_t total = gbp_write_records(L, lhot, lcold, S, empty, G.rec + r0 * G.RW, G.RW, slist, scan_lds);
+    if (threadIdx.x == 0) {
+        G.icount[item] = total;
+        G.irec[item] = (uint64_t)r0;
+        uint32_t fl = 0;
+        int64_t at = r0 + total;
+        for (int j = 0; j < 2; j++) {
+            const uint64_t *h = lhot + (size_t)(S + j) * L.RH;
+            if (h[1] == 0) continue;
+            const uint64_t *c = lcold + (size_t)(S + j) * L.RC;
+            if (nit == 1) {
+                gbp_copy_record(L, h, c, G.rec + (G.n_rows + j) * G.RW);
+            } else {
+                gbp_copy_record(L, h, c, G.rec + at * G.RW);
+                at++;
+                fl |= 1u << j;
+            }
+        }
+        G.rflags[item] = fl;
+        if (nit > 1 && chunk == 0) G.mlist[atomicAdd((unsigned long long *)&G.ctl[10], 1ull)] = (uint32_t)p;
+    }
+}
+
+// one block per partition with several items: merge their partial records
+__global__ void __launch_bounds__(GB_THREADS) k_gbp_combine(GbLayout L, GbpArgs G) {
+    extern __shared__ uint64_t dynw[];   // 8-byte aligned after the static arrays
+    __shared__ uint16_t slist[1024];
+    __shared__ uint32_t scan_lds[GB_THREADS / VG_WAVE];
+    if ((uint64_t)blockIdx.x >= G.ctl[10]) return;
+    const int S = G.S;
+    uint64_t *lhot = dynw;
+    uint64_t *lcold = lhot + (size_t)(S + 2) * L.RH;
+    const uint64_t empty = L.key_is_double ? VG_EMPTY_F64 : VG_EMPTY_I64;
+    gb_lds_init(L, lhot, lcold, S, empty);
+    const int64_t p = G.mlist[blockIdx.x];
+    const int64_t i0 = (int64_t)G.itembase[p], i1 = (int64_t)G.itembase[p + 1];
+    bool spilled = false;
+    for (int64_t it = i0; it < i1; it++) {
+        const int64_t cnt = G.icount[it];
+        const uint32_t fl = G.rflags[it];
+        const int64_t nres = __builtin_popcount(fl);
+        const uint64_t *base = G.rec + (int64_t)G.irec[it] * G.RW;
+        for (int64_t k = threadIdx.x; k < cnt + nres; k += GB_THREADS) {
+            const uint64_t *r = base + k * G.RW;
+            int cls = VG_KEY_REGULAR;
+            if (k >= cnt) cls = (k == cnt && (fl & 1)) ? VG_KEY_NA : VG_KEY_SPECIAL;
+            int64_t ls = gbp_lds_slot(lhot, S, L.RH, cls, r[0], empty);
+            if (ls >= 0) {
+                gb_merge(L, lhot + (size_t)ls * L.RH, lcold + (size_t)ls * L.RC, r, r + L.RH);
+            } else {
+                int64_t gs = gb_global_slot(L, G.hot, G.cap, cls, r[0], 0, G.ctl);
+                spilled = true;
+                if (gs >= 0) gb_merge(L, G.hot + (size_t)gs * L.RH, G.cold + (size_t)gs * L.RC, r, r + L.RH);
+            }
+        }
+    }
+    if (spilled) G.ctl[9] = 1;
+    __syncthreads();
+    uint32_t total = gbp_write_records(L, lhot, lcold, S, empty, G.rec + (int64_t)G.irec[i0] * G.RW, G.RW, slist,
+                                       scan_lds);
+    for (int64_t it = i0 + 1 + threadIdx.x; it < i1; it += GB_THREADS) G.icount[it] = 0;
+    if (threadIdx.x == 0) {
+        G.icount[i0] = total;
+        for (int j = 0; j < 2; j++) {
+            const uint64_t *h = lhot + (size_t)(S + j) * L.RH;
+            if (h[1]) gbp_copy_record(L, h, lcold + (size_t)(S + j) * L.RC, G.rec + (G.n_rows + j) * G.RW);
+        }
+    }
+}
+
+// finish of a run: (sort key, record) pairs per item at scanned offsets
+__global__ void __launch_bounds__(GB_THREADS)
+k_gbp_collect(GbLayout L, RecSrc R, int order, const uint32_t *icount, const uint64_t *irec, const uint64_t *ioff,
+              uint64_t *skeys, uint32_t *svals, uint64_t *por, uint64_t *pand) {
+    __shared__ uint64_t lor[2 * GB_THREADS / VG_WAVE];
+    const int64_t item = blockIdx.x;
+    const int64_t cnt = icount[item];
+    const int64_t base = cnt ? (int64_t)irec[item] : 0;
+    const uint64_t off = ioff[item];
+    uint64_t o = 0, a = ~0ull;
+    for (int64_t k = threadIdx.x; k < cnt; k += GB_THREADS) {
+        uint64_t sk = 0;
+        gb_sort_entry(L, R, order, base + k, sk);
+        skeys[off + k] = sk;
+        svals[off + k] = (uint32_t)(base + k);
+        o |= sk;
+        a &= sk;
+    }
+    gb_block_or_and(o, a, lor);
+    if (threadIdx.x == 0) {
+        por[item] = o;
+        pand[item] = a;
+    }
+}
+
+// the reserved records of a run join the sort after the regular ones
+__global__ void k_gbp_reserved(GbLayout L, RecSrc R, int order, uint64_t *skeys, uint32_t *svals, uint64_t *ctl) {
+    if (threadIdx.x != 0) return;
+    ctl[8] = R.hot[R.na_slot * R.hstride + 1];
+    const int64_t slots[2] = {R.sp_slot, R.na_slot};
+    for (int j = 0; j < 2; j++) {
+        uint64_t sk;
+        if (!gb_sort_entry(L, R, order, slots[j], sk)) continue;
+        uint64_t at = ctl[5]++;
+        skeys[at] = sk;
+        svals[at] = (uint32_t)slots[j];
+        ctl[6] |= sk;
+        ctl[7] &= sk;
+    }
+}
+
+// a run's records into the hash table (before another feed, or when some
+// record did not fit an LDS table)
+__global__ void __launch_bounds__(GB_THREADS)
+k_gbp_flush(GbLayout L, const uint64_t *rec, int RW, const uint32_t *icount, const uint64_t *irec, int64_t n_rows,
+            uint64_t *hot, uint64_t *cold, int64_t cap, uint64_t *ctl) {
+    const int64_t item = blockIdx.x;
+    const int64_t cnt = icount[item];
+    const int64_t base = cnt ? (int64_t)irec[item] : 0;
+    for (int64_t k = threadIdx.x; k < cnt; k += GB_THREADS) {
+        const uint64_t *r = rec + (base + k) * RW;
+        int64_t gs = gb_global_slot(L, hot, cap, VG_KEY_REGULAR, r[0], 0, ctl);
+        if (gs >= 0) gb_merge(L, hot + (size_t)gs * L.RH, cold + (size_t)gs * L.RC, r, r + L.RH);
+    }
+    if (item == 0 && threadIdx.x < 2) {
+        const uint64_t *r = rec + (n_rows + threadIdx.x) * RW;
+        int64_t gs = cap + threadIdx.x;   // NA -> cap, EMPTY-valued key -> cap + 1
+        if (r[1]) gb_merge(L, hot + (size_t)gs * L.RH, cold + (size_t)gs * L.RC, r, r + L.RH);
+    }
+}
+
+// ---- host side of the partitioned feed
+
+static int gbp_lds_slots(const vgpu_groupby *g) {
+    size_t rw = (size_t)(g->L.RH + g->L.RC) * 8;
+    int S = 512;
+    while (S >= 64 && (size_t)(S + 2) * rw > 61440) S >>= 1;
+    return S >= 64 ? S : 0;
+}
+
+static bool gbp_wanted(const vgpu_groupby *g, int64_t n_logical, int64_t groups) {
+    return !g->any_fed && !g->L.key_none && groups >= vg_tune(VG_TUNE_GB_PART_MIN_GROUPS) &&
+           n_logical >= GB_SK_MIN_ROWS &&
+           n_logical < ((int64_t)1 << 32) - 2 && gbp_lds_slots(g) > 0;
+}
+
+// carves 256-byte aligned pieces out of one buffer
+struct Carve {
+    uint8_t *base;
+    size_t off = 0;
+    template <class T> T *take(size_t count) {
+        T *p = reinterpret_cast<T *>(base + off);
+        off += (count * sizeof(T) + 255) & ~(size_t)255;
+        return p;
+    }
+};
+
+static void gbp_layout(GbpArgs &G, uint8_t *base, size_t *bytes) {
+    const int64_t NB = (int64_t)1 << G.B1, NS = (int64_t)1 << G.B2;
+    const int64_t ML = std::min<int64_t>(G.P, G.n_rows / GBP_CH + 1);
+    Carve c{base};
+    G.H1 = c.take<uint32_t>(NB * G.G1);
+    G.O1 = c.take<uint64_t>(NB * G.G1 + 1);
+    G.blkbase2 = c.take<uint64_t>(NB + 1);
+    G.H2 = c.take<uint32_t>(G.NB2max * NS);
+    G.O2 = c.take<uint64_t>(G.NB2max * NS + 1);
+    G.pstart = c.take<uint64_t>(G.P + 1);
+    G.nitems = c.take<uint32_t>(G.P);
+    G.itembase = c.take<uint64_t>(G.P + 1);
+    G.icount = c.take<uint32_t>(G.NImax);
+    G.irec = c.take<uint64_t>(G.NImax);
+    G.rflags = c.take<uint32_t>(G.NImax);
+    G.mlist = c.take<uint32_t>(ML);
+    *bytes = c.off;
+}
+
+static int gbp_feed(vgpu_groupby *g, FeedArgs A, const InSet &ins, const PredInfo &pi, const vgpu_prog &prog,
+                    const ColSet &pcs, int64_t groups) {
+    hipStream_t st = vg_stream();
+    const GbLayout &L = g->L;
+    const int64_t n = A.n_logical;
+    GbpArgs G;
+    memset(&G, 0, sizeof G);
+    G.W = L.n_inputs + 2;
+    G.RW = L.RH + L.RC;
+    G.S = gbp_lds_slots(g);
+    int64_t want = std::max<int64_t>(64, 2 * groups / G.S);
+    int B = 6;
+    while (((int64_t)1 << B) < want && B < 2 * GBP_MAX_BITS) B++;
+    G.B1 = (B + 1) / 2;
+    G.B2 = B - G.B1;
+    G.P = (int64_t)1 << B;
+    G.G1 = (int)std::min<int64_t>(GBP_G1, (n + 4095) / 4096);
+    G.T1 = ((n + G.G1 - 1) / G.G1 + GB_THREADS - 1) / GB_THREADS * GB_THREADS;
+    G.n_rows = n;
+    G.pos_base = A.pos_base;
+    G.NB2max = ((int64_t)1 << G.B1) + n / GBP_T2 + 1;
+    G.NImax = G.P + n / GBP_CH + 1;
+    size_t aux = 0;
+    gbp_layout(G, nullptr, &aux);
+    size_t stage = (size_t)n * G.W * 8;
+    if (g->gbp_s1.reserve(stage) || g->gbp_s2.reserve(stage) || g->gbp_aux.reserve(aux) ||
+        g->run_rec.reserve((size_t)(n + 2) * G.RW * 8))
+        return -1;
+    gbp_layout(G, g->gbp_aux.as<uint8_t>(), &aux);
+    G.S1 = g->gbp_s1.as<uint64_t>();
+    G.S2 = g->gbp_s2.as<uint64_t>();
+    G.rec = g->run_rec.as<uint64_t>();
+    // records that do not fit an LDS table fall back to the hash table
+    if (g->cap == 0 && gb_alloc_table(g, pow2_at_least(2 * std::max<int64_t>(65536, groups / 16)))) return -1;
+    G.hot = g->hot.as<uint64_t>();
+    G.cold = g->cold.as<uint64_t>();
+    G.ctl = g->ctl.as<uint64_t>();
+    G.cap = g->cap;
+    A.ctl = G.ctl;
+
+    const int64_t NB = (int64_t)1 << G.B1, NS = (int64_t)1 << G.B2;
+    size_t lds1 = (size_t)A.vm_lds + NB * 8;
+    vg_kt_begin();
+    k_gbp_hist1<<<G.G1, GB_THREADS, lds1, st>>>(A, G, pi, prog, pcs);
+    VG_LAUNCH_CHECK();
+    if (vg_scan_u32_to_u64(G.H1, G.O1, NB * G.G1, G.O1 + NB * G.G1, g->sort_tmp)) return -1;
+    k_gbp_scatter1<<<G.G1, GB_THREADS, lds1, st>>>(L, A, G, ins, pi, prog, pcs);
+    VG_LAUNCH_CHECK();
+    k_gbp_plan2<<<1, 1024, 0, st>>>(G);
+    VG_LAUNCH_CHECK();
+    k_gbp_hist2<<<(unsigned)G.NB2max, GB_THREADS, 0, st>>>(G);
+    VG_LAUNCH_CHECK();
+    if (vg_scan_u32_to_u64(G.H2, G.O2, G.NB2max * NS, nullptr, g->sort_tmp)) return -1;
+    k_gbp_scatter2<<<(unsigned)G.NB2max, GB_THREADS, 0, st>>>(G);
+    VG_LAUNCH_CHECK();
+    k_gbp_pstart<<<vg_blocks(G.P + 1, 256), 256, 0, st>>>(G);
+    VG_LAUNCH_CHECK();
+    k_gbp_nitems<<<vg_blocks(G.P, 256), 256, 0, st>>>(G);
+    VG_LAUNCH_CHECK();
+    if (vg_scan_u32_to_u64(G.nitems, G.itembase, G.P, G.itembase + G.P, g->sort_tmp)) return -1;
+    k_gbp_init_reserved<<<1, 128, 0, st>>>(L, G);
+    VG_LAUNCH_CHECK();
+    size_t lds = (size_t)(G.S + 2) * G.RW * 8;
+    k_gbp_agg<<<(unsigned)G.NImax, GB_THREADS, lds, st>>>(L, G);
+    VG_LAUNCH_CHECK();
+    k_gbp_combine<<<(unsigned)std::min<int64_t>(G.P, n / GBP_CH + 1), GB_THREADS, lds, st>>>(L, G);
+    VG_LAUNCH_CHECK();
+    vg_kt_end();
+    g->gbp = G;
+    g->run = 1;
+    g->any_fed = 1;
+    return 0;
+}
+
+static RecSrc gbp_run_src(const vgpu_groupby *g) {
+    const GbpArgs &G = g->gbp;
+    RecSrc R;
+    R.hot = G.rec;
+    R.cold = G.rec + g->L.RH;
+    R.hstride = G.RW;
+    R.cstride = G.RW;
+    R.na_slot = G.n_rows;
+    R.sp_slot = G.n_rows + 1;
+    return R;
+}
+
+// folds the run into the hash table
+static int gbp_flush(vgpu_groupby *g) {
+    const GbpArgs &G = g->gbp;
+    hipStream_t st = vg_stream();
+    if (g->cmp.reserve((size_t)(G.NImax + 1) * 8)) return -1;
+    uint64_t *ioff = g->cmp.as<uint64_t>();
+    if (vg_scan_u32_to_u64(G.icount, ioff, G.NImax, ioff + G.NImax, g->sort_tmp)) return -1;
+    uint64_t groups = 0;
+    VG_TRY(hipMemcpyAsync(&groups, ioff + G.NImax, 8, hipMemcpyDeviceToHost, st));
+    VG_TRY(hipStreamSynchronize(st));
+    if (gb_ensure_capacity(g, (int64_t)groups + 2)) return -1;
+    k_gbp_flush<<<(unsigned)G.NImax, GB_THREADS, 0, st>>>(g->L, G.rec, G.RW, G.icount, G.irec, G.n_rows,
+                                                          g->hot.as<uint64_t>(), g->cold.as<uint64_t>(), g->cap,
+                                                          g->ctl.as<uint64_t>());
+    VG_LAUNCH_CHECK();
+    g->run = 0;
+    return 0;
+}
+
+// sort pairs of a run's records (the counterpart of gb_compact)
+static int gbp_collect(vgpu_groupby *g, const RecSrc &R) {
+    const GbpArgs &G = g->gbp;
+    hipStream_t st = vg_stream();
+    int64_t NI = G.NImax;
+    if (g->sort_keys.reserve((size_t)(G.n_rows + 2) * 8) || g->sort_vals.reserve((size_t)(G.n_rows + 2) * 4) ||
+        g->cmp.reserve((size_t)NI * 24 + 256))
+        return -1;
+    uint64_t *ioff = g->cmp.as<uint64_t>();
+    uint64_t *por = ioff + NI, *pand = por + NI;
+    uint64_t *ctl = g->ctl.as<uint64_t>();
+    if (vg_scan_u32_to_u64(G.icount, ioff, NI, ctl + 5, g->sort_tmp)) return -1;
+    k_gbp_collect<<<(unsigned)NI, GB_THREADS, 0, st>>>(g->L, R, g->order, G.icount, G.irec, ioff,
+                                                       g->sort_keys.as<uint64_t>(), g->sort_vals.as<uint32_t>(), por,
+                                                       pand);
+    VG_LAUNCH_CHECK();
+    k_gb_or_and_final<<<1, GB_THREADS, 0, st>>>(por, pand, NI, ctl);
+    VG_LAUNCH_CHECK();
+    k_gbp_reserved<<<1, 64, 0, st>>>(g->L, R, g->order, g->sort_keys.as<uint64_t>(), g->sort_vals.as<uint32_t>(),
+                                     ctl);
+    VG_LAUNCH_CHECK();
+    return 0;
+}
+
 // LDS slots for privatization: the table must hold the expected groups at
 // load <= 1/2 within the LDS budget
 static int gb_lds_slots(const vgpu_groupby *g, size_t vm_bytes) {
@@ -1155,17 +1934,21 @@ extern "C" int vgpu_groupby_feed(vgpu_groupby *g, const vgpu_col *key, const vgp
             vg_set_error("groupby: input %d type %d differs from declared %d", i, inputs[i].type, g->L.in_type[i]);
             return -1;
         }
+    if (g->run && gbp_flush(g)) return -1;
     // groups created by this feed are bounded by its rows, by the hint, or
     // (a large feed without a hint) by its sketched key count
     int64_t incoming = n_logical;
+    int64_t groups = g->hint;
     if (g->hint > 0 && incoming > g->hint) incoming = g->hint;
     if (g->hint <= 0 && !g->L.key_none && n_logical >= GB_SK_MIN_ROWS) {
         int64_t est;
         if (gb_sketch(g, key, sel, n_logical, &est)) return -1;
         g->est_groups = std::max(g->est_groups, est);
         incoming = std::min(incoming, est + est / 4 + 4096);
+        groups = est;
     }
-    if (gb_ensure_capacity(g, incoming)) return -1;
+    const bool partitioned = gbp_wanted(g, n_logical, groups);
+    if (!partitioned && gb_ensure_capacity(g, incoming)) return -1;
 
     FeedArgs A;
     memset(&A, 0, sizeof A);
@@ -1190,6 +1973,7 @@ extern "C" int vgpu_groupby_feed(vgpu_groupby *g, const vgpu_col *key, const vgp
     size_t vm = (pi.present && !pi.fast) ? vg_vm_lds_bytes(pred, GB_THREADS) : 0;
     vm = (vm + 15) & ~(size_t)15;
     A.vm_lds = (int32_t)vm;
+    if (partitioned) return gbp_feed(g, A, ins, pi, prog, pcs, groups);
     A.lds_slots = gb_lds_slots(g, vm);
     size_t lds = vm + (A.lds_slots ? (size_t)(A.lds_slots + 2) * (g->L.RH + g->L.RC) * 8 : 0);
     vg_kt_begin();
@@ -1276,6 +2060,7 @@ extern "C" int vgpu_groupby_feed_join(vgpu_groupby *g, const vgpu_feed_spec *sp)
         vg_set_error("groupby: feed after finish");
         return -1;
     }
+    if (g->run && gbp_flush(g)) return -1;
     int64_t n_logical = sp->sel ? sp->n_sel : sp->n_rows;
     if (n_logical <= 0) return 0;
     if (sp->n_probe_cols < 0 || sp->n_probe_cols > VGPU_MAX_COLS || sp->n_build_cols < 0 ||
@@ -1419,6 +2204,39 @@ extern "C" int vgpu_groupby_feed_join(vgpu_groupby *g, const vgpu_feed_spec *sp)
     return 0;
 }
 
+static RecSrc gb_table_src(const vgpu_groupby *g) {
+    RecSrc R;
+    R.hot = g->hot.as<uint64_t>();
+    R.cold = g->cold.as<uint64_t>();
+    R.hstride = g->L.RH;
+    R.cstride = g->L.RC;
+    R.na_slot = g->cap;
+    R.sp_slot = g->cap + 1;
+    return R;
+}
+
+// (sort key, slot) pairs of every sorted record of the table into
+// sort_keys / sort_vals; count -> ctl[5], OR / AND of the keys -> ctl[6] /
+// ctl[7], NA rows -> ctl[8]
+static int gb_compact(vgpu_groupby *g, const RecSrc &R, int64_t n_slots) {
+    hipStream_t s = vg_stream();
+    int64_t nb = (n_slots + GB_CMP_SLOTS - 1) / GB_CMP_SLOTS;
+    if (g->cmp.reserve((size_t)nb * 32)) return -1;
+    uint64_t *boff = g->cmp.as<uint64_t>();
+    uint64_t *por = boff + nb, *pand = por + nb;
+    uint32_t *bcount = reinterpret_cast<uint32_t *>(pand + nb);
+    uint64_t *ctl = g->ctl.as<uint64_t>();
+    k_gb_cmp_count<<<(unsigned)nb, GB_THREADS, 0, s>>>(g->L, R, n_slots, g->order, bcount);
+    VG_LAUNCH_CHECK();
+    if (vg_scan_u32_to_u64(bcount, boff, nb, ctl + 5, g->sort_tmp)) return -1;
+    k_gb_cmp_write<<<(unsigned)nb, GB_THREADS, 0, s>>>(g->L, R, n_slots, g->order, boff, g->sort_keys.as<uint64_t>(),
+                                                       g->sort_vals.as<uint32_t>(), por, pand, ctl);
+    VG_LAUNCH_CHECK();
+    k_gb_or_and_final<<<1, GB_THREADS, 0, s>>>(por, pand, nb, ctl);
+    VG_LAUNCH_CHECK();
+    return 0;
+}
+
 static int highest_bit(uint64_t x) { return x ? 63 - __builtin_clzll(x) : -1; }
 static int lowest_bit(uint64_t x) { return x ? __builtin_ctzll(x) : 64; }
 
@@ -1426,17 +2244,29 @@ extern "C" int64_t vgpu_groupby_finish(vgpu_groupby *g) {
     if (g->finished) return g->n_groups;
     hipStream_t s = vg_stream();
     if (g->cap == 0 && gb_alloc_table(g, 64)) return -1;
-    int64_t slots = g->cap + 2;
-    if (g->sort_keys.reserve((size_t)slots * 8) || g->sort_vals.reserve((size_t)slots * 4)) return -1;
-    k_gb_ctl_init<<<1, 64, 0, s>>>(g->ctl.as<uint64_t>(), 1);
-    k_gb_compact<<<vg_blocks(slots, 256), 256, 0, s>>>(g->L, g->hot.as<uint64_t>(), g->cap, g->order,
-                                                        g->ctl.as<uint64_t>(), g->sort_keys.as<uint64_t>(),
-                                                        g->sort_vals.as<uint32_t>());
-    VG_LAUNCH_CHECK();
-    // the one synchronization of finish: group count, sort-key bit range,
-    // overflow flag and the NA group's size
+    // the one synchronization of finish (in the common case): group count,
+    // sort-key bit range, overflow flag and the NA group's size
     uint64_t ctl[GB_CTL_WORDS];
-    if (gb_read_ctl(g, ctl)) return -1;
+    RecSrc R;
+    bool from_table = true;
+    if (g->run) {
+        // records of a partitioned feed sort directly, unless some records
+        // went to the hash table; then the run is folded into it first
+        R = gbp_run_src(g);
+        k_gb_ctl_init<<<1, 64, 0, s>>>(g->ctl.as<uint64_t>(), 1);
+        VG_LAUNCH_CHECK();
+        if (gbp_collect(g, R) || gb_read_ctl(g, ctl)) return -1;
+        from_table = ctl[0] != 0 || ctl[9] != 0;
+        if (from_table && gbp_flush(g)) return -1;
+    }
+    if (from_table) {
+        int64_t slots = g->cap + 2;
+        if (g->sort_keys.reserve((size_t)slots * 8) || g->sort_vals.reserve((size_t)slots * 4)) return -1;
+        R = gb_table_src(g);
+        k_gb_ctl_init<<<1, 64, 0, s>>>(g->ctl.as<uint64_t>(), 1);
+        VG_LAUNCH_CHECK();
+        if (gb_compact(g, R, slots) || gb_read_ctl(g, ctl)) return -1;
+    }
     if (ctl[1]) {
         vg_set_error("groupby: hash table overflow (more groups than the table was sized for)");
         return -1;
@@ -1463,13 +2293,13 @@ extern "C" int64_t vgpu_groupby_finish(vgpu_groupby *g) {
         O.valid[a] = g->out_valid[a].as<uint64_t>();
     }
     if (n_out > 0) {
-        k_gb_emit<<<vg_blocks(n_out, 256), 256, 0, s>>>(g->L, g->aggs, g->hot.as<uint64_t>(), g->cold.as<uint64_t>(),
-                                                         g->cap, g->sort_vals.as<uint32_t>(), n_sorted, na_last, n_out,
-                                                         8, O, g->ctl.as<uint64_t>());
+        k_gb_emit<<<vg_blocks(n_out, 256), 256, 0, s>>>(g->L, g->aggs, R, g->sort_vals.as<uint32_t>(), n_sorted,
+                                                         na_last, n_out, 8, O, g->ctl.as<uint64_t>());
         VG_LAUNCH_CHECK();
     }
     g->n_groups = n_out;
     g->finished = 1;
+    g->run = 0;
     return n_out;
 }
 
@@ -1526,6 +2356,11 @@ extern "C" void vgpu_groupby_destroy(vgpu_groupby *g) {
     g->sort_keys.release();
     g->sort_vals.release();
     g->sort_tmp.release();
+    g->cmp.release();
+    g->gbp_s1.release();
+    g->gbp_s2.release();
+    g->gbp_aux.release();
+    g->run_rec.release();
     g->out_keys.release();
     g->out_kvalid.release();
     for (auto &b : g->out_vals) b.release();

@@ -186,8 +186,8 @@ extern "C" double vgpu_timer_stop(void) {
 // launch-shape knobs
 static const char *g_tune_names[VG_TUNE_COUNT] = {"gb.lds.max_blocks", "gb.lds.rows_per_block", "gb.glb.max_blocks",
                                                   "gb.glb.rows_per_block", "gb.lds.max_slots", "gb.lds.budget",
-                                                  "gb.star.items"};
-static int64_t g_tune[VG_TUNE_COUNT] = {1024, 8192, 16384, 1024, 4096, 65536, 4};
+                                                  "gb.star.items", "gb.part.min_groups"};
+static int64_t g_tune[VG_TUNE_COUNT] = {1024, 8192, 16384, 1024, 4096, 65536, 4, 65536};
 
 int64_t vg_tune(int knob) { return g_tune[knob]; }
 
