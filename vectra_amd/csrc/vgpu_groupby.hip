@@ -890,19 +890,20 @@ k_gb_cmp_write(GbLayout L, RecSrc R, int64_t n_slots, int order, const uint64_t 
     }
 }
 
-// final OR / AND of the per-block partials -> ctl[6] / ctl[7]
+// OR / AND of the per-block partials into ctl[6] / ctl[7] (which start at
+// 0 / all-ones): a block-strided reduction, one atomic pair per block
 __global__ void __launch_bounds__(GB_THREADS)
 k_gb_or_and_final(const uint64_t *por, const uint64_t *pand, int64_t nb, uint64_t *ctl) {
     __shared__ uint64_t lor[2 * GB_THREADS / VG_WAVE];
     uint64_t o = 0, a = ~0ull;
-    for (int64_t i = threadIdx.x; i < nb; i += GB_THREADS) {
+    for (int64_t i = (int64_t)blockIdx.x * GB_THREADS + threadIdx.x; i < nb; i += (int64_t)gridDim.x * GB_THREADS) {
         o |= por[i];
         a &= pand[i];
     }
     gb_block_or_and(o, a, lor);
     if (threadIdx.x == 0) {
-        ctl[6] = o;
-        ctl[7] = a;
+        if (o) atomicOr((unsigned long long *)&ctl[6], (unsigned long long)o);
+        if (~a) atomicAnd((unsigned long long *)&ctl[7], (unsigned long long)a);
     }
 }
 
@@ -953,14 +954,41 @@ __device__ __forceinline__ void gb_finish_agg(const GbLayout &L, const vgpu_agg 
     }
 }
 
-__global__ void k_gb_emit(GbLayout L, GbAggs AG, RecSrc R, const uint32_t *order_slots, int64_t n_sorted,
-                          int na_last, int64_t n_out, int key_width, EmitOut O, uint64_t *ctl) {
+// Records whose words fit GB_EMIT_STAGE_WORDS are staged through LDS: the
+// block copies its records word by word (consecutive lanes, consecutive
+// words of one record) before each thread finishes its group, instead of
+// every lane gathering the words of its own record.
+#define GB_EMIT_STAGE_WORDS 24
+
+template <bool STAGED>
+__global__ void __launch_bounds__(GB_THREADS)
+k_gb_emit(GbLayout L, GbAggs AG, RecSrc R, const uint32_t *order_slots, int64_t n_sorted, int na_last, int64_t n_out,
+          int key_width, EmitOut O, uint64_t *ctl) {
+    extern __shared__ uint64_t dynw[];
+    __shared__ int64_t sidx[GB_THREADS];
     int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool in = g < n_out;
     int64_t s = 0;
     if (in) s = (na_last && g == n_sorted) ? R.na_slot : (int64_t)order_slots[g];
-    const uint64_t *h = R.hot + s * R.hstride;
-    const uint64_t *c = R.cold + s * R.cstride;
+    const uint64_t *h, *c;
+    if (STAGED) {
+        const int RW = L.RH + L.RC;
+        sidx[threadIdx.x] = s;
+        __syncthreads();
+        const int64_t g0 = (int64_t)blockIdx.x * blockDim.x;
+        const int cnt = n_out - g0 < GB_THREADS ? (int)(n_out - g0) : GB_THREADS;
+        for (int q = threadIdx.x; q < cnt * RW; q += GB_THREADS) {
+            int k = q / RW, w = q - k * RW;
+            int64_t sk = sidx[k];
+            dynw[q] = w < L.RH ? R.hot[sk * R.hstride + w] : R.cold[sk * R.cstride + (w - L.RH)];
+        }
+        __syncthreads();
+        h = dynw + threadIdx.x * RW;
+        c = h + L.RH;
+    } else {
+        h = R.hot + s * R.hstride;
+        c = R.cold + s * R.cstride;
+    }
     bool kvalid = in && s != R.na_slot;
     if (in && !L.key_none) {
         uint64_t kb = 0;
@@ -1277,6 +1305,17 @@ extern "C" vgpu_groupby *vgpu_groupby_create(int key_type, int n_inputs, const i
     return g;
 }
 
+// row copies with 16-byte accesses when rows have an even number of words
+// (row starts are then 16-byte aligned)
+__device__ __forceinline__ void gbp_copy_row(uint64_t *d, const uint64_t *s, int W) {
+    if ((W & 1) == 0) {
+        for (int k = 0; k < W; k += 2)
+            *reinterpret_cast<ulonglong2 *>(d + k) = *reinterpret_cast<const ulonglong2 *>(s + k);
+    } else {
+        for (int k = 0; k < W; k++) d[k] = s[k];
+    }
+}
+
 __device__ __forceinline__ uint64_t gbp_hash(int cls, uint64_t kn) {
     if (cls == VG_KEY_NA) return 0x9e3779b97f4a7c15ull;       // every NA row in one partition
     if (cls == VG_KEY_SPECIAL) return 0x7f4a7c159e3779b9ull;  // every EMPTY-valued key row in one partition
@@ -1361,15 +1400,28 @@ k_gbp_scatter1(GbLayout L, FeedArgs A, GbpArgs G, InSet ins, PredInfo pi, vgpu_p
         }
         if (!ok) continue;
         uint64_t dst = atomicAdd(&cur[gbp_hash(cls, kn) >> (64 - G.B1)], 1ull);
-        uint64_t *w = G.S1 + dst * G.W;
+        uint64_t w[VGPU_MAX_INPUTS + 2];
         uint64_t meta = (uint64_t)(uint32_t)li | ((uint64_t)cls << 32);
         w[0] = kn;
-        for (int i = 0; i < L.n_inputs; i++) {
-            bool v = vg_valid(ins.c[i].validity, row);
-            w[1 + i] = v ? vg_load_bits(ins.c[i], row) : 0;
-            meta |= (uint64_t)v << (34 + i);
+#pragma unroll
+        for (int i = 0; i < VGPU_MAX_INPUTS; i++) {
+            if (i < L.n_inputs) {
+                bool v = vg_valid(ins.c[i].validity, row);
+                w[1 + i] = v ? vg_load_bits(ins.c[i], row) : 0;
+                meta |= (uint64_t)v << (34 + i);
+            }
         }
-        w[G.W - 1] = meta;
+        uint64_t *d = G.S1 + dst * G.W;
+        if (G.W == 4) {
+            // the common two-input row: two 16-byte stores
+            reinterpret_cast<ulonglong2 *>(d)[0] = make_ulonglong2(w[0], w[1]);
+            reinterpret_cast<ulonglong2 *>(d)[1] = make_ulonglong2(w[2], meta);
+        } else {
+#pragma unroll
+            for (int k = 0; k < VGPU_MAX_INPUTS + 1; k++)
+                if (k < G.W - 1) d[k] = w[k];
+            d[G.W - 1] = meta;
+        }
     }
 }
 
@@ -1454,8 +1506,7 @@ __global__ void __launch_bounds__(GB_THREADS) k_gbp_scatter2(GbpArgs G) {
     for (int64_t r = r0 + threadIdx.x; r < r1; r += GB_THREADS) {
         const uint64_t *w = G.S1 + r * G.W;
         uint64_t dst = atomicAdd(&cur[gbp_sub(G, w)], 1ull);
-        uint64_t *d = G.S2 + dst * G.W;
-        for (int k = 0; k < G.W; k++) d[k] = w[k];
+        gbp_copy_row(G.S2 + dst * G.W, w, G.W);
     }
 }
 
@@ -1525,11 +1576,13 @@ __device__ __forceinline__ void gbp_copy_record(const GbLayout &L, const uint64_
     for (int w = 0; w < L.RC; w++) dst[L.RH + w] = c[w];
 }
 
-// LDS slot of a staged or partial record's key, or -1 (window exhausted)
+// LDS slot of a staged or partial record's key, or -1 (table full): a
+// partition holds a bounded number of groups, so probing may scan the
+// whole table
 __device__ __forceinline__ int64_t gbp_lds_slot(uint64_t *lhot, int S, int RH, int cls, uint64_t kn, uint64_t empty) {
     if (cls == VG_KEY_NA) return S;
     if (cls == VG_KEY_SPECIAL) return S + 1;
-    return gb_slot(lhot, S, RH, kn, empty, GB_LDS_PROBES, nullptr);
+    return gb_slot(lhot, S, RH, kn, empty, S, nullptr);
 }
 
 __global__ void __launch_bounds__(GB_THREADS) k_gbp_agg(GbLayout L, GbpArgs G) {
@@ -1877,7 +1930,7 @@ static int gbp_collect(vgpu_groupby *g, const RecSrc &R) {
                                                        g->sort_keys.as<uint64_t>(), g->sort_vals.as<uint32_t>(), por,
                                                        pand);
     VG_LAUNCH_CHECK();
-    k_gb_or_and_final<<<1, GB_THREADS, 0, st>>>(por, pand, NI, ctl);
+    k_gb_or_and_final<<<vg_blocks(NI, 4 * GB_THREADS, 256), GB_THREADS, 0, st>>>(por, pand, NI, ctl);
     VG_LAUNCH_CHECK();
     k_gbp_reserved<<<1, 64, 0, st>>>(g->L, R, g->order, g->sort_keys.as<uint64_t>(), g->sort_vals.as<uint32_t>(),
                                      ctl);
@@ -2232,7 +2285,7 @@ static int gb_compact(vgpu_groupby *g, const RecSrc &R, int64_t n_slots) {
     k_gb_cmp_write<<<(unsigned)nb, GB_THREADS, 0, s>>>(g->L, R, n_slots, g->order, boff, g->sort_keys.as<uint64_t>(),
                                                        g->sort_vals.as<uint32_t>(), por, pand, ctl);
     VG_LAUNCH_CHECK();
-    k_gb_or_and_final<<<1, GB_THREADS, 0, s>>>(por, pand, nb, ctl);
+    k_gb_or_and_final<<<vg_blocks(nb, 4 * GB_THREADS, 256), GB_THREADS, 0, s>>>(por, pand, nb, ctl);
     VG_LAUNCH_CHECK();
     return 0;
 }
@@ -2293,8 +2346,13 @@ extern "C" int64_t vgpu_groupby_finish(vgpu_groupby *g) {
         O.valid[a] = g->out_valid[a].as<uint64_t>();
     }
     if (n_out > 0) {
-        k_gb_emit<<<vg_blocks(n_out, 256), 256, 0, s>>>(g->L, g->aggs, R, g->sort_vals.as<uint32_t>(), n_sorted,
-                                                         na_last, n_out, 8, O, g->ctl.as<uint64_t>());
+        const int RW = g->L.RH + g->L.RC;
+        if (RW <= GB_EMIT_STAGE_WORDS)
+            k_gb_emit<true><<<vg_blocks(n_out, GB_THREADS), GB_THREADS, (size_t)GB_THREADS * RW * 8, s>>>(
+                g->L, g->aggs, R, g->sort_vals.as<uint32_t>(), n_sorted, na_last, n_out, 8, O, g->ctl.as<uint64_t>());
+        else
+            k_gb_emit<false><<<vg_blocks(n_out, GB_THREADS), GB_THREADS, 0, s>>>(
+                g->L, g->aggs, R, g->sort_vals.as<uint32_t>(), n_sorted, na_last, n_out, 8, O, g->ctl.as<uint64_t>());
         VG_LAUNCH_CHECK();
     }
     g->n_groups = n_out;

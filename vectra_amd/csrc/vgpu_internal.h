@@ -102,7 +102,9 @@ int vg_scan_u64(const uint64_t *in, uint64_t *out, int64_t n, uint64_t *total,
 int vg_sort_pairs(uint64_t *keys, uint32_t *vals, int64_t n, int begin_bit,
                   int end_bit, DevBuf &tmp);
 
-static inline unsigned vg_blocks(int64_t n, int64_t per_block, int64_t cap = 1 << 20) {
+// blocks for n items at per_block each, at most cap (kernels with one thread
+// per item rely on the default, which covers any grid the hardware takes)
+static inline unsigned vg_blocks(int64_t n, int64_t per_block, int64_t cap = 0x7FFFFFFF) {
     int64_t b = (n + per_block - 1) / per_block;
     if (b < 1) b = 1;
     if (b > cap) b = cap;
