@@ -273,6 +273,51 @@ int vgpu_groupby_feed_join(vgpu_groupby *g, const vgpu_feed_spec *spec);
 int vgpu_sort_pairs_u64(uint64_t *keys, uint32_t *values, int64_t n,
                         int begin_bit, int end_bit);
 
+/* ---- batched queries ----------------------------------------------------
+ * Many independent filter -> group-by queries of one shape run as one
+ * batch: per query q, the rows of inputs[] (and pred_cols[]) where the
+ * shared predicate holds -- with the immediate of instruction param_insn
+ * replaced by param's bits -- grouped by key and aggregated as
+ * vgpu_groupby would (same aggregates, order, NA / NaN rules).  Each
+ * query is the reference's tbl |> filter() |> group_by() |> summarise()
+ * (src/filter.c:13-84, src/group_agg.c:591-672).
+ *
+ *   vgpu_batch_plan     sizes every query's table from a device sketch of
+ *                       its keys (one synchronization), lays out the work
+ *                       items and runs the batch once; the columns must
+ *                       stay in place until the batch is destroyed
+ *   vgpu_batch_run      enqueues the batch again on the same columns:
+ *                       device work only (one persistent work-queue feed
+ *                       launch, one finish for all queries)
+ *   vgpu_batch_capture  records that device work as a hipGraph; later
+ *                       runs replay it
+ *   vgpu_batch_counts   groups and output offset of every query
+ *                       (synchronizes); fails if a run's keys left the
+ *                       sort range found at plan (columns changed)
+ *   vgpu_batch_fetch    query q's result, like vgpu_groupby_fetch        */
+typedef struct vgpu_batch vgpu_batch;
+
+typedef struct {
+    vgpu_col        key;        /* ignored for VGPU_KEY_NONE              */
+    const vgpu_col *inputs;     /* n_inputs columns                       */
+    const vgpu_col *pred_cols;  /* n_pred_cols predicate column slots     */
+    int64_t         n_rows;
+    uint64_t        param;      /* bits of the per-query constant         */
+} vgpu_query;
+
+vgpu_batch *vgpu_batch_create(int key_type, int n_inputs,
+                              const int32_t *input_types, int n_aggs,
+                              const vgpu_agg *aggs, int order,
+                              const vgpu_prog *pred, int n_pred_cols,
+                              int param_insn);
+int     vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_queries);
+int     vgpu_batch_run(vgpu_batch *b);
+int     vgpu_batch_capture(vgpu_batch *b);
+int     vgpu_batch_counts(vgpu_batch *b, int64_t *n_groups, int64_t *offsets);
+int64_t vgpu_batch_fetch(vgpu_batch *b, int q, void *keys, uint8_t *key_validity,
+                         double *const *agg_values, uint8_t *const *agg_validity);
+void    vgpu_batch_destroy(vgpu_batch *b);
+
 /* ---- tuning -----------------------------------------------------------
  * Launch-shape knobs the autotuner sets (defaults in parentheses):
  *   "gb.lds.max_blocks" (1024)  "gb.lds.rows_per_block" (8192)

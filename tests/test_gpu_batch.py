@@ -1,0 +1,200 @@
+"""Batched filter -> group-by queries (vgpu_batch_*) vs the oracle, query by
+query: each query must equal tbl |> filter(x > t_q) |> group_by(g) |>
+summarise(...) run alone through the reference's nodes (src/filter.c,
+src/group_agg.c, src/agg_ops.c).  Keys, order, counts and min/max exact;
+double sums and means within 1e-12 * sum|x| of the group."""
+import numpy as np
+import pytest
+
+from helpers import assert_tables_equal, dbl_with_specials, rng, with_na
+
+pytestmark = pytest.mark.gpu
+
+AGGS = [("n", "n", None, 0), ("cx", "count", "x", 0), ("sx", "sum", "x", 1), ("mx", "mean", "x", 1),
+        ("lo", "min", "x", 0), ("hi", "max", "x", 1), ("s0", "sum", "x", 0)]
+CLOSE = {"sx", "mx", "s0"}
+SIZES = [0, 1, 50, 1000, 32767, 32768, 32769, 100_000, 250_000]
+CARDS = [1, 3, 40, 500, 3000, 20000]
+
+
+def make_queries(seed, n_queries, key_kind="int"):
+    r = rng(seed)
+    out = []
+    for q in range(n_queries):
+        n = SIZES[q] if q < len(SIZES) else int(10 ** r.uniform(0, 5.3))
+        card = CARDS[q % len(CARDS)]
+        if key_kind == "int":
+            kv = r.integers(0, card, n) * 7919 - 5000
+            kv[r.random(n) < 0.01] = np.iinfo(np.int64).min
+            key = (kv, with_na(r, n, 0.02))
+        else:
+            kv = np.round(r.normal(size=n) * card) / 4.0
+            u = r.random(n)
+            kv[u < 0.01] = np.nan
+            z = (u > 0.5) & (u < 0.52)
+            kv[z] = np.where(r.random(z.sum()) < 0.5, 0.0, -0.0)
+            key = (kv, with_na(r, n, 0.02))
+        x = dbl_with_specials(r, n, 0.01, 0.02)
+        out.append({"g": key, "x": (x, with_na(r, n, 0.02)), "t": float(r.uniform(-2.0, 1.0)),
+                    "ax": np.abs(np.nan_to_num(x, nan=0.0))})
+    return out
+
+
+def oracle_query(nv, qd, temp_dir, keys=True, pred=None):
+    cols = {"x": qd["x"], "ax": qd["ax"]}
+    if keys:
+        cols["g"] = qd["g"]
+    t = nv.Table(cols, rg_rows=0)
+    node = nv.filter_node("oracle", t.scan(), pred or ("cmp", ">", ("col", "x"), ("dbl", qd["t"])))
+    aggs = list(AGGS) + [("__sa", "sum", "ax", 1), ("__cnt", "count", "x", 0)]
+    out = nv.collect(nv.group_agg_node("oracle", node, ["g"] if keys else [], aggs, temp_dir))
+    sa = out.pop("__sa")[0]
+    cnt = out.pop("__cnt")[0]
+    close = {name: (sa if kind == "sum" else sa / np.maximum(cnt, 1)) for name, kind, _, _ in AGGS if name in CLOSE}
+    return out, close
+
+
+def batch_aggs():
+    names = {"n": "n", "count": "count", "sum": "sum", "mean": "mean", "min": "min", "max": "max"}
+    return [(names[kind], -1 if col is None else 0, rm) for _, kind, col, rm in AGGS]
+
+
+def renamed(got, keys=True):
+    out = {}
+    if keys:
+        out["g"] = got["key"]
+    for i, (name, _, _, _) in enumerate(AGGS):
+        out[name] = got[f"a{i}"]
+    return out
+
+
+def device_queries(nv, qs, key_type, keys=True):
+    cols, queries = [], []
+    for qd in qs:
+        n = len(qd["x"][0])
+        x = nv.DeviceColumn(*qd["x"])
+        k = nv.DeviceColumn(*qd["g"]) if keys else None
+        cols += [x] + ([k] if k else [])
+        queries.append({"key": k, "inputs": [x], "pred_cols": [x], "n_rows": n, "param": qd["t"]})
+    return cols, queries
+
+
+def run_batch(nv, qs, order, key_type=0, keys=True):
+    pred = nv.cmp_const_prog(0, nv.VEC_DOUBLE, "GT", 0.0)   # x > <param>, the constant is instruction 1
+    b = nv.Batch(key_type if keys else nv.VGPU_KEY_NONE, [nv.VEC_DOUBLE], batch_aggs(), order, pred, 1, 1)
+    cols, queries = device_queries(nv, qs, key_type, keys)
+    b.plan(queries)
+    return b, cols
+
+
+@pytest.mark.parametrize("temp_dir", [None, "/tmp"])
+def test_batch_matches_oracle(gpu, oracle, temp_dir):
+    qs = make_queries(5, 40)
+    order = gpu.VGPU_ORDER_KEY if temp_dir else gpu.VGPU_ORDER_FIRST_SEEN
+    b, cols = run_batch(gpu, qs, order)
+    n_groups, offs = b.counts()
+    assert np.all(offs[1:] == np.cumsum(n_groups)[:-1])
+    for q, qd in enumerate(qs):
+        exp, close = oracle_query(gpu, qd, temp_dir)
+        assert_tables_equal(renamed(b.fetch(q)), exp, close)
+    b.free()
+    for c in cols:
+        c.free()
+
+
+def test_batch_double_keys(gpu, oracle):
+    """NaN keys (one group per row), signed zeros, first-seen order"""
+    qs = make_queries(7, 16, key_kind="dbl")
+    b, cols = run_batch(gpu, qs, gpu.VGPU_ORDER_FIRST_SEEN, key_type=gpu.VEC_DOUBLE)
+    for q, qd in enumerate(qs):
+        exp, close = oracle_query(gpu, qd, None)
+        assert_tables_equal(renamed(b.fetch(q)), exp, close)
+    b.free()
+    for c in cols:
+        c.free()
+
+
+def test_batch_ungrouped(gpu, oracle):
+    qs = make_queries(9, 12)
+    b, cols = run_batch(gpu, qs, gpu.VGPU_ORDER_KEY, keys=False)
+    for q, qd in enumerate(qs):
+        exp, close = oracle_query(gpu, qd, "/tmp", keys=False)
+        assert_tables_equal(renamed(b.fetch(q), keys=False), exp, close)
+    b.free()
+    for c in cols:
+        c.free()
+
+
+def test_batch_vm_predicate(gpu, oracle):
+    """a predicate the VM evaluates: (x > t_q) & (g != -5000), t_q per query"""
+    qs = make_queries(11, 20)
+    ins = [("COL", gpu.VEC_DOUBLE, 0, 0, 0, 0), ("CONST", gpu.VEC_DOUBLE, 0, 0, 0, 0.0),
+           ("GT", gpu.VEC_DOUBLE, 0, 1, 0, 0), ("COL", gpu.VEC_INT64, 0, 0, 1, 0),
+           ("CONST", gpu.VEC_INT64, 0, 0, 0, -5000), ("NE", gpu.VEC_INT64, 3, 4, 0, 0),
+           ("AND", gpu.VEC_BOOL, 2, 5, 0, 0)]
+    pred = gpu.compile_prog(ins)
+    b = gpu.Batch(gpu.VEC_INT64, [gpu.VEC_DOUBLE], batch_aggs(), gpu.VGPU_ORDER_KEY, pred, 2, 1)
+    cols, queries = device_queries(gpu, qs, gpu.VEC_INT64)
+    for q, qd in zip(queries, qs):
+        q["pred_cols"] = [q["inputs"][0], q["key"]]
+    b.plan(queries)
+    for q, qd in enumerate(qs):
+        p = ("and", ("cmp", ">", ("col", "x"), ("dbl", qd["t"])), ("cmp", "!=", ("col", "g"), ("int", -5000)))
+        exp, close = oracle_query(gpu, qd, "/tmp", pred=p)
+        assert_tables_equal(renamed(b.fetch(q)), exp, close)
+    b.free()
+    for c in cols:
+        c.free()
+
+
+def test_batch_graph_replays(gpu, oracle):
+    """the captured graph replayed three times, the program's own eager work
+    in between: every replay matches the oracle"""
+    qs = make_queries(13, 24)
+    b, cols = run_batch(gpu, qs, gpu.VGPU_ORDER_KEY)
+    b.capture()
+    expected = [oracle_query(gpu, qd, "/tmp") for qd in qs]
+    for rep in range(3):
+        b.run()
+        gpu.vgpu().vgpu_synchronize()
+        scratch = gpu.DeviceColumn(np.arange(1000, dtype=np.int64))   # eager work between replays
+        scratch.free()
+        for q in (0, 5, 8, 17, 23):
+            exp, close = expected[q]
+            assert_tables_equal(renamed(b.fetch(q)), exp, close)
+    b.free()
+    for c in cols:
+        c.free()
+
+
+def test_batch_same_alone_and_in_the_mix(gpu, oracle):
+    """a query's result does not depend on the queries batched with it"""
+    qs = make_queries(17, 30)
+    b, cols = run_batch(gpu, qs, gpu.VGPU_ORDER_KEY)
+    for q in (3, 7, 29):
+        alone, cols1 = run_batch(gpu, [qs[q]], gpu.VGPU_ORDER_KEY)
+        exp, close = oracle_query(gpu, qs[q], "/tmp")
+        assert_tables_equal(renamed(b.fetch(q)), renamed(alone.fetch(0)), close)
+        assert_tables_equal(renamed(alone.fetch(0)), exp, close)
+        alone.free()
+        for c in cols1:
+            c.free()
+    b.free()
+    for c in cols:
+        c.free()
+
+
+def test_batch_flags_changed_inputs(gpu, oracle):
+    """keys rewritten after plan beyond the planned sort range: the run is
+    reported as invalid instead of returning misordered groups"""
+    qs = make_queries(19, 6)
+    b, cols = run_batch(gpu, qs, gpu.VGPU_ORDER_KEY)
+    key_col = cols[2 * 4 + 1]     # query 4's key column (x, g per query)
+    big = np.full(key_col.n, 2**61, np.int64)
+    gpu._vcheck(gpu.vgpu().vgpu_memcpy_h2d(key_col.data, big.ctypes.data, big.nbytes))
+    b.run()
+    with pytest.raises(gpu.VectraError, match="inputs changed since plan"):
+        b.counts()
+    b.free()
+    for c in cols:
+        c.free()

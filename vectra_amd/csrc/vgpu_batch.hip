@@ -1,0 +1,814 @@
+// vgpu_batch.hip -- many independent filter -> group-by queries of one
+// shape (key type, inputs, aggregates, order, predicate program with one
+// per-query constant) executed as one batch.
+//
+// Each query q is what vectra runs for
+//     tbl_q |> filter(<pred with constant param_q>) |> group_by(key) |>
+//     summarise(<aggs>)
+// (src/filter.c:13-84, src/group_agg.c:155-672, src/agg_ops.c:25-269), with
+// the same results as one vgpu_groupby per query.
+//
+// Design.  Planning sizes every query's hash table from a cardinality
+// sketch taken in one launch over all queries (one synchronization) and
+// lays the tables out back to back in one arena of group records (the
+// record model of vgpu_groupby_core.h).  A run is then pure device work,
+// capturable as a hipGraph:
+//   1. init      every table, every query's control words, the work queue;
+//   2. feed      ONE persistent launch: blocks pull (query, row tile) items
+//                from an atomic work queue, filter the tile, and aggregate
+//                it into an LDS-private table (queries with few groups) or
+//                straight into the query's HBM table;
+//   3. finish    compaction of every table into (sort key, slot) pairs,
+//                a stable radix sort by the order key and then by query, so
+//                that every query's groups are contiguous and in its order,
+//                and one emit launch for all groups of all queries.
+// The sort's bit range comes from the first run (inside plan); later runs
+// check on the device that their keys stay in that range and flag the
+// batch otherwise (vgpu_batch_counts reports it).
+#include "vgpu_groupby_core.h"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#define BT_TILE 32768              // rows per work item
+#define BT_SK_BITS 10              // sketch registers per query: 2^10
+#define BT_SK_REGS (1 << BT_SK_BITS)
+#define BT_LDS_BUDGET 61440        // dynamic LDS of the feed blocks
+#define BT_BLOCKS_PER_CU 4
+#define BT_CTL_ERR 9               // batch control word: inputs left the plan's key range
+
+struct BtQuery {
+    vgpu_col key;
+    InSet ins;
+    ColSet pcs;
+    PredInfo pi;
+    int64_t n_rows;
+    int64_t base;       // first arena slot of the query's table
+    int64_t cap;        // hashed slots (pow2); NA / EMPTY-valued key records at cap, cap + 1
+    int32_t lds_slots;  // 0 = aggregate straight into the HBM table
+    int32_t vm;         // predicate evaluated by the VM (program in progs[q])
+};
+
+struct BtItem {
+    uint32_t q;
+    uint32_t part;      // row tile (feed) or slot chunk (compaction)
+};
+
+struct BtArgs {
+    const BtQuery *qs;
+    const vgpu_prog *progs;
+    const BtItem *items;
+    int64_t n_items;
+    unsigned long long *queue;
+    uint64_t *hot, *cold;
+    uint64_t *ctl;      // GB_CTL_WORDS per query, then the batch's own GB_CTL_WORDS
+    int32_t vm_lds;
+    int32_t n_queries;
+};
+
+struct vgpu_batch {
+    GbLayout L;
+    GbAggs aggs;
+    int32_t order;
+    int32_t key_type;
+    vgpu_prog prog;
+    int32_t has_pred = 0;
+    int32_t n_pred_cols = 0;
+    int32_t param_insn = -1;
+    // plan
+    int32_t n_queries = 0;
+    int64_t n_slots = 0;            // arena slots
+    int64_t n_items = 0, n_cblocks = 0;
+    int32_t S_max = 0;
+    int32_t vm_lds = 0;
+    int64_t feed_grid = 1;          // persistent feed blocks: a few per CU
+    int32_t planned = 0;
+    int64_t n_pairs = 0;            // sorted records of the plan's run
+    int32_t lo = 0, hi = 0, qbits = 0;
+    uint64_t range_mask = 0;
+    DevBuf qs, progs, items, cmap, queue, hot, cold, ctl, qbase, qcount, qoff;
+    DevBuf bcount, boff, por, pand, skeys, svals, sort_tmp, scan_tmp;
+    DevBuf out_keys, out_kvalid;
+    std::vector<DevBuf> out_vals, out_valid;
+    std::vector<int64_t> h_count, h_off;
+    int32_t counts_valid = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+
+// per-query cardinality sketch (HyperLogLog registers, NaN-key rows): the
+// feed's items, one block per item at a time
+__global__ void __launch_bounds__(GB_THREADS)
+k_bt_sketch(BtArgs B, int key_none, uint32_t *regs, unsigned long long *nan_rows) {
+    __shared__ uint32_t r[BT_SK_REGS];
+    __shared__ unsigned long long nn;
+    for (int64_t it = blockIdx.x; it < B.n_items; it += gridDim.x) {
+        for (int i = threadIdx.x; i < BT_SK_REGS; i += GB_THREADS) r[i] = 0;
+        if (threadIdx.x == 0) nn = 0;
+        __syncthreads();
+        const BtItem item = B.items[it];
+        const BtQuery &Q = B.qs[item.q];
+        const int64_t r0 = (int64_t)item.part * BT_TILE;
+        const int64_t r1 = r0 + BT_TILE < Q.n_rows ? r0 + BT_TILE : Q.n_rows;
+        unsigned long long my_nan = 0;
+        if (!key_none)
+            for (int64_t row = r0 + threadIdx.x; row < r1; row += GB_THREADS) {
+                uint64_t kn;
+                int cls = vg_key_class(Q.key, row, kn);
+                if (cls == VG_KEY_NAN) { my_nan++; continue; }
+                if (cls != VG_KEY_REGULAR) continue;
+                uint64_t h = vg_mix64(kn);
+                uint32_t idx = (uint32_t)(h >> (64 - BT_SK_BITS));
+                uint64_t rest = (h << BT_SK_BITS) | (1ull << (BT_SK_BITS - 1));
+                uint32_t rank = (uint32_t)__builtin_clzll(rest) + 1;
+                if (r[idx] < rank) atomicMax(&r[idx], rank);
+            }
+        if (my_nan) atomicAdd(&nn, my_nan);
+        __syncthreads();
+        uint32_t *gr = regs + (size_t)item.q * BT_SK_REGS;
+        for (int i = threadIdx.x; i < BT_SK_REGS; i += GB_THREADS)
+            if (r[i]) atomicMax(&gr[i], r[i]);
+        if (threadIdx.x == 0 && nn) atomicAdd(&nan_rows[item.q], nn);
+        __syncthreads();
+    }
+}
+
+__global__ void k_bt_clear_u32(uint32_t *p, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// control words of every query and of the batch, per-query group counts,
+// the work queue
+__global__ void k_bt_init_ctl(uint64_t *ctl, int n_queries, uint64_t *qcount, unsigned long long *queue) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t words = (int64_t)(n_queries + 1) * GB_CTL_WORDS;
+    if (i < words) {
+        int w = (int)(i % GB_CTL_WORDS);
+        ctl[i] = (w == 2 || w == 3 || w == 7) ? ~0ull : 0ull;
+    }
+    if (i < n_queries) qcount[i] = 0;
+    if (i == 0) *queue = 0;
+}
+
+__global__ void __launch_bounds__(GB_THREADS) k_bt_feed(GbLayout L, BtArgs B) {
+    extern __shared__ uint64_t dynw[];
+    __shared__ long long s_item;
+    uint8_t *dyn = reinterpret_cast<uint8_t *>(dynw);
+    uint64_t *regs = dynw;
+    const uint64_t empty = L.key_is_double ? VG_EMPTY_F64 : VG_EMPTY_I64;
+    uint64_t *lhot = reinterpret_cast<uint64_t *>(dyn + B.vm_lds);
+    for (;;) {
+        if (threadIdx.x == 0) s_item = (long long)atomicAdd(B.queue, 1ull);
+        __syncthreads();
+        const int64_t it = s_item;
+        __syncthreads();
+        if (it >= B.n_items) break;
+        const BtItem item = B.items[it];
+        const BtQuery &Q = B.qs[item.q];
+        const vgpu_prog &prog = B.progs[Q.vm ? item.q : 0];
+        uint8_t *regv = dyn + (size_t)(Q.vm ? prog.n_insn : 0) * GB_THREADS * 8;
+        const int S = Q.lds_slots;
+        uint64_t *lcold = lhot + (size_t)(S + 2) * L.RH;
+        uint64_t *hot = B.hot + Q.base * L.RH;
+        uint64_t *cold = B.cold + Q.base * L.RC;
+        uint64_t *ctl = B.ctl + (size_t)item.q * GB_CTL_WORDS;
+        if (S) gb_lds_init(L, lhot, lcold, S, empty);
+        const int64_t r0 = (int64_t)item.part * BT_TILE;
+        const int64_t r1 = r0 + BT_TILE < Q.n_rows ? r0 + BT_TILE : Q.n_rows;
+        for (int64_t row = r0 + threadIdx.x; row < r1; row += GB_THREADS) {
+            if (!vg_pred_row(Q.pi, prog, Q.pcs, row, regs, regv, GB_THREADS, threadIdx.x)) continue;
+            uint64_t kn = 0;
+            int cls = VG_KEY_REGULAR;
+            if (!L.key_none) {
+                cls = vg_key_class(Q.key, row, kn);
+                if (L.key_is_double && cls == VG_KEY_REGULAR && kn == 0)
+                    gb_note_zero_key(ctl, vg_load_bits(Q.key, row), (uint64_t)row);
+            }
+            int64_t ls = S ? gb_lds_slot(lhot, S, L.RH, cls, kn, empty) : -1;
+            if (ls >= 0) {
+                gb_apply(L, Q.ins, row, (uint64_t)row, lhot + (size_t)ls * L.RH, lcold + (size_t)ls * L.RC);
+            } else {
+                int64_t gs = gb_global_slot(L, hot, Q.cap, cls, kn, (uint64_t)row, ctl);
+                if (gs >= 0) gb_apply(L, Q.ins, row, (uint64_t)row, hot + (size_t)gs * L.RH, cold + (size_t)gs * L.RC);
+            }
+        }
+        if (S) gb_lds_merge(L, lhot, lcold, S, empty, hot, cold, Q.cap, ctl);
+        __syncthreads();
+    }
+}
+
+// sort entry of arena record (query q, table slot s): like gb_sort_entry,
+// but the NA group of the key order takes the largest key instead of being
+// set aside, so a stable sort puts it after the query's other groups
+__device__ __forceinline__ bool bt_sort_entry(const GbLayout &L, const RecSrc &R, int order, int64_t s, uint64_t &sk) {
+    if (order == VGPU_ORDER_KEY && s == R.na_slot) {
+        if (R.hot[s * R.hstride + 1] == 0) return false;
+        sk = ~0ull;
+        return true;
+    }
+    return gb_sort_entry(L, R, order, s, sk);
+}
+
+__device__ __forceinline__ RecSrc bt_src(const GbLayout &L, const BtArgs &B, const BtQuery &Q) {
+    RecSrc R;
+    R.hot = B.hot + Q.base * L.RH;
+    R.cold = B.cold + Q.base * L.RC;
+    R.hstride = L.RH;
+    R.cstride = L.RC;
+    R.na_slot = Q.cap;
+    R.sp_slot = Q.cap + 1;
+    return R;
+}
+
+// compaction blocks: item = (query, chunk of GB_CMP_SLOTS table slots)
+__global__ void __launch_bounds__(GB_THREADS)
+k_bt_cmp_count(GbLayout L, BtArgs B, const BtItem *cmap, int order, uint32_t *bcount, uint64_t *qcount) {
+    __shared__ uint32_t lds[GB_THREADS / VG_WAVE];
+    const BtItem item = cmap[blockIdx.x];
+    const BtQuery &Q = B.qs[item.q];
+    const RecSrc R = bt_src(L, B, Q);
+    const int64_t base = (int64_t)item.part * GB_CMP_SLOTS, n_slots = Q.cap + 2;
+    uint32_t c = 0;
+    for (int j = 0; j < GB_CMP_PER_THREAD; j++) {
+        int64_t s = base + j * GB_THREADS + threadIdx.x;
+        uint64_t sk;
+        if (s < n_slots && bt_sort_entry(L, R, order, s, sk)) c++;
+    }
+    uint32_t total;
+    gb_block_prefix(c, lds, total);
+    if (threadIdx.x == 0) {
+        bcount[blockIdx.x] = total;
+        if (total) atomicAdd((unsigned long long *)&qcount[item.q], (unsigned long long)total);
+    }
+}
+
+__global__ void __launch_bounds__(GB_THREADS)
+k_bt_cmp_write(GbLayout L, BtArgs B, const BtItem *cmap, int order, const uint64_t *boff, uint64_t *skeys,
+               uint32_t *svals, uint64_t *por, uint64_t *pand) {
+    __shared__ uint32_t lds[GB_THREADS / VG_WAVE];
+    __shared__ uint64_t lor[2 * GB_THREADS / VG_WAVE];
+    const BtItem item = cmap[blockIdx.x];
+    const BtQuery &Q = B.qs[item.q];
+    const RecSrc R = bt_src(L, B, Q);
+    const int64_t base = (int64_t)item.part * GB_CMP_SLOTS, n_slots = Q.cap + 2;
+    uint64_t sk[GB_CMP_PER_THREAD];
+    uint32_t mask = 0;
+    uint64_t o = 0, a = ~0ull;
+#pragma unroll
+    for (int j = 0; j < GB_CMP_PER_THREAD; j++) {
+        int64_t s = base + j * GB_THREADS + threadIdx.x;
+        sk[j] = 0;
+        if (s < n_slots && bt_sort_entry(L, R, order, s, sk[j])) {
+            mask |= 1u << j;
+            o |= sk[j];
+            a &= sk[j];
+        }
+    }
+    uint32_t total;
+    uint64_t off = boff[blockIdx.x] + gb_block_prefix(__builtin_popcount(mask), lds, total);
+#pragma unroll
+    for (int j = 0; j < GB_CMP_PER_THREAD; j++)
+        if (mask & (1u << j)) {
+            skeys[off] = sk[j];
+            svals[off] = (uint32_t)(Q.base + base + j * GB_THREADS + threadIdx.x);
+            off++;
+        }
+    gb_block_or_and(o, a, lor);
+    if (threadIdx.x == 0) {
+        por[blockIdx.x] = o;
+        pand[blockIdx.x] = a;
+    }
+}
+
+// the run's keys must vary only within the plan's sort range, and the run
+// must sort as many records as the plan; otherwise the batch is flagged
+__global__ void k_bt_guard(uint64_t *bctl, uint64_t range_mask, const uint64_t *total, uint64_t n_pairs) {
+    uint64_t varying = bctl[6] ^ bctl[7];
+    if ((varying & ~range_mask) || *total != n_pairs) bctl[BT_CTL_ERR] = 1;
+}
+
+// query of every sorted record (the second sort's key)
+__global__ void k_bt_qkeys(const uint32_t *svals, int64_t n, const uint64_t *qbase, int n_queries, uint64_t *qk) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t s = svals[i];
+    int lo = 0, hi = n_queries - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (qbase[mid] <= s) lo = mid;
+        else hi = mid - 1;
+    }
+    qk[i] = (uint64_t)lo;
+}
+
+template <bool STAGED>
+__global__ void __launch_bounds__(GB_THREADS)
+k_bt_emit(GbLayout L, GbAggs AG, BtArgs B, const uint32_t *svals, const uint64_t *qk, int64_t n_out, EmitOut O) {
+    extern __shared__ uint64_t dynw[];
+    __shared__ int64_t sidx[GB_THREADS];
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool in = g < n_out;
+    int q = in ? (int)qk[g] : 0;
+    const BtQuery &Q = B.qs[q];
+    RecSrc R = bt_src(L, B, Q);
+    int64_t s = in ? (int64_t)svals[g] - Q.base : 0;   // slot within the query's table
+    const uint64_t *h, *c;
+    if (STAGED) {
+        RecSrc A;   // the whole arena: staging reads by arena slot
+        A.hot = B.hot;
+        A.cold = B.cold;
+        A.hstride = L.RH;
+        A.cstride = L.RC;
+        A.na_slot = A.sp_slot = -1;
+        sidx[threadIdx.x] = in ? (int64_t)svals[g] : 0;
+        __syncthreads();
+        const int64_t g0 = (int64_t)blockIdx.x * blockDim.x;
+        const int cnt = n_out - g0 < GB_THREADS ? (int)(n_out - g0) : GB_THREADS;
+        gb_stage_records(L, A, cnt, [&](int k) { return sidx[k]; }, dynw);
+        __syncthreads();
+        h = dynw + threadIdx.x * (L.RH + L.RC);
+        c = h + L.RH;
+    } else {
+        h = R.hot + s * R.hstride;
+        c = R.cold + s * R.cstride;
+    }
+    gb_emit_row(L, AG, R, g, in, s, h, c, n_out, 8, O, B.ctl + (size_t)q * GB_CTL_WORDS);
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+extern "C" vgpu_batch *vgpu_batch_create(int key_type, int n_inputs, const int32_t *input_types, int n_aggs,
+                                         const vgpu_agg *aggs, int order, const vgpu_prog *pred, int n_pred_cols,
+                                         int param_insn) {
+    if (vg_ensure_init()) return nullptr;
+    if (key_type != VGPU_INT64 && key_type != VGPU_DOUBLE && key_type != VGPU_BOOL && key_type != VGPU_KEY_NONE) {
+        vg_set_error("batch: unsupported key type %d", key_type);
+        return nullptr;
+    }
+    if (n_inputs < 0 || n_inputs > VGPU_MAX_INPUTS || n_aggs < 0 || n_aggs > VGPU_MAX_AGGS) {
+        vg_set_error("batch: %d inputs / %d aggregates exceed the limits (%d / %d)", n_inputs, n_aggs,
+                     VGPU_MAX_INPUTS, VGPU_MAX_AGGS);
+        return nullptr;
+    }
+    if (order != VGPU_ORDER_FIRST_SEEN && order != VGPU_ORDER_KEY) {
+        vg_set_error("batch: unknown order %d", order);
+        return nullptr;
+    }
+    if (pred && (pred->n_insn <= 0 || pred->n_insn > VGPU_MAX_INSN || n_pred_cols < 0 ||
+                 n_pred_cols > VGPU_MAX_COLS)) {
+        vg_set_error("batch: invalid predicate program");
+        return nullptr;
+    }
+    if (pred) {
+        for (int i = 0; i < pred->n_insn; i++)
+            if (pred->insn[i].op == VGPU_OP_COL && (pred->insn[i].col < 0 || pred->insn[i].col >= n_pred_cols)) {
+                vg_set_error("batch: predicate reads column slot %d of %d", pred->insn[i].col, n_pred_cols);
+                return nullptr;
+            }
+        if (param_insn >= 0 && (param_insn >= pred->n_insn || pred->insn[param_insn].op != VGPU_OP_CONST)) {
+            vg_set_error("batch: parameter instruction %d is not a constant of the predicate", param_insn);
+            return nullptr;
+        }
+    }
+    vgpu_batch *b = new vgpu_batch();
+    b->order = key_type == VGPU_KEY_NONE ? VGPU_ORDER_FIRST_SEEN : order;
+    b->key_type = key_type;
+    b->aggs.n = n_aggs;
+    for (int a = 0; a < n_aggs; a++) b->aggs.a[a] = aggs[a];
+    memset(&b->prog, 0, sizeof b->prog);
+    if (pred) {
+        b->prog = *pred;
+        b->has_pred = 1;
+        b->n_pred_cols = n_pred_cols;
+        b->param_insn = param_insn;
+    }
+    if (gb_layout(b->L, b->order, key_type, n_inputs, input_types, n_aggs, aggs)) {
+        delete b;
+        return nullptr;
+    }
+    b->out_vals.resize(n_aggs);
+    b->out_valid.resize(n_aggs);
+    return b;
+}
+
+static int bt_enqueue(vgpu_batch *b, bool first);
+
+// estimate of one query's groups from its sketch registers
+static int64_t bt_estimate(const uint32_t *r, uint64_t nan_rows) {
+    double m = BT_SK_REGS, z = 0;
+    int zeros = 0;
+    for (int i = 0; i < BT_SK_REGS; i++) {
+        z += std::ldexp(1.0, -(int)r[i]);
+        zeros += r[i] == 0;
+    }
+    double e = 0.7213 / (1.0 + 1.079 / m) * m * m / z;
+    if (e <= 2.5 * m && zeros > 0) e = m * std::log(m / zeros);
+    return (int64_t)std::ceil(e) + (int64_t)nan_rows + 2;
+}
+
+extern "C" int vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_queries) {
+    if (b->planned) {
+        vg_set_error("batch: already planned");
+        return -1;
+    }
+    if (n_queries <= 0) {
+        vg_set_error("batch: no queries");
+        return -1;
+    }
+    const GbLayout &L = b->L;
+    hipStream_t st = vg_stream();
+    std::vector<BtQuery> qs((size_t)n_queries);
+    std::vector<vgpu_prog> progs((size_t)n_queries);
+    std::vector<BtItem> items;
+    int any_vm = 0;
+    for (int q = 0; q < n_queries; q++) {
+        const vgpu_query &in = queries[q];
+        BtQuery &Q = qs[q];
+        memset(&Q, 0, sizeof Q);
+        if (in.n_rows < 0 || in.n_rows > (int64_t)0xFFFFFFFF * BT_TILE) {
+            vg_set_error("batch: query %d has %lld rows", q, (long long)in.n_rows);
+            return -1;
+        }
+        if (!L.key_none) {
+            if (in.key.type != b->key_type) {
+                vg_set_error("batch: query %d key type %d differs from declared %d", q, in.key.type, b->key_type);
+                return -1;
+            }
+            Q.key = in.key;
+        }
+        for (int i = 0; i < L.n_inputs; i++) {
+            if (!in.inputs || in.inputs[i].type != L.in_type[i]) {
+                vg_set_error("batch: query %d input %d does not have the declared type", q, i);
+                return -1;
+            }
+            Q.ins.c[i] = in.inputs[i];
+        }
+        for (int i = 0; i < b->n_pred_cols; i++) {
+            if (!in.pred_cols) {
+                vg_set_error("batch: query %d has no predicate columns", q);
+                return -1;
+            }
+            Q.pcs.c[i] = in.pred_cols[i];
+        }
+        Q.n_rows = in.n_rows;
+        memset(&progs[q], 0, sizeof progs[q]);
+        if (b->has_pred) {
+            progs[q] = b->prog;
+            if (b->param_insn >= 0) memcpy(&progs[q].insn[b->param_insn].imm, &in.param, 8);
+            Q.pi = vg_pred_info(&progs[q]);
+            Q.vm = !Q.pi.fast;
+            any_vm |= Q.vm;
+        }
+        for (int64_t t = 0; t * BT_TILE < in.n_rows; t++) items.push_back(BtItem{(uint32_t)q, (uint32_t)t});
+    }
+    b->n_queries = n_queries;
+    b->n_items = (int64_t)items.size();
+    size_t vm = any_vm ? (vg_vm_lds_bytes(&b->prog, GB_THREADS) + 15) & ~(size_t)15 : 0;
+    b->vm_lds = (int32_t)vm;
+    if (b->qs.reserve(qs.size() * sizeof(BtQuery)) || b->progs.reserve(progs.size() * sizeof(vgpu_prog)) ||
+        b->items.reserve(std::max<size_t>(items.size(), 1) * sizeof(BtItem)) ||
+        b->ctl.reserve((size_t)(n_queries + 1) * GB_CTL_WORDS * 8) || b->queue.reserve(8) ||
+        b->qcount.reserve((size_t)n_queries * 8) || b->qoff.reserve((size_t)(n_queries + 1) * 8) ||
+        b->qbase.reserve((size_t)(n_queries + 1) * 8))
+        return -1;
+    VG_TRY(hipMemcpyAsync(b->progs.ptr, progs.data(), progs.size() * sizeof(vgpu_prog), hipMemcpyHostToDevice, st));
+    if (!items.empty())
+        VG_TRY(hipMemcpyAsync(b->items.ptr, items.data(), items.size() * sizeof(BtItem), hipMemcpyHostToDevice, st));
+
+    // 1. sketch every query's keys (one launch), size the tables
+    std::vector<int64_t> est((size_t)n_queries, 1);
+    if (!L.key_none && b->n_items > 0) {
+        DevBuf sk;
+        size_t regs = (size_t)n_queries * BT_SK_REGS;
+        if (sk.reserve(regs * 4 + (size_t)n_queries * 8)) return -1;
+        uint32_t *d_regs = sk.as<uint32_t>();
+        auto *d_nan = reinterpret_cast<unsigned long long *>(d_regs + regs);
+        k_bt_clear_u32<<<vg_blocks((int64_t)regs + 2 * n_queries, 256), 256, 0, st>>>(d_regs,
+                                                                                      (int64_t)regs + 2 * n_queries);
+        VG_LAUNCH_CHECK();
+        VG_TRY(hipMemcpyAsync(b->qs.ptr, qs.data(), qs.size() * sizeof(BtQuery), hipMemcpyHostToDevice, st));
+        BtArgs A;
+        memset(&A, 0, sizeof A);
+        A.qs = b->qs.as<BtQuery>();
+        A.items = b->items.as<BtItem>();
+        A.n_items = b->n_items;
+        k_bt_sketch<<<vg_blocks(b->n_items, 1, 4096), GB_THREADS, 0, st>>>(A, L.key_none, d_regs, d_nan);
+        VG_LAUNCH_CHECK();
+        std::vector<uint32_t> h(regs);
+        std::vector<uint64_t> hn((size_t)n_queries);
+        VG_TRY(hipMemcpyAsync(h.data(), d_regs, regs * 4, hipMemcpyDeviceToHost, st));
+        VG_TRY(hipMemcpyAsync(hn.data(), d_nan, (size_t)n_queries * 8, hipMemcpyDeviceToHost, st));
+        VG_TRY(hipStreamSynchronize(st));
+        sk.release();
+        for (int q = 0; q < n_queries; q++) est[q] = bt_estimate(&h[(size_t)q * BT_SK_REGS], hn[q]);
+    }
+    const size_t rw = (size_t)(L.RH + L.RC) * 8;
+    std::vector<uint64_t> qbase((size_t)n_queries + 1);
+    std::vector<BtItem> cmap;
+    int64_t slots = 0;
+    int S_max = 0;
+    for (int q = 0; q < n_queries; q++) {
+        BtQuery &Q = qs[q];
+        int64_t g = std::min<int64_t>(std::max<int64_t>(Q.n_rows, 1), est[q] + est[q] / 4 + 16);
+        Q.cap = pow2_at_least(2 * g);
+        Q.base = slots;
+        qbase[q] = (uint64_t)slots;
+        slots += Q.cap + 2;
+        // LDS privatization when the query's groups fit and its tiles are long enough to amortize
+        int64_t S = pow2_at_least(2 * est[q]);
+        if (L.key_none) S = 64;
+        Q.lds_slots = 0;
+        if ((size_t)(S + 2) * rw + vm <= BT_LDS_BUDGET && S * 8 <= std::min<int64_t>(Q.n_rows, BT_TILE)) {
+            Q.lds_slots = (int32_t)S;
+            S_max = std::max<int>(S_max, (int)S);
+        }
+        for (int64_t c = 0; c * GB_CMP_SLOTS < Q.cap + 2; c++) cmap.push_back(BtItem{(uint32_t)q, (uint32_t)c});
+    }
+    if (slots >= ((int64_t)1 << 32)) {
+        vg_set_error("batch: %lld table slots exceed the 2^32 the finish addresses", (long long)slots);
+        return -1;
+    }
+    qbase[n_queries] = (uint64_t)slots;
+    b->n_slots = slots;
+    b->S_max = S_max;
+    b->n_cblocks = (int64_t)cmap.size();
+    int dev = 0, cus = 0;
+    VG_TRY(hipGetDevice(&dev));
+    VG_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    b->feed_grid = std::max<int64_t>(1, std::min<int64_t>(b->n_items, (int64_t)cus * BT_BLOCKS_PER_CU));
+    int qbits = 0;
+    while (((int64_t)1 << qbits) < n_queries) qbits++;
+    b->qbits = qbits;
+    if (b->hot.reserve((size_t)slots * L.RH * 8) || b->cold.reserve((size_t)slots * L.RC * 8) ||
+        b->cmap.reserve(cmap.size() * sizeof(BtItem)) || b->bcount.reserve((size_t)b->n_cblocks * 4) ||
+        b->boff.reserve((size_t)b->n_cblocks * 8 + 8) || b->por.reserve((size_t)b->n_cblocks * 8) ||
+        b->pand.reserve((size_t)b->n_cblocks * 8) || b->skeys.reserve((size_t)slots * 8) ||
+        b->svals.reserve((size_t)slots * 4))
+        return -1;
+    VG_TRY(hipMemcpyAsync(b->qs.ptr, qs.data(), qs.size() * sizeof(BtQuery), hipMemcpyHostToDevice, st));
+    VG_TRY(hipMemcpyAsync(b->cmap.ptr, cmap.data(), cmap.size() * sizeof(BtItem), hipMemcpyHostToDevice, st));
+    VG_TRY(hipMemcpyAsync(b->qbase.ptr, qbase.data(), qbase.size() * 8, hipMemcpyHostToDevice, st));
+
+    // 2. the first run fixes the sort range and the record count
+    if (bt_enqueue(b, true)) return -1;
+    b->planned = 1;
+    return 0;
+}
+
+// one run's device work; `first` (inside plan) synchronizes once to learn
+// the sort range and the number of records
+static int bt_enqueue(vgpu_batch *b, bool first) {
+    const GbLayout &L = b->L;
+    hipStream_t st = vg_stream();
+    BtArgs A;
+    memset(&A, 0, sizeof A);
+    A.qs = b->qs.as<BtQuery>();
+    A.progs = b->progs.as<vgpu_prog>();
+    A.items = b->items.as<BtItem>();
+    A.n_items = b->n_items;
+    A.queue = b->queue.as<unsigned long long>();
+    A.hot = b->hot.as<uint64_t>();
+    A.cold = b->cold.as<uint64_t>();
+    A.ctl = b->ctl.as<uint64_t>();
+    A.vm_lds = b->vm_lds;
+    A.n_queries = b->n_queries;
+    uint64_t *bctl = A.ctl + (size_t)b->n_queries * GB_CTL_WORDS;
+    uint64_t *qcount = b->qcount.as<uint64_t>();
+    const uint64_t empty = L.key_is_double ? VG_EMPTY_F64 : VG_EMPTY_I64;
+
+    // 1. init
+    k_gb_init<<<vg_blocks(b->n_slots, 256, 8192), 256, 0, st>>>(A.hot, A.cold, b->n_slots, L.RH, L.RC,
+                                                                 L.hot_init_ones, L.cold_init_ones, empty);
+    VG_LAUNCH_CHECK();
+    int64_t ctl_words = (int64_t)(b->n_queries + 1) * GB_CTL_WORDS;
+    k_bt_init_ctl<<<vg_blocks(std::max<int64_t>(ctl_words, b->n_queries), 256), 256, 0, st>>>(
+        A.ctl, b->n_queries, qcount, A.queue);
+    VG_LAUNCH_CHECK();
+    // 2. one persistent feed launch
+    if (b->n_items > 0) {
+        size_t lds = (size_t)b->vm_lds + (b->S_max ? (size_t)(b->S_max + 2) * (L.RH + L.RC) * 8 : 0);
+        vg_kt_begin();
+        k_bt_feed<<<(unsigned)b->feed_grid, GB_THREADS, lds, st>>>(L, A);
+        VG_LAUNCH_CHECK();
+        vg_kt_end();
+    }
+    // 3. finish: compaction into (key, arena slot) pairs
+    uint32_t *bcount = b->bcount.as<uint32_t>();
+    uint64_t *boff = b->boff.as<uint64_t>();
+    uint64_t *total = boff + b->n_cblocks;
+    k_bt_cmp_count<<<(unsigned)b->n_cblocks, GB_THREADS, 0, st>>>(L, A, b->cmap.as<BtItem>(), b->order, bcount,
+                                                                  qcount);
+    VG_LAUNCH_CHECK();
+    if (vg_scan_u32_to_u64(bcount, boff, b->n_cblocks, total, b->scan_tmp)) return -1;
+    k_bt_cmp_write<<<(unsigned)b->n_cblocks, GB_THREADS, 0, st>>>(L, A, b->cmap.as<BtItem>(), b->order, boff,
+                                                                  b->skeys.as<uint64_t>(), b->svals.as<uint32_t>(),
+                                                                  b->por.as<uint64_t>(), b->pand.as<uint64_t>());
+    VG_LAUNCH_CHECK();
+    k_gb_or_and_final<<<vg_blocks(b->n_cblocks, 4 * GB_THREADS, 256), GB_THREADS, 0, st>>>(
+        b->por.as<uint64_t>(), b->pand.as<uint64_t>(), b->n_cblocks, bctl);
+    VG_LAUNCH_CHECK();
+    if (first) {
+        uint64_t h[GB_CTL_WORDS], n_pairs = 0;
+        VG_TRY(hipMemcpyAsync(h, bctl, sizeof h, hipMemcpyDeviceToHost, st));
+        VG_TRY(hipMemcpyAsync(&n_pairs, total, 8, hipMemcpyDeviceToHost, st));
+        VG_TRY(hipStreamSynchronize(st));
+        uint64_t diff = h[6] ^ h[7];
+        b->n_pairs = (int64_t)n_pairs;
+        b->lo = diff ? __builtin_ctzll(diff) : 0;
+        b->hi = diff ? 64 - __builtin_clzll(diff) : 0;
+        b->range_mask = 0;
+        for (int i = b->lo; i < b->hi; i++) b->range_mask |= 1ull << i;
+        size_t vwords = (size_t)((b->n_pairs + 63) / 64 + 1);
+        if (b->out_keys.reserve((size_t)(b->n_pairs + 1) * 8) || b->out_kvalid.reserve(vwords * 8)) return -1;
+        for (int a = 0; a < b->aggs.n; a++)
+            if (b->out_vals[a].reserve((size_t)(b->n_pairs + 1) * 8) || b->out_valid[a].reserve(vwords * 8))
+                return -1;
+    }
+    k_bt_guard<<<1, 1, 0, st>>>(bctl, b->range_mask, total, (uint64_t)b->n_pairs);
+    VG_LAUNCH_CHECK();
+    int64_t n = b->n_pairs;
+    uint64_t *skeys = b->skeys.as<uint64_t>();
+    uint32_t *svals = b->svals.as<uint32_t>();
+    // stable sort by the order key, then by query
+    if (n > 1 && b->hi > b->lo && vg_sort_pairs(skeys, svals, n, b->lo, b->hi, b->sort_tmp)) return -1;
+    if (n > 0) {
+        k_bt_qkeys<<<vg_blocks(n, 256), 256, 0, st>>>(svals, n, b->qbase.as<uint64_t>(), b->n_queries, skeys);
+        VG_LAUNCH_CHECK();
+    }
+    if (n > 1 && b->qbits > 0 && vg_sort_pairs(skeys, svals, n, 0, b->qbits, b->sort_tmp)) return -1;
+    // per-query output offsets
+    if (vg_scan_u64(qcount, b->qoff.as<uint64_t>(), b->n_queries, b->qoff.as<uint64_t>() + b->n_queries,
+                    b->scan_tmp))
+        return -1;
+    // emit every group of every query
+    if (n > 0) {
+        EmitOut O;
+        memset(&O, 0, sizeof O);
+        O.keys = b->out_keys.ptr;
+        O.kvalid = b->out_kvalid.as<uint64_t>();
+        for (int a = 0; a < b->aggs.n; a++) {
+            O.vals[a] = b->out_vals[a].as<double>();
+            O.valid[a] = b->out_valid[a].as<uint64_t>();
+        }
+        const int RW = L.RH + L.RC;
+        if (RW <= GB_EMIT_STAGE_WORDS)
+            k_bt_emit<true><<<vg_blocks(n, GB_THREADS), GB_THREADS, (size_t)GB_THREADS * RW * 8, st>>>(
+                L, b->aggs, A, svals, skeys, n, O);
+        else
+            k_bt_emit<false><<<vg_blocks(n, GB_THREADS), GB_THREADS, 0, st>>>(L, b->aggs, A, svals, skeys, n, O);
+        VG_LAUNCH_CHECK();
+    }
+    b->counts_valid = 0;
+    return 0;
+}
+
+extern "C" int vgpu_batch_run(vgpu_batch *b) {
+    if (!b->planned) {
+        vg_set_error("batch: run before plan");
+        return -1;
+    }
+    if (b->exec) {
+        VG_TRY(hipGraphLaunch(b->exec, vg_stream()));
+        b->counts_valid = 0;
+        return 0;
+    }
+    return bt_enqueue(b, false);
+}
+
+extern "C" int vgpu_batch_capture(vgpu_batch *b) {
+    if (!b->planned) {
+        vg_set_error("batch: capture before plan");
+        return -1;
+    }
+    if (b->exec) return 0;
+    hipStream_t st = vg_stream();
+    VG_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = bt_enqueue(b, false);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(st, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return -1;
+    }
+    if (e != hipSuccess) {
+        vg_set_error("batch: capture failed: %s", hipGetErrorString(e));
+        return -1;
+    }
+    e = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        b->exec = nullptr;
+        vg_set_error("batch: graph instantiation failed: %s", hipGetErrorString(e));
+        return -1;
+    }
+    b->graph = graph;
+    return 0;
+}
+
+static int bt_read_counts(vgpu_batch *b) {
+    if (b->counts_valid) return 0;
+    hipStream_t st = vg_stream();
+    int Q = b->n_queries;
+    std::vector<uint64_t> cnt((size_t)Q), off((size_t)Q + 1);
+    uint64_t bctl[GB_CTL_WORDS];
+    VG_TRY(hipMemcpyAsync(cnt.data(), b->qcount.ptr, (size_t)Q * 8, hipMemcpyDeviceToHost, st));
+    VG_TRY(hipMemcpyAsync(off.data(), b->qoff.ptr, (size_t)(Q + 1) * 8, hipMemcpyDeviceToHost, st));
+    VG_TRY(hipMemcpyAsync(bctl, b->ctl.as<uint64_t>() + (size_t)Q * GB_CTL_WORDS, sizeof bctl,
+                          hipMemcpyDeviceToHost, st));
+    VG_TRY(hipStreamSynchronize(st));
+    if (bctl[BT_CTL_ERR]) {
+        vg_set_error("batch: the inputs changed since plan (keys outside the planned sort range or a "
+                     "different number of groups); plan a new batch");
+        return -1;
+    }
+    b->h_count.assign(cnt.begin(), cnt.end());
+    b->h_off.assign(off.begin(), off.end() - 1);
+    b->counts_valid = 1;
+    return 0;
+}
+
+extern "C" int vgpu_batch_counts(vgpu_batch *b, int64_t *n_groups, int64_t *offsets) {
+    if (!b->planned) {
+        vg_set_error("batch: counts before plan");
+        return -1;
+    }
+    if (bt_read_counts(b)) return -1;
+    for (int q = 0; q < b->n_queries; q++) {
+        if (n_groups) n_groups[q] = b->h_count[q];
+        if (offsets) offsets[q] = b->h_off[q];
+    }
+    return 0;
+}
+
+// copies bits [off, off + n) of a device bitmap into dst (bit 0 = first)
+static int bt_copy_bits(uint8_t *dst, const uint8_t *dev_bits, int64_t off, int64_t n) {
+    if (n <= 0) return 0;
+    int64_t b0 = off / 8, b1 = (off + n + 7) / 8;
+    std::vector<uint8_t> tmp((size_t)(b1 - b0 + 1), 0);
+    VG_TRY(hipMemcpyAsync(tmp.data(), dev_bits + b0, (size_t)(b1 - b0), hipMemcpyDeviceToHost, vg_stream()));
+    VG_TRY(hipStreamSynchronize(vg_stream()));
+    int sh = (int)(off & 7);
+    memset(dst, 0, (size_t)((n + 7) / 8));
+    for (int64_t i = 0; i < n; i++) {
+        int64_t src = i + sh;
+        if ((tmp[(size_t)(src >> 3)] >> (src & 7)) & 1) dst[i >> 3] |= (uint8_t)(1u << (i & 7));
+    }
+    return 0;
+}
+
+extern "C" int64_t vgpu_batch_fetch(vgpu_batch *b, int q, void *keys, uint8_t *key_validity,
+                                    double *const *agg_values, uint8_t *const *agg_validity) {
+    if (!b->planned) {
+        vg_set_error("batch: fetch before plan");
+        return -1;
+    }
+    if (q < 0 || q >= b->n_queries) {
+        vg_set_error("batch: query %d of %d", q, b->n_queries);
+        return -1;
+    }
+    if (bt_read_counts(b)) return -1;
+    hipStream_t st = vg_stream();
+    int64_t n = b->h_count[q], off = b->h_off[q];
+    if (n > 0) {
+        if (keys && !b->L.key_none)
+            VG_TRY(hipMemcpyAsync(keys, b->out_keys.as<uint64_t>() + off, (size_t)n * 8, hipMemcpyDeviceToHost, st));
+        for (int a = 0; a < b->aggs.n; a++)
+            if (agg_values && agg_values[a])
+                VG_TRY(hipMemcpyAsync(agg_values[a], b->out_vals[a].as<double>() + off, (size_t)n * 8,
+                                      hipMemcpyDeviceToHost, st));
+        VG_TRY(hipStreamSynchronize(st));
+        if (key_validity && bt_copy_bits(key_validity, b->out_kvalid.as<uint8_t>(), off, n)) return -1;
+        for (int a = 0; a < b->aggs.n; a++)
+            if (agg_validity && agg_validity[a] && bt_copy_bits(agg_validity[a], b->out_valid[a].as<uint8_t>(), off, n))
+                return -1;
+    }
+    uint64_t qctl[GB_CTL_WORDS];
+    VG_TRY(hipMemcpyAsync(qctl, b->ctl.as<uint64_t>() + (size_t)q * GB_CTL_WORDS, sizeof qctl, hipMemcpyDeviceToHost,
+                          st));
+    VG_TRY(hipStreamSynchronize(st));
+    return (int64_t)qctl[4];
+}
+
+extern "C" void vgpu_batch_destroy(vgpu_batch *b) {
+    if (!b) return;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    // graph replays may still run: the buffers go back to the pool only
+    // after the stream drains
+    (void)hipStreamSynchronize(vg_stream());
+    DevBuf *bufs[] = {&b->qs, &b->progs, &b->items, &b->cmap, &b->queue, &b->hot, &b->cold, &b->ctl, &b->qbase,
+                      &b->qcount, &b->qoff, &b->bcount, &b->boff, &b->por, &b->pand, &b->skeys, &b->svals,
+                      &b->sort_tmp, &b->scan_tmp, &b->out_keys, &b->out_kvalid};
+    for (DevBuf *p : bufs) p->release();
+    for (auto &x : b->out_vals) x.release();
+    for (auto &x : b->out_valid) x.release();
+    delete b;
+}

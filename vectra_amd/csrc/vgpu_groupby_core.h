@@ -481,35 +481,13 @@ __device__ __forceinline__ void gb_finish_agg(const GbLayout &L, const vgpu_agg 
 // every lane gathering the words of its own record.
 #define GB_EMIT_STAGE_WORDS 24
 
-template <bool STAGED>
-__global__ void __launch_bounds__(GB_THREADS)
-k_gb_emit(GbLayout L, GbAggs AG, RecSrc R, const uint32_t *order_slots, int64_t n_sorted, int na_last, int64_t n_out,
-          int key_width, EmitOut O, uint64_t *ctl) {
-    extern __shared__ uint64_t dynw[];
-    __shared__ int64_t sidx[GB_THREADS];
-    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool in = g < n_out;
-    int64_t s = 0;
-    if (in) s = (na_last && g == n_sorted) ? R.na_slot : (int64_t)order_slots[g];
-    const uint64_t *h, *c;
-    if (STAGED) {
-        const int RW = L.RH + L.RC;
-        sidx[threadIdx.x] = s;
-        __syncthreads();
-        const int64_t g0 = (int64_t)blockIdx.x * blockDim.x;
-        const int cnt = n_out - g0 < GB_THREADS ? (int)(n_out - g0) : GB_THREADS;
-        for (int q = threadIdx.x; q < cnt * RW; q += GB_THREADS) {
-            int k = q / RW, w = q - k * RW;
-            int64_t sk = sidx[k];
-            dynw[q] = w < L.RH ? R.hot[sk * R.hstride + w] : R.cold[sk * R.cstride + (w - L.RH)];
-        }
-        __syncthreads();
-        h = dynw + threadIdx.x * RW;
-        c = h + L.RH;
-    } else {
-        h = R.hot + s * R.hstride;
-        c = R.cold + s * R.cstride;
-    }
+// one output row g of the finished result: key, key validity bit, every
+// aggregate and its validity bit (bitmap words written by lane 0 of each
+// wave from ballots, so every lane of the wave must call it); h / c are
+// the group's record words, s its slot in R, ctl its query's control words
+__device__ __forceinline__ void gb_emit_row(const GbLayout &L, const GbAggs &AG, const RecSrc &R, int64_t g, bool in,
+                                            int64_t s, const uint64_t *h, const uint64_t *c, int64_t n_out,
+                                            int key_width, const EmitOut &O, uint64_t *ctl) {
     bool kvalid = in && s != R.na_slot;
     if (in && !L.key_none) {
         uint64_t kb = 0;
@@ -537,6 +515,47 @@ k_gb_emit(GbLayout L, GbAggs AG, RecSrc R, const uint32_t *order_slots, int64_t 
         if (vg_lane() == 0 && (g >> 6) * 64 < n_out) O.valid[a][g >> 6] = w;
     }
     if (warns) atomicAdd((unsigned long long *)&ctl[4], (unsigned long long)warns);
+}
+
+// copies the records of the block's output rows into LDS word by word
+// (consecutive lanes, consecutive words of one record); rec_of(k) gives the
+// record slot of the block's k-th row
+template <class SlotOf>
+__device__ __forceinline__ void gb_stage_records(const GbLayout &L, const RecSrc &R, int cnt, SlotOf rec_of,
+                                                 uint64_t *dst) {
+    const int RW = L.RH + L.RC;
+    for (int q = threadIdx.x; q < cnt * RW; q += GB_THREADS) {
+        int k = q / RW, w = q - k * RW;
+        int64_t sk = rec_of(k);
+        dst[q] = w < L.RH ? R.hot[sk * R.hstride + w] : R.cold[sk * R.cstride + (w - L.RH)];
+    }
+}
+
+template <bool STAGED>
+__global__ void __launch_bounds__(GB_THREADS)
+k_gb_emit(GbLayout L, GbAggs AG, RecSrc R, const uint32_t *order_slots, int64_t n_sorted, int na_last, int64_t n_out,
+          int key_width, EmitOut O, uint64_t *ctl) {
+    extern __shared__ uint64_t dynw[];
+    __shared__ int64_t sidx[GB_THREADS];
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool in = g < n_out;
+    int64_t s = 0;
+    if (in) s = (na_last && g == n_sorted) ? R.na_slot : (int64_t)order_slots[g];
+    const uint64_t *h, *c;
+    if (STAGED) {
+        sidx[threadIdx.x] = s;
+        __syncthreads();
+        const int64_t g0 = (int64_t)blockIdx.x * blockDim.x;
+        const int cnt = n_out - g0 < GB_THREADS ? (int)(n_out - g0) : GB_THREADS;
+        gb_stage_records(L, R, cnt, [&](int k) { return sidx[k]; }, dynw);
+        __syncthreads();
+        h = dynw + threadIdx.x * (L.RH + L.RC);
+        c = h + L.RH;
+    } else {
+        h = R.hot + s * R.hstride;
+        c = R.cold + s * R.cstride;
+    }
+    gb_emit_row(L, AG, R, g, in, s, h, c, n_out, key_width, O, ctl);
 }
 
 // ---------------------------------------------------------------------------

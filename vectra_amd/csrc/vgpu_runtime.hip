@@ -318,11 +318,26 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const TIn *in, uint
         *total_out = run;
 }
 
+// stream work of the scan and the sort is kernels only (no memset or copy
+// nodes), so both can be captured into a hipGraph
+__global__ void k_set_u64(uint64_t *p, uint64_t v) { *p = v; }
+
+__global__ void k_copy_pairs(const uint64_t *ks, const uint32_t *vs, uint64_t *kd, uint32_t *vd, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        kd[i] = ks[i];
+        vd[i] = vs[i];
+    }
+}
+
 template <class TIn>
 static int scan_impl(const TIn *in, uint64_t *out, int64_t n, uint64_t *total, DevBuf &tmp, size_t tmp_off) {
     hipStream_t s = vg_stream();
     if (n <= 0) {
-        if (total) VG_TRY(hipMemsetAsync(total, 0, 8, s));
+        if (total) {
+            k_set_u64<<<1, 1, 0, s>>>(total, 0);
+            VG_LAUNCH_CHECK();
+        }
         return 0;
     }
     int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
@@ -460,8 +475,8 @@ int vg_sort_pairs(uint64_t *keys, uint32_t *vals, int64_t n, int begin_bit, int 
         passes++;
     }
     if (passes & 1) {
-        VG_TRY(hipMemcpyAsync(keys, kin, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
-        VG_TRY(hipMemcpyAsync(vals, vin, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+        k_copy_pairs<<<vg_blocks(n, 256), 256, 0, s>>>(kin, vin, keys, vals, n);
+        VG_LAUNCH_CHECK();
     }
     scan_view.ptr = nullptr;   // borrowed view: nothing to free
     scan_view.cap = 0;

@@ -117,6 +117,11 @@ class VgpuInput(C.Structure):
                 ("col_b", C.c_int32), ("reserved", C.c_int32)]
 
 
+class VgpuQuery(C.Structure):
+    _fields_ = [("key", VgpuCol), ("inputs", C.POINTER(VgpuCol)), ("pred_cols", C.POINTER(VgpuCol)),
+                ("n_rows", C.c_int64), ("param", C.c_uint64)]
+
+
 class VgpuFeedSpec(C.Structure):
     _fields_ = [("n_rows", C.c_int64), ("sel", C.c_void_p), ("n_sel", C.c_int64), ("pos_base", C.c_int64),
                 ("pred", C.POINTER(VgpuProg)), ("join", C.c_void_p), ("join_kind", C.c_int32),
@@ -232,6 +237,16 @@ def vgpu() -> C.CDLL:
         _proto(lib, "vgpu_tune_get", C.c_int64, [C.c_char_p])
         _proto(lib, "vgpu_kernel_timing", C.c_int, [C.c_int])
         _proto(lib, "vgpu_kernel_time_ms", C.c_double, [C.POINTER(C.c_int64), C.c_int])
+        _proto(lib, "vgpu_batch_create", C.c_void_p, [C.c_int, C.c_int, C.POINTER(C.c_int32), C.c_int,
+                                                      C.POINTER(VgpuAgg), C.c_int, C.POINTER(VgpuProg), C.c_int,
+                                                      C.c_int])
+        _proto(lib, "vgpu_batch_plan", C.c_int, [C.c_void_p, C.POINTER(VgpuQuery), C.c_int])
+        _proto(lib, "vgpu_batch_run", C.c_int, [C.c_void_p])
+        _proto(lib, "vgpu_batch_capture", C.c_int, [C.c_void_p])
+        _proto(lib, "vgpu_batch_counts", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p])
+        _proto(lib, "vgpu_batch_fetch", C.c_int64, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                                    C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)])
+        _proto(lib, "vgpu_batch_destroy", None, [C.c_void_p])
         _libs["vgpu"] = lib
     return _libs["vgpu"]
 
@@ -639,6 +654,15 @@ class DeviceColumn:
             self.valid = lib.vgpu_malloc(max(host.bitmap.nbytes, 8))
             _vcheck(lib.vgpu_memcpy_h2d(self.valid, host.bitmap.ctypes.data, host.bitmap.nbytes))
 
+    @classmethod
+    def wrap(cls, data_ptr: int, n: int, type_: int, valid_ptr: int | None = None, keep: Any = None) -> "DeviceColumn":
+        """A view of device memory the caller owns (e.g. a torch tensor's
+        storage, kept alive through `keep`); free() leaves it alone."""
+        self = cls.__new__(cls)
+        self.n, self.type, self.data, self.valid = n, type_, data_ptr, valid_ptr
+        self._keep, self._view = keep, True
+        return self
+
     @property
     def col(self) -> VgpuCol:
         c = VgpuCol()
@@ -649,6 +673,9 @@ class DeviceColumn:
 
     def free(self) -> None:
         lib = vgpu()
+        if getattr(self, "_view", False):
+            self.data = self.valid = self._keep = None
+            return
         if self.data:
             lib.vgpu_free(self.data)
         if self.valid:
@@ -794,4 +821,86 @@ class GroupBy:
     def free(self) -> None:
         if self.h:
             vgpu().vgpu_groupby_destroy(self.h)
+            self.h = None
+
+
+class Batch:
+    """Kernel-level batch of filter -> group-by queries of one shape
+    (include/vgpu.h, vgpu_batch_*)."""
+
+    def __init__(self, key_type: int, input_types: Sequence[int], aggs: Sequence[tuple], order: int,
+                 pred: VgpuProg | None = None, n_pred_cols: int = 0, param_insn: int = -1):
+        lib = vgpu()
+        self.n_aggs = len(aggs)
+        self.key_type = key_type
+        it = (C.c_int32 * max(1, len(input_types)))(*input_types)
+        ag = (VgpuAgg * max(1, len(aggs)))()
+        for i, (kind, inp, na_rm) in enumerate(aggs):
+            ag[i].kind = AGG_NAMES[kind] if isinstance(kind, str) else kind
+            ag[i].input = inp
+            ag[i].na_rm = na_rm
+        self._pred = pred
+        self.h = lib.vgpu_batch_create(key_type, len(input_types), it, len(aggs), ag, order,
+                                       C.byref(pred) if pred is not None else None, n_pred_cols, param_insn)
+        if not self.h:
+            raise VectraError(lib.vgpu_last_error().decode())
+
+    def plan(self, queries: Sequence[dict]) -> None:
+        """queries: dicts with key (DeviceColumn or None), inputs, pred_cols,
+        n_rows and param (float or int: its 8 bytes)."""
+        qa = (VgpuQuery * len(queries))()
+        keep = []
+        for i, q in enumerate(queries):
+            if q.get("key") is not None:
+                qa[i].key = q["key"].col
+            ins = col_array(q.get("inputs", ()))
+            pcs = col_array(q.get("pred_cols", ()))
+            keep += [ins, pcs]
+            qa[i].inputs = C.cast(ins, C.POINTER(VgpuCol))
+            qa[i].pred_cols = C.cast(pcs, C.POINTER(VgpuCol))
+            qa[i].n_rows = q["n_rows"]
+            p = q.get("param", 0)
+            qa[i].param = int(np.array([p], np.float64).view(np.uint64)[0]) if isinstance(p, float) else int(p) & (2**64 - 1)
+        self.n_queries = len(queries)
+        self._keep = (qa, keep)
+        _vcheck(vgpu().vgpu_batch_plan(self.h, qa, len(queries)))
+
+    def run(self) -> None:
+        _vcheck(vgpu().vgpu_batch_run(self.h))
+
+    def capture(self) -> None:
+        _vcheck(vgpu().vgpu_batch_capture(self.h))
+
+    def counts(self) -> tuple[np.ndarray, np.ndarray]:
+        n = np.zeros(self.n_queries, np.int64)
+        off = np.zeros(self.n_queries, np.int64)
+        _vcheck(vgpu().vgpu_batch_counts(self.h, n.ctypes.data, off.ctypes.data))
+        return n, off
+
+    def fetch(self, q: int) -> dict[str, tuple[np.ndarray, np.ndarray]]:
+        n_groups, _ = self.counts()
+        n = int(n_groups[q])
+        keys = np.zeros(max(n, 1), np.int64)
+        kv = np.zeros(max((n + 7) // 8, 1), np.uint8)
+        vals = [np.zeros(max(n, 1), np.float64) for _ in range(self.n_aggs)]
+        valid = [np.zeros(max((n + 7) // 8, 1), np.uint8) for _ in range(self.n_aggs)]
+        vp = (C.c_void_p * max(1, self.n_aggs))(*[v.ctypes.data for v in vals])
+        mp = (C.c_void_p * max(1, self.n_aggs))(*[m.ctypes.data for m in valid])
+        w = vgpu().vgpu_batch_fetch(self.h, q, keys.ctypes.data, kv.ctypes.data, vp, mp)
+        _vcheck(w)
+        unpack = lambda b: np.unpackbits(b, bitorder="little")[:n].astype(bool)
+        out = {}
+        if self.key_type == VEC_DOUBLE:
+            out["key"] = (keys[:n].view(np.float64), unpack(kv))
+        elif self.key_type == VEC_BOOL:
+            out["key"] = (keys[:n].astype(bool), unpack(kv))
+        elif self.key_type != VGPU_KEY_NONE:
+            out["key"] = (keys[:n], unpack(kv))
+        for i in range(self.n_aggs):
+            out[f"a{i}"] = (vals[i][:n], unpack(valid[i]))
+        return out
+
+    def free(self) -> None:
+        if self.h:
+            vgpu().vgpu_batch_destroy(self.h)
             self.h = None
