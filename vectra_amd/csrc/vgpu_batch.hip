@@ -35,7 +35,8 @@
 #define BT_SK_BITS 10              // sketch registers per query: 2^10
 #define BT_SK_REGS (1 << BT_SK_BITS)
 #define BT_LDS_BUDGET 61440        // dynamic LDS of the feed blocks
-#define BT_BLOCKS_PER_CU 4
+#define BT_BLOCKS_PER_CU 8
+#define BT_U 4                     // rows in flight per thread in the feed
 #define BT_CTL_ERR 9               // batch control word: inputs left the plan's key range
 
 struct BtQuery {
@@ -159,6 +160,8 @@ __global__ void k_bt_init_ctl(uint64_t *ctl, int n_queries, uint64_t *qcount, un
 __global__ void __launch_bounds__(GB_THREADS) k_bt_feed(GbLayout L, BtArgs B) {
     extern __shared__ uint64_t dynw[];
     __shared__ long long s_item;
+    __shared__ InSet s_ins;
+    __shared__ ColSet s_pcs;
     uint8_t *dyn = reinterpret_cast<uint8_t *>(dynw);
     uint64_t *regs = dynw;
     const uint64_t empty = L.key_is_double ? VG_EMPTY_F64 : VG_EMPTY_I64;
@@ -166,39 +169,66 @@ __global__ void __launch_bounds__(GB_THREADS) k_bt_feed(GbLayout L, BtArgs B) {
     for (;;) {
         if (threadIdx.x == 0) s_item = (long long)atomicAdd(B.queue, 1ull);
         __syncthreads();
-        const int64_t it = s_item;
-        __syncthreads();
+        // the item is block-uniform: its query's fields load once into
+        // scalar registers instead of being re-read from memory per row
+        const int64_t it = (int64_t)__builtin_amdgcn_readfirstlane((int)s_item);
         if (it >= B.n_items) break;
         const BtItem item = B.items[it];
-        const BtQuery &Q = B.qs[item.q];
-        const vgpu_prog &prog = B.progs[Q.vm ? item.q : 0];
-        uint8_t *regv = dyn + (size_t)(Q.vm ? prog.n_insn : 0) * GB_THREADS * 8;
-        const int S = Q.lds_slots;
+        const BtQuery &G = B.qs[item.q];
+        // column sets to LDS, scalars to registers (block-uniform values)
+        for (int w = threadIdx.x; w < (int)(sizeof(InSet) / 4); w += GB_THREADS)
+            reinterpret_cast<uint32_t *>(&s_ins)[w] = reinterpret_cast<const uint32_t *>(&G.ins)[w];
+        for (int w = threadIdx.x; w < (int)(sizeof(ColSet) / 4); w += GB_THREADS)
+            reinterpret_cast<uint32_t *>(&s_pcs)[w] = reinterpret_cast<const uint32_t *>(&G.pcs)[w];
+        const vgpu_col key = G.key;
+        const PredInfo pi = G.pi;
+        const int64_t n_rows = G.n_rows, cap = G.cap, tbase = G.base;
+        const int S = G.lds_slots, vm = G.vm;
+        __syncthreads();
+        const vgpu_prog &prog = B.progs[vm ? item.q : 0];
+        uint8_t *regv = dyn + (size_t)(vm ? prog.n_insn : 0) * GB_THREADS * 8;
         uint64_t *lcold = lhot + (size_t)(S + 2) * L.RH;
-        uint64_t *hot = B.hot + Q.base * L.RH;
-        uint64_t *cold = B.cold + Q.base * L.RC;
+        uint64_t *hot = B.hot + tbase * L.RH;
+        uint64_t *cold = B.cold + tbase * L.RC;
         uint64_t *ctl = B.ctl + (size_t)item.q * GB_CTL_WORDS;
         if (S) gb_lds_init(L, lhot, lcold, S, empty);
         const int64_t r0 = (int64_t)item.part * BT_TILE;
-        const int64_t r1 = r0 + BT_TILE < Q.n_rows ? r0 + BT_TILE : Q.n_rows;
-        for (int64_t row = r0 + threadIdx.x; row < r1; row += GB_THREADS) {
-            if (!vg_pred_row(Q.pi, prog, Q.pcs, row, regs, regv, GB_THREADS, threadIdx.x)) continue;
-            uint64_t kn = 0;
-            int cls = VG_KEY_REGULAR;
-            if (!L.key_none) {
-                cls = vg_key_class(Q.key, row, kn);
-                if (L.key_is_double && cls == VG_KEY_REGULAR && kn == 0)
-                    gb_note_zero_key(ctl, vg_load_bits(Q.key, row), (uint64_t)row);
+        const int64_t r1 = r0 + BT_TILE < n_rows ? r0 + BT_TILE : n_rows;
+        // BT_U rows per thread per step: their predicate and key loads are
+        // issued together, so each wave keeps several loads in flight
+        for (int64_t base = r0 + threadIdx.x; base < r1; base += (int64_t)GB_THREADS * BT_U) {
+            bool ok[BT_U];
+            int cls[BT_U];
+            uint64_t kn[BT_U];
+#pragma unroll
+            for (int u = 0; u < BT_U; u++) {
+                int64_t row = base + (int64_t)u * GB_THREADS;
+                ok[u] = row < r1 && vg_pred_row(pi, prog, s_pcs, row, regs, regv, GB_THREADS, threadIdx.x);
             }
-            int64_t ls = S ? gb_lds_slot(lhot, S, L.RH, cls, kn, empty) : -1;
-            if (ls >= 0) {
-                gb_apply(L, Q.ins, row, (uint64_t)row, lhot + (size_t)ls * L.RH, lcold + (size_t)ls * L.RC);
-            } else {
-                int64_t gs = gb_global_slot(L, hot, Q.cap, cls, kn, (uint64_t)row, ctl);
-                if (gs >= 0) gb_apply(L, Q.ins, row, (uint64_t)row, hot + (size_t)gs * L.RH, cold + (size_t)gs * L.RC);
+#pragma unroll
+            for (int u = 0; u < BT_U; u++) {
+                int64_t row = base + (int64_t)u * GB_THREADS;
+                kn[u] = 0;
+                cls[u] = VG_KEY_REGULAR;
+                if (ok[u] && !L.key_none) cls[u] = vg_key_class(key, row, kn[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < BT_U; u++) {
+                if (!ok[u]) continue;
+                int64_t row = base + (int64_t)u * GB_THREADS;
+                if (L.key_is_double && cls[u] == VG_KEY_REGULAR && kn[u] == 0)
+                    gb_note_zero_key(ctl, vg_load_bits(key, row), (uint64_t)row);
+                int64_t ls = S ? gb_lds_slot(lhot, S, L.RH, cls[u], kn[u], empty) : -1;
+                if (ls >= 0) {
+                    gb_apply(L, s_ins, row, (uint64_t)row, lhot + (size_t)ls * L.RH, lcold + (size_t)ls * L.RC);
+                } else {
+                    int64_t gs = gb_global_slot(L, hot, cap, cls[u], kn[u], (uint64_t)row, ctl);
+                    if (gs >= 0)
+                        gb_apply(L, s_ins, row, (uint64_t)row, hot + (size_t)gs * L.RH, cold + (size_t)gs * L.RC);
+                }
             }
         }
-        if (S) gb_lds_merge(L, lhot, lcold, S, empty, hot, cold, Q.cap, ctl);
+        if (S) gb_lds_merge(L, lhot, lcold, S, empty, hot, cold, cap, ctl);
         __syncthreads();
     }
 }

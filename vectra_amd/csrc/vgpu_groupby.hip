@@ -106,6 +106,8 @@ struct vgpu_groupby {
     int64_t warn_count = 0;
 };
 
+#define GB_U 4
+
 __global__ void __launch_bounds__(GB_THREADS)
 k_gb_feed(GbLayout L, FeedArgs A, InSet ins, PredInfo pi, vgpu_prog prog, ColSet pcs) {
     extern __shared__ uint8_t dyn[];
@@ -117,24 +119,43 @@ k_gb_feed(GbLayout L, FeedArgs A, InSet ins, PredInfo pi, vgpu_prog prog, ColSet
     uint64_t *lcold = lhot + (size_t)(S + 2) * L.RH;
     const uint64_t empty = L.key_is_double ? VG_EMPTY_F64 : VG_EMPTY_I64;
     if (S) gb_lds_init(L, lhot, lcold, S, empty);
+    // GB_U rows per thread per step (a grid stride apart): their selection,
+    // predicate and key loads are issued together, so each wave keeps
+    // several loads in flight
     const int64_t stride = (int64_t)gridDim.x * GB_THREADS;
-    for (int64_t li = (int64_t)blockIdx.x * GB_THREADS + tid; li < A.n_logical; li += stride) {
-        int64_t row = A.sel ? (int64_t)A.sel[li] : li;
-        if (!vg_pred_row(pi, prog, pcs, row, regs, regv, GB_THREADS, tid)) continue;
-        uint64_t pos = (uint64_t)(A.pos_base + li);
-        uint64_t kn = 0;
-        int cls = VG_KEY_REGULAR;
-        if (!L.key_none) {
-            cls = vg_key_class(A.key, row, kn);
-            if (L.key_is_double && cls == VG_KEY_REGULAR && kn == 0)
-                gb_note_zero_key(A.ctl, vg_load_bits(A.key, row), pos);
+    for (int64_t base = (int64_t)blockIdx.x * GB_THREADS + tid; base < A.n_logical; base += stride * GB_U) {
+        int64_t row[GB_U];
+        bool ok[GB_U];
+        int cls[GB_U];
+        uint64_t kn[GB_U];
+#pragma unroll
+        for (int u = 0; u < GB_U; u++) {
+            int64_t li = base + u * stride;
+            ok[u] = li < A.n_logical;
+            row[u] = ok[u] ? (A.sel ? (int64_t)A.sel[li] : li) : 0;
         }
-        int64_t ls = S ? gb_lds_slot(lhot, S, L.RH, cls, kn, empty) : -1;
-        if (ls >= 0) {
-            gb_apply(L, ins, row, pos, lhot + (size_t)ls * L.RH, lcold + (size_t)ls * L.RC);
-        } else {
-            int64_t gs = gb_global_slot(L, A.hot, A.cap, cls, kn, pos, A.ctl);
-            if (gs >= 0) gb_apply(L, ins, row, pos, A.hot + (size_t)gs * L.RH, A.cold + (size_t)gs * L.RC);
+#pragma unroll
+        for (int u = 0; u < GB_U; u++) ok[u] = ok[u] && vg_pred_row(pi, prog, pcs, row[u], regs, regv, GB_THREADS, tid);
+#pragma unroll
+        for (int u = 0; u < GB_U; u++) {
+            kn[u] = 0;
+            cls[u] = VG_KEY_REGULAR;
+            if (ok[u] && !L.key_none) cls[u] = vg_key_class(A.key, row[u], kn[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < GB_U; u++) {
+            if (!ok[u]) continue;
+            uint64_t pos = (uint64_t)(A.pos_base + base + u * stride);
+            if (L.key_is_double && cls[u] == VG_KEY_REGULAR && kn[u] == 0)
+                gb_note_zero_key(A.ctl, vg_load_bits(A.key, row[u]), pos);
+            int64_t ls = S ? gb_lds_slot(lhot, S, L.RH, cls[u], kn[u], empty) : -1;
+            if (ls >= 0) {
+                gb_apply(L, ins, row[u], pos, lhot + (size_t)ls * L.RH, lcold + (size_t)ls * L.RC);
+            } else {
+                int64_t gs = gb_global_slot(L, A.hot, A.cap, cls[u], kn[u], pos, A.ctl);
+                if (gs >= 0)
+                    gb_apply(L, ins, row[u], pos, A.hot + (size_t)gs * L.RH, A.cold + (size_t)gs * L.RC);
+            }
         }
     }
     if (S) gb_lds_merge(L, lhot, lcold, S, empty, A.hot, A.cold, A.cap, A.ctl);
