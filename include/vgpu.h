@@ -288,7 +288,8 @@ int vgpu_sort_pairs_u64(uint64_t *keys, uint32_t *values, int64_t n,
  *                       stay in place until the batch is destroyed
  *   vgpu_batch_run      enqueues the batch again on the same columns:
  *                       device work only (one persistent work-queue feed
- *                       launch, one finish for all queries)
+ *                       launch, one finish for all queries); queries that
+ *                       all read the same columns share one scan of them
  *   vgpu_batch_capture  records that device work as a hipGraph; later
  *                       runs replay it
  *   vgpu_batch_counts   groups and output offset of every query
@@ -298,7 +299,8 @@ int vgpu_sort_pairs_u64(uint64_t *keys, uint32_t *values, int64_t n,
 typedef struct vgpu_batch vgpu_batch;
 
 typedef struct {
-    vgpu_col        key;        /* ignored for VGPU_KEY_NONE              */
+    vgpu_col        key;        /* ignored for VGPU_KEY_NONE and joins    */
+    vgpu_col        probe_key;  /* join batches: the probe side's key     */
     const vgpu_col *inputs;     /* n_inputs columns                       */
     const vgpu_col *pred_cols;  /* n_pred_cols predicate column slots     */
     int64_t         n_rows;
@@ -310,8 +312,16 @@ vgpu_batch *vgpu_batch_create(int key_type, int n_inputs,
                               const vgpu_agg *aggs, int order,
                               const vgpu_prog *pred, int n_pred_cols,
                               int param_insn);
+/* Joins (before plan): every query's rows inner-join table t on
+ * probe_key (src/join.c:262-370, matches in the reference's chain order);
+ * the group key is the table's embedded payload column key_payload. */
+int     vgpu_batch_join(vgpu_batch *b, const vgpu_jtable *t, int key_payload);
 int     vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_queries);
 int     vgpu_batch_run(vgpu_batch *b);
+/* Runs the first n_active queries again with new constants params[q]
+ * (n_active < n_queries only when every query reads the same columns: a
+ * shared scan); synchronizes once to size the finish.  Drops a capture. */
+int     vgpu_batch_rerun(vgpu_batch *b, const uint64_t *params, int n_active);
 int     vgpu_batch_capture(vgpu_batch *b);
 int     vgpu_batch_counts(vgpu_batch *b, int64_t *n_groups, int64_t *offsets);
 int64_t vgpu_batch_fetch(vgpu_batch *b, int q, void *keys, uint8_t *key_validity,

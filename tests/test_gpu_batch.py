@@ -198,3 +198,94 @@ def test_batch_flags_changed_inputs(gpu, oracle):
     b.free()
     for c in cols:
         c.free()
+
+
+def shared_table(seed, n, card):
+    r = rng(seed)
+    kv = r.integers(0, card, n) * 3 - 100
+    kv[r.random(n) < 0.01] = np.iinfo(np.int64).min
+    x = dbl_with_specials(r, n, 0.01, 0.02)
+    return {"g": (kv, with_na(r, n, 0.02)), "x": (x, with_na(r, n, 0.02)), "ax": np.abs(np.nan_to_num(x, nan=0.0))}
+
+
+def test_batch_shared_scan_and_rerun(gpu, oracle):
+    """queries over the same columns share one scan; a rerun with new
+    constants on fewer queries matches the oracle too"""
+    base = shared_table(23, 700_000, 900)
+    thr = [-1.5, -0.5, 0.0, 0.3, 0.9, 1.7, -3.0, 2.5, 0.1, -0.2, 0.6, 1.1]
+    qs = [dict(base, t=t) for t in thr]
+    pred = gpu.cmp_const_prog(0, gpu.VEC_DOUBLE, "GT", 0.0)
+    b = gpu.Batch(gpu.VEC_INT64, [gpu.VEC_DOUBLE], batch_aggs(), gpu.VGPU_ORDER_KEY, pred, 1, 1)
+    x = gpu.DeviceColumn(*base["x"])
+    k = gpu.DeviceColumn(*base["g"])
+    b.plan([{"key": k, "inputs": [x], "pred_cols": [x], "n_rows": len(base["ax"]), "param": t} for t in thr])
+    for q, qd in enumerate(qs):
+        exp, close = oracle_query(gpu, qd, "/tmp")
+        assert_tables_equal(renamed(b.fetch(q)), exp, close)
+    new = [0.5, -1.0, 1.9, -0.1, 0.05]
+    b.rerun(new + [0.0] * (len(thr) - len(new)), n_active=len(new))
+    n_groups, _ = b.counts()
+    assert np.all(n_groups[len(new):] == 0)
+    for q, t in enumerate(new):
+        exp, close = oracle_query(gpu, dict(base, t=t), "/tmp")
+        assert_tables_equal(renamed(b.fetch(q)), exp, close)
+    b.free()
+    x.free()
+    k.free()
+
+
+def join_tables(seed, n_fact, n_dim, dup=True):
+    r = rng(seed)
+    dk = r.permutation(np.arange(n_dim, dtype=np.int64) * 5 + 11)
+    if dup:   # a few keys repeat on the build side
+        dk[: n_dim // 50] = dk[n_dim // 50: 2 * (n_dim // 50)]
+    attr = r.integers(0, 40, n_dim)
+    av = with_na(r, n_dim, 0.02)
+    fk = np.where(r.random(n_fact) < 0.9, dk[r.integers(0, n_dim, n_fact)], -7)
+    x = dbl_with_specials(r, n_fact, 0.01, 0.02)
+    fact = {"fk": (fk, with_na(r, n_fact, 0.01)), "x": (x, with_na(r, n_fact, 0.02)),
+            "ax": np.abs(np.nan_to_num(x, nan=0.0))}
+    dim = {"dk": dk, "attr": (attr, av)}
+    return fact, dim
+
+
+def oracle_join_query(nv, fact, dim, t, temp_dir):
+    f = nv.filter_node("oracle", nv.Table(fact, rg_rows=0).scan(), ("cmp", ">", ("col", "x"), ("dbl", t)))
+    j = nv.join_node("oracle", f, nv.Table(dim, rg_rows=0).scan(), "inner", [(0, 0)])
+    aggs = list(AGGS) + [("__sa", "sum", "ax", 1), ("__cnt", "count", "x", 0)]
+    out = nv.collect(nv.group_agg_node("oracle", j, ["attr"], aggs, temp_dir))
+    sa = out.pop("__sa")[0]
+    cnt = out.pop("__cnt")[0]
+    close = {name: (sa if kind == "sum" else sa / np.maximum(cnt, 1)) for name, kind, _, _ in AGGS if name in CLOSE}
+    out = {("g" if k == "attr" else k): v for k, v in out.items()}
+    return out, close
+
+
+@pytest.mark.parametrize("temp_dir,dup", [("/tmp", True), (None, True), ("/tmp", False)])
+def test_batch_join(gpu, oracle, temp_dir, dup):
+    """filter(x > t_q) |> inner_join(dim, fk = dk) |> group_by(attr): the
+    table stays resident across the queries; build keys that repeat"""
+    fact, dim = join_tables(29, 400_000, 20_000, dup)
+    thr = [-1.0, 0.0, 0.7, 1.5, -2.5, 0.2]
+    dkc = gpu.DeviceColumn(dim["dk"])
+    atc = gpu.DeviceColumn(*dim["attr"])
+    jt = gpu.JoinTable(dkc, [atc])
+    fkc = gpu.DeviceColumn(*fact["fk"])
+    xc = gpu.DeviceColumn(*fact["x"])
+    order = gpu.VGPU_ORDER_KEY if temp_dir else gpu.VGPU_ORDER_FIRST_SEEN
+    pred = gpu.cmp_const_prog(0, gpu.VEC_DOUBLE, "GT", 0.0)
+    b = gpu.Batch(gpu.VEC_INT64, [gpu.VEC_DOUBLE], batch_aggs(), order, pred, 1, 1)
+    b.join(jt, 0)
+    b.plan([{"probe_key": fkc, "inputs": [xc], "pred_cols": [xc], "n_rows": len(fact["ax"]), "param": t}
+            for t in thr])
+    for q, t in enumerate(thr):
+        exp, close = oracle_join_query(gpu, fact, dim, t, temp_dir)
+        assert_tables_equal(renamed(b.fetch(q)), exp, close)
+    b.rerun([0.4, -0.6], n_active=2)
+    for q, t in enumerate([0.4, -0.6]):
+        exp, close = oracle_join_query(gpu, fact, dim, t, temp_dir)
+        assert_tables_equal(renamed(b.fetch(q)), exp, close)
+    b.free()
+    jt.free()
+    for c in (dkc, atc, fkc, xc):
+        c.free()

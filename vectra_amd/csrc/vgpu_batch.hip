@@ -26,6 +26,7 @@
 // check on the device that their keys stay in that range and flag the
 // batch otherwise (vgpu_batch_counts reports it).
 #include "vgpu_groupby_core.h"
+#include "vgpu_join_table.h"
 
 #include <algorithm>
 #include <cmath>
@@ -37,10 +38,12 @@
 #define BT_LDS_BUDGET 61440        // dynamic LDS of the feed blocks
 #define BT_BLOCKS_PER_CU 8
 #define BT_U 4                     // rows in flight per thread in the feed
+#define BT_STILE 16384             // rows per tile of a shared scan
 #define BT_CTL_ERR 9               // batch control word: inputs left the plan's key range
 
 struct BtQuery {
     vgpu_col key;
+    vgpu_col probe_key;  // join batches: the probe side's join key
     InSet ins;
     ColSet pcs;
     PredInfo pi;
@@ -56,7 +59,18 @@ struct BtItem {
     uint32_t part;      // row tile (feed) or slot chunk (compaction)
 };
 
+// inner join of every query's probe key against one resident table; the
+// group key is an embedded payload column of the table's entries
+struct BtJoin {
+    JView jv;
+    int32_t on;
+    int32_t key_payload;   // payload column that is the group key
+    int32_t key_type;      // its type
+    int32_t pad;
+};
+
 struct BtArgs {
+    BtJoin jn;
     const BtQuery *qs;
     const vgpu_prog *progs;
     const BtItem *items;
@@ -66,6 +80,9 @@ struct BtArgs {
     uint64_t *ctl;      // GB_CTL_WORDS per query, then the batch's own GB_CTL_WORDS
     int32_t vm_lds;
     int32_t n_queries;
+    // shared scans: every query reads the same rows (see k_bt_feed_shared)
+    int32_t K, J;
+    int64_t n_tiles;
 };
 
 struct vgpu_batch {
@@ -96,6 +113,17 @@ struct vgpu_batch {
     int32_t counts_valid = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // join
+    BtJoin join;
+    int32_t join_key_type = 0;
+    vgpu_col join_payload;
+    int64_t join_rows = 0;
+    // shared scans
+    int32_t shared = 0;
+    int32_t n_active = 0;            // queries the shared feed runs (rerun may lower it)
+    int64_t n_tiles = 0;
+    std::vector<BtQuery> h_qs;
+    std::vector<vgpu_prog> h_progs;
 };
 
 // ---------------------------------------------------------------------------
@@ -157,6 +185,112 @@ __global__ void k_bt_init_ctl(uint64_t *ctl, int n_queries, uint64_t *qcount, un
     if (i == 0) *queue = 0;
 }
 
+// Fields of one query a feed block works on: column sets in LDS, scalars
+// in registers (block-uniform).
+struct BtLocal {
+    vgpu_col key, probe_key;
+    PredInfo pi;
+    int64_t n_rows, cap, tbase;
+    int32_t S, vm;
+};
+
+__device__ __forceinline__ BtLocal bt_load_query(const BtQuery &G, InSet &s_ins, ColSet &s_pcs) {
+    for (int w = threadIdx.x; w < (int)(sizeof(InSet) / 4); w += GB_THREADS)
+        reinterpret_cast<uint32_t *>(&s_ins)[w] = reinterpret_cast<const uint32_t *>(&G.ins)[w];
+    for (int w = threadIdx.x; w < (int)(sizeof(ColSet) / 4); w += GB_THREADS)
+        reinterpret_cast<uint32_t *>(&s_pcs)[w] = reinterpret_cast<const uint32_t *>(&G.pcs)[w];
+    BtLocal Q;
+    Q.key = G.key;
+    Q.probe_key = G.probe_key;
+    Q.pi = G.pi;
+    Q.n_rows = G.n_rows;
+    Q.cap = G.cap;
+    Q.tbase = G.base;
+    Q.S = G.lds_slots;
+    Q.vm = G.vm;
+    return Q;
+}
+
+// one group update: LDS table when the key has a slot there, else the
+// query's HBM table
+__device__ __forceinline__ void bt_update(const GbLayout &L, const InSet &ins, int64_t row, uint64_t pos, int cls,
+                                          uint64_t kn, int S, uint64_t *lhot, uint64_t *lcold, uint64_t *hot,
+                                          uint64_t *cold, int64_t cap, uint64_t *ctl, uint64_t empty) {
+    int64_t ls = S ? gb_lds_slot(lhot, S, L.RH, cls, kn, empty) : -1;
+    if (ls >= 0) {
+        gb_apply(L, ins, row, pos, lhot + (size_t)ls * L.RH, lcold + (size_t)ls * L.RC);
+    } else {
+        int64_t gs = gb_global_slot(L, hot, cap, cls, kn, pos, ctl);
+        if (gs >= 0) gb_apply(L, ins, row, pos, hot + (size_t)gs * L.RH, cold + (size_t)gs * L.RC);
+    }
+}
+
+// rows [r0, r1) of one query: BT_U rows per thread per step, their
+// predicate and key (or join probe) loads issued together so each wave
+// keeps several loads in flight
+__device__ __forceinline__ void bt_rows(const GbLayout &L, const BtArgs &B, const BtLocal &Q, const InSet &ins,
+                                        const ColSet &pcs, const vgpu_prog &prog, uint64_t *regs, uint8_t *regv,
+                                        int64_t r0, int64_t r1, uint64_t *lhot, uint64_t *lcold, uint64_t *hot,
+                                        uint64_t *cold, uint64_t *ctl) {
+    const uint64_t empty = L.key_is_double ? VG_EMPTY_F64 : VG_EMPTY_I64;
+    const int S = Q.S;
+    for (int64_t base = r0 + threadIdx.x; base < r1; base += (int64_t)GB_THREADS * BT_U) {
+        bool ok[BT_U];
+        int cls[BT_U];
+        uint64_t kn[BT_U];
+        const uint64_t *ent[BT_U];
+#pragma unroll
+        for (int u = 0; u < BT_U; u++) {
+            int64_t row = base + (int64_t)u * GB_THREADS;
+            ok[u] = row < r1 && vg_pred_row(Q.pi, prog, pcs, row, regs, regv, GB_THREADS, threadIdx.x);
+        }
+        if (B.jn.on) {
+#pragma unroll
+            for (int u = 0; u < BT_U; u++)
+                ent[u] = ok[u] ? jt_find(B.jn.jv, Q.probe_key, base + (int64_t)u * GB_THREADS) : nullptr;
+#pragma unroll
+            for (int u = 0; u < BT_U; u++) {
+                if (!ent[u]) continue;
+                int64_t row = base + (int64_t)u * GB_THREADS;
+                const uint64_t segw = ent[u][1];
+                const uint32_t cnt = jt_count(B.jn.jv, segw), off = (uint32_t)(segw >> 32);
+                const int p = B.jn.key_payload;
+                for (uint32_t j = 0; j < cnt; j++) {
+                    bool v;
+                    uint64_t bits;
+                    if (j == 0) {
+                        v = (segw >> (24 + p)) & 1;
+                        bits = ent[u][2 + p];
+                    } else {
+                        v = B.jn.jv.cval[p][off + j];
+                        bits = B.jn.jv.cpay[p][off + j];
+                    }
+                    uint64_t k;
+                    int c = vg_key_class_bits(B.jn.key_type, v, v ? bits : 0, k);
+                    uint64_t pos = ((uint64_t)row << 12) | (j < 0xFFFu ? j : 0xFFFu);
+                    if (L.key_is_double && c == VG_KEY_REGULAR && k == 0) gb_note_zero_key(ctl, bits, pos);
+                    bt_update(L, ins, row, pos, c, k, S, lhot, lcold, hot, cold, Q.cap, ctl, empty);
+                }
+            }
+            continue;
+        }
+#pragma unroll
+        for (int u = 0; u < BT_U; u++) {
+            kn[u] = 0;
+            cls[u] = VG_KEY_REGULAR;
+            if (ok[u] && !L.key_none) cls[u] = vg_key_class(Q.key, base + (int64_t)u * GB_THREADS, kn[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < BT_U; u++) {
+            if (!ok[u]) continue;
+            int64_t row = base + (int64_t)u * GB_THREADS;
+            if (L.key_is_double && cls[u] == VG_KEY_REGULAR && kn[u] == 0)
+                gb_note_zero_key(ctl, vg_load_bits(Q.key, row), (uint64_t)row);
+            bt_update(L, ins, row, (uint64_t)row, cls[u], kn[u], S, lhot, lcold, hot, cold, Q.cap, ctl, empty);
+        }
+    }
+}
+
 __global__ void __launch_bounds__(GB_THREADS) k_bt_feed(GbLayout L, BtArgs B) {
     extern __shared__ uint64_t dynw[];
     __shared__ long long s_item;
@@ -169,68 +303,66 @@ __global__ void __launch_bounds__(GB_THREADS) k_bt_feed(GbLayout L, BtArgs B) {
     for (;;) {
         if (threadIdx.x == 0) s_item = (long long)atomicAdd(B.queue, 1ull);
         __syncthreads();
-        // the item is block-uniform: its query's fields load once into
-        // scalar registers instead of being re-read from memory per row
+        // the item is block-uniform: its query's fields are read once per
+        // item (bt_load_query), not per row
         const int64_t it = (int64_t)__builtin_amdgcn_readfirstlane((int)s_item);
         if (it >= B.n_items) break;
         const BtItem item = B.items[it];
-        const BtQuery &G = B.qs[item.q];
-        // column sets to LDS, scalars to registers (block-uniform values)
-        for (int w = threadIdx.x; w < (int)(sizeof(InSet) / 4); w += GB_THREADS)
-            reinterpret_cast<uint32_t *>(&s_ins)[w] = reinterpret_cast<const uint32_t *>(&G.ins)[w];
-        for (int w = threadIdx.x; w < (int)(sizeof(ColSet) / 4); w += GB_THREADS)
-            reinterpret_cast<uint32_t *>(&s_pcs)[w] = reinterpret_cast<const uint32_t *>(&G.pcs)[w];
-        const vgpu_col key = G.key;
-        const PredInfo pi = G.pi;
-        const int64_t n_rows = G.n_rows, cap = G.cap, tbase = G.base;
-        const int S = G.lds_slots, vm = G.vm;
+        const BtLocal Q = bt_load_query(B.qs[item.q], s_ins, s_pcs);
         __syncthreads();
-        const vgpu_prog &prog = B.progs[vm ? item.q : 0];
-        uint8_t *regv = dyn + (size_t)(vm ? prog.n_insn : 0) * GB_THREADS * 8;
+        const int S = Q.S;
+        const vgpu_prog &prog = B.progs[Q.vm ? item.q : 0];
+        uint8_t *regv = dyn + (size_t)(Q.vm ? prog.n_insn : 0) * GB_THREADS * 8;
         uint64_t *lcold = lhot + (size_t)(S + 2) * L.RH;
-        uint64_t *hot = B.hot + tbase * L.RH;
-        uint64_t *cold = B.cold + tbase * L.RC;
+        uint64_t *hot = B.hot + Q.tbase * L.RH;
+        uint64_t *cold = B.cold + Q.tbase * L.RC;
         uint64_t *ctl = B.ctl + (size_t)item.q * GB_CTL_WORDS;
         if (S) gb_lds_init(L, lhot, lcold, S, empty);
         const int64_t r0 = (int64_t)item.part * BT_TILE;
-        const int64_t r1 = r0 + BT_TILE < n_rows ? r0 + BT_TILE : n_rows;
-        // BT_U rows per thread per step: their predicate and key loads are
-        // issued together, so each wave keeps several loads in flight
-        for (int64_t base = r0 + threadIdx.x; base < r1; base += (int64_t)GB_THREADS * BT_U) {
-            bool ok[BT_U];
-            int cls[BT_U];
-            uint64_t kn[BT_U];
-#pragma unroll
-            for (int u = 0; u < BT_U; u++) {
-                int64_t row = base + (int64_t)u * GB_THREADS;
-                ok[u] = row < r1 && vg_pred_row(pi, prog, s_pcs, row, regs, regv, GB_THREADS, threadIdx.x);
-            }
-#pragma unroll
-            for (int u = 0; u < BT_U; u++) {
-                int64_t row = base + (int64_t)u * GB_THREADS;
-                kn[u] = 0;
-                cls[u] = VG_KEY_REGULAR;
-                if (ok[u] && !L.key_none) cls[u] = vg_key_class(key, row, kn[u]);
-            }
-#pragma unroll
-            for (int u = 0; u < BT_U; u++) {
-                if (!ok[u]) continue;
-                int64_t row = base + (int64_t)u * GB_THREADS;
-                if (L.key_is_double && cls[u] == VG_KEY_REGULAR && kn[u] == 0)
-                    gb_note_zero_key(ctl, vg_load_bits(key, row), (uint64_t)row);
-                int64_t ls = S ? gb_lds_slot(lhot, S, L.RH, cls[u], kn[u], empty) : -1;
-                if (ls >= 0) {
-                    gb_apply(L, s_ins, row, (uint64_t)row, lhot + (size_t)ls * L.RH, lcold + (size_t)ls * L.RC);
-                } else {
-                    int64_t gs = gb_global_slot(L, hot, cap, cls[u], kn[u], (uint64_t)row, ctl);
-                    if (gs >= 0)
-                        gb_apply(L, s_ins, row, (uint64_t)row, hot + (size_t)gs * L.RH, cold + (size_t)gs * L.RC);
-                }
-            }
-        }
-        if (S) gb_lds_merge(L, lhot, lcold, S, empty, hot, cold, cap, ctl);
+        const int64_t r1 = r0 + BT_TILE < Q.n_rows ? r0 + BT_TILE : Q.n_rows;
+        bt_rows(L, B, Q, s_ins, s_pcs, prog, regs, regv, r0, r1, lhot, lcold, hot, cold, ctl);
+        if (S) gb_lds_merge(L, lhot, lcold, S, empty, hot, cold, Q.cap, ctl);
         __syncthreads();
     }
+}
+
+// Shared scans: every query reads the same rows.  Each block keeps one
+// query for the whole launch (its LDS table is initialized and merged
+// once) and walks a strided sequence of row tiles; the blocks of all K
+// queries on one XCD (blocks are dispatched to XCDs round-robin by index)
+// walk the same tiles at the same time, so a tile comes from HBM once per
+// XCD and is re-read by the other queries' blocks from that XCD's L2.
+#define BT_XCDS 8
+
+__global__ void __launch_bounds__(GB_THREADS) k_bt_feed_shared(GbLayout L, BtArgs B) {
+    extern __shared__ uint64_t dynw[];
+    __shared__ InSet s_ins;
+    __shared__ ColSet s_pcs;
+    uint8_t *dyn = reinterpret_cast<uint8_t *>(dynw);
+    uint64_t *regs = dynw;
+    const uint64_t empty = L.key_is_double ? VG_EMPTY_F64 : VG_EMPTY_I64;
+    uint64_t *lhot = reinterpret_cast<uint64_t *>(dyn + B.vm_lds);
+    const int xcd = blockIdx.x % BT_XCDS, local = blockIdx.x / BT_XCDS;
+    const int q = local % B.K, j = local / B.K;
+    if (q >= B.n_queries || j >= B.J) return;
+    const BtLocal Q = bt_load_query(B.qs[q], s_ins, s_pcs);
+    __syncthreads();
+    const int S = Q.S;
+    const vgpu_prog &prog = B.progs[Q.vm ? q : 0];
+    uint8_t *regv = dyn + (size_t)(Q.vm ? prog.n_insn : 0) * GB_THREADS * 8;
+    uint64_t *lcold = lhot + (size_t)(S + 2) * L.RH;
+    uint64_t *hot = B.hot + Q.tbase * L.RH;
+    uint64_t *cold = B.cold + Q.tbase * L.RC;
+    uint64_t *ctl = B.ctl + (size_t)q * GB_CTL_WORDS;
+    if (S) gb_lds_init(L, lhot, lcold, S, empty);
+    for (int64_t i = 0;; i++) {
+        const int64_t t = xcd + (int64_t)BT_XCDS * (j + (int64_t)B.J * i);
+        if (t >= B.n_tiles) break;
+        const int64_t r0 = t * BT_STILE;
+        const int64_t r1 = r0 + BT_STILE < Q.n_rows ? r0 + BT_STILE : Q.n_rows;
+        bt_rows(L, B, Q, s_ins, s_pcs, prog, regs, regv, r0, r1, lhot, lcold, hot, cold, ctl);
+    }
+    if (S) gb_lds_merge(L, lhot, lcold, S, empty, hot, cold, Q.cap, ctl);
 }
 
 // sort entry of arena record (query q, table slot s): like gb_sort_entry,
@@ -426,7 +558,33 @@ extern "C" vgpu_batch *vgpu_batch_create(int key_type, int n_inputs, const int32
     }
     b->out_vals.resize(n_aggs);
     b->out_valid.resize(n_aggs);
+    memset(&b->join, 0, sizeof b->join);
+    memset(&b->join_payload, 0, sizeof b->join_payload);
     return b;
+}
+
+extern "C" int vgpu_batch_join(vgpu_batch *b, const vgpu_jtable *t, int key_payload) {
+    if (b->planned) {
+        vg_set_error("batch: join after plan");
+        return -1;
+    }
+    if (!t || key_payload < 0 || key_payload >= t->n_payload) {
+        vg_set_error("batch: the group key must be an embedded payload column of the join table");
+        return -1;
+    }
+    if (b->L.key_none || t->payload[key_payload].type != b->key_type) {
+        vg_set_error("batch: payload column %d has type %d, the batch groups by type %d", key_payload,
+                     t->payload[key_payload].type, b->key_type);
+        return -1;
+    }
+    b->join.jv = t->view();
+    b->join.on = 1;
+    b->join.key_payload = key_payload;
+    b->join.key_type = b->key_type;
+    b->join_key_type = t->key_type;
+    b->join_payload = t->payload[key_payload];
+    b->join_rows = t->n_build;
+    return 0;
 }
 
 static int bt_enqueue(vgpu_batch *b, bool first);
@@ -467,7 +625,14 @@ extern "C" int vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_q
             vg_set_error("batch: query %d has %lld rows", q, (long long)in.n_rows);
             return -1;
         }
-        if (!L.key_none) {
+        if (b->join.on) {
+            if (in.probe_key.type != b->join_key_type) {
+                vg_set_error("batch: query %d probe key type %d differs from the join table's %d", q,
+                             in.probe_key.type, b->join_key_type);
+                return -1;
+            }
+            Q.probe_key = in.probe_key;
+        } else if (!L.key_none) {
             if (in.key.type != b->key_type) {
                 vg_set_error("batch: query %d key type %d differs from declared %d", q, in.key.type, b->key_type);
                 return -1;
@@ -501,6 +666,18 @@ extern "C" int vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_q
     }
     b->n_queries = n_queries;
     b->n_items = (int64_t)items.size();
+    // a shared scan when every query reads the same columns
+    auto same_col = [](const vgpu_col &a, const vgpu_col &c) { return a.data == c.data && a.validity == c.validity; };
+    bool shared = n_queries > 1;
+    for (int q = 1; q < n_queries && shared; q++) {
+        const BtQuery &A = qs[0], &C2 = qs[q];
+        shared = A.n_rows == C2.n_rows && same_col(A.key, C2.key) && same_col(A.probe_key, C2.probe_key);
+        for (int i = 0; i < L.n_inputs && shared; i++) shared = same_col(A.ins.c[i], C2.ins.c[i]);
+        for (int i = 0; i < b->n_pred_cols && shared; i++) shared = same_col(A.pcs.c[i], C2.pcs.c[i]);
+    }
+    b->shared = shared;
+    b->n_active = n_queries;
+    b->n_tiles = shared ? (qs[0].n_rows + BT_STILE - 1) / BT_STILE : 0;
     size_t vm = any_vm ? (vg_vm_lds_bytes(&b->prog, GB_THREADS) + 15) & ~(size_t)15 : 0;
     b->vm_lds = (int32_t)vm;
     if (b->qs.reserve(qs.size() * sizeof(BtQuery)) || b->progs.reserve(progs.size() * sizeof(vgpu_prog)) ||
@@ -513,32 +690,56 @@ extern "C" int vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_q
     if (!items.empty())
         VG_TRY(hipMemcpyAsync(b->items.ptr, items.data(), items.size() * sizeof(BtItem), hipMemcpyHostToDevice, st));
 
-    // 1. sketch every query's keys (one launch), size the tables
+    // 1. sketch every query's keys (one launch), size the tables; a join
+    // batch sketches the group-key payload column of the build side once, a
+    // shared scan the one key column once
     std::vector<int64_t> est((size_t)n_queries, 1);
-    if (!L.key_none && b->n_items > 0) {
-        DevBuf sk;
-        size_t regs = (size_t)n_queries * BT_SK_REGS;
-        if (sk.reserve(regs * 4 + (size_t)n_queries * 8)) return -1;
+    std::vector<BtQuery> sq;      // the columns the sketch reads
+    std::vector<BtItem> sitems;
+    if (b->join.on) {
+        BtQuery P;
+        memset(&P, 0, sizeof P);
+        P.key = b->join_payload;
+        P.n_rows = b->join_rows;
+        sq.push_back(P);
+    } else if (shared) {
+        sq.push_back(qs[0]);
+    } else {
+        sq = qs;
+    }
+    for (int q = 0; q < (int)sq.size(); q++)
+        for (int64_t t = 0; t * BT_TILE < sq[q].n_rows; t++) sitems.push_back(BtItem{(uint32_t)q, (uint32_t)t});
+    if (!L.key_none && !sitems.empty()) {
+        const int nq = (int)sq.size();
+        const int64_t ni = (int64_t)sitems.size();
+        DevBuf sk, dq, di;
+        size_t regs = (size_t)nq * BT_SK_REGS;
+        if (sk.reserve(regs * 4 + (size_t)nq * 8) || dq.reserve(sq.size() * sizeof(BtQuery)) ||
+            di.reserve(sitems.size() * sizeof(BtItem)))
+            return -1;
         uint32_t *d_regs = sk.as<uint32_t>();
         auto *d_nan = reinterpret_cast<unsigned long long *>(d_regs + regs);
-        k_bt_clear_u32<<<vg_blocks((int64_t)regs + 2 * n_queries, 256), 256, 0, st>>>(d_regs,
-                                                                                      (int64_t)regs + 2 * n_queries);
+        k_bt_clear_u32<<<vg_blocks((int64_t)regs + 2 * nq, 256), 256, 0, st>>>(d_regs, (int64_t)regs + 2 * nq);
         VG_LAUNCH_CHECK();
-        VG_TRY(hipMemcpyAsync(b->qs.ptr, qs.data(), qs.size() * sizeof(BtQuery), hipMemcpyHostToDevice, st));
+        VG_TRY(hipMemcpyAsync(dq.ptr, sq.data(), sq.size() * sizeof(BtQuery), hipMemcpyHostToDevice, st));
+        VG_TRY(hipMemcpyAsync(di.ptr, sitems.data(), sitems.size() * sizeof(BtItem), hipMemcpyHostToDevice, st));
         BtArgs A;
         memset(&A, 0, sizeof A);
-        A.qs = b->qs.as<BtQuery>();
-        A.items = b->items.as<BtItem>();
-        A.n_items = b->n_items;
-        k_bt_sketch<<<vg_blocks(b->n_items, 1, 4096), GB_THREADS, 0, st>>>(A, L.key_none, d_regs, d_nan);
+        A.qs = dq.as<BtQuery>();
+        A.items = di.as<BtItem>();
+        A.n_items = ni;
+        k_bt_sketch<<<vg_blocks(ni, 1, 4096), GB_THREADS, 0, st>>>(A, L.key_none, d_regs, d_nan);
         VG_LAUNCH_CHECK();
         std::vector<uint32_t> h(regs);
-        std::vector<uint64_t> hn((size_t)n_queries);
+        std::vector<uint64_t> hn((size_t)nq);
         VG_TRY(hipMemcpyAsync(h.data(), d_regs, regs * 4, hipMemcpyDeviceToHost, st));
-        VG_TRY(hipMemcpyAsync(hn.data(), d_nan, (size_t)n_queries * 8, hipMemcpyDeviceToHost, st));
+        VG_TRY(hipMemcpyAsync(hn.data(), d_nan, (size_t)nq * 8, hipMemcpyDeviceToHost, st));
         VG_TRY(hipStreamSynchronize(st));
         sk.release();
-        for (int q = 0; q < n_queries; q++) est[q] = bt_estimate(&h[(size_t)q * BT_SK_REGS], hn[q]);
+        dq.release();
+        di.release();
+        for (int q = 0; q < n_queries; q++)
+            est[q] = bt_estimate(&h[(size_t)(nq == n_queries ? q : 0) * BT_SK_REGS], hn[nq == n_queries ? q : 0]);
     }
     const size_t rw = (size_t)(L.RH + L.RC) * 8;
     std::vector<uint64_t> qbase((size_t)n_queries + 1);
@@ -547,7 +748,9 @@ extern "C" int vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_q
     int S_max = 0;
     for (int q = 0; q < n_queries; q++) {
         BtQuery &Q = qs[q];
-        int64_t g = std::min<int64_t>(std::max<int64_t>(Q.n_rows, 1), est[q] + est[q] / 4 + 16);
+        int64_t bound = b->join.on ? std::max<int64_t>(Q.n_rows, 1) * std::max<int64_t>(b->join_rows, 1)
+                                   : std::max<int64_t>(Q.n_rows, 1);
+        int64_t g = std::min<int64_t>(bound, est[q] + est[q] / 4 + 16);
         Q.cap = pow2_at_least(2 * g);
         Q.base = slots;
         qbase[q] = (uint64_t)slots;
@@ -556,7 +759,8 @@ extern "C" int vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_q
         int64_t S = pow2_at_least(2 * est[q]);
         if (L.key_none) S = 64;
         Q.lds_slots = 0;
-        if ((size_t)(S + 2) * rw + vm <= BT_LDS_BUDGET && S * 8 <= std::min<int64_t>(Q.n_rows, BT_TILE)) {
+        int64_t tile = shared ? (int64_t)BT_STILE * 64 : (int64_t)BT_TILE;   // rows one LDS table sees
+        if ((size_t)(S + 2) * rw + vm <= BT_LDS_BUDGET && S * 8 <= std::min<int64_t>(Q.n_rows, tile)) {
             Q.lds_slots = (int32_t)S;
             S_max = std::max<int>(S_max, (int)S);
         }
@@ -585,6 +789,8 @@ extern "C" int vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_q
         return -1;
     VG_TRY(hipMemcpyAsync(b->qs.ptr, qs.data(), qs.size() * sizeof(BtQuery), hipMemcpyHostToDevice, st));
     VG_TRY(hipMemcpyAsync(b->cmap.ptr, cmap.data(), cmap.size() * sizeof(BtItem), hipMemcpyHostToDevice, st));
+    b->h_qs = qs;
+    b->h_progs = progs;
     VG_TRY(hipMemcpyAsync(b->qbase.ptr, qbase.data(), qbase.size() * 8, hipMemcpyHostToDevice, st));
 
     // 2. the first run fixes the sort range and the record count
@@ -610,6 +816,10 @@ static int bt_enqueue(vgpu_batch *b, bool first) {
     A.ctl = b->ctl.as<uint64_t>();
     A.vm_lds = b->vm_lds;
     A.n_queries = b->n_queries;
+    A.jn = b->join;
+    A.K = b->n_active;
+    A.J = std::max<int>(1, (int)(b->feed_grid / ((int64_t)BT_XCDS * std::max(b->n_active, 1))));
+    A.n_tiles = b->n_tiles;
     uint64_t *bctl = A.ctl + (size_t)b->n_queries * GB_CTL_WORDS;
     uint64_t *qcount = b->qcount.as<uint64_t>();
     const uint64_t empty = L.key_is_double ? VG_EMPTY_F64 : VG_EMPTY_I64;
@@ -626,7 +836,10 @@ static int bt_enqueue(vgpu_batch *b, bool first) {
     if (b->n_items > 0) {
         size_t lds = (size_t)b->vm_lds + (b->S_max ? (size_t)(b->S_max + 2) * (L.RH + L.RC) * 8 : 0);
         vg_kt_begin();
-        k_bt_feed<<<(unsigned)b->feed_grid, GB_THREADS, lds, st>>>(L, A);
+        if (b->shared)
+            k_bt_feed_shared<<<(unsigned)((int64_t)BT_XCDS * A.K * A.J), GB_THREADS, lds, st>>>(L, A);
+        else
+            k_bt_feed<<<(unsigned)b->feed_grid, GB_THREADS, lds, st>>>(L, A);
         VG_LAUNCH_CHECK();
         vg_kt_end();
     }
@@ -711,6 +924,37 @@ extern "C" int vgpu_batch_run(vgpu_batch *b) {
         return 0;
     }
     return bt_enqueue(b, false);
+}
+
+extern "C" int vgpu_batch_rerun(vgpu_batch *b, const uint64_t *params, int n_active) {
+    if (!b->planned) {
+        vg_set_error("batch: rerun before plan");
+        return -1;
+    }
+    if (n_active < 1 || n_active > b->n_queries || (!b->shared && n_active != b->n_queries)) {
+        vg_set_error("batch: %d active queries of %d (fewer only when every query reads the same rows)", n_active,
+                     b->n_queries);
+        return -1;
+    }
+    // a rerun re-derives the sort range and the output sizes: replays of
+    // an earlier capture no longer apply
+    if (b->exec) {
+        (void)hipGraphExecDestroy(b->exec);
+        (void)hipGraphDestroy(b->graph);
+        b->exec = nullptr;
+        b->graph = nullptr;
+    }
+    if (params && b->has_pred && b->param_insn >= 0)
+        for (int q = 0; q < n_active; q++) {
+            memcpy(&b->h_progs[q].insn[b->param_insn].imm, &params[q], 8);
+            b->h_qs[q].pi = vg_pred_info(&b->h_progs[q]);
+        }
+    hipStream_t st = vg_stream();
+    VG_TRY(hipMemcpyAsync(b->qs.ptr, b->h_qs.data(), b->h_qs.size() * sizeof(BtQuery), hipMemcpyHostToDevice, st));
+    VG_TRY(hipMemcpyAsync(b->progs.ptr, b->h_progs.data(), b->h_progs.size() * sizeof(vgpu_prog),
+                          hipMemcpyHostToDevice, st));
+    b->n_active = n_active;
+    return bt_enqueue(b, true);
 }
 
 extern "C" int vgpu_batch_capture(vgpu_batch *b) {

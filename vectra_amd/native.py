@@ -118,8 +118,8 @@ class VgpuInput(C.Structure):
 
 
 class VgpuQuery(C.Structure):
-    _fields_ = [("key", VgpuCol), ("inputs", C.POINTER(VgpuCol)), ("pred_cols", C.POINTER(VgpuCol)),
-                ("n_rows", C.c_int64), ("param", C.c_uint64)]
+    _fields_ = [("key", VgpuCol), ("probe_key", VgpuCol), ("inputs", C.POINTER(VgpuCol)),
+                ("pred_cols", C.POINTER(VgpuCol)), ("n_rows", C.c_int64), ("param", C.c_uint64)]
 
 
 class VgpuFeedSpec(C.Structure):
@@ -242,6 +242,8 @@ def vgpu() -> C.CDLL:
                                                       C.c_int])
         _proto(lib, "vgpu_batch_plan", C.c_int, [C.c_void_p, C.POINTER(VgpuQuery), C.c_int])
         _proto(lib, "vgpu_batch_run", C.c_int, [C.c_void_p])
+        _proto(lib, "vgpu_batch_join", C.c_int, [C.c_void_p, C.c_void_p, C.c_int])
+        _proto(lib, "vgpu_batch_rerun", C.c_int, [C.c_void_p, C.c_void_p, C.c_int])
         _proto(lib, "vgpu_batch_capture", C.c_int, [C.c_void_p])
         _proto(lib, "vgpu_batch_counts", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p])
         _proto(lib, "vgpu_batch_fetch", C.c_int64, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
@@ -853,6 +855,8 @@ class Batch:
         for i, q in enumerate(queries):
             if q.get("key") is not None:
                 qa[i].key = q["key"].col
+            if q.get("probe_key") is not None:
+                qa[i].probe_key = q["probe_key"].col
             ins = col_array(q.get("inputs", ()))
             pcs = col_array(q.get("pred_cols", ()))
             keep += [ins, pcs]
@@ -865,8 +869,16 @@ class Batch:
         self._keep = (qa, keep)
         _vcheck(vgpu().vgpu_batch_plan(self.h, qa, len(queries)))
 
+    def join(self, table: "JoinTable", key_payload: int) -> None:
+        self._join = table
+        _vcheck(vgpu().vgpu_batch_join(self.h, table.h, key_payload))
+
     def run(self) -> None:
         _vcheck(vgpu().vgpu_batch_run(self.h))
+
+    def rerun(self, params: Sequence[float], n_active: int | None = None) -> None:
+        p = np.asarray(params, np.float64).view(np.uint64)
+        _vcheck(vgpu().vgpu_batch_rerun(self.h, p.ctypes.data, len(p) if n_active is None else n_active))
 
     def capture(self) -> None:
         _vcheck(vgpu().vgpu_batch_capture(self.h))
