@@ -326,6 +326,8 @@ int     vgpu_batch_capture(vgpu_batch *b);
 int     vgpu_batch_counts(vgpu_batch *b, int64_t *n_groups, int64_t *offsets);
 int64_t vgpu_batch_fetch(vgpu_batch *b, int q, void *keys, uint8_t *key_validity,
                          double *const *agg_values, uint8_t *const *agg_validity);
+/* device bytes the batch holds (tables, sort and output buffers) */
+int64_t vgpu_batch_device_bytes(const vgpu_batch *b);
 void    vgpu_batch_destroy(vgpu_batch *b);
 
 /* ---- tuning -----------------------------------------------------------
@@ -336,6 +338,8 @@ void    vgpu_batch_destroy(vgpu_batch *b);
  *   "gb.star.items" (4: rows in flight per thread of the fused star probe)
  *   "gb.part.min_groups" (65536: groups from which a first feed of at
  *                         least 2^20 rows is hash-partitioned)
+ *   "batch.shared_scan"  (1: off; 2: batches whose queries all read the
+ *                         same columns share one scan of them)
  * Returns -1 for an unknown name or a value <= 0. */
 int     vgpu_tune_set(const char *name, int64_t value);
 int64_t vgpu_tune_get(const char *name);

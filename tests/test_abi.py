@@ -42,6 +42,41 @@ def test_node_exports(native):
     assert not missing, missing
 
 
+def test_serve_exports(native):
+    lib = C.CDLL(str(ROOT / "vectra_amd" / "lib" / "libvectra_gpu.so"))
+    names = declared("vx_serve.h")
+    assert {"vx_serve_run", "vx_result_free"} <= names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_serve_struct_layouts(native, tmp_path):
+    """the serving loop's structs as ctypes sees them match the C compiler"""
+    from vectra_amd import native as nv
+    src = tmp_path / "l.c"
+    src.write_text(r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "vx_serve.h"
+int main(void) {
+  printf("%zu %zu %zu %zu\n", sizeof(vx_request), sizeof(vx_result), sizeof(vx_serve_cfg), sizeof(vx_serve_stats));
+  printf("%zu %zu %zu %zu\n", offsetof(vx_result, n_groups), offsetof(vx_serve_cfg, n_dim),
+         offsetof(vx_serve_cfg, state_budget), offsetof(vx_serve_stats, resident_bytes));
+  printf("%zu %zu\n", sizeof(vgpu_query), offsetof(vgpu_query, n_rows));
+  return 0;
+}
+""")
+    exe = tmp_path / "l"
+    subprocess.run(["gcc", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    got = [list(map(int, line.split())) for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                                   check=True).stdout.split("\n") if line]
+    assert got[0] == [C.sizeof(nv.VxRequest), C.sizeof(nv.VxResult), C.sizeof(nv.VxServeCfg),
+                      C.sizeof(nv.VxServeStats)]
+    assert got[1] == [nv.VxResult.n_groups.offset, nv.VxServeCfg.n_dim.offset, nv.VxServeCfg.state_budget.offset,
+                      nv.VxServeStats.resident_bytes.offset]
+    assert got[2] == [C.sizeof(nv.VgpuQuery), nv.VgpuQuery.n_rows.offset]
+
+
 def test_struct_layouts(native, tmp_path):
     """offsets of the fields the optimizer / explain read, compiled by gcc"""
     src = tmp_path / "lay.c"

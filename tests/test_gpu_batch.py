@@ -208,9 +208,21 @@ def shared_table(seed, n, card):
     return {"g": (kv, with_na(r, n, 0.02)), "x": (x, with_na(r, n, 0.02)), "ax": np.abs(np.nan_to_num(x, nan=0.0))}
 
 
-def test_batch_shared_scan_and_rerun(gpu, oracle):
-    """queries over the same columns share one scan; a rerun with new
-    constants on fewer queries matches the oracle too"""
+@pytest.mark.parametrize("shared", [1, 2])
+def test_batch_shared_scan_and_rerun(gpu, oracle, shared):
+    """queries over the same columns (one scan of them shared when the
+    batch.shared_scan knob is 2); a rerun with new constants on fewer
+    queries matches the oracle too"""
+    lib = gpu.vgpu()
+    old = lib.vgpu_tune_get(b"batch.shared_scan")
+    lib.vgpu_tune_set(b"batch.shared_scan", shared)
+    try:
+        _shared_scan_and_rerun(gpu)
+    finally:
+        lib.vgpu_tune_set(b"batch.shared_scan", old)
+
+
+def _shared_scan_and_rerun(gpu):
     base = shared_table(23, 700_000, 900)
     thr = [-1.5, -0.5, 0.0, 0.3, 0.9, 1.7, -3.0, 2.5, 0.1, -0.2, 0.6, 1.1]
     qs = [dict(base, t=t) for t in thr]
@@ -261,8 +273,19 @@ def oracle_join_query(nv, fact, dim, t, temp_dir):
     return out, close
 
 
-@pytest.mark.parametrize("temp_dir,dup", [("/tmp", True), (None, True), ("/tmp", False)])
-def test_batch_join(gpu, oracle, temp_dir, dup):
+@pytest.mark.parametrize("temp_dir,dup,shared", [("/tmp", True, 1), (None, True, 1), ("/tmp", False, 1),
+                                                 ("/tmp", True, 2)])
+def test_batch_join(gpu, oracle, temp_dir, dup, shared):
+    lib = gpu.vgpu()
+    old = lib.vgpu_tune_get(b"batch.shared_scan")
+    lib.vgpu_tune_set(b"batch.shared_scan", shared)
+    try:
+        _batch_join(gpu, temp_dir, dup)
+    finally:
+        lib.vgpu_tune_set(b"batch.shared_scan", old)
+
+
+def _batch_join(gpu, temp_dir, dup):
     """filter(x > t_q) |> inner_join(dim, fk = dk) |> group_by(attr): the
     table stays resident across the queries; build keys that repeat"""
     fact, dim = join_tables(29, 400_000, 20_000, dup)

@@ -32,7 +32,8 @@
 #include <cmath>
 #include <vector>
 
-#define BT_TILE 32768              // rows per work item
+#define BT_TILE_MIN 32768          // rows per work item: at least ...
+#define BT_TILE_MAX (1 << 21)      // ... and at most; chosen per batch from its total rows
 #define BT_SK_BITS 10              // sketch registers per query: 2^10
 #define BT_SK_REGS (1 << BT_SK_BITS)
 #define BT_LDS_BUDGET 61440        // dynamic LDS of the feed blocks
@@ -83,6 +84,9 @@ struct BtArgs {
     // shared scans: every query reads the same rows (see k_bt_feed_shared)
     int32_t K, J;
     int64_t n_tiles;
+    int32_t lds_cold;   // LDS tables carry cold words (some input has NAs or a double min / max)
+    int32_t n_active;   // queries a run feeds (the first n_active)
+    int64_t tile;       // rows per work item
 };
 
 struct vgpu_batch {
@@ -100,7 +104,10 @@ struct vgpu_batch {
     int64_t n_items = 0, n_cblocks = 0;
     int32_t S_max = 0;
     int32_t vm_lds = 0;
+    int32_t lds_cold = 1;
     int64_t feed_grid = 1;          // persistent feed blocks: a few per CU
+    int64_t shared_resident = 1;    // feed blocks resident at once (LDS and wave limits)
+    int64_t tile = BT_TILE_MIN;     // rows per work item
     int32_t planned = 0;
     int64_t n_pairs = 0;            // sorted records of the plan's run
     int32_t lo = 0, hi = 0, qbits = 0;
@@ -142,8 +149,8 @@ k_bt_sketch(BtArgs B, int key_none, uint32_t *regs, unsigned long long *nan_rows
         __syncthreads();
         const BtItem item = B.items[it];
         const BtQuery &Q = B.qs[item.q];
-        const int64_t r0 = (int64_t)item.part * BT_TILE;
-        const int64_t r1 = r0 + BT_TILE < Q.n_rows ? r0 + BT_TILE : Q.n_rows;
+        const int64_t r0 = (int64_t)item.part * B.tile;
+        const int64_t r1 = r0 + B.tile < Q.n_rows ? r0 + B.tile : Q.n_rows;
         unsigned long long my_nan = 0;
         if (!key_none)
             for (int64_t row = r0 + threadIdx.x; row < r1; row += GB_THREADS) {
@@ -218,7 +225,7 @@ __device__ __forceinline__ void bt_update(const GbLayout &L, const InSet &ins, i
                                           uint64_t *cold, int64_t cap, uint64_t *ctl, uint64_t empty) {
     int64_t ls = S ? gb_lds_slot(lhot, S, L.RH, cls, kn, empty) : -1;
     if (ls >= 0) {
-        gb_apply(L, ins, row, pos, lhot + (size_t)ls * L.RH, lcold + (size_t)ls * L.RC);
+        gb_apply(L, ins, row, pos, lhot + (size_t)ls * L.RH, lcold ? lcold + (size_t)ls * L.RC : nullptr);
     } else {
         int64_t gs = gb_global_slot(L, hot, cap, cls, kn, pos, ctl);
         if (gs >= 0) gb_apply(L, ins, row, pos, hot + (size_t)gs * L.RH, cold + (size_t)gs * L.RC);
@@ -300,30 +307,43 @@ __global__ void __launch_bounds__(GB_THREADS) k_bt_feed(GbLayout L, BtArgs B) {
     uint64_t *regs = dynw;
     const uint64_t empty = L.key_is_double ? VG_EMPTY_F64 : VG_EMPTY_I64;
     uint64_t *lhot = reinterpret_cast<uint64_t *>(dyn + B.vm_lds);
+    // The block keeps its LDS table across consecutive items of one query
+    // (items come query-major off the queue) and merges it into the query's
+    // HBM table only when the query changes: once per block and query
+    // rather than once per item.
+    int cur = -1;
+    BtLocal Q;
+    uint64_t *lcold = nullptr, *hot = nullptr, *cold = nullptr, *ctl = nullptr;
     for (;;) {
         if (threadIdx.x == 0) s_item = (long long)atomicAdd(B.queue, 1ull);
         __syncthreads();
-        // the item is block-uniform: its query's fields are read once per
-        // item (bt_load_query), not per row
+        // the item is block-uniform
         const int64_t it = (int64_t)__builtin_amdgcn_readfirstlane((int)s_item);
+        __syncthreads();   // every wave has read s_item before thread 0 may overwrite it
         if (it >= B.n_items) break;
         const BtItem item = B.items[it];
-        const BtLocal Q = bt_load_query(B.qs[item.q], s_ins, s_pcs);
-        __syncthreads();
-        const int S = Q.S;
-        const vgpu_prog &prog = B.progs[Q.vm ? item.q : 0];
-        uint8_t *regv = dyn + (size_t)(Q.vm ? prog.n_insn : 0) * GB_THREADS * 8;
-        uint64_t *lcold = lhot + (size_t)(S + 2) * L.RH;
-        uint64_t *hot = B.hot + Q.tbase * L.RH;
-        uint64_t *cold = B.cold + Q.tbase * L.RC;
-        uint64_t *ctl = B.ctl + (size_t)item.q * GB_CTL_WORDS;
-        if (S) gb_lds_init(L, lhot, lcold, S, empty);
-        const int64_t r0 = (int64_t)item.part * BT_TILE;
-        const int64_t r1 = r0 + BT_TILE < Q.n_rows ? r0 + BT_TILE : Q.n_rows;
-        bt_rows(L, B, Q, s_ins, s_pcs, prog, regs, regv, r0, r1, lhot, lcold, hot, cold, ctl);
-        if (S) gb_lds_merge(L, lhot, lcold, S, empty, hot, cold, Q.cap, ctl);
+        if ((int)item.q >= B.n_active) continue;   // a rerun of fewer queries: block-uniform skip
+        if ((int)item.q != cur) {
+            if (cur >= 0 && Q.S) gb_lds_merge(L, lhot, lcold, Q.S, empty, hot, cold, Q.cap, ctl);
+            __syncthreads();
+            Q = bt_load_query(B.qs[item.q], s_ins, s_pcs);
+            __syncthreads();
+            cur = (int)item.q;
+            lcold = B.lds_cold ? lhot + (size_t)(Q.S + 2) * L.RH : nullptr;
+            hot = B.hot + Q.tbase * L.RH;
+            cold = B.cold + Q.tbase * L.RC;
+            ctl = B.ctl + (size_t)cur * GB_CTL_WORDS;
+            if (Q.S) gb_lds_init(L, lhot, lcold, Q.S, empty);
+        }
+        const BtLocal Qi = Q;   // this item's copy stays in registers through the row loop
+        const vgpu_prog &prog = B.progs[Qi.vm ? cur : 0];
+        uint8_t *regv = dyn + (size_t)(Qi.vm ? prog.n_insn : 0) * GB_THREADS * 8;
+        const int64_t r0 = (int64_t)item.part * B.tile;
+        const int64_t r1 = r0 + B.tile < Qi.n_rows ? r0 + B.tile : Qi.n_rows;
+        bt_rows(L, B, Qi, s_ins, s_pcs, prog, regs, regv, r0, r1, lhot, lcold, hot, cold, ctl);
         __syncthreads();
     }
+    if (cur >= 0 && Q.S) gb_lds_merge(L, lhot, lcold, Q.S, empty, hot, cold, Q.cap, ctl);
 }
 
 // Shared scans: every query reads the same rows.  Each block keeps one
@@ -350,7 +370,7 @@ __global__ void __launch_bounds__(GB_THREADS) k_bt_feed_shared(GbLayout L, BtArg
     const int S = Q.S;
     const vgpu_prog &prog = B.progs[Q.vm ? q : 0];
     uint8_t *regv = dyn + (size_t)(Q.vm ? prog.n_insn : 0) * GB_THREADS * 8;
-    uint64_t *lcold = lhot + (size_t)(S + 2) * L.RH;
+    uint64_t *lcold = B.lds_cold ? lhot + (size_t)(S + 2) * L.RH : nullptr;
     uint64_t *hot = B.hot + Q.tbase * L.RH;
     uint64_t *cold = B.cold + Q.tbase * L.RC;
     uint64_t *ctl = B.ctl + (size_t)q * GB_CTL_WORDS;
@@ -617,11 +637,24 @@ extern "C" int vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_q
     std::vector<vgpu_prog> progs((size_t)n_queries);
     std::vector<BtItem> items;
     int any_vm = 0;
+    // rows per work item: enough items for every resident block several
+    // times over, as few as that allows (each item initializes and merges
+    // an LDS table)
+    {
+        // sized for the largest query, which a rerun may run alone: about
+        // 16 items per CU keep the rounds of resident blocks balanced
+        int64_t largest = 0;
+        for (int q = 0; q < n_queries; q++) largest = std::max<int64_t>(largest, queries[q].n_rows);
+        int64_t want = largest / ((int64_t)vg_cus() * 16);
+        int64_t t = BT_TILE_MIN;
+        while (t < want && t < BT_TILE_MAX) t <<= 1;
+        b->tile = t;
+    }
     for (int q = 0; q < n_queries; q++) {
         const vgpu_query &in = queries[q];
         BtQuery &Q = qs[q];
         memset(&Q, 0, sizeof Q);
-        if (in.n_rows < 0 || in.n_rows > (int64_t)0xFFFFFFFF * BT_TILE) {
+        if (in.n_rows < 0 || in.n_rows > (int64_t)0xFFFFFFFF * BT_TILE_MIN) {
             vg_set_error("batch: query %d has %lld rows", q, (long long)in.n_rows);
             return -1;
         }
@@ -662,7 +695,7 @@ extern "C" int vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_q
             Q.vm = !Q.pi.fast;
             any_vm |= Q.vm;
         }
-        for (int64_t t = 0; t * BT_TILE < in.n_rows; t++) items.push_back(BtItem{(uint32_t)q, (uint32_t)t});
+        for (int64_t t = 0; t * b->tile < in.n_rows; t++) items.push_back(BtItem{(uint32_t)q, (uint32_t)t});
     }
     b->n_queries = n_queries;
     b->n_items = (int64_t)items.size();
@@ -675,7 +708,10 @@ extern "C" int vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_q
         for (int i = 0; i < L.n_inputs && shared; i++) shared = same_col(A.ins.c[i], C2.ins.c[i]);
         for (int i = 0; i < b->n_pred_cols && shared; i++) shared = same_col(A.pcs.c[i], C2.pcs.c[i]);
     }
-    b->shared = shared;
+    // measured on MI355X with 1B-row tables: the per-row work, not HBM,
+    // bounds these queries and query blocks drift apart, so shared scans
+    // ran slower than one query after another; they stay behind a knob
+    b->shared = shared && vg_tune(VG_TUNE_BT_SHARED) > 1;
     b->n_active = n_queries;
     b->n_tiles = shared ? (qs[0].n_rows + BT_STILE - 1) / BT_STILE : 0;
     size_t vm = any_vm ? (vg_vm_lds_bytes(&b->prog, GB_THREADS) + 15) & ~(size_t)15 : 0;
@@ -708,7 +744,7 @@ extern "C" int vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_q
         sq = qs;
     }
     for (int q = 0; q < (int)sq.size(); q++)
-        for (int64_t t = 0; t * BT_TILE < sq[q].n_rows; t++) sitems.push_back(BtItem{(uint32_t)q, (uint32_t)t});
+        for (int64_t t = 0; t * b->tile < sq[q].n_rows; t++) sitems.push_back(BtItem{(uint32_t)q, (uint32_t)t});
     if (!L.key_none && !sitems.empty()) {
         const int nq = (int)sq.size();
         const int64_t ni = (int64_t)sitems.size();
@@ -728,6 +764,7 @@ extern "C" int vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_q
         A.qs = dq.as<BtQuery>();
         A.items = di.as<BtItem>();
         A.n_items = ni;
+        A.tile = b->tile;
         k_bt_sketch<<<vg_blocks(ni, 1, 4096), GB_THREADS, 0, st>>>(A, L.key_none, d_regs, d_nan);
         VG_LAUNCH_CHECK();
         std::vector<uint32_t> h(regs);
@@ -741,7 +778,13 @@ extern "C" int vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_q
         for (int q = 0; q < n_queries; q++)
             est[q] = bt_estimate(&h[(size_t)(nq == n_queries ? q : 0) * BT_SK_REGS], hn[nq == n_queries ? q : 0]);
     }
-    const size_t rw = (size_t)(L.RH + L.RC) * 8;
+    // LDS tables drop the cold words when no input can touch them
+    bool cold = false;
+    for (int i = 0; i < L.n_inputs; i++) cold |= L.c_nan[i] >= 0;
+    for (int q = 0; q < n_queries && !cold; q++)
+        for (int i = 0; i < L.n_inputs; i++) cold |= qs[q].ins.c[i].validity != nullptr;
+    b->lds_cold = cold;
+    const size_t rw = (size_t)(L.RH + (cold ? L.RC : 0)) * 8;
     std::vector<uint64_t> qbase((size_t)n_queries + 1);
     std::vector<BtItem> cmap;
     int64_t slots = 0;
@@ -759,7 +802,7 @@ extern "C" int vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_q
         int64_t S = pow2_at_least(2 * est[q]);
         if (L.key_none) S = 64;
         Q.lds_slots = 0;
-        int64_t tile = shared ? (int64_t)BT_STILE * 64 : (int64_t)BT_TILE;   // rows one LDS table sees
+        int64_t tile = shared ? (int64_t)BT_STILE * 64 : b->tile;   // rows one LDS table sees
         if ((size_t)(S + 2) * rw + vm <= BT_LDS_BUDGET && S * 8 <= std::min<int64_t>(Q.n_rows, tile)) {
             Q.lds_slots = (int32_t)S;
             S_max = std::max<int>(S_max, (int)S);
@@ -778,6 +821,12 @@ extern "C" int vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_q
     VG_TRY(hipGetDevice(&dev));
     VG_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     b->feed_grid = std::max<int64_t>(1, std::min<int64_t>(b->n_items, (int64_t)cus * BT_BLOCKS_PER_CU));
+    {
+        // 160 KB of LDS and 32 waves (8 blocks of 256 threads) per CU
+        size_t lds = (size_t)b->vm_lds + (S_max ? (size_t)(S_max + 2) * rw : 0) + 512;
+        int64_t per_cu = std::min<int64_t>(8, (int64_t)(160 * 1024 / std::max<size_t>(lds, 1)));
+        b->shared_resident = std::max<int64_t>(BT_XCDS, (int64_t)cus * std::max<int64_t>(per_cu, 1));
+    }
     int qbits = 0;
     while (((int64_t)1 << qbits) < n_queries) qbits++;
     b->qbits = qbits;
@@ -818,8 +867,13 @@ static int bt_enqueue(vgpu_batch *b, bool first) {
     A.n_queries = b->n_queries;
     A.jn = b->join;
     A.K = b->n_active;
-    A.J = std::max<int>(1, (int)(b->feed_grid / ((int64_t)BT_XCDS * std::max(b->n_active, 1))));
+    // shared scans: every query's blocks must be resident together so that
+    // they read each tile from the XCD's L2 at about the same time
+    A.J = std::max<int>(1, (int)(b->shared_resident / ((int64_t)BT_XCDS * std::max(b->n_active, 1))));
     A.n_tiles = b->n_tiles;
+    A.lds_cold = b->lds_cold;
+    A.n_active = b->n_active;
+    A.tile = b->tile;
     uint64_t *bctl = A.ctl + (size_t)b->n_queries * GB_CTL_WORDS;
     uint64_t *qcount = b->qcount.as<uint64_t>();
     const uint64_t empty = L.key_is_double ? VG_EMPTY_F64 : VG_EMPTY_I64;
@@ -834,7 +888,8 @@ static int bt_enqueue(vgpu_batch *b, bool first) {
     VG_LAUNCH_CHECK();
     // 2. one persistent feed launch
     if (b->n_items > 0) {
-        size_t lds = (size_t)b->vm_lds + (b->S_max ? (size_t)(b->S_max + 2) * (L.RH + L.RC) * 8 : 0);
+        size_t lds = (size_t)b->vm_lds +
+                     (b->S_max ? (size_t)(b->S_max + 2) * (L.RH + (b->lds_cold ? L.RC : 0)) * 8 : 0);
         vg_kt_begin();
         if (b->shared)
             k_bt_feed_shared<<<(unsigned)((int64_t)BT_XCDS * A.K * A.J), GB_THREADS, lds, st>>>(L, A);
@@ -931,9 +986,8 @@ extern "C" int vgpu_batch_rerun(vgpu_batch *b, const uint64_t *params, int n_act
         vg_set_error("batch: rerun before plan");
         return -1;
     }
-    if (n_active < 1 || n_active > b->n_queries || (!b->shared && n_active != b->n_queries)) {
-        vg_set_error("batch: %d active queries of %d (fewer only when every query reads the same rows)", n_active,
-                     b->n_queries);
+    if (n_active < 1 || n_active > b->n_queries) {
+        vg_set_error("batch: %d active queries of %d", n_active, b->n_queries);
         return -1;
     }
     // a rerun re-derives the sort range and the output sizes: replays of
@@ -1069,6 +1123,17 @@ extern "C" int64_t vgpu_batch_fetch(vgpu_batch *b, int q, void *keys, uint8_t *k
                           st));
     VG_TRY(hipStreamSynchronize(st));
     return (int64_t)qctl[4];
+}
+
+extern "C" int64_t vgpu_batch_device_bytes(const vgpu_batch *b) {
+    const DevBuf *bufs[] = {&b->qs, &b->progs, &b->items, &b->cmap, &b->queue, &b->hot, &b->cold, &b->ctl,
+                            &b->qbase, &b->qcount, &b->qoff, &b->bcount, &b->boff, &b->por, &b->pand, &b->skeys,
+                            &b->svals, &b->sort_tmp, &b->scan_tmp, &b->out_keys, &b->out_kvalid};
+    int64_t total = 0;
+    for (const DevBuf *p : bufs) total += (int64_t)p->cap;
+    for (auto &x : b->out_vals) total += (int64_t)x.cap;
+    for (auto &x : b->out_valid) total += (int64_t)x.cap;
+    return total;
 }
 
 extern "C" void vgpu_batch_destroy(vgpu_batch *b) {

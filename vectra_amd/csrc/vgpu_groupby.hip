@@ -116,14 +116,17 @@ k_gb_feed(GbLayout L, FeedArgs A, InSet ins, PredInfo pi, vgpu_prog prog, ColSet
     uint8_t *regv = dyn + (size_t)prog.n_insn * GB_THREADS * 8;
     uint64_t *lhot = reinterpret_cast<uint64_t *>(dyn + A.vm_lds);
     const int S = A.lds_slots;
-    uint64_t *lcold = lhot + (size_t)(S + 2) * L.RH;
+    uint64_t *lcold = A.lds_cold ? lhot + (size_t)(S + 2) * L.RH : nullptr;
     const uint64_t empty = L.key_is_double ? VG_EMPTY_F64 : VG_EMPTY_I64;
     if (S) gb_lds_init(L, lhot, lcold, S, empty);
-    // GB_U rows per thread per step (a grid stride apart): their selection,
-    // predicate and key loads are issued together, so each wave keeps
-    // several loads in flight
-    const int64_t stride = (int64_t)gridDim.x * GB_THREADS;
-    for (int64_t base = (int64_t)blockIdx.x * GB_THREADS + tid; base < A.n_logical; base += stride * GB_U) {
+    // Blocks walk tiles of GB_U * GB_THREADS contiguous rows; each thread's
+    // GB_U rows of a tile have their selection, predicate and key loads
+    // issued together, so each wave keeps several loads in flight within
+    // one DRAM neighbourhood
+    const int64_t tile = (int64_t)GB_THREADS * GB_U;
+    for (int64_t t0 = (int64_t)blockIdx.x * tile; t0 < A.n_logical; t0 += (int64_t)gridDim.x * tile) {
+        const int64_t base = t0 + tid;
+        const int64_t stride = GB_THREADS;
         int64_t row[GB_U];
         bool ok[GB_U];
         int cls[GB_U];
@@ -150,7 +153,7 @@ k_gb_feed(GbLayout L, FeedArgs A, InSet ins, PredInfo pi, vgpu_prog prog, ColSet
                 gb_note_zero_key(A.ctl, vg_load_bits(A.key, row[u]), pos);
             int64_t ls = S ? gb_lds_slot(lhot, S, L.RH, cls[u], kn[u], empty) : -1;
             if (ls >= 0) {
-                gb_apply(L, ins, row[u], pos, lhot + (size_t)ls * L.RH, lcold + (size_t)ls * L.RC);
+                gb_apply(L, ins, row[u], pos, lhot + (size_t)ls * L.RH, lcold ? lcold + (size_t)ls * L.RC : nullptr);
             } else {
                 int64_t gs = gb_global_slot(L, A.hot, A.cap, cls[u], kn[u], pos, A.ctl);
                 if (gs >= 0)
@@ -1361,10 +1364,10 @@ static int gbp_collect(vgpu_groupby *g, const RecSrc &R) {
 
 // LDS slots for privatization: the table must hold the expected groups at
 // load <= 1/2 within the LDS budget
-static int gb_lds_slots(const vgpu_groupby *g, size_t vm_bytes) {
+static int gb_lds_slots(const vgpu_groupby *g, size_t vm_bytes, bool cold = true) {
     int64_t groups = g->L.key_none ? 1 : g->hint > 0 ? g->hint : g->est_groups;
     if (groups <= 0) return 0;
-    size_t rec = (size_t)(g->L.RH + g->L.RC) * 8;
+    size_t rec = (size_t)(g->L.RH + (cold ? g->L.RC : 0)) * 8;
     size_t lim = (size_t)vg_tune(VG_TUNE_GB_LDS_BUDGET);
     size_t budget = lim > vm_bytes ? lim - vm_bytes : 0;
     int64_t want = pow2_at_least(2 * groups);
@@ -1372,10 +1375,13 @@ static int gb_lds_slots(const vgpu_groupby *g, size_t vm_bytes) {
     return (int)want;
 }
 
-static unsigned gb_grid(int64_t n_logical, int lds_slots) {
-    // privatized blocks amortize their merge over many rows
+static unsigned gb_grid(int64_t n_logical, int lds_slots, size_t lds_bytes) {
+    // privatized blocks amortize their merge over many rows; no more of
+    // them than fit on the device at once (a second, partial round of
+    // blocks would leave most CUs idle)
     if (lds_slots)
-        return vg_blocks(n_logical, vg_tune(VG_TUNE_GB_LDS_ROWS), vg_tune(VG_TUNE_GB_LDS_MAX_BLOCKS));
+        return vg_blocks(n_logical, vg_tune(VG_TUNE_GB_LDS_ROWS),
+                         std::min<int64_t>(vg_tune(VG_TUNE_GB_LDS_MAX_BLOCKS), vg_resident_blocks(lds_bytes)));
     return vg_blocks(n_logical, vg_tune(VG_TUNE_GB_GLB_ROWS), vg_tune(VG_TUNE_GB_GLB_MAX_BLOCKS));
 }
 
@@ -1448,10 +1454,11 @@ extern "C" int vgpu_groupby_feed(vgpu_groupby *g, const vgpu_col *key, const vgp
     vm = (vm + 15) & ~(size_t)15;
     A.vm_lds = (int32_t)vm;
     if (partitioned) return gbp_feed(g, A, ins, pi, prog, pcs, groups);
-    A.lds_slots = gb_lds_slots(g, vm);
-    size_t lds = vm + (A.lds_slots ? (size_t)(A.lds_slots + 2) * (g->L.RH + g->L.RC) * 8 : 0);
+    A.lds_cold = gb_needs_cold(g->L, inputs);
+    A.lds_slots = gb_lds_slots(g, vm, A.lds_cold);
+    size_t lds = vm + (A.lds_slots ? (size_t)(A.lds_slots + 2) * (g->L.RH + (A.lds_cold ? g->L.RC : 0)) * 8 : 0);
     vg_kt_begin();
-    k_gb_feed<<<gb_grid(n_logical, A.lds_slots), GB_THREADS, lds, vg_stream()>>>(g->L, A, ins, pi, prog, pcs);
+    k_gb_feed<<<gb_grid(n_logical, A.lds_slots, lds), GB_THREADS, lds, vg_stream()>>>(g->L, A, ins, pi, prog, pcs);
     VG_LAUNCH_CHECK();
     vg_kt_end();
     g->any_fed = 1;
@@ -1583,6 +1590,21 @@ extern "C" int vgpu_groupby_feed_join(vgpu_groupby *g, const vgpu_feed_spec *sp)
     }
     int64_t incoming = n_logical * (sp->join ? std::max<int64_t>(1, sp->join->max_dup) : 1);
     if (g->hint > 0 && incoming > g->hint) incoming = g->hint;
+    // without a group count, sketch the column the groups come from once: a
+    // build column (its distinct values bound the groups, plus the NA group
+    // of unmatched rows) or a large probe column
+    if (g->hint <= 0 && !g->L.key_none && g->est_groups == 0) {
+        int64_t est = 0;
+        if (sp->key_side == 1 && sp->join && sp->join->n_build > 0) {
+            if (gb_sketch(g, &sp->build_cols[sp->key_col], nullptr, sp->join->n_build, &est)) return -1;
+        } else if (sp->key_side == 0 && n_logical >= GB_SK_MIN_ROWS) {
+            if (gb_sketch(g, &sp->probe_cols[sp->key_col], sp->sel, n_logical, &est)) return -1;
+        }
+        if (est > 0) {
+            g->est_groups = est + 1;
+            incoming = std::min(incoming, est + est / 4 + 4096);
+        }
+    }
     if (gb_ensure_capacity(g, incoming)) return -1;
 
     JoinFeedArgs A;
@@ -1671,7 +1693,7 @@ extern "C" int vgpu_groupby_feed_join(vgpu_groupby *g, const vgpu_feed_spec *sp)
         return 0;
     }
     vg_kt_begin();
-    k_gb_feed_join<<<gb_grid(n_logical, A.lds_slots), GB_THREADS, lds, vg_stream()>>>(g->L, A, cp, jv, pi, prog);
+    k_gb_feed_join<<<gb_grid(n_logical, A.lds_slots, lds), GB_THREADS, lds, vg_stream()>>>(g->L, A, cp, jv, pi, prog);
     VG_LAUNCH_CHECK();
     vg_kt_end();
     g->any_fed = 1;

@@ -93,13 +93,15 @@ __device__ __forceinline__ void gb_update_input(const GbLayout &L, uint64_t *hot
     if (L.w_max[i] >= 0) gb_atomic_max(&hot[L.w_max[i]], o);
 }
 
-// merge a whole record (LDS partial -> HBM)
+// merge a whole record (LDS partial -> HBM); sc == nullptr for a partial
+// without cold words, kept only when no input can touch them (no input has
+// NAs and none has a double min / max)
 __device__ __forceinline__ void gb_merge(const GbLayout &L, uint64_t *dh, uint64_t *dc,
                                          const uint64_t *sh, const uint64_t *sc) {
     gb_atomic_add(&dh[1], sh[1]);
     if (L.w_fpos >= 0) gb_atomic_min(&dh[L.w_fpos], sh[L.w_fpos]);
     for (int i = 0; i < L.n_inputs; i++) {
-        if (sc[L.c_na[i]]) gb_atomic_add(&dc[L.c_na[i]], sc[L.c_na[i]]);
+        if (sc && sc[L.c_na[i]]) gb_atomic_add(&dc[L.c_na[i]], sc[L.c_na[i]]);
         if (L.w_fvpos[i] >= 0) gb_atomic_min(&dh[L.w_fvpos[i]], sh[L.w_fvpos[i]]);
         if (L.w_sum[i] >= 0) {
             if (L.in_type[i] == VGPU_DOUBLE) gb_atomic_addf(&dh[L.w_sum[i]], vg_bits_to_f64(sh[L.w_sum[i]]));
@@ -107,7 +109,7 @@ __device__ __forceinline__ void gb_merge(const GbLayout &L, uint64_t *dh, uint64
         }
         if (L.w_min[i] >= 0) gb_atomic_min(&dh[L.w_min[i]], sh[L.w_min[i]]);
         if (L.w_max[i] >= 0) gb_atomic_max(&dh[L.w_max[i]], sh[L.w_max[i]]);
-        if (L.c_nan[i] >= 0) {
+        if (sc && L.c_nan[i] >= 0) {
             if (sc[L.c_nan[i]] != GB_NOPOS) gb_atomic_min(&dc[L.c_nan[i]], sc[L.c_nan[i]]);
             if (sc[L.c_zneg[i]] != GB_NOPOS) gb_atomic_min(&dc[L.c_zneg[i]], sc[L.c_zneg[i]]);
             if (sc[L.c_zpos[i]] != GB_NOPOS) gb_atomic_min(&dc[L.c_zpos[i]], sc[L.c_zpos[i]]);
@@ -204,7 +206,16 @@ struct FeedArgs {
     int64_t cap;
     int32_t lds_slots;     // 0 = no LDS privatization
     int32_t vm_lds;        // bytes of VM registers at the start of dyn LDS
+    int32_t lds_cold;      // LDS records carry cold words (an input has NAs or a double min / max)
+    int32_t pad;
 };
+
+// cold words can be touched only by NA inputs and by double min / max
+static inline bool gb_needs_cold(const GbLayout &L, const vgpu_col *inputs) {
+    for (int i = 0; i < L.n_inputs; i++)
+        if (L.c_nan[i] >= 0 || (inputs && inputs[i].validity)) return true;
+    return false;
+}
 
 // The LDS-private table holds S hashed records plus two reserved ones (NA
 // key, EMPTY-valued int64 key), like the HBM table's slots cap / cap+1, so
@@ -215,6 +226,7 @@ __device__ __forceinline__ void gb_lds_init(const GbLayout &L, uint64_t *lhot, u
         uint64_t *h = lhot + (size_t)s * L.RH;
         h[0] = empty;
         for (int w = 1; w < L.RH; w++) h[w] = ((L.hot_init_ones >> w) & 1) ? ~0ull : 0ull;
+        if (!lcold) continue;   // a table without cold words (see gb_lds_merge)
         uint64_t *c = lcold + (size_t)s * L.RC;
         for (int w = 0; w < L.RC; w++) c[w] = ((L.cold_init_ones >> w) & 1) ? ~0ull : 0ull;
     }
@@ -244,7 +256,8 @@ __device__ __forceinline__ void gb_lds_merge(const GbLayout &L, const uint64_t *
             if (h[1] == 0) continue;
             gs = cap + (s - S);
         }
-        if (gs >= 0) gb_merge(L, hot + (size_t)gs * L.RH, cold + (size_t)gs * L.RC, h, lcold + (size_t)s * L.RC);
+        if (gs >= 0)
+            gb_merge(L, hot + (size_t)gs * L.RH, cold + (size_t)gs * L.RC, h, lcold ? lcold + (size_t)s * L.RC : nullptr);
     }
 }
 

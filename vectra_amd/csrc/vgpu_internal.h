@@ -21,7 +21,7 @@ void vg_kt_end();
 // launch-shape knobs (see vgpu_tune_set)
 enum { VG_TUNE_GB_LDS_MAX_BLOCKS, VG_TUNE_GB_LDS_ROWS, VG_TUNE_GB_GLB_MAX_BLOCKS, VG_TUNE_GB_GLB_ROWS,
        VG_TUNE_GB_LDS_MAX_SLOTS, VG_TUNE_GB_LDS_BUDGET, VG_TUNE_GB_STAR_ITEMS, VG_TUNE_GB_PART_MIN_GROUPS,
-       VG_TUNE_COUNT };
+       VG_TUNE_BT_SHARED, VG_TUNE_COUNT };
 int64_t vg_tune(int knob);
 
 #define VG_TRY(call)                                                          \
@@ -101,6 +101,9 @@ int vg_scan_u64(const uint64_t *in, uint64_t *out, int64_t n, uint64_t *total,
 // stable LSD radix sort of (key, value) pairs on bits [begin_bit, end_bit)
 int vg_sort_pairs(uint64_t *keys, uint32_t *vals, int64_t n, int begin_bit,
                   int end_bit, DevBuf &tmp);
+
+int vg_cus();
+int64_t vg_resident_blocks(size_t lds_bytes_per_block);
 
 // blocks for n items at per_block each, at most cap (kernels with one thread
 // per item rely on the default, which covers any grid the hardware takes)

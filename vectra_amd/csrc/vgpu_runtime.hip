@@ -3,6 +3,8 @@
 // and join builds.
 #include "vgpu_internal.h"
 
+#include <algorithm>
+
 #include <stdarg.h>
 #include <stdio.h>
 #include <mutex>
@@ -56,6 +58,26 @@ extern "C" int vgpu_init(int device) {
 int vg_ensure_init() {
     if (g_stream) return 0;
     return vgpu_init(0);
+}
+
+// compute units of the current device (256 on MI355X)
+int vg_cus() {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    return cus;
+}
+
+// blocks of `lds` bytes and 256 threads that fit on the device at once
+// (160 KB of LDS and 8 such blocks per CU)
+int64_t vg_resident_blocks(size_t lds) {
+    int64_t per_cu = 8;
+    if (lds > 0) per_cu = std::min<int64_t>(8, (int64_t)((160 * 1024) / lds));
+    return (int64_t)vg_cus() * std::max<int64_t>(per_cu, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -186,8 +208,8 @@ extern "C" double vgpu_timer_stop(void) {
 // launch-shape knobs
 static const char *g_tune_names[VG_TUNE_COUNT] = {"gb.lds.max_blocks", "gb.lds.rows_per_block", "gb.glb.max_blocks",
                                                   "gb.glb.rows_per_block", "gb.lds.max_slots", "gb.lds.budget",
-                                                  "gb.star.items", "gb.part.min_groups"};
-static int64_t g_tune[VG_TUNE_COUNT] = {1024, 8192, 16384, 1024, 4096, 65536, 4, 65536};
+                                                  "gb.star.items", "gb.part.min_groups", "batch.shared_scan"};
+static int64_t g_tune[VG_TUNE_COUNT] = {1024, 8192, 16384, 1024, 4096, 65536, 4, 65536, 1};
 
 int64_t vg_tune(int knob) { return g_tune[knob]; }
 

@@ -122,6 +122,29 @@ class VgpuQuery(C.Structure):
                 ("pred_cols", C.POINTER(VgpuCol)), ("n_rows", C.c_int64), ("param", C.c_uint64)]
 
 
+class VxRequest(C.Structure):
+    _fields_ = [("cls", C.c_int32), ("pad", C.c_int32), ("threshold", C.c_double), ("arrival_s", C.c_double)]
+
+
+class VxResult(C.Structure):
+    _fields_ = [("start_s", C.c_double), ("end_s", C.c_double), ("pass_", C.c_int32), ("batch", C.c_int32),
+                ("n_groups", C.c_int64), ("keys", C.POINTER(C.c_int64)), ("key_valid", C.POINTER(C.c_uint8)),
+                ("sum", C.POINTER(C.c_double)), ("sum_valid", C.POINTER(C.c_uint8)),
+                ("count", C.POINTER(C.c_double))]
+
+
+class VxServeCfg(C.Structure):
+    _fields_ = [("fact_key", VgpuCol), ("fact_value", VgpuCol), ("fact_group", VgpuCol), ("n_rows", C.c_int64),
+                ("dim_key", VgpuCol), ("dim_attr", VgpuCol), ("n_dim", C.c_int64), ("batched", C.c_int32),
+                ("max_batch", C.c_int32), ("state_budget", C.c_int64), ("keep_results", C.c_int32),
+                ("pad", C.c_int32)]
+
+
+class VxServeStats(C.Structure):
+    _fields_ = [("n_done", C.c_int64), ("n_passes", C.c_int32), ("max_pass", C.c_int32), ("wall_s", C.c_double),
+                ("setup_s", C.c_double), ("state_bytes", C.c_int64), ("resident_bytes", C.c_int64)]
+
+
 class VgpuFeedSpec(C.Structure):
     _fields_ = [("n_rows", C.c_int64), ("sel", C.c_void_p), ("n_sel", C.c_int64), ("pos_base", C.c_int64),
                 ("pred", C.POINTER(VgpuProg)), ("join", C.c_void_p), ("join_kind", C.c_int32),
@@ -271,6 +294,9 @@ def nodes() -> C.CDLL:
         lib = C.CDLL(str(_need(LIB / "libvectra_gpu.so")))
         _node_protos(lib, "")
         _proto(lib, "vx_try_compile_expr", C.c_int, [EXPR_P, BATCH_P, C.POINTER(CompiledExpr)])
+        _proto(lib, "vx_serve_run", C.c_int, [C.POINTER(VxServeCfg), C.POINTER(VxRequest), C.c_int,
+                                              C.POINTER(VxResult), C.POINTER(VxServeStats)])
+        _proto(lib, "vx_result_free", None, [C.POINTER(VxResult)])
         _libs["nodes"] = lib
     return _libs["nodes"]
 
@@ -916,3 +942,43 @@ class Batch:
         if self.h:
             vgpu().vgpu_batch_destroy(self.h)
             self.h = None
+
+
+def serve(fact: dict, dim: dict, requests: Sequence[tuple], batched: bool, max_batch: int = 32,
+          state_budget: int = 0, keep_results: bool = True) -> tuple[list[dict], dict]:
+    """Runs the C request loop (include/vx_serve.h) over device columns.
+    fact: {"key", "value", "group"} DeviceColumns; dim: {"key", "attr"};
+    requests: (cls, threshold, arrival_s) in arrival order."""
+    n = len(requests)
+    cfg = VxServeCfg()
+    cfg.fact_key, cfg.fact_value, cfg.fact_group = fact["key"].col, fact["value"].col, fact["group"].col
+    cfg.n_rows = fact["value"].n
+    cfg.dim_key, cfg.dim_attr = dim["key"].col, dim["attr"].col
+    cfg.n_dim = dim["key"].n
+    cfg.batched = 1 if batched else 0
+    cfg.max_batch = max_batch
+    cfg.state_budget = state_budget
+    cfg.keep_results = 1 if keep_results else 0
+    reqs = (VxRequest * max(n, 1))()
+    for i, (cls, t, at) in enumerate(requests):
+        reqs[i].cls, reqs[i].threshold, reqs[i].arrival_s = cls, t, at
+    out = (VxResult * max(n, 1))()
+    st = VxServeStats()
+    lib = nodes()
+    if lib.vx_serve_run(C.byref(cfg), reqs, n, out, C.byref(st)) != 0:
+        raise VectraError(vgpu().vgpu_last_error().decode())
+    res = []
+    for i in range(n):
+        r = out[i]
+        d = {"start_s": r.start_s, "end_s": r.end_s, "pass": r.pass_, "batch": r.batch, "n_groups": r.n_groups}
+        if keep_results:
+            g = r.n_groups
+            bits = lambda p: np.unpackbits(np.ctypeslib.as_array(p, ((g + 7) // 8 + 1,)).copy(),
+                                           bitorder="little")[:g].astype(bool)
+            d["key"] = (np.ctypeslib.as_array(r.keys, (max(g, 1),))[:g].copy(), bits(r.key_valid))
+            d["s"] = (np.ctypeslib.as_array(r.sum, (max(g, 1),))[:g].copy(), bits(r.sum_valid))
+            d["n"] = (np.ctypeslib.as_array(r.count, (max(g, 1),))[:g].copy(), np.ones(g, bool))
+            lib.vx_result_free(C.byref(out[i]))
+        res.append(d)
+    stats = {k: getattr(st, k) for k, _ in VxServeStats._fields_}
+    return res, stats
