@@ -340,9 +340,23 @@ void    vgpu_batch_destroy(vgpu_batch *b);
  *                         least 2^20 rows is hash-partitioned)
  *   "batch.shared_scan"  (1: off; 2: batches whose queries all read the
  *                         same columns share one scan of them)
+ *   "gb.lds.slots_per_group" (2: LDS table slots per expected group, the
+ *                         table's inverse load factor; 2..8)
  * Returns -1 for an unknown name or a value <= 0. */
 int     vgpu_tune_set(const char *name, int64_t value);
 int64_t vgpu_tune_get(const char *name);
+/* A knob value for group-bys expecting at most max_groups groups (a
+ * cardinality bucket of the autotuner's table; the smallest covering
+ * bucket that sets the knob wins, else the global value). */
+int     vgpu_tune_set_for(int64_t max_groups, const char *name, int64_t value);
+/* the value a group-by expecting `groups` groups uses (-1: unknown name) */
+int64_t vgpu_tune_get_for(int64_t groups, const char *name);
+/* Loads a table written by the autotuner (vectra_amd/autotune.py): lines
+ * "max_groups name value", max_groups 0 for global values.  Returns the
+ * number of settings or -1. */
+int     vgpu_tune_load(const char *path);
+/* Drops every bucket and restores the built-in global values. */
+void    vgpu_tune_reset(void);
 
 /* ---- timing ---------------------------------------------------------
  * Event timing of the library's own stream: elapsed milliseconds of all

@@ -1198,7 +1198,7 @@ static int gbp_lds_slots(const vgpu_groupby *g) {
 }
 
 static bool gbp_wanted(const vgpu_groupby *g, int64_t n_logical, int64_t groups) {
-    return !g->any_fed && !g->L.key_none && groups >= vg_tune(VG_TUNE_GB_PART_MIN_GROUPS) &&
+    return !g->any_fed && !g->L.key_none && groups >= vg_tune_for(VG_TUNE_GB_PART_MIN_GROUPS, groups) &&
            n_logical >= GB_SK_MIN_ROWS &&
            n_logical < ((int64_t)1 << 32) - 2 && gbp_lds_slots(g) > 0;
 }
@@ -1364,25 +1364,32 @@ static int gbp_collect(vgpu_groupby *g, const RecSrc &R) {
 
 // LDS slots for privatization: the table must hold the expected groups at
 // load <= 1/2 within the LDS budget
+// groups the group-by expects (hint, sketch or groups seen; 0 = unknown)
+static int64_t gb_groups(const vgpu_groupby *g) {
+    return g->L.key_none ? 1 : g->hint > 0 ? g->hint : g->est_groups;
+}
+
 static int gb_lds_slots(const vgpu_groupby *g, size_t vm_bytes, bool cold = true) {
-    int64_t groups = g->L.key_none ? 1 : g->hint > 0 ? g->hint : g->est_groups;
+    int64_t groups = gb_groups(g);
     if (groups <= 0) return 0;
     size_t rec = (size_t)(g->L.RH + (cold ? g->L.RC : 0)) * 8;
-    size_t lim = (size_t)vg_tune(VG_TUNE_GB_LDS_BUDGET);
+    size_t lim = (size_t)vg_tune_for(VG_TUNE_GB_LDS_BUDGET, groups);
     size_t budget = lim > vm_bytes ? lim - vm_bytes : 0;
-    int64_t want = pow2_at_least(2 * groups);
-    if (want > vg_tune(VG_TUNE_GB_LDS_MAX_SLOTS) || (size_t)(want + 2) * rec > budget) return 0;
+    int64_t want = pow2_at_least(vg_tune_for(VG_TUNE_GB_LDS_SLOTS_PER_GROUP, groups) * groups);
+    if (want > vg_tune_for(VG_TUNE_GB_LDS_MAX_SLOTS, groups) || (size_t)(want + 2) * rec > budget) return 0;
     return (int)want;
 }
 
-static unsigned gb_grid(int64_t n_logical, int lds_slots, size_t lds_bytes) {
+static unsigned gb_grid(int64_t n_logical, int lds_slots, size_t lds_bytes, int64_t groups) {
     // privatized blocks amortize their merge over many rows; no more of
     // them than fit on the device at once (a second, partial round of
     // blocks would leave most CUs idle)
     if (lds_slots)
-        return vg_blocks(n_logical, vg_tune(VG_TUNE_GB_LDS_ROWS),
-                         std::min<int64_t>(vg_tune(VG_TUNE_GB_LDS_MAX_BLOCKS), vg_resident_blocks(lds_bytes)));
-    return vg_blocks(n_logical, vg_tune(VG_TUNE_GB_GLB_ROWS), vg_tune(VG_TUNE_GB_GLB_MAX_BLOCKS));
+        return vg_blocks(n_logical, vg_tune_for(VG_TUNE_GB_LDS_ROWS, groups),
+                         std::min<int64_t>(vg_tune_for(VG_TUNE_GB_LDS_MAX_BLOCKS, groups),
+                                           vg_resident_blocks(lds_bytes)));
+    return vg_blocks(n_logical, vg_tune_for(VG_TUNE_GB_GLB_ROWS, groups),
+                     vg_tune_for(VG_TUNE_GB_GLB_MAX_BLOCKS, groups));
 }
 
 extern "C" int vgpu_groupby_feed(vgpu_groupby *g, const vgpu_col *key, const vgpu_col *inputs, int64_t n_rows,
@@ -1458,7 +1465,7 @@ extern "C" int vgpu_groupby_feed(vgpu_groupby *g, const vgpu_col *key, const vgp
     A.lds_slots = gb_lds_slots(g, vm, A.lds_cold);
     size_t lds = vm + (A.lds_slots ? (size_t)(A.lds_slots + 2) * (g->L.RH + (A.lds_cold ? g->L.RC : 0)) * 8 : 0);
     vg_kt_begin();
-    k_gb_feed<<<gb_grid(n_logical, A.lds_slots, lds), GB_THREADS, lds, vg_stream()>>>(g->L, A, ins, pi, prog, pcs);
+    k_gb_feed<<<gb_grid(n_logical, A.lds_slots, lds, gb_groups(g)), GB_THREADS, lds, vg_stream()>>>(g->L, A, ins, pi, prog, pcs);
     VG_LAUNCH_CHECK();
     vg_kt_end();
     g->any_fed = 1;
@@ -1677,9 +1684,11 @@ extern "C" int vgpu_groupby_feed_join(vgpu_groupby *g, const vgpu_feed_spec *sp)
         st.lds_slots = A.lds_slots;
         st.vm_lds = 0;
         size_t slds = A.lds_slots ? (size_t)(A.lds_slots + 2) * (g->L.RH + g->L.RC) * 8 : 0;
-        int items = (int)vg_tune(VG_TUNE_GB_STAR_ITEMS);
-        unsigned grid = vg_blocks(n_logical, vg_tune(A.lds_slots ? VG_TUNE_GB_LDS_ROWS : VG_TUNE_GB_GLB_ROWS),
-                                  vg_tune(A.lds_slots ? VG_TUNE_GB_LDS_MAX_BLOCKS : VG_TUNE_GB_GLB_MAX_BLOCKS));
+        const int64_t groups = gb_groups(g);
+        int items = (int)vg_tune_for(VG_TUNE_GB_STAR_ITEMS, groups);
+        unsigned grid = vg_blocks(n_logical, vg_tune_for(A.lds_slots ? VG_TUNE_GB_LDS_ROWS : VG_TUNE_GB_GLB_ROWS, groups),
+                                  vg_tune_for(A.lds_slots ? VG_TUNE_GB_LDS_MAX_BLOCKS : VG_TUNE_GB_GLB_MAX_BLOCKS,
+                                              groups));
         vg_kt_begin();
         switch (items) {
         case 1: k_gb_star<1><<<grid, GB_THREADS, slds, vg_stream()>>>(g->L, st, jv, pi, cp.probe); break;
@@ -1693,7 +1702,7 @@ extern "C" int vgpu_groupby_feed_join(vgpu_groupby *g, const vgpu_feed_spec *sp)
         return 0;
     }
     vg_kt_begin();
-    k_gb_feed_join<<<gb_grid(n_logical, A.lds_slots, lds), GB_THREADS, lds, vg_stream()>>>(g->L, A, cp, jv, pi, prog);
+    k_gb_feed_join<<<gb_grid(n_logical, A.lds_slots, lds, gb_groups(g)), GB_THREADS, lds, vg_stream()>>>(g->L, A, cp, jv, pi, prog);
     VG_LAUNCH_CHECK();
     vg_kt_end();
     g->any_fed = 1;

@@ -7,6 +7,7 @@
 
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <mutex>
 #include <vector>
 
@@ -31,6 +32,8 @@ extern "C" const char *vgpu_last_error(void) { return g_err; }
 
 hipStream_t vg_stream() { return g_stream; }
 
+extern "C" int vgpu_tune_load(const char *path);
+
 extern "C" int vgpu_init(int device) {
     std::lock_guard<std::mutex> lk(g_init_mu);
     if (g_device == device && g_stream) return 0;
@@ -49,6 +52,9 @@ extern "C" int vgpu_init(int device) {
         return -1;
     }
     if (!g_stream) VG_TRY(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    // an autotuner table named by the environment applies from the start
+    const char *tuning = getenv("VECTRA_GPU_TUNING");
+    if (tuning && *tuning && vgpu_tune_load(tuning) < 0) return -1;
     if (!g_t0) VG_TRY(hipEventCreate(&g_t0));
     if (!g_t1) VG_TRY(hipEventCreate(&g_t1));
     g_device = device;
@@ -205,30 +211,128 @@ extern "C" double vgpu_timer_stop(void) {
     return (double)ms;
 }
 
-// launch-shape knobs
+// launch-shape knobs: global values, and per cardinality bucket overrides
+// (the autotuner's table: the first bucket whose max_groups covers a
+// group-by's expected groups and sets a knob decides it)
 static const char *g_tune_names[VG_TUNE_COUNT] = {"gb.lds.max_blocks", "gb.lds.rows_per_block", "gb.glb.max_blocks",
                                                   "gb.glb.rows_per_block", "gb.lds.max_slots", "gb.lds.budget",
-                                                  "gb.star.items", "gb.part.min_groups", "batch.shared_scan"};
-static int64_t g_tune[VG_TUNE_COUNT] = {1024, 8192, 16384, 1024, 4096, 65536, 4, 65536, 1};
+                                                  "gb.star.items", "gb.part.min_groups", "batch.shared_scan",
+                                                  "gb.lds.slots_per_group"};
+static int64_t g_tune[VG_TUNE_COUNT] = {1024, 8192, 16384, 1024, 4096, 65536, 4, 65536, 1, 2};
+
+struct TuneBucket {
+    int64_t max_groups;
+    int64_t val[VG_TUNE_COUNT];
+    uint32_t set;   // bit i: knob i overridden
+};
+static std::vector<TuneBucket> g_buckets;   // ascending max_groups
+static std::mutex g_tune_mu;
 
 int64_t vg_tune(int knob) { return g_tune[knob]; }
 
-extern "C" int vgpu_tune_set(const char *name, int64_t value) {
+int64_t vg_tune_for(int knob, int64_t groups) {
+    if (groups > 0)
+        for (const TuneBucket &b : g_buckets)
+            if (groups <= b.max_groups) {
+                if (b.set & (1u << knob)) return b.val[knob];
+                break;
+            }
+    return g_tune[knob];
+}
+
+static int tune_index(const char *name) {
     for (int i = 0; i < VG_TUNE_COUNT; i++)
-        if (strcmp(name, g_tune_names[i]) == 0) {
-            if (value <= 0) {
-                vg_set_error("vgpu_tune_set: %s must be positive", name);
-                return -1;
-            }
-            if (i == VG_TUNE_GB_LDS_BUDGET && value > 65536) {
-                vg_set_error("vgpu_tune_set: %s is at most 65536 bytes", name);
-                return -1;
-            }
-            g_tune[i] = value;
-            return 0;
-        }
-    vg_set_error("vgpu_tune_set: unknown knob %s", name);
+        if (strcmp(name, g_tune_names[i]) == 0) return i;
     return -1;
+}
+
+static int tune_check(int i, const char *name, int64_t value) {
+    if (i < 0) {
+        vg_set_error("vgpu_tune_set: unknown knob %s", name);
+        return -1;
+    }
+    if (value <= 0) {
+        vg_set_error("vgpu_tune_set: %s must be positive", name);
+        return -1;
+    }
+    if (i == VG_TUNE_GB_LDS_BUDGET && value > 65536) {
+        vg_set_error("vgpu_tune_set: %s is at most 65536 bytes", name);
+        return -1;
+    }
+    if (i == VG_TUNE_GB_LDS_SLOTS_PER_GROUP && (value < 2 || value > 8)) {
+        vg_set_error("vgpu_tune_set: %s is between 2 and 8", name);
+        return -1;
+    }
+    return 0;
+}
+
+extern "C" int vgpu_tune_set(const char *name, int64_t value) {
+    int i = tune_index(name);
+    if (tune_check(i, name, value)) return -1;
+    g_tune[i] = value;
+    return 0;
+}
+
+extern "C" int vgpu_tune_set_for(int64_t max_groups, const char *name, int64_t value) {
+    int i = tune_index(name);
+    if (tune_check(i, name, value)) return -1;
+    if (max_groups <= 0) {
+        vg_set_error("vgpu_tune_set_for: max_groups must be positive");
+        return -1;
+    }
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_buckets.begin();
+    while (it != g_buckets.end() && it->max_groups < max_groups) ++it;
+    if (it == g_buckets.end() || it->max_groups != max_groups) {
+        TuneBucket b;
+        memset(&b, 0, sizeof b);
+        b.max_groups = max_groups;
+        it = g_buckets.insert(it, b);
+    }
+    it->val[i] = value;
+    it->set |= 1u << i;
+    return 0;
+}
+
+extern "C" int64_t vgpu_tune_get_for(int64_t groups, const char *name) {
+    int i = tune_index(name);
+    return i < 0 ? -1 : vg_tune_for(i, groups);
+}
+
+extern "C" void vgpu_tune_reset(void) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_buckets.clear();
+    const int64_t defaults[VG_TUNE_COUNT] = {1024, 8192, 16384, 1024, 4096, 65536, 4, 65536, 1, 2};
+    memcpy(g_tune, defaults, sizeof g_tune);
+}
+
+// "max_groups name value" per line (max_groups 0 = the global value);
+// '#' starts a comment
+extern "C" int vgpu_tune_load(const char *path) {
+    FILE *f = fopen(path, "r");
+    if (!f) {
+        vg_set_error("vgpu_tune_load: cannot open %s", path);
+        return -1;
+    }
+    char line[256];
+    int n = 0, rc = 0;
+    while (rc == 0 && fgets(line, sizeof line, f)) {
+        char *h = strchr(line, '#');
+        if (h) *h = 0;
+        long long mg, v;
+        char name[64];
+        int k = sscanf(line, "%lld %63s %lld", &mg, name, &v);
+        if (k <= 0) continue;
+        if (k != 3) {
+            vg_set_error("vgpu_tune_load: %s: malformed line", path);
+            rc = -1;
+            break;
+        }
+        rc = mg > 0 ? vgpu_tune_set_for(mg, name, v) : vgpu_tune_set(name, v);
+        n++;
+    }
+    fclose(f);
+    return rc ? -1 : n;
 }
 
 extern "C" int64_t vgpu_tune_get(const char *name) {

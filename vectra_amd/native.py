@@ -258,6 +258,10 @@ def vgpu() -> C.CDLL:
         _proto(lib, "vgpu_timer_stop", C.c_double, [])
         _proto(lib, "vgpu_tune_set", C.c_int, [C.c_char_p, C.c_int64])
         _proto(lib, "vgpu_tune_get", C.c_int64, [C.c_char_p])
+        _proto(lib, "vgpu_tune_set_for", C.c_int, [C.c_int64, C.c_char_p, C.c_int64])
+        _proto(lib, "vgpu_tune_load", C.c_int, [C.c_char_p])
+        _proto(lib, "vgpu_tune_reset", None, [])
+        _proto(lib, "vgpu_tune_get_for", C.c_int64, [C.c_int64, C.c_char_p])
         _proto(lib, "vgpu_kernel_timing", C.c_int, [C.c_int])
         _proto(lib, "vgpu_kernel_time_ms", C.c_double, [C.POINTER(C.c_int64), C.c_int])
         _proto(lib, "vgpu_batch_create", C.c_void_p, [C.c_int, C.c_int, C.POINTER(C.c_int32), C.c_int,
@@ -272,7 +276,13 @@ def vgpu() -> C.CDLL:
         _proto(lib, "vgpu_batch_fetch", C.c_int64, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                                     C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)])
         _proto(lib, "vgpu_batch_destroy", None, [C.c_void_p])
+        _proto(lib, "vgpu_batch_device_bytes", C.c_int64, [C.c_void_p])
         _libs["vgpu"] = lib
+        # the autotuner's table for this build, unless the environment names one
+        table = PKG / "tuning.txt"
+        if table.exists() and not os.environ.get("VECTRA_GPU_TUNING"):
+            if lib.vgpu_tune_load(str(table).encode()) < 0:
+                raise VectraError(lib.vgpu_last_error().decode())
     return _libs["vgpu"]
 
 
