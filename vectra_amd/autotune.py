@@ -128,9 +128,13 @@ def main(argv=None) -> int:
     lines = []
     t0 = time.time()
     for max_groups, card in BUCKETS:
-        if card <= 4096:
+        if card <= 256:
             grid = {"gb.lds.slots_per_group": [2, 4], "gb.lds.max_blocks": [256, 512, 768, 1024, 2048],
                     "gb.lds.rows_per_block": [2048, 8192, 32768]}
+        elif card <= 4096:
+            # past the LDS table's reach the choice is HBM atomics or the partitioned path
+            grid = {"gb.part.min_groups": [card // 2, 1 << 40], "gb.lds.slots_per_group": [1, 2],
+                    "gb.lds.max_blocks": [512, 1024, 2048], "gb.lds.rows_per_block": [8192, 32768]}
         else:
             grid = {"gb.part.min_groups": [16384, 1 << 40], "gb.glb.rows_per_block": [512, 1024, 4096]}
         r = tune_groupby(card, args.rows, grid, log)
