@@ -133,7 +133,7 @@ def main(argv=None) -> int:
                     "gb.lds.rows_per_block": [2048, 8192, 32768]}
         elif card <= 4096:
             # past the LDS table's reach the choice is HBM atomics or the partitioned path
-            grid = {"gb.part.min_groups": [card // 2, 1 << 40], "gb.lds.slots_per_group": [1, 2],
+            grid = {"gb.part.min_groups": [card // 2, 1 << 40], "gb.lds.slots_per_group": [2, 4],
                     "gb.lds.max_blocks": [512, 1024, 2048], "gb.lds.rows_per_block": [8192, 32768]}
         else:
             grid = {"gb.part.min_groups": [16384, 1 << 40], "gb.glb.rows_per_block": [512, 1024, 4096]}
