@@ -328,6 +328,13 @@ int64_t vgpu_batch_fetch(vgpu_batch *b, int q, void *keys, uint8_t *key_validity
                          double *const *agg_values, uint8_t *const *agg_validity);
 /* device bytes the batch holds (tables, sort and output buffers) */
 int64_t vgpu_batch_device_bytes(const vgpu_batch *b);
+/* 1 when the planned batch runs as ONE bucketed pass: every query reads
+ * the same columns through a `col < <= > >= param` filter whose constant
+ * alone differs, the group key is an int64 / bool column of a dense range
+ * (at most 2^24 values and one LDS table of (key, bucket) records), and
+ * the aggregates are counts, sums and means of inputs without NAs in key
+ * order (knob "batch.bucketed"); 0 otherwise */
+int     vgpu_batch_bucketed(const vgpu_batch *b);
 void    vgpu_batch_destroy(vgpu_batch *b);
 
 /* ---- tuning -----------------------------------------------------------
@@ -342,6 +349,11 @@ void    vgpu_batch_destroy(vgpu_batch *b);
  *                         same columns share one scan of them)
  *   "gb.lds.slots_per_group" (2: LDS table slots per expected group, the
  *                         table's inverse load factor; 2..8)
+ *   "batch.bucketed"     (2: on; 1: off) batches whose queries read the
+ *                         same columns and differ only in the constant of
+ *                         a `col < <= > >= param` filter aggregate every
+ *                         row once, keyed by (group, the number of the
+ *                         queries it passes)
  * Returns -1 for an unknown name or a value <= 0. */
 int     vgpu_tune_set(const char *name, int64_t value);
 int64_t vgpu_tune_get(const char *name);

@@ -131,6 +131,14 @@ struct vgpu_batch {
     int64_t n_tiles = 0;
     std::vector<BtQuery> h_qs;
     std::vector<vgpu_prog> h_progs;
+    // bucketed passes (k_bt_chain_feed)
+    int32_t chain = 0;
+    int64_t ch_kmin = 0, ch_rpb = 0;
+    int32_t ch_E = 0, ch_m = 0, ch_nsum = 0, ch_G = 0, ch_f64 = 0, ch_key_type = 0;
+    int32_t ch_sum_in[4] = {0, 0, 0, 0}, ch_sum_pred[4] = {0, 0, 0, 0};
+    size_t ch_lds = 0;
+    DevBuf ch_aux, ch_slab, ch_tab;
+    std::vector<uint8_t> h_chain;   // constants in chain order, then chain positions (uploaded)
 };
 
 // ---------------------------------------------------------------------------
@@ -524,6 +532,274 @@ k_bt_emit(GbLayout L, GbAggs AG, BtArgs B, const uint32_t *svals, const uint64_t
 }
 
 // ---------------------------------------------------------------------------
+// bucketed passes: queries that differ only in a comparison constant
+//
+// When every query of a batch reads the same columns and filters them with
+// `col op param_q` (op one of <, <=, >, >=), the row sets the queries
+// select form a chain under inclusion: ordered by decreasing size (by
+// param, with a NaN param at the end its comparison rule puts it: NaN
+// compares as "equal", so >= and <= select every row and > and < none),
+// a row that passes the query at chain position c passes every query
+// before it.  A row is then described by ONE number T, the count of the
+// queries it passes (found by binary search over the constants in chain
+// order, each comparison made exactly as the per-query filter makes it),
+// and the query at position c selects the rows with T >= c.  One scan
+// aggregates every selected row once into a dense LDS table keyed by
+// (group key, T); the query at position c gets, per group, the sum of the
+// group's buckets c..K.  Counts and integer sums are exact; double sums
+// are the same values added in another order.
+//
+// Shape: int64 / bool group key over a dense range (the NA group one more
+// entry), count / sum / mean aggregates of inputs without NAs, key order.
+// Blocks of 1,024 threads take contiguous row ranges, write their whole
+// table to a slab of their own, one launch sums the slabs, one expands
+// every (query, group) into the query's table of the arena; the finish is
+// the batch's usual one.
+// ---------------------------------------------------------------------------
+
+#define BT_CH_THREADS 1024
+#define BT_CH_U 4
+#define BT_CH_MAX_SUMS 4
+#define BT_CH_LDS (160 * 1024 - 2048)   // dynamic LDS a bucketed feed block may take
+
+struct BtChainArgs {
+    vgpu_col key;          // the group key column (batches without a join)
+    vgpu_col probe_key;    // join batches: the probe key column
+    vgpu_col pcol;         // the compared column
+    InSet ins;
+    BtJoin jn;
+    PredInfo pi;           // the comparison (operator, types); constants in ks
+    const uint64_t *ks;    // the active queries' constants in chain order
+    const int32_t *rank;   // per query: its chain position (1..K)
+    uint64_t *slab;        // per block: E*m counts, then E*m*n_sum sums
+    uint64_t *tab;         // the slabs summed
+    uint64_t *bctl;        // the batch's control words
+    int64_t n_rows, rows_per_block;
+    int64_t kmin;          // group key of entry 0
+    int32_t E;             // entries: the key range, then the NA group
+    int32_t m;             // bucket slots per entry (the batch's queries)
+    int32_t K;             // active queries
+    int32_t n_sum, key_type, G;
+    int32_t f64_mask;      // bit s: sum s adds doubles (else int64 / bool bits, wrapping)
+    int32_t sum_in[BT_CH_MAX_SUMS];
+    int32_t sum_pred[BT_CH_MAX_SUMS];   // sum s reads the compared column itself
+};
+
+// the count of the active queries a valid compared value passes
+__device__ __forceinline__ int bt_chain_T(const BtChainArgs &A, const uint64_t *s_k, uint64_t bits) {
+    int lo = 0, hi = A.K;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (vg_fast_cmp(A.pi, bits, s_k[mid - 1])) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// dense entry of a group key, the NA group last; -1 outside the plan's range
+__device__ __forceinline__ int64_t bt_chain_entry(const BtChainArgs &A, bool valid, uint64_t bits) {
+    if (!valid) return A.E - 1;
+    const uint64_t d = bits - (uint64_t)A.kmin;
+    return d < (uint64_t)(A.E - 1) ? (int64_t)d : -1;
+}
+
+__device__ __forceinline__ void bt_chain_add(const BtChainArgs &A, uint32_t *cnt, uint64_t *sums, int64_t e, int T,
+                                             const uint64_t (&sv)[BT_CH_MAX_SUMS]) {
+    const int64_t idx = e * A.m + (T - 1);
+    atomicAdd(&cnt[idx], 1u);
+#pragma unroll
+    for (int s = 0; s < BT_CH_MAX_SUMS; s++) {
+        if (s >= A.n_sum) break;
+        uint64_t *p = &sums[idx * A.n_sum + s];
+        if ((A.f64_mask >> s) & 1) atomicAdd(reinterpret_cast<double *>(p), vg_bits_to_f64(sv[s]));
+        else atomicAdd((unsigned long long *)p, (unsigned long long)sv[s]);
+    }
+}
+
+__device__ __forceinline__ void bt_chain_load_sums(const BtChainArgs &A, int64_t row, uint64_t pv,
+                                                   uint64_t (&sv)[BT_CH_MAX_SUMS]) {
+#pragma unroll
+    for (int s = 0; s < BT_CH_MAX_SUMS; s++) {
+        sv[s] = 0;
+        if (s < A.n_sum) sv[s] = A.sum_pred[s] ? pv : vg_load_bits(A.ins.c[A.sum_in[s]], row);
+    }
+}
+
+__global__ void __launch_bounds__(BT_CH_THREADS) k_bt_chain_feed(BtChainArgs A) {
+    extern __shared__ uint64_t dynw[];
+    uint64_t *s_k = dynw;                                   // m constants
+    uint64_t *sums = dynw + A.m;                            // E*m*n_sum
+    const int64_t Em = (int64_t)A.E * A.m;
+    uint32_t *cnt = reinterpret_cast<uint32_t *>(sums + Em * A.n_sum);   // E*m
+    for (int i = threadIdx.x; i < A.K; i += BT_CH_THREADS) s_k[i] = A.ks[i];
+    for (int64_t i = threadIdx.x; i < Em * A.n_sum; i += BT_CH_THREADS) sums[i] = 0;
+    for (int64_t i = threadIdx.x; i < Em; i += BT_CH_THREADS) cnt[i] = 0;
+    __syncthreads();
+    const int64_t r0 = (int64_t)blockIdx.x * A.rows_per_block;
+    const int64_t r1 = r0 + A.rows_per_block < A.n_rows ? r0 + A.rows_per_block : A.n_rows;
+    bool err = false;
+    // every column a row needs is loaded in one phase (the columns are read
+    // whole anyway: selectivities leave no cache line unread), then the
+    // bucket is found and the row aggregated
+    for (int64_t base = r0 + threadIdx.x; base < r1; base += (int64_t)BT_CH_THREADS * BT_CH_U) {
+        int T[BT_CH_U];
+        uint64_t pv[BT_CH_U], kb[BT_CH_U];
+        bool kv[BT_CH_U];
+        uint64_t sv[BT_CH_U][BT_CH_MAX_SUMS];
+        const vgpu_col &kc = A.jn.on ? A.probe_key : A.key;
+#pragma unroll
+        for (int u = 0; u < BT_CH_U; u++) {
+            const int64_t row = base + (int64_t)u * BT_CH_THREADS;
+            const bool in = row < r1;
+            const bool ok = in && vg_valid(A.pcol.validity, row);
+            pv[u] = ok ? vg_load_bits(A.pcol, row) : 0;
+            T[u] = ok ? 1 : 0;
+            kv[u] = in && vg_valid(kc.validity, row);
+            kb[u] = kv[u] ? vg_load_bits(kc, row) : 0;
+            bt_chain_load_sums(A, in ? row : r0, pv[u], sv[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < BT_CH_U; u++)
+            if (T[u]) T[u] = bt_chain_T(A, s_k, pv[u]);
+        if (A.jn.on) {
+            const uint64_t *ent[BT_CH_U];
+#pragma unroll
+            for (int u = 0; u < BT_CH_U; u++) {
+                ent[u] = nullptr;
+                if (T[u]) {
+                    uint64_t kn;
+                    const int cls = vg_key_class_bits(kc.type, kv[u], kb[u], kn);
+                    ent[u] = jt_find_key(A.jn.jv, cls, kn);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < BT_CH_U; u++) {
+                if (!ent[u]) continue;
+                const uint64_t segw = ent[u][1];
+                const uint32_t n = jt_count(A.jn.jv, segw), off = (uint32_t)(segw >> 32);
+                const int p = A.jn.key_payload;
+                for (uint32_t j = 0; j < n; j++) {
+                    bool v;
+                    uint64_t bits;
+                    if (j == 0) {
+                        v = (segw >> (24 + p)) & 1;
+                        bits = ent[u][2 + p];
+                    } else {
+                        v = A.jn.jv.cval[p][off + j];
+                        bits = A.jn.jv.cpay[p][off + j];
+                    }
+                    const int64_t e = bt_chain_entry(A, v, bits);
+                    if (e < 0) err = true;
+                    else bt_chain_add(A, cnt, sums, e, T[u], sv[u]);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < BT_CH_U; u++) {
+                if (!T[u]) continue;
+                const int64_t e = bt_chain_entry(A, kv[u], kb[u]);
+                if (e < 0) err = true;
+                else bt_chain_add(A, cnt, sums, e, T[u], sv[u]);
+            }
+        }
+    }
+    if (err) atomicOr((unsigned long long *)&A.bctl[BT_CTL_ERR], 1ull);
+    __syncthreads();
+    uint64_t *slab = A.slab + (size_t)blockIdx.x * (size_t)Em * (1 + A.n_sum);
+    for (int64_t i = threadIdx.x; i < Em; i += BT_CH_THREADS) slab[i] = cnt[i];
+    for (int64_t i = threadIdx.x; i < Em * A.n_sum; i += BT_CH_THREADS) slab[Em + i] = sums[i];
+}
+
+// tab[w] = the sum over the blocks' slabs of word w (in block order)
+__global__ void k_bt_chain_reduce(BtChainArgs A) {
+    const int64_t Em = (int64_t)A.E * A.m, W = Em * (1 + A.n_sum);
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= W) return;
+    const bool f64 = w >= Em && ((A.f64_mask >> (int)((w - Em) % A.n_sum)) & 1);
+    if (f64) {
+        double acc = 0.0;
+        for (int g = 0; g < A.G; g++) acc += vg_bits_to_f64(A.slab[(size_t)g * W + w]);
+        A.tab[w] = vg_f64_to_bits(acc);
+    } else {
+        uint64_t acc = 0;
+        for (int g = 0; g < A.G; g++) acc += A.slab[(size_t)g * W + w];
+        A.tab[w] = acc;
+    }
+}
+
+// every (active query, entry): the merge of the entry's buckets from the
+// query's chain position on, into the query's table when it has rows
+__global__ void k_bt_chain_expand(GbLayout L, BtArgs B, BtChainArgs A) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)A.K * A.E) return;
+    const int q = (int)(i / A.E), e = (int)(i % A.E);
+    const int r = A.rank[q];
+    const int64_t Em = (int64_t)A.E * A.m;
+    uint64_t n = 0;
+    uint64_t su[BT_CH_MAX_SUMS];
+    double sf[BT_CH_MAX_SUMS];
+#pragma unroll
+    for (int s = 0; s < BT_CH_MAX_SUMS; s++) {
+        su[s] = 0;
+        sf[s] = 0.0;
+    }
+    for (int b = r - 1; b < A.K; b++) {
+        const int64_t idx = (int64_t)e * A.m + b;
+        n += A.tab[idx];
+#pragma unroll
+        for (int s = 0; s < BT_CH_MAX_SUMS; s++) {
+            if (s >= A.n_sum) break;
+            const uint64_t w = A.tab[Em + idx * A.n_sum + s];
+            if ((A.f64_mask >> s) & 1) sf[s] += vg_bits_to_f64(w);
+            else su[s] += w;
+        }
+    }
+    if (n == 0) return;
+    const BtQuery &Q = B.qs[q];
+    uint64_t *hot = B.hot + (size_t)Q.base * L.RH;
+    uint64_t *ctl = B.ctl + (size_t)q * GB_CTL_WORDS;
+    uint64_t kn = 0;
+    const int cls = e == A.E - 1 ? VG_KEY_NA
+                                 : vg_key_class_bits(A.key_type, true, (uint64_t)A.kmin + (uint64_t)e, kn);
+    const int64_t gs = gb_global_slot(L, hot, Q.cap, cls, kn, 0, ctl);
+    if (gs < 0) return;
+    uint64_t *h = hot + (size_t)gs * L.RH;
+    gb_atomic_add(&h[1], n);
+#pragma unroll
+    for (int s = 0; s < BT_CH_MAX_SUMS; s++) {
+        if (s >= A.n_sum) break;
+        uint64_t *w = &h[L.w_sum[A.sum_in[s]]];
+        if ((A.f64_mask >> s) & 1) gb_atomic_addf(w, sf[s]);
+        else gb_atomic_add(w, su[s]);
+    }
+}
+
+// min and max (order-preserving maps) of the valid values of an int64 / bool column
+__global__ void k_bt_minmax(vgpu_col c, int64_t n, unsigned long long *mm) {
+    __shared__ unsigned long long lo, hi;
+    if (threadIdx.x == 0) {
+        lo = ~0ull;
+        hi = 0;
+    }
+    __syncthreads();
+    unsigned long long a = ~0ull, z = 0;
+    for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < n;
+         row += (int64_t)gridDim.x * blockDim.x) {
+        if (!vg_valid(c.validity, row)) continue;
+        const unsigned long long o = vg_load_bits(c, row) ^ 0x8000000000000000ull;
+        a = o < a ? o : a;
+        z = o > z ? o : z;
+    }
+    if (a != ~0ull) atomicMin(&lo, a);
+    if (z) atomicMax(&hi, z);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (lo != ~0ull) atomicMin(&mm[0], lo);
+        if (hi) atomicMax(&mm[1], hi);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 
@@ -622,6 +898,123 @@ static int64_t bt_estimate(const uint32_t *r, uint64_t nan_rows) {
     return (int64_t)std::ceil(e) + (int64_t)nan_rows + 2;
 }
 
+// ---- host side of the bucketed passes
+
+// Can the batch's queries (all reading Q0's columns) share one bucketed
+// aggregation?  Fills the sums' description when they can.
+static bool bt_chain_shape(vgpu_batch *b, const BtQuery &Q0) {
+    const GbLayout &L = b->L;
+    if (vg_tune(VG_TUNE_BT_BUCKETED) < 2 || !b->has_pred || b->param_insn < 0 || !Q0.pi.fast) return false;
+    const int op = Q0.pi.cmp_op;
+    if (op != VGPU_OP_LT && op != VGPU_OP_LE && op != VGPU_OP_GT && op != VGPU_OP_GE) return false;
+    if (L.key_none || L.key_is_double || b->order != VGPU_ORDER_KEY || L.w_fpos >= 0) return false;
+    const int kt = b->join.on ? b->join.key_type : b->key_type;
+    if (kt != VGPU_INT64 && kt != VGPU_BOOL) return false;
+    for (int a = 0; a < b->aggs.n; a++) {
+        const int k = b->aggs.a[a].kind;
+        if (k != VGPU_AGG_COUNT_STAR && k != VGPU_AGG_COUNT && k != VGPU_AGG_SUM && k != VGPU_AGG_MEAN) return false;
+    }
+    const vgpu_col &pc = Q0.pcs.c[Q0.pi.col];
+    int ns = 0, f64 = 0;
+    for (int i = 0; i < L.n_inputs; i++) {
+        if (Q0.ins.c[i].validity) return false;
+        if (L.w_sum[i] < 0) continue;
+        if (ns == BT_CH_MAX_SUMS) return false;
+        b->ch_sum_in[ns] = i;
+        b->ch_sum_pred[ns] = Q0.ins.c[i].data == pc.data && Q0.ins.c[i].type == pc.type;
+        if (L.in_type[i] == VGPU_DOUBLE) f64 |= 1 << ns;
+        ns++;
+    }
+    b->ch_nsum = ns;
+    b->ch_f64 = f64;
+    b->ch_key_type = kt;
+    return true;
+}
+
+// key range of the group key (one synchronization), table and launch
+// sizes; leaves b->chain at 0 when the dense table does not fit in LDS
+static int bt_chain_plan(vgpu_batch *b, const BtQuery &Q0) {
+    hipStream_t st = vg_stream();
+    const vgpu_col kc = b->join.on ? b->join_payload : Q0.key;
+    const int64_t kn = b->join.on ? b->join_rows : Q0.n_rows;
+    DevBuf mm;
+    if (mm.reserve(16)) return -1;
+    unsigned long long h[2] = {~0ull, 0ull};
+    VG_TRY(hipMemcpyAsync(mm.ptr, h, 16, hipMemcpyHostToDevice, st));
+    if (kn > 0) {
+        k_bt_minmax<<<vg_blocks(kn, 256, 2048), 256, 0, st>>>(kc, kn, mm.as<unsigned long long>());
+        VG_LAUNCH_CHECK();
+    }
+    VG_TRY(hipMemcpyAsync(h, mm.ptr, 16, hipMemcpyDeviceToHost, st));
+    VG_TRY(hipStreamSynchronize(st));
+    int64_t range = 0;   // valid key values lo..hi
+    if (h[0] <= h[1]) {
+        const unsigned long long d = h[1] - h[0];
+        if (d >= (1ull << 24)) return 0;
+        range = (int64_t)d + 1;
+    }
+    const int64_t E = range + 1, m = b->n_queries, Em = E * m;
+    const size_t lds = (size_t)m * 8 + (size_t)Em * b->ch_nsum * 8 + (size_t)Em * 4;
+    if (lds > BT_CH_LDS) return 0;
+    const int64_t step = (int64_t)BT_CH_THREADS * BT_CH_U;
+    const int64_t bpc = lds <= 78 * 1024 ? 2 : 1;   // 2 blocks of 1,024 threads fill a CU's 32 waves
+    int64_t G = std::min<int64_t>((int64_t)vg_cus() * bpc, std::max<int64_t>(1, (Q0.n_rows + step - 1) / step));
+    const int64_t rpb = (Q0.n_rows + G - 1) / G;
+    if (rpb > (int64_t)0xFFFFFFFF) return 0;   // LDS counts are 32-bit
+    const size_t W = (size_t)Em * (1 + b->ch_nsum);
+    if (b->ch_slab.reserve((size_t)G * W * 8) || b->ch_tab.reserve(W * 8) ||
+        b->ch_aux.reserve((size_t)m * 12 + 16))
+        return -1;
+    if (lds > 65536)
+        VG_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(k_bt_chain_feed),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    b->ch_kmin = h[0] <= h[1] ? (int64_t)(h[0] ^ 0x8000000000000000ull) : 0;
+    b->ch_E = (int32_t)E;
+    b->ch_m = (int32_t)m;
+    b->ch_G = (int32_t)G;
+    b->ch_rpb = rpb;
+    b->ch_lds = lds;
+    b->chain = 1;
+    return 0;
+}
+
+// The active queries' chain: ordered by decreasing selected set.  A larger
+// constant selects fewer rows for > and >=, more for < and <=; a NaN
+// constant selects every row for >= and <= (first) and none for > and <
+// (last).  Uploads the constants in chain order and every query's position.
+static int bt_chain_upload(vgpu_batch *b) {
+    const int K = b->n_active, m = b->ch_m;
+    const PredInfo &P = b->h_qs[0].pi;
+    const bool dbl = P.cmp_type == VGPU_DOUBLE;
+    const bool asc = P.cmp_op == VGPU_OP_GT || P.cmp_op == VGPU_OP_GE;
+    const bool nan_all = P.cmp_op == VGPU_OP_GE || P.cmp_op == VGPU_OP_LE;
+    auto f64 = [](uint64_t k) {
+        double d;
+        memcpy(&d, &k, 8);
+        return d;
+    };
+    auto cls = [&](uint64_t k) { return (dbl && std::isnan(f64(k))) ? (nan_all ? 0 : 2) : 1; };
+    std::vector<int> ord((size_t)K);
+    for (int q = 0; q < K; q++) ord[q] = q;
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
+        const uint64_t kx = b->h_qs[x].pi.k, ky = b->h_qs[y].pi.k;
+        const int cx = cls(kx), cy = cls(ky);
+        if (cx != cy) return cx < cy;
+        if (cx != 1) return false;
+        if (dbl) return asc ? f64(kx) < f64(ky) : f64(kx) > f64(ky);
+        return asc ? (int64_t)kx < (int64_t)ky : (int64_t)kx > (int64_t)ky;
+    });
+    b->h_chain.assign((size_t)m * 12, 0);
+    uint64_t *ks = reinterpret_cast<uint64_t *>(b->h_chain.data());
+    int32_t *rank = reinterpret_cast<int32_t *>(b->h_chain.data() + (size_t)m * 8);
+    for (int pos = 0; pos < K; pos++) {
+        ks[pos] = b->h_qs[ord[pos]].pi.k;
+        rank[ord[pos]] = pos + 1;
+    }
+    VG_TRY(hipMemcpyAsync(b->ch_aux.ptr, b->h_chain.data(), b->h_chain.size(), hipMemcpyHostToDevice, vg_stream()));
+    return 0;
+}
+
 extern "C" int vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_queries) {
     if (b->planned) {
         vg_set_error("batch: already planned");
@@ -701,13 +1094,14 @@ extern "C" int vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_q
     b->n_items = (int64_t)items.size();
     // a shared scan when every query reads the same columns
     auto same_col = [](const vgpu_col &a, const vgpu_col &c) { return a.data == c.data && a.validity == c.validity; };
-    bool shared = n_queries > 1;
-    for (int q = 1; q < n_queries && shared; q++) {
+    bool same = true;
+    for (int q = 1; q < n_queries && same; q++) {
         const BtQuery &A = qs[0], &C2 = qs[q];
-        shared = A.n_rows == C2.n_rows && same_col(A.key, C2.key) && same_col(A.probe_key, C2.probe_key);
-        for (int i = 0; i < L.n_inputs && shared; i++) shared = same_col(A.ins.c[i], C2.ins.c[i]);
-        for (int i = 0; i < b->n_pred_cols && shared; i++) shared = same_col(A.pcs.c[i], C2.pcs.c[i]);
+        same = A.n_rows == C2.n_rows && same_col(A.key, C2.key) && same_col(A.probe_key, C2.probe_key);
+        for (int i = 0; i < L.n_inputs && same; i++) same = same_col(A.ins.c[i], C2.ins.c[i]);
+        for (int i = 0; i < b->n_pred_cols && same; i++) same = same_col(A.pcs.c[i], C2.pcs.c[i]);
     }
+    const bool shared = same && n_queries > 1;
     // measured on MI355X with 1B-row tables: the per-row work, not HBM,
     // bounds these queries and query blocks drift apart, so shared scans
     // ran slower than one query after another; they stay behind a knob
@@ -842,7 +1236,12 @@ extern "C" int vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_q
     b->h_progs = progs;
     VG_TRY(hipMemcpyAsync(b->qbase.ptr, qbase.data(), qbase.size() * 8, hipMemcpyHostToDevice, st));
 
-    // 2. the first run fixes the sort range and the record count
+    // 2. queries that differ only in a comparison constant share one
+    // bucketed aggregation
+    if (same && bt_chain_shape(b, qs[0]) && bt_chain_plan(b, qs[0])) return -1;
+    if (b->chain && bt_chain_upload(b)) return -1;
+
+    // 3. the first run fixes the sort range and the record count
     if (bt_enqueue(b, true)) return -1;
     b->planned = 1;
     return 0;
@@ -886,8 +1285,46 @@ static int bt_enqueue(vgpu_batch *b, bool first) {
     k_bt_init_ctl<<<vg_blocks(std::max<int64_t>(ctl_words, b->n_queries), 256), 256, 0, st>>>(
         A.ctl, b->n_queries, qcount, A.queue);
     VG_LAUNCH_CHECK();
-    // 2. one persistent feed launch
-    if (b->n_items > 0) {
+    // 2. one persistent feed launch, or the bucketed pass
+    if (b->chain && b->n_items > 0) {
+        const BtQuery &Q0 = b->h_qs[0];
+        BtChainArgs C;
+        memset(&C, 0, sizeof C);
+        C.key = Q0.key;
+        C.probe_key = Q0.probe_key;
+        C.pcol = Q0.pcs.c[Q0.pi.col];
+        C.ins = Q0.ins;
+        C.jn = b->join;
+        C.pi = Q0.pi;
+        C.ks = b->ch_aux.as<uint64_t>();
+        C.rank = reinterpret_cast<const int32_t *>(b->ch_aux.as<uint8_t>() + (size_t)b->ch_m * 8);
+        C.slab = b->ch_slab.as<uint64_t>();
+        C.tab = b->ch_tab.as<uint64_t>();
+        C.bctl = bctl;
+        C.n_rows = Q0.n_rows;
+        C.rows_per_block = b->ch_rpb;
+        C.kmin = b->ch_kmin;
+        C.E = b->ch_E;
+        C.m = b->ch_m;
+        C.K = b->n_active;
+        C.n_sum = b->ch_nsum;
+        C.key_type = b->ch_key_type;
+        C.G = b->ch_G;
+        C.f64_mask = b->ch_f64;
+        for (int s2 = 0; s2 < BT_CH_MAX_SUMS; s2++) {
+            C.sum_in[s2] = b->ch_sum_in[s2];
+            C.sum_pred[s2] = b->ch_sum_pred[s2];
+        }
+        const int64_t W = (int64_t)C.E * C.m * (1 + C.n_sum);
+        vg_kt_begin();
+        k_bt_chain_feed<<<(unsigned)C.G, BT_CH_THREADS, b->ch_lds, st>>>(C);
+        VG_LAUNCH_CHECK();
+        k_bt_chain_reduce<<<vg_blocks(W, 256), 256, 0, st>>>(C);
+        VG_LAUNCH_CHECK();
+        k_bt_chain_expand<<<vg_blocks((int64_t)C.K * C.E, 256), 256, 0, st>>>(L, A, C);
+        VG_LAUNCH_CHECK();
+        vg_kt_end();
+    } else if (b->n_items > 0) {
         size_t lds = (size_t)b->vm_lds +
                      (b->S_max ? (size_t)(b->S_max + 2) * (L.RH + (b->lds_cold ? L.RC : 0)) * 8 : 0);
         vg_kt_begin();
@@ -1008,6 +1445,7 @@ extern "C" int vgpu_batch_rerun(vgpu_batch *b, const uint64_t *params, int n_act
     VG_TRY(hipMemcpyAsync(b->progs.ptr, b->h_progs.data(), b->h_progs.size() * sizeof(vgpu_prog),
                           hipMemcpyHostToDevice, st));
     b->n_active = n_active;
+    if (b->chain && bt_chain_upload(b)) return -1;
     return bt_enqueue(b, true);
 }
 
@@ -1128,13 +1566,16 @@ extern "C" int64_t vgpu_batch_fetch(vgpu_batch *b, int q, void *keys, uint8_t *k
 extern "C" int64_t vgpu_batch_device_bytes(const vgpu_batch *b) {
     const DevBuf *bufs[] = {&b->qs, &b->progs, &b->items, &b->cmap, &b->queue, &b->hot, &b->cold, &b->ctl,
                             &b->qbase, &b->qcount, &b->qoff, &b->bcount, &b->boff, &b->por, &b->pand, &b->skeys,
-                            &b->svals, &b->sort_tmp, &b->scan_tmp, &b->out_keys, &b->out_kvalid};
+                            &b->svals, &b->sort_tmp, &b->scan_tmp, &b->out_keys, &b->out_kvalid,
+                            &b->ch_aux, &b->ch_slab, &b->ch_tab};
     int64_t total = 0;
     for (const DevBuf *p : bufs) total += (int64_t)p->cap;
     for (auto &x : b->out_vals) total += (int64_t)x.cap;
     for (auto &x : b->out_valid) total += (int64_t)x.cap;
     return total;
 }
+
+extern "C" int vgpu_batch_bucketed(const vgpu_batch *b) { return b && b->planned && b->chain ? 1 : 0; }
 
 extern "C" void vgpu_batch_destroy(vgpu_batch *b) {
     if (!b) return;

@@ -52,18 +52,22 @@ __host__ __device__ __forceinline__ int vg_result_type(const vgpu_insn &in) {
 }
 
 // value of a loaded column in the comparison type
-__host__ __device__ __forceinline__ bool vg_fast_pred(const PredInfo &pi, const ColSet &cs, int64_t row) {
-    const vgpu_col &c = cs.c[pi.col];
-    if (!vg_valid(c.validity, row)) return false;
-    uint64_t bits = vg_load_bits(c, row);
+// `value op k` for the loaded bits of a valid value of the fast path's column
+__host__ __device__ __forceinline__ bool vg_fast_cmp(const PredInfo &pi, uint64_t bits, uint64_t k) {
     int cmp;
     if (pi.cmp_type == VGPU_DOUBLE) {
         double v = pi.col_type == VGPU_DOUBLE ? vg_bits_to_f64(bits) : (double)(int64_t)bits;
-        cmp = vg_cmp3_f64(v, vg_bits_to_f64(pi.k));
+        cmp = vg_cmp3_f64(v, vg_bits_to_f64(k));
     } else {
-        cmp = vg_cmp3_i64((int64_t)bits, (int64_t)pi.k);
+        cmp = vg_cmp3_i64((int64_t)bits, (int64_t)k);
     }
     return vg_cmp_apply(cmp, pi.cmp_op);
+}
+
+__host__ __device__ __forceinline__ bool vg_fast_pred(const PredInfo &pi, const ColSet &cs, int64_t row) {
+    const vgpu_col &c = cs.c[pi.col];
+    if (!vg_valid(c.validity, row)) return false;
+    return vg_fast_cmp(pi, vg_load_bits(c, row), pi.k);
 }
 
 // General VM.  regs/regv: LDS arrays [n_insn][stride], this thread's column

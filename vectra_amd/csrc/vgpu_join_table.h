@@ -67,11 +67,9 @@ struct vgpu_jtable {
     }
 };
 
-// Entry of a probe key, or nullptr: NA and NaN keys never match
-// (src/join.c:85, ==).
-__device__ __forceinline__ const uint64_t *jt_find(const JView &v, const vgpu_col &kc, int64_t row) {
-    uint64_t kn;
-    int cls = vg_key_class(kc, row, kn);
+// Entry of a classified probe key (vg_key_class), or nullptr: NA and NaN
+// keys never match (src/join.c:85, ==).
+__device__ __forceinline__ const uint64_t *jt_find_key(const JView &v, int cls, uint64_t kn) {
     if (cls == VG_KEY_NA || cls == VG_KEY_NAN) return nullptr;
     if (cls == VG_KEY_SPECIAL) return v.ent + (size_t)v.cap * v.E;
     const uint64_t empty = v.key_is_double ? VG_EMPTY_F64 : VG_EMPTY_I64;
@@ -85,6 +83,12 @@ __device__ __forceinline__ const uint64_t *jt_find(const JView &v, const vgpu_co
         s = (s + 1) & mask;
     }
     return nullptr;
+}
+
+__device__ __forceinline__ const uint64_t *jt_find(const JView &v, const vgpu_col &kc, int64_t row) {
+    uint64_t kn;
+    const int cls = vg_key_class(kc, row, kn);
+    return jt_find_key(v, cls, kn);
 }
 
 __device__ __forceinline__ uint32_t jt_count(const JView &v, uint64_t segw) {

@@ -217,8 +217,9 @@ extern "C" double vgpu_timer_stop(void) {
 static const char *g_tune_names[VG_TUNE_COUNT] = {"gb.lds.max_blocks", "gb.lds.rows_per_block", "gb.glb.max_blocks",
                                                   "gb.glb.rows_per_block", "gb.lds.max_slots", "gb.lds.budget",
                                                   "gb.star.items", "gb.part.min_groups", "batch.shared_scan",
-                                                  "gb.lds.slots_per_group"};
-static int64_t g_tune[VG_TUNE_COUNT] = {1024, 8192, 16384, 1024, 4096, 65536, 4, 65536, 1, 2};
+                                                  "gb.lds.slots_per_group", "batch.bucketed"};
+#define VG_TUNE_DEFAULTS {1024, 8192, 16384, 1024, 4096, 65536, 4, 65536, 1, 2, 2}
+static int64_t g_tune[VG_TUNE_COUNT] = VG_TUNE_DEFAULTS;
 
 struct TuneBucket {
     int64_t max_groups;
@@ -302,7 +303,7 @@ extern "C" int64_t vgpu_tune_get_for(int64_t groups, const char *name) {
 extern "C" void vgpu_tune_reset(void) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_buckets.clear();
-    const int64_t defaults[VG_TUNE_COUNT] = {1024, 8192, 16384, 1024, 4096, 65536, 4, 65536, 1, 2};
+    const int64_t defaults[VG_TUNE_COUNT] = VG_TUNE_DEFAULTS;
     memcpy(g_tune, defaults, sizeof g_tune);
 }
 

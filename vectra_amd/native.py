@@ -277,6 +277,7 @@ def vgpu() -> C.CDLL:
                                                     C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)])
         _proto(lib, "vgpu_batch_destroy", None, [C.c_void_p])
         _proto(lib, "vgpu_batch_device_bytes", C.c_int64, [C.c_void_p])
+        _proto(lib, "vgpu_batch_bucketed", C.c_int, [C.c_void_p])
         _libs["vgpu"] = lib
         # the autotuner's table for this build, unless the environment names one
         table = PKG / "tuning.txt"
@@ -862,6 +863,13 @@ class GroupBy:
             self.h = None
 
 
+def _param_bits(p) -> int:
+    """the 8 bytes of a predicate constant: a float's double bits, an int's two's complement"""
+    if isinstance(p, float):
+        return int(np.array([p], np.float64).view(np.uint64)[0])
+    return int(p) & (2**64 - 1)
+
+
 class Batch:
     """Kernel-level batch of filter -> group-by queries of one shape
     (include/vgpu.h, vgpu_batch_*)."""
@@ -899,8 +907,7 @@ class Batch:
             qa[i].inputs = C.cast(ins, C.POINTER(VgpuCol))
             qa[i].pred_cols = C.cast(pcs, C.POINTER(VgpuCol))
             qa[i].n_rows = q["n_rows"]
-            p = q.get("param", 0)
-            qa[i].param = int(np.array([p], np.float64).view(np.uint64)[0]) if isinstance(p, float) else int(p) & (2**64 - 1)
+            qa[i].param = _param_bits(q.get("param", 0))
         self.n_queries = len(queries)
         self._keep = (qa, keep)
         _vcheck(vgpu().vgpu_batch_plan(self.h, qa, len(queries)))
@@ -912,9 +919,13 @@ class Batch:
     def run(self) -> None:
         _vcheck(vgpu().vgpu_batch_run(self.h))
 
-    def rerun(self, params: Sequence[float], n_active: int | None = None) -> None:
-        p = np.asarray(params, np.float64).view(np.uint64)
+    def rerun(self, params: Sequence[float | int], n_active: int | None = None) -> None:
+        """params: floats (double constants) or ints (int64 constants)"""
+        p = np.array([_param_bits(x) for x in params], np.uint64)
         _vcheck(vgpu().vgpu_batch_rerun(self.h, p.ctypes.data, len(p) if n_active is None else n_active))
+
+    def bucketed(self) -> bool:
+        return bool(vgpu().vgpu_batch_bucketed(self.h))
 
     def capture(self) -> None:
         _vcheck(vgpu().vgpu_batch_capture(self.h))
