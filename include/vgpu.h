@@ -354,6 +354,10 @@ void    vgpu_batch_destroy(vgpu_batch *b);
  *                         a `col < <= > >= param` filter aggregate every
  *                         row once, keyed by (group, the number of the
  *                         queries it passes)
+ *   "gb.star.dense"      (2: on; 1: off) fused join -> aggregate feeds of
+ *                         the common star shape (unique build keys, an
+ *                         int64 / bool payload group key of a short range)
+ *                         aggregate into a dense LDS table
  * Returns -1 for an unknown name or a value <= 0. */
 int     vgpu_tune_set(const char *name, int64_t value);
 int64_t vgpu_tune_get(const char *name);

@@ -67,6 +67,59 @@ def test_fused_join_aggregate(gpu, oracle, kind, dup, payload):
         c.free()
 
 
+@pytest.mark.parametrize("kind", ["inner", "left"])
+@pytest.mark.parametrize("temp_dir", ["/tmp", None])
+def test_fused_dense_group_table(gpu, oracle, kind, temp_dir):
+    """the dense-LDS star kernel (unique build keys, int64 payload group key
+    of a short range) against the oracle and against the general kernel
+    (knob gb.star.dense = 1): NA group keys and NA weights in the dimension,
+    NA and INT64_MIN probe keys, first-seen and key order"""
+    nv = gpu
+    r = rng(11)
+    fkey, value, dim_key, attr, weight = star_schema(7, 150_000, 12_000)
+    dim_key[5] = np.iinfo(np.int64).min
+    fkey[r.random(len(fkey)) < 0.01] = np.iinfo(np.int64).min
+    fvalid = r.random(len(fkey)) >= 0.02
+    avalid = r.random(len(attr)) >= 0.05
+    wvalid = r.random(len(weight)) >= 0.03
+    left = nv.Table({"fkey": (fkey, fvalid), "value": value}, rg_rows=1 << 20)
+    right = nv.Table({"key": dim_key, "attr": (attr, avalid), "weight": (weight, wvalid)})
+    j = nv.collect(nv.join_node("oracle", left.scan(), right.scan(), kind, [(0, 0)]))
+    wv, wm = j["weight"]
+    prod = np.where(wm, j["value"][0] * wv, 0.0)
+    t = nv.Table({"attr": j["attr"], "s_in": (prod, wm), "ab": np.abs(prod)})
+    exp = nv.collect(nv.group_agg_node("oracle", t.scan(), ["attr"],
+                                       [("s", "sum", "s_in", 0), ("n", "n", None, 0), ("c", "count", "s_in", 0),
+                                        ("m", "mean", "s_in", 1), ("sa", "sum", "ab", 0)], temp_dir))
+    sa = exp.pop("sa")[0]
+    fk, val = nv.DeviceColumn(fkey, fvalid), nv.DeviceColumn(value)
+    dk, at, wt = nv.DeviceColumn(dim_key), nv.DeviceColumn(attr, avalid), nv.DeviceColumn(weight, wvalid)
+    jt = nv.JoinTable(dk, [at, wt])
+    order = nv.VGPU_ORDER_KEY if temp_dir else nv.VGPU_ORDER_FIRST_SEEN
+    lib = nv.vgpu()
+    outs = []
+    for dense in (2, 1):
+        old = lib.vgpu_tune_get(b"gb.star.dense")
+        lib.vgpu_tune_set(b"gb.star.dense", dense)
+        try:
+            gb = nv.GroupBy(nv.VEC_INT64, [nv.VEC_DOUBLE], [("sum", 0, 0), ("n", -1, 0), ("count", 0, 0),
+                                                              ("mean", 0, 1)], order, 101)
+            gb.feed_join(len(fkey), [fk, val], [dk, at, wt], [(0, 1, 1, 2)], key=(1, 1), join=jt,
+                         join_kind=nv.JOIN_NAMES[kind], probe_key_col=0)
+            gb.finish()
+            got = gb.fetch()
+            gb.free()
+        finally:
+            lib.vgpu_tune_set(b"gb.star.dense", old)
+        got = {"attr": got["key"], "s": got["a0"], "n": got["a1"], "c": got["a2"], "m": got["a3"]}
+        cnt = np.maximum(exp["c"][0], 1)
+        assert_tables_equal(got, exp, {"s": sa, "m": sa / cnt})
+        outs.append(got)
+    jt.free()
+    for c in (fk, val, dk, at, wt):
+        c.free()
+
+
 def test_fused_filter_groupby(gpu, oracle):
     nv = gpu
     r = rng(9)

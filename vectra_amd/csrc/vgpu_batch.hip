@@ -774,30 +774,6 @@ __global__ void k_bt_chain_expand(GbLayout L, BtArgs B, BtChainArgs A) {
     }
 }
 
-// min and max (order-preserving maps) of the valid values of an int64 / bool column
-__global__ void k_bt_minmax(vgpu_col c, int64_t n, unsigned long long *mm) {
-    __shared__ unsigned long long lo, hi;
-    if (threadIdx.x == 0) {
-        lo = ~0ull;
-        hi = 0;
-    }
-    __syncthreads();
-    unsigned long long a = ~0ull, z = 0;
-    for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < n;
-         row += (int64_t)gridDim.x * blockDim.x) {
-        if (!vg_valid(c.validity, row)) continue;
-        const unsigned long long o = vg_load_bits(c, row) ^ 0x8000000000000000ull;
-        a = o < a ? o : a;
-        z = o > z ? o : z;
-    }
-    if (a != ~0ull) atomicMin(&lo, a);
-    if (z) atomicMax(&hi, z);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        if (lo != ~0ull) atomicMin(&mm[0], lo);
-        if (hi) atomicMax(&mm[1], hi);
-    }
-}
 
 // ---------------------------------------------------------------------------
 // host side
@@ -942,7 +918,7 @@ static int bt_chain_plan(vgpu_batch *b, const BtQuery &Q0) {
     unsigned long long h[2] = {~0ull, 0ull};
     VG_TRY(hipMemcpyAsync(mm.ptr, h, 16, hipMemcpyHostToDevice, st));
     if (kn > 0) {
-        k_bt_minmax<<<vg_blocks(kn, 256, 2048), 256, 0, st>>>(kc, kn, mm.as<unsigned long long>());
+        k_key_minmax<<<vg_blocks(kn, 256, 2048), 256, 0, st>>>(kc, kn, mm.as<unsigned long long>());
         VG_LAUNCH_CHECK();
     }
     VG_TRY(hipMemcpyAsync(h, mm.ptr, 16, hipMemcpyDeviceToHost, st));

@@ -220,11 +220,16 @@ static inline bool gb_needs_cold(const GbLayout &L, const vgpu_col *inputs) {
 // The LDS-private table holds S hashed records plus two reserved ones (NA
 // key, EMPTY-valued int64 key), like the HBM table's slots cap / cap+1, so
 // that no row class funnels every block into one HBM record.
+//
+// A dense table (dense = true) serves int64 / bool keys of a short range:
+// key k owns record k - kmin, its key word preset, so a row finds its
+// record without hashing or probing; keys outside the range take the HBM
+// table.  The merge skips the records no row touched (n == 0).
 __device__ __forceinline__ void gb_lds_init(const GbLayout &L, uint64_t *lhot, uint64_t *lcold, int S,
-                                            uint64_t empty) {
+                                            uint64_t empty, bool dense = false, int64_t kmin = 0) {
     for (int s = threadIdx.x; s < S + 2; s += GB_THREADS) {
         uint64_t *h = lhot + (size_t)s * L.RH;
-        h[0] = empty;
+        h[0] = (dense && s < S) ? (uint64_t)kmin + (uint64_t)s : empty;
         for (int w = 1; w < L.RH; w++) h[w] = ((L.hot_init_ones >> w) & 1) ? ~0ull : 0ull;
         if (!lcold) continue;   // a table without cold words (see gb_lds_merge)
         uint64_t *c = lcold + (size_t)s * L.RC;
@@ -242,6 +247,15 @@ __device__ __forceinline__ int64_t gb_lds_slot(uint64_t *lhot, int S, int RH, in
     return gb_slot(lhot, S, RH, kn, empty, GB_LDS_PROBES, nullptr);
 }
 
+// record of a key in a dense table; -1: outside the range (or a NaN key)
+__device__ __forceinline__ int64_t gb_lds_dense_slot(int S, int cls, uint64_t kn, int64_t kmin) {
+    if (cls == VG_KEY_NA) return S;
+    if (cls == VG_KEY_SPECIAL) return S + 1;
+    if (cls == VG_KEY_NAN) return -1;
+    const uint64_t d = kn - (uint64_t)kmin;
+    return d < (uint64_t)S ? (int64_t)d : -1;
+}
+
 __device__ __forceinline__ void gb_lds_merge(const GbLayout &L, const uint64_t *lhot, const uint64_t *lcold, int S,
                                              uint64_t empty, uint64_t *hot, uint64_t *cold, int64_t cap,
                                              uint64_t *ctl) {
@@ -250,7 +264,7 @@ __device__ __forceinline__ void gb_lds_merge(const GbLayout &L, const uint64_t *
         const uint64_t *h = lhot + (size_t)s * L.RH;
         int64_t gs;
         if (s < S) {
-            if (h[0] == empty) continue;
+            if (h[0] == empty || h[1] == 0) continue;
             gs = gb_global_slot(L, hot, cap, VG_KEY_REGULAR, h[0], 0, ctl);
         } else {
             if (h[1] == 0) continue;

@@ -181,6 +181,25 @@ static vgpu_jtable *join_build_impl(const vgpu_col *key, int64_t n_build, const 
     if (vg_scan_u32_to_u64(cnt.as<uint32_t>(), off.as<uint64_t>(), slots, total, t->scan_tmp)) return fail();
     k_jt_seg<<<vg_blocks(slots, 256), 256, 0, s>>>(cnt.as<uint32_t>(), off.as<uint64_t>(), slots,
                                                    t->ent.as<uint64_t>(), E, misc.as<uint32_t>() + 1);
+    // value ranges of the int64 / bool payload columns, read back with the
+    // build's counts (one synchronization)
+    DevBuf pmm;
+    unsigned long long hr[2 * JT_MAX_PAYLOAD];
+    if (n_payload > 0) {
+        for (int p = 0; p < n_payload; p++) {
+            hr[2 * p] = ~0ull;
+            hr[2 * p + 1] = 0;
+        }
+        if (pmm.reserve(sizeof hr) ||
+            hipMemcpyAsync(pmm.ptr, hr, (size_t)n_payload * 16, hipMemcpyHostToDevice, s) != hipSuccess)
+            return fail();
+        for (int p = 0; p < n_payload; p++)
+            if (n_build > 0 && (payload[p].type == VGPU_INT64 || payload[p].type == VGPU_BOOL))
+                k_key_minmax<<<vg_blocks(n_build, 256, 1024), 256, 0, s>>>(payload[p], n_build,
+                                                                            pmm.as<unsigned long long>() + 2 * p);
+        if (hipMemcpyAsync(hr, pmm.ptr, (size_t)n_payload * 16, hipMemcpyDeviceToHost, s) != hipSuccess)
+            return fail();
+    }
     uint64_t hm[8];
     if (hipMemcpyAsync(hm, misc.ptr, 64, hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
@@ -196,6 +215,13 @@ static vgpu_jtable *join_build_impl(const vgpu_col *key, int64_t n_build, const 
         vg_set_error("join_build: a key repeats %u times; payload embedding needs fewer than 2^24", maxc);
         return fail();
     }
+    for (int p = 0; p < n_payload; p++) {
+        const bool ranged = (payload[p].type == VGPU_INT64 || payload[p].type == VGPU_BOOL) && hr[2 * p] <= hr[2 * p + 1];
+        t->pay_ranged[p] = ranged;
+        t->pay_lo[p] = ranged ? (int64_t)(hr[2 * p] ^ 0x8000000000000000ull) : 0;
+        t->pay_hi[p] = ranged ? (int64_t)(hr[2 * p + 1] ^ 0x8000000000000000ull) : 0;
+    }
+    pmm.release();
     t->n_valid = (int64_t)hm[2];
     t->max_dup = (int32_t)std::max<uint32_t>(maxc, t->n_valid > 0 ? 1u : 0u);
     if (t->rows.reserve((size_t)std::max<int64_t>(n_build, 1) * 4)) return fail();

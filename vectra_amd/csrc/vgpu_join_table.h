@@ -42,6 +42,10 @@ struct vgpu_jtable {
     int32_t E = 2;
     int32_t n_payload = 0;
     vgpu_col payload[JT_MAX_PAYLOAD];
+    // value range of every int64 / bool payload column (pay_ranged[p] = 0:
+    // a double column, or no valid values)
+    int64_t pay_lo[JT_MAX_PAYLOAD] = {0}, pay_hi[JT_MAX_PAYLOAD] = {0};
+    int32_t pay_ranged[JT_MAX_PAYLOAD] = {0};
     const uint64_t *bkey = nullptr;   // caller's build key column (kept alive by the caller)
     DevBuf ent, rows;
     DevBuf cpay[JT_MAX_PAYLOAD], cval[JT_MAX_PAYLOAD];

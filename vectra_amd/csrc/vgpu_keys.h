@@ -37,3 +37,31 @@ __device__ __forceinline__ int vg_key_class(const vgpu_col &c, int64_t row, uint
 }
 
 __device__ __forceinline__ uint64_t vg_key_hash(uint64_t kn) { return vg_mix64(kn ^ 0x9e3779b97f4a7c15ull); }
+
+// min and max over the valid values of an int64 / bool column, as
+// order-preserving unsigned maps (bits ^ sign): mm[0] takes the min by
+// atomicMin, mm[1] the max by atomicMax (start them at ~0 and 0); a column
+// without valid values leaves mm[0] > mm[1]
+static __global__ void k_key_minmax(vgpu_col c, int64_t n, unsigned long long *mm) {
+    __shared__ unsigned long long lo, hi;
+    if (threadIdx.x == 0) {
+        lo = ~0ull;
+        hi = 0;
+    }
+    __syncthreads();
+    unsigned long long a = ~0ull, z = 0;
+    for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < n;
+         row += (int64_t)gridDim.x * blockDim.x) {
+        if (!vg_valid(c.validity, row)) continue;
+        const unsigned long long o = vg_load_bits(c, row) ^ 0x8000000000000000ull;
+        a = o < a ? o : a;
+        z = o > z ? o : z;
+    }
+    if (a != ~0ull) atomicMin(&lo, a);
+    if (z) atomicMax(&hi, z);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (lo != ~0ull) atomicMin(&mm[0], lo);
+        if (hi) atomicMax(&mm[1], hi);
+    }
+}
