@@ -358,6 +358,8 @@ void    vgpu_batch_destroy(vgpu_batch *b);
  *                         the common star shape (unique build keys, an
  *                         int64 / bool payload group key of a short range)
  *                         aggregate into a dense LDS table
+ *   "join.bloom"         (2: on; 1: off) those feeds test the join table's
+ *                         filter of build keys before loading an entry
  * Returns -1 for an unknown name or a value <= 0. */
 int     vgpu_tune_set(const char *name, int64_t value);
 int64_t vgpu_tune_get(const char *name);

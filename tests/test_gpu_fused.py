@@ -67,16 +67,19 @@ def test_fused_join_aggregate(gpu, oracle, kind, dup, payload):
         c.free()
 
 
-@pytest.mark.parametrize("kind", ["inner", "left"])
-@pytest.mark.parametrize("temp_dir", ["/tmp", None])
-def test_fused_dense_group_table(gpu, oracle, kind, temp_dir):
+@pytest.mark.parametrize("kind,temp_dir,n_fact", [("inner", "/tmp", 150_000), ("left", "/tmp", 150_000),
+                                                   ("inner", None, 150_000), ("left", None, 150_000),
+                                                   ("inner", "/tmp", 1_200_000), ("left", None, 1_200_000)])
+def test_fused_dense_group_table(gpu, oracle, kind, temp_dir, n_fact):
     """the dense-LDS star kernel (unique build keys, int64 payload group key
     of a short range) against the oracle and against the general kernel
-    (knob gb.star.dense = 1): NA group keys and NA weights in the dimension,
-    NA and INT64_MIN probe keys, first-seen and key order"""
+    (knob gb.star.dense = 1), with and without the join table's key filter
+    (knob join.bloom): NA group keys and NA weights in the dimension,
+    NA and INT64_MIN probe keys, first-seen and key order, a feed of more
+    than 2^20 rows"""
     nv = gpu
     r = rng(11)
-    fkey, value, dim_key, attr, weight = star_schema(7, 150_000, 12_000)
+    fkey, value, dim_key, attr, weight = star_schema(7, n_fact, 12_000)
     dim_key[5] = np.iinfo(np.int64).min
     fkey[r.random(len(fkey)) < 0.01] = np.iinfo(np.int64).min
     fvalid = r.random(len(fkey)) >= 0.02
@@ -98,9 +101,10 @@ def test_fused_dense_group_table(gpu, oracle, kind, temp_dir):
     order = nv.VGPU_ORDER_KEY if temp_dir else nv.VGPU_ORDER_FIRST_SEEN
     lib = nv.vgpu()
     outs = []
-    for dense in (2, 1):
-        old = lib.vgpu_tune_get(b"gb.star.dense")
+    for dense, bloom in ((2, 2), (2, 1), (1, 2)):
+        old = [lib.vgpu_tune_get(k) for k in (b"gb.star.dense", b"join.bloom")]
         lib.vgpu_tune_set(b"gb.star.dense", dense)
+        lib.vgpu_tune_set(b"join.bloom", bloom)
         try:
             gb = nv.GroupBy(nv.VEC_INT64, [nv.VEC_DOUBLE], [("sum", 0, 0), ("n", -1, 0), ("count", 0, 0),
                                                               ("mean", 0, 1)], order, 101)
@@ -110,7 +114,8 @@ def test_fused_dense_group_table(gpu, oracle, kind, temp_dir):
             got = gb.fetch()
             gb.free()
         finally:
-            lib.vgpu_tune_set(b"gb.star.dense", old)
+            for k, v in zip((b"gb.star.dense", b"join.bloom"), old):
+                lib.vgpu_tune_set(k, v)
         got = {"attr": got["key"], "s": got["a0"], "n": got["a1"], "c": got["a2"], "m": got["a3"]}
         cnt = np.maximum(exp["c"][0], 1)
         assert_tables_equal(got, exp, {"s": sa, "m": sa / cnt})

@@ -544,6 +544,8 @@ struct StarDenseArgs {
     vgpu_col pval;          // probe operand (input modes 0 and 2)
     const uint64_t *ent;    // join entries: key, segment word, payload 0, payload 1
     uint64_t jmask;         // join slots - 1 (the EMPTY-valued key's entry at jmask + 1)
+    const uint64_t *bloom;  // the table's filter of build keys (nullptr: none)
+    uint64_t bloom_mask;
     int64_t n_logical, pos_base;
     int64_t kmin;           // group key of dense record 0
     uint64_t *hot, *cold, *ctl;
@@ -558,6 +560,18 @@ struct StarDenseArgs {
     int32_t sum_f64;               // the input is double
 };
 
+__device__ __forceinline__ void star_dense_update(const StarDenseArgs &A, uint64_t *h, uint64_t *c, uint64_t pos,
+                                                  InVal v) {
+    gb_atomic_add(&h[1], 1);
+    if (A.w_fpos >= 0) gb_atomic_min(&h[A.w_fpos], pos);
+    if (!v.valid) {
+        gb_atomic_add(&c[A.c_na], 1);
+    } else if (A.w_sum >= 0) {
+        if (A.sum_f64) gb_atomic_addf(&h[A.w_sum], vg_bits_to_f64(v.bits));
+        else gb_atomic_add(&h[A.w_sum], v.bits);
+    }
+}
+
 // IN_MODE: 0 = probe column, 1 = payload column, 2 = their product
 template <int ITEMS, int IN_MODE, bool LEFT>
 __global__ void __launch_bounds__(GB_THREADS) k_gb_star_dense(GbLayout L, StarDenseArgs A) {
@@ -570,7 +584,7 @@ __global__ void __launch_bounds__(GB_THREADS) k_gb_star_dense(GbLayout L, StarDe
     const int64_t step = (int64_t)gridDim.x * GB_THREADS * ITEMS;
     for (int64_t base = (int64_t)blockIdx.x * GB_THREADS * ITEMS; base < A.n_logical; base += step) {
         bool live[ITEMS], found[ITEMS], pv_ok[ITEMS];
-        uint64_t kn[ITEMS], pv[ITEMS], slot[ITEMS];
+        uint64_t kn[ITEMS], pv[ITEMS], slot[ITEMS], bw[ITEMS], bb[ITEMS];
         uint64_t e0[ITEMS], e1[ITEMS], e2[ITEMS], e3[ITEMS];
         // every row's probe-side loads, then every first entry load
 #pragma unroll
@@ -578,19 +592,29 @@ __global__ void __launch_bounds__(GB_THREADS) k_gb_star_dense(GbLayout L, StarDe
             const int64_t li = base + (int64_t)i * GB_THREADS + tid;
             live[i] = li < A.n_logical;
             const bool kv = live[i] && vg_valid(A.pkey.validity, li);
-            const int cls = vg_key_class_bits(A.pkey.type, kv, kv ? vg_load_bits(A.pkey, li) : 0, kn[i]);
+            const int cls = vg_key_class_bits(A.pkey.type, kv, kv ? vg_load_bits_stream(A.pkey, li) : 0, kn[i]);
             found[i] = cls == VG_KEY_REGULAR || cls == VG_KEY_SPECIAL;
-            slot[i] = cls == VG_KEY_SPECIAL ? A.jmask + 1 : (vg_key_hash(kn[i]) & A.jmask);
+            const uint64_t kh = vg_key_hash(kn[i]);
+            slot[i] = cls == VG_KEY_SPECIAL ? A.jmask + 1 : (kh & A.jmask);
+            e0[i] = e1[i] = e2[i] = e3[i] = 0;
+            // the filter word now; a regular key whose bits are missing is
+            // absent from the table and loads no entry
+            bw[i] = ~0ull;
+            bb[i] = 0;
+            if (A.bloom && cls == VG_KEY_REGULAR) {
+                bw[i] = A.bloom[jt_bloom_word(kh, A.bloom_mask)];
+                bb[i] = jt_bloom_bits(kh);
+            }
             pv_ok[i] = false;
             pv[i] = 0;
             if (IN_MODE != 1) {
                 pv_ok[i] = live[i] && vg_valid(A.pval.validity, li);
-                pv[i] = pv_ok[i] ? vg_load_bits(A.pval, li) : 0;
+                pv[i] = pv_ok[i] ? vg_load_bits_stream(A.pval, li) : 0;
             }
         }
 #pragma unroll
         for (int i = 0; i < ITEMS; i++) {
-            e0[i] = e1[i] = e2[i] = e3[i] = 0;
+            found[i] = found[i] && (bw[i] & bb[i]) == bb[i];
             if (found[i]) {
                 const uint64_t *e = A.ent + slot[i] * 4;
                 e0[i] = e[0];
@@ -646,23 +670,13 @@ __global__ void __launch_bounds__(GB_THREADS) k_gb_star_dense(GbLayout L, StarDe
                 }
             }
             const int64_t ls = gb_lds_dense_slot(A.S, gcls, gk, A.kmin);
-            uint64_t *h, *c;
+            // one expansion per address space, so the LDS one compiles to
+            // ds_* atomics rather than flat ones
             if (ls >= 0) {
-                h = lhot + (size_t)ls * A.RH;
-                c = lcold + (size_t)ls * A.RC;
+                star_dense_update(A, lhot + (size_t)ls * A.RH, lcold + (size_t)ls * A.RC, pos, v);
             } else {
                 const int64_t gs = gb_global_slot(L, A.hot, A.cap, gcls, gk, pos, A.ctl);
-                if (gs < 0) continue;
-                h = A.hot + (size_t)gs * A.RH;
-                c = A.cold + (size_t)gs * A.RC;
-            }
-            gb_atomic_add(&h[1], 1);
-            if (A.w_fpos >= 0) gb_atomic_min(&h[A.w_fpos], pos);
-            if (!v.valid) {
-                gb_atomic_add(&c[A.c_na], 1);
-            } else if (A.w_sum >= 0) {
-                if (A.sum_f64) gb_atomic_addf(&h[A.w_sum], vg_bits_to_f64(v.bits));
-                else gb_atomic_add(&h[A.w_sum], v.bits);
+                if (gs >= 0) star_dense_update(A, A.hot + (size_t)gs * A.RH, A.cold + (size_t)gs * A.RC, pos, v);
             }
         }
     }
@@ -1146,6 +1160,18 @@ __device__ __forceinline__ int64_t gbp_lds_slot(uint64_t *lhot, int S, int RH, i
     return gb_slot(lhot, S, RH, kn, empty, S, nullptr);
 }
 
+// a staged row (key, inputs, meta word) into one record
+__device__ __forceinline__ void gbp_apply_staged(const GbLayout &L, uint64_t *h, uint64_t *c, const uint64_t *w,
+                                                 uint64_t meta, uint64_t pos) {
+    gb_update_row(L, h, pos);
+    for (int i = 0; i < L.n_inputs; i++) {
+        InVal v;
+        v.valid = (meta >> (34 + i)) & 1;
+        v.bits = w[1 + i];
+        gb_update_input(L, h, c, i, pos, v);
+    }
+}
+
 __global__ void __launch_bounds__(GB_THREADS) k_gbp_agg(GbLayout L, GbpArgs G) {
     extern __shared__ uint64_t dynw[];   // 8-byte aligned after the static arrays
     __shared__ uint16_t slist[1024];
@@ -1173,23 +1199,13 @@ __global__ void __launch_bounds__(GB_THREADS) k_gbp_agg(GbLayout L, GbpArgs G) {
         int cls = (int)((meta >> 32) & 3);
         uint64_t pos = (uint64_t)G.pos_base + (uint32_t)meta;
         int64_t ls = gbp_lds_slot(lhot, S, L.RH, cls, kn, empty);
-        uint64_t *h, *c;
+        // one expansion per address space: ds_* atomics for the LDS record
         if (ls >= 0) {
-            h = lhot + (size_t)ls * L.RH;
-            c = lcold + (size_t)ls * L.RC;
+            gbp_apply_staged(L, lhot + (size_t)ls * L.RH, lcold + (size_t)ls * L.RC, w, meta, pos);
         } else {
             int64_t gs = gb_global_slot(L, G.hot, G.cap, cls, kn, pos, G.ctl);
             spilled = true;
-            if (gs < 0) continue;
-            h = G.hot + (size_t)gs * L.RH;
-            c = G.cold + (size_t)gs * L.RC;
-        }
-        gb_update_row(L, h, pos);
-        for (int i = 0; i < L.n_inputs; i++) {
-            InVal v;
-            v.valid = (meta >> (34 + i)) & 1;
-            v.bits = w[1 + i];
-            gb_update_input(L, h, c, i, pos, v);
+            if (gs >= 0) gbp_apply_staged(L, G.hot + (size_t)gs * L.RH, G.cold + (size_t)gs * L.RC, w, meta, pos);
         }
     }
     if (spilled) G.ctl[9] = 1;
@@ -1704,6 +1720,10 @@ static bool gb_star_dense_shape(const vgpu_groupby *g, const vgpu_feed_spec *sp,
     JView jv = t->view();
     sd->ent = jv.ent;
     sd->jmask = (uint64_t)jv.cap - 1;
+    if (vg_tune(VG_TUNE_JOIN_BLOOM) >= 2) {
+        sd->bloom = jv.bloom;
+        sd->bloom_mask = jv.bloom_mask;
+    }
     sd->n_logical = st.n_logical;
     sd->pos_base = st.pos_base;
     sd->kmin = t->pay_lo[p];

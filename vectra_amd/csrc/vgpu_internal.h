@@ -21,7 +21,7 @@ void vg_kt_end();
 // launch-shape knobs (see vgpu_tune_set)
 enum { VG_TUNE_GB_LDS_MAX_BLOCKS, VG_TUNE_GB_LDS_ROWS, VG_TUNE_GB_GLB_MAX_BLOCKS, VG_TUNE_GB_GLB_ROWS,
        VG_TUNE_GB_LDS_MAX_SLOTS, VG_TUNE_GB_LDS_BUDGET, VG_TUNE_GB_STAR_ITEMS, VG_TUNE_GB_PART_MIN_GROUPS,
-       VG_TUNE_BT_SHARED, VG_TUNE_GB_LDS_SLOTS_PER_GROUP, VG_TUNE_BT_BUCKETED, VG_TUNE_GB_STAR_DENSE, VG_TUNE_COUNT };
+       VG_TUNE_BT_SHARED, VG_TUNE_GB_LDS_SLOTS_PER_GROUP, VG_TUNE_BT_BUCKETED, VG_TUNE_GB_STAR_DENSE, VG_TUNE_JOIN_BLOOM, VG_TUNE_COUNT };
 int64_t vg_tune(int knob);
 // the knob for a group-by expecting `groups` groups (tuning table buckets)
 int64_t vg_tune_for(int knob, int64_t groups);
@@ -141,6 +141,13 @@ __host__ __device__ __forceinline__ bool vg_valid(const uint8_t *validity, int64
 __host__ __device__ __forceinline__ uint64_t vg_load_bits(const vgpu_col &c, int64_t row) {
     if (c.type == VGPU_BOOL) return (uint64_t)static_cast<const uint8_t *>(c.data)[row];
     return static_cast<const uint64_t *>(c.data)[row];
+}
+
+// the same for a column streamed once: the load is marked non-temporal so
+// its lines do not push the probed table's hot lines out of the caches
+__device__ __forceinline__ uint64_t vg_load_bits_stream(const vgpu_col &c, int64_t row) {
+    if (c.type == VGPU_BOOL) return (uint64_t)static_cast<const uint8_t *>(c.data)[row];
+    return __builtin_nontemporal_load(static_cast<const uint64_t *>(c.data) + row);
 }
 
 __host__ __device__ __forceinline__ double vg_bits_to_f64(uint64_t b) { return __builtin_bit_cast(double, b); }

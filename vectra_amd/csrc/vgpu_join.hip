@@ -36,8 +36,13 @@ __global__ void k_jt_init(uint64_t *ent, int64_t n, int E, uint64_t empty, uint3
     }
 }
 
+__global__ void k_jt_zero(uint64_t *p, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
 __global__ void k_jt_insert(vgpu_col key, int64_t n, uint64_t *ent, int E, int64_t cap, uint32_t *cnt,
-                            uint32_t *slot_of, uint32_t *err) {
+                            uint32_t *slot_of, uint32_t *err, uint64_t *bloom, uint64_t bloom_mask) {
     int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     uint64_t kn;
@@ -50,7 +55,9 @@ __global__ void k_jt_insert(vgpu_col key, int64_t n, uint64_t *ent, int E, int64
     if (cls == VG_KEY_REGULAR) {
         const uint64_t empty = key.type == VGPU_DOUBLE ? VG_EMPTY_F64 : VG_EMPTY_I64;
         uint64_t mask = (uint64_t)cap - 1;
-        uint64_t h = vg_key_hash(kn) & mask;
+        const uint64_t kh = vg_key_hash(kn);
+        if (bloom) atomicOr((unsigned long long *)&bloom[jt_bloom_word(kh, bloom_mask)], jt_bloom_bits(kh));
+        uint64_t h = kh & mask;
         s = -1;
         for (int64_t probes = 0; probes < cap; probes++) {
             uint64_t *kp = &ent[h * E];
@@ -171,11 +178,18 @@ static vgpu_jtable *join_build_impl(const vgpu_col *key, int64_t n_build, const 
     uint64_t empty = t->key_type == VGPU_DOUBLE ? VG_EMPTY_F64 : VG_EMPTY_I64;
     if (hipMemsetAsync(misc.ptr, 0, 64, s) != hipSuccess) { vg_set_error("join_build: memset failed"); return fail(); }
     k_jt_init<<<vg_blocks(slots, 256, 8192), 256, 0, s>>>(t->ent.as<uint64_t>(), slots, E, empty, cnt.as<uint32_t>());
+    // the filter: about 16 bits per build key, a power of two of words
+    t->bloom_words = 64;
+    while (t->bloom_words < (uint64_t)n_build / 4 && t->bloom_words < (1ull << 22)) t->bloom_words <<= 1;
+    if (t->bloom.reserve(t->bloom_words * 8)) return fail();
+    k_jt_zero<<<vg_blocks((int64_t)t->bloom_words, 256), 256, 0, s>>>(t->bloom.as<uint64_t>(),
+                                                                     (int64_t)t->bloom_words);
     if (n_build > 0) {
         vgpu_col k = *key;
         k_jt_insert<<<vg_blocks(n_build, JT_THREADS), JT_THREADS, 0, s>>>(k, n_build, t->ent.as<uint64_t>(), E, cap,
                                                                           cnt.as<uint32_t>(), slot_of.as<uint32_t>(),
-                                                                          misc.as<uint32_t>());
+                                                                          misc.as<uint32_t>(), t->bloom.as<uint64_t>(),
+                                                                          t->bloom_words - 1);
     }
     uint64_t *total = misc.as<uint64_t>() + 2;
     if (vg_scan_u32_to_u64(cnt.as<uint32_t>(), off.as<uint64_t>(), slots, total, t->scan_tmp)) return fail();

@@ -31,7 +31,21 @@ struct JView {
     int32_t n_payload;
     int32_t key_is_double;
     int32_t pad;
+    const uint64_t *bloom;   // blocked Bloom filter of the regular build keys (jt_bloom_*)
+    uint64_t bloom_mask;     // its words - 1
 };
+
+// Blocked Bloom filter over the regular build keys: one 64-bit word per
+// key, three bits in it, all from the key's table hash above the bits that
+// pick its slot (slots use the low bits; tables stay below 2^24 slots of
+// hash for the word index to be independent, beyond that it is only less
+// selective).  About 16 filter bits per key: a probe of an absent key
+// passes with a probability near 1%, and skips the table's random line
+// otherwise.
+__host__ __device__ __forceinline__ uint64_t jt_bloom_bits(uint64_t h) {
+    return (1ull << ((h >> 46) & 63)) | (1ull << ((h >> 52) & 63)) | (1ull << ((h >> 58) & 63));
+}
+__host__ __device__ __forceinline__ uint64_t jt_bloom_word(uint64_t h, uint64_t mask) { return (h >> 24) & mask; }
 
 struct vgpu_jtable {
     int32_t key_type = VGPU_INT64;
@@ -47,7 +61,8 @@ struct vgpu_jtable {
     int64_t pay_lo[JT_MAX_PAYLOAD] = {0}, pay_hi[JT_MAX_PAYLOAD] = {0};
     int32_t pay_ranged[JT_MAX_PAYLOAD] = {0};
     const uint64_t *bkey = nullptr;   // caller's build key column (kept alive by the caller)
-    DevBuf ent, rows;
+    DevBuf ent, rows, bloom;
+    uint64_t bloom_words = 0;
     DevBuf cpay[JT_MAX_PAYLOAD], cval[JT_MAX_PAYLOAD];
     // probe scratch
     DevBuf cnt, un, off_main, off_un, out_probe, out_build, totals, scan_tmp, sort_tmp;
@@ -67,6 +82,8 @@ struct vgpu_jtable {
         v.E = E;
         v.n_payload = n_payload;
         v.key_is_double = key_type == VGPU_DOUBLE;
+        v.bloom = bloom_words ? bloom.as<uint64_t>() : nullptr;
+        v.bloom_mask = bloom_words ? bloom_words - 1 : 0;
         return v;
     }
 };
