@@ -360,6 +360,9 @@ void    vgpu_batch_destroy(vgpu_batch *b);
  *                         aggregate into a dense LDS table
  *   "join.bloom"         (2: on; 1: off) those feeds test the join table's
  *                         filter of build keys before loading an entry
+ *   "gb.lds.copies"      (2: at most this many copies of an LDS table of up
+ *                         to 256 slots, lane l updating copy l % copies, as
+ *                         the LDS budget allows; 1 = one table)
  * Returns -1 for an unknown name or a value <= 0. */
 int     vgpu_tune_set(const char *name, int64_t value);
 int64_t vgpu_tune_get(const char *name);

@@ -114,11 +114,18 @@ k_gb_feed(GbLayout L, FeedArgs A, InSet ins, PredInfo pi, vgpu_prog prog, ColSet
     const int tid = threadIdx.x;
     uint64_t *regs = reinterpret_cast<uint64_t *>(dyn);
     uint8_t *regv = dyn + (size_t)prog.n_insn * GB_THREADS * 8;
-    uint64_t *lhot = reinterpret_cast<uint64_t *>(dyn + A.vm_lds);
+    uint64_t *lhot0 = reinterpret_cast<uint64_t *>(dyn + A.vm_lds);
     const int S = A.lds_slots;
-    uint64_t *lcold = A.lds_cold ? lhot + (size_t)(S + 2) * L.RH : nullptr;
+    // Few groups: every block keeps lds_copies copies of its table and lane
+    // l updates copy l % lds_copies, so the lanes of one atomic instruction
+    // collide on a record lds_copies times less often.
+    const int NC = A.lds_copies > 1 ? A.lds_copies : 1;
+    uint64_t *lcold0 = A.lds_cold ? lhot0 + (size_t)NC * (S + 2) * L.RH : nullptr;
+    const int copy = (tid & (VG_WAVE - 1)) % NC;
+    uint64_t *lhot = lhot0 + (size_t)copy * (S + 2) * L.RH;
+    uint64_t *lcold = lcold0 ? lcold0 + (size_t)copy * (S + 2) * L.RC : nullptr;
     const uint64_t empty = L.key_is_double ? VG_EMPTY_F64 : VG_EMPTY_I64;
-    if (S) gb_lds_init(L, lhot, lcold, S, empty);
+    if (S) gb_lds_init(L, lhot0, lcold0, NC * (S + 2) - 2, empty);
     // Blocks walk tiles of GB_U * GB_THREADS contiguous rows; each thread's
     // GB_U rows of a tile have their selection, predicate and key loads
     // issued together, so each wave keeps several loads in flight within
@@ -161,7 +168,25 @@ k_gb_feed(GbLayout L, FeedArgs A, InSet ins, PredInfo pi, vgpu_prog prog, ColSet
             }
         }
     }
-    if (S) gb_lds_merge(L, lhot, lcold, S, empty, A.hot, A.cold, A.cap, A.ctl);
+    if (S) {
+        // the copies fold into copy 0 in LDS; one merge into HBM
+        for (int c = 1; c < NC; c++) {
+            __syncthreads();
+            uint64_t *ch = lhot0 + (size_t)c * (S + 2) * L.RH;
+            uint64_t *cc = lcold0 ? lcold0 + (size_t)c * (S + 2) * L.RC : nullptr;
+            for (int s = tid; s < S + 2; s += GB_THREADS) {
+                const uint64_t *h = ch + (size_t)s * L.RH;
+                if (h[1] == 0) continue;
+                const int64_t d = s < S ? gb_lds_slot(lhot0, S, L.RH, VG_KEY_REGULAR, h[0], empty) : s;
+                if (d >= 0)
+                    gb_merge(L, lhot0 + (size_t)d * L.RH, lcold0 ? lcold0 + (size_t)d * L.RC : nullptr, h,
+                             cc ? cc + (size_t)s * L.RC : nullptr);
+                else   // copy 0 has no room for the key: straight to HBM
+                    gb_merge_global(L, h, cc ? cc + (size_t)s * L.RC : nullptr, A.hot, A.cold, A.cap, A.ctl);
+            }
+        }
+        gb_lds_merge(L, lhot0, lcold0, S, empty, A.hot, A.cold, A.cap, A.ctl);
+    }
 }
 
 // fused probe -> aggregate ----------------------------------------------------
@@ -1528,7 +1553,10 @@ static int gb_lds_slots(const vgpu_groupby *g, size_t vm_bytes, bool cold = true
     size_t rec = (size_t)(g->L.RH + (cold ? g->L.RC : 0)) * 8;
     size_t lim = (size_t)vg_tune_for(VG_TUNE_GB_LDS_BUDGET, groups);
     size_t budget = lim > vm_bytes ? lim - vm_bytes : 0;
-    int64_t want = pow2_at_least(vg_tune_for(VG_TUNE_GB_LDS_SLOTS_PER_GROUP, groups) * groups);
+    // the NA and EMPTY-valued key groups the count includes have reserved
+    // records of their own: only the rest are hashed
+    const int64_t hashed = std::max<int64_t>(groups - 2, 1);
+    int64_t want = pow2_at_least(vg_tune_for(VG_TUNE_GB_LDS_SLOTS_PER_GROUP, groups) * hashed);
     if (want > vg_tune_for(VG_TUNE_GB_LDS_MAX_SLOTS, groups) || (size_t)(want + 2) * rec > budget) return 0;
     return (int)want;
 }
@@ -1616,7 +1644,15 @@ extern "C" int vgpu_groupby_feed(vgpu_groupby *g, const vgpu_col *key, const vgp
     if (partitioned) return gbp_feed(g, A, ins, pi, prog, pcs, groups);
     A.lds_cold = gb_needs_cold(g->L, inputs);
     A.lds_slots = gb_lds_slots(g, vm, A.lds_cold);
-    size_t lds = vm + (A.lds_slots ? (size_t)(A.lds_slots + 2) * (g->L.RH + (A.lds_cold ? g->L.RC : 0)) * 8 : 0);
+    const size_t table = A.lds_slots ? (size_t)(A.lds_slots + 2) * (g->L.RH + (A.lds_cold ? g->L.RC : 0)) * 8 : 0;
+    // small tables are replicated while the copies stay within the budget
+    A.lds_copies = 1;
+    if (A.lds_slots && A.lds_slots <= 256) {
+        const size_t budget = (size_t)vg_tune_for(VG_TUNE_GB_LDS_BUDGET, gb_groups(g));
+        const int want = (int)vg_tune_for(VG_TUNE_GB_LDS_COPIES, gb_groups(g));
+        while (A.lds_copies * 2 <= want && vm + table * A.lds_copies * 2 <= budget) A.lds_copies *= 2;
+    }
+    size_t lds = vm + table * A.lds_copies;
     vg_kt_begin();
     k_gb_feed<<<gb_grid(n_logical, A.lds_slots, lds, gb_groups(g)), GB_THREADS, lds, vg_stream()>>>(g->L, A, ins, pi, prog, pcs);
     VG_LAUNCH_CHECK();

@@ -207,7 +207,7 @@ struct FeedArgs {
     int32_t lds_slots;     // 0 = no LDS privatization
     int32_t vm_lds;        // bytes of VM registers at the start of dyn LDS
     int32_t lds_cold;      // LDS records carry cold words (an input has NAs or a double min / max)
-    int32_t pad;
+    int32_t lds_copies;    // copies of the LDS table, lane l updating copy l % lds_copies (1, 2, 4, 8)
 };
 
 // cold words can be touched only by NA inputs and by double min / max
@@ -245,6 +245,13 @@ __device__ __forceinline__ int64_t gb_lds_slot(uint64_t *lhot, int S, int RH, in
     if (cls == VG_KEY_SPECIAL) return S + 1;
     if (cls == VG_KEY_NAN) return -1;
     return gb_slot(lhot, S, RH, kn, empty, GB_LDS_PROBES, nullptr);
+}
+
+// a regular key's LDS record into the HBM table
+__device__ __forceinline__ void gb_merge_global(const GbLayout &L, const uint64_t *h, const uint64_t *c, uint64_t *hot,
+                                                uint64_t *cold, int64_t cap, uint64_t *ctl) {
+    const int64_t gs = gb_global_slot(L, hot, cap, VG_KEY_REGULAR, h[0], 0, ctl);
+    if (gs >= 0) gb_merge(L, hot + (size_t)gs * L.RH, cold + (size_t)gs * L.RC, h, c);
 }
 
 // record of a key in a dense table; -1: outside the range (or a NaN key)
