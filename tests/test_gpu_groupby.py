@@ -189,10 +189,13 @@ def test_groupby_high_cardinality_growth(gpu, oracle):
 
 
 @pytest.mark.parametrize("key_kind,card,temp_dir", [("int", 8, "/tmp"), ("int", 10**9, "/tmp"),
-                                                    ("int", 3000, None), ("dbl", 64, None)])
+                                                    ("int", 3000, None), ("dbl", 64, None),
+                                                    ("int", 300, "/tmp"), ("int", 1000, None)])
 def test_groupby_large_single_feed(gpu, oracle, key_kind, card, temp_dir):
     """one batch of more than 2^20 rows and no group-count hint: a key sketch
-    sizes the table and picks the LDS-private path"""
+    sizes the table and picks the path -- replicated LDS tables for a few
+    keys, a denser LDS table when the tuned load does not fit, the
+    partitioned path when no LDS table holds the wide records"""
     mk = lambda: table(gpu, 1_200_000, 11, key_kind, card, 0, p_na=0.01, p_nan=0.01)
     check(gpu, mk, ["g"], ALL_AGGS, temp_dir)
 

@@ -1,12 +1,15 @@
 """Autotuner of the group-by launch shapes.
 
 For each cardinality bucket it runs the group-by the bucket stands for on
-the GPU (50M rows, int64 keys uniform over the bucket's cardinality, one
-double input, sum and n, key order) under every setting of the knobs that
-decide its shape -- LDS table slots per group (the table's inverse load
-factor), blocks and rows per block of the LDS-privatized feed, and the
-group count from which a large feed is hash-partitioned -- times the feed
-kernels with HIP events, and keeps the fastest setting per bucket.  The
+the GPU (50M rows, int64 keys uniform over the bucket's cardinality, key
+order) with two record widths -- sum and n of one double input, and n with
+sum / min / max / mean of a double and an int64 input -- under every
+setting of the knobs that decide its shape -- LDS table slots per group
+(the table's inverse load factor), copies of a small LDS table, blocks and
+rows per block of the LDS-privatized feed, and the group count from which
+a large feed is hash-partitioned -- times the feed kernels with HIP
+events, and keeps per bucket the setting whose worse relative slowdown
+over the two widths is smallest.  The
 fused join -> aggregate shape (a unique-key dimension of 1M rows, 20M
 probe rows) picks the rows in flight per thread of its probe.
 
@@ -47,6 +50,14 @@ def _feed_ms(run, reps: int = 3) -> float:
     return float(np.median(times))
 
 
+# two record widths: one sum and a count (3 words), and n with sum / min /
+# max / mean of a double and an int64 input (about 14 words); a setting is
+# scored by its worse relative slowdown over the two
+SHAPES = {"narrow": [("sum", 0, 0), ("n", -1, 0)],
+          "wide": [("n", -1, 0), ("sum", 0, 0), ("min", 0, 0), ("max", 0, 0), ("mean", 0, 0),
+                   ("sum", 1, 0), ("min", 1, 0), ("max", 1, 0), ("mean", 1, 0)]}
+
+
 def tune_groupby(card: int, rows: int, grid: dict, log) -> dict:
     import torch
     lib = nv.vgpu()
@@ -54,29 +65,39 @@ def tune_groupby(card: int, rows: int, grid: dict, log) -> dict:
     g.manual_seed(card)
     key = torch.randint(0, card, (rows,), dtype=torch.int64, device="cuda", generator=g)
     val = torch.randn(rows, dtype=torch.float64, device="cuda", generator=g)
+    ival = torch.randint(-10**9, 10**9, (rows,), dtype=torch.int64, device="cuda", generator=g)
     torch.cuda.synchronize()
     kc = nv.DeviceColumn.wrap(key.data_ptr(), rows, nv.VEC_INT64, keep=key)
     vc = nv.DeviceColumn.wrap(val.data_ptr(), rows, nv.VEC_DOUBLE, keep=val)
+    ic = nv.DeviceColumn.wrap(ival.data_ptr(), rows, nv.VEC_INT64, keep=ival)
 
-    def run():
-        gb = nv.GroupBy(nv.VEC_INT64, [nv.VEC_DOUBLE], [("sum", 0, 0), ("n", -1, 0)], nv.VGPU_ORDER_KEY, 0)
-        gb.feed(kc, [vc], rows)
-        gb.finish()
-        gb.free()
+    def runner(shape):
+        aggs = SHAPES[shape]
+        types, cols = ([nv.VEC_DOUBLE], [vc]) if shape == "narrow" else ([nv.VEC_DOUBLE, nv.VEC_INT64], [vc, ic])
 
-    best = None
+        def run():
+            gb = nv.GroupBy(nv.VEC_INT64, types, aggs, nv.VGPU_ORDER_KEY, 0)
+            gb.feed(kc, cols, rows)
+            gb.finish()
+            gb.free()
+        return run
+
+    runs = {s: runner(s) for s in SHAPES}
     names = list(grid)
+    results = []
     for values in itertools.product(*(grid[n] for n in names)):
         for n, v in zip(names, values):
             lib.vgpu_tune_set(n.encode(), v)
-        ms = _feed_ms(run)
-        log(f"  card {card:>9,}: {dict(zip(names, values))} -> {ms:.3f} ms")
-        if best is None or ms < best[0]:
-            best = (ms, dict(zip(names, values)))
+        ms = {s: _feed_ms(r) for s, r in runs.items()}
+        results.append((dict(zip(names, values)), ms))
+        log(f"  card {card:>9,}: {dict(zip(names, values))} -> "
+            + ", ".join(f"{s} {t:.3f} ms" for s, t in ms.items()))
     lib.vgpu_tune_reset()
-    del key, val
+    best_of = {s: min(r[1][s] for r in results) for s in SHAPES}
+    knobs, ms = min(results, key=lambda r: max(r[1][s] / best_of[s] for s in SHAPES))
+    del key, val, ival
     torch.cuda.empty_cache()
-    return {"ms": best[0], "knobs": best[1]}
+    return {"ms": ms, "knobs": knobs}
 
 
 def tune_star(log) -> dict:
@@ -129,11 +150,8 @@ def main(argv=None) -> int:
     t0 = time.time()
     for max_groups, card in BUCKETS:
         if card <= 256:
-            # (gb.lds.copies is left at its default: on this one-sum record
-            # it moves the feed by 1%, on the wide records of many aggregates
-            # two copies are best and four slow the feed twofold)
-            grid = {"gb.lds.slots_per_group": [2, 4], "gb.lds.max_blocks": [256, 512, 1024, 2048],
-                    "gb.lds.rows_per_block": [2048, 8192, 32768]}
+            grid = {"gb.lds.slots_per_group": [2, 4], "gb.lds.copies": [1, 2, 4],
+                    "gb.lds.max_blocks": [1024, 2048], "gb.lds.rows_per_block": [4096, 32768]}
         elif card <= 4096:
             # past the LDS table's reach the choice is HBM atomics or the partitioned path
             grid = {"gb.part.min_groups": [card // 2, 1 << 40], "gb.lds.slots_per_group": [2, 4],
@@ -141,9 +159,10 @@ def main(argv=None) -> int:
         else:
             grid = {"gb.part.min_groups": [16384, 1 << 40], "gb.glb.rows_per_block": [512, 1024, 4096]}
         r = tune_groupby(card, args.rows, grid, log)
-        log(f"bucket <= {max_groups:,} groups: best {r['knobs']} ({r['ms']:.3f} ms)")
+        log(f"bucket <= {max_groups:,} groups: best {r['knobs']} ({r['ms']})")
         for name, value in r["knobs"].items():
-            lines.append(f"{max_groups} {name} {value}   # {card:,} keys: {r['ms']:.3f} ms")
+            t = ", ".join(f"{s} {v:.3f} ms" for s, v in r["ms"].items())
+            lines.append(f"{max_groups} {name} {value}   # {card:,} keys: {t}")
     r = tune_star(log)
     lines.append(f"128 gb.star.items {r['knobs']['gb.star.items']}   # join -> group_by(attr): {r['ms']:.3f} ms")
     head = [f"# group-by launch shapes per cardinality bucket, measured by vectra_amd/autotune.py",

@@ -1375,10 +1375,15 @@ static int gbp_lds_slots(const vgpu_groupby *g) {
     return S >= 64 ? S : 0;
 }
 
-static bool gbp_wanted(const vgpu_groupby *g, int64_t n_logical, int64_t groups) {
-    return !g->any_fed && !g->L.key_none && groups >= vg_tune_for(VG_TUNE_GB_PART_MIN_GROUPS, groups) &&
-           n_logical >= GB_SK_MIN_ROWS &&
+// can this feed take the partitioned path?  (the first feed of a group-by,
+// large, with its group count estimated)
+static bool gbp_possible(const vgpu_groupby *g, int64_t n_logical, int64_t groups) {
+    return !g->any_fed && !g->L.key_none && groups > 0 && n_logical >= GB_SK_MIN_ROWS &&
            n_logical < ((int64_t)1 << 32) - 2 && gbp_lds_slots(g) > 0;
+}
+
+static bool gbp_wanted(const vgpu_groupby *g, int64_t n_logical, int64_t groups) {
+    return gbp_possible(g, n_logical, groups) && groups >= vg_tune_for(VG_TUNE_GB_PART_MIN_GROUPS, groups);
 }
 
 // carves 256-byte aligned pieces out of one buffer
@@ -1556,8 +1561,12 @@ static int gb_lds_slots(const vgpu_groupby *g, size_t vm_bytes, bool cold = true
     // the NA and EMPTY-valued key groups the count includes have reserved
     // records of their own: only the rest are hashed
     const int64_t hashed = std::max<int64_t>(groups - 2, 1);
+    const int64_t max_slots = vg_tune_for(VG_TUNE_GB_LDS_MAX_SLOTS, groups);
     int64_t want = pow2_at_least(vg_tune_for(VG_TUNE_GB_LDS_SLOTS_PER_GROUP, groups) * hashed);
-    if (want > vg_tune_for(VG_TUNE_GB_LDS_MAX_SLOTS, groups) || (size_t)(want + 2) * rec > budget) return 0;
+    // a table that does not fit at the tuned load may still fit at a load
+    // of up to 0.8: far better than every row updating HBM records
+    if (want > max_slots || (size_t)(want + 2) * rec > budget) want = pow2_at_least(hashed + hashed / 4 + 1);
+    if (want > max_slots || (size_t)(want + 2) * rec > budget) return 0;
     return (int)want;
 }
 
@@ -1565,10 +1574,15 @@ static unsigned gb_grid(int64_t n_logical, int lds_slots, size_t lds_bytes, int6
     // privatized blocks amortize their merge over many rows; no more of
     // them than fit on the device at once (a second, partial round of
     // blocks would leave most CUs idle)
-    if (lds_slots)
-        return vg_blocks(n_logical, vg_tune_for(VG_TUNE_GB_LDS_ROWS, groups),
-                         std::min<int64_t>(vg_tune_for(VG_TUNE_GB_LDS_MAX_BLOCKS, groups),
-                                           vg_resident_blocks(lds_bytes)));
+    if (lds_slots) {
+        const int64_t cap = std::min<int64_t>(vg_tune_for(VG_TUNE_GB_LDS_MAX_BLOCKS, groups),
+                                              vg_resident_blocks(lds_bytes));
+        // a short feed (a row group of a few 100K rows) still spreads over
+        // the CUs: at least one block per 2,048 rows, up to one per CU
+        const int64_t floor = std::min<int64_t>(vg_cus(), (n_logical + 2047) / 2048);
+        return (unsigned)std::max<int64_t>(vg_blocks(n_logical, vg_tune_for(VG_TUNE_GB_LDS_ROWS, groups), cap),
+                                           std::min<int64_t>(floor, cap));
+    }
     return vg_blocks(n_logical, vg_tune_for(VG_TUNE_GB_GLB_ROWS, groups),
                      vg_tune_for(VG_TUNE_GB_GLB_MAX_BLOCKS, groups));
 }
@@ -1615,7 +1629,14 @@ extern "C" int vgpu_groupby_feed(vgpu_groupby *g, const vgpu_col *key, const vgp
         incoming = std::min(incoming, est + est / 4 + 4096);
         groups = est;
     }
-    const bool partitioned = gbp_wanted(g, n_logical, groups);
+    PredInfo pi = vg_pred_info(pred);
+    size_t vm = (pi.present && !pi.fast) ? vg_vm_lds_bytes(pred, GB_THREADS) : 0;
+    vm = (vm + 15) & ~(size_t)15;
+    const bool lds_cold = gb_needs_cold(g->L, inputs);
+    // partitioned: many groups, or groups that no LDS table holds (every
+    // row would otherwise update a few hundred HBM records in contention)
+    const bool partitioned = gbp_wanted(g, n_logical, groups) ||
+                             (gbp_possible(g, n_logical, groups) && gb_lds_slots(g, vm, lds_cold) == 0);
     if (!partitioned && gb_ensure_capacity(g, incoming)) return -1;
 
     FeedArgs A;
@@ -1631,18 +1652,15 @@ extern "C" int vgpu_groupby_feed(vgpu_groupby *g, const vgpu_col *key, const vgp
     InSet ins;
     memset(&ins, 0, sizeof ins);
     for (int i = 0; i < g->L.n_inputs; i++) ins.c[i] = inputs[i];
-    PredInfo pi = vg_pred_info(pred);
     vgpu_prog prog;
     memset(&prog, 0, sizeof prog);
     if (pred) prog = *pred;
     ColSet pcs;
     memset(&pcs, 0, sizeof pcs);
     for (int i = 0; i < n_pred_cols && pred_cols; i++) pcs.c[i] = pred_cols[i];
-    size_t vm = (pi.present && !pi.fast) ? vg_vm_lds_bytes(pred, GB_THREADS) : 0;
-    vm = (vm + 15) & ~(size_t)15;
     A.vm_lds = (int32_t)vm;
     if (partitioned) return gbp_feed(g, A, ins, pi, prog, pcs, groups);
-    A.lds_cold = gb_needs_cold(g->L, inputs);
+    A.lds_cold = lds_cold;
     A.lds_slots = gb_lds_slots(g, vm, A.lds_cold);
     const size_t table = A.lds_slots ? (size_t)(A.lds_slots + 2) * (g->L.RH + (A.lds_cold ? g->L.RC : 0)) * 8 : 0;
     // small tables are replicated while the copies stay within the budget
