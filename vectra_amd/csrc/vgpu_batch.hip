@@ -1171,8 +1171,10 @@ extern "C" int vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_q
         // LDS privatization when the query's groups fit and its tiles are long enough to amortize
         // (the estimate counts the NA and EMPTY-valued key groups, which
         // have reserved records)
-        int64_t S = pow2_at_least(vg_tune_for(VG_TUNE_GB_LDS_SLOTS_PER_GROUP, est[q]) *
-                                  std::max<int64_t>(est[q] - 2, 1));
+        const int64_t hashed = std::max<int64_t>(est[q] - 2, 1);
+        int64_t S = pow2_at_least(vg_tune_for(VG_TUNE_GB_LDS_SLOTS_PER_GROUP, est[q]) * hashed);
+        // (a load of up to 0.8 when the tuned load does not fit)
+        if ((size_t)(S + 2) * rw + vm > BT_LDS_BUDGET) S = pow2_at_least(hashed + hashed / 4 + 1);
         if (L.key_none) S = 64;
         Q.lds_slots = 0;
         int64_t tile = shared ? (int64_t)BT_STILE * 64 : b->tile;   // rows one LDS table sees
