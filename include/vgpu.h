@@ -363,6 +363,10 @@ void    vgpu_batch_destroy(vgpu_batch *b);
  *   "gb.lds.copies"      (2: at most this many copies of an LDS table of up
  *                         to 256 slots, lane l updating copy l % copies, as
  *                         the LDS budget allows; 1 = one table)
+ *   "gb.narrow"          (2: on; 1: off) group-by feeds of n / count / sum /
+ *                         mean over at most two inputs, with a fast filter or
+ *                         none, run a lean kernel (dense LDS table when the
+ *                         feed's sketch saw a short int64 / bool key range)
  * Returns -1 for an unknown name or a value <= 0. */
 int     vgpu_tune_set(const char *name, int64_t value);
 int64_t vgpu_tune_get(const char *name);

@@ -84,6 +84,34 @@ def test_groupby_matches_oracle(gpu, oracle, temp_dir, key_kind, card, rg):
     check(gpu, mk, ["g"], ALL_AGGS, temp_dir)
 
 
+# n / count / sum / mean of at most two inputs: the narrow feed kernel
+NARROW_AGGS = [("n", "n", None, 0), ("cx", "count", "x", 0), ("sx", "sum", "x", 0), ("sx_rm", "sum", "x", 1),
+               ("mx_rm", "mean", "x", 1), ("sv", "sum", "v", 1), ("mv", "mean", "v", 0)]
+
+
+@pytest.mark.parametrize("temp_dir", [None, "/tmp"])
+@pytest.mark.parametrize("key_kind,card", [("int", 40), ("int", 30000), ("bool", 0), ("dbl", 8), ("none", 0)])
+@pytest.mark.parametrize("rg", [0, 3001])
+def test_groupby_narrow(gpu, oracle, temp_dir, key_kind, card, rg):
+    """the narrow kernel: NA / NaN / +-0 / INT64_MIN keys, NA inputs, one
+    batch or many, with and without a filter in front"""
+    p_nan = 0.0 if key_kind == "dbl" and temp_dir is not None else 0.02
+    mk = lambda: table(gpu, 40000, 31, key_kind, card, rg, p_nan=p_nan)
+    keys = [] if key_kind == "none" else ["g"]
+    check(gpu, mk, keys, NARROW_AGGS, temp_dir)
+    check(gpu, mk, keys, NARROW_AGGS, temp_dir, ("cmp", ">", ("col", "x"), ("dbl", 0.1)))
+
+
+@pytest.mark.parametrize("card,temp_dir", [(300, "/tmp"), (300, None), (3000, "/tmp"), (10**9, None)])
+def test_groupby_narrow_large_feed(gpu, oracle, card, temp_dir):
+    """one feed of 1.2M rows: the sketch's key range makes a dense LDS table
+    for a few hundred or thousand int64 keys (NA and INT64_MIN keys take
+    their reserved records), a hashed one otherwise"""
+    mk = lambda: table(gpu, 1_200_000, 37, "int", card, 0, p_na=0.01, p_nan=0.01)
+    check(gpu, mk, ["g"], NARROW_AGGS, temp_dir)
+    check(gpu, mk, ["g"], NARROW_AGGS, temp_dir, ("cmp", ">", ("col", "x"), ("dbl", -0.2)))
+
+
 @pytest.mark.parametrize("rg", [0, 5000])
 def test_ungrouped_summarise(gpu, oracle, rg):
     mk = lambda: table(gpu, 30000, 5, "none", 0, rg)

@@ -88,6 +88,8 @@ struct vgpu_groupby {
     int64_t n_groups = 0;     // after finish
     int64_t hint = 0;
     int64_t est_groups = 0;   // groups expected when no hint was given (sketch or groups seen); 0 = unknown
+    int32_t sk_ranged = 0;    // the last sketch saw regular int64 / bool keys in [sk_lo, sk_hi]
+    int64_t sk_lo = 0, sk_hi = 0;
     int32_t finished = 0;
     int32_t any_fed = 0;
     DevBuf hot, cold;
@@ -187,6 +189,105 @@ k_gb_feed(GbLayout L, FeedArgs A, InSet ins, PredInfo pi, vgpu_prog prog, ColSet
         }
         gb_lds_merge(L, lhot0, lcold0, S, empty, A.hot, A.cold, A.cap, A.ctl);
     }
+}
+
+// The narrow shape of most group-bys: n / count / sum / mean of at most two
+// inputs, a fast predicate or none.  The record words it touches come as
+// plain integers, so the row loop keeps few values in scalar registers.
+// DENSE: int64 / bool keys own record k - kmin of the LDS table (the range
+// the feed's sketch saw), found without hashing.
+struct NarrowArgs {
+    int64_t n_logical, pos_base;
+    const int32_t *sel;
+    vgpu_col key;
+    vgpu_col in[2];
+    vgpu_col pcol;            // the fast predicate's column (pi.present)
+    PredInfo pi;
+    uint64_t *hot, *cold, *ctl;
+    int64_t cap, kmin;
+    int32_t S, RH, RC, lds_cold;
+    int32_t w_fpos, key_double, key_none, pad;
+    int32_t w_sum[2], c_na[2], in_f64[2];   // per input: sum word (-1: none), NA count word, double sum
+};
+
+template <int NI>
+__device__ __forceinline__ void narrow_update(const NarrowArgs &A, uint64_t *h, uint64_t *c, uint64_t pos,
+                                              const bool (&iv)[2], const uint64_t (&ib)[2]) {
+    gb_atomic_add(&h[1], 1);
+    if (A.w_fpos >= 0) gb_atomic_min(&h[A.w_fpos], pos);
+#pragma unroll
+    for (int i = 0; i < NI; i++) {
+        if (!iv[i]) {
+            gb_atomic_add(&c[A.c_na[i]], 1);
+        } else if (A.w_sum[i] >= 0) {
+            if (A.in_f64[i]) gb_atomic_addf(&h[A.w_sum[i]], vg_bits_to_f64(ib[i]));
+            else gb_atomic_add(&h[A.w_sum[i]], ib[i]);
+        }
+    }
+}
+
+template <int NI, bool DENSE>
+__global__ void __launch_bounds__(GB_THREADS) k_gb_feed_narrow(GbLayout L, NarrowArgs A) {
+    extern __shared__ uint64_t dyn64[];
+    const int tid = threadIdx.x;
+    const int S = A.S;
+    uint64_t *lhot = dyn64;
+    uint64_t *lcold = A.lds_cold ? lhot + (size_t)(S + 2) * A.RH : nullptr;
+    const uint64_t empty = A.key_double ? VG_EMPTY_F64 : VG_EMPTY_I64;
+    if (S) gb_lds_init(L, lhot, lcold, S, empty, DENSE, A.kmin);
+    const int64_t tile = (int64_t)GB_THREADS * GB_U;
+    for (int64_t t0 = (int64_t)blockIdx.x * tile; t0 < A.n_logical; t0 += (int64_t)gridDim.x * tile) {
+        int64_t row[GB_U];
+        bool ok[GB_U];
+        int cls[GB_U];
+        uint64_t kn[GB_U];
+        bool iv[GB_U][2];
+        uint64_t ib[GB_U][2];
+#pragma unroll
+        for (int u = 0; u < GB_U; u++) {
+            const int64_t li = t0 + tid + (int64_t)u * GB_THREADS;
+            ok[u] = li < A.n_logical;
+            row[u] = ok[u] ? (A.sel ? (int64_t)A.sel[li] : li) : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < GB_U; u++)
+            if (ok[u] && A.pi.present)
+                ok[u] = vg_valid(A.pcol.validity, row[u]) && vg_fast_cmp(A.pi, vg_load_bits(A.pcol, row[u]), A.pi.k);
+#pragma unroll
+        for (int u = 0; u < GB_U; u++) {
+            kn[u] = 0;
+            cls[u] = VG_KEY_REGULAR;
+            if (ok[u] && !A.key_none) cls[u] = vg_key_class(A.key, row[u], kn[u]);
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                iv[u][i] = false;
+                ib[u][i] = 0;
+                if (i < NI && ok[u]) {
+                    iv[u][i] = vg_valid(A.in[i].validity, row[u]);
+                    ib[u][i] = iv[u][i] ? vg_load_bits(A.in[i], row[u]) : 0;
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < GB_U; u++) {
+            if (!ok[u]) continue;
+            const uint64_t pos = (uint64_t)(A.pos_base + t0 + tid + (int64_t)u * GB_THREADS);
+            if (A.key_double && cls[u] == VG_KEY_REGULAR && kn[u] == 0)
+                gb_note_zero_key(A.ctl, vg_load_bits(A.key, row[u]), pos);
+            int64_t ls = -1;
+            if (S) ls = DENSE ? gb_lds_dense_slot(S, cls[u], kn[u], A.kmin) : gb_lds_slot(lhot, S, A.RH, cls[u], kn[u], empty);
+            // one expansion per address space: ds_* atomics for the LDS record
+            if (ls >= 0) {
+                narrow_update<NI>(A, lhot + (size_t)ls * A.RH, lcold ? lcold + (size_t)ls * A.RC : nullptr, pos, iv[u],
+                                  ib[u]);
+            } else {
+                const int64_t gs = gb_global_slot(L, A.hot, A.cap, cls[u], kn[u], pos, A.ctl);
+                if (gs >= 0)
+                    narrow_update<NI>(A, A.hot + (size_t)gs * A.RH, A.cold + (size_t)gs * A.RC, pos, iv[u], ib[u]);
+            }
+        }
+    }
+    if (S) gb_lds_merge(L, lhot, lcold, S, empty, A.hot, A.cold, A.cap, A.ctl);
 }
 
 // fused probe -> aggregate ----------------------------------------------------
@@ -765,10 +866,17 @@ static int gb_read_ctl(vgpu_groupby *g, uint64_t *ctl_host) {
 __global__ void __launch_bounds__(GB_THREADS)
 k_gb_sketch(vgpu_col key, const int32_t *sel, int64_t n_logical, uint32_t *regs, unsigned long long *nan_rows) {
     __shared__ uint32_t r[GB_SK_REGS];
-    __shared__ unsigned long long nn;
+    __shared__ unsigned long long nn, lo, hi;
     for (int i = threadIdx.x; i < GB_SK_REGS; i += GB_THREADS) r[i] = 0;
-    if (threadIdx.x == 0) nn = 0;
+    if (threadIdx.x == 0) {
+        nn = 0;
+        lo = ~0ull;
+        hi = 0;
+    }
     __syncthreads();
+    // the regular keys' range (order-preserving map), for dense LDS tables
+    unsigned long long my_lo = ~0ull, my_hi = 0;
+    const bool ranged = key.type == VGPU_INT64 || key.type == VGPU_BOOL;
     unsigned long long my_nan = 0;
     const int64_t stride = (int64_t)gridDim.x * GB_THREADS;
     for (int64_t li = (int64_t)blockIdx.x * GB_THREADS + threadIdx.x; li < n_logical; li += stride) {
@@ -777,6 +885,11 @@ k_gb_sketch(vgpu_col key, const int32_t *sel, int64_t n_logical, uint32_t *regs,
         int cls = vg_key_class(key, row, kn);
         if (cls == VG_KEY_NAN) { my_nan++; continue; }
         if (cls != VG_KEY_REGULAR) continue;
+        if (ranged) {
+            const unsigned long long o = kn ^ 0x8000000000000000ull;
+            my_lo = o < my_lo ? o : my_lo;
+            my_hi = o > my_hi ? o : my_hi;
+        }
         uint64_t h = vg_mix64(kn);
         uint32_t idx = (uint32_t)(h >> (64 - GB_SK_BITS));
         uint64_t rest = (h << GB_SK_BITS) | (1ull << (GB_SK_BITS - 1));
@@ -785,22 +898,33 @@ k_gb_sketch(vgpu_col key, const int32_t *sel, int64_t n_logical, uint32_t *regs,
         if (r[idx] < rank) atomicMax(&r[idx], rank);
     }
     if (my_nan) atomicAdd(&nn, my_nan);
+    if (my_lo != ~0ull) atomicMin(&lo, my_lo);
+    if (my_hi) atomicMax(&hi, my_hi);
     __syncthreads();
     for (int i = threadIdx.x; i < GB_SK_REGS; i += GB_THREADS)
         if (r[i]) atomicMax(&regs[i], r[i]);
-    if (threadIdx.x == 0 && nn) atomicAdd(nan_rows, nn);
+    if (threadIdx.x == 0) {
+        if (nn) atomicAdd(nan_rows, nn);
+        if (lo != ~0ull) atomicMin(&nan_rows[1], lo);
+        if (hi) atomicMax(&nan_rows[2], hi);
+    }
 }
 
+// nan_rows[0] = NaN-key rows, [1] / [2] = min / max of the regular keys
 __global__ void k_gb_sketch_clear(uint32_t *regs, unsigned long long *nan_rows) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < GB_SK_REGS) regs[i] = 0;
-    if (i == 0) *nan_rows = 0;
+    if (i == 0) {
+        nan_rows[0] = 0;
+        nan_rows[1] = ~0ull;
+        nan_rows[2] = 0;
+    }
 }
 
 // estimated groups of a feed: distinct regular keys + one per NaN row + the
 // two reserved groups (NA key, EMPTY-valued key)
 static int gb_sketch(vgpu_groupby *g, const vgpu_col *key, const int32_t *sel, int64_t n_logical, int64_t *est) {
-    size_t bytes = GB_SK_REGS * 4 + 8;
+    size_t bytes = GB_SK_REGS * 4 + 24;
     if (g->sketch.reserve(bytes)) return -1;
     uint32_t *regs = g->sketch.as<uint32_t>();
     auto *nan_rows = reinterpret_cast<unsigned long long *>(regs + GB_SK_REGS);
@@ -809,7 +933,7 @@ static int gb_sketch(vgpu_groupby *g, const vgpu_col *key, const int32_t *sel, i
     k_gb_sketch<<<vg_blocks(n_logical, 65536, 2048), GB_THREADS, 0, vg_stream()>>>(*key, sel, n_logical, regs,
                                                                                      nan_rows);
     VG_LAUNCH_CHECK();
-    std::vector<uint32_t> h(GB_SK_REGS + 2);
+    std::vector<uint32_t> h(GB_SK_REGS + 6);
     VG_TRY(hipMemcpyAsync(h.data(), regs, bytes, hipMemcpyDeviceToHost, vg_stream()));
     VG_TRY(hipStreamSynchronize(vg_stream()));
     double m = GB_SK_REGS, z = 0;
@@ -820,9 +944,14 @@ static int gb_sketch(vgpu_groupby *g, const vgpu_col *key, const int32_t *sel, i
     }
     double e = 0.7213 / (1.0 + 1.079 / m) * m * m / z;
     if (e <= 2.5 * m && zeros > 0) e = m * std::log(m / zeros);
-    uint64_t nan;
+    uint64_t nan, lo, hi;
     memcpy(&nan, &h[GB_SK_REGS], 8);
+    memcpy(&lo, &h[GB_SK_REGS + 2], 8);
+    memcpy(&hi, &h[GB_SK_REGS + 4], 8);
     *est = (int64_t)std::ceil(e) + (int64_t)nan + 2;
+    g->sk_ranged = lo <= hi;
+    g->sk_lo = g->sk_ranged ? (int64_t)(lo ^ 0x8000000000000000ull) : 0;
+    g->sk_hi = g->sk_ranged ? (int64_t)(hi ^ 0x8000000000000000ull) : 0;
     return 0;
 }
 
@@ -1587,6 +1716,76 @@ static unsigned gb_grid(int64_t n_logical, int lds_slots, size_t lds_bytes, int6
                      vg_tune_for(VG_TUNE_GB_GLB_MAX_BLOCKS, groups));
 }
 
+// n / count / sum / mean of at most two inputs, a fast predicate or none
+static bool gb_narrow_shape(const GbLayout &L, const PredInfo &pi) {
+    if (L.n_inputs > 2 || (pi.present && !pi.fast)) return false;
+    for (int i = 0; i < L.n_inputs; i++)
+        if (L.w_min[i] >= 0 || L.w_max[i] >= 0 || L.w_fvpos[i] >= 0) return false;
+    return true;
+}
+
+static int gb_feed_narrow(vgpu_groupby *g, const FeedArgs &F, const vgpu_col *inputs, const PredInfo &pi,
+                          const vgpu_col *pred_cols, int64_t groups, bool sketched) {
+    const GbLayout &L = g->L;
+    NarrowArgs A;
+    memset(&A, 0, sizeof A);
+    A.n_logical = F.n_logical;
+    A.pos_base = F.pos_base;
+    A.sel = F.sel;
+    A.key = F.key;
+    for (int i = 0; i < L.n_inputs; i++) {
+        A.in[i] = inputs[i];
+        A.w_sum[i] = L.w_sum[i];
+        A.c_na[i] = L.c_na[i];
+        A.in_f64[i] = L.in_type[i] == VGPU_DOUBLE;
+    }
+    A.pi = pi;
+    if (pi.present) A.pcol = pred_cols[pi.col];
+    A.hot = F.hot;
+    A.cold = F.cold;
+    A.ctl = F.ctl;
+    A.cap = F.cap;
+    A.RH = L.RH;
+    A.RC = L.RC;
+    A.lds_cold = F.lds_cold;
+    A.w_fpos = L.w_fpos;
+    A.key_double = L.key_is_double;
+    A.key_none = L.key_none;
+    const size_t rec = (size_t)(L.RH + (F.lds_cold ? L.RC : 0)) * 8;
+    const size_t budget = (size_t)vg_tune_for(VG_TUNE_GB_LDS_BUDGET, std::max<int64_t>(groups, 1));
+    // dense when this feed's sketch saw a short range of int64 / bool keys
+    bool dense = false;
+    if (sketched && g->sk_ranged && !L.key_is_double && !L.key_none) {
+        const uint64_t range = (uint64_t)g->sk_hi - (uint64_t)g->sk_lo + 1;
+        const int64_t hashed = std::max<int64_t>(gb_groups(g) - 2, 1);
+        if (range != 0 && range <= 4096 && (range + 2) * rec <= budget && (int64_t)range <= 4 * hashed + 64) {
+            dense = true;
+            A.S = (int32_t)range;
+            A.kmin = g->sk_lo;
+        }
+    }
+    if (!dense) A.S = gb_lds_slots(g, 0, F.lds_cold != 0);
+    const size_t lds = A.S ? (size_t)(A.S + 2) * rec : 0;
+    const unsigned grid = gb_grid(F.n_logical, A.S, lds, gb_groups(g));
+    hipStream_t st = vg_stream();
+    vg_kt_begin();
+    // NI is the exact input count: an instantiation reads inputs 0..NI-1
+    if (L.n_inputs == 0) {
+        if (dense) k_gb_feed_narrow<0, true><<<grid, GB_THREADS, lds, st>>>(L, A);
+        else k_gb_feed_narrow<0, false><<<grid, GB_THREADS, lds, st>>>(L, A);
+    } else if (L.n_inputs == 1) {
+        if (dense) k_gb_feed_narrow<1, true><<<grid, GB_THREADS, lds, st>>>(L, A);
+        else k_gb_feed_narrow<1, false><<<grid, GB_THREADS, lds, st>>>(L, A);
+    } else {
+        if (dense) k_gb_feed_narrow<2, true><<<grid, GB_THREADS, lds, st>>>(L, A);
+        else k_gb_feed_narrow<2, false><<<grid, GB_THREADS, lds, st>>>(L, A);
+    }
+    VG_LAUNCH_CHECK();
+    vg_kt_end();
+    g->any_fed = 1;
+    return 0;
+}
+
 extern "C" int vgpu_groupby_feed(vgpu_groupby *g, const vgpu_col *key, const vgpu_col *inputs, int64_t n_rows,
                                  const int32_t *sel, int64_t n_sel, int64_t pos_base, const vgpu_prog *pred,
                                  const vgpu_col *pred_cols, int n_pred_cols) {
@@ -1622,9 +1821,11 @@ extern "C" int vgpu_groupby_feed(vgpu_groupby *g, const vgpu_col *key, const vgp
     int64_t incoming = n_logical;
     int64_t groups = g->hint;
     if (g->hint > 0 && incoming > g->hint) incoming = g->hint;
+    bool sketched = false;
     if (g->hint <= 0 && !g->L.key_none && n_logical >= GB_SK_MIN_ROWS) {
         int64_t est;
         if (gb_sketch(g, key, sel, n_logical, &est)) return -1;
+        sketched = true;
         g->est_groups = std::max(g->est_groups, est);
         incoming = std::min(incoming, est + est / 4 + 4096);
         groups = est;
@@ -1661,6 +1862,8 @@ extern "C" int vgpu_groupby_feed(vgpu_groupby *g, const vgpu_col *key, const vgp
     A.vm_lds = (int32_t)vm;
     if (partitioned) return gbp_feed(g, A, ins, pi, prog, pcs, groups);
     A.lds_cold = lds_cold;
+    if (gb_narrow_shape(g->L, pi) && vg_tune(VG_TUNE_GB_NARROW) >= 2)
+        return gb_feed_narrow(g, A, inputs, pi, pred_cols, groups, sketched);
     A.lds_slots = gb_lds_slots(g, vm, A.lds_cold);
     const size_t table = A.lds_slots ? (size_t)(A.lds_slots + 2) * (g->L.RH + (A.lds_cold ? g->L.RC : 0)) * 8 : 0;
     // small tables are replicated while the copies stay within the budget
