@@ -531,42 +531,82 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const uint64_t *keys, in
     hist[(int64_t)threadIdx.x * n_tiles + blockIdx.x] = h[threadIdx.x];
 }
 
+// One pass of the stable LSD sort: the block's tile of RS_TILE pairs is
+// ranked by digit (rounds in index order, waves in order, lanes by ballot
+// match), placed in LDS in digit order, and written out so that one
+// digit's pairs of the tile land on consecutive global positions: runs of
+// about RS_TILE / 256 pairs instead of one scattered 8-byte write per pair.
 __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const uint64_t *keys_in, const uint32_t *vals_in,
                                                            uint64_t *keys_out, uint32_t *vals_out, int64_t n,
                                                            int shift, const uint64_t *offsets, int64_t n_tiles) {
-    __shared__ uint64_t run[256];
+    __shared__ uint64_t sk[RS_TILE];
+    __shared__ uint32_t sv[RS_TILE];
+    __shared__ uint32_t start[256];   // the digit's first position in the tile
+    __shared__ uint32_t fill[256];    // the digit's pairs placed so far
     __shared__ uint32_t wcnt[4][256];
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    run[t] = offsets[(int64_t)t * n_tiles + blockIdx.x];
+    __shared__ uint64_t gpos[256];    // the digit's first global position of this tile
+    const int t = threadIdx.x, wave = t >> 6;
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    const int cnt = n - base < RS_TILE ? (int)(n - base) : RS_TILE;
+    uint64_t kr[RS_ROUNDS];
+    uint32_t vr[RS_ROUNDS];
+    fill[t] = 0;
     for (int w = 0; w < 4; w++) wcnt[w][t] = 0;
+    gpos[t] = offsets[(int64_t)t * n_tiles + blockIdx.x];
     __syncthreads();
-    int64_t base = (int64_t)blockIdx.x * RS_TILE;
+#pragma unroll
     for (int r = 0; r < RS_ROUNDS; r++) {
-        int64_t idx = base + (int64_t)r * RS_THREADS + t;
-        bool in = idx < n;
-        uint64_t k = in ? keys_in[idx] : 0;
-        uint32_t v = in ? vals_in[idx] : 0;
-        uint32_t d = (uint32_t)(k >> shift) & 0xFF;
+        const int i = r * RS_THREADS + t;
+        const bool in = i < cnt;
+        kr[r] = in ? keys_in[base + i] : 0;
+        vr[r] = in ? vals_in[base + i] : 0;
+        if (in) atomicAdd(&fill[(uint32_t)(kr[r] >> shift) & 0xFF], 1u);
+    }
+    __syncthreads();
+    // exclusive scan of the tile's digit counts
+    const uint32_t c = fill[t];
+    start[t] = c;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const uint32_t x = t >= off ? start[t - off] : 0;
+        __syncthreads();
+        start[t] += x;
+        __syncthreads();
+    }
+    start[t] -= c;
+    fill[t] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; r++) {
+        const int i = r * RS_THREADS + t;
+        const bool in = i < cnt;
+        const uint32_t d = (uint32_t)(kr[r] >> shift) & 0xFF;
         uint64_t m = vg_ballot(in);
 #pragma unroll
         for (int b = 0; b < 8; b++) {
-            uint64_t bb = vg_ballot((d >> b) & 1);
+            const uint64_t bb = vg_ballot((d >> b) & 1);
             m &= ((d >> b) & 1) ? bb : ~bb;
         }
-        int rank = vg_lane_prefix(m);
+        const int rank = vg_lane_prefix(m);
         if (in && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(m);
         __syncthreads();
         if (in) {
-            uint64_t pos = run[d] + rank;
+            uint32_t pos = start[d] + fill[d] + (uint32_t)rank;
             for (int w = 0; w < wave; w++) pos += wcnt[w][d];
-            keys_out[pos] = k;
-            vals_out[pos] = v;
+            sk[pos] = kr[r];
+            sv[pos] = vr[r];
         }
-        (void)lane;
         __syncthreads();
-        run[t] += (uint64_t)wcnt[0][t] + wcnt[1][t] + wcnt[2][t] + wcnt[3][t];
+        fill[t] += wcnt[0][t] + wcnt[1][t] + wcnt[2][t] + wcnt[3][t];
         for (int w = 0; w < 4; w++) wcnt[w][t] = 0;
         __syncthreads();
+    }
+    for (int i = t; i < cnt; i += RS_THREADS) {
+        const uint64_t k = sk[i];
+        const uint32_t d = (uint32_t)(k >> shift) & 0xFF;
+        const uint64_t p = gpos[d] + (uint64_t)(i - (int)start[d]);
+        keys_out[p] = k;
+        vals_out[p] = sv[i];
     }
 }
 
