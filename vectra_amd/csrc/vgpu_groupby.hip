@@ -1232,6 +1232,32 @@ __global__ void __launch_bounds__(GB_THREADS) k_gbp_scatter2(GbpArgs G) {
     const int NS = 1 << G.B2;
     for (int i = threadIdx.x; i < NS; i += GB_THREADS) cur[i] = G.O2[gbp_h2_index(G, b, i, chunk, nblk)];
     __syncthreads();
+    if (G.W == 4) {
+        // four rows per thread loaded before any is placed
+        for (int64_t r = r0 + threadIdx.x; r < r1; r += (int64_t)GB_THREADS * 4) {
+            ulonglong2 a[4], c[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int64_t rr = r + (int64_t)u * GB_THREADS;
+                a[u] = c[u] = make_ulonglong2(0, 0);
+                if (rr < r1) {
+                    const ulonglong2 *src = reinterpret_cast<const ulonglong2 *>(G.S1 + rr * 4);
+                    a[u] = src[0];
+                    c[u] = src[1];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                if (r + (int64_t)u * GB_THREADS >= r1) continue;
+                const uint64_t w[4] = {a[u].x, a[u].y, c[u].x, c[u].y};
+                const uint64_t dst = atomicAdd(&cur[gbp_sub(G, w)], 1ull);
+                ulonglong2 *d = reinterpret_cast<ulonglong2 *>(G.S2 + dst * 4);
+                d[0] = a[u];
+                d[1] = c[u];
+            }
+        }
+        return;
+    }
     for (int64_t r = r0 + threadIdx.x; r < r1; r += GB_THREADS) {
         const uint64_t *w = G.S1 + r * G.W;
         uint64_t dst = atomicAdd(&cur[gbp_sub(G, w)], 1ull);
@@ -1326,6 +1352,24 @@ __device__ __forceinline__ void gbp_apply_staged(const GbLayout &L, uint64_t *h,
     }
 }
 
+// one staged row (G.W words: key, inputs, meta) into the item's LDS table,
+// or the HBM table when the LDS table has no room for its key
+__device__ __forceinline__ void gbp_agg_row(const GbLayout &L, const GbpArgs &G, uint64_t *lhot, uint64_t *lcold,
+                                            int S, uint64_t empty, const uint64_t *w, bool &spilled) {
+    const uint64_t kn = w[0], meta = w[G.W - 1];
+    const int cls = (int)((meta >> 32) & 3);
+    const uint64_t pos = (uint64_t)G.pos_base + (uint32_t)meta;
+    const int64_t ls = gbp_lds_slot(lhot, S, L.RH, cls, kn, empty);
+    // one expansion per address space: ds_* atomics for the LDS record
+    if (ls >= 0) {
+        gbp_apply_staged(L, lhot + (size_t)ls * L.RH, lcold + (size_t)ls * L.RC, w, meta, pos);
+    } else {
+        const int64_t gs = gb_global_slot(L, G.hot, G.cap, cls, kn, pos, G.ctl);
+        spilled = true;
+        if (gs >= 0) gbp_apply_staged(L, G.hot + (size_t)gs * L.RH, G.cold + (size_t)gs * L.RC, w, meta, pos);
+    }
+}
+
 __global__ void __launch_bounds__(GB_THREADS) k_gbp_agg(GbLayout L, GbpArgs G) {
     extern __shared__ uint64_t dynw[];   // 8-byte aligned after the static arrays
     __shared__ uint16_t slist[1024];
@@ -1347,20 +1391,30 @@ __global__ void __launch_bounds__(GB_THREADS) k_gbp_agg(GbLayout L, GbpArgs G) {
     const int64_t r0 = (int64_t)G.pstart[p] + chunk * GBP_CH;
     const int64_t r1 = r0 + GBP_CH < pe ? r0 + GBP_CH : pe;
     bool spilled = false;
-    for (int64_t r = r0 + threadIdx.x; r < r1; r += GB_THREADS) {
-        const uint64_t *w = G.S2 + r * G.W;
-        uint64_t kn = w[0], meta = w[G.W - 1];
-        int cls = (int)((meta >> 32) & 3);
-        uint64_t pos = (uint64_t)G.pos_base + (uint32_t)meta;
-        int64_t ls = gbp_lds_slot(lhot, S, L.RH, cls, kn, empty);
-        // one expansion per address space: ds_* atomics for the LDS record
-        if (ls >= 0) {
-            gbp_apply_staged(L, lhot + (size_t)ls * L.RH, lcold + (size_t)ls * L.RC, w, meta, pos);
-        } else {
-            int64_t gs = gb_global_slot(L, G.hot, G.cap, cls, kn, pos, G.ctl);
-            spilled = true;
-            if (gs >= 0) gbp_apply_staged(L, G.hot + (size_t)gs * L.RH, G.cold + (size_t)gs * L.RC, w, meta, pos);
+    if (G.W == 4) {
+        // the two-input row: four rows per thread loaded before any is applied
+        for (int64_t r = r0 + threadIdx.x; r < r1; r += (int64_t)GB_THREADS * 4) {
+            ulonglong2 a[4], b[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int64_t rr = r + (int64_t)u * GB_THREADS;
+                a[u] = b[u] = make_ulonglong2(0, 0);
+                if (rr < r1) {
+                    const ulonglong2 *src = reinterpret_cast<const ulonglong2 *>(G.S2 + rr * 4);
+                    a[u] = src[0];
+                    b[u] = src[1];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                if (r + (int64_t)u * GB_THREADS >= r1) continue;
+                const uint64_t w[4] = {a[u].x, a[u].y, b[u].x, b[u].y};
+                gbp_agg_row(L, G, lhot, lcold, S, empty, w, spilled);
+            }
         }
+    } else {
+        for (int64_t r = r0 + threadIdx.x; r < r1; r += GB_THREADS)
+            gbp_agg_row(L, G, lhot, lcold, S, empty, G.S2 + r * G.W, spilled);
     }
     if (spilled) G.ctl[9] = 1;
     __syncthreads();
