@@ -251,6 +251,20 @@ def test_groupby_partitioned(gpu, oracle, part_threshold, key_kind, card, temp_d
     check(gpu, mk, ["g"], ALL_AGGS, temp_dir)
 
 
+TWO_INPUT_AGGS = [a for a in ALL_AGGS if a[2] in (None, "x", "v")]
+
+
+@pytest.mark.parametrize("key_kind,card,temp_dir", [("int", 3000, "/tmp"), ("int", 10**9, None),
+                                                    ("dbl", 4000, None)])
+def test_groupby_partitioned_two_inputs(gpu, oracle, part_threshold, key_kind, card, temp_dir):
+    """the two-input staged row (four words), whose passes batch four rows
+    per thread: NA / INT64_MIN / NaN keys, signed zeros, NA and NaN inputs"""
+    p_nan = 0.0 if key_kind == "dbl" and temp_dir is not None else 0.01
+    mk = lambda: table(gpu, 1_200_000, 19, key_kind, card, 0, p_na=0.02, p_nan=p_nan)
+    check(gpu, mk, ["g"], TWO_INPUT_AGGS, temp_dir)
+    check(gpu, mk, ["g"], TWO_INPUT_AGGS, temp_dir, ("cmp", ">", ("col", "x"), ("dbl", -0.3)))
+
+
 @pytest.mark.parametrize("temp_dir", [None, "/tmp"])
 def test_groupby_partitioned_hot_keys(gpu, oracle, part_threshold, temp_dir):
     """partitions far above one item (a hot key, a 30% NA key): partial
