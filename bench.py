@@ -661,8 +661,8 @@ def main():
     ap.add_argument("--c5-max-batch", type=int, default=8)
     ap.add_argument("--c5-seconds", type=float, default=3.0)
     ap.add_argument("--c5-slo-ms", type=float, default=100.0)
-    ap.add_argument("--c5-plain-rates", type=float, nargs="*", default=[10, 25, 40, 55, 70])
-    ap.add_argument("--c5-batched-rates", type=float, nargs="*", default=[50, 100, 170, 250, 350, 500, 650])
+    ap.add_argument("--c5-plain-rates", type=float, nargs="*", default=[10, 25, 40, 55, 70, 85, 100])
+    ap.add_argument("--c5-batched-rates", type=float, nargs="*", default=[50, 100, 200, 350, 500, 650, 800, 1000])
     ap.add_argument("--c3-rows", type=int, default=200_000_000)
     ap.add_argument("--fact-rows", type=int, default=50_000_000)
     ap.add_argument("--dim-rows", type=int, default=1_000_000)
