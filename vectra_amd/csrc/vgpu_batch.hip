@@ -41,6 +41,7 @@
 #define BT_U 4                     // rows in flight per thread in the feed
 #define BT_STILE 16384             // rows per tile of a shared scan
 #define BT_CTL_ERR 9               // batch control word: inputs left the plan's key range
+#define BT_SPLIT_SLOTS 256         // queries with larger LDS tables run in the feed's second launch
 
 struct BtQuery {
     vgpu_col key;
@@ -106,6 +107,12 @@ struct vgpu_batch {
     int32_t vm_lds = 0;
     int32_t lds_cold = 1;
     int64_t feed_grid = 1;          // persistent feed blocks: a few per CU
+    // the feed's items in two classes, one launch each: queries with small
+    // or no LDS tables (many blocks per CU) first, then queries with large
+    // LDS tables (their LDS size limits the blocks per CU)
+    int64_t n_items_a = 0;
+    int32_t S_max_a = 0, S_max_b = 0;
+    int64_t feed_grid_b = 1;
     int64_t shared_resident = 1;    // feed blocks resident at once (LDS and wave limits)
     int64_t tile = BT_TILE_MIN;     // rows per work item
     int32_t planned = 0;
@@ -197,7 +204,10 @@ __global__ void k_bt_init_ctl(uint64_t *ctl, int n_queries, uint64_t *qcount, un
         ctl[i] = (w == 2 || w == 3 || w == 7) ? ~0ull : 0ull;
     }
     if (i < n_queries) qcount[i] = 0;
-    if (i == 0) *queue = 0;
+    if (i == 0) {
+        queue[0] = 0;
+        queue[1] = 0;
+    }
 }
 
 // Fields of one query a feed block works on: column sets in LDS, scalars
@@ -1088,7 +1098,7 @@ extern "C" int vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_q
     b->vm_lds = (int32_t)vm;
     if (b->qs.reserve(qs.size() * sizeof(BtQuery)) || b->progs.reserve(progs.size() * sizeof(vgpu_prog)) ||
         b->items.reserve(std::max<size_t>(items.size(), 1) * sizeof(BtItem)) ||
-        b->ctl.reserve((size_t)(n_queries + 1) * GB_CTL_WORDS * 8) || b->queue.reserve(8) ||
+        b->ctl.reserve((size_t)(n_queries + 1) * GB_CTL_WORDS * 8) || b->queue.reserve(16) ||
         b->qcount.reserve((size_t)n_queries * 8) || b->qoff.reserve((size_t)(n_queries + 1) * 8) ||
         b->qbase.reserve((size_t)(n_queries + 1) * 8))
         return -1;
@@ -1196,6 +1206,35 @@ extern "C" int vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_q
     VG_TRY(hipGetDevice(&dev));
     VG_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     b->feed_grid = std::max<int64_t>(1, std::min<int64_t>(b->n_items, (int64_t)cus * BT_BLOCKS_PER_CU));
+    {
+        // items of the queries with small or no LDS tables, then the rest
+        // (query-major within each class, as the blocks' sticky tables want)
+        std::vector<BtItem> ordered;
+        ordered.reserve(items.size());
+        int Sa = 0, Sb = 0;
+        for (int pass = 0; pass < 2; pass++)
+            for (int q = 0; q < n_queries; q++) {
+                const bool big = qs[q].lds_slots > BT_SPLIT_SLOTS;
+                if (big != (pass == 1)) continue;
+                if (big) Sb = std::max<int>(Sb, qs[q].lds_slots);
+                else Sa = std::max<int>(Sa, qs[q].lds_slots);
+                for (int64_t t = 0; t * b->tile < qs[q].n_rows; t++) ordered.push_back(BtItem{(uint32_t)q, (uint32_t)t});
+                if (pass == 0) b->n_items_a = (int64_t)ordered.size();
+            }
+        if (!ordered.empty())
+            VG_TRY(hipMemcpyAsync(b->items.ptr, ordered.data(), ordered.size() * sizeof(BtItem), hipMemcpyHostToDevice,
+                                  st));
+        b->S_max_a = Sa;
+        b->S_max_b = Sb;
+        auto grid_for = [&](int64_t n, int Sx) {
+            const size_t lds = (size_t)b->vm_lds + (Sx ? (size_t)(Sx + 2) * rw : 0) + 512;
+            const int64_t per_cu = std::max<int64_t>(1, std::min<int64_t>(BT_BLOCKS_PER_CU,
+                                                                           (int64_t)(160 * 1024 / lds)));
+            return std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)cus * per_cu));
+        };
+        b->feed_grid = grid_for(b->n_items_a, Sa);
+        b->feed_grid_b = grid_for(b->n_items - b->n_items_a, Sb);
+    }
     {
         // 160 KB of LDS and 32 waves (8 blocks of 256 threads) per CU
         size_t lds = (size_t)b->vm_lds + (S_max ? (size_t)(S_max + 2) * rw : 0) + 512;
@@ -1306,14 +1345,31 @@ static int bt_enqueue(vgpu_batch *b, bool first) {
         VG_LAUNCH_CHECK();
         vg_kt_end();
     } else if (b->n_items > 0) {
-        size_t lds = (size_t)b->vm_lds +
-                     (b->S_max ? (size_t)(b->S_max + 2) * (L.RH + (b->lds_cold ? L.RC : 0)) * 8 : 0);
+        const size_t rwb = (size_t)(L.RH + (b->lds_cold ? L.RC : 0)) * 8;
         vg_kt_begin();
-        if (b->shared)
+        if (b->shared) {
+            const size_t lds = (size_t)b->vm_lds + (b->S_max ? (size_t)(b->S_max + 2) * rwb : 0);
             k_bt_feed_shared<<<(unsigned)((int64_t)BT_XCDS * A.K * A.J), GB_THREADS, lds, st>>>(L, A);
-        else
-            k_bt_feed<<<(unsigned)b->feed_grid, GB_THREADS, lds, st>>>(L, A);
-        VG_LAUNCH_CHECK();
+            VG_LAUNCH_CHECK();
+        } else {
+            // small-table and HBM-table queries, then large-table queries
+            if (b->n_items_a > 0) {
+                BtArgs Aa = A;
+                Aa.n_items = b->n_items_a;
+                const size_t lds = (size_t)b->vm_lds + (b->S_max_a ? (size_t)(b->S_max_a + 2) * rwb : 0);
+                k_bt_feed<<<(unsigned)b->feed_grid, GB_THREADS, lds, st>>>(L, Aa);
+                VG_LAUNCH_CHECK();
+            }
+            if (b->n_items > b->n_items_a) {
+                BtArgs Ab = A;
+                Ab.items = A.items + b->n_items_a;
+                Ab.n_items = b->n_items - b->n_items_a;
+                Ab.queue = A.queue + 1;
+                const size_t lds = (size_t)b->vm_lds + (size_t)(b->S_max_b + 2) * rwb;
+                k_bt_feed<<<(unsigned)b->feed_grid_b, GB_THREADS, lds, st>>>(L, Ab);
+                VG_LAUNCH_CHECK();
+            }
+        }
         vg_kt_end();
     }
     // 3. finish: compaction into (key, arena slot) pairs
