@@ -242,3 +242,60 @@ def test_bucketed_join(gpu, oracle, dup):
     jt.free()
     for c in (dkc, atc, fkc, pc, xc, vc):
         c.free()
+
+
+@pytest.mark.parametrize("join", [False, True])
+def test_bucketed_sums_the_compared_column(gpu, oracle, join):
+    """filter(p > t_q) |> [inner_join(dim) |>] group_by(...) |> summarise(
+    n = n(), c = count(p), s = sum(p), m = mean(p)): the summed input is the
+    compared column itself (the scan keeps one column per row, eight rows
+    per thread and step)"""
+    n = 260_000
+    if join:
+        tbl, dim = join_tables(101, n, 20_000, False)
+        key = "attr"
+    else:
+        tbl, dim = make_table(103, n), None
+        key = "g"
+    r = rng(107)
+    p = r.normal(0.0, 1.0, n)
+    p[::1013] = 0.5                          # rows equal to a threshold
+    tbl["p"] = p
+    ts = [0.0, 1.0, -1.0, 0.5, 0.5, 2.5, -0.25, 1.75]
+    lib_aggs = [("n", -1, 0), ("count", 0, 0), ("sum", 0, 0), ("mean", 0, 0)]
+    pc = gpu.DeviceColumn(p)
+    pred = gpu.cmp_const_prog(0, gpu.VEC_DOUBLE, "GT", 0.0)
+    b = gpu.Batch(gpu.VEC_INT64, [gpu.VEC_DOUBLE], lib_aggs, gpu.VGPU_ORDER_KEY, pred, 1, 1)
+    owned = [pc]
+    if join:
+        dkc, atc = gpu.DeviceColumn(dim["dk"]), gpu.DeviceColumn(*dim["attr"])
+        jt = gpu.JoinTable(dkc, [atc])
+        fkc = gpu.DeviceColumn(*tbl["fk"])
+        owned += [dkc, atc, fkc]
+        b.join(jt, 0)
+        qs = [{"probe_key": fkc, "inputs": [pc], "pred_cols": [pc], "n_rows": n, "param": t} for t in ts]
+    else:
+        gc = gpu.DeviceColumn(*tbl["g"])
+        owned.append(gc)
+        qs = [{"key": gc, "inputs": [pc], "pred_cols": [pc], "n_rows": n, "param": t} for t in ts]
+    b.plan(qs)
+    assert b.bucketed()
+    cols = {"p": p, "ap": np.abs(p)}
+    cols.update({"fk": tbl["fk"]} if join else {"g": tbl["g"]})
+    for q, t in enumerate(ts):
+        node = gpu.filter_node("oracle", gpu.Table(cols, rg_rows=0).scan(), ("cmp", ">", ("col", "p"), ("dbl", t)))
+        if join:
+            node = gpu.join_node("oracle", node, gpu.Table(dim, rg_rows=0).scan(), "inner", [(2, 0)])
+        out = gpu.collect(gpu.group_agg_node(
+            "oracle", node, [key], [("n", "n", None, 0), ("c", "count", "p", 0), ("s", "sum", "p", 0),
+                                    ("m", "mean", "p", 0), ("__sa", "sum", "ap", 1)], "/tmp"))
+        sa = out.pop("__sa")[0]
+        exp = {("g" if k == key else k): v for k, v in out.items()}
+        got = b.fetch(q)
+        got = {"g": got["key"], "n": got["a0"], "c": got["a1"], "s": got["a2"], "m": got["a3"]}
+        assert_tables_equal(got, exp, {"s": sa, "m": sa / np.maximum(exp["c"][0], 1)})
+    b.free()
+    if join:
+        jt.free()
+    for c in owned:
+        c.free()

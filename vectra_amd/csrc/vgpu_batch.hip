@@ -568,8 +568,8 @@ k_bt_emit(GbLayout L, GbAggs AG, BtArgs B, const uint32_t *svals, const uint64_t
 // ---------------------------------------------------------------------------
 
 #define BT_CH_THREADS 1024
-#define BT_CH_U 4
-#define BT_CH_MAX_SUMS 4
+#define BT_CH_U 8
+#define BT_CH_MAX_SUMS 2
 #define BT_CH_LDS (160 * 1024 - 2048)   // dynamic LDS a bucketed feed block may take
 
 struct BtChainArgs {
@@ -626,16 +626,107 @@ __device__ __forceinline__ void bt_chain_add(const BtChainArgs &A, uint32_t *cnt
     }
 }
 
-__device__ __forceinline__ void bt_chain_load_sums(const BtChainArgs &A, int64_t row, uint64_t pv,
-                                                   uint64_t (&sv)[BT_CH_MAX_SUMS]) {
+// one thread's rows of one step: compared value and group / probe key of
+// each, the summed inputs (SP: every sum reads the compared column, so none
+// is kept), bit u of ok / kv: row u is in range with a valid compared
+// value / a valid key
+template <int U, bool SP>
+struct BtRows {
+    uint64_t pv[U], kb[U];
+    uint64_t sv[SP ? 1 : U][BT_CH_MAX_SUMS];
+    uint32_t ok, kv;
+};
+
+template <int U, bool SP>
+__device__ __forceinline__ void bt_chain_rows(const BtChainArgs &A, const vgpu_col &kc, int64_t base, int64_t r1,
+                                              BtRows<U, SP> &R) {
+    R.ok = 0;
+    R.kv = 0;
 #pragma unroll
-    for (int s = 0; s < BT_CH_MAX_SUMS; s++) {
-        sv[s] = 0;
-        if (s < A.n_sum) sv[s] = A.sum_pred[s] ? pv : vg_load_bits(A.ins.c[A.sum_in[s]], row);
+    for (int u = 0; u < U; u++) {
+        const int64_t row = base + (int64_t)u * BT_CH_THREADS;
+        const bool in = row < r1;
+        const bool ok = in && vg_valid(A.pcol.validity, row);
+        const bool kv = in && vg_valid(kc.validity, row);
+        R.pv[u] = ok ? vg_load_bits_stream(A.pcol, row) : 0;
+        R.kb[u] = kv ? vg_load_bits_stream(kc, row) : 0;
+        R.ok |= (uint32_t)ok << u;
+        R.kv |= (uint32_t)kv << u;
+        if (!SP) {
+#pragma unroll
+            for (int s = 0; s < BT_CH_MAX_SUMS; s++) {
+                uint64_t v = 0;
+                if (s < A.n_sum && ok) v = A.sum_pred[s] ? R.pv[u] : vg_load_bits_stream(A.ins.c[A.sum_in[s]], row);
+                R.sv[SP ? 0 : u][s] = v;
+            }
+        }
     }
 }
 
+// the bucket of every row of a step that passes at least one query, and
+// its aggregation into the block's table
+template <int U, bool SP>
+__device__ __forceinline__ void bt_chain_rows_add(const BtChainArgs &A, const vgpu_col &kc, const uint64_t *s_k,
+                                                  uint32_t *cnt, uint64_t *sums, const BtRows<U, SP> &R, bool &err) {
+    int T[U];
+    uint64_t sv[U][BT_CH_MAX_SUMS];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        T[u] = (R.ok >> u) & 1 ? bt_chain_T(A, s_k, R.pv[u]) : 0;
+#pragma unroll
+        for (int s = 0; s < BT_CH_MAX_SUMS; s++) sv[u][s] = SP ? R.pv[u] : R.sv[SP ? 0 : u][s];
+    }
+    if (A.jn.on) {
+        const uint64_t *ent[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            ent[u] = nullptr;
+            if (T[u]) {
+                uint64_t kn;
+                const int cls = vg_key_class_bits(kc.type, (R.kv >> u) & 1, R.kb[u], kn);
+                ent[u] = jt_find_key(A.jn.jv, cls, kn);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (!ent[u]) continue;
+            const uint64_t segw = ent[u][1];
+            const uint32_t n = jt_count(A.jn.jv, segw), off = (uint32_t)(segw >> 32);
+            const int p = A.jn.key_payload;
+            for (uint32_t j = 0; j < n; j++) {
+                bool v;
+                uint64_t bits;
+                if (j == 0) {
+                    v = (segw >> (24 + p)) & 1;
+                    bits = ent[u][2 + p];
+                } else {
+                    v = A.jn.jv.cval[p][off + j];
+                    bits = A.jn.jv.cpay[p][off + j];
+                }
+                const int64_t e = bt_chain_entry(A, v, bits);
+                if (e < 0) err = true;
+                else bt_chain_add(A, cnt, sums, e, T[u], sv[u]);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (!T[u]) continue;
+            const int64_t e = bt_chain_entry(A, (R.kv >> u) & 1, R.kb[u]);
+            if (e < 0) err = true;
+            else bt_chain_add(A, cnt, sums, e, T[u], sv[u]);
+        }
+    }
+}
+
+// SP: every sum reads the compared column (8 rows per thread and step),
+// else 4 rows per thread and step.  The next step's columns are loaded
+// before the current step is aggregated, so a wave keeps its loads in
+// flight while it works on LDS: with one 1,024-thread block per CU the
+// scan is otherwise bound by the loads the 16 waves have outstanding.
+template <bool SP>
 __global__ void __launch_bounds__(BT_CH_THREADS) k_bt_chain_feed(BtChainArgs A) {
+    constexpr int U = SP ? BT_CH_U : BT_CH_U / 2;
     extern __shared__ uint64_t dynw[];
     uint64_t *s_k = dynw;                                   // m constants
     uint64_t *sums = dynw + A.m;                            // E*m*n_sum
@@ -647,71 +738,15 @@ __global__ void __launch_bounds__(BT_CH_THREADS) k_bt_chain_feed(BtChainArgs A) 
     __syncthreads();
     const int64_t r0 = (int64_t)blockIdx.x * A.rows_per_block;
     const int64_t r1 = r0 + A.rows_per_block < A.n_rows ? r0 + A.rows_per_block : A.n_rows;
+    const vgpu_col &kc = A.jn.on ? A.probe_key : A.key;
+    const int64_t step = (int64_t)BT_CH_THREADS * U;
     bool err = false;
-    // every column a row needs is loaded in one phase (the columns are read
-    // whole anyway: selectivities leave no cache line unread), then the
-    // bucket is found and the row aggregated
-    for (int64_t base = r0 + threadIdx.x; base < r1; base += (int64_t)BT_CH_THREADS * BT_CH_U) {
-        int T[BT_CH_U];
-        uint64_t pv[BT_CH_U], kb[BT_CH_U];
-        bool kv[BT_CH_U];
-        uint64_t sv[BT_CH_U][BT_CH_MAX_SUMS];
-        const vgpu_col &kc = A.jn.on ? A.probe_key : A.key;
-#pragma unroll
-        for (int u = 0; u < BT_CH_U; u++) {
-            const int64_t row = base + (int64_t)u * BT_CH_THREADS;
-            const bool in = row < r1;
-            const bool ok = in && vg_valid(A.pcol.validity, row);
-            pv[u] = ok ? vg_load_bits(A.pcol, row) : 0;
-            T[u] = ok ? 1 : 0;
-            kv[u] = in && vg_valid(kc.validity, row);
-            kb[u] = kv[u] ? vg_load_bits(kc, row) : 0;
-            bt_chain_load_sums(A, in ? row : r0, pv[u], sv[u]);
-        }
-#pragma unroll
-        for (int u = 0; u < BT_CH_U; u++)
-            if (T[u]) T[u] = bt_chain_T(A, s_k, pv[u]);
-        if (A.jn.on) {
-            const uint64_t *ent[BT_CH_U];
-#pragma unroll
-            for (int u = 0; u < BT_CH_U; u++) {
-                ent[u] = nullptr;
-                if (T[u]) {
-                    uint64_t kn;
-                    const int cls = vg_key_class_bits(kc.type, kv[u], kb[u], kn);
-                    ent[u] = jt_find_key(A.jn.jv, cls, kn);
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < BT_CH_U; u++) {
-                if (!ent[u]) continue;
-                const uint64_t segw = ent[u][1];
-                const uint32_t n = jt_count(A.jn.jv, segw), off = (uint32_t)(segw >> 32);
-                const int p = A.jn.key_payload;
-                for (uint32_t j = 0; j < n; j++) {
-                    bool v;
-                    uint64_t bits;
-                    if (j == 0) {
-                        v = (segw >> (24 + p)) & 1;
-                        bits = ent[u][2 + p];
-                    } else {
-                        v = A.jn.jv.cval[p][off + j];
-                        bits = A.jn.jv.cpay[p][off + j];
-                    }
-                    const int64_t e = bt_chain_entry(A, v, bits);
-                    if (e < 0) err = true;
-                    else bt_chain_add(A, cnt, sums, e, T[u], sv[u]);
-                }
-            }
-        } else {
-#pragma unroll
-            for (int u = 0; u < BT_CH_U; u++) {
-                if (!T[u]) continue;
-                const int64_t e = bt_chain_entry(A, kv[u], kb[u]);
-                if (e < 0) err = true;
-                else bt_chain_add(A, cnt, sums, e, T[u], sv[u]);
-            }
-        }
+    BtRows<U, SP> cur, nxt;
+    bt_chain_rows(A, kc, r0 + threadIdx.x, r1, cur);
+    for (int64_t base = r0 + threadIdx.x; base < r1; base += step) {
+        bt_chain_rows(A, kc, base + step, r1, nxt);
+        bt_chain_rows_add(A, kc, s_k, cnt, sums, cur, err);
+        cur = nxt;
     }
     if (err) atomicOr((unsigned long long *)&A.bctl[BT_CTL_ERR], 1ull);
     __syncthreads();
@@ -951,9 +986,12 @@ static int bt_chain_plan(vgpu_batch *b, const BtQuery &Q0) {
     if (b->ch_slab.reserve((size_t)G * W * 8) || b->ch_tab.reserve(W * 8) ||
         b->ch_aux.reserve((size_t)m * 12 + 16))
         return -1;
-    if (lds > 65536)
-        VG_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(k_bt_chain_feed),
+    if (lds > 65536) {
+        VG_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(k_bt_chain_feed<true>),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        VG_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(k_bt_chain_feed<false>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
     b->ch_kmin = h[0] <= h[1] ? (int64_t)(h[0] ^ 0x8000000000000000ull) : 0;
     b->ch_E = (int32_t)E;
     b->ch_m = (int32_t)m;
@@ -1337,7 +1375,10 @@ static int bt_enqueue(vgpu_batch *b, bool first) {
         }
         const int64_t W = (int64_t)C.E * C.m * (1 + C.n_sum);
         vg_kt_begin();
-        k_bt_chain_feed<<<(unsigned)C.G, BT_CH_THREADS, b->ch_lds, st>>>(C);
+        bool sp = true;
+        for (int s2 = 0; s2 < C.n_sum; s2++) sp = sp && C.sum_pred[s2];
+        if (sp) k_bt_chain_feed<true><<<(unsigned)C.G, BT_CH_THREADS, b->ch_lds, st>>>(C);
+        else k_bt_chain_feed<false><<<(unsigned)C.G, BT_CH_THREADS, b->ch_lds, st>>>(C);
         VG_LAUNCH_CHECK();
         k_bt_chain_reduce<<<vg_blocks(W, 256), 256, 0, st>>>(C);
         VG_LAUNCH_CHECK();
