@@ -367,6 +367,10 @@ void    vgpu_batch_destroy(vgpu_batch *b);
  *                         mean over at most two inputs, with a fast filter or
  *                         none, run a lean kernel (dense LDS table when the
  *                         feed's sketch saw a short int64 / bool key range)
+ *   "batch.dense"        (2: on; 1: off) batch queries whose sketch saw an
+ *                         int64 / bool key range no longer than their hashed
+ *                         table get a dense table instead: key k owns record
+ *                         k - min, found without hashing or probing
  * Returns -1 for an unknown name or a value <= 0. */
 int     vgpu_tune_set(const char *name, int64_t value);
 int64_t vgpu_tune_get(const char *name);

@@ -312,3 +312,63 @@ def _batch_join(gpu, temp_dir, dup):
     jt.free()
     for c in (dkc, atc, fkc, xc):
         c.free()
+
+
+def make_dense_queries(seed, n_queries):
+    """int64 keys from a short range lo .. lo + card - 1 (lo negative or
+    positive) with INT64_MIN and NA rows: the batch gives such queries dense
+    tables (record k - lo), in LDS and in HBM"""
+    r = rng(seed)
+    cards = [1, 2, 40, 300, 2500, 20000, 70000]
+    out = []
+    for q in range(n_queries):
+        n = int(10 ** r.uniform(3, 5.5)) if q >= 3 else [0, 1, 700][q]
+        card = cards[q % len(cards)]
+        lo = int(r.integers(-10**6, 10**6))
+        kv = lo + r.integers(0, card, n)
+        kv[r.random(n) < 0.01] = np.iinfo(np.int64).min
+        x = dbl_with_specials(r, n, 0.01, 0.02)
+        out.append({"g": (kv, with_na(r, n, 0.02)), "x": (x, with_na(r, n, 0.02)),
+                    "t": float(r.uniform(-2.0, 1.0)), "ax": np.abs(np.nan_to_num(x, nan=0.0))})
+    return out
+
+
+@pytest.mark.parametrize("temp_dir,dense", [("/tmp", 2), (None, 2), ("/tmp", 1)])
+def test_batch_dense_key_ranges(gpu, oracle, temp_dir, dense):
+    """queries whose keys span a short range, with the dense tables on (2)
+    and off (1): the same groups, order and values as the oracle"""
+    qs = make_dense_queries(23, 24)
+    order = gpu.VGPU_ORDER_KEY if temp_dir else gpu.VGPU_ORDER_FIRST_SEEN
+    lib = gpu.vgpu()
+    old = lib.vgpu_tune_get(b"batch.dense")
+    lib.vgpu_tune_set(b"batch.dense", dense)
+    try:
+        b, cols = run_batch(gpu, qs, order)
+    finally:
+        lib.vgpu_tune_set(b"batch.dense", old)
+    for rep in range(2):
+        if rep:
+            b.run()
+        for q, qd in enumerate(qs):
+            exp, close = oracle_query(gpu, qd, temp_dir)
+            assert_tables_equal(renamed(b.fetch(q)), exp, close)
+    b.free()
+    for c in cols:
+        c.free()
+
+
+def test_batch_dense_flags_keys_outside_range(gpu, oracle):
+    """a dense query's keys rewritten after plan to values outside the
+    planned range: the run is reported as invalid"""
+    qs = make_dense_queries(29, 6)
+    b, cols = run_batch(gpu, qs, gpu.VGPU_ORDER_KEY)
+    key_col = cols[2 * 5 + 1]     # query 5 (20,000-key range): x, g per query
+    lo = int(qs[5]["g"][0][qs[5]["g"][0] != np.iinfo(np.int64).min].min())
+    vals = np.full(key_col.n, lo - 3, np.int64)
+    gpu._vcheck(gpu.vgpu().vgpu_memcpy_h2d(key_col.data, vals.ctypes.data, vals.nbytes))
+    b.run()
+    with pytest.raises(gpu.VectraError, match="inputs changed since plan"):
+        b.counts()
+    b.free()
+    for c in cols:
+        c.free()

@@ -54,6 +54,9 @@ struct BtQuery {
     int64_t cap;        // hashed slots (pow2); NA / EMPTY-valued key records at cap, cap + 1
     int32_t lds_slots;  // 0 = aggregate straight into the HBM table
     int32_t vm;         // predicate evaluated by the VM (program in progs[q])
+    int64_t kmin;       // dense tables: the key of record 0
+    int32_t dense;      // key k owns record k - kmin of cap records (HBM and LDS tables)
+    int32_t pad_;
 };
 
 struct BtItem {
@@ -152,16 +155,23 @@ struct vgpu_batch {
 // kernels
 // ---------------------------------------------------------------------------
 
-// per-query cardinality sketch (HyperLogLog registers, NaN-key rows): the
-// feed's items, one block per item at a time
+// per-query cardinality sketch (HyperLogLog registers, NaN-key rows) and
+// range of the regular keys as int64 (mm[2q] = max of ~ordered key, i.e.
+// the complement of the smallest; mm[2q + 1] = the largest ordered key;
+// both start at 0): the feed's items, one block per item at a time
 __global__ void __launch_bounds__(GB_THREADS)
-k_bt_sketch(BtArgs B, int key_none, uint32_t *regs, unsigned long long *nan_rows) {
+k_bt_sketch(BtArgs B, int key_none, uint32_t *regs, unsigned long long *nan_rows, unsigned long long *mm) {
     __shared__ uint32_t r[BT_SK_REGS];
-    __shared__ unsigned long long nn;
+    __shared__ unsigned long long nn, s_lo, s_hi;
     for (int64_t it = blockIdx.x; it < B.n_items; it += gridDim.x) {
         for (int i = threadIdx.x; i < BT_SK_REGS; i += GB_THREADS) r[i] = 0;
-        if (threadIdx.x == 0) nn = 0;
+        if (threadIdx.x == 0) {
+            nn = 0;
+            s_lo = 0;
+            s_hi = 0;
+        }
         __syncthreads();
+        unsigned long long lo = 0, hi = 0;
         const BtItem item = B.items[it];
         const BtQuery &Q = B.qs[item.q];
         const int64_t r0 = (int64_t)item.part * B.tile;
@@ -173,6 +183,9 @@ k_bt_sketch(BtArgs B, int key_none, uint32_t *regs, unsigned long long *nan_rows
                 int cls = vg_key_class(Q.key, row, kn);
                 if (cls == VG_KEY_NAN) { my_nan++; continue; }
                 if (cls != VG_KEY_REGULAR) continue;
+                const uint64_t o = vg_i64_ordered(kn);
+                lo = ~o > lo ? ~o : lo;
+                hi = o > hi ? o : hi;
                 uint64_t h = vg_mix64(kn);
                 uint32_t idx = (uint32_t)(h >> (64 - BT_SK_BITS));
                 uint64_t rest = (h << BT_SK_BITS) | (1ull << (BT_SK_BITS - 1));
@@ -180,11 +193,17 @@ k_bt_sketch(BtArgs B, int key_none, uint32_t *regs, unsigned long long *nan_rows
                 if (r[idx] < rank) atomicMax(&r[idx], rank);
             }
         if (my_nan) atomicAdd(&nn, my_nan);
+        if (lo) atomicMax(&s_lo, lo);
+        if (hi) atomicMax(&s_hi, hi);
         __syncthreads();
         uint32_t *gr = regs + (size_t)item.q * BT_SK_REGS;
         for (int i = threadIdx.x; i < BT_SK_REGS; i += GB_THREADS)
             if (r[i]) atomicMax(&gr[i], r[i]);
-        if (threadIdx.x == 0 && nn) atomicAdd(&nan_rows[item.q], nn);
+        if (threadIdx.x == 0) {
+            if (nn) atomicAdd(&nan_rows[item.q], nn);
+            if (s_lo) atomicMax(&mm[2 * (size_t)item.q], s_lo);
+            if (s_hi) atomicMax(&mm[2 * (size_t)item.q + 1], s_hi);
+        }
         __syncthreads();
     }
 }
@@ -215,8 +234,8 @@ __global__ void k_bt_init_ctl(uint64_t *ctl, int n_queries, uint64_t *qcount, un
 struct BtLocal {
     vgpu_col key, probe_key;
     PredInfo pi;
-    int64_t n_rows, cap, tbase;
-    int32_t S, vm;
+    int64_t n_rows, cap, tbase, kmin;
+    int32_t S, vm, dense;
 };
 
 __device__ __forceinline__ BtLocal bt_load_query(const BtQuery &G, InSet &s_ins, ColSet &s_pcs) {
@@ -233,19 +252,64 @@ __device__ __forceinline__ BtLocal bt_load_query(const BtQuery &G, InSet &s_ins,
     Q.tbase = G.base;
     Q.S = G.lds_slots;
     Q.vm = G.vm;
+    Q.kmin = G.kmin;
+    Q.dense = G.dense;
     return Q;
+}
+
+// record of a key in a query's HBM table: hashed (gb_global_slot), or in a
+// dense table the key's own record; a key outside the planned range flags
+// the batch (its inputs changed since plan)
+__device__ __forceinline__ int64_t bt_table_slot(const GbLayout &L, const BtArgs &B, int dense, int64_t kmin,
+                                                 uint64_t *hot, int64_t cap, int cls, uint64_t kn, uint64_t pos,
+                                                 uint64_t *ctl) {
+    if (!dense || cls != VG_KEY_REGULAR) return gb_global_slot(L, hot, cap, cls, kn, pos, ctl);
+    const uint64_t d = kn - (uint64_t)kmin;
+    if (d < (uint64_t)cap) return (int64_t)d;
+    atomicOr((unsigned long long *)&B.ctl[(size_t)B.n_queries * GB_CTL_WORDS + BT_CTL_ERR], 1ull);
+    return -1;
+}
+
+// the block's LDS table of a query: dense when its HBM table is (keys
+// preset), else hashed
+__device__ __forceinline__ void bt_lds_init(const GbLayout &L, const BtLocal &Q, uint64_t *lhot, uint64_t *lcold,
+                                            uint64_t empty) {
+    gb_lds_init(L, lhot, lcold, Q.S, empty, Q.dense != 0, Q.kmin);
+}
+
+// the LDS table into the query's HBM table (records no row touched skipped)
+__device__ __forceinline__ void bt_lds_merge(const GbLayout &L, const BtArgs &B, const BtLocal &Q,
+                                             const uint64_t *lhot, const uint64_t *lcold, uint64_t empty,
+                                             uint64_t *hot, uint64_t *cold, uint64_t *ctl) {
+    __syncthreads();
+    for (int s = threadIdx.x; s < Q.S + 2; s += GB_THREADS) {
+        const uint64_t *h = lhot + (size_t)s * L.RH;
+        int64_t gs;
+        if (s < Q.S) {
+            if (h[0] == empty || h[1] == 0) continue;
+            gs = bt_table_slot(L, B, Q.dense, Q.kmin, hot, Q.cap, VG_KEY_REGULAR, h[0], 0, ctl);
+        } else {
+            if (h[1] == 0) continue;
+            gs = Q.cap + (s - Q.S);
+        }
+        if (gs >= 0)
+            gb_merge(L, hot + (size_t)gs * L.RH, cold + (size_t)gs * L.RC, h, lcold ? lcold + (size_t)s * L.RC : nullptr);
+    }
 }
 
 // one group update: LDS table when the key has a slot there, else the
 // query's HBM table
-__device__ __forceinline__ void bt_update(const GbLayout &L, const InSet &ins, int64_t row, uint64_t pos, int cls,
-                                          uint64_t kn, int S, uint64_t *lhot, uint64_t *lcold, uint64_t *hot,
-                                          uint64_t *cold, int64_t cap, uint64_t *ctl, uint64_t empty) {
-    int64_t ls = S ? gb_lds_slot(lhot, S, L.RH, cls, kn, empty) : -1;
+__device__ __forceinline__ void bt_update(const GbLayout &L, const BtArgs &B, const BtLocal &Q, const InSet &ins,
+                                          int64_t row, uint64_t pos, int cls, uint64_t kn, uint64_t *lhot,
+                                          uint64_t *lcold, uint64_t *hot, uint64_t *cold, uint64_t *ctl,
+                                          uint64_t empty) {
+    const int S = Q.S;
+    int64_t ls = -1;
+    if (S) ls = Q.dense ? gb_lds_dense_slot(S, cls, kn, Q.kmin) : gb_lds_slot(lhot, S, L.RH, cls, kn, empty);
     if (ls >= 0) {
         gb_apply(L, ins, row, pos, lhot + (size_t)ls * L.RH, lcold ? lcold + (size_t)ls * L.RC : nullptr);
     } else {
-        int64_t gs = gb_global_slot(L, hot, cap, cls, kn, pos, ctl);
+        int64_t gs = bt_table_slot(L, B, Q.dense, Q.kmin, hot, Q.cap, cls, kn, pos, ctl);
         if (gs >= 0) gb_apply(L, ins, row, pos, hot + (size_t)gs * L.RH, cold + (size_t)gs * L.RC);
     }
 }
@@ -258,7 +322,6 @@ __device__ __forceinline__ void bt_rows(const GbLayout &L, const BtArgs &B, cons
                                         int64_t r0, int64_t r1, uint64_t *lhot, uint64_t *lcold, uint64_t *hot,
                                         uint64_t *cold, uint64_t *ctl) {
     const uint64_t empty = L.key_is_double ? VG_EMPTY_F64 : VG_EMPTY_I64;
-    const int S = Q.S;
     for (int64_t base = r0 + threadIdx.x; base < r1; base += (int64_t)GB_THREADS * BT_U) {
         bool ok[BT_U];
         int cls[BT_U];
@@ -294,7 +357,7 @@ __device__ __forceinline__ void bt_rows(const GbLayout &L, const BtArgs &B, cons
                     int c = vg_key_class_bits(B.jn.key_type, v, v ? bits : 0, k);
                     uint64_t pos = ((uint64_t)row << 12) | (j < 0xFFFu ? j : 0xFFFu);
                     if (L.key_is_double && c == VG_KEY_REGULAR && k == 0) gb_note_zero_key(ctl, bits, pos);
-                    bt_update(L, ins, row, pos, c, k, S, lhot, lcold, hot, cold, Q.cap, ctl, empty);
+                    bt_update(L, B, Q, ins, row, pos, c, k, lhot, lcold, hot, cold, ctl, empty);
                 }
             }
             continue;
@@ -311,7 +374,7 @@ __device__ __forceinline__ void bt_rows(const GbLayout &L, const BtArgs &B, cons
             int64_t row = base + (int64_t)u * GB_THREADS;
             if (L.key_is_double && cls[u] == VG_KEY_REGULAR && kn[u] == 0)
                 gb_note_zero_key(ctl, vg_load_bits(Q.key, row), (uint64_t)row);
-            bt_update(L, ins, row, (uint64_t)row, cls[u], kn[u], S, lhot, lcold, hot, cold, Q.cap, ctl, empty);
+            bt_update(L, B, Q, ins, row, (uint64_t)row, cls[u], kn[u], lhot, lcold, hot, cold, ctl, empty);
         }
     }
 }
@@ -342,7 +405,7 @@ __global__ void __launch_bounds__(GB_THREADS) k_bt_feed(GbLayout L, BtArgs B) {
         const BtItem item = B.items[it];
         if ((int)item.q >= B.n_active) continue;   // a rerun of fewer queries: block-uniform skip
         if ((int)item.q != cur) {
-            if (cur >= 0 && Q.S) gb_lds_merge(L, lhot, lcold, Q.S, empty, hot, cold, Q.cap, ctl);
+            if (cur >= 0 && Q.S) bt_lds_merge(L, B, Q, lhot, lcold, empty, hot, cold, ctl);
             __syncthreads();
             Q = bt_load_query(B.qs[item.q], s_ins, s_pcs);
             __syncthreads();
@@ -351,7 +414,7 @@ __global__ void __launch_bounds__(GB_THREADS) k_bt_feed(GbLayout L, BtArgs B) {
             hot = B.hot + Q.tbase * L.RH;
             cold = B.cold + Q.tbase * L.RC;
             ctl = B.ctl + (size_t)cur * GB_CTL_WORDS;
-            if (Q.S) gb_lds_init(L, lhot, lcold, Q.S, empty);
+            if (Q.S) bt_lds_init(L, Q, lhot, lcold, empty);
         }
         const BtLocal Qi = Q;   // this item's copy stays in registers through the row loop
         const vgpu_prog &prog = B.progs[Qi.vm ? cur : 0];
@@ -361,7 +424,7 @@ __global__ void __launch_bounds__(GB_THREADS) k_bt_feed(GbLayout L, BtArgs B) {
         bt_rows(L, B, Qi, s_ins, s_pcs, prog, regs, regv, r0, r1, lhot, lcold, hot, cold, ctl);
         __syncthreads();
     }
-    if (cur >= 0 && Q.S) gb_lds_merge(L, lhot, lcold, Q.S, empty, hot, cold, Q.cap, ctl);
+    if (cur >= 0 && Q.S) bt_lds_merge(L, B, Q, lhot, lcold, empty, hot, cold, ctl);
 }
 
 // Shared scans: every query reads the same rows.  Each block keeps one
@@ -392,7 +455,7 @@ __global__ void __launch_bounds__(GB_THREADS) k_bt_feed_shared(GbLayout L, BtArg
     uint64_t *hot = B.hot + Q.tbase * L.RH;
     uint64_t *cold = B.cold + Q.tbase * L.RC;
     uint64_t *ctl = B.ctl + (size_t)q * GB_CTL_WORDS;
-    if (S) gb_lds_init(L, lhot, lcold, S, empty);
+    if (S) bt_lds_init(L, Q, lhot, lcold, empty);
     for (int64_t i = 0;; i++) {
         const int64_t t = xcd + (int64_t)BT_XCDS * (j + (int64_t)B.J * i);
         if (t >= B.n_tiles) break;
@@ -400,16 +463,25 @@ __global__ void __launch_bounds__(GB_THREADS) k_bt_feed_shared(GbLayout L, BtArg
         const int64_t r1 = r0 + BT_STILE < Q.n_rows ? r0 + BT_STILE : Q.n_rows;
         bt_rows(L, B, Q, s_ins, s_pcs, prog, regs, regv, r0, r1, lhot, lcold, hot, cold, ctl);
     }
-    if (S) gb_lds_merge(L, lhot, lcold, S, empty, hot, cold, Q.cap, ctl);
+    if (S) bt_lds_merge(L, B, Q, lhot, lcold, empty, hot, cold, ctl);
 }
 
 // sort entry of arena record (query q, table slot s): like gb_sort_entry,
 // but the NA group of the key order takes the largest key instead of being
 // set aside, so a stable sort puts it after the query's other groups
-__device__ __forceinline__ bool bt_sort_entry(const GbLayout &L, const RecSrc &R, int order, int64_t s, uint64_t &sk) {
+// (a dense table's record s below cap holds key kmin + s when a row
+// reached it: its key word is left empty until the compaction writes it)
+__device__ __forceinline__ bool bt_sort_entry(const GbLayout &L, const RecSrc &R, const BtQuery &Q, int order,
+                                              int64_t s, uint64_t &sk) {
     if (order == VGPU_ORDER_KEY && s == R.na_slot) {
         if (R.hot[s * R.hstride + 1] == 0) return false;
         sk = ~0ull;
+        return true;
+    }
+    if (Q.dense && s < Q.cap) {
+        const uint64_t *h = R.hot + s * R.hstride;
+        if (h[1] == 0) return false;
+        sk = order == VGPU_ORDER_FIRST_SEEN ? h[L.w_fpos] : vg_i64_ordered((uint64_t)Q.kmin + (uint64_t)s);
         return true;
     }
     return gb_sort_entry(L, R, order, s, sk);
@@ -438,7 +510,7 @@ k_bt_cmp_count(GbLayout L, BtArgs B, const BtItem *cmap, int order, uint32_t *bc
     for (int j = 0; j < GB_CMP_PER_THREAD; j++) {
         int64_t s = base + j * GB_THREADS + threadIdx.x;
         uint64_t sk;
-        if (s < n_slots && bt_sort_entry(L, R, order, s, sk)) c++;
+        if (s < n_slots && bt_sort_entry(L, R, Q, order, s, sk)) c++;
     }
     uint32_t total;
     gb_block_prefix(c, lds, total);
@@ -464,8 +536,9 @@ k_bt_cmp_write(GbLayout L, BtArgs B, const BtItem *cmap, int order, const uint64
     for (int j = 0; j < GB_CMP_PER_THREAD; j++) {
         int64_t s = base + j * GB_THREADS + threadIdx.x;
         sk[j] = 0;
-        if (s < n_slots && bt_sort_entry(L, R, order, s, sk[j])) {
+        if (s < n_slots && bt_sort_entry(L, R, Q, order, s, sk[j])) {
             mask |= 1u << j;
+            if (Q.dense && s < Q.cap) B.hot[(size_t)(Q.base + s) * L.RH] = (uint64_t)Q.kmin + (uint64_t)s;
             o |= sk[j];
             a &= sk[j];
         }
@@ -806,7 +879,7 @@ __global__ void k_bt_chain_expand(GbLayout L, BtArgs B, BtChainArgs A) {
     uint64_t kn = 0;
     const int cls = e == A.E - 1 ? VG_KEY_NA
                                  : vg_key_class_bits(A.key_type, true, (uint64_t)A.kmin + (uint64_t)e, kn);
-    const int64_t gs = gb_global_slot(L, hot, Q.cap, cls, kn, 0, ctl);
+    const int64_t gs = bt_table_slot(L, B, Q.dense, Q.kmin, hot, Q.cap, cls, kn, 0, ctl);
     if (gs < 0) return;
     uint64_t *h = hot + (size_t)gs * L.RH;
     gb_atomic_add(&h[1], n);
@@ -1126,6 +1199,7 @@ extern "C" int vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_q
         for (int i = 0; i < b->n_pred_cols && same; i++) same = same_col(A.pcs.c[i], C2.pcs.c[i]);
     }
     const bool shared = same && n_queries > 1;
+    const bool dense_ok = !L.key_none && !L.key_is_double && vg_tune(VG_TUNE_BT_DENSE) > 1;
     // measured on MI355X with 1B-row tables: the per-row work, not HBM,
     // bounds these queries and query blocks drift apart, so shared scans
     // ran slower than one query after another; they stay behind a knob
@@ -1148,6 +1222,7 @@ extern "C" int vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_q
     // batch sketches the group-key payload column of the build side once, a
     // shared scan the one key column once
     std::vector<int64_t> est((size_t)n_queries, 1);
+    std::vector<uint64_t> klo((size_t)n_queries, ~0ull), khi((size_t)n_queries, 0);   // ordered key range
     std::vector<BtQuery> sq;      // the columns the sketch reads
     std::vector<BtItem> sitems;
     if (b->join.on) {
@@ -1168,12 +1243,13 @@ extern "C" int vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_q
         const int64_t ni = (int64_t)sitems.size();
         DevBuf sk, dq, di;
         size_t regs = (size_t)nq * BT_SK_REGS;
-        if (sk.reserve(regs * 4 + (size_t)nq * 8) || dq.reserve(sq.size() * sizeof(BtQuery)) ||
+        if (sk.reserve(regs * 4 + (size_t)nq * 24) || dq.reserve(sq.size() * sizeof(BtQuery)) ||
             di.reserve(sitems.size() * sizeof(BtItem)))
             return -1;
         uint32_t *d_regs = sk.as<uint32_t>();
         auto *d_nan = reinterpret_cast<unsigned long long *>(d_regs + regs);
-        k_bt_clear_u32<<<vg_blocks((int64_t)regs + 2 * nq, 256), 256, 0, st>>>(d_regs, (int64_t)regs + 2 * nq);
+        unsigned long long *d_mm = d_nan + nq;
+        k_bt_clear_u32<<<vg_blocks((int64_t)regs + 6 * nq, 256), 256, 0, st>>>(d_regs, (int64_t)regs + 6 * nq);
         VG_LAUNCH_CHECK();
         VG_TRY(hipMemcpyAsync(dq.ptr, sq.data(), sq.size() * sizeof(BtQuery), hipMemcpyHostToDevice, st));
         VG_TRY(hipMemcpyAsync(di.ptr, sitems.data(), sitems.size() * sizeof(BtItem), hipMemcpyHostToDevice, st));
@@ -1183,18 +1259,23 @@ extern "C" int vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_q
         A.items = di.as<BtItem>();
         A.n_items = ni;
         A.tile = b->tile;
-        k_bt_sketch<<<vg_blocks(ni, 1, 4096), GB_THREADS, 0, st>>>(A, L.key_none, d_regs, d_nan);
+        k_bt_sketch<<<vg_blocks(ni, 1, 4096), GB_THREADS, 0, st>>>(A, L.key_none, d_regs, d_nan, d_mm);
         VG_LAUNCH_CHECK();
         std::vector<uint32_t> h(regs);
-        std::vector<uint64_t> hn((size_t)nq);
+        std::vector<uint64_t> hn((size_t)nq), hm((size_t)nq * 2);
         VG_TRY(hipMemcpyAsync(h.data(), d_regs, regs * 4, hipMemcpyDeviceToHost, st));
         VG_TRY(hipMemcpyAsync(hn.data(), d_nan, (size_t)nq * 8, hipMemcpyDeviceToHost, st));
+        VG_TRY(hipMemcpyAsync(hm.data(), d_mm, (size_t)nq * 16, hipMemcpyDeviceToHost, st));
         VG_TRY(hipStreamSynchronize(st));
         sk.release();
         dq.release();
         di.release();
-        for (int q = 0; q < n_queries; q++)
-            est[q] = bt_estimate(&h[(size_t)(nq == n_queries ? q : 0) * BT_SK_REGS], hn[nq == n_queries ? q : 0]);
+        for (int q = 0; q < n_queries; q++) {
+            const int sq_i = nq == n_queries ? q : 0;
+            est[q] = bt_estimate(&h[(size_t)sq_i * BT_SK_REGS], hn[sq_i]);
+            klo[q] = ~hm[2 * (size_t)sq_i];
+            khi[q] = hm[2 * (size_t)sq_i + 1];
+        }
     }
     // LDS tables drop the cold words when no input can touch them
     bool cold = false;
@@ -1213,6 +1294,15 @@ extern "C" int vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_q
                                    : std::max<int64_t>(Q.n_rows, 1);
         int64_t g = std::min<int64_t>(bound, est[q] + est[q] / 4 + 16);
         Q.cap = pow2_at_least(2 * g);
+        // a dense table when the keys' range is no longer than the hashed
+        // table would be (every key the sketch saw is in the range)
+        Q.dense = 0;
+        Q.kmin = 0;
+        if (dense_ok && klo[q] <= khi[q] && khi[q] - klo[q] < (uint64_t)Q.cap) {
+            Q.dense = 1;
+            Q.kmin = (int64_t)(klo[q] ^ 0x8000000000000000ull);
+            Q.cap = (int64_t)(khi[q] - klo[q]) + 1;
+        }
         Q.base = slots;
         qbase[q] = (uint64_t)slots;
         slots += Q.cap + 2;
@@ -1224,6 +1314,7 @@ extern "C" int vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_q
         // (a load of up to 0.8 when the tuned load does not fit)
         if ((size_t)(S + 2) * rw + vm > BT_LDS_BUDGET) S = pow2_at_least(hashed + hashed / 4 + 1);
         if (L.key_none) S = 64;
+        if (Q.dense) S = Q.cap;   // the LDS table is dense too: one record per key of the range
         Q.lds_slots = 0;
         int64_t tile = shared ? (int64_t)BT_STILE * 64 : b->tile;   // rows one LDS table sees
         if ((size_t)(S + 2) * rw + vm <= BT_LDS_BUDGET && S * 8 <= std::min<int64_t>(Q.n_rows, tile)) {
