@@ -396,6 +396,8 @@ def run_c4(args, nv):
     import torch
     Q = args.c4_queries
     rows, card, thr = c4_plan(Q)
+    if args.c4_card > 0:     # every query with the same key cardinality (a sweep point, not the workload)
+        card[:] = args.c4_card
     total_rows = int(rows.sum())
     log(f"[bench] c4: {Q} queries, {total_rows:,} rows ({total_rows * 16 / 1e9:.1f} GB), generating on the device")
     keys, vals = [], []
@@ -490,7 +492,8 @@ def run_c4(args, nv):
                                "[1e3, 1e7], key cardinality log-uniform [1, 1e5] (int64), value N(0,1), selectivity "
                                "U[1%, 50%]; one persistent feed launch per batch, replayed as a hipGraph; "
                                "device-resident inputs",
-                   "queries": Q, "rows": total_rows, "groups": int(n_groups.sum())},
+                   "queries": Q, "rows": total_rows, "groups": int(n_groups.sum()),
+                   "fixed_key_cardinality": args.c4_card or None, "knobs": args.tune or None},
         "batched": {"queries_per_s": Q / (ms / 1e3), "ms_per_batch": ms, "plan_ms": plan_ms},
         "plain_gpu": {"queries_per_s": Q / plain_s, "total_ms": plain_s * 1e3,
                       "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99))},
@@ -656,6 +659,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3-high", "c3-low", "c4", "c5"])
     ap.add_argument("--c4-queries", type=int, default=4096)
+    ap.add_argument("--c4-card", type=int, default=0, help="fix every query's key cardinality (0: log-uniform)")
     ap.add_argument("--c5-rows", type=int, default=1_000_000_000)
     ap.add_argument("--c5-dim", type=int, default=10_000_000)
     ap.add_argument("--c5-max-batch", type=int, default=8)
@@ -668,6 +672,8 @@ def main():
     ap.add_argument("--dim-rows", type=int, default=1_000_000)
     ap.add_argument("--cpu-sample", type=int, default=10_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tune", action="append", default=[], metavar="NAME=VALUE",
+                    help="set a launch knob (vgpu_tune_set) before the run, e.g. batch.dense=1")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="copy 1 GiB three times (reference kernel for rocprofv3 FETCH_SIZE / WRITE_SIZE scaling)")
     args = ap.parse_args()
@@ -689,6 +695,11 @@ def main():
     lib = nv.vgpu()
     if lib.vgpu_init(0) != 0:
         raise SystemExit("bench: " + lib.vgpu_last_error().decode())
+    for kv in args.tune:
+        name, _, val = kv.partition("=")
+        if lib.vgpu_tune_set(name.encode(), int(val)) != 0:
+            raise SystemExit(f"bench: unknown knob or value {kv!r}")
+        log(f"[bench] knob {name} = {val}")
     if args.workload == "c1":
         out = run_c1(args, nv)
     elif args.workload == "c3-high":
