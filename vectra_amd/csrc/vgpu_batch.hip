@@ -36,12 +36,16 @@
 #define BT_TILE_MAX (1 << 21)      // ... and at most; chosen per batch from its total rows
 #define BT_SK_BITS 10              // sketch registers per query: 2^10
 #define BT_SK_REGS (1 << BT_SK_BITS)
-#define BT_LDS_BUDGET 61440        // dynamic LDS of the feed blocks
+#define BT_LDS_BUDGET 61440        // dynamic LDS of the 256-thread feed blocks
+#define BT_LDS_BIG (160 * 1024 - 2048)   // ... of the 1,024-thread blocks of the large-table launch
+#define BT_BIG_THREADS 1024
 #define BT_BLOCKS_PER_CU 8
 #define BT_U 4                     // rows in flight per thread in the feed
 #define BT_STILE 16384             // rows per tile of a shared scan
 #define BT_CTL_ERR 9               // batch control word: inputs left the plan's key range
-#define BT_SPLIT_SLOTS 256         // queries with larger LDS tables run in the feed's second launch
+#define BT_SPLIT_SLOTS 256         // queries with larger LDS tables run in a later launch of the feed
+#define BT_CLASSES 3               // feed launches: small / mid LDS tables (256 threads), large (1,024)
+#define BT_MID_BYTES (160 * 1024 / BT_BLOCKS_PER_CU - 512)   // LDS of a mid table: 8 blocks per CU still fit
 
 struct BtQuery {
     vgpu_col key;
@@ -109,13 +113,17 @@ struct vgpu_batch {
     int32_t S_max = 0;
     int32_t vm_lds = 0;
     int32_t lds_cold = 1;
-    int64_t feed_grid = 1;          // persistent feed blocks: a few per CU
     // the feed's items in two classes, one launch each: queries with small
     // or no LDS tables (many blocks per CU) first, then queries with large
     // LDS tables (their LDS size limits the blocks per CU)
-    int64_t n_items_a = 0;
-    int32_t S_max_a = 0, S_max_b = 0;
-    int64_t feed_grid_b = 1;
+    // the feed's launches, one per class of LDS table size: items
+    // [cls_first[c], cls_first[c + 1]) of the item list, threads per block,
+    // rows per item, largest table, grid
+    int64_t cls_first[BT_CLASSES + 1] = {};
+    int32_t cls_nt[BT_CLASSES] = {GB_THREADS, GB_THREADS, GB_THREADS};
+    int64_t cls_tile[BT_CLASSES] = {};
+    int32_t cls_S[BT_CLASSES] = {};
+    int64_t cls_grid[BT_CLASSES] = {};
     int64_t shared_resident = 1;    // feed blocks resident at once (LDS and wave limits)
     int64_t tile = BT_TILE_MIN;     // rows per work item
     int32_t planned = 0;
@@ -224,8 +232,7 @@ __global__ void k_bt_init_ctl(uint64_t *ctl, int n_queries, uint64_t *qcount, un
     }
     if (i < n_queries) qcount[i] = 0;
     if (i == 0) {
-        queue[0] = 0;
-        queue[1] = 0;
+        for (int c = 0; c < BT_CLASSES; c++) queue[c] = 0;
     }
 }
 
@@ -238,10 +245,11 @@ struct BtLocal {
     int32_t S, vm, dense;
 };
 
+template <int NT>
 __device__ __forceinline__ BtLocal bt_load_query(const BtQuery &G, InSet &s_ins, ColSet &s_pcs) {
-    for (int w = threadIdx.x; w < (int)(sizeof(InSet) / 4); w += GB_THREADS)
+    for (int w = threadIdx.x; w < (int)(sizeof(InSet) / 4); w += NT)
         reinterpret_cast<uint32_t *>(&s_ins)[w] = reinterpret_cast<const uint32_t *>(&G.ins)[w];
-    for (int w = threadIdx.x; w < (int)(sizeof(ColSet) / 4); w += GB_THREADS)
+    for (int w = threadIdx.x; w < (int)(sizeof(ColSet) / 4); w += NT)
         reinterpret_cast<uint32_t *>(&s_pcs)[w] = reinterpret_cast<const uint32_t *>(&G.pcs)[w];
     BtLocal Q;
     Q.key = G.key;
@@ -272,17 +280,19 @@ __device__ __forceinline__ int64_t bt_table_slot(const GbLayout &L, const BtArgs
 
 // the block's LDS table of a query: dense when its HBM table is (keys
 // preset), else hashed
+template <int NT>
 __device__ __forceinline__ void bt_lds_init(const GbLayout &L, const BtLocal &Q, uint64_t *lhot, uint64_t *lcold,
                                             uint64_t empty) {
-    gb_lds_init(L, lhot, lcold, Q.S, empty, Q.dense != 0, Q.kmin);
+    gb_lds_init<NT>(L, lhot, lcold, Q.S, empty, Q.dense != 0, Q.kmin);
 }
 
 // the LDS table into the query's HBM table (records no row touched skipped)
+template <int NT>
 __device__ __forceinline__ void bt_lds_merge(const GbLayout &L, const BtArgs &B, const BtLocal &Q,
                                              const uint64_t *lhot, const uint64_t *lcold, uint64_t empty,
                                              uint64_t *hot, uint64_t *cold, uint64_t *ctl) {
     __syncthreads();
-    for (int s = threadIdx.x; s < Q.S + 2; s += GB_THREADS) {
+    for (int s = threadIdx.x; s < Q.S + 2; s += NT) {
         const uint64_t *h = lhot + (size_t)s * L.RH;
         int64_t gs;
         if (s < Q.S) {
@@ -317,29 +327,30 @@ __device__ __forceinline__ void bt_update(const GbLayout &L, const BtArgs &B, co
 // rows [r0, r1) of one query: BT_U rows per thread per step, their
 // predicate and key (or join probe) loads issued together so each wave
 // keeps several loads in flight
+template <int NT>
 __device__ __forceinline__ void bt_rows(const GbLayout &L, const BtArgs &B, const BtLocal &Q, const InSet &ins,
                                         const ColSet &pcs, const vgpu_prog &prog, uint64_t *regs, uint8_t *regv,
                                         int64_t r0, int64_t r1, uint64_t *lhot, uint64_t *lcold, uint64_t *hot,
                                         uint64_t *cold, uint64_t *ctl) {
     const uint64_t empty = L.key_is_double ? VG_EMPTY_F64 : VG_EMPTY_I64;
-    for (int64_t base = r0 + threadIdx.x; base < r1; base += (int64_t)GB_THREADS * BT_U) {
+    for (int64_t base = r0 + threadIdx.x; base < r1; base += (int64_t)NT * BT_U) {
         bool ok[BT_U];
         int cls[BT_U];
         uint64_t kn[BT_U];
         const uint64_t *ent[BT_U];
 #pragma unroll
         for (int u = 0; u < BT_U; u++) {
-            int64_t row = base + (int64_t)u * GB_THREADS;
-            ok[u] = row < r1 && vg_pred_row(Q.pi, prog, pcs, row, regs, regv, GB_THREADS, threadIdx.x);
+            int64_t row = base + (int64_t)u * NT;
+            ok[u] = row < r1 && vg_pred_row(Q.pi, prog, pcs, row, regs, regv, NT, threadIdx.x);
         }
         if (B.jn.on) {
 #pragma unroll
             for (int u = 0; u < BT_U; u++)
-                ent[u] = ok[u] ? jt_find(B.jn.jv, Q.probe_key, base + (int64_t)u * GB_THREADS) : nullptr;
+                ent[u] = ok[u] ? jt_find(B.jn.jv, Q.probe_key, base + (int64_t)u * NT) : nullptr;
 #pragma unroll
             for (int u = 0; u < BT_U; u++) {
                 if (!ent[u]) continue;
-                int64_t row = base + (int64_t)u * GB_THREADS;
+                int64_t row = base + (int64_t)u * NT;
                 const uint64_t segw = ent[u][1];
                 const uint32_t cnt = jt_count(B.jn.jv, segw), off = (uint32_t)(segw >> 32);
                 const int p = B.jn.key_payload;
@@ -366,12 +377,12 @@ __device__ __forceinline__ void bt_rows(const GbLayout &L, const BtArgs &B, cons
         for (int u = 0; u < BT_U; u++) {
             kn[u] = 0;
             cls[u] = VG_KEY_REGULAR;
-            if (ok[u] && !L.key_none) cls[u] = vg_key_class(Q.key, base + (int64_t)u * GB_THREADS, kn[u]);
+            if (ok[u] && !L.key_none) cls[u] = vg_key_class(Q.key, base + (int64_t)u * NT, kn[u]);
         }
 #pragma unroll
         for (int u = 0; u < BT_U; u++) {
             if (!ok[u]) continue;
-            int64_t row = base + (int64_t)u * GB_THREADS;
+            int64_t row = base + (int64_t)u * NT;
             if (L.key_is_double && cls[u] == VG_KEY_REGULAR && kn[u] == 0)
                 gb_note_zero_key(ctl, vg_load_bits(Q.key, row), (uint64_t)row);
             bt_update(L, B, Q, ins, row, (uint64_t)row, cls[u], kn[u], lhot, lcold, hot, cold, ctl, empty);
@@ -379,7 +390,8 @@ __device__ __forceinline__ void bt_rows(const GbLayout &L, const BtArgs &B, cons
     }
 }
 
-__global__ void __launch_bounds__(GB_THREADS) k_bt_feed(GbLayout L, BtArgs B) {
+template <int NT>
+__global__ void __launch_bounds__(NT) k_bt_feed(GbLayout L, BtArgs B) {
     extern __shared__ uint64_t dynw[];
     __shared__ long long s_item;
     __shared__ InSet s_ins;
@@ -405,26 +417,26 @@ __global__ void __launch_bounds__(GB_THREADS) k_bt_feed(GbLayout L, BtArgs B) {
         const BtItem item = B.items[it];
         if ((int)item.q >= B.n_active) continue;   // a rerun of fewer queries: block-uniform skip
         if ((int)item.q != cur) {
-            if (cur >= 0 && Q.S) bt_lds_merge(L, B, Q, lhot, lcold, empty, hot, cold, ctl);
+            if (cur >= 0 && Q.S) bt_lds_merge<NT>(L, B, Q, lhot, lcold, empty, hot, cold, ctl);
             __syncthreads();
-            Q = bt_load_query(B.qs[item.q], s_ins, s_pcs);
+            Q = bt_load_query<NT>(B.qs[item.q], s_ins, s_pcs);
             __syncthreads();
             cur = (int)item.q;
             lcold = B.lds_cold ? lhot + (size_t)(Q.S + 2) * L.RH : nullptr;
             hot = B.hot + Q.tbase * L.RH;
             cold = B.cold + Q.tbase * L.RC;
             ctl = B.ctl + (size_t)cur * GB_CTL_WORDS;
-            if (Q.S) bt_lds_init(L, Q, lhot, lcold, empty);
+            if (Q.S) bt_lds_init<NT>(L, Q, lhot, lcold, empty);
         }
         const BtLocal Qi = Q;   // this item's copy stays in registers through the row loop
         const vgpu_prog &prog = B.progs[Qi.vm ? cur : 0];
-        uint8_t *regv = dyn + (size_t)(Qi.vm ? prog.n_insn : 0) * GB_THREADS * 8;
+        uint8_t *regv = dyn + (size_t)(Qi.vm ? prog.n_insn : 0) * NT * 8;
         const int64_t r0 = (int64_t)item.part * B.tile;
         const int64_t r1 = r0 + B.tile < Qi.n_rows ? r0 + B.tile : Qi.n_rows;
-        bt_rows(L, B, Qi, s_ins, s_pcs, prog, regs, regv, r0, r1, lhot, lcold, hot, cold, ctl);
+        bt_rows<NT>(L, B, Qi, s_ins, s_pcs, prog, regs, regv, r0, r1, lhot, lcold, hot, cold, ctl);
         __syncthreads();
     }
-    if (cur >= 0 && Q.S) bt_lds_merge(L, B, Q, lhot, lcold, empty, hot, cold, ctl);
+    if (cur >= 0 && Q.S) bt_lds_merge<NT>(L, B, Q, lhot, lcold, empty, hot, cold, ctl);
 }
 
 // Shared scans: every query reads the same rows.  Each block keeps one
@@ -446,7 +458,7 @@ __global__ void __launch_bounds__(GB_THREADS) k_bt_feed_shared(GbLayout L, BtArg
     const int xcd = blockIdx.x % BT_XCDS, local = blockIdx.x / BT_XCDS;
     const int q = local % B.K, j = local / B.K;
     if (q >= B.n_queries || j >= B.J) return;
-    const BtLocal Q = bt_load_query(B.qs[q], s_ins, s_pcs);
+    const BtLocal Q = bt_load_query<GB_THREADS>(B.qs[q], s_ins, s_pcs);
     __syncthreads();
     const int S = Q.S;
     const vgpu_prog &prog = B.progs[Q.vm ? q : 0];
@@ -455,15 +467,15 @@ __global__ void __launch_bounds__(GB_THREADS) k_bt_feed_shared(GbLayout L, BtArg
     uint64_t *hot = B.hot + Q.tbase * L.RH;
     uint64_t *cold = B.cold + Q.tbase * L.RC;
     uint64_t *ctl = B.ctl + (size_t)q * GB_CTL_WORDS;
-    if (S) bt_lds_init(L, Q, lhot, lcold, empty);
+    if (S) bt_lds_init<GB_THREADS>(L, Q, lhot, lcold, empty);
     for (int64_t i = 0;; i++) {
         const int64_t t = xcd + (int64_t)BT_XCDS * (j + (int64_t)B.J * i);
         if (t >= B.n_tiles) break;
         const int64_t r0 = t * BT_STILE;
         const int64_t r1 = r0 + BT_STILE < Q.n_rows ? r0 + BT_STILE : Q.n_rows;
-        bt_rows(L, B, Q, s_ins, s_pcs, prog, regs, regv, r0, r1, lhot, lcold, hot, cold, ctl);
+        bt_rows<GB_THREADS>(L, B, Q, s_ins, s_pcs, prog, regs, regv, r0, r1, lhot, lcold, hot, cold, ctl);
     }
-    if (S) bt_lds_merge(L, B, Q, lhot, lcold, empty, hot, cold, ctl);
+    if (S) bt_lds_merge<GB_THREADS>(L, B, Q, lhot, lcold, empty, hot, cold, ctl);
 }
 
 // sort entry of arena record (query q, table slot s): like gb_sort_entry,
@@ -1210,7 +1222,7 @@ extern "C" int vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_q
     b->vm_lds = (int32_t)vm;
     if (b->qs.reserve(qs.size() * sizeof(BtQuery)) || b->progs.reserve(progs.size() * sizeof(vgpu_prog)) ||
         b->items.reserve(std::max<size_t>(items.size(), 1) * sizeof(BtItem)) ||
-        b->ctl.reserve((size_t)(n_queries + 1) * GB_CTL_WORDS * 8) || b->queue.reserve(16) ||
+        b->ctl.reserve((size_t)(n_queries + 1) * GB_CTL_WORDS * 8) || b->queue.reserve(BT_CLASSES * 8) ||
         b->qcount.reserve((size_t)n_queries * 8) || b->qoff.reserve((size_t)(n_queries + 1) * 8) ||
         b->qbase.reserve((size_t)(n_queries + 1) * 8))
         return -1;
@@ -1288,6 +1300,20 @@ extern "C" int vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_q
     std::vector<BtItem> cmap;
     int64_t slots = 0;
     int S_max = 0;
+    // Tables above BT_SPLIT_SLOTS records run in launches of their own: those
+    // that still let 8 blocks of 256 threads share a CU in one, larger ones
+    // in another whose blocks are 1,024 threads sharing one table of up to
+    // BT_LDS_BIG bytes, so a table of a few thousand groups still has 16
+    // waves per CU (not with a VM predicate, whose LDS registers are sized
+    // for 256 threads: its large tables stay in 256-thread blocks within
+    // BT_LDS_BUDGET). A 1,024-thread block takes four tiles per item: as
+    // many rows per thread as the others.
+    const bool big_ok = !shared && !vm;
+    const size_t budget = big_ok ? BT_LDS_BIG : BT_LDS_BUDGET;
+    const int64_t tile_big = b->tile * (BT_BIG_THREADS / GB_THREADS);
+    auto big_table = [&](int64_t S) {
+        return big_ok && S > BT_SPLIT_SLOTS && (size_t)(S + 2) * rw + vm > BT_MID_BYTES;
+    };
     for (int q = 0; q < n_queries; q++) {
         BtQuery &Q = qs[q];
         int64_t bound = b->join.on ? std::max<int64_t>(Q.n_rows, 1) * std::max<int64_t>(b->join_rows, 1)
@@ -1312,12 +1338,14 @@ extern "C" int vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_q
         const int64_t hashed = std::max<int64_t>(est[q] - 2, 1);
         int64_t S = pow2_at_least(vg_tune_for(VG_TUNE_GB_LDS_SLOTS_PER_GROUP, est[q]) * hashed);
         // (a load of up to 0.8 when the tuned load does not fit)
-        if ((size_t)(S + 2) * rw + vm > BT_LDS_BUDGET) S = pow2_at_least(hashed + hashed / 4 + 1);
+        if ((size_t)(S + 2) * rw + vm > budget) S = pow2_at_least(hashed + hashed / 4 + 1);
         if (L.key_none) S = 64;
         if (Q.dense) S = Q.cap;   // the LDS table is dense too: one record per key of the range
         Q.lds_slots = 0;
         int64_t tile = shared ? (int64_t)BT_STILE * 64 : b->tile;   // rows one LDS table sees
-        if ((size_t)(S + 2) * rw + vm <= BT_LDS_BUDGET && S * 8 <= std::min<int64_t>(Q.n_rows, tile)) {
+        if (big_table(S)) tile = tile_big;
+        if ((size_t)(S + 2) * rw + vm <= (S > BT_SPLIT_SLOTS ? budget : BT_LDS_BUDGET) &&
+            S * 8 <= std::min<int64_t>(Q.n_rows, tile)) {
             Q.lds_slots = (int32_t)S;
             S_max = std::max<int>(S_max, (int)S);
         }
@@ -1334,35 +1362,42 @@ extern "C" int vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_q
     int dev = 0, cus = 0;
     VG_TRY(hipGetDevice(&dev));
     VG_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    b->feed_grid = std::max<int64_t>(1, std::min<int64_t>(b->n_items, (int64_t)cus * BT_BLOCKS_PER_CU));
     {
         // items of the queries with small or no LDS tables, then the rest
         // (query-major within each class, as the blocks' sticky tables want)
         std::vector<BtItem> ordered;
         ordered.reserve(items.size());
-        int Sa = 0, Sb = 0;
-        for (int pass = 0; pass < 2; pass++)
+        auto cls_of = [&](const BtQuery &Q) {
+            if (Q.lds_slots <= BT_SPLIT_SLOTS) return 0;
+            return big_table(Q.lds_slots) ? 2 : 1;
+        };
+        for (int c = 0; c < BT_CLASSES; c++) {
+            b->cls_first[c] = (int64_t)ordered.size();
+            b->cls_nt[c] = c == 2 ? BT_BIG_THREADS : GB_THREADS;
+            b->cls_tile[c] = c == 2 ? tile_big : b->tile;
+            b->cls_S[c] = 0;
             for (int q = 0; q < n_queries; q++) {
-                const bool big = qs[q].lds_slots > BT_SPLIT_SLOTS;
-                if (big != (pass == 1)) continue;
-                if (big) Sb = std::max<int>(Sb, qs[q].lds_slots);
-                else Sa = std::max<int>(Sa, qs[q].lds_slots);
-                for (int64_t t = 0; t * b->tile < qs[q].n_rows; t++) ordered.push_back(BtItem{(uint32_t)q, (uint32_t)t});
-                if (pass == 0) b->n_items_a = (int64_t)ordered.size();
+                if (cls_of(qs[q]) != c) continue;
+                b->cls_S[c] = std::max<int>(b->cls_S[c], qs[q].lds_slots);
+                for (int64_t t = 0; t * b->cls_tile[c] < qs[q].n_rows; t++)
+                    ordered.push_back(BtItem{(uint32_t)q, (uint32_t)t});
             }
+        }
+        b->cls_first[BT_CLASSES] = (int64_t)ordered.size();
+        b->n_items = (int64_t)ordered.size();
         if (!ordered.empty())
             VG_TRY(hipMemcpyAsync(b->items.ptr, ordered.data(), ordered.size() * sizeof(BtItem), hipMemcpyHostToDevice,
                                   st));
-        b->S_max_a = Sa;
-        b->S_max_b = Sb;
-        auto grid_for = [&](int64_t n, int Sx) {
-            const size_t lds = (size_t)b->vm_lds + (Sx ? (size_t)(Sx + 2) * rw : 0) + 512;
-            const int64_t per_cu = std::max<int64_t>(1, std::min<int64_t>(BT_BLOCKS_PER_CU,
-                                                                           (int64_t)(160 * 1024 / lds)));
-            return std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)cus * per_cu));
-        };
-        b->feed_grid = grid_for(b->n_items_a, Sa);
-        b->feed_grid_b = grid_for(b->n_items - b->n_items_a, Sb);
+        for (int c = 0; c < BT_CLASSES; c++) {
+            const size_t lds = (size_t)b->vm_lds + (b->cls_S[c] ? (size_t)(b->cls_S[c] + 2) * rw : 0);
+            const int64_t per_cu = std::max<int64_t>(
+                1, std::min<int64_t>(BT_BLOCKS_PER_CU * GB_THREADS / b->cls_nt[c], (int64_t)(160 * 1024 / (lds + 512))));
+            b->cls_grid[c] = std::max<int64_t>(
+                1, std::min<int64_t>(b->cls_first[c + 1] - b->cls_first[c], (int64_t)cus * per_cu));
+            if (b->cls_nt[c] == BT_BIG_THREADS && lds > 65536)
+                VG_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(k_bt_feed<BT_BIG_THREADS>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        }
     }
     {
         // 160 KB of LDS and 32 waves (8 blocks of 256 threads) per CU
@@ -1484,21 +1519,20 @@ static int bt_enqueue(vgpu_batch *b, bool first) {
             k_bt_feed_shared<<<(unsigned)((int64_t)BT_XCDS * A.K * A.J), GB_THREADS, lds, st>>>(L, A);
             VG_LAUNCH_CHECK();
         } else {
-            // small-table and HBM-table queries, then large-table queries
-            if (b->n_items_a > 0) {
-                BtArgs Aa = A;
-                Aa.n_items = b->n_items_a;
-                const size_t lds = (size_t)b->vm_lds + (b->S_max_a ? (size_t)(b->S_max_a + 2) * rwb : 0);
-                k_bt_feed<<<(unsigned)b->feed_grid, GB_THREADS, lds, st>>>(L, Aa);
-                VG_LAUNCH_CHECK();
-            }
-            if (b->n_items > b->n_items_a) {
-                BtArgs Ab = A;
-                Ab.items = A.items + b->n_items_a;
-                Ab.n_items = b->n_items - b->n_items_a;
-                Ab.queue = A.queue + 1;
-                const size_t lds = (size_t)b->vm_lds + (size_t)(b->S_max_b + 2) * rwb;
-                k_bt_feed<<<(unsigned)b->feed_grid_b, GB_THREADS, lds, st>>>(L, Ab);
+            // small-table and HBM-table queries, then mid-size and large LDS tables
+            for (int c = 0; c < BT_CLASSES; c++) {
+                const int64_t n = b->cls_first[c + 1] - b->cls_first[c];
+                if (n <= 0) continue;
+                BtArgs Ac = A;
+                Ac.items = A.items + b->cls_first[c];
+                Ac.n_items = n;
+                Ac.tile = b->cls_tile[c];
+                Ac.queue = A.queue + c;
+                const size_t lds = (size_t)b->vm_lds + (b->cls_S[c] ? (size_t)(b->cls_S[c] + 2) * rwb : 0);
+                if (b->cls_nt[c] == BT_BIG_THREADS)
+                    k_bt_feed<BT_BIG_THREADS><<<(unsigned)b->cls_grid[c], BT_BIG_THREADS, lds, st>>>(L, Ac);
+                else
+                    k_bt_feed<GB_THREADS><<<(unsigned)b->cls_grid[c], GB_THREADS, lds, st>>>(L, Ac);
                 VG_LAUNCH_CHECK();
             }
         }

@@ -225,9 +225,10 @@ static inline bool gb_needs_cold(const GbLayout &L, const vgpu_col *inputs) {
 // key k owns record k - kmin, its key word preset, so a row finds its
 // record without hashing or probing; keys outside the range take the HBM
 // table.  The merge skips the records no row touched (n == 0).
+template <int NT = GB_THREADS>
 __device__ __forceinline__ void gb_lds_init(const GbLayout &L, uint64_t *lhot, uint64_t *lcold, int S,
                                             uint64_t empty, bool dense = false, int64_t kmin = 0) {
-    for (int s = threadIdx.x; s < S + 2; s += GB_THREADS) {
+    for (int s = threadIdx.x; s < S + 2; s += NT) {
         uint64_t *h = lhot + (size_t)s * L.RH;
         h[0] = (dense && s < S) ? (uint64_t)kmin + (uint64_t)s : empty;
         for (int w = 1; w < L.RH; w++) h[w] = ((L.hot_init_ones >> w) & 1) ? ~0ull : 0ull;
