@@ -372,3 +372,43 @@ def test_batch_dense_flags_keys_outside_range(gpu, oracle):
     b.free()
     for c in cols:
         c.free()
+
+
+@pytest.mark.parametrize("temp_dir", ["/tmp", None])
+def test_batch_dense_tables_of_every_size(gpu, oracle, temp_dir):
+    """sum / n / mean of a double without NAs (three-word records) over key
+    ranges of 100 to 30,000 keys: LDS tables of up to 256 records, mid-size
+    and large dense LDS tables, a dense LDS table covering the first keys of
+    a range too wide for all of it (the rest in HBM), and HBM alone"""
+    r = rng(31)
+    aggs = [("s", "sum", "x", 0), ("n", "n", None, 0), ("m", "mean", "x", 0)]
+    qs = []
+    for q, card in enumerate([100, 700, 3000, 6000, 9000, 12000, 30000, 9000]):
+        n = 150_000 if q < 7 else 40_000
+        lo = int(r.integers(-5000, 5000))
+        kv = lo + r.integers(0, card, n)
+        kv[r.random(n) < 0.005] = np.iinfo(np.int64).min
+        qs.append({"g": (kv, with_na(r, n, 0.01)), "x": r.normal(size=n), "t": float(r.uniform(-1.5, 0.5))})
+    order = gpu.VGPU_ORDER_KEY if temp_dir else gpu.VGPU_ORDER_FIRST_SEEN
+    pred = gpu.cmp_const_prog(0, gpu.VEC_DOUBLE, "GT", 0.0)
+    b = gpu.Batch(gpu.VEC_INT64, [gpu.VEC_DOUBLE], [("sum", 0, 0), ("n", -1, 0), ("mean", 0, 0)], order, pred, 1, 1)
+    cols, queries = [], []
+    for qd in qs:
+        x, k = gpu.DeviceColumn(qd["x"]), gpu.DeviceColumn(*qd["g"])
+        cols += [x, k]
+        queries.append({"key": k, "inputs": [x], "pred_cols": [x], "n_rows": len(qd["x"]), "param": qd["t"]})
+    b.plan(queries)
+    for rep in range(2):
+        if rep:
+            b.run()
+        for q, qd in enumerate(qs):
+            t = gpu.Table({"x": qd["x"], "ax": np.abs(qd["x"]), "g": qd["g"]}, rg_rows=0)
+            node = gpu.filter_node("oracle", t.scan(), ("cmp", ">", ("col", "x"), ("dbl", qd["t"])))
+            exp = gpu.collect(gpu.group_agg_node("oracle", node, ["g"], aggs + [("__sa", "sum", "ax", 0)], temp_dir))
+            sa = exp.pop("__sa")[0]
+            got = b.fetch(q)
+            got = {"g": got["key"], "s": got["a0"], "n": got["a1"], "m": got["a2"]}
+            assert_tables_equal(got, exp, {"s": sa, "m": sa / np.maximum(exp["n"][0], 1)})
+    b.free()
+    for c in cols:
+        c.free()

@@ -1340,7 +1340,14 @@ extern "C" int vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_q
         // (a load of up to 0.8 when the tuned load does not fit)
         if ((size_t)(S + 2) * rw + vm > budget) S = pow2_at_least(hashed + hashed / 4 + 1);
         if (L.key_none) S = 64;
-        if (Q.dense) S = Q.cap;   // the LDS table is dense too: one record per key of the range
+        if (Q.dense) {
+            // the LDS table is dense too: one record per key of the range, or
+            // the records of the range's first keys when at least half of the
+            // range fits (rows of the other keys update the HBM table)
+            S = Q.cap;
+            const int64_t fit = (int64_t)((budget - std::min(budget, vm)) / rw) - 2;
+            if (S > fit && 2 * fit >= S) S = fit;
+        }
         Q.lds_slots = 0;
         int64_t tile = shared ? (int64_t)BT_STILE * 64 : b->tile;   // rows one LDS table sees
         if (big_table(S)) tile = tile_big;
