@@ -16,6 +16,8 @@
  *   AggSpec, GroupAggNode ............................. src/group_agg.h:7-23
  *   JoinKind, JoinKey, JoinNode prefix ................ src/join.h:6-61
  *   FilterNode ........................................ src/filter.h:7-11
+ *   Vtr1Header, Vtr1ColStat, Vtr1RowGroup, Vtr1File ... src/vtr1.h:7-37
+ *   ScanNode .......................................... src/scan.h:9-16
  *
  * Ownership rules are the reference's: every buffer reachable from a
  * VecBatch returned by next_batch() is malloc/calloc memory that the
@@ -27,6 +29,7 @@
 
 #include <stdint.h>
 #include <stddef.h>
+#include <stdio.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -235,6 +238,62 @@ GroupAggNode *group_agg_node_create(VecNode *child,
 JoinNode *join_node_create(VecNode *left, VecNode *right,
                            JoinKind kind, int n_keys, JoinKey *keys,
                            const char *suffix_x, const char *suffix_y);
+
+/* ---- .vtr files and the scan node ------------------------------------
+ * The GPU nodes read a scan child's open file directly (row-group offsets,
+ * schema, per-column statistics, the pushed-down predicate), so these
+ * layouts are the reference's. */
+
+/* src/vtr1.h:7-12 */
+typedef struct {
+    uint16_t  version;
+    VecSchema schema;
+    uint32_t  n_rowgroups;
+} Vtr1Header;
+
+/* src/vtr1.h:14-22 -- per column and row group (version 3 files) */
+typedef struct {
+    uint8_t has_stats;    /* 0: string column, or no valid value */
+    union {
+        struct { int64_t min, max; } i64;
+        struct { double min, max; } dbl;
+        struct { uint8_t min, max; } bln;   /* some valid FALSE / some valid TRUE */
+    };
+} Vtr1ColStat;
+
+/* src/vtr1.h:24-29 */
+typedef struct {
+    int64_t      file_offset;   /* of the row group's n_rows field */
+    int64_t      n_rows;
+    Vtr1ColStat *col_stats;     /* n_cols entries; NULL before version 3 */
+} Vtr1RowGroup;
+
+/* src/vtr1.h:31-36 */
+typedef struct {
+    FILE         *fp;
+    Vtr1Header    header;
+    Vtr1RowGroup *rowgroups;
+} Vtr1File;
+
+/* src/scan.h:9-16 -- the optimizer sets predicate (borrowed from a
+ * FilterNode above, pred_borrowed = 1) for row-group pruning */
+typedef struct {
+    VecNode    base;
+    Vtr1File  *file;
+    int       *col_mask;      /* file columns the scan reads */
+    uint32_t   next_rg;
+    VecExpr   *predicate;
+    int        pred_borrowed;
+} ScanNode;
+
+/* src/vtr1.h:39-55, src/scan.h:18-21 */
+Vtr1File *vtr1_open(const char *path);
+VecBatch *vtr1_read_rowgroup(Vtr1File *file, uint32_t rg_idx, const int *col_mask);
+void      vtr1_close(Vtr1File *file);
+void      vtr1_write(const char *path, const VecBatch *batch);
+void      vtr1_write_header(FILE *fp, const VecSchema *schema, uint32_t n_rowgroups);
+void      vtr1_write_rowgroup(FILE *fp, const VecBatch *batch);
+ScanNode *scan_node_create(const char *path, int *col_indices, int n_selected);
 
 #ifdef __cplusplus
 }

@@ -60,6 +60,24 @@ int         vgpu_synchronize(void);
 void       *vgpu_host_alloc(size_t bytes);
 int         vgpu_host_free(void *ptr);
 
+/* Asynchronous uploads (no reference counterpart: the reference reads row
+ * groups into host memory, src/vtr1.c:306-392).  A copy stream of the
+ * library moves staged host buffers to the device while the library stream
+ * runs kernels on earlier ones.  Slot s (0..3) orders one set of buffers:
+ *   vgpu_upload_async(s, dst, src, n)  enqueues a copy of n bytes from
+ *       pinned src; it starts after the library-stream work marked by the
+ *       last vgpu_upload_release(s) (so dst may still be read until then);
+ *   vgpu_upload_acquire(s)   later library-stream work waits for the copies
+ *       enqueued on slot s so far;
+ *   vgpu_upload_release(s)   marks the library-stream work enqueued so far
+ *       as the last reader of slot s's device buffers;
+ *   vgpu_upload_host_wait(s) blocks until slot s's copies are done (their
+ *       pinned sources may be overwritten). */
+int         vgpu_upload_async(int slot, void *dst_device, const void *src_pinned, size_t bytes);
+int         vgpu_upload_acquire(int slot);
+int         vgpu_upload_release(int slot);
+int         vgpu_upload_host_wait(int slot);
+
 /* ---- expression programs ---------------------------------------------
  * A straight-line program in SSA form: instruction i writes register i and
  * reads registers a and b (indices < i).  Types are the VGPU_* column
@@ -371,6 +389,10 @@ void    vgpu_batch_destroy(vgpu_batch *b);
  *                         int64 / bool key range no longer than their hashed
  *                         table get a dense table instead: key k owns record
  *                         k - min, found without hashing or probing
+ *   "scan.device_feed"   (2: on; 1: off) GPU group-by nodes over a .vtr
+ *                         scan (or a filter over one) read its row groups
+ *                         into pinned buffers and copy them on the copy
+ *                         stream, the filter evaluated in the feed
  * Returns -1 for an unknown name or a value <= 0. */
 int     vgpu_tune_set(const char *name, int64_t value);
 int64_t vgpu_tune_get(const char *name);

@@ -102,3 +102,33 @@ int main(void) {
     assert lines[1].split()[:2] == ["56", "64"]
     assert lines[2].split() == ["64", "72", "80", "88", "100"]
     assert lines[3].split() == ["56", "64", "72", "76"]
+
+
+def test_vtr_struct_layouts(native, tmp_path):
+    """the .vtr handle and scan node layouts the GPU nodes read from a scan
+    child (src/vtr1.h:7-37, src/scan.h:9-16 on LP64, worked out by hand:
+    Vtr1ColStat = u8 + 16-byte union at 8; Vtr1Header = u16, VecSchema at 8
+    (32 bytes), u32 at 40; Vtr1File = FILE *, header at 8, rowgroups at 56;
+    ScanNode = VecNode (56), file, col_mask, next_rg at 72, predicate at 80,
+    pred_borrowed at 88)"""
+    src = tmp_path / "v.c"
+    src.write_text(r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "vectra_abi.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu\n", sizeof(Vtr1ColStat), offsetof(Vtr1ColStat, i64.min), sizeof(Vtr1RowGroup),
+         offsetof(Vtr1RowGroup, col_stats), sizeof(Vtr1Header));
+  printf("%zu %zu %zu %zu\n", offsetof(Vtr1Header, schema), offsetof(Vtr1Header, n_rowgroups),
+         offsetof(Vtr1File, header), offsetof(Vtr1File, rowgroups));
+  printf("%zu %zu %zu %zu %zu %zu\n", offsetof(ScanNode, file), offsetof(ScanNode, col_mask),
+         offsetof(ScanNode, next_rg), offsetof(ScanNode, predicate), offsetof(ScanNode, pred_borrowed),
+         sizeof(ScanNode));
+  return 0;
+}''')
+    exe = tmp_path / "v"
+    subprocess.run(["gcc", "-std=c11", f"-I{ROOT / 'include'}", str(src), "-o", str(exe)], check=True)
+    lines = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    assert lines[0].split() == ["24", "8", "24", "16", "48"]
+    assert lines[1].split() == ["8", "40", "8", "56"]
+    assert lines[2].split() == ["56", "64", "72", "80", "88", "96"]

@@ -38,7 +38,7 @@ C_FLAGS = ["-std=c11", "-O2", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-paramete
            f"-I{INCLUDE}", f"-I{HOST}"]
 
 HIP_SOURCES = ["vgpu_runtime.hip", "vgpu_filter.hip", "vgpu_groupby.hip", "vgpu_join.hip", "vgpu_batch.hip"]
-NODE_SOURCES = ["gpu_common.c", "filter_node.c", "group_agg_node.c", "join_node.c", "serve.c"]
+NODE_SOURCES = ["gpu_common.c", "devscan.c", "filter_node.c", "group_agg_node.c", "join_node.c", "serve.c"]
 
 
 def _run(cmd: list[str], verbose: bool) -> None:
@@ -71,7 +71,7 @@ def build(verbose: bool = False) -> dict[str, Path]:
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *map(str, objs), "-o", str(libvgpu)], verbose)
 
     libcore = LIB / "libvx_core.so"
-    _run([CC, *C_FLAGS, "-shared", "-Wl,-soname,libvx_core.so", str(HOST / "vx_core.c"), "-o", str(libcore)],
+    _run([CC, *C_FLAGS, "-shared", "-Wl,-soname,libvx_core.so", str(HOST / "vx_core.c"), str(HOST / "vx_vtr.c"), "-o", str(libcore)],
          verbose)
 
     libnodes = LIB / "libvectra_gpu.so"

@@ -197,6 +197,59 @@ extern "C" int vgpu_host_free(void *p) {
     return 0;
 }
 
+// ---- asynchronous uploads ---------------------------------------------
+// A second stream (default priority, like the library stream) carries the
+// host -> device copies of staged inputs; per slot, one event marks the
+// slot's copies done and one marks the library-stream work that last read
+// the slot's device buffers.
+#define VG_UPLOAD_SLOTS 4
+static hipStream_t g_copy = nullptr;
+static hipEvent_t g_up_done[VG_UPLOAD_SLOTS], g_up_read[VG_UPLOAD_SLOTS];
+static bool g_up_read_set[VG_UPLOAD_SLOTS];
+
+static int vg_upload_slot(int slot) {
+    if (vg_ensure_init()) return -1;
+    if (slot < 0 || slot >= VG_UPLOAD_SLOTS) {
+        vg_set_error("upload slot %d out of range (0..%d)", slot, VG_UPLOAD_SLOTS - 1);
+        return -1;
+    }
+    if (!g_copy) {
+        VG_TRY(hipStreamCreateWithFlags(&g_copy, hipStreamNonBlocking));
+        for (int s = 0; s < VG_UPLOAD_SLOTS; s++) {
+            VG_TRY(hipEventCreateWithFlags(&g_up_done[s], hipEventDisableTiming));
+            VG_TRY(hipEventCreateWithFlags(&g_up_read[s], hipEventDisableTiming));
+        }
+    }
+    return 0;
+}
+
+extern "C" int vgpu_upload_async(int slot, void *dst, const void *src, size_t bytes) {
+    if (vg_upload_slot(slot)) return -1;
+    if (g_up_read_set[slot]) VG_TRY(hipStreamWaitEvent(g_copy, g_up_read[slot], 0));
+    if (bytes) VG_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_copy));
+    VG_TRY(hipEventRecord(g_up_done[slot], g_copy));
+    return 0;
+}
+
+extern "C" int vgpu_upload_acquire(int slot) {
+    if (vg_upload_slot(slot)) return -1;
+    VG_TRY(hipStreamWaitEvent(g_stream, g_up_done[slot], 0));
+    return 0;
+}
+
+extern "C" int vgpu_upload_release(int slot) {
+    if (vg_upload_slot(slot)) return -1;
+    VG_TRY(hipEventRecord(g_up_read[slot], g_stream));
+    g_up_read_set[slot] = true;
+    return 0;
+}
+
+extern "C" int vgpu_upload_host_wait(int slot) {
+    if (vg_upload_slot(slot)) return -1;
+    VG_TRY(hipEventSynchronize(g_up_done[slot]));
+    return 0;
+}
+
 extern "C" int vgpu_timer_start(void) {
     if (vg_ensure_init()) return -1;
     VG_TRY(hipEventRecord(g_t0, g_stream));
@@ -219,8 +272,8 @@ static const char *g_tune_names[VG_TUNE_COUNT] = {"gb.lds.max_blocks", "gb.lds.r
                                                   "gb.star.items", "gb.part.min_groups", "batch.shared_scan",
                                                   "gb.lds.slots_per_group", "batch.bucketed",
                                                   "gb.star.dense", "join.bloom",
-                                                  "gb.lds.copies", "gb.narrow", "batch.dense"};
-#define VG_TUNE_DEFAULTS {1024, 8192, 16384, 1024, 4096, 65536, 4, 65536, 1, 2, 2, 2, 2, 2, 2, 2}
+                                                  "gb.lds.copies", "gb.narrow", "batch.dense", "scan.device_feed"};
+#define VG_TUNE_DEFAULTS {1024, 8192, 16384, 1024, 4096, 65536, 4, 65536, 1, 2, 2, 2, 2, 2, 2, 2, 2}
 static int64_t g_tune[VG_TUNE_COUNT] = VG_TUNE_DEFAULTS;
 
 struct TuneBucket {

@@ -13,7 +13,7 @@
  * Every child batch is copied to the device and fed to one vgpu_groupby;
  * the finished table comes back as malloc'd columns.
  */
-#include "gpu_common.h"
+#include "devscan.h"
 
 #include <stdlib.h>
 #include <string.h>
@@ -22,7 +22,80 @@ typedef struct {
     DevCol key;
     DevCol in[VGPU_MAX_INPUTS];
     DevMem sel;
+    VxDevScan ds;      /* open while a .vtr child is fed straight to the device */
+    int ds_open;
 } GroupGpu;
+
+/* file column of a scan's k-th output column (its columns in file order) */
+static int scan_file_col(const ScanNode *sn, int k) {
+    const int n = sn->file->header.schema.n_cols;
+    for (int c = 0; c < n; c++)
+        if (sn->col_mask[c] && k-- == 0) return c;
+    return -1;
+}
+
+/* the index of file column c in fcols[0..*n), appended when absent */
+static int fed_slot(int *fcols, int *n, int c) {
+    for (int j = 0; j < *n; j++)
+        if (fcols[j] == c) return j;
+    if (*n >= VX_DS_COLS) vectra_error("GPU scan feed: more than %d columns", VX_DS_COLS);
+    fcols[*n] = c;
+    return (*n)++;
+}
+
+/* A child that is a .vtr scan, or a filter over one, is fed from the file
+ * straight to the device (knob "scan.device_feed"): row groups pruned by
+ * the scan's pushed-down predicate as the scan prunes them, the filter's
+ * predicate evaluated inside the group-by feed instead of as a selection
+ * vector that comes back to the host.  Same groups, order and values as
+ * pulling the child's batches: positions count the rows of the groups fed,
+ * which orders first occurrences as the filtered batches would. */
+static int feed_from_scan(GroupAggNode *ga, vgpu_groupby *gb, int key_idx, int n_in, const int *in_col) {
+    VecNode *c = ga->child;
+    FilterNode *fn = NULL;
+    if (c->kind && strcmp(c->kind, "FilterNode") == 0) {
+        fn = (FilterNode *)c;
+        c = fn->child;
+    }
+    if (!c->kind || strcmp(c->kind, "ScanNode") != 0 || vgpu_tune_get("scan.device_feed") <= 1) return 0;
+    ScanNode *sn = (ScanNode *)c;
+    GroupGpu *g = (GroupGpu *)ga->gpu;
+    int fcols[VX_DS_COLS], nf = 0, kslot = -1, islot[VGPU_MAX_INPUTS], pslot[VGPU_MAX_COLS];
+    if (key_idx >= 0) kslot = fed_slot(fcols, &nf, scan_file_col(sn, key_idx));
+    for (int s = 0; s < n_in; s++) islot[s] = fed_slot(fcols, &nf, scan_file_col(sn, in_col[s]));
+    CompiledExpr cx;
+    memset(&cx, 0, sizeof cx);
+    if (fn) {
+        /* compiled against the scan's schema (names and types only) */
+        const VecSchema *os = &sn->base.output_schema;
+        VecBatch *shape = vec_batch_alloc(os->n_cols, 0);
+        for (int k = 0; k < os->n_cols; k++) {
+            shape->columns[k] = vec_array_alloc(os->col_types[k], 0);
+            shape->col_names[k] = vx_strdup(os->col_names[k]);
+        }
+        const int rc = vx_try_compile_expr(fn->predicate, shape, &cx);
+        vec_batch_free(shape);
+        if (rc) vectra_error("%s", vx_last_error());
+        if (cx.result_type != VEC_BOOL) vectra_error("filter predicate must evaluate to boolean");
+        for (int p = 0; p < cx.n_cols; p++) pslot[p] = fed_slot(fcols, &nf, scan_file_col(sn, cx.col_idx[p]));
+    }
+    vx_devscan_open(&g->ds, sn, fcols, nf);
+    g->ds_open = 1;
+    vgpu_col cols[VX_DS_COLS], ins[VGPU_MAX_INPUTS], pcols[VGPU_MAX_COLS];
+    int64_t n, pos0;
+    while (vx_devscan_next(&g->ds, cols, &n, &pos0)) {
+        for (int s = 0; s < n_in; s++) ins[s] = cols[islot[s]];
+        for (int p = 0; p < cx.n_cols; p++) pcols[p] = cols[pslot[p]];
+        if (vgpu_groupby_feed(gb, kslot >= 0 ? &cols[kslot] : NULL, ins, n, NULL, 0, pos0, fn ? &cx.prog : NULL,
+                              fn ? pcols : NULL, fn ? cx.n_cols : 0) < 0) {
+            vgpu_groupby_destroy(gb);
+            vectra_error("GPU: %s", vgpu_last_error());
+        }
+    }
+    vx_devscan_close(&g->ds);
+    g->ds_open = 0;
+    return 1;
+}
 
 static VecBatch *gpu_group_run(GroupAggNode *ga) {
     GroupGpu *g = (GroupGpu *)ga->gpu;
@@ -74,7 +147,8 @@ static VecBatch *gpu_group_run(GroupAggNode *ga) {
 
     int64_t pos = 0;
     VecBatch *b;
-    while ((b = ga->child->next_batch(ga->child)) != NULL) {
+    const int from_scan = feed_from_scan(ga, gb, key_idx, n_in, in_col);
+    while (!from_scan && (b = ga->child->next_batch(ga->child)) != NULL) {
         int64_t nl = vec_batch_logical_rows(b);
         if (nl > 0) {
             vgpu_col key, ins[VGPU_MAX_INPUTS];
@@ -172,6 +246,7 @@ static void gpu_group_free(VecNode *self) {
         devcol_free(&g->key);
         for (int i = 0; i < VGPU_MAX_INPUTS; i++) devcol_free(&g->in[i]);
         devmem_free(&g->sel);
+        if (g->ds_open) vx_devscan_close(&g->ds);
         free(g);
     }
     vec_schema_free(&ga->base.output_schema);

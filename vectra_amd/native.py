@@ -211,6 +211,10 @@ def core() -> C.CDLL:
                                               C.c_char_p, C.POINTER(NODE_P)])
         _proto(lib, "vx_try_join", C.c_int, [C.c_void_p, NODE_P, NODE_P, C.c_int, C.c_int, C.c_void_p, C.c_char_p,
                                              C.c_char_p, C.POINTER(NODE_P)])
+        _proto(lib, "vx_try_scan", C.c_int, [C.c_char_p, C.POINTER(C.c_int), C.c_int, C.POINTER(NODE_P)])
+        _proto(lib, "vx_try_vtr_write", C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_char_p), C.POINTER(VecArray),
+                                                  C.c_int64, C.c_int64])
+        _proto(lib, "vx_pushdown_predicates", None, [NODE_P])
         _libs["core"] = lib
     return _libs["core"]
 
@@ -442,6 +446,39 @@ class Table:
         node = core().vx_mem_scan_create(len(names), cnames, arr, self.n, self.rg_rows)
         node._keep = self   # the scan borrows these buffers for its lifetime
         return node
+
+
+def vtr_write(path: str | os.PathLike, cols: dict[str, Any], rg_rows: int = 0) -> None:
+    """Writes named host columns as a version-3 .vtr file of row groups of
+    rg_rows rows (src/vtr1.c:39-135 layout, statistics per row group)."""
+    t = Table(cols)
+    names = list(t.cols)
+    arr = (VecArray * max(1, len(names)))()
+    for i, k in enumerate(names):
+        arr[i] = t.cols[k].as_vecarray()
+    cnames = (C.c_char_p * max(1, len(names)))(*[k.encode() for k in names])
+    if core().vx_try_vtr_write(str(path).encode(), len(names), cnames, arr, t.n, rg_rows) != 0:
+        raise VectraError(core().vx_last_error().decode())
+
+
+def scan_vtr(path: str | os.PathLike, col_indices: Sequence[int] | None = None) -> Any:
+    """scan_node_create over a .vtr file: all columns, or the given file
+    column indices (output in file order)."""
+    out = NODE_P()
+    if col_indices is None:
+        rc = core().vx_try_scan(str(path).encode(), None, 0, C.byref(out))
+    else:
+        idx = (C.c_int * max(1, len(col_indices)))(*col_indices)
+        rc = core().vx_try_scan(str(path).encode(), idx, len(col_indices), C.byref(out))
+    if rc != 0:
+        raise VectraError(core().vx_last_error().decode())
+    return out
+
+
+def pushdown(node: Any) -> Any:
+    """The optimizer's predicate pushdown onto a .vtr scan below a filter."""
+    core().vx_pushdown_predicates(node)
+    return node
 
 
 # --------------------------------------------------------------------------
