@@ -289,6 +289,98 @@ def c3_data(n, card, seed=0x7665637472610300):
     return {"key": key, "v1": v1, "v2": v2}, present
 
 
+def run_c1_vtr(args, nv):
+    """The C1 query read from a .vtr file: filter(x > 0) |> group_by(key) |>
+    summarise(total = sum(value), n = n(), avg = mean(value)) through the
+    drop-in nodes over scan_node_create, the filter's predicate pushed onto
+    the scan.  With the device scan feed (knob scan.device_feed) the GPU
+    group-by reads each row group into pinned staging and copies it on the
+    copy stream while the previous group's kernels run; without it the
+    nodes pull host batches (malloc'd by the scan, uploaded by the filter,
+    selection vector back to the host, uploaded again by the group-by).
+    The file is written first and read from the page cache, so the rates
+    include PCIe and the file reads but not a disk."""
+    import ctypes as C
+    n, rg = args.vtr_rows, args.vtr_rg
+    path = Path(args.vtr_path)
+    r = np.random.default_rng(0x7665637472610101)
+    cols = {"id": np.arange(n, dtype=np.int64), "key": r.integers(0, 1000, n), "x": r.normal(size=n),
+            "value": r.uniform(0, 100, n)}
+    t0 = time.perf_counter()
+    nv.vtr_write(path, cols, rg_rows=rg)
+    log(f"[bench] c1-vtr: wrote {n:,} rows in row groups of {rg:,} to {path} "
+        f"({path.stat().st_size / 1e9:.2f} GB) in {time.perf_counter() - t0:.1f} s")
+    aggs = [("total", "sum", "value", 0), ("n", "n", None, 0), ("avg", "mean", "value", 0)]
+    pred = ("cmp", ">", ("col", "x"), ("dbl", 0.0))
+
+    def query(engine):
+        node = nv.pushdown(nv.filter_node(engine, nv.scan_vtr(path), pred))
+        return nv.collect(nv.group_agg_node(engine, node, ["key"], aggs, "/tmp"))
+
+    lib = nv.vgpu()
+
+    def timed(feed):
+        old = lib.vgpu_tune_get(b"scan.device_feed")
+        lib.vgpu_tune_set(b"scan.device_feed", feed)
+        try:
+            for _ in range(args.warmup):
+                res = query("gpu")
+            lib.vgpu_synchronize()
+            t1 = time.perf_counter()
+            for _ in range(args.steps):
+                query("gpu")
+            lib.vgpu_synchronize()
+            return res, (time.perf_counter() - t1) * 1e3 / args.steps
+        finally:
+            lib.vgpu_tune_set(b"scan.device_feed", old)
+
+    res_on, ms_on = timed(2)
+    res_off, ms_off = timed(1)
+    keep = cols["x"] > 0
+    for res in (res_on, res_off):
+        assert int(res["n"][0].sum()) == int(keep.sum()) and list(res["key"][0]) == list(range(1000))
+    # kernels of one device-fed query
+    lib.vgpu_kernel_timing(1)
+    lib.vgpu_kernel_time_ms(None, 1)
+    query("gpu")
+    nl = C.c_int64()
+    kms = lib.vgpu_kernel_time_ms(C.byref(nl), 1)
+    lib.vgpu_kernel_timing(0)
+    alg = n * 24 + 1000 * 4 * 8       # key, x, value read once; 1,000 x 4 outputs
+    log(f"[bench] c1-vtr: device feed {ms_on:.1f} ms/query ({n / ms_on / 1e3:.0f} M rows/s), host batches "
+        f"{ms_off:.1f} ms/query ({n / ms_off / 1e3:.0f} M rows/s); kernels of one query {kms:.1f} ms")
+    cpu = None
+    if not args.no_cpu:
+        nv.load_oracle(ROOT / "oracle" / "_build" / "liboracle.so")
+        t1 = time.perf_counter()
+        exp = query("oracle")
+        secs = time.perf_counter() - t1
+        assert list(exp["key"][0]) == list(res_on["key"][0]) and np.array_equal(exp["n"][0], res_on["n"][0])
+        assert np.allclose(exp["total"][0], res_on["total"][0], rtol=1e-12, atol=0)
+        cpu = {"value": n / secs, "unit": "rows/s", "cores": 1, "kind": "port",
+               "sample": f"the full query over the same file ({n:,} rows), oracle nodes over the CPU scan "
+                         f"({secs:.1f} s)"}
+    path.unlink()
+    return {
+        "metric": METRIC, "value": n / (ms_on / 1e3), "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_on, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded numpy generators), written to a .vtr file",
+        "config": {"workload": f"filter(x > 0) |> group_by(key) |> summarise(total = sum(value), n = n(), "
+                               f"avg = mean(value)) over a .vtr file of {n:,} rows in row groups of {rg:,} "
+                               f"(id, key U{{0..999}}, x N(0,1), value U(0,100)); drop-in ScanNode -> FilterNode "
+                               f"-> GroupAggNode, predicate pushed onto the scan; file in the page cache, rates "
+                               f"include the file reads and PCIe",
+                   "rows": n, "row_groups": (n + rg - 1) // rg},
+        "host_batches": {"ms_per_query": ms_off, "rows_per_s": n / (ms_off / 1e3)},
+        "device_feed_vs_host_batches": ms_off / ms_on,
+        "roofline": {"bound": "hbm", "achieved": alg / (kms / 1e3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": alg / (kms / 1e3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "all kernels of one device-fed query", "bytes_per_launch": alg / max(nl.value, 1),
+                     "avg_launch_ms": kms / max(nl.value, 1)},
+        "cpu_baseline": cpu,
+    }
+
+
 def run_c3(args, nv, card):
     n = args.c3_rows
     log(f"[bench] generating c3 data: {n:,} rows, {card:,} key values")
@@ -657,7 +749,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3-high", "c3-low", "c4", "c5"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c1-vtr", "c2", "c3-high", "c3-low", "c4", "c5"])
     ap.add_argument("--c4-queries", type=int, default=4096)
     ap.add_argument("--c4-card", type=int, default=0, help="fix every query's key cardinality (0: log-uniform)")
     ap.add_argument("--c5-rows", type=int, default=1_000_000_000)
@@ -668,6 +760,9 @@ def main():
     ap.add_argument("--c5-plain-rates", type=float, nargs="*", default=[10, 25, 40, 55, 70, 85, 100])
     ap.add_argument("--c5-batched-rates", type=float, nargs="*", default=[50, 100, 200, 350, 500, 650, 800, 1000])
     ap.add_argument("--c3-rows", type=int, default=200_000_000)
+    ap.add_argument("--vtr-rows", type=int, default=50_000_000)
+    ap.add_argument("--vtr-rg", type=int, default=1_000_000)
+    ap.add_argument("--vtr-path", default="/tmp/vectra_bench_c1.vtr")
     ap.add_argument("--fact-rows", type=int, default=50_000_000)
     ap.add_argument("--dim-rows", type=int, default=1_000_000)
     ap.add_argument("--cpu-sample", type=int, default=10_000_000)
@@ -702,6 +797,8 @@ def main():
         log(f"[bench] knob {name} = {val}")
     if args.workload == "c1":
         out = run_c1(args, nv)
+    elif args.workload == "c1-vtr":
+        out = run_c1_vtr(args, nv)
     elif args.workload == "c3-high":
         out = run_c3(args, nv, 50_000_000)
     elif args.workload == "c3-low":

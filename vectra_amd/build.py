@@ -76,7 +76,7 @@ def build(verbose: bool = False) -> dict[str, Path]:
 
     libnodes = LIB / "libvectra_gpu.so"
     _run([CC, *C_FLAGS, "-shared", *[str(HOST / s) for s in NODE_SOURCES],
-          f"-L{LIB}", "-lvgpu", "-lvx_core", "-Wl,-rpath,$ORIGIN", "-o", str(libnodes)], verbose)
+          f"-L{LIB}", "-lvgpu", "-lvx_core", "-lpthread", "-Wl,-rpath,$ORIGIN", "-o", str(libnodes)], verbose)
     return {"vgpu": libvgpu, "core": libcore, "nodes": libnodes}
 
 
