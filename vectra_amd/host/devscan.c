@@ -12,6 +12,7 @@
  */
 #define _GNU_SOURCE
 #include "devscan.h"
+#include "vx_zonemap.h"
 
 #include <pthread.h>
 #include <stdio.h>

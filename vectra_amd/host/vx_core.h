@@ -93,11 +93,7 @@ int vx_try_group(vx_group_ctor fn, VecNode *child, int n_keys, char **key_names,
 int vx_try_join(vx_join_ctor fn, VecNode *left, VecNode *right, JoinKind kind, int n_keys,
                 JoinKey *keys, const char *sx, const char *sy, VecNode **out);
 
-/* ---- .vtr zone maps (src/scan.c:10-117) ------------------------------
- * 0 when the row group whose statistics are `stats` cannot hold a row
- * satisfying e (a `col op literal` comparison, or an AND / OR of them),
- * else 1. */
-int vx_vtr_might_match(const VecExpr *e, const Vtr1ColStat *stats, const VecSchema *schema);
+/* ---- .vtr harness entry points -------------------------------------- */
 /* the optimizer's pushdown of a filter's predicate onto the .vtr scan
  * below it (src/optimize.c:353-369), along a chain of filters */
 void vx_pushdown_predicates(VecNode *node);
