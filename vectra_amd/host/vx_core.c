@@ -304,7 +304,9 @@ VecNode *vx_mem_scan_create(int n_cols, const char *const *names,
     m->rg_rows = rg_rows;
     m->base.next_batch = mem_scan_next;
     m->base.free_node = mem_scan_free;
-    m->base.kind = "ScanNode";
+    /* not "ScanNode": code that meets that kind reads the .vtr scan's
+     * struct (file handle, pushed-down predicate) */
+    m->base.kind = "MemScanNode";
     return &m->base;
 }
 
