@@ -43,6 +43,43 @@ static int fed_slot(int *fcols, int *n, int c) {
     return (*n)++;
 }
 
+/* the .vtr scan under a child that is one, or a filter over one */
+static ScanNode *scan_below(VecNode *c) {
+    if (c->kind && strcmp(c->kind, "FilterNode") == 0) c = ((FilterNode *)c)->child;
+    return c->kind && strcmp(c->kind, "ScanNode") == 0 ? (ScanNode *)c : NULL;
+}
+
+/* A bound on the groups of an int64 / bool key read from a .vtr scan: its
+ * value range over every row group's statistics, plus the NA group (0 when
+ * the file has no statistics or the range is no tighter than the rows).
+ * With it the group table is sized once and trusted, as for a caller's
+ * group count, instead of reading its occupancy back before every batch. */
+static int64_t scan_group_bound(const ScanNode *sn, int key_out_col) {
+    const Vtr1File *f = sn->file;
+    const VecSchema *fs = &f->header.schema;
+    int c = -1;
+    for (int i = 0, k = key_out_col; i < fs->n_cols; i++)
+        if (sn->col_mask[i] && k-- == 0) c = i;
+    if (c < 0 || f->header.version < 3) return 0;
+    if (fs->col_types[c] == VEC_BOOL) return 3;
+    if (fs->col_types[c] != VEC_INT64) return 0;
+    int64_t rows = 0, lo = 0, hi = 0;
+    int any = 0;
+    for (uint32_t g = 0; g < f->header.n_rowgroups; g++) {
+        const Vtr1RowGroup *rg = &f->rowgroups[g];
+        rows += rg->n_rows;
+        if (!rg->col_stats || !rg->col_stats[c].has_stats) continue;
+        const int64_t a = rg->col_stats[c].i64.min, b = rg->col_stats[c].i64.max;
+        if (!any || a < lo) lo = a;
+        if (!any || b > hi) hi = b;
+        any = 1;
+    }
+    if (!any) return rows > 0 ? 1 : 0;
+    const uint64_t span = (uint64_t)hi - (uint64_t)lo;
+    if (span >= (uint64_t)rows) return 0;
+    return (int64_t)span + 2;
+}
+
 /* A child that is a .vtr scan, or a filter over one, is fed from the file
  * straight to the device (knob "scan.device_feed"): row groups pruned by
  * the scan's pushed-down predicate as the scan prunes them, the filter's
@@ -141,8 +178,10 @@ static VecBatch *gpu_group_run(GroupAggNode *ga) {
         aggs[a].input = slot;
     }
     int order = ga->use_sorted ? VGPU_ORDER_KEY : VGPU_ORDER_FIRST_SEEN;
+    const ScanNode *below = scan_below(ga->child);
+    const int64_t hint = below && key_idx >= 0 ? scan_group_bound(below, key_idx) : 0;
     vgpu_groupby *gb = vgpu_groupby_create(ga->n_keys ? (int)key_type : VGPU_KEY_NONE, n_in, in_type,
-                                           ga->n_aggs, aggs, order, 0);
+                                           ga->n_aggs, aggs, order, hint);
     if (!gb) vectra_error("GPU: %s", vgpu_last_error());
 
     int64_t pos = 0;
