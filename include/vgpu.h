@@ -145,8 +145,13 @@ typedef struct {
     int32_t kind;     /* VGPU_AGG_* */
     int32_t input;    /* input slot, -1 for COUNT_STAR */
     int32_t na_rm;
-    int32_t reserved;
+    int32_t reserved; /* 0, or VGPU_AGG_RAW */
 } vgpu_agg;
+
+/* vgpu_agg.reserved flag: MIN / MAX of an int64 input deliver the exact
+ * value's 64 bits in the double output slot (reinterpret them as int64)
+ * instead of its conversion to double */
+#define VGPU_AGG_RAW 1
 
 #define VGPU_MAX_INPUTS 8
 #define VGPU_MAX_AGGS   16
@@ -170,6 +175,16 @@ int vgpu_groupby_feed(vgpu_groupby *g, const vgpu_col *key,
                       const int32_t *sel, int64_t n_sel, int64_t pos_base,
                       const vgpu_prog *pred, const vgpu_col *pred_cols,
                       int n_pred_cols);
+
+/* One 64-bit code per physical row of the tuple of n_keys (1..8) key
+ * columns (int64 / double / bool, device), into codes (device, n_rows):
+ * equal tuples (NA equal to NA, -0 to +0) get equal codes; a tuple with a
+ * NaN component gets a code drawn from its position pos_base + row, as it
+ * forms a group of its own (src/hash.c:48-78, NaN equals nothing).  Codes
+ * of different tuples collide with probability about 2^-64 per pair: a
+ * multi-key group-by over codes checks its groups (src/group_agg.c:155-281
+ * with n_keys > 1; the node carries min / max / count of every key). */
+int vgpu_key_codes(const vgpu_col *keys, int n_keys, int64_t n_rows, int64_t pos_base, int64_t *codes);
 
 /* Finishes on the device and orders the groups; returns the group count. */
 int64_t vgpu_groupby_finish(vgpu_groupby *g);

@@ -1840,6 +1840,64 @@ static int gb_feed_narrow(vgpu_groupby *g, const FeedArgs &F, const vgpu_col *in
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// multi-column keys: one 64-bit code per row of the tuple
+// ---------------------------------------------------------------------------
+
+#define VG_MAX_KEY_COLS 8
+
+struct KeyCodeArgs {
+    vgpu_col k[VG_MAX_KEY_COLS];
+    int32_t n_keys;
+    int64_t n_rows, pos_base;
+    int64_t *codes;
+};
+
+// Each component's class and normalized bits (NA, -0 as +0) are mixed in
+// order; a tuple with a NaN component is a group of its own (NaN equals
+// nothing), so its code is drawn from its row position instead.
+__global__ void k_key_codes(KeyCodeArgs A) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < A.n_rows; r += stride) {
+        uint64_t h = 0x6a09e667f3bcc909ull;
+        bool nan = false;
+        for (int j = 0; j < A.n_keys; j++) {
+            uint64_t kn;
+            const int cls = vg_key_class(A.k[j], r, kn);
+            nan |= cls == VG_KEY_NAN;
+            const uint64_t comp = cls == VG_KEY_NA ? 0x243f6a8885a308d3ull : kn;
+            h = vg_mix64(h ^ vg_mix64(comp + (uint64_t)(cls == VG_KEY_NA) * 0x13198a2e03707344ull + (uint64_t)j));
+        }
+        if (nan) h = vg_mix64(0xa4093822299f31d0ull ^ (uint64_t)(A.pos_base + r)) ^ 0x082efa98ec4e6c89ull;
+        A.codes[r] = (int64_t)h;
+    }
+}
+
+extern "C" int vgpu_key_codes(const vgpu_col *keys, int n_keys, int64_t n_rows, int64_t pos_base, int64_t *codes) {
+    if (vg_ensure_init()) return -1;
+    if (n_keys < 1 || n_keys > VG_MAX_KEY_COLS) {
+        vg_set_error("key codes: %d key columns (1..%d)", n_keys, VG_MAX_KEY_COLS);
+        return -1;
+    }
+    if (n_rows <= 0) return 0;
+    KeyCodeArgs A;
+    memset(&A, 0, sizeof A);
+    for (int j = 0; j < n_keys; j++) {
+        if (keys[j].type != VGPU_INT64 && keys[j].type != VGPU_DOUBLE && keys[j].type != VGPU_BOOL) {
+            vg_set_error("key codes: key %d has unsupported type %d", j, keys[j].type);
+            return -1;
+        }
+        A.k[j] = keys[j];
+    }
+    A.n_keys = n_keys;
+    A.n_rows = n_rows;
+    A.pos_base = pos_base;
+    A.codes = codes;
+    k_key_codes<<<vg_blocks(n_rows, 256, 4096), 256, 0, vg_stream()>>>(A);
+    VG_LAUNCH_CHECK();
+    return 0;
+}
+
 extern "C" int vgpu_groupby_feed(vgpu_groupby *g, const vgpu_col *key, const vgpu_col *inputs, int64_t n_rows,
                                  const int32_t *sel, int64_t n_sel, int64_t pos_base, const vgpu_prog *pred,
                                  const vgpu_col *pred_cols, int n_pred_cols) {

@@ -506,7 +506,8 @@ __device__ __forceinline__ void gb_finish_agg(const GbLayout &L, const vgpu_agg 
         if (bits == 0 && c[L.c_zneg[i]] < c[L.c_zpos[i]]) bits = 0x8000000000000000ull;
         out = vg_bits_to_f64(bits);
     } else {
-        out = (double)(int64_t)(o ^ 0x8000000000000000ull);
+        const int64_t v = (int64_t)(o ^ 0x8000000000000000ull);
+        out = (ag.reserved & VGPU_AGG_RAW) ? vg_bits_to_f64((uint64_t)v) : (double)v;
     }
 }
 

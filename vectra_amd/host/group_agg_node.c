@@ -22,6 +22,7 @@ typedef struct {
     DevCol key;
     DevCol in[VGPU_MAX_INPUTS];
     DevMem sel;
+    DevMem codes;      /* multi-column keys: one tuple code per row */
     VxDevScan ds;      /* open while a .vtr child is fed straight to the device */
     int ds_open;
 } GroupGpu;
@@ -134,11 +135,271 @@ static int feed_from_scan(GroupAggNode *ga, vgpu_groupby *gb, int key_idx, int n
     return 1;
 }
 
+/* ---- several grouping columns ----------------------------------------
+ * The groups of a key tuple (src/group_agg.c:155-281 with n_keys > 1;
+ * equality per column as src/hash.c:48-78: NA equals NA, -0 equals +0, NaN
+ * equals nothing) are the groups of a 64-bit code of the tuple
+ * (vgpu_key_codes), grouped in first-occurrence order like one int64 key.
+ * Every key column also feeds hidden aggregates -- min and max (exact
+ * int64 bits) or, for bool, the sum, and the valid count -- which give the
+ * group's key values back and prove that no two different tuples shared a
+ * code: within a group each column must be all NA or all valid with
+ * min == max (sum 0 or all TRUE). In key order (temp_dir set) the groups
+ * are then sorted stably by the tuple, each column ascending with NA last
+ * (NaN after the numbers, in first-occurrence order). */
+#define MK_MAX_KEYS 8
+
+typedef struct {
+    int       m;
+    VecType   type[MK_MAX_KEYS];
+    const double  *v[MK_MAX_KEYS];     /* key values (int64 as raw bits) */
+    const uint8_t *ok[MK_MAX_KEYS];    /* 1 = valid */
+} TupleCmp;
+
+/* rank within a column: number, NaN, NA */
+static int tuple_rank(const TupleCmp *t, int j, int64_t g) {
+    if (!t->ok[j][g]) return 2;
+    return t->type[j] == VEC_DOUBLE && t->v[j][g] != t->v[j][g] ? 1 : 0;
+}
+
+static int tuple_cmp(const TupleCmp *t, int64_t a, int64_t b) {
+    for (int j = 0; j < t->m; j++) {
+        const int ra = tuple_rank(t, j, a), rb = tuple_rank(t, j, b);
+        if (ra != rb) return ra < rb ? -1 : 1;
+        if (ra) continue;
+        if (t->type[j] == VEC_INT64) {
+            int64_t x, y;
+            memcpy(&x, &t->v[j][a], 8);
+            memcpy(&y, &t->v[j][b], 8);
+            if (x != y) return x < y ? -1 : 1;
+        } else if (t->v[j][a] != t->v[j][b]) {
+            return t->v[j][a] < t->v[j][b] ? -1 : 1;
+        }
+    }
+    return 0;
+}
+
+/* stable merge sort of idx[0..n) by tuple */
+static void tuple_sort(const TupleCmp *t, int64_t *idx, int64_t *tmp, int64_t n) {
+    for (int64_t w = 1; w < n; w *= 2) {
+        for (int64_t lo = 0; lo < n; lo += 2 * w) {
+            const int64_t mid = lo + w < n ? lo + w : n, hi = lo + 2 * w < n ? lo + 2 * w : n;
+            int64_t i = lo, j = mid, k = lo;
+            while (i < mid && j < hi) tmp[k++] = tuple_cmp(t, idx[j], idx[i]) < 0 ? idx[j++] : idx[i++];
+            while (i < mid) tmp[k++] = idx[i++];
+            while (j < hi) tmp[k++] = idx[j++];
+        }
+        memcpy(idx, tmp, (size_t)n * sizeof *idx);
+    }
+}
+
+static int slot_for(const VecSchema *cs, int ci, int *in_col, int32_t *in_type, int *n_in) {
+    for (int s = 0; s < *n_in; s++)
+        if (in_col[s] == ci) return s;
+    if (*n_in >= VGPU_MAX_INPUTS) vectra_error("GPU summarise: more than %d input columns", VGPU_MAX_INPUTS);
+    in_col[*n_in] = ci;
+    in_type[*n_in] = (int32_t)cs->col_types[ci];
+    return (*n_in)++;
+}
+
+static VecBatch *gpu_group_run_multi(GroupAggNode *ga) {
+    GroupGpu *g = (GroupGpu *)ga->gpu;
+    const VecSchema *cs = &ga->child->output_schema;
+    const int m = ga->n_keys;
+    if (m > MK_MAX_KEYS) vectra_error("GPU group_by: %d grouping columns exceed %d", m, MK_MAX_KEYS);
+    int kcol[MK_MAX_KEYS], kslot[MK_MAX_KEYS];
+    VecType ktype[MK_MAX_KEYS];
+    for (int j = 0; j < m; j++) {
+        kcol[j] = vec_schema_find_col(cs, ga->key_names[j]);
+        if (kcol[j] < 0) vectra_error("group_by: column not found: %s", ga->key_names[j]);
+        ktype[j] = cs->col_types[kcol[j]];
+        if (ktype[j] == VEC_STRING)
+            vectra_error("GPU group_by: string key '%s' is not grouped on the GPU", ga->key_names[j]);
+    }
+    int n_in = 0, in_col[VGPU_MAX_INPUTS];
+    int32_t in_type[VGPU_MAX_INPUTS];
+    vgpu_agg aggs[VGPU_MAX_AGGS];
+    const int hidden = 1 + 3 * m;
+    if (ga->n_aggs + hidden > VGPU_MAX_AGGS)
+        vectra_error("GPU summarise: %d aggregates over %d grouping columns exceed the GPU path's %d (each "
+                     "grouping column takes 3, plus one)", ga->n_aggs, m, VGPU_MAX_AGGS);
+    for (int a = 0; a < ga->n_aggs; a++) {
+        const AggSpec *sp = &ga->agg_specs[a];
+        aggs[a].kind = (int32_t)sp->kind;
+        aggs[a].na_rm = sp->na_rm;
+        aggs[a].reserved = 0;
+        aggs[a].input = -1;
+        if (sp->kind == AGG_COUNT_STAR) continue;
+        const int ci = vec_schema_find_col(cs, sp->input_col);
+        if (ci < 0) vectra_error("summarise: column not found: %s", sp->input_col);
+        if (cs->col_types[ci] == VEC_STRING)
+            vectra_error("GPU summarise: string column '%s' is not aggregated on the GPU", sp->input_col);
+        aggs[a].input = slot_for(cs, ci, in_col, in_type, &n_in);
+    }
+    for (int j = 0; j < m; j++) kslot[j] = slot_for(cs, kcol[j], in_col, in_type, &n_in);
+    /* hidden aggregates: n, then per key column (min | sum), (max | -), count */
+    int na = ga->n_aggs;
+    const int a_n = na;
+    aggs[na++] = (vgpu_agg){VGPU_AGG_COUNT_STAR, -1, 0, 0};
+    int a_lo[MK_MAX_KEYS], a_hi[MK_MAX_KEYS], a_cnt[MK_MAX_KEYS];
+    for (int j = 0; j < m; j++) {
+        if (ktype[j] == VEC_BOOL) {
+            a_lo[j] = na;
+            aggs[na++] = (vgpu_agg){VGPU_AGG_SUM, kslot[j], 0, 0};
+            a_hi[j] = -1;
+        } else {
+            a_lo[j] = na;
+            aggs[na++] = (vgpu_agg){VGPU_AGG_MIN, kslot[j], 0, VGPU_AGG_RAW};
+            a_hi[j] = na;
+            aggs[na++] = (vgpu_agg){VGPU_AGG_MAX, kslot[j], 0, VGPU_AGG_RAW};
+        }
+        a_cnt[j] = na;
+        aggs[na++] = (vgpu_agg){VGPU_AGG_COUNT, kslot[j], 0, 0};
+    }
+    vgpu_groupby *gb = vgpu_groupby_create(VGPU_INT64, n_in, in_type, na, aggs, VGPU_ORDER_FIRST_SEEN, 0);
+    if (!gb) vectra_error("GPU: %s", vgpu_last_error());
+
+    int64_t pos = 0, phys = 0;
+    VecBatch *b;
+    while ((b = ga->child->next_batch(ga->child)) != NULL) {
+        const int64_t nl = vec_batch_logical_rows(b);
+        if (nl > 0) {
+            vgpu_col ins[VGPU_MAX_INPUTS], keys[MK_MAX_KEYS];
+            for (int s2 = 0; s2 < n_in; s2++) {
+                if (b->columns[in_col[s2]].type != (VecType)in_type[s2]) {
+                    vgpu_groupby_destroy(gb);
+                    vec_batch_free(b);
+                    vectra_error("summarise: column type changed between batches");
+                }
+                devcol_upload(&g->in[s2], &b->columns[in_col[s2]], b->n_rows);
+                ins[s2] = g->in[s2].col;
+            }
+            for (int j = 0; j < m; j++) keys[j] = ins[kslot[j]];
+            int64_t *codes = (int64_t *)devmem_reserve(&g->codes, (size_t)b->n_rows * 8);
+            vgpu_col key = {VGPU_INT64, 0, codes, NULL};
+            const int32_t *d_sel = NULL;
+            if (b->sel) {
+                d_sel = (const int32_t *)devmem_reserve(&g->sel, (size_t)b->sel_n * 4);
+                VX_GPU(vgpu_memcpy_h2d((void *)d_sel, b->sel, (size_t)b->sel_n * 4));
+            }
+            /* codes of NaN tuples come from physical positions, unique over batches */
+            if (vgpu_key_codes(keys, m, b->n_rows, phys, codes) < 0 ||
+                vgpu_groupby_feed(gb, &key, ins, b->n_rows, d_sel, b->sel ? b->sel_n : 0, pos, NULL, NULL, 0) < 0) {
+                vgpu_groupby_destroy(gb);
+                vec_batch_free(b);
+                vectra_error("GPU: %s", vgpu_last_error());
+            }
+            VX_GPU(vgpu_synchronize());
+        }
+        pos += nl;
+        phys += b->n_rows;
+        vec_batch_free(b);
+    }
+    const int64_t ng = vgpu_groupby_finish(gb);
+    if (ng < 0) {
+        vgpu_groupby_destroy(gb);
+        vectra_error("GPU: %s", vgpu_last_error());
+    }
+    const size_t cap = (size_t)(ng > 0 ? ng : 1);
+    double **vals = (double **)calloc((size_t)na, sizeof(double *));
+    uint8_t **valid = (uint8_t **)calloc((size_t)na, sizeof(uint8_t *));
+    uint64_t *codes_h = (uint64_t *)malloc(cap * 8);
+    if (!vals || !valid || !codes_h) vectra_error("alloc failed");
+    for (int a = 0; a < na; a++) {
+        vals[a] = (double *)malloc(cap * 8);
+        valid[a] = (uint8_t *)calloc((cap + 7) / 8, 1);
+        if (!vals[a] || !valid[a]) vectra_error("alloc failed");
+    }
+    const int64_t warns = vgpu_groupby_fetch(gb, codes_h, NULL, vals, valid);
+    vgpu_groupby_destroy(gb);
+    free(codes_h);
+    if (warns < 0) vectra_error("GPU: %s", vgpu_last_error());
+    /* the key values, and the check that every group holds one tuple */
+    TupleCmp t;
+    t.m = m;
+    uint8_t *kok[MK_MAX_KEYS];
+    double *kv[MK_MAX_KEYS];
+    int collided = 0;
+    for (int j = 0; j < m; j++) {
+        t.type[j] = ktype[j];
+        kv[j] = (double *)malloc(cap * 8);
+        kok[j] = (uint8_t *)malloc(cap);
+        if (!kv[j] || !kok[j]) vectra_error("alloc failed");
+        for (int64_t r = 0; r < ng; r++) {
+            const double n = vals[a_n][r], c = vals[a_cnt[j]][r];
+            kok[j][r] = c > 0;
+            if (c != 0 && c != n) collided = 1;
+            if (c == 0) {
+                kv[j][r] = 0;
+                continue;
+            }
+            const double lo = vals[a_lo[j]][r];
+            if (ktype[j] == VEC_BOOL) {
+                if (lo != 0 && lo != c) collided = 1;
+                kv[j][r] = lo > 0 ? 1.0 : 0.0;
+            } else if (ktype[j] == VEC_INT64) {
+                if (memcmp(&lo, &vals[a_hi[j]][r], 8) != 0) collided = 1;
+                kv[j][r] = lo;
+            } else {
+                if (n > 1 && !(lo == vals[a_hi[j]][r])) collided = 1;
+                kv[j][r] = lo;
+            }
+        }
+        t.v[j] = kv[j];
+        t.ok[j] = kok[j];
+    }
+    if (collided) {
+        for (int a = 0; a < na; a++) { free(vals[a]); free(valid[a]); }
+        for (int j = 0; j < m; j++) { free(kv[j]); free(kok[j]); }
+        free(vals);
+        free(valid);
+        vectra_error("GPU group_by: two different key tuples share a 64-bit code (probability about 2^-64 "
+                     "per pair of groups)");
+    }
+    int64_t *order = (int64_t *)malloc(cap * sizeof(int64_t));
+    int64_t *tmp = (int64_t *)malloc(cap * sizeof(int64_t));
+    if (!order || !tmp) vectra_error("alloc failed");
+    for (int64_t r = 0; r < ng; r++) order[r] = r;
+    if (ga->use_sorted) tuple_sort(&t, order, tmp, ng);
+    free(tmp);
+    VecBatch *res = vec_batch_alloc(m + ga->n_aggs, ng);
+    for (int j = 0; j < m; j++) {
+        VecArray a = vec_array_alloc(ktype[j], ng);
+        for (int64_t r = 0; r < ng; r++) {
+            const int64_t s2 = order[r];
+            if (!kok[j][s2]) continue;
+            vec_array_set_valid(&a, r);
+            if (ktype[j] == VEC_BOOL) a.buf.bln[r] = kv[j][s2] > 0;
+            else if (ktype[j] == VEC_INT64) memcpy(&a.buf.i64[r], &kv[j][s2], 8);
+            else a.buf.dbl[r] = kv[j][s2];
+        }
+        res->columns[j] = a;
+        res->col_names[j] = vx_strdup(ga->key_names[j]);
+    }
+    for (int a = 0; a < ga->n_aggs; a++) {
+        VecArray o = vec_array_alloc(VEC_DOUBLE, ng);
+        for (int64_t r = 0; r < ng; r++) {
+            const int64_t s2 = order[r];
+            o.buf.dbl[r] = vals[a][s2];
+            if ((valid[a][s2 >> 3] >> (s2 & 7)) & 1) vec_array_set_valid(&o, r);
+        }
+        res->columns[m + a] = o;
+        res->col_names[m + a] = vx_strdup(ga->agg_specs[a].output_name);
+    }
+    free(order);
+    for (int a = 0; a < na; a++) { free(vals[a]); free(valid[a]); }
+    for (int j = 0; j < m; j++) { free(kv[j]); free(kok[j]); }
+    free(vals);
+    free(valid);
+    for (int64_t w = 0; w < warns; w++)
+        vectra_warning("no non-missing arguments to min/max; returning Inf/-Inf");
+    return res;
+}
+
 static VecBatch *gpu_group_run(GroupAggNode *ga) {
     GroupGpu *g = (GroupGpu *)ga->gpu;
     const VecSchema *cs = &ga->child->output_schema;
-    if (ga->n_keys > 1)
-        vectra_error("GPU group_by: %d grouping columns; the GPU path groups by one key", ga->n_keys);
+    if (ga->n_keys > 1) return gpu_group_run_multi(ga);
     int key_idx = -1;
     VecType key_type = VEC_INT64;
     if (ga->n_keys == 1) {
@@ -285,6 +546,7 @@ static void gpu_group_free(VecNode *self) {
         devcol_free(&g->key);
         for (int i = 0; i < VGPU_MAX_INPUTS; i++) devcol_free(&g->in[i]);
         devmem_free(&g->sel);
+        devmem_free(&g->codes);
         if (g->ds_open) vx_devscan_close(&g->ds);
         free(g);
     }
