@@ -184,7 +184,11 @@ int vgpu_groupby_feed(vgpu_groupby *g, const vgpu_col *key,
  * of different tuples collide with probability about 2^-64 per pair: a
  * multi-key group-by over codes checks its groups (src/group_agg.c:155-281
  * with n_keys > 1; the node carries min / max / count of every key). */
-int vgpu_key_codes(const vgpu_col *keys, int n_keys, int64_t n_rows, int64_t pos_base, int64_t *codes);
+int vgpu_key_codes(const vgpu_col *keys, int n_keys, int64_t n_rows, int64_t pos_base, int64_t *codes,
+                   uint8_t *valid);
+/* (valid: optional device bitmap of (n_rows + 63) / 64 * 8 bytes, 8-byte
+ * aligned; bit r set when no component of row r is NA or NaN -- the rows a
+ * join may match, src/join.c:77-113) */
 
 /* Finishes on the device and orders the groups; returns the group count. */
 int64_t vgpu_groupby_finish(vgpu_groupby *g);

@@ -1851,29 +1851,38 @@ struct KeyCodeArgs {
     int32_t n_keys;
     int64_t n_rows, pos_base;
     int64_t *codes;
+    uint64_t *valid;   // optional: bit r = no component of row r is NA or NaN
 };
 
 // Each component's class and normalized bits (NA, -0 as +0) are mixed in
 // order; a tuple with a NaN component is a group of its own (NaN equals
 // nothing), so its code is drawn from its row position instead.
-__global__ void k_key_codes(KeyCodeArgs A) {
+// Rows are taken 64 to a wave in order (block size and grid stride are
+// multiples of 64), so a wave's ballot is one whole word of the bitmap.
+__global__ void __launch_bounds__(256) k_key_codes(KeyCodeArgs A) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < A.n_rows; r += stride) {
         uint64_t h = 0x6a09e667f3bcc909ull;
-        bool nan = false;
+        bool nan = false, na = false;
         for (int j = 0; j < A.n_keys; j++) {
             uint64_t kn;
             const int cls = vg_key_class(A.k[j], r, kn);
             nan |= cls == VG_KEY_NAN;
+            na |= cls == VG_KEY_NA;
             const uint64_t comp = cls == VG_KEY_NA ? 0x243f6a8885a308d3ull : kn;
             h = vg_mix64(h ^ vg_mix64(comp + (uint64_t)(cls == VG_KEY_NA) * 0x13198a2e03707344ull + (uint64_t)j));
         }
         if (nan) h = vg_mix64(0xa4093822299f31d0ull ^ (uint64_t)(A.pos_base + r)) ^ 0x082efa98ec4e6c89ull;
         A.codes[r] = (int64_t)h;
+        if (A.valid) {
+            const uint64_t m = vg_ballot(!nan && !na);
+            if (vg_lane() == 0) A.valid[r >> 6] = m;
+        }
     }
 }
 
-extern "C" int vgpu_key_codes(const vgpu_col *keys, int n_keys, int64_t n_rows, int64_t pos_base, int64_t *codes) {
+extern "C" int vgpu_key_codes(const vgpu_col *keys, int n_keys, int64_t n_rows, int64_t pos_base, int64_t *codes,
+                              uint8_t *valid) {
     if (vg_ensure_init()) return -1;
     if (n_keys < 1 || n_keys > VG_MAX_KEY_COLS) {
         vg_set_error("key codes: %d key columns (1..%d)", n_keys, VG_MAX_KEY_COLS);
@@ -1893,6 +1902,11 @@ extern "C" int vgpu_key_codes(const vgpu_col *keys, int n_keys, int64_t n_rows, 
     A.n_rows = n_rows;
     A.pos_base = pos_base;
     A.codes = codes;
+    A.valid = reinterpret_cast<uint64_t *>(valid);
+    if (valid && (reinterpret_cast<uintptr_t>(valid) & 7)) {
+        vg_set_error("key codes: the validity bitmap must be 8-byte aligned");
+        return -1;
+    }
     k_key_codes<<<vg_blocks(n_rows, 256, 4096), 256, 0, vg_stream()>>>(A);
     VG_LAUNCH_CHECK();
     return 0;

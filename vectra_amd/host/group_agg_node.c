@@ -283,7 +283,7 @@ static VecBatch *gpu_group_run_multi(GroupAggNode *ga) {
                 VX_GPU(vgpu_memcpy_h2d((void *)d_sel, b->sel, (size_t)b->sel_n * 4));
             }
             /* codes of NaN tuples come from physical positions, unique over batches */
-            if (vgpu_key_codes(keys, m, b->n_rows, phys, codes) < 0 ||
+            if (vgpu_key_codes(keys, m, b->n_rows, phys, codes, NULL) < 0 ||
                 vgpu_groupby_feed(gb, &key, ins, b->n_rows, d_sel, b->sel ? b->sel_n : 0, pos, NULL, NULL, 0) < 0) {
                 vgpu_groupby_destroy(gb);
                 vec_batch_free(b);

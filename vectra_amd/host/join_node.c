@@ -20,6 +20,7 @@
 #include <string.h>
 
 #define FINALIZE_ROWS 65536   /* src/join.c:487 */
+#define JN_MAX_KEYS 8
 
 typedef struct {
     int       built;
@@ -39,12 +40,80 @@ typedef struct {
     DevMem    out_data, out_valid;
     int64_t  *h_build;        /* FULL finalize: unmatched build rows */
     int64_t   n_unmatched, fin_cursor;
+    /* several key columns: the table is keyed by tuple codes */
+    int       multi;
+    VecType   common[JN_MAX_KEYS];
+    DevCol    pcast[JN_MAX_KEYS];
+    DevMem    bcode, bvalid, pcode, pvalid;
 } JoinGpu;
+
+/* Several key columns (src/join.c:67-113 hashes and compares the tuple):
+ * the table is built on a 64-bit code of each build row's tuple, every
+ * column coerced to the pair's common type, rows with an NA or NaN
+ * component left out (they never match); probes use the probe tuples'
+ * codes, and every matched pair's tuples are compared afterwards, so a code
+ * shared by two different tuples (about 2^-64 per pair) is an error, never
+ * a wrong match. */
+static void multi_build_side(JoinNode *jn) {
+    JoinGpu *g = (JoinGpu *)jn->gpu;
+    const int m = jn->n_keys;
+    const VecSchema *rs = &jn->right->output_schema;
+    const VecSchema *ls = &jn->left->output_schema;
+    if (m > JN_MAX_KEYS) vectra_error("GPU join: %d key columns exceed %d", m, JN_MAX_KEYS);
+    g->multi = 1;
+    g->r_ncols = rs->n_cols;
+    g->n_build = vx_concat_batches(jn->right, &g->r_host);
+    g->first_only = 0;
+    for (int k = 0; k < m; k++) {
+        const VecType lt = ls->col_types[jn->keys[k].left_col], rt = rs->col_types[jn->keys[k].right_col];
+        if (lt == VEC_STRING || rt == VEC_STRING) vectra_error("GPU join: string keys are not joined on the GPU");
+        g->common[k] = vx_common_type(lt, rt);
+        /* A left / full join continues its chain walk comparing every probe
+         * key in its original type with the coerced build key's bytes
+         * (src/join.c:398-400, 413-415): with several keys which later
+         * matches survive depends on every column's bytes, which this path
+         * does not reproduce, so it does not take the join. */
+        if ((jn->kind == JOIN_LEFT || jn->kind == JOIN_FULL) && lt != g->common[k])
+            vectra_error("GPU join: left / full join on %d key columns with the probe key '%s' coerced to %s is "
+                         "not evaluated on the GPU", m, ls->col_names[jn->keys[k].left_col],
+                         vx_type_name(g->common[k]));
+    }
+    g->r_dev = (DevCol *)calloc((size_t)(g->r_ncols > 0 ? g->r_ncols : 1), sizeof(DevCol));
+    for (int c = 0; c < g->r_ncols; c++) {
+        if (g->r_host[c].type == VEC_STRING) continue;
+        devcol_upload(&g->r_dev[c], &g->r_host[c], g->n_build);
+    }
+    vgpu_col keys[JN_MAX_KEYS];
+    const size_t nb = (size_t)(g->n_build > 0 ? g->n_build : 1);
+    for (int k = 0; k < m; k++) {
+        const int rk = jn->keys[k].right_col;
+        DevCol *dc = &g->r_dev[rk];
+        if ((VecType)dc->col.type != g->common[k]) {
+            void *dst = vgpu_malloc(nb * 8);
+            if (!dst) vectra_error("GPU: %s", vgpu_last_error());
+            VX_GPU(vgpu_cast(&dc->col, g->n_build, (int)g->common[k], dst));
+            vgpu_free(dc->d_data);
+            dc->d_data = dst;
+            dc->cap_data = nb * 8;
+            dc->col.data = dst;
+            dc->col.type = (int32_t)g->common[k];
+        }
+        keys[k] = dc->col;
+    }
+    int64_t *codes = (int64_t *)devmem_reserve(&g->bcode, nb * 8);
+    uint8_t *valid = (uint8_t *)devmem_reserve(&g->bvalid, (nb + 63) / 64 * 8);
+    VX_GPU(vgpu_key_codes(keys, m, g->n_build, 0, codes, valid));
+    vgpu_col bkey = {VGPU_INT64, 0, codes, valid};
+    g->jt = vgpu_join_build(&bkey, g->n_build);
+    if (!g->jt) vectra_error("GPU: %s", vgpu_last_error());
+}
 
 static void join_build_side(JoinNode *jn) {
     JoinGpu *g = (JoinGpu *)jn->gpu;
-    if (jn->n_keys != 1)
-        vectra_error("GPU join: %d key columns; the GPU path joins on one key", jn->n_keys);
+    if (jn->n_keys > 1) {
+        multi_build_side(jn);
+        return;
+    }
     const VecSchema *rs = &jn->right->output_schema;
     const VecSchema *ls = &jn->left->output_schema;
     g->r_ncols = rs->n_cols;
@@ -133,6 +202,44 @@ static VecBatch *make_output(JoinNode *jn, int64_t n) {
     return res;
 }
 
+/* one key value of a host column in the pair's common type, as a double
+ * and as an int64 (a double's bits for double columns) */
+static void key_value(const VecArray *a, int64_t r, VecType common, double *d, int64_t *i) {
+    if (common == VEC_DOUBLE) {
+        *d = a->type == VEC_DOUBLE ? a->buf.dbl[r] : a->type == VEC_INT64 ? (double)a->buf.i64[r] : (double)a->buf.bln[r];
+        *i = 0;
+    } else {
+        *i = a->type == VEC_INT64 ? a->buf.i64[r] : (int64_t)a->buf.bln[r];
+        *d = 0;
+    }
+}
+
+static VecBatch *probe_batch_multi(JoinNode *jn, VecBatch *pb);
+
+/* every matched pair (build row >= 0) joins equal tuples */
+static void verify_pairs(JoinNode *jn, const VecBatch *pb, const int32_t *d_prow, const int64_t *d_brow, int64_t n) {
+    JoinGpu *g = (JoinGpu *)jn->gpu;
+    int32_t *pr = (int32_t *)malloc((size_t)(n > 0 ? n : 1) * 4);
+    int64_t *br = (int64_t *)malloc((size_t)(n > 0 ? n : 1) * 8);
+    if (!pr || !br) vectra_error("alloc failed");
+    VX_GPU(vgpu_memcpy_d2h(pr, d_prow, (size_t)n * 4));
+    VX_GPU(vgpu_memcpy_d2h(br, d_brow, (size_t)n * 8));
+    int bad = 0;
+    for (int64_t i = 0; i < n && !bad; i++) {
+        if (br[i] < 0) continue;
+        for (int k = 0; k < jn->n_keys; k++) {
+            double pd, bd;
+            int64_t pi, bi;
+            key_value(&pb->columns[jn->keys[k].left_col], pr[i], g->common[k], &pd, &pi);
+            key_value(&g->r_host[jn->keys[k].right_col], br[i], g->common[k], &bd, &bi);
+            if (g->common[k] == VEC_DOUBLE ? !(pd == bd) : pi != bi) bad = 1;
+        }
+    }
+    free(pr);
+    free(br);
+    if (bad) vectra_error("GPU join: two different key tuples share a 64-bit code (probability about 2^-64 per pair)");
+}
+
 static VecBatch *probe_batch(JoinNode *jn, VecBatch *pb) {
     JoinGpu *g = (JoinGpu *)jn->gpu;
     int lnc = pb->n_cols;
@@ -140,6 +247,7 @@ static VecBatch *probe_batch(JoinNode *jn, VecBatch *pb) {
     int lk = jn->keys[0].left_col;
     int64_t nl = vec_batch_logical_rows(pb);
     if (nl == 0) return NULL;   /* empty probe batch: no output (src/join.c:451-459) */
+    if (g->multi) return probe_batch_multi(jn, pb);
     /* probe key, coerced to the build key type (src/join.c:268-289) */
     devcol_upload(&g->l_dev[lk], &pb->columns[lk], pb->n_rows);
     vgpu_col pkey = g->l_dev[lk].col;
@@ -203,6 +311,115 @@ static VecBatch *probe_batch(JoinNode *jn, VecBatch *pb) {
     return res;
 }
 
+/* gathers the left columns through d_prow and, unless semi / anti, the
+ * right non-key columns through d_brow (n output rows) */
+static VecBatch *gather_output(JoinNode *jn, VecBatch *pb, const int32_t *d_prow, const int64_t *d_brow, int64_t n) {
+    JoinGpu *g = (JoinGpu *)jn->gpu;
+    const int lnc = pb->n_cols;
+    int64_t *h_p = NULL, *h_b = NULL;
+    int need_host = 0;
+    for (int c = 0; c < lnc; c++) need_host |= pb->columns[c].type == VEC_STRING;
+    if (jn->kind != JOIN_SEMI && jn->kind != JOIN_ANTI)
+        for (int j = 0; j < g->n_r_nonkey; j++) need_host |= g->r_host[g->r_nonkey[j]].type == VEC_STRING;
+    if (need_host) {
+        int32_t *tmp = (int32_t *)malloc((size_t)n * 4);
+        h_p = (int64_t *)malloc((size_t)n * 8);
+        h_b = (int64_t *)malloc((size_t)n * 8);
+        VX_GPU(vgpu_memcpy_d2h(tmp, d_prow, (size_t)n * 4));
+        if (d_brow) VX_GPU(vgpu_memcpy_d2h(h_b, d_brow, (size_t)n * 8));
+        for (int64_t i = 0; i < n; i++) h_p[i] = tmp[i];
+        free(tmp);
+    }
+    VecBatch *res = make_output(jn, n);
+    for (int c = 0; c < lnc; c++) {
+        const VecArray *src = &pb->columns[c];
+        if (src->type == VEC_STRING) {
+            res->columns[c] = gather_strings(src, h_p, n);
+            continue;
+        }
+        devcol_upload(&g->l_dev[c], src, pb->n_rows);
+        res->columns[c] = gather_device(g, &g->l_dev[c].col, src->type, d_prow, 0, n);
+    }
+    if (jn->kind != JOIN_SEMI && jn->kind != JOIN_ANTI) {
+        for (int j = 0; j < g->n_r_nonkey; j++) {
+            const int rc = g->r_nonkey[j];
+            if (g->r_host[rc].type == VEC_STRING) {
+                res->columns[lnc + j] = gather_strings(&g->r_host[rc], h_b, n);
+                continue;
+            }
+            res->columns[lnc + j] = gather_device(g, &g->r_dev[rc].col, g->r_host[rc].type, d_brow, 1, n);
+        }
+    }
+    free(h_p);
+    free(h_b);
+    return res;
+}
+
+/* Several key columns: the probe tuples' codes against the build codes.
+ * Semi and anti joins are derived from an inner probe, whose pairs can be
+ * checked: semi keeps each probe row with a pair once, anti the logical
+ * rows without one, both in probe order (src/join.c:371-384). */
+static VecBatch *probe_batch_multi(JoinNode *jn, VecBatch *pb) {
+    JoinGpu *g = (JoinGpu *)jn->gpu;
+    const int m = jn->n_keys;
+    vgpu_col keys[JN_MAX_KEYS];
+    for (int k = 0; k < m; k++) {
+        const int lk = jn->keys[k].left_col;
+        devcol_upload(&g->l_dev[lk], &pb->columns[lk], pb->n_rows);
+        keys[k] = g->l_dev[lk].col;
+        if ((VecType)keys[k].type != g->common[k]) {
+            devcol_reserve(&g->pcast[k], (size_t)(pb->n_rows > 0 ? pb->n_rows : 1) * 8, 8);
+            VX_GPU(vgpu_cast(&keys[k], pb->n_rows, (int)g->common[k], g->pcast[k].d_data));
+            keys[k].data = g->pcast[k].d_data;
+            keys[k].type = (int32_t)g->common[k];
+        }
+    }
+    const size_t np = (size_t)(pb->n_rows > 0 ? pb->n_rows : 1);
+    int64_t *codes = (int64_t *)devmem_reserve(&g->pcode, np * 8);
+    uint8_t *valid = (uint8_t *)devmem_reserve(&g->pvalid, (np + 63) / 64 * 8);
+    VX_GPU(vgpu_key_codes(keys, m, pb->n_rows, 0, codes, valid));
+    vgpu_col pkey = {VGPU_INT64, 0, codes, valid};
+    const int32_t *d_sel = NULL;
+    if (pb->sel) {
+        d_sel = (const int32_t *)devmem_reserve(&g->sel, (size_t)(pb->sel_n > 0 ? pb->sel_n : 1) * 4);
+        VX_GPU(vgpu_memcpy_h2d((void *)d_sel, pb->sel, (size_t)pb->sel_n * 4));
+    }
+    const int semi_anti = jn->kind == JOIN_SEMI || jn->kind == JOIN_ANTI;
+    int64_t n_out = 0;
+    VX_GPU(vgpu_join_probe(g->jt, semi_anti ? (int)JOIN_INNER : (int)jn->kind, &pkey, pb->n_rows, d_sel,
+                           pb->sel ? pb->sel_n : 0, g->first_only, &n_out));
+    const int32_t *d_prow = NULL;
+    const int64_t *d_brow = NULL;
+    if (n_out > 0) {
+        vgpu_join_pairs(g->jt, &d_prow, &d_brow);
+        verify_pairs(jn, pb, d_prow, d_brow, n_out);
+    }
+    if (!semi_anti) return n_out > 0 ? gather_output(jn, pb, d_prow, d_brow, n_out) : NULL;
+    /* probe rows with a pair, in probe order */
+    const int64_t nl = vec_batch_logical_rows(pb);
+    int32_t *pr = (int32_t *)malloc((size_t)(n_out > 0 ? n_out : 1) * 4);
+    int32_t *rows = (int32_t *)malloc((size_t)(nl > 0 ? nl : 1) * 4);
+    uint8_t *hit = (uint8_t *)calloc((size_t)(pb->n_rows > 0 ? pb->n_rows : 1), 1);
+    if (!pr || !rows || !hit) vectra_error("alloc failed");
+    if (n_out > 0) VX_GPU(vgpu_memcpy_d2h(pr, d_prow, (size_t)n_out * 4));
+    for (int64_t i = 0; i < n_out; i++) hit[pr[i]] = 1;
+    int64_t nr = 0;
+    for (int64_t l = 0; l < nl; l++) {
+        const int32_t r = pb->sel ? pb->sel[l] : (int32_t)l;
+        if (hit[r] == (jn->kind == JOIN_SEMI)) rows[nr++] = r;
+    }
+    free(pr);
+    free(hit);
+    VecBatch *res = NULL;
+    if (nr > 0) {
+        int32_t *d_rows = (int32_t *)devmem_reserve(&g->sel, (size_t)nr * 4);
+        VX_GPU(vgpu_memcpy_h2d(d_rows, rows, (size_t)nr * 4));
+        res = gather_output(jn, pb, d_rows, NULL, nr);
+    }
+    free(rows);
+    return res;
+}
+
 /* FULL: one batch of never-matched build rows (src/join.c:487-551) */
 static VecBatch *finalize_batch(JoinNode *jn) {
     JoinGpu *g = (JoinGpu *)jn->gpu;
@@ -230,7 +447,9 @@ static VecBatch *finalize_batch(JoinNode *jn) {
     int lnc = ls->n_cols;
     VecBatch *res = make_output(jn, n);
     for (int c = 0; c < lnc; c++) {
-        int rsrc = (c == jn->keys[0].left_col) ? jn->keys[0].right_col : -1;
+        int rsrc = -1;
+        for (int k = 0; k < jn->n_keys; k++)
+            if (c == jn->keys[k].left_col) rsrc = jn->keys[k].right_col;
         VecType t = ls->col_types[c];
         if (rsrc < 0) {
             res->columns[c] = vec_array_alloc(t, n);   /* all NA */
@@ -298,6 +517,11 @@ static void gpu_join_free(VecNode *self) {
         }
         for (int c = 0; c < 64; c++) devcol_free(&g->l_dev[c]);
         devcol_free(&g->pkey_cast);
+        for (int k = 0; k < JN_MAX_KEYS; k++) devcol_free(&g->pcast[k]);
+        devmem_free(&g->bcode);
+        devmem_free(&g->bvalid);
+        devmem_free(&g->pcode);
+        devmem_free(&g->pvalid);
         devmem_free(&g->sel);
         devmem_free(&g->out_data);
         devmem_free(&g->out_valid);
