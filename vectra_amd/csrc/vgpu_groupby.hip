@@ -103,8 +103,12 @@ struct vgpu_groupby {
     DevBuf gbp_s1, gbp_s2, gbp_aux, run_rec;
     GbpArgs gbp;
     int32_t run = 0;          // the records of a partitioned feed await finish
-    DevBuf out_keys, out_kvalid;
-    std::vector<DevBuf> out_vals, out_valid;
+    // the finished result in one buffer: keys, key validity, then per
+    // aggregate its values and validity (offsets in bytes), so a small
+    // result comes back in one copy
+    DevBuf out;
+    size_t off_kvalid = 0, out_bytes = 0;
+    std::vector<size_t> off_vals, off_valid;
     int64_t warn_count = 0;
 };
 
@@ -1029,8 +1033,8 @@ extern "C" vgpu_groupby *vgpu_groupby_create(int key_type, int n_inputs, const i
         delete g;
         return nullptr;
     }
-    g->out_vals.resize(n_aggs);
-    g->out_valid.resize(n_aggs);
+    g->off_vals.resize(n_aggs);
+    g->off_valid.resize(n_aggs);
     return g;
 }
 
@@ -2403,15 +2407,26 @@ extern "C" int64_t vgpu_groupby_finish(vgpu_groupby *g) {
     }
     int64_t n_out = n_sorted + na_last;
     size_t vwords = (size_t)((n_out + 63) / 64 + 1);
-    if (g->out_keys.reserve((size_t)(n_out + 1) * 8) || g->out_kvalid.reserve(vwords * 8)) return -1;
+    const size_t col_bytes = (size_t)(n_out + 1) * 8, bits_bytes = vwords * 8;
+    size_t off = col_bytes;
+    g->off_kvalid = off;
+    off += bits_bytes;
+    for (int a = 0; a < g->aggs.n; a++) {
+        g->off_vals[a] = off;
+        off += col_bytes;
+        g->off_valid[a] = off;
+        off += bits_bytes;
+    }
+    g->out_bytes = off;
+    if (g->out.reserve(off)) return -1;
+    uint8_t *ob = g->out.as<uint8_t>();
     EmitOut O;
     memset(&O, 0, sizeof O);
-    O.keys = g->out_keys.ptr;
-    O.kvalid = g->out_kvalid.as<uint64_t>();
+    O.keys = ob;
+    O.kvalid = reinterpret_cast<uint64_t *>(ob + g->off_kvalid);
     for (int a = 0; a < g->aggs.n; a++) {
-        if (g->out_vals[a].reserve((size_t)(n_out + 1) * 8) || g->out_valid[a].reserve(vwords * 8)) return -1;
-        O.vals[a] = g->out_vals[a].as<double>();
-        O.valid[a] = g->out_valid[a].as<uint64_t>();
+        O.vals[a] = reinterpret_cast<double *>(ob + g->off_vals[a]);
+        O.valid[a] = reinterpret_cast<uint64_t *>(ob + g->off_valid[a]);
     }
     if (n_out > 0) {
         const int RW = g->L.RH + g->L.RC;
@@ -2438,17 +2453,41 @@ extern "C" int64_t vgpu_groupby_fetch(vgpu_groupby *g, void *keys, uint8_t *key_
     hipStream_t s = vg_stream();
     int64_t n = g->n_groups;
     size_t vb = (size_t)(n + 7) / 8;
+    const uint8_t *ob = g->out.as<uint8_t>();
+    // bool keys are emitted as 8-byte 0/1 words; callers narrow them
+    uint8_t *dst_keys = keys && !g->L.key_none ? static_cast<uint8_t *>(keys) : nullptr;
+    // a small result: the whole buffer and the control words in two copies
+    // to pinned staging, one synchronization, then host copies (each
+    // separate copy to pageable memory costs tens of microseconds)
+    const size_t staged = g->out_bytes + sizeof(uint64_t) * GB_CTL_WORDS;
+    uint8_t *pin = staged <= VG_STAGE_MAX ? static_cast<uint8_t *>(vg_pinned_stage(staged)) : nullptr;
+    if (pin) {
+        if (n > 0) VG_TRY(hipMemcpyAsync(pin, ob, g->out_bytes, hipMemcpyDeviceToHost, s));
+        VG_TRY(hipMemcpyAsync(pin + g->out_bytes, g->ctl.ptr, sizeof(uint64_t) * GB_CTL_WORDS, hipMemcpyDeviceToHost,
+                              s));
+        VG_TRY(hipStreamSynchronize(s));
+        if (n > 0) {
+            if (dst_keys) memcpy(dst_keys, pin, (size_t)n * 8);
+            if (key_validity) memcpy(key_validity, pin + g->off_kvalid, vb);
+            for (int a = 0; a < g->aggs.n; a++) {
+                if (agg_values && agg_values[a]) memcpy(agg_values[a], pin + g->off_vals[a], (size_t)n * 8);
+                if (agg_validity && agg_validity[a]) memcpy(agg_validity[a], pin + g->off_valid[a], vb);
+            }
+        }
+        uint64_t w4;
+        memcpy(&w4, pin + g->out_bytes + 4 * sizeof(uint64_t), sizeof w4);
+        g->warn_count = (int64_t)w4;
+        return g->warn_count;
+    }
     uint64_t ctl[GB_CTL_WORDS];
     if (n > 0) {
-        // bool keys are emitted as 8-byte 0/1 words; callers narrow them
-        if (keys && !g->L.key_none)
-            VG_TRY(hipMemcpyAsync(keys, g->out_keys.ptr, (size_t)n * 8, hipMemcpyDeviceToHost, s));
-        if (key_validity) VG_TRY(hipMemcpyAsync(key_validity, g->out_kvalid.ptr, vb, hipMemcpyDeviceToHost, s));
+        if (dst_keys) VG_TRY(hipMemcpyAsync(dst_keys, ob, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+        if (key_validity) VG_TRY(hipMemcpyAsync(key_validity, ob + g->off_kvalid, vb, hipMemcpyDeviceToHost, s));
         for (int a = 0; a < g->aggs.n; a++) {
             if (agg_values && agg_values[a])
-                VG_TRY(hipMemcpyAsync(agg_values[a], g->out_vals[a].ptr, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+                VG_TRY(hipMemcpyAsync(agg_values[a], ob + g->off_vals[a], (size_t)n * 8, hipMemcpyDeviceToHost, s));
             if (agg_validity && agg_validity[a])
-                VG_TRY(hipMemcpyAsync(agg_validity[a], g->out_valid[a].ptr, vb, hipMemcpyDeviceToHost, s));
+                VG_TRY(hipMemcpyAsync(agg_validity[a], ob + g->off_valid[a], vb, hipMemcpyDeviceToHost, s));
         }
     }
     VG_TRY(hipMemcpyAsync(ctl, g->ctl.ptr, sizeof ctl, hipMemcpyDeviceToHost, s));
@@ -2463,11 +2502,12 @@ extern "C" int vgpu_groupby_result_device(vgpu_groupby *g, const void **keys, co
         vg_set_error("groupby: result before finish");
         return -1;
     }
-    if (keys) *keys = g->out_keys.ptr;
-    if (key_validity) *key_validity = g->out_kvalid.as<uint8_t>();
+    const uint8_t *ob = g->out.as<uint8_t>();
+    if (keys) *keys = ob;
+    if (key_validity) *key_validity = ob + g->off_kvalid;
     if (agg >= 0 && agg < g->aggs.n) {
-        if (values) *values = g->out_vals[agg].as<double>();
-        if (validity) *validity = g->out_valid[agg].as<uint8_t>();
+        if (values) *values = reinterpret_cast<const double *>(ob + g->off_vals[agg]);
+        if (validity) *validity = ob + g->off_valid[agg];
     }
     return 0;
 }
@@ -2487,9 +2527,6 @@ extern "C" void vgpu_groupby_destroy(vgpu_groupby *g) {
     g->gbp_s2.release();
     g->gbp_aux.release();
     g->run_rec.release();
-    g->out_keys.release();
-    g->out_kvalid.release();
-    for (auto &b : g->out_vals) b.release();
-    for (auto &b : g->out_valid) b.release();
+    g->out.release();
     delete g;
 }

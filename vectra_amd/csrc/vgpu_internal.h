@@ -61,6 +61,11 @@ int64_t vg_tune_for(int knob, int64_t groups);
 // queries allocates nothing and never takes hipFree's implicit device
 // synchronization.  vg_pool_trim() hands cached blocks back to the runtime.
 void *vg_pool_alloc(size_t bytes, size_t *granted);
+// the library's pinned host staging buffer for small device -> host results
+// (at least `bytes`, at most VG_STAGE_MAX; nullptr when larger or when the
+// allocation fails); valid until the next call, library stream only
+#define VG_STAGE_MAX ((size_t)4 << 20)
+void *vg_pinned_stage(size_t bytes);
 void vg_pool_free(void *ptr, size_t bytes);
 void vg_pool_trim();
 

@@ -184,6 +184,22 @@ extern "C" int vgpu_synchronize(void) {
     return 0;
 }
 
+static void *g_stage = nullptr;
+static size_t g_stage_cap = 0;
+
+void *vg_pinned_stage(size_t bytes) {
+    if (bytes > VG_STAGE_MAX) return nullptr;
+    if (bytes <= g_stage_cap) return g_stage;
+    size_t want = g_stage_cap ? g_stage_cap : 65536;
+    while (want < bytes) want *= 2;
+    void *p = nullptr;
+    if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) return nullptr;
+    if (g_stage) (void)hipHostFree(g_stage);   // no copy into the old buffer is pending: callers synchronize
+    g_stage = p;
+    g_stage_cap = want;
+    return g_stage;
+}
+
 extern "C" void *vgpu_host_alloc(size_t bytes) {
     if (vg_ensure_init()) return nullptr;
     void *p = nullptr;
