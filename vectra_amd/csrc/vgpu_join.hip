@@ -179,8 +179,10 @@ static vgpu_jtable *join_build_impl(const vgpu_col *key, int64_t n_build, const 
     if (hipMemsetAsync(misc.ptr, 0, 64, s) != hipSuccess) { vg_set_error("join_build: memset failed"); return fail(); }
     k_jt_init<<<vg_blocks(slots, 256, 8192), 256, 0, s>>>(t->ent.as<uint64_t>(), slots, E, empty, cnt.as<uint32_t>());
     // the filter: about 16 bits per build key, a power of two of words
+    // one 64-bit word per join.bloom_keys build keys (tuned; 4: about 16 bits a key)
+    const uint64_t kpw = (uint64_t)std::max<int64_t>(1, vg_tune(VG_TUNE_JOIN_BLOOM_KPW));
     t->bloom_words = 64;
-    while (t->bloom_words < (uint64_t)n_build / 4 && t->bloom_words < (1ull << 22)) t->bloom_words <<= 1;
+    while (t->bloom_words < (uint64_t)n_build / kpw && t->bloom_words < (1ull << 22)) t->bloom_words <<= 1;
     if (t->bloom.reserve(t->bloom_words * 8)) return fail();
     k_jt_zero<<<vg_blocks((int64_t)t->bloom_words, 256), 256, 0, s>>>(t->bloom.as<uint64_t>(),
                                                                      (int64_t)t->bloom_words);
