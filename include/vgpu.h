@@ -414,6 +414,8 @@ void    vgpu_batch_destroy(vgpu_batch *b);
  *                         stream, the filter evaluated in the feed
  *   "join.bloom_keys"    (4) build keys per 64-bit word of a join table's
  *                         filter (3 bits a key), read when a table is built
+ *   "join.load_pct"      (50) highest load of a join table's slots, in
+ *                         percent (10..95), read when a table is built
  * Returns -1 for an unknown name or a value <= 0. */
 int     vgpu_tune_set(const char *name, int64_t value);
 int64_t vgpu_tune_get(const char *name);
