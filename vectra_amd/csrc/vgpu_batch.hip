@@ -909,9 +909,9 @@ __global__ void k_bt_chain_expand(GbLayout L, BtArgs B, BtChainArgs A) {
 // host side
 // ---------------------------------------------------------------------------
 
-extern "C" vgpu_batch *vgpu_batch_create(int key_type, int n_inputs, const int32_t *input_types, int n_aggs,
-                                         const vgpu_agg *aggs, int order, const vgpu_prog *pred, int n_pred_cols,
-                                         int param_insn) {
+static vgpu_batch *batch_create_body(int key_type, int n_inputs, const int32_t *input_types, int n_aggs,
+                                     const vgpu_agg *aggs, int order, const vgpu_prog *pred, int n_pred_cols,
+                                     int param_insn) {
     if (vg_ensure_init()) return nullptr;
     if (key_type != VGPU_INT64 && key_type != VGPU_DOUBLE && key_type != VGPU_BOOL && key_type != VGPU_KEY_NONE) {
         vg_set_error("batch: unsupported key type %d", key_type);
@@ -965,7 +965,14 @@ extern "C" vgpu_batch *vgpu_batch_create(int key_type, int n_inputs, const int32
     return b;
 }
 
-extern "C" int vgpu_batch_join(vgpu_batch *b, const vgpu_jtable *t, int key_payload) {
+extern "C" vgpu_batch *vgpu_batch_create(int key_type, int n_inputs, const int32_t *input_types, int n_aggs,
+                                         const vgpu_agg *aggs, int order, const vgpu_prog *pred, int n_pred_cols,
+                                         int param_insn) {
+    return vg_guard<vgpu_batch *>(nullptr, "vgpu_batch_create", [&] { return batch_create_body(key_type, n_inputs,
+        input_types, n_aggs, aggs, order, pred, n_pred_cols, param_insn); });
+}
+
+static int batch_join_body(vgpu_batch *b, const vgpu_jtable *t, int key_payload) {
     if (b->planned) {
         vg_set_error("batch: join after plan");
         return -1;
@@ -987,6 +994,10 @@ extern "C" int vgpu_batch_join(vgpu_batch *b, const vgpu_jtable *t, int key_payl
     b->join_payload = t->payload[key_payload];
     b->join_rows = t->n_build;
     return 0;
+}
+
+extern "C" int vgpu_batch_join(vgpu_batch *b, const vgpu_jtable *t, int key_payload) {
+    return vg_guard<int>(-1, "vgpu_batch_join", [&] { return batch_join_body(b, t, key_payload); });
 }
 
 static int bt_enqueue(vgpu_batch *b, bool first);
@@ -1124,7 +1135,7 @@ static int bt_chain_upload(vgpu_batch *b) {
     return 0;
 }
 
-extern "C" int vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_queries) {
+static int batch_plan_body(vgpu_batch *b, const vgpu_query *queries, int n_queries) {
     if (b->planned) {
         vg_set_error("batch: already planned");
         return -1;
@@ -1438,6 +1449,10 @@ extern "C" int vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_q
     return 0;
 }
 
+extern "C" int vgpu_batch_plan(vgpu_batch *b, const vgpu_query *queries, int n_queries) {
+    return vg_guard<int>(-1, "vgpu_batch_plan", [&] { return batch_plan_body(b, queries, n_queries); });
+}
+
 // one run's device work; `first` (inside plan) synchronizes once to learn
 // the sort range and the number of records
 static int bt_enqueue(vgpu_batch *b, bool first) {
@@ -1615,7 +1630,7 @@ static int bt_enqueue(vgpu_batch *b, bool first) {
     return 0;
 }
 
-extern "C" int vgpu_batch_run(vgpu_batch *b) {
+static int batch_run_body(vgpu_batch *b) {
     if (!b->planned) {
         vg_set_error("batch: run before plan");
         return -1;
@@ -1628,7 +1643,11 @@ extern "C" int vgpu_batch_run(vgpu_batch *b) {
     return bt_enqueue(b, false);
 }
 
-extern "C" int vgpu_batch_rerun(vgpu_batch *b, const uint64_t *params, int n_active) {
+extern "C" int vgpu_batch_run(vgpu_batch *b) {
+    return vg_guard<int>(-1, "vgpu_batch_run", [&] { return batch_run_body(b); });
+}
+
+static int batch_rerun_body(vgpu_batch *b, const uint64_t *params, int n_active) {
     if (!b->planned) {
         vg_set_error("batch: rerun before plan");
         return -1;
@@ -1659,7 +1678,11 @@ extern "C" int vgpu_batch_rerun(vgpu_batch *b, const uint64_t *params, int n_act
     return bt_enqueue(b, true);
 }
 
-extern "C" int vgpu_batch_capture(vgpu_batch *b) {
+extern "C" int vgpu_batch_rerun(vgpu_batch *b, const uint64_t *params, int n_active) {
+    return vg_guard<int>(-1, "vgpu_batch_rerun", [&] { return batch_rerun_body(b, params, n_active); });
+}
+
+static int batch_capture_body(vgpu_batch *b) {
     if (!b->planned) {
         vg_set_error("batch: capture before plan");
         return -1;
@@ -1689,6 +1712,10 @@ extern "C" int vgpu_batch_capture(vgpu_batch *b) {
     return 0;
 }
 
+extern "C" int vgpu_batch_capture(vgpu_batch *b) {
+    return vg_guard<int>(-1, "vgpu_batch_capture", [&] { return batch_capture_body(b); });
+}
+
 static int bt_read_counts(vgpu_batch *b) {
     if (b->counts_valid) return 0;
     hipStream_t st = vg_stream();
@@ -1711,7 +1738,7 @@ static int bt_read_counts(vgpu_batch *b) {
     return 0;
 }
 
-extern "C" int vgpu_batch_counts(vgpu_batch *b, int64_t *n_groups, int64_t *offsets) {
+static int batch_counts_body(vgpu_batch *b, int64_t *n_groups, int64_t *offsets) {
     if (!b->planned) {
         vg_set_error("batch: counts before plan");
         return -1;
@@ -1722,6 +1749,10 @@ extern "C" int vgpu_batch_counts(vgpu_batch *b, int64_t *n_groups, int64_t *offs
         if (offsets) offsets[q] = b->h_off[q];
     }
     return 0;
+}
+
+extern "C" int vgpu_batch_counts(vgpu_batch *b, int64_t *n_groups, int64_t *offsets) {
+    return vg_guard<int>(-1, "vgpu_batch_counts", [&] { return batch_counts_body(b, n_groups, offsets); });
 }
 
 // copies bits [off, off + n) of a device bitmap into dst (bit 0 = first)
@@ -1740,8 +1771,8 @@ static int bt_copy_bits(uint8_t *dst, const uint8_t *dev_bits, int64_t off, int6
     return 0;
 }
 
-extern "C" int64_t vgpu_batch_fetch(vgpu_batch *b, int q, void *keys, uint8_t *key_validity,
-                                    double *const *agg_values, uint8_t *const *agg_validity) {
+static int64_t batch_fetch_body(vgpu_batch *b, int q, void *keys, uint8_t *key_validity, double *const *agg_values,
+                                uint8_t *const *agg_validity) {
     if (!b->planned) {
         vg_set_error("batch: fetch before plan");
         return -1;
@@ -1771,6 +1802,12 @@ extern "C" int64_t vgpu_batch_fetch(vgpu_batch *b, int q, void *keys, uint8_t *k
                           st));
     VG_TRY(hipStreamSynchronize(st));
     return (int64_t)qctl[4];
+}
+
+extern "C" int64_t vgpu_batch_fetch(vgpu_batch *b, int q, void *keys, uint8_t *key_validity, double *const *agg_values,
+                                    uint8_t *const *agg_validity) {
+    return vg_guard<int64_t>(-1, "vgpu_batch_fetch", [&] { return batch_fetch_body(b, q, keys, key_validity, agg_values,
+        agg_validity); });
 }
 
 extern "C" int64_t vgpu_batch_device_bytes(const vgpu_batch *b) {

@@ -1001,8 +1001,8 @@ static int gb_ensure_capacity(vgpu_groupby *g, int64_t incoming) {
     return 0;
 }
 
-extern "C" vgpu_groupby *vgpu_groupby_create(int key_type, int n_inputs, const int32_t *input_types, int n_aggs,
-                                             const vgpu_agg *aggs, int order, int64_t expected_groups) {
+static vgpu_groupby *groupby_create_body(int key_type, int n_inputs, const int32_t *input_types, int n_aggs,
+                                         const vgpu_agg *aggs, int order, int64_t expected_groups) {
     if (vg_ensure_init()) return nullptr;
     if (key_type != VGPU_INT64 && key_type != VGPU_DOUBLE && key_type != VGPU_BOOL && key_type != VGPU_KEY_NONE) {
         vg_set_error("groupby: unsupported key type %d", key_type);
@@ -1036,6 +1036,12 @@ extern "C" vgpu_groupby *vgpu_groupby_create(int key_type, int n_inputs, const i
     g->off_vals.resize(n_aggs);
     g->off_valid.resize(n_aggs);
     return g;
+}
+
+extern "C" vgpu_groupby *vgpu_groupby_create(int key_type, int n_inputs, const int32_t *input_types, int n_aggs,
+                                             const vgpu_agg *aggs, int order, int64_t expected_groups) {
+    return vg_guard<vgpu_groupby *>(nullptr, "vgpu_groupby_create", [&] { return groupby_create_body(key_type, n_inputs,
+        input_types, n_aggs, aggs, order, expected_groups); });
 }
 
 // row copies with 16-byte accesses when rows have an even number of words
@@ -1916,9 +1922,9 @@ extern "C" int vgpu_key_codes(const vgpu_col *keys, int n_keys, int64_t n_rows, 
     return 0;
 }
 
-extern "C" int vgpu_groupby_feed(vgpu_groupby *g, const vgpu_col *key, const vgpu_col *inputs, int64_t n_rows,
-                                 const int32_t *sel, int64_t n_sel, int64_t pos_base, const vgpu_prog *pred,
-                                 const vgpu_col *pred_cols, int n_pred_cols) {
+static int groupby_feed_body(vgpu_groupby *g, const vgpu_col *key, const vgpu_col *inputs, int64_t n_rows,
+                             const int32_t *sel, int64_t n_sel, int64_t pos_base, const vgpu_prog *pred,
+                             const vgpu_col *pred_cols, int n_pred_cols) {
     if (g->finished) {
         vg_set_error("groupby: feed after finish");
         return -1;
@@ -2010,6 +2016,13 @@ extern "C" int vgpu_groupby_feed(vgpu_groupby *g, const vgpu_col *key, const vgp
     vg_kt_end();
     g->any_fed = 1;
     return 0;
+}
+
+extern "C" int vgpu_groupby_feed(vgpu_groupby *g, const vgpu_col *key, const vgpu_col *inputs, int64_t n_rows,
+                                 const int32_t *sel, int64_t n_sel, int64_t pos_base, const vgpu_prog *pred,
+                                 const vgpu_col *pred_cols, int n_pred_cols) {
+    return vg_guard<int>(-1, "vgpu_groupby_feed", [&] { return groupby_feed_body(g, key, inputs, n_rows, sel, n_sel,
+        pos_base, pred, pred_cols, n_pred_cols); });
 }
 
 // Is this feed the star shape k_gb_star handles?  (join with one or two
@@ -2134,7 +2147,7 @@ static bool gb_star_dense_shape(const vgpu_groupby *g, const vgpu_feed_spec *sp,
     return true;
 }
 
-extern "C" int vgpu_groupby_feed_join(vgpu_groupby *g, const vgpu_feed_spec *sp) {
+static int groupby_feed_join_body(vgpu_groupby *g, const vgpu_feed_spec *sp) {
     if (g->finished) {
         vg_set_error("groupby: feed after finish");
         return -1;
@@ -2329,6 +2342,10 @@ extern "C" int vgpu_groupby_feed_join(vgpu_groupby *g, const vgpu_feed_spec *sp)
     return 0;
 }
 
+extern "C" int vgpu_groupby_feed_join(vgpu_groupby *g, const vgpu_feed_spec *sp) {
+    return vg_guard<int>(-1, "vgpu_groupby_feed_join", [&] { return groupby_feed_join_body(g, sp); });
+}
+
 static RecSrc gb_table_src(const vgpu_groupby *g) {
     RecSrc R;
     R.hot = g->hot.as<uint64_t>();
@@ -2365,7 +2382,7 @@ static int gb_compact(vgpu_groupby *g, const RecSrc &R, int64_t n_slots) {
 static int highest_bit(uint64_t x) { return x ? 63 - __builtin_clzll(x) : -1; }
 static int lowest_bit(uint64_t x) { return x ? __builtin_ctzll(x) : 64; }
 
-extern "C" int64_t vgpu_groupby_finish(vgpu_groupby *g) {
+static int64_t groupby_finish_body(vgpu_groupby *g) {
     if (g->finished) return g->n_groups;
     hipStream_t s = vg_stream();
     if (g->cap == 0 && gb_alloc_table(g, 64)) return -1;
@@ -2444,8 +2461,12 @@ extern "C" int64_t vgpu_groupby_finish(vgpu_groupby *g) {
     return n_out;
 }
 
-extern "C" int64_t vgpu_groupby_fetch(vgpu_groupby *g, void *keys, uint8_t *key_validity, double *const *agg_values,
-                                      uint8_t *const *agg_validity) {
+extern "C" int64_t vgpu_groupby_finish(vgpu_groupby *g) {
+    return vg_guard<int64_t>(-1, "vgpu_groupby_finish", [&] { return groupby_finish_body(g); });
+}
+
+static int64_t groupby_fetch_body(vgpu_groupby *g, void *keys, uint8_t *key_validity, double *const *agg_values,
+                                  uint8_t *const *agg_validity) {
     if (!g->finished) {
         vg_set_error("groupby: fetch before finish");
         return -1;
@@ -2494,6 +2515,12 @@ extern "C" int64_t vgpu_groupby_fetch(vgpu_groupby *g, void *keys, uint8_t *key_
     VG_TRY(hipStreamSynchronize(s));
     g->warn_count = (int64_t)ctl[4];
     return g->warn_count;
+}
+
+extern "C" int64_t vgpu_groupby_fetch(vgpu_groupby *g, void *keys, uint8_t *key_validity, double *const *agg_values,
+                                      uint8_t *const *agg_validity) {
+    return vg_guard<int64_t>(-1, "vgpu_groupby_fetch", [&] { return groupby_fetch_body(g, keys, key_validity,
+        agg_values, agg_validity); });
 }
 
 extern "C" int vgpu_groupby_result_device(vgpu_groupby *g, const void **keys, const uint8_t **key_validity, int agg,

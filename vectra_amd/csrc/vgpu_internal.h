@@ -5,6 +5,8 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <new>
+
 #include "vgpu.h"
 
 // ---------------------------------------------------------------------------
@@ -12,6 +14,22 @@
 // ---------------------------------------------------------------------------
 
 void vg_set_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+
+// C entry points whose work uses the C++ library (containers, new) run it
+// through vg_guard: an exception becomes the library error and the entry
+// point's failure value instead of unwinding through C callers.
+template <typename R, typename F>
+static inline R vg_guard(R failed, const char *where, F &&body) {
+    try {
+        return body();
+    } catch (const std::bad_alloc &) {
+        vg_set_error("%s: host memory exhausted", where);
+    } catch (...) {
+        vg_set_error("%s: unexpected C++ exception", where);
+    }
+    return failed;
+}
+
 hipStream_t vg_stream();
 int vg_ensure_init();
 // bracket a main kernel launch for vgpu_kernel_time_ms (no-ops unless enabled)

@@ -294,12 +294,14 @@ static vgpu_jtable *join_build_impl(const vgpu_col *key, int64_t n_build, const 
 }
 
 extern "C" vgpu_jtable *vgpu_join_build(const vgpu_col *key, int64_t n_build) {
-    return join_build_impl(key, n_build, nullptr, 0);
+    return vg_guard<vgpu_jtable *>(nullptr, "vgpu_join_build",
+                                   [&] { return join_build_impl(key, n_build, nullptr, 0); });
 }
 
 extern "C" vgpu_jtable *vgpu_join_build_payload(const vgpu_col *key, int64_t n_build, const vgpu_col *payload,
                                                 int n_payload) {
-    return join_build_impl(key, n_build, payload, n_payload);
+    return vg_guard<vgpu_jtable *>(nullptr, "vgpu_join_build_payload",
+                                   [&] { return join_build_impl(key, n_build, payload, n_payload); });
 }
 
 extern "C" int64_t vgpu_join_build_rows(const vgpu_jtable *t) { return t ? t->n_build : -1; }
@@ -370,8 +372,8 @@ k_jp_write(JView v, int64_t n_logical, const int32_t *sel, int kind, const uint3
     }
 }
 
-extern "C" int vgpu_join_probe(vgpu_jtable *t, int kind, const vgpu_col *probe_key, int64_t n_rows,
-                               const int32_t *sel, int64_t n_sel, int first_match_only, int64_t *n_out) {
+static int join_probe_body(vgpu_jtable *t, int kind, const vgpu_col *probe_key, int64_t n_rows, const int32_t *sel,
+                           int64_t n_sel, int first_match_only, int64_t *n_out) {
     if (kind < VGPU_JOIN_INNER || kind > VGPU_JOIN_ANTI) {
         vg_set_error("join_probe: unknown join kind %d", kind);
         return -1;
@@ -424,6 +426,12 @@ extern "C" int vgpu_join_probe(vgpu_jtable *t, int kind, const vgpu_col *probe_k
     *n_out = total;
     t->last_n_out = total;
     return 0;
+}
+
+extern "C" int vgpu_join_probe(vgpu_jtable *t, int kind, const vgpu_col *probe_key, int64_t n_rows, const int32_t *sel,
+                               int64_t n_sel, int first_match_only, int64_t *n_out) {
+    return vg_guard<int>(-1, "vgpu_join_probe", [&] { return join_probe_body(t, kind, probe_key, n_rows, sel, n_sel,
+        first_match_only, n_out); });
 }
 
 extern "C" int vgpu_join_pairs(vgpu_jtable *t, const int32_t **probe_rows, const int64_t **build_rows) {

@@ -127,7 +127,14 @@ void vg_pool_free(void *ptr, size_t bytes) {
     size_t r;
     int k = pool_class(bytes ? bytes : 1, &r);
     std::lock_guard<std::mutex> lk(g_pool_mu);
-    g_pool[k].push_back(ptr);
+    try {
+        g_pool[k].push_back(ptr);
+    } catch (const std::bad_alloc &) {
+        // no room to cache it: give the block back (hipFree waits for the
+        // device work that may still read it)
+        hipError_t e = hipFree(ptr);
+        (void)e;
+    }
 }
 
 void vg_pool_trim() {
@@ -359,7 +366,12 @@ extern "C" int vgpu_tune_set_for(int64_t max_groups, const char *name, int64_t v
         TuneBucket b;
         memset(&b, 0, sizeof b);
         b.max_groups = max_groups;
-        it = g_buckets.insert(it, b);
+        try {
+            it = g_buckets.insert(it, b);
+        } catch (const std::bad_alloc &) {
+            vg_set_error("vgpu_tune_set_for: host memory exhausted");
+            return -1;
+        }
     }
     it->val[i] = value;
     it->set |= 1u << i;
