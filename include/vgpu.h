@@ -190,6 +190,18 @@ int vgpu_key_codes(const vgpu_col *keys, int n_keys, int64_t n_rows, int64_t pos
  * aligned; bit r set when no component of row r is NA or NaN -- the rows a
  * join may match, src/join.c:77-113) */
 
+/* String key columns (device: offsets[n_rows + 1] into chars, validity
+ * optional): per row two independent 64-bit hashes of its bytes into h1 and
+ * h2 (h2 may be NULL; NA rows get 0) and its position pos_base + row into
+ * pos (may be NULL).  h1 (with the column's validity) stands for the string
+ * in vgpu_key_codes and in join tables; h2 and the position let a group-by
+ * prove that each group holds one string (min == max of h2) and find a row
+ * that holds it (min of the position).  Replaces vec_hash_value's string
+ * case (src/hash.c:35-41) and the join's probe hashing of string keys
+ * (src/join.c:335-341). */
+int vgpu_string_keys(const int64_t *offsets, const uint8_t *chars, const uint8_t *validity, int64_t n_rows,
+                     int64_t pos_base, int64_t *h1, int64_t *h2, int64_t *pos);
+
 /* Finishes on the device and orders the groups; returns the group count. */
 int64_t vgpu_groupby_finish(vgpu_groupby *g);
 

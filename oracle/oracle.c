@@ -578,11 +578,14 @@ typedef struct {
 } OGroup;
 
 /* growing dense copy of whole input rows (keys + agg inputs), used as the
- * key arena for the hash path and as the materialized input of the sort */
+ * key arena for the hash path and as the materialized input of the sort;
+ * string values are appended to the column's data (builder.c's layout:
+ * offsets[n + 1] into one byte buffer) */
 typedef struct {
     int       nc;
     VecType  *types;
     VecArray *cols;
+    int64_t  *dcap;     /* per string column: bytes allocated for data */
     int64_t   n, cap;
 } RowStore;
 
@@ -591,9 +594,16 @@ static void rs_init(RowStore *s, int nc, const VecType *types) {
     s->types = (VecType *)malloc(sizeof(VecType) * (size_t)(nc ? nc : 1));
     memcpy(s->types, types, sizeof(VecType) * (size_t)nc);
     s->cols = (VecArray *)calloc((size_t)(nc ? nc : 1), sizeof(VecArray));
+    s->dcap = (int64_t *)calloc((size_t)(nc ? nc : 1), sizeof(int64_t));
     s->n = 0;
     s->cap = 64;
-    for (int c = 0; c < nc; c++) s->cols[c] = vec_array_alloc(types[c], s->cap);
+    for (int c = 0; c < nc; c++) {
+        s->cols[c] = vec_array_alloc(types[c], s->cap);
+        if (types[c] == VEC_STRING) {
+            s->dcap[c] = 256;
+            s->cols[c].buf.str.data = (char *)malloc(256);
+        }
+    }
 }
 
 static void rs_append(RowStore *s, const VecArray *const *src, int64_t row) {
@@ -601,9 +611,12 @@ static void rs_append(RowStore *s, const VecArray *const *src, int64_t row) {
         int64_t nc2 = s->cap * 2;
         for (int c = 0; c < s->nc; c++) {
             VecArray *a = &s->cols[c];
-            if (a->type == VEC_STRING) vectra_error("oracle: string columns are not on the hot path");
-            size_t w = a->type == VEC_BOOL ? 1 : 8;
-            a->buf.i64 = (int64_t *)realloc(a->buf.i64, (size_t)nc2 * w);
+            if (a->type == VEC_STRING) {
+                a->buf.str.offsets = (int64_t *)realloc(a->buf.str.offsets, (size_t)(nc2 + 1) * 8);
+            } else {
+                size_t w = a->type == VEC_BOOL ? 1 : 8;
+                a->buf.i64 = (int64_t *)realloc(a->buf.i64, (size_t)nc2 * w);
+            }
             a->validity = (uint8_t *)realloc(a->validity, (size_t)vec_validity_bytes(nc2));
             memset(a->validity + vec_validity_bytes(s->cap), 0,
                    (size_t)(vec_validity_bytes(nc2) - vec_validity_bytes(s->cap)));
@@ -615,8 +628,24 @@ static void rs_append(RowStore *s, const VecArray *const *src, int64_t row) {
         VecArray *a = &s->cols[c];
         const VecArray *x = src[c];
         if (x->type != a->type) vectra_error("oracle: column type changed between batches");
-        if (vec_array_is_valid(x, row)) {
-            vec_array_set_valid(a, s->n);
+        const int ok = vec_array_is_valid(x, row);
+        if (ok) vec_array_set_valid(a, s->n);
+        if (a->type == VEC_STRING) {
+            const int64_t at = s->n ? a->buf.str.offsets[s->n] : 0;
+            int64_t len = 0;
+            if (ok) {
+                const int64_t b0 = x->buf.str.offsets[row];
+                len = x->buf.str.offsets[row + 1] - b0;
+                if (at + len > s->dcap[c]) {
+                    while (at + len > s->dcap[c]) s->dcap[c] *= 2;
+                    a->buf.str.data = (char *)realloc(a->buf.str.data, (size_t)s->dcap[c]);
+                }
+                memcpy(a->buf.str.data + at, x->buf.str.data + b0, (size_t)len);
+            }
+            a->buf.str.offsets[s->n] = at;
+            a->buf.str.offsets[s->n + 1] = at + len;
+            a->buf.str.data_len = at + len;
+        } else if (ok) {
             if (a->type == VEC_BOOL) a->buf.bln[s->n] = x->buf.bln[row];
             else a->buf.i64[s->n] = x->buf.i64[row];   /* 8 raw bytes */
         }
@@ -628,6 +657,30 @@ static void rs_free(RowStore *s) {
     for (int c = 0; c < s->nc; c++) vec_array_free(&s->cols[c]);
     free(s->cols);
     free(s->types);
+    free(s->dcap);
+}
+
+/* out[0..n) = the strings s[rows[i]] (NA for rows[i] < 0 or an NA value),
+ * written as one offsets + data array (src/builder.c:110-138) */
+static void str_gather(VecArray *d, const VecArray *s, const int64_t *rows, int64_t n) {
+    int64_t total = 0;
+    for (int64_t i = 0; i < n; i++)
+        if (rows[i] >= 0 && vec_array_is_valid(s, rows[i]))
+            total += s->buf.str.offsets[rows[i] + 1] - s->buf.str.offsets[rows[i]];
+    free(d->buf.str.data);
+    d->buf.str.data = (char *)malloc((size_t)(total > 0 ? total : 1));
+    d->buf.str.data_len = total;
+    int64_t at = 0;
+    for (int64_t i = 0; i < n; i++) {
+        d->buf.str.offsets[i] = at;
+        const int64_t r = rows[i];
+        if (r < 0 || !vec_array_is_valid(s, r)) continue;
+        vec_array_set_valid(d, i);
+        const int64_t b0 = s->buf.str.offsets[r], len = s->buf.str.offsets[r + 1] - b0;
+        memcpy(d->buf.str.data + at, s->buf.str.data + b0, (size_t)len);
+        at += len;
+    }
+    d->buf.str.offsets[n] = at;
 }
 
 static int resolve(const VecSchema *s, const char *name, const char *what) {
@@ -720,7 +773,13 @@ static VecBatch *ogroup_hash(OGroup *g) {
     VecBatch *out = vec_batch_alloc(nk + na, ngroups);
     for (int k = 0; k < nk; k++) {
         VecArray d = vec_array_alloc(ktypes[k], ngroups);
-        for (int64_t i = 0; i < ngroups; i++) {
+        if (d.type == VEC_STRING) {
+            int64_t *ix = (int64_t *)malloc((size_t)(ngroups > 0 ? ngroups : 1) * 8);
+            for (int64_t i = 0; i < ngroups; i++) ix[i] = i;
+            str_gather(&d, &arena.cols[k], ix, ngroups);
+            free(ix);
+        }
+        for (int64_t i = 0; i < ngroups && d.type != VEC_STRING; i++) {
             if (!vec_array_is_valid(&arena.cols[k], i)) continue;
             vec_array_set_valid(&d, i);
             if (d.type == VEC_BOOL) d.buf.bln[i] = arena.cols[k].buf.bln[i];
@@ -760,9 +819,16 @@ static int sort_cmp_value(const VecArray *a, int64_t ra, int64_t rb) {
     }
     case VEC_BOOL:
         return (int)a->buf.bln[ra] - (int)a->buf.bln[rb];
-    default:
-        vectra_error("oracle: string sort keys are not on the hot path");
+    case VEC_STRING: {
+        /* bytes over the shorter length, then the shorter first (src/sort.c:58-66) */
+        const int64_t sa = a->buf.str.offsets[ra], la = a->buf.str.offsets[ra + 1] - sa;
+        const int64_t sb = a->buf.str.offsets[rb], lb = a->buf.str.offsets[rb + 1] - sb;
+        const int c = memcmp(a->buf.str.data + sa, a->buf.str.data + sb, (size_t)(la < lb ? la : lb));
+        if (c) return c < 0 ? -1 : 1;
+        return la < lb ? -1 : la > lb ? 1 : 0;
     }
+    }
+    return 0;
 }
 
 typedef struct { const VecArray *cols; int nk; } SortCtx;
@@ -802,7 +868,7 @@ static int same_group(const VecArray *cols, int nk, int64_t a, int64_t b) {
         case VEC_INT64:  if (x->buf.i64[a] != x->buf.i64[b]) return 0; break;
         case VEC_DOUBLE: if (x->buf.dbl[a] != x->buf.dbl[b]) return 0; break;
         case VEC_BOOL:   if (x->buf.bln[a] != x->buf.bln[b]) return 0; break;
-        default: return 0;
+        case VEC_STRING: if (!group_val_equal(x, a, x, b)) return 0; break;
         }
     }
     return 1;
@@ -868,7 +934,8 @@ static VecBatch *ogroup_sorted(OGroup *g) {
     VecBatch *out = vec_batch_alloc(nk + na, ngroups);
     for (int k = 0; k < nk; k++) {
         VecArray d = vec_array_alloc(ctypes[k], ngroups);
-        for (int64_t q = 0; q < ngroups; q++) {
+        if (d.type == VEC_STRING) str_gather(&d, &st.cols[k], first_row, ngroups);
+        for (int64_t q = 0; q < ngroups && d.type != VEC_STRING; q++) {
             int64_t r = first_row[q];
             if (!vec_array_is_valid(&st.cols[k], r)) continue;
             vec_array_set_valid(&d, q);
@@ -985,7 +1052,7 @@ static int join_eq(const VecArray *pc, const int *pk, const VecArray *bc,
         case VEC_INT64:  if (p->buf.i64[pr] != b->buf.i64[br]) return 0; break;
         case VEC_DOUBLE: if (p->buf.dbl[pr] != b->buf.dbl[br]) return 0; break;
         case VEC_BOOL:   if (p->buf.bln[pr] != b->buf.bln[br]) return 0; break;
-        default: return 0;
+        case VEC_STRING: if (!group_val_equal(p, pr, b, br)) return 0; break;
         }
     }
     return 1;
@@ -1161,12 +1228,10 @@ static VecBatch *ojoin_probe(OJoin *j, VecBatch *pb) {
     VecBatch *res = vec_batch_alloc(onc, out.n);
     for (int c = 0; c < onc; c++) {
         VecArray d = vec_array_alloc(j->base.output_schema.col_types[c], out.n);
-        if (c < lnc) {
-            for (int64_t i = 0; i < out.n; i++) put_value(&d, i, &pb->columns[c], out.p[i]);
-        } else {
-            const VecArray *s = &j->r_cols[j->r_nonkey[c - lnc]];
-            for (int64_t i = 0; i < out.n; i++) put_value(&d, i, s, out.b[i]);
-        }
+        const VecArray *s = c < lnc ? &pb->columns[c] : &j->r_cols[j->r_nonkey[c - lnc]];
+        const int64_t *rows = c < lnc ? out.p : out.b;
+        if (d.type == VEC_STRING) str_gather(&d, s, rows, out.n);
+        else for (int64_t i = 0; i < out.n; i++) put_value(&d, i, s, rows[i]);
         res->columns[c] = d;
         res->col_names[c] = vx_strdup(j->base.output_schema.col_names[c]);
     }
@@ -1194,7 +1259,8 @@ static VecBatch *ojoin_finalize(OJoin *j) {
         } else {
             s = &j->r_cols[j->r_nonkey[c - lnc]];
         }
-        if (s) for (int64_t i = 0; i < out.n; i++) put_value(&d, i, s, out.b[i]);
+        if (s && d.type == VEC_STRING) str_gather(&d, s, out.b, out.n);
+        else if (s) for (int64_t i = 0; i < out.n; i++) put_value(&d, i, s, out.b[i]);
         res->columns[c] = d;
         res->col_names[c] = vx_strdup(j->base.output_schema.col_names[c]);
     }

@@ -1,0 +1,141 @@
+"""String keys through the GPU nodes against the oracle nodes: group_by on
+a string column alone or with numeric columns (src/group_agg.c with
+VEC_STRING keys: src/hash.c:35-41 hashing, :58-66 equality; src/sort.c:58-66
+order in key order) and joins on string keys (src/join.c:96-108, 335-341),
+every join kind, with NA strings, empty strings, shared prefixes, multi-byte
+UTF-8, host row groups and a filter's selection vectors.  Keys, order,
+counts and join rows exact; double sums and means within 1e-12 * sum|x|."""
+import numpy as np
+import pytest
+
+from helpers import assert_tables_equal, rng, with_na
+
+pytestmark = pytest.mark.gpu
+
+
+def strings(r, n, n_distinct):
+    pool = ["", "a", "ab", "abc", "b", "ba", "Z", "é", "ü1", "x" * 40, "x" * 41] + [
+        f"key-{i:06d}" + "#" * (i % 9) for i in range(max(0, n_distinct - 11))]
+    pool = pool[:n_distinct]
+    return np.array([pool[i] for i in r.integers(0, len(pool), n)], dtype=object)
+
+
+def group_table(seed, n, n_distinct):
+    r = rng(seed)
+    x = r.normal(size=n)
+    return {"g": (strings(r, n, n_distinct), with_na(r, n, 0.05)), "k": (r.integers(0, 4, n), with_na(r, n, 0.03)),
+            "x": (x, with_na(r, n, 0.02)), "v": r.integers(-2**40, 2**40, n), "ax": np.abs(x)}
+
+
+AGGS = [("n", "n", None, 0), ("sx", "sum", "x", 1), ("mx", "mean", "x", 1), ("lo", "min", "v", 0),
+        ("hi", "max", "x", 1)]
+
+
+def group(nv, engine, cols, keys, temp_dir, rg_rows, pred, aggs):
+    node = nv.Table(cols, rg_rows=rg_rows).scan()
+    if pred is not None:
+        node = nv.filter_node(engine, node, pred)
+    return nv.collect(nv.group_agg_node(engine, node, keys, aggs, temp_dir))
+
+
+@pytest.mark.parametrize("keys,temp_dir,pred,n_distinct", [
+    (["g"], None, None, 50),
+    (["g"], "/tmp", None, 50),
+    (["g"], None, ("cmp", ">", ("col", "x"), ("dbl", 0.3)), 5000),
+    (["g", "k"], "/tmp", None, 300),
+    (["k", "g"], None, ("cmp", "<", ("col", "v"), ("int", 0)), 300),
+])
+def test_string_groupby_matches_oracle(gpu, oracle, keys, temp_dir, pred, n_distinct):
+    cols = group_table(101 + n_distinct, 60_000, n_distinct)
+    got = group(gpu, "gpu", cols, keys, temp_dir, 7_000, pred, AGGS)
+    exp = group(gpu, "oracle", cols, keys, temp_dir, 7_000, pred,
+                AGGS + [("__sa", "sum", "ax", 1), ("__cx", "count", "x", 0)])
+    sa = exp.pop("__sa")[0]
+    cnt = exp.pop("__cx")[0]
+    assert_tables_equal(got, exp, {"sx": sa, "mx": sa / np.maximum(cnt, 1)})
+
+
+def test_string_groupby_known_answers(gpu):
+    """test-groupby.R:1-60 through the GPU node"""
+    def q(cols, aggs, td=None):
+        return gpu.collect(gpu.group_agg_node("gpu", gpu.Table(cols).scan(), ["g"], aggs, td))
+
+    out = q({"g": np.array(["a", "b", "a", "b", "a"]), "x": np.arange(1, 6)}, [("cnt", "n", None, 0)])
+    assert list(out["g"][0]) == ["a", "b"] and list(out["cnt"][0]) == [3, 2]
+    out = q({"g": np.array(["a", "a", "b", "b"]), "x": np.array([10., 20., 30., 40.])}, [("avg", "mean", "x", 0)])
+    assert list(out["avg"][0]) == [15, 35]
+    out = q({"g": np.array(["a", "a", "b", "b"]), "x": np.array([5., 1., 8., 3.])},
+            [("lo", "min", "x", 0), ("hi", "max", "x", 0)])
+    assert list(out["lo"][0]) == [1, 3] and list(out["hi"][0]) == [5, 8]
+    for td in (None, "/tmp"):
+        out = q({"g": (np.array(["a", "", "a", ""]), np.array([1, 0, 1, 0], bool)), "x": np.array([1., 2., 3., 4.])},
+                [("total", "sum", "x", 0)], td)
+        assert list(out["g"][1]) == [True, False] and list(out["total"][0]) == [4, 6]
+
+
+def test_string_groupby_empty_input(gpu, oracle):
+    cols = {"g": np.array(["a", "b"], dtype=object), "x": np.array([1., 2.])}
+    pred = ("cmp", ">", ("col", "x"), ("dbl", 5.0))
+    for td in (None, "/tmp"):
+        got = group(gpu, "gpu", cols, ["g"], td, 0, pred, [("n", "n", None, 0)])
+        exp = group(gpu, "oracle", cols, ["g"], td, 0, pred, [("n", "n", None, 0)])
+        assert_tables_equal(got, exp)
+
+
+def join_tables(nv, seed, n_left, n_right, n_distinct, composite):
+    def mk():
+        r = rng(seed)
+        lk, rk = strings(r, n_left, n_distinct), strings(r, n_right, n_distinct)
+        lcols = {"name": (lk, with_na(r, n_left, 0.05))}
+        rcols = {"w": r.random(n_right), "name": (rk, with_na(r, n_right, 0.05))}
+        if composite:
+            lcols["a"] = r.integers(0, 3, n_left)
+            rcols["a"] = r.integers(0, 3, n_right).astype(np.float64)
+        lcols["x"] = r.normal(size=n_left)
+        rcols["label"] = strings(r, n_right, 20)
+        return nv.Table(lcols, rg_rows=700), nv.Table(rcols, rg_rows=500)
+    return mk
+
+
+@pytest.mark.parametrize("kind", ["inner", "left", "semi", "anti", "full"])
+@pytest.mark.parametrize("composite", [False, True])
+def test_string_join_matches_oracle(gpu, oracle, kind, composite):
+    """a string key, or a number and a string key (the left number is
+    coerced to the right's double: inner / semi / anti only, as the GPU
+    path declines coerced left / full joins on several keys)"""
+    mk = join_tables(gpu, 151, 5000, 1800, 400, composite)
+    keys = [(0, 1)] + ([(1, 2)] if composite else [])
+    if composite and kind in ("left", "full"):
+        left, right = mk()
+        node = gpu.join_node("gpu", left.scan(), right.scan(), kind, keys)
+        with pytest.raises(gpu.VectraError, match="coerced to double is not evaluated on the GPU"):
+            gpu.next_batches(node)
+        return
+    out = {}
+    for engine in ("gpu", "oracle"):
+        left, right = mk()
+        lnode = gpu.filter_node(engine, left.scan(), ("cmp", ">", ("col", "x"), ("dbl", -0.8)))
+        out[engine] = gpu.next_batches(gpu.join_node(engine, lnode, right.scan(), kind, keys))
+    got, exp = out["gpu"], out["oracle"]
+    assert len(got) == len(exp), f"{len(got)} batches vs {len(exp)}"
+    for g, e in zip(got, exp):
+        assert_tables_equal(g["cols"], e["cols"])
+
+
+def test_string_join_known_answers(gpu):
+    """test-joins.R:135-163 through the GPU node"""
+    L = gpu.Table({"name": np.array(["alice", "bob"]), "x": np.array([1., 2.])})
+    R = gpu.Table({"name": np.array(["bob", "carol"]), "y": np.array([20., 30.])})
+    out = gpu.collect(gpu.join_node("gpu", L.scan(), R.scan(), "inner", [(0, 0)]))
+    assert list(out["name"][0]) == ["bob"] and list(out["y"][0]) == [20]
+    L = gpu.Table({"a": np.array([1., 1., 2.]), "b": np.array(["x", "y", "x"]), "v1": np.array([10., 20., 30.])})
+    R = gpu.Table({"a": np.array([1., 2.]), "b": np.array(["x", "x"]), "v2": np.array([100., 300.])})
+    out = gpu.collect(gpu.join_node("gpu", L.scan(), R.scan(), "inner", [(0, 0), (1, 1)]))
+    assert sorted(out["v1"][0]) == [10, 30]
+
+
+def test_string_key_against_number_is_an_error(gpu):
+    L = gpu.Table({"name": np.array(["a"]), "x": np.array([1.])})
+    R = gpu.Table({"name": np.array([1.]), "y": np.array([2.])})
+    with pytest.raises(gpu.VectraError, match="key type mismatch"):
+        gpu.join_node("gpu", L.scan(), R.scan(), "inner", [(0, 0)])

@@ -24,7 +24,7 @@ def str_array(nv, strings):
     a.validity = None
     a.buf.str.offsets = offs.ctypes.data_as(C.POINTER(C.c_int64))
     buf = C.create_string_buffer(data, max(1, len(data)))
-    a.buf.str.data = C.cast(buf, C.c_char_p)
+    a.buf.str.data = C.addressof(buf)
     return a, (offs, buf)
 
 
@@ -287,3 +287,105 @@ def test_join_known_answers(native, oracle):
     R = nv.Table({"id": np.array([1, 2]), "val": np.array([100., 200.])})
     out = nv.collect(nv.join_node("oracle", L.scan(), R.scan(), "left", [(0, 0)]))
     assert list(out) == ["id", "val.x", "val.y"]
+
+
+# ---------------------------------------------------------------- string keys
+
+def str_keys(r, n, n_distinct, p_na=0.1):
+    """random strings from a pool of n_distinct values: empty, one byte,
+    shared prefixes, multi-byte UTF-8 and lengths up to 40 bytes"""
+    pool = ["", "a", "ab", "abc", "b", "ba", "Z", "é", "ü1", "x" * 40] + [
+        f"k{i:04d}" + "-" * (i % 7) for i in range(max(0, n_distinct - 10))]
+    pool = pool[:n_distinct]
+    vals = np.array([pool[i] for i in r.integers(0, len(pool), n)], dtype=object)
+    return vals, r.random(n) >= p_na
+
+
+def test_groupby_string_key_known_answers(native, oracle):
+    """test-groupby.R:1-60 with their string keys"""
+    nv = native
+
+    def q(cols, aggs, td=None):
+        return nv.collect(nv.group_agg_node("oracle", nv.Table(cols).scan(), ["g"], aggs, td))
+
+    out = q({"g": np.array(["a", "b", "a", "b", "a"]), "x": np.arange(1, 6)}, [("cnt", "n", None, 0)])
+    assert list(out["g"][0]) == ["a", "b"] and list(out["cnt"][0]) == [3, 2]
+    out = q({"g": np.array(["a", "a", "b", "b"]), "x": np.array([1., 2., 3., 4.])}, [("total", "sum", "x", 0)])
+    assert list(out["g"][0]) == ["a", "b"] and list(out["total"][0]) == [3, 7]
+    out = q({"g": np.array(["a", "a", "b", "b"]), "x": np.array([5., 1., 8., 3.])},
+            [("lo", "min", "x", 0), ("hi", "max", "x", 0)])
+    assert list(out["lo"][0]) == [1, 3] and list(out["hi"][0]) == [5, 8]
+    for td in (None, "/tmp"):
+        out = q({"g": (np.array(["a", "", "a", ""]), np.array([1, 0, 1, 0], bool)), "x": np.array([1., 2., 3., 4.])},
+                [("total", "sum", "x", 0)], td)
+        assert list(out["g"][1]) == [True, False] and list(out["total"][0]) == [4, 6]
+
+
+@pytest.mark.parametrize("order", ["first", "sorted"])
+def test_groupby_string_key_brute_force(native, oracle, order):
+    """first-occurrence order (hash path) or byte order with shorter
+    prefixes first and NA last (sort path, src/sort.c:58-66)"""
+    nv = native
+    r = rng(31)
+    n = 3000
+    g, gv = str_keys(r, n, 40)
+    x = r.normal(size=n)
+    out = nv.collect(nv.group_agg_node("oracle", nv.Table({"g": (g, gv), "x": x}, rg_rows=700).scan(), ["g"],
+                                       [("n", "n", None, 0), ("s", "sum", "x", 0)],
+                                       "/tmp" if order == "sorted" else None))
+    keys = [g[i].encode() if gv[i] else None for i in range(n)]
+    seen = list(dict.fromkeys(keys))
+    if order == "sorted":
+        seen = sorted(k for k in seen if k is not None) + ([None] if None in seen else [])
+    got = [v.encode() if ok else None for v, ok in zip(*out["g"])]
+    assert got == seen
+    for i, k in enumerate(seen):
+        rows = [j for j in range(n) if keys[j] == k]
+        assert out["n"][0][i] == len(rows)
+        assert math.isclose(out["s"][0][i], math.fsum(x[rows]), rel_tol=1e-12, abs_tol=1e-12)
+
+
+def test_groupby_string_and_int_keys(native, oracle):
+    """two grouping columns, one of them strings (test-groupby.R:62-73: four groups)"""
+    nv = native
+    out = nv.collect(nv.group_agg_node("oracle", nv.Table({"a": np.array(["x", "x", "y", "y"]),
+                                                           "b": np.array([1, 2, 1, 2]),
+                                                           "v": np.array([10., 20., 30., 40.])}).scan(),
+                                       ["a", "b"], [("s", "sum", "v", 0)], "/tmp"))
+    assert list(out["a"][0]) == ["x", "x", "y", "y"] and list(out["b"][0]) == [1, 2, 1, 2]
+    assert list(out["s"][0]) == [10, 20, 30, 40]
+
+
+def test_join_string_key_known_answers(native, oracle):
+    """test-joins.R:135-163 (a string key; a number and a string key)"""
+    nv = native
+    L = nv.Table({"name": np.array(["alice", "bob"]), "x": np.array([1., 2.])})
+    R = nv.Table({"name": np.array(["bob", "carol"]), "y": np.array([20., 30.])})
+    out = nv.collect(nv.join_node("oracle", L.scan(), R.scan(), "inner", [(0, 0)]))
+    assert list(out["name"][0]) == ["bob"] and list(out["y"][0]) == [20]
+    L = nv.Table({"a": np.array([1., 1., 2.]), "b": np.array(["x", "y", "x"]), "v1": np.array([10., 20., 30.])})
+    R = nv.Table({"a": np.array([1., 2.]), "b": np.array(["x", "x"]), "v2": np.array([100., 300.])})
+    out = nv.collect(nv.join_node("oracle", L.scan(), R.scan(), "inner", [(0, 0), (1, 1)]))
+    assert sorted(out["v1"][0]) == [10, 30]
+
+
+@pytest.mark.parametrize("kind", ["inner", "left", "semi", "anti", "full"])
+def test_join_string_key_brute_force(native, oracle, kind):
+    nv = native
+    r = rng(33)
+    nl, nr = 300, 120
+    lk, lv = str_keys(r, nl, 30)
+    rk, rv = str_keys(r, nr, 30)
+    left = nv.Table({"k": (lk, lv), "i": np.arange(nl)}, rg_rows=64)
+    right = nv.Table({"k": (rk, rv), "j": np.arange(nr)})
+    out = nv.collect(nv.join_node("oracle", left.scan(), right.scan(), kind, [(0, 0)]))
+    exp = brute_join(list(lk), lv, list(rk), rv, kind, 64)
+    assert len(out["i"][0]) == len(exp)
+    for row, (i, j) in enumerate(exp):
+        assert (i is None and not out["i"][1][row]) or out["i"][0][row] == i
+        if kind in ("inner", "left", "full"):
+            assert (j is None and not out["j"][1][row]) or out["j"][0][row] == j
+        if kind != "full" or i is not None:
+            assert out["k"][1][row] == lv[i] and (not lv[i] or out["k"][0][row] == lk[i])
+        else:
+            assert out["k"][1][row] == rv[j] and (not rv[j] or out["k"][0][row] == rk[j])

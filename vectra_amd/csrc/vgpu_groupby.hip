@@ -1922,6 +1922,54 @@ extern "C" int vgpu_key_codes(const vgpu_col *keys, int n_keys, int64_t n_rows, 
     return 0;
 }
 
+// String keys: two independent 64-bit hashes of each row's bytes -- FNV-1a
+// (Fowler, Noll, Vo; the reference hashes strings with it, src/hash.c:35-41)
+// and a multiply-xorshift byte hash seeded with the length and finished with
+// vg_mix64 -- and the row's position. One thread per row reads its bytes in
+// order; neighbouring rows' bytes are neighbours, so a wave's loads share
+// cache lines.
+__global__ void __launch_bounds__(256) k_string_keys(const int64_t *off, const uint8_t *chars, const uint8_t *valid,
+                                                     int64_t n, int64_t pos_base, int64_t *h1, int64_t *h2,
+                                                     int64_t *pos) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
+        uint64_t a = 0, b = 0;
+        if (!valid || ((valid[r >> 3] >> (r & 7)) & 1)) {
+            const int64_t s = off[r], e = off[r + 1];
+            a = 0xcbf29ce484222325ull;
+            b = 0x9e3779b97f4a7c15ull ^ (uint64_t)(e - s);
+            for (int64_t k = s; k < e; k++) {
+                const uint64_t c = chars[k];
+                a = (a ^ c) * 0x100000001b3ull;
+                b = (b ^ c) * 0xd6e8feb86659fd93ull;
+                b ^= b >> 32;
+            }
+            b = vg_mix64(b);
+        }
+        h1[r] = (int64_t)a;
+        if (h2) h2[r] = (int64_t)b;
+        if (pos) pos[r] = pos_base + r;
+    }
+}
+
+extern "C" int vgpu_string_keys(const int64_t *offsets, const uint8_t *chars, const uint8_t *validity, int64_t n_rows,
+                                int64_t pos_base, int64_t *h1, int64_t *h2, int64_t *pos) {
+    if (vg_ensure_init()) return -1;
+    if (n_rows < 0) {
+        vg_set_error("string keys: negative row count");
+        return -1;
+    }
+    if (n_rows == 0) return 0;
+    if (!offsets || !h1) {
+        vg_set_error("string keys: offsets and h1 are required");
+        return -1;
+    }
+    k_string_keys<<<vg_blocks(n_rows, 256, 4096), 256, 0, vg_stream()>>>(offsets, chars, validity, n_rows, pos_base,
+                                                                          h1, h2, pos);
+    VG_LAUNCH_CHECK();
+    return 0;
+}
+
 static int groupby_feed_body(vgpu_groupby *g, const vgpu_col *key, const vgpu_col *inputs, int64_t n_rows,
                              const int32_t *sel, int64_t n_sel, int64_t pos_base, const vgpu_prog *pred,
                              const vgpu_col *pred_cols, int n_pred_cols) {

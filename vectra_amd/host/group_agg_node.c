@@ -25,6 +25,10 @@ typedef struct {
     DevMem codes;      /* multi-column keys: one tuple code per row */
     VxDevScan ds;      /* open while a .vtr child is fed straight to the device */
     int ds_open;
+    /* string key columns: the batch's offsets, bytes and validity, and the
+     * two hashes and positions computed from them */
+    DevMem s_off[VGPU_MAX_COLS], s_chr[VGPU_MAX_COLS], s_val[VGPU_MAX_COLS];
+    DevMem s_h1[VGPU_MAX_COLS], s_h2[VGPU_MAX_COLS], s_pos[VGPU_MAX_COLS];
 } GroupGpu;
 
 /* file column of a scan's k-th output column (its columns in file order) */
@@ -144,9 +148,14 @@ static int feed_from_scan(GroupAggNode *ga, vgpu_groupby *gb, int key_idx, int n
  * int64 bits) or, for bool, the sum, and the valid count -- which give the
  * group's key values back and prove that no two different tuples shared a
  * code: within a group each column must be all NA or all valid with
- * min == max (sum 0 or all TRUE). In key order (temp_dir set) the groups
- * are then sorted stably by the tuple, each column ascending with NA last
- * (NaN after the numbers, in first-occurrence order). */
+ * min == max (sum 0 or all TRUE). A string column enters the code as a
+ * 64-bit hash of its bytes (vgpu_string_keys) and feeds a second,
+ * independent hash to min / max (one string per group) and the row
+ * position to min: the batches' string columns stay on the host until the
+ * end, and each group's string is read from that row. In key order
+ * (temp_dir set) the groups are then sorted stably by the tuple, each
+ * column ascending with NA last (NaN after the numbers, in
+ * first-occurrence order; strings by bytes, a prefix first, src/sort.c:58-66). */
 #define MK_MAX_KEYS 8
 
 typedef struct {
@@ -154,6 +163,8 @@ typedef struct {
     VecType   type[MK_MAX_KEYS];
     const double  *v[MK_MAX_KEYS];     /* key values (int64 as raw bits) */
     const uint8_t *ok[MK_MAX_KEYS];    /* 1 = valid */
+    const char *const *sp[MK_MAX_KEYS];   /* string keys: bytes and length */
+    const int64_t *sl[MK_MAX_KEYS];
 } TupleCmp;
 
 /* rank within a column: number, NaN, NA */
@@ -167,7 +178,12 @@ static int tuple_cmp(const TupleCmp *t, int64_t a, int64_t b) {
         const int ra = tuple_rank(t, j, a), rb = tuple_rank(t, j, b);
         if (ra != rb) return ra < rb ? -1 : 1;
         if (ra) continue;
-        if (t->type[j] == VEC_INT64) {
+        if (t->type[j] == VEC_STRING) {
+            const int64_t la = t->sl[j][a], lb = t->sl[j][b];
+            const int c = memcmp(t->sp[j][a], t->sp[j][b], (size_t)(la < lb ? la : lb));
+            if (c) return c < 0 ? -1 : 1;
+            if (la != lb) return la < lb ? -1 : 1;
+        } else if (t->type[j] == VEC_INT64) {
             int64_t x, y;
             memcpy(&x, &t->v[j][a], 8);
             memcpy(&y, &t->v[j][b], 8);
@@ -202,6 +218,29 @@ static int slot_for(const VecSchema *cs, int ci, int *in_col, int32_t *in_type, 
     return (*n_in)++;
 }
 
+/* a batch's string key column, kept on the host until the groups' strings
+ * are read from it; base = physical position of its first row */
+typedef struct {
+    VecArray a;
+    int64_t  base;
+} KeptStr;
+
+static int synth_slot(int *in_col, int32_t *in_type, int *n_in, int code) {
+    if (*n_in >= VGPU_MAX_INPUTS) vectra_error("GPU summarise: more than %d input columns", VGPU_MAX_INPUTS);
+    in_col[*n_in] = code;
+    in_type[*n_in] = VGPU_INT64;
+    return (*n_in)++;
+}
+
+static void kept_free(KeptStr **kept, int m, int64_t n) {
+    for (int j = 0; j < m; j++) {
+        if (!kept[j]) continue;
+        for (int64_t i = 0; i < n; i++) vec_array_free(&kept[j][i].a);
+        free(kept[j]);
+        kept[j] = NULL;
+    }
+}
+
 static VecBatch *gpu_group_run_multi(GroupAggNode *ga) {
     GroupGpu *g = (GroupGpu *)ga->gpu;
     const VecSchema *cs = &ga->child->output_schema;
@@ -213,16 +252,16 @@ static VecBatch *gpu_group_run_multi(GroupAggNode *ga) {
         kcol[j] = vec_schema_find_col(cs, ga->key_names[j]);
         if (kcol[j] < 0) vectra_error("group_by: column not found: %s", ga->key_names[j]);
         ktype[j] = cs->col_types[kcol[j]];
-        if (ktype[j] == VEC_STRING)
-            vectra_error("GPU group_by: string key '%s' is not grouped on the GPU", ga->key_names[j]);
     }
+    int n_str = 0;
+    for (int j = 0; j < m; j++) n_str += ktype[j] == VEC_STRING;
     int n_in = 0, in_col[VGPU_MAX_INPUTS];
     int32_t in_type[VGPU_MAX_INPUTS];
     vgpu_agg aggs[VGPU_MAX_AGGS];
-    const int hidden = 1 + 3 * m;
+    const int hidden = 1 + 3 * m + n_str;
     if (ga->n_aggs + hidden > VGPU_MAX_AGGS)
         vectra_error("GPU summarise: %d aggregates over %d grouping columns exceed the GPU path's %d (each "
-                     "grouping column takes 3, plus one)", ga->n_aggs, m, VGPU_MAX_AGGS);
+                     "grouping column takes 3, a string column 4, plus one)", ga->n_aggs, m, VGPU_MAX_AGGS);
     for (int a = 0; a < ga->n_aggs; a++) {
         const AggSpec *sp = &ga->agg_specs[a];
         aggs[a].kind = (int32_t)sp->kind;
@@ -236,13 +275,29 @@ static VecBatch *gpu_group_run_multi(GroupAggNode *ga) {
             vectra_error("GPU summarise: string column '%s' is not aggregated on the GPU", sp->input_col);
         aggs[a].input = slot_for(cs, ci, in_col, in_type, &n_in);
     }
-    for (int j = 0; j < m; j++) kslot[j] = slot_for(cs, kcol[j], in_col, in_type, &n_in);
-    /* hidden aggregates: n, then per key column (min | sum), (max | -), count */
+    /* a string key's slots are its second hash and the row position */
+    int pslot[MK_MAX_KEYS];
+    for (int j = 0; j < m; j++) {
+        if (ktype[j] == VEC_STRING) {
+            kslot[j] = synth_slot(in_col, in_type, &n_in, -2 - 2 * j);
+            pslot[j] = synth_slot(in_col, in_type, &n_in, -3 - 2 * j);
+        } else {
+            kslot[j] = slot_for(cs, kcol[j], in_col, in_type, &n_in);
+            pslot[j] = -1;
+        }
+    }
+    /* hidden aggregates: n, then per key column (min | sum), (max | -),
+     * count, and for strings the smallest row position */
     int na = ga->n_aggs;
     const int a_n = na;
     aggs[na++] = (vgpu_agg){VGPU_AGG_COUNT_STAR, -1, 0, 0};
-    int a_lo[MK_MAX_KEYS], a_hi[MK_MAX_KEYS], a_cnt[MK_MAX_KEYS];
+    int a_lo[MK_MAX_KEYS], a_hi[MK_MAX_KEYS], a_cnt[MK_MAX_KEYS], a_pos[MK_MAX_KEYS];
     for (int j = 0; j < m; j++) {
+        a_pos[j] = -1;
+        if (ktype[j] == VEC_STRING) {
+            a_pos[j] = na;
+            aggs[na++] = (vgpu_agg){VGPU_AGG_MIN, pslot[j], 0, VGPU_AGG_RAW};
+        }
         if (ktype[j] == VEC_BOOL) {
             a_lo[j] = na;
             aggs[na++] = (vgpu_agg){VGPU_AGG_SUM, kslot[j], 0, 0};
@@ -260,12 +315,15 @@ static VecBatch *gpu_group_run_multi(GroupAggNode *ga) {
     if (!gb) vectra_error("GPU: %s", vgpu_last_error());
 
     int64_t pos = 0, phys = 0;
+    KeptStr *kept[MK_MAX_KEYS] = {0};
+    int64_t n_kept = 0, cap_kept = 0;
     VecBatch *b;
     while ((b = ga->child->next_batch(ga->child)) != NULL) {
         const int64_t nl = vec_batch_logical_rows(b);
         if (nl > 0) {
             vgpu_col ins[VGPU_MAX_INPUTS], keys[MK_MAX_KEYS];
             for (int s2 = 0; s2 < n_in; s2++) {
+                if (in_col[s2] < 0) continue;   /* a string key's hash or positions: below */
                 if (b->columns[in_col[s2]].type != (VecType)in_type[s2]) {
                     vgpu_groupby_destroy(gb);
                     vec_batch_free(b);
@@ -274,7 +332,59 @@ static VecBatch *gpu_group_run_multi(GroupAggNode *ga) {
                 devcol_upload(&g->in[s2], &b->columns[in_col[s2]], b->n_rows);
                 ins[s2] = g->in[s2].col;
             }
-            for (int j = 0; j < m; j++) keys[j] = ins[kslot[j]];
+            if (n_str > 0 && n_kept == cap_kept) {
+                cap_kept = cap_kept ? 2 * cap_kept : 16;
+                for (int j = 0; j < m; j++)
+                    if (ktype[j] == VEC_STRING) {
+                        kept[j] = (KeptStr *)realloc(kept[j], (size_t)cap_kept * sizeof(KeptStr));
+                        if (!kept[j]) vectra_error("alloc failed");
+                    }
+            }
+            for (int j = 0; j < m; j++) {
+                if (ktype[j] != VEC_STRING) {
+                    keys[j] = ins[kslot[j]];
+                    continue;
+                }
+                VecArray *sa = &b->columns[kcol[j]];
+                if (sa->type != VEC_STRING) {
+                    vgpu_groupby_destroy(gb);
+                    vec_batch_free(b);
+                    kept_free(kept, m, n_kept);
+                    vectra_error("group_by: column type changed between batches");
+                }
+                const int64_t nb = b->n_rows, bytes = sa->buf.str.offsets[nb];
+                int64_t *d_off = (int64_t *)devmem_reserve(&g->s_off[j], (size_t)(nb + 1) * 8);
+                uint8_t *d_chr = (uint8_t *)devmem_reserve(&g->s_chr[j], (size_t)(bytes > 0 ? bytes : 1));
+                VX_GPU(vgpu_memcpy_h2d(d_off, sa->buf.str.offsets, (size_t)(nb + 1) * 8));
+                if (bytes > 0) VX_GPU(vgpu_memcpy_h2d(d_chr, sa->buf.str.data, (size_t)bytes));
+                uint8_t *d_val = NULL;
+                if (!vx_all_valid(sa->validity, nb)) {
+                    const size_t vb = (size_t)vec_validity_bytes(nb);
+                    d_val = (uint8_t *)devmem_reserve(&g->s_val[j], vb + 8);
+                    VX_GPU(vgpu_memcpy_h2d(d_val, sa->validity, vb));
+                }
+                int64_t *h1 = (int64_t *)devmem_reserve(&g->s_h1[j], (size_t)nb * 8);
+                int64_t *h2 = (int64_t *)devmem_reserve(&g->s_h2[j], (size_t)nb * 8);
+                int64_t *ps = (int64_t *)devmem_reserve(&g->s_pos[j], (size_t)nb * 8);
+                VX_GPU(vgpu_string_keys(d_off, d_chr, d_val, nb, phys, h1, h2, ps));
+                keys[j] = (vgpu_col){VGPU_INT64, 0, h1, d_val};
+                ins[kslot[j]] = (vgpu_col){VGPU_INT64, 0, h2, d_val};
+                ins[pslot[j]] = (vgpu_col){VGPU_INT64, 0, ps, NULL};
+                /* the column moves out of the batch (its bytes are copied
+                 * when the batch only borrows them) */
+                KeptStr *k = &kept[j][n_kept];
+                k->a = *sa;
+                k->base = phys;
+                if (!sa->owns_data) {
+                    char *d = (char *)malloc((size_t)(bytes > 0 ? bytes : 1));
+                    if (!d) vectra_error("alloc failed");
+                    if (bytes > 0) memcpy(d, sa->buf.str.data, (size_t)bytes);
+                    k->a.buf.str.data = d;
+                    k->a.owns_data = 1;
+                }
+                memset(sa, 0, sizeof *sa);
+            }
+            if (n_str > 0) n_kept++;
             int64_t *codes = (int64_t *)devmem_reserve(&g->codes, (size_t)b->n_rows * 8);
             vgpu_col key = {VGPU_INT64, 0, codes, NULL};
             const int32_t *d_sel = NULL;
@@ -298,6 +408,7 @@ static VecBatch *gpu_group_run_multi(GroupAggNode *ga) {
     const int64_t ng = vgpu_groupby_finish(gb);
     if (ng < 0) {
         vgpu_groupby_destroy(gb);
+        kept_free(kept, m, n_kept);
         vectra_error("GPU: %s", vgpu_last_error());
     }
     const size_t cap = (size_t)(ng > 0 ? ng : 1);
@@ -313,28 +424,63 @@ static VecBatch *gpu_group_run_multi(GroupAggNode *ga) {
     const int64_t warns = vgpu_groupby_fetch(gb, codes_h, NULL, vals, valid);
     vgpu_groupby_destroy(gb);
     free(codes_h);
-    if (warns < 0) vectra_error("GPU: %s", vgpu_last_error());
+    if (warns < 0) {
+        kept_free(kept, m, n_kept);
+        vectra_error("GPU: %s", vgpu_last_error());
+    }
     /* the key values, and the check that every group holds one tuple */
     TupleCmp t;
+    memset(&t, 0, sizeof t);
     t.m = m;
     uint8_t *kok[MK_MAX_KEYS];
     double *kv[MK_MAX_KEYS];
+    const char **ksp[MK_MAX_KEYS] = {0};
+    int64_t *ksl[MK_MAX_KEYS] = {0};
     int collided = 0;
     for (int j = 0; j < m; j++) {
         t.type[j] = ktype[j];
         kv[j] = (double *)malloc(cap * 8);
         kok[j] = (uint8_t *)malloc(cap);
         if (!kv[j] || !kok[j]) vectra_error("alloc failed");
+        if (ktype[j] == VEC_STRING) {
+            ksp[j] = (const char **)malloc(cap * sizeof(char *));
+            ksl[j] = (int64_t *)malloc(cap * 8);
+            if (!ksp[j] || !ksl[j]) vectra_error("alloc failed");
+        }
         for (int64_t r = 0; r < ng; r++) {
             const double n = vals[a_n][r], c = vals[a_cnt[j]][r];
             kok[j][r] = c > 0;
             if (c != 0 && c != n) collided = 1;
+            if (ksp[j]) {
+                ksp[j][r] = "";
+                ksl[j][r] = 0;
+            }
             if (c == 0) {
                 kv[j][r] = 0;
                 continue;
             }
             const double lo = vals[a_lo[j]][r];
-            if (ktype[j] == VEC_BOOL) {
+            if (ktype[j] == VEC_STRING) {
+                if (memcmp(&lo, &vals[a_hi[j]][r], 8) != 0) collided = 1;
+                /* the group's string, at its smallest row position */
+                int64_t p;
+                memcpy(&p, &vals[a_pos[j]][r], 8);
+                int64_t lo_i = 0, hi_i = n_kept - 1;
+                while (lo_i < hi_i) {
+                    const int64_t mid = (lo_i + hi_i + 1) / 2;
+                    if (kept[j][mid].base <= p) lo_i = mid;
+                    else hi_i = mid - 1;
+                }
+                const VecArray *sa = n_kept > 0 ? &kept[j][lo_i].a : NULL;
+                const int64_t row = n_kept > 0 ? p - kept[j][lo_i].base : -1;
+                if (!sa || row < 0 || row >= sa->length || !vec_array_is_valid(sa, row)) {
+                    collided = 1;
+                } else {
+                    ksp[j][r] = sa->buf.str.data + sa->buf.str.offsets[row];
+                    ksl[j][r] = sa->buf.str.offsets[row + 1] - sa->buf.str.offsets[row];
+                }
+                kv[j][r] = 0;
+            } else if (ktype[j] == VEC_BOOL) {
                 if (lo != 0 && lo != c) collided = 1;
                 kv[j][r] = lo > 0 ? 1.0 : 0.0;
             } else if (ktype[j] == VEC_INT64) {
@@ -347,12 +493,15 @@ static VecBatch *gpu_group_run_multi(GroupAggNode *ga) {
         }
         t.v[j] = kv[j];
         t.ok[j] = kok[j];
+        t.sp[j] = ksp[j];
+        t.sl[j] = ksl[j];
     }
     if (collided) {
         for (int a = 0; a < na; a++) { free(vals[a]); free(valid[a]); }
-        for (int j = 0; j < m; j++) { free(kv[j]); free(kok[j]); }
+        for (int j = 0; j < m; j++) { free(kv[j]); free(kok[j]); free(ksp[j]); free(ksl[j]); }
         free(vals);
         free(valid);
+        kept_free(kept, m, n_kept);
         vectra_error("GPU group_by: two different key tuples share a 64-bit code (probability about 2^-64 "
                      "per pair of groups)");
     }
@@ -365,7 +514,25 @@ static VecBatch *gpu_group_run_multi(GroupAggNode *ga) {
     VecBatch *res = vec_batch_alloc(m + ga->n_aggs, ng);
     for (int j = 0; j < m; j++) {
         VecArray a = vec_array_alloc(ktype[j], ng);
-        for (int64_t r = 0; r < ng; r++) {
+        if (ktype[j] == VEC_STRING) {
+            int64_t total = 0;
+            for (int64_t r = 0; r < ng; r++)
+                if (kok[j][order[r]]) total += ksl[j][order[r]];
+            a.buf.str.data = (char *)malloc((size_t)(total > 0 ? total : 1));
+            if (!a.buf.str.data) vectra_error("alloc failed");
+            a.buf.str.data_len = total;
+            int64_t at = 0;
+            for (int64_t r = 0; r < ng; r++) {
+                const int64_t s2 = order[r];
+                a.buf.str.offsets[r] = at;
+                if (!kok[j][s2]) continue;
+                vec_array_set_valid(&a, r);
+                memcpy(a.buf.str.data + at, ksp[j][s2], (size_t)ksl[j][s2]);
+                at += ksl[j][s2];
+            }
+            a.buf.str.offsets[ng] = at;
+        }
+        for (int64_t r = 0; r < ng && ktype[j] != VEC_STRING; r++) {
             const int64_t s2 = order[r];
             if (!kok[j][s2]) continue;
             vec_array_set_valid(&a, r);
@@ -388,9 +555,10 @@ static VecBatch *gpu_group_run_multi(GroupAggNode *ga) {
     }
     free(order);
     for (int a = 0; a < na; a++) { free(vals[a]); free(valid[a]); }
-    for (int j = 0; j < m; j++) { free(kv[j]); free(kok[j]); }
+    for (int j = 0; j < m; j++) { free(kv[j]); free(kok[j]); free(ksp[j]); free(ksl[j]); }
     free(vals);
     free(valid);
+    kept_free(kept, m, n_kept);
     for (int64_t w = 0; w < warns; w++)
         vectra_warning("no non-missing arguments to min/max; returning Inf/-Inf");
     return res;
@@ -400,6 +568,10 @@ static VecBatch *gpu_group_run(GroupAggNode *ga) {
     GroupGpu *g = (GroupGpu *)ga->gpu;
     const VecSchema *cs = &ga->child->output_schema;
     if (ga->n_keys > 1) return gpu_group_run_multi(ga);
+    if (ga->n_keys == 1) {
+        const int ki = vec_schema_find_col(cs, ga->key_names[0]);
+        if (ki >= 0 && cs->col_types[ki] == VEC_STRING) return gpu_group_run_multi(ga);
+    }
     int key_idx = -1;
     VecType key_type = VEC_INT64;
     if (ga->n_keys == 1) {
@@ -547,6 +719,14 @@ static void gpu_group_free(VecNode *self) {
         for (int i = 0; i < VGPU_MAX_INPUTS; i++) devcol_free(&g->in[i]);
         devmem_free(&g->sel);
         devmem_free(&g->codes);
+        for (int j = 0; j < VGPU_MAX_COLS; j++) {
+            devmem_free(&g->s_off[j]);
+            devmem_free(&g->s_chr[j]);
+            devmem_free(&g->s_val[j]);
+            devmem_free(&g->s_h1[j]);
+            devmem_free(&g->s_h2[j]);
+            devmem_free(&g->s_pos[j]);
+        }
         if (g->ds_open) vx_devscan_close(&g->ds);
         free(g);
     }

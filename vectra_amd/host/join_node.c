@@ -45,7 +45,32 @@ typedef struct {
     VecType   common[JN_MAX_KEYS];
     DevCol    pcast[JN_MAX_KEYS];
     DevMem    bcode, bvalid, pcode, pvalid;
+    /* string key columns: offsets, bytes, validity and hash (build side
+     * first, then each probe batch) */
+    DevMem    s_off[JN_MAX_KEYS], s_chr[JN_MAX_KEYS], s_val[JN_MAX_KEYS], s_h1[JN_MAX_KEYS];
 } JoinGpu;
+
+/* the rows [0, n) of host string column a as a device int64 column of
+ * their hashes (vgpu_string_keys) with the column's validity: the key that
+ * stands for the string in tuple codes */
+static vgpu_col string_key_col(JoinGpu *g, int k, const VecArray *a, int64_t n) {
+    const int64_t bytes = n > 0 ? a->buf.str.offsets[n] : 0;
+    int64_t *d_off = (int64_t *)devmem_reserve(&g->s_off[k], (size_t)(n + 1) * 8);
+    uint8_t *d_chr = (uint8_t *)devmem_reserve(&g->s_chr[k], (size_t)(bytes > 0 ? bytes : 1));
+    int64_t *h1 = (int64_t *)devmem_reserve(&g->s_h1[k], (size_t)(n > 0 ? n : 1) * 8);
+    uint8_t *d_val = NULL;
+    if (n > 0) {
+        VX_GPU(vgpu_memcpy_h2d(d_off, a->buf.str.offsets, (size_t)(n + 1) * 8));
+        if (bytes > 0) VX_GPU(vgpu_memcpy_h2d(d_chr, a->buf.str.data, (size_t)bytes));
+        if (!vx_all_valid(a->validity, n)) {
+            const size_t vb = (size_t)vec_validity_bytes(n);
+            d_val = (uint8_t *)devmem_reserve(&g->s_val[k], vb + 8);
+            VX_GPU(vgpu_memcpy_h2d(d_val, a->validity, vb));
+        }
+        VX_GPU(vgpu_string_keys(d_off, d_chr, d_val, n, 0, h1, NULL, NULL));
+    }
+    return (vgpu_col){VGPU_INT64, 0, h1, d_val};
+}
 
 /* Several key columns (src/join.c:67-113 hashes and compares the tuple):
  * the table is built on a 64-bit code of each build row's tuple, every
@@ -53,7 +78,9 @@ typedef struct {
  * component left out (they never match); probes use the probe tuples'
  * codes, and every matched pair's tuples are compared afterwards, so a code
  * shared by two different tuples (about 2^-64 per pair) is an error, never
- * a wrong match. */
+ * a wrong match.  A string column (one key or several) enters the codes as
+ * a 64-bit hash of its bytes; the pairs' strings are compared byte for
+ * byte (src/join.c:96-108). */
 static void multi_build_side(JoinNode *jn) {
     JoinGpu *g = (JoinGpu *)jn->gpu;
     const int m = jn->n_keys;
@@ -66,7 +93,11 @@ static void multi_build_side(JoinNode *jn) {
     g->first_only = 0;
     for (int k = 0; k < m; k++) {
         const VecType lt = ls->col_types[jn->keys[k].left_col], rt = rs->col_types[jn->keys[k].right_col];
-        if (lt == VEC_STRING || rt == VEC_STRING) vectra_error("GPU join: string keys are not joined on the GPU");
+        if (lt == VEC_STRING || rt == VEC_STRING) {
+            if (lt != rt) vectra_error("GPU join: string key joined to a %s key", vx_type_name(lt == VEC_STRING ? rt : lt));
+            g->common[k] = VEC_STRING;
+            continue;
+        }
         g->common[k] = vx_common_type(lt, rt);
         /* A left / full join continues its chain walk comparing every probe
          * key in its original type with the coerced build key's bytes
@@ -87,6 +118,10 @@ static void multi_build_side(JoinNode *jn) {
     const size_t nb = (size_t)(g->n_build > 0 ? g->n_build : 1);
     for (int k = 0; k < m; k++) {
         const int rk = jn->keys[k].right_col;
+        if (g->common[k] == VEC_STRING) {
+            keys[k] = string_key_col(g, k, &g->r_host[rk], g->n_build);
+            continue;
+        }
         DevCol *dc = &g->r_dev[rk];
         if ((VecType)dc->col.type != g->common[k]) {
             void *dst = vgpu_malloc(nb * 8);
@@ -110,7 +145,8 @@ static void multi_build_side(JoinNode *jn) {
 
 static void join_build_side(JoinNode *jn) {
     JoinGpu *g = (JoinGpu *)jn->gpu;
-    if (jn->n_keys > 1) {
+    if (jn->n_keys > 1 || jn->left->output_schema.col_types[jn->keys[0].left_col] == VEC_STRING ||
+        jn->right->output_schema.col_types[jn->keys[0].right_col] == VEC_STRING) {
         multi_build_side(jn);
         return;
     }
@@ -228,6 +264,17 @@ static void verify_pairs(JoinNode *jn, const VecBatch *pb, const int32_t *d_prow
     for (int64_t i = 0; i < n && !bad; i++) {
         if (br[i] < 0) continue;
         for (int k = 0; k < jn->n_keys; k++) {
+            if (g->common[k] == VEC_STRING) {
+                const VecArray *pa = &pb->columns[jn->keys[k].left_col], *ba = &g->r_host[jn->keys[k].right_col];
+                if (!vec_array_is_valid(pa, pr[i]) || !vec_array_is_valid(ba, br[i])) {
+                    bad = 1;
+                    continue;
+                }
+                const int64_t ps = pa->buf.str.offsets[pr[i]], pl = pa->buf.str.offsets[pr[i] + 1] - ps;
+                const int64_t bs = ba->buf.str.offsets[br[i]], bl = ba->buf.str.offsets[br[i] + 1] - bs;
+                if (pl != bl || memcmp(pa->buf.str.data + ps, ba->buf.str.data + bs, (size_t)pl) != 0) bad = 1;
+                continue;
+            }
             double pd, bd;
             int64_t pi, bi;
             key_value(&pb->columns[jn->keys[k].left_col], pr[i], g->common[k], &pd, &pi);
@@ -365,6 +412,10 @@ static VecBatch *probe_batch_multi(JoinNode *jn, VecBatch *pb) {
     vgpu_col keys[JN_MAX_KEYS];
     for (int k = 0; k < m; k++) {
         const int lk = jn->keys[k].left_col;
+        if (g->common[k] == VEC_STRING) {
+            keys[k] = string_key_col(g, k, &pb->columns[lk], pb->n_rows);
+            continue;
+        }
         devcol_upload(&g->l_dev[lk], &pb->columns[lk], pb->n_rows);
         keys[k] = g->l_dev[lk].col;
         if ((VecType)keys[k].type != g->common[k]) {
@@ -517,7 +568,13 @@ static void gpu_join_free(VecNode *self) {
         }
         for (int c = 0; c < 64; c++) devcol_free(&g->l_dev[c]);
         devcol_free(&g->pkey_cast);
-        for (int k = 0; k < JN_MAX_KEYS; k++) devcol_free(&g->pcast[k]);
+        for (int k = 0; k < JN_MAX_KEYS; k++) {
+            devcol_free(&g->pcast[k]);
+            devmem_free(&g->s_off[k]);
+            devmem_free(&g->s_chr[k]);
+            devmem_free(&g->s_val[k]);
+            devmem_free(&g->s_h1[k]);
+        }
         devmem_free(&g->bcode);
         devmem_free(&g->bvalid);
         devmem_free(&g->pcode);

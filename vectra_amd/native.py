@@ -34,7 +34,9 @@ JOIN_NAMES = {"inner": JOIN_INNER, "left": JOIN_LEFT, "full": JOIN_FULL, "semi":
 # --------------------------------------------------------------------------
 
 class _Str(C.Structure):
-    _fields_ = [("offsets", C.POINTER(C.c_int64)), ("data", C.c_char_p), ("data_len", C.c_int64)]
+    # data as a plain address: a c_char_p field would be read back as a
+    # temporary bytes object cut at the first NUL
+    _fields_ = [("offsets", C.POINTER(C.c_int64)), ("data", C.c_void_p), ("data_len", C.c_int64)]
 
 
 class _Buf(C.Union):
@@ -387,7 +389,13 @@ class Column:
 
     def __init__(self, values: np.ndarray, valid: np.ndarray | None = None):
         values = np.asarray(values)
-        if values.dtype == np.bool_:
+        if values.dtype.kind in "OU":   # strings: one offsets + data array (utf-8)
+            enc = [str(v).encode() for v in values]
+            self.type = VEC_STRING
+            self.values = np.zeros(len(enc) + 1, np.int64)
+            self.values[1:] = np.cumsum([len(e) for e in enc]) if enc else []
+            self.data = C.create_string_buffer(b"".join(enc), max(1, int(self.values[-1])))
+        elif values.dtype == np.bool_:
             self.type = VEC_BOOL
             self.values = np.ascontiguousarray(values.astype(np.uint8))
         elif np.issubdtype(values.dtype, np.integer):
@@ -398,7 +406,7 @@ class Column:
             self.values = np.ascontiguousarray(values.astype(np.float64))
         else:
             raise TypeError(f"unsupported column dtype {values.dtype}")
-        n = len(self.values)
+        n = len(self.values) - (1 if self.type == VEC_STRING else 0)
         if valid is None:
             self.bitmap = None
         else:
@@ -414,7 +422,12 @@ class Column:
         a.length = self.n
         a.owns_data = 1
         a.validity = self.bitmap.ctypes.data_as(C.POINTER(C.c_uint8)) if self.bitmap is not None else None
-        if self.type == VEC_BOOL:
+        if self.type == VEC_STRING:
+            a.buf.str.offsets = self.values.ctypes.data_as(C.POINTER(C.c_int64))
+            a.buf.str.data = C.addressof(self.data)
+            a.buf.str.data_len = int(self.values[-1])
+            a.owns_data = 0
+        elif self.type == VEC_BOOL:
             a.buf.bln = self.values.ctypes.data_as(C.POINTER(C.c_uint8))
         elif self.type == VEC_INT64:
             a.buf.i64 = self.values.ctypes.data_as(C.POINTER(C.c_int64))
@@ -649,7 +662,7 @@ def batch_to_numpy(b: Any) -> dict[str, tuple[np.ndarray, np.ndarray]]:
             vals = np.ctypeslib.as_array(a.buf.bln, shape=(n,)).copy()
         else:
             offs = np.ctypeslib.as_array(a.buf.str.offsets, shape=(n + 1,))
-            data = C.string_at(C.cast(a.buf.str.data, C.c_void_p), int(offs[-1])) if offs[-1] > 0 else b""
+            data = C.string_at(a.buf.str.data, int(offs[-1])) if offs[-1] > 0 else b""
             vals = np.array([data[offs[i]:offs[i + 1]].decode() for i in range(n)], dtype=object)
         vals = np.where(valid, vals, 0) if a.type != VEC_STRING else vals
         out[name] = (vals, valid)
