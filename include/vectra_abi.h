@@ -239,6 +239,15 @@ JoinNode *join_node_create(VecNode *left, VecNode *right,
                            JoinKind kind, int n_keys, JoinKey *keys,
                            const char *suffix_x, const char *suffix_y);
 
+/* The plan text of a node tree, as explain() prints it (the node lines of
+ * src/r_bridge.c:575-698, C_node_plan without the optimizer pass): one line
+ * per node in pre-order, "<2 spaces per level><kind> [<annotation>]\n",
+ * the reference's annotation for ScanNode / FilterNode / GroupAggNode /
+ * JoinNode, and for GPU nodes "; gpu: ..." saying what the device does.
+ * Writes at most size bytes (NUL-terminated when size > 0) and returns the
+ * length of the whole text, as snprintf does (-1 for a NULL root). */
+int64_t vx_explain(const VecNode *root, char *buf, int64_t size);
+
 /* ---- .vtr files and the scan node ------------------------------------
  * The GPU nodes read a scan child's open file directly (row-group offsets,
  * schema, per-column statistics, the pushed-down predicate), so these

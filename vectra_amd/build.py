@@ -38,7 +38,8 @@ C_FLAGS = ["-std=c11", "-O2", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-paramete
            f"-I{INCLUDE}", f"-I{HOST}"]
 
 HIP_SOURCES = ["vgpu_runtime.hip", "vgpu_filter.hip", "vgpu_groupby.hip", "vgpu_join.hip", "vgpu_batch.hip"]
-NODE_SOURCES = ["gpu_common.c", "devscan.c", "filter_node.c", "group_agg_node.c", "join_node.c", "serve.c"]
+NODE_SOURCES = ["gpu_common.c", "devscan.c", "filter_node.c", "group_agg_node.c", "join_node.c", "serve.c",
+                "explain.c"]
 
 
 def _run(cmd: list[str], verbose: bool) -> None:

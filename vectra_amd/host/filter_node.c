@@ -88,3 +88,8 @@ FilterNode *filter_node_create(VecNode *child, VecExpr *predicate) {
     fn->base.kind = "FilterNode";
     return fn;
 }
+
+int vx_gpu_filter_note(const VecNode *n, char *buf, int size) {
+    if (n->next_batch != gpu_filter_next) return -1;
+    return snprintf(buf, (size_t)size, "gpu: predicate on the device, selection vector");
+}

@@ -775,3 +775,22 @@ GroupAggNode *group_agg_node_create(VecNode *child, int n_keys, char **key_names
     ga->base.kind = "GroupAggNode";
     return ga;
 }
+
+int vx_gpu_group_note(const VecNode *n, char *buf, int size) {
+    if (n->next_batch != gpu_group_next) return -1;
+    const GroupAggNode *ga = (const GroupAggNode *)n;
+    const VecSchema *cs = &ga->child->output_schema;
+    int strs = 0;
+    for (int k = 0; k < ga->n_keys; k++) {
+        const int i = vec_schema_find_col(cs, ga->key_names[k]);
+        strs += i >= 0 && cs->col_types[i] == VEC_STRING;
+    }
+    const int tuple = ga->n_keys > 1 || strs > 0;
+    const VecNode *c = ga->child;
+    const int filtered = c->kind && strcmp(c->kind, "FilterNode") == 0;
+    if (filtered) c = ((const FilterNode *)c)->child;
+    const int feed = !tuple && c->kind && strcmp(c->kind, "ScanNode") == 0 && vgpu_tune_get("scan.device_feed") > 1;
+    return snprintf(buf, (size_t)size, "gpu: %s group-by%s, %s%s", tuple ? "tuple-code" : "hash",
+                    strs ? " (string keys hashed on the device)" : "", ga->use_sorted ? "key order" : "first-seen order",
+                    feed ? (filtered ? ", device scan feed with the filter in the feed" : ", device scan feed") : "");
+}

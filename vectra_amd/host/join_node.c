@@ -661,3 +661,14 @@ JoinNode *join_node_create(VecNode *left, VecNode *right, JoinKind kind, int n_k
     jn->base.kind = "JoinNode";
     return jn;
 }
+
+int vx_gpu_join_note(const VecNode *n, char *buf, int size) {
+    if (n->next_batch != gpu_join_next) return -1;
+    const JoinNode *jn = (const JoinNode *)n;
+    int strs = 0;
+    for (int k = 0; k < jn->n_keys; k++)
+        strs += jn->left->output_schema.col_types[jn->keys[k].left_col] == VEC_STRING;
+    const int tuple = jn->n_keys > 1 || strs > 0;
+    return snprintf(buf, (size_t)size, "gpu: %s table built on the device%s, probe per batch",
+                    tuple ? "tuple-code" : "hash", strs ? " (string keys hashed on the device)" : "");
+}

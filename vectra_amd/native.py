@@ -314,8 +314,20 @@ def nodes() -> C.CDLL:
         _proto(lib, "vx_serve_run", C.c_int, [C.POINTER(VxServeCfg), C.POINTER(VxRequest), C.c_int,
                                               C.POINTER(VxResult), C.POINTER(VxServeStats)])
         _proto(lib, "vx_result_free", None, [C.POINTER(VxResult)])
+        _proto(lib, "vx_explain", C.c_int64, [NODE_P, C.c_char_p, C.c_int64])
         _libs["nodes"] = lib
     return _libs["nodes"]
+
+
+def explain(root: Any) -> list[str]:
+    """The plan lines of a node tree (vx_explain; src/r_bridge.c:575-698)."""
+    lib = nodes()
+    n = lib.vx_explain(root, None, 0)
+    if n < 0:
+        raise VectraError("explain: no node")
+    buf = C.create_string_buffer(int(n) + 1)
+    lib.vx_explain(root, buf, int(n) + 1)
+    return buf.value.decode().splitlines()
 
 
 def host_eval_batches(table: "Table", pred: tuple) -> list[np.ndarray]:
