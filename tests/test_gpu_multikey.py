@@ -125,3 +125,24 @@ def test_multikey_join_matches_oracle(gpu, oracle, kind, types):
     for g, e in zip(got, exp):
         assert g["sel"] is None and e["sel"] is None
         assert_tables_equal(g["cols"], e["cols"])
+
+
+def test_multikey_key_order_many_groups(gpu, oracle):
+    """key order over about 150,000 groups of (int64, double, bool) tuples
+    with NA in every column: the group order is sorted on the device (a
+    stable sort per column and class), against the oracle's merge sort"""
+    r = rng(97)
+    n = 400_000
+    a = r.integers(-5000, 5000, n) * 3
+    b = np.round(r.normal(size=n) * 8) / 4
+    z = r.random(n) < 0.05
+    b[z] = np.where(r.random(z.sum()) < 0.5, 0.0, -0.0)
+    cols = {"a": (a, with_na(r, n, 0.02)), "b": (b, with_na(r, n, 0.02)), "c": (r.random(n) < 0.5, with_na(r, n, 0.1)),
+            "x": r.normal(size=n), "v": r.integers(-2**40, 2**40, n)}
+    cols["ax"] = np.abs(cols["x"])
+    aggs = [("n", "n", None, 0), ("sx", "sum", "x", 0), ("hv", "max", "v", 0)]
+    got = run(gpu, "gpu", cols, ["a", "b", "c"], "/tmp", 100_000, None, aggs)
+    exp = run(gpu, "oracle", cols, ["a", "b", "c"], "/tmp", 100_000, None, aggs + [("__sa", "sum", "ax", 0)])
+    assert len(got["n"][0]) > 100_000
+    sa = exp.pop("__sa")[0]
+    assert_tables_equal(got, exp, {"sx": sa})

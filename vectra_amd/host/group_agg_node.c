@@ -209,6 +209,69 @@ static void tuple_sort(const TupleCmp *t, int64_t *idx, int64_t *tmp, int64_t n)
     }
 }
 
+/* The same order on the device for numeric key columns: a stable LSD sort
+ * of the group order (first-seen to begin with), column by column from the
+ * last, each by its value -- int64 with the sign bit flipped, double by its
+ * IEEE bits mapped to an unsigned order (-0 as +0), bool 0 / 1 -- and then,
+ * where the column has NaN or NA, by its class (number, NaN, NA).  Groups
+ * equal on a column keep their relative order, as in tuple_sort.  Returns 0
+ * (nothing done) for string keys and for fewer than DEV_SORT_MIN groups. */
+#define DEV_SORT_MIN 1024
+
+static uint64_t order_bits(const TupleCmp *t, int j, int64_t g) {
+    if (t->type[j] == VEC_BOOL) return t->v[j][g] > 0;
+    uint64_t u;
+    if (t->type[j] == VEC_INT64) {
+        memcpy(&u, &t->v[j][g], 8);
+        return u ^ 0x8000000000000000ull;
+    }
+    const double x = t->v[j][g] == 0.0 ? 0.0 : t->v[j][g];
+    memcpy(&u, &x, 8);
+    return (u >> 63) ? ~u : u | 0x8000000000000000ull;
+}
+
+static int device_tuple_order(const TupleCmp *t, int64_t *order, int64_t n) {
+    if (n < DEV_SORT_MIN || n >= ((int64_t)1 << 32)) return 0;
+    for (int j = 0; j < t->m; j++)
+        if (t->type[j] == VEC_STRING) return 0;
+    uint64_t *hk = (uint64_t *)malloc((size_t)n * 8);
+    uint32_t *hp = (uint32_t *)malloc((size_t)n * 4);
+    uint64_t *dk = (uint64_t *)vgpu_malloc((size_t)n * 8);
+    uint32_t *dp = (uint32_t *)vgpu_malloc((size_t)n * 4);
+    if (!hk || !hp || !dk || !dp) {
+        free(hk);
+        free(hp);
+        if (dk) vgpu_free(dk);
+        if (dp) vgpu_free(dp);
+        return 0;   /* the host sort takes it */
+    }
+    for (int64_t i = 0; i < n; i++) hp[i] = (uint32_t)order[i];
+    int rc = 0;
+    for (int j = t->m - 1; j >= 0 && rc == 0; j--) {
+        for (int pass = 0; pass < 2 && rc == 0; pass++) {
+            int special = 0;
+            for (int64_t i = 0; i < n; i++) {
+                const int r = tuple_rank(t, j, hp[i]);
+                special |= r != 0;
+                hk[i] = pass == 0 ? (r == 0 ? order_bits(t, j, hp[i]) : 0) : (uint64_t)r;
+            }
+            if (pass == 1 && !special) continue;
+            const int end_bit = pass == 0 ? (t->type[j] == VEC_BOOL ? 1 : 64) : 2;
+            if (vgpu_memcpy_h2d(dk, hk, (size_t)n * 8) < 0 || vgpu_memcpy_h2d(dp, hp, (size_t)n * 4) < 0 ||
+                vgpu_sort_pairs_u64(dk, dp, n, 0, end_bit) < 0 || vgpu_memcpy_d2h(hp, dp, (size_t)n * 4) < 0)
+                rc = -1;
+        }
+    }
+    vgpu_free(dk);
+    vgpu_free(dp);
+    free(hk);
+    if (rc == 0)
+        for (int64_t i = 0; i < n; i++) order[i] = hp[i];
+    free(hp);
+    if (rc) vectra_error("GPU: %s", vgpu_last_error());
+    return 1;
+}
+
 static int slot_for(const VecSchema *cs, int ci, int *in_col, int32_t *in_type, int *n_in) {
     for (int s = 0; s < *n_in; s++)
         if (in_col[s] == ci) return s;
@@ -509,7 +572,7 @@ static VecBatch *gpu_group_run_multi(GroupAggNode *ga) {
     int64_t *tmp = (int64_t *)malloc(cap * sizeof(int64_t));
     if (!order || !tmp) vectra_error("alloc failed");
     for (int64_t r = 0; r < ng; r++) order[r] = r;
-    if (ga->use_sorted) tuple_sort(&t, order, tmp, ng);
+    if (ga->use_sorted && !device_tuple_order(&t, order, ng)) tuple_sort(&t, order, tmp, ng);
     free(tmp);
     VecBatch *res = vec_batch_alloc(m + ga->n_aggs, ng);
     for (int j = 0; j < m; j++) {
