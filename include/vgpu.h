@@ -316,6 +316,16 @@ typedef struct {
 
 int vgpu_groupby_feed_join(vgpu_groupby *g, const vgpu_feed_spec *spec);
 
+/* Of n join pairs (probe_rows[i], build_rows[i]; device; build row < 0 =
+ * no match), the number whose key tuples differ -- a column NA on either
+ * side or unequal values (double by ==) -- into *n_unequal.  Each column's
+ * probe and build sides have one type (int64 / double / bool), the pair's
+ * common type.  The check of a join on tuple codes (src/join.c:77-113 is
+ * the comparison it restates): a nonzero count means two tuples shared a
+ * code. */
+int vgpu_pairs_unequal(const vgpu_col *probe_keys, const vgpu_col *build_keys, int n_keys,
+                       const int32_t *probe_rows, const int64_t *build_rows, int64_t n, int64_t *n_unequal);
+
 /* ---- device radix sort (exposed for tests) ----------------------------
  * Stable ascending sort of n (key, value) pairs on key bits
  * [begin_bit, end_bit), device buffers, in place. */

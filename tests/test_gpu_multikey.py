@@ -146,3 +146,31 @@ def test_multikey_key_order_many_groups(gpu, oracle):
     assert len(got["n"][0]) > 100_000
     sa = exp.pop("__sa")[0]
     assert_tables_equal(got, exp, {"sx": sa})
+
+
+def test_pairs_unequal_counts_differing_tuples(gpu):
+    """the device pair check of joins on tuple codes: a pair counts when a
+    column is NA on either side or its values differ (double by ==: -0
+    equals +0, NaN equals nothing); pairs without a build row are skipped"""
+    import ctypes as C
+    lib = gpu.vgpu()
+    pa = gpu.DeviceColumn(np.array([1, 2, 3, 4, 5, 6]), np.array([1, 1, 1, 0, 1, 1], bool))
+    pb = gpu.DeviceColumn(np.array([0.0, 1.5, -0.0, 2.0, np.nan, 7.0]))
+    ba = gpu.DeviceColumn(np.array([1, 2, 3, 4, 5, 9]))
+    bb = gpu.DeviceColumn(np.array([0.0, 1.5, 0.0, 2.0, np.nan, 7.0]))
+    pr = np.array([0, 1, 2, 3, 4, 5, 5], np.int32)
+    br = np.array([0, 1, 2, 3, 4, 5, -1], np.int64)
+    dpr, dbr = lib.vgpu_malloc(pr.nbytes), lib.vgpu_malloc(br.nbytes)
+    assert lib.vgpu_memcpy_h2d(dpr, pr.ctypes.data, pr.nbytes) == 0
+    assert lib.vgpu_memcpy_h2d(dbr, br.ctypes.data, br.nbytes) == 0
+    P = (gpu.VgpuCol * 2)(pa.col, pb.col)
+    B = (gpu.VgpuCol * 2)(ba.col, bb.col)
+    out = C.c_int64(-1)
+    assert lib.vgpu_pairs_unequal(P, B, 2, dpr, dbr, len(pr), C.byref(out)) == 0, lib.vgpu_last_error()
+    # pair 3: probe NA; pair 4: NaN; pair 5: 6 vs 9; pair 6: no build row; pair 2: -0 == +0
+    assert out.value == 3
+    assert lib.vgpu_pairs_unequal(P, B, 1, dpr, dbr, 3, C.byref(out)) == 0 and out.value == 0
+    for c in (pa, pb, ba, bb):
+        c.free()
+    lib.vgpu_free(dpr)
+    lib.vgpu_free(dbr)

@@ -527,6 +527,73 @@ extern "C" int vgpu_gather(const vgpu_col *src, const void *idx, int idx_is_64, 
     return 0;
 }
 
+// Pairs of a join on tuple codes whose tuples differ: a pair counts when a
+// key column is NA on either side or its values differ (double by ==, so -0
+// equals +0; int64 and bool by their bits). Both sides of a column carry the
+// pair's common type.
+#define VG_MAX_PAIR_KEYS 8
+struct PairKeys {
+    vgpu_col p[VG_MAX_PAIR_KEYS], b[VG_MAX_PAIR_KEYS];
+    int32_t n;
+};
+
+__global__ void __launch_bounds__(256) k_pairs_unequal(PairKeys K, const int32_t *prow, const int64_t *brow, int64_t n,
+                                                       unsigned long long *bad) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    unsigned long long local = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int64_t b = brow[i];
+        if (b < 0) continue;
+        const int64_t p = prow[i];
+        bool eq = true;
+        for (int j = 0; j < K.n && eq; j++) {
+            if (!vg_valid(K.p[j].validity, p) || !vg_valid(K.b[j].validity, b)) {
+                eq = false;
+                break;
+            }
+            const uint64_t x = vg_load_bits(K.p[j], p), y = vg_load_bits(K.b[j], b);
+            eq = K.p[j].type == VGPU_DOUBLE ? vg_bits_to_f64(x) == vg_bits_to_f64(y) : x == y;
+        }
+        local += !eq;
+    }
+    if (local) atomicAdd(bad, local);
+}
+
+extern "C" int vgpu_pairs_unequal(const vgpu_col *probe_keys, const vgpu_col *build_keys, int n_keys,
+                                  const int32_t *probe_rows, const int64_t *build_rows, int64_t n, int64_t *n_unequal) {
+    if (vg_ensure_init()) return -1;
+    if (n_keys < 1 || n_keys > VG_MAX_PAIR_KEYS) {
+        vg_set_error("pairs_unequal: %d key columns (1..%d)", n_keys, VG_MAX_PAIR_KEYS);
+        return -1;
+    }
+    PairKeys K;
+    memset(&K, 0, sizeof K);
+    for (int j = 0; j < n_keys; j++) {
+        const int t = probe_keys[j].type;
+        if (t != build_keys[j].type || (t != VGPU_INT64 && t != VGPU_DOUBLE && t != VGPU_BOOL)) {
+            vg_set_error("pairs_unequal: key %d has types %d and %d", j, t, build_keys[j].type);
+            return -1;
+        }
+        K.p[j] = probe_keys[j];
+        K.b[j] = build_keys[j];
+    }
+    K.n = n_keys;
+    *n_unequal = 0;
+    if (n <= 0) return 0;
+    hipStream_t s = vg_stream();
+    DevBuf cnt;
+    if (cnt.reserve(8)) return -1;
+    VG_TRY(hipMemsetAsync(cnt.ptr, 0, 8, s));
+    k_pairs_unequal<<<vg_blocks(n, 256, 4096), 256, 0, s>>>(K, probe_rows, build_rows, n,
+                                                             cnt.as<unsigned long long>());
+    VG_LAUNCH_CHECK();
+    unsigned long long h = 0;
+    VG_TRY(hipMemcpyAsync(&h, cnt.ptr, 8, hipMemcpyDeviceToHost, s));
+    VG_TRY(hipStreamSynchronize(s));
+    *n_unequal = (int64_t)h;
+    return 0;
+}
+
 __global__ void k_cast(vgpu_col src, int64_t n, int dst_type, uint64_t *dst) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;

@@ -252,9 +252,22 @@ static void key_value(const VecArray *a, int64_t r, VecType common, double *d, i
 
 static VecBatch *probe_batch_multi(JoinNode *jn, VecBatch *pb);
 
-/* every matched pair (build row >= 0) joins equal tuples */
-static void verify_pairs(JoinNode *jn, const VecBatch *pb, const int32_t *d_prow, const int64_t *d_brow, int64_t n) {
+/* every matched pair (build row >= 0) joins equal tuples: on the device
+ * when every key column is a number (pkeys: the probe's key columns in the
+ * common types), else on the host */
+static void verify_pairs(JoinNode *jn, const VecBatch *pb, const vgpu_col *pkeys, const int32_t *d_prow,
+                         const int64_t *d_brow, int64_t n) {
     JoinGpu *g = (JoinGpu *)jn->gpu;
+    int strings = 0;
+    for (int k = 0; k < jn->n_keys; k++) strings |= g->common[k] == VEC_STRING;
+    if (!strings) {
+        vgpu_col bkeys[JN_MAX_KEYS];
+        for (int k = 0; k < jn->n_keys; k++) bkeys[k] = g->r_dev[jn->keys[k].right_col].col;
+        int64_t bad = 0;
+        VX_GPU(vgpu_pairs_unequal(pkeys, bkeys, jn->n_keys, d_prow, d_brow, n, &bad));
+        if (bad) vectra_error("GPU join: two different key tuples share a 64-bit code (probability about 2^-64 per pair)");
+        return;
+    }
     int32_t *pr = (int32_t *)malloc((size_t)(n > 0 ? n : 1) * 4);
     int64_t *br = (int64_t *)malloc((size_t)(n > 0 ? n : 1) * 8);
     if (!pr || !br) vectra_error("alloc failed");
@@ -443,7 +456,7 @@ static VecBatch *probe_batch_multi(JoinNode *jn, VecBatch *pb) {
     const int64_t *d_brow = NULL;
     if (n_out > 0) {
         vgpu_join_pairs(g->jt, &d_prow, &d_brow);
-        verify_pairs(jn, pb, d_prow, d_brow, n_out);
+        verify_pairs(jn, pb, keys, d_prow, d_brow, n_out);
     }
     if (!semi_anti) return n_out > 0 ? gather_output(jn, pb, d_prow, d_brow, n_out) : NULL;
     /* probe rows with a pair, in probe order */

@@ -260,6 +260,8 @@ def vgpu() -> C.CDLL:
                                              C.c_void_p])
         _proto(lib, "vgpu_cast", C.c_int, [C.POINTER(VgpuCol), C.c_int64, C.c_int, C.c_void_p])
         _proto(lib, "vgpu_sort_pairs_u64", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_int])
+        _proto(lib, "vgpu_pairs_unequal", C.c_int, [C.POINTER(VgpuCol), C.POINTER(VgpuCol), C.c_int, C.c_void_p,
+                                                    C.c_void_p, C.c_int64, C.POINTER(C.c_int64)])
         _proto(lib, "vgpu_timer_start", C.c_int, [])
         _proto(lib, "vgpu_timer_stop", C.c_double, [])
         _proto(lib, "vgpu_tune_set", C.c_int, [C.c_char_p, C.c_int64])
