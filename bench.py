@@ -200,6 +200,88 @@ def run_c2(args, nv):
 
 # ---------------------------------------------------------------- c1: drop-in nodes
 
+def run_strings(args, nv):
+    """group_by(s, k) |> summarise(n = n(), total = sum(x)) and
+    inner_join(dim, by = "s") |> group_by(w) |> summarise(n = n()) over
+    string keys: 10M fact rows in host row groups of 1M (s from 20,000
+    strings of 6-24 bytes, k U{0..9}, x N(0,1)), a 20,000-row dim table with
+    an int64 w; through the drop-in GroupAggNode and JoinNode (key strings
+    staged over PCIe and hashed on the device)."""
+    import ctypes as C
+    n = args.str_rows
+    r = np.random.default_rng(0x7665637472610600)
+    pool = np.array([f"customer/{i:05d}/" + "abcdefgh"[: i % 9] for i in range(20_000)], dtype=object)
+    s = pool[r.integers(0, len(pool), n)]
+    cols = {"s": s, "k": r.integers(0, 10, n), "x": r.normal(size=n)}
+    fact = nv.Table(cols, rg_rows=1_000_000)
+    dim = nv.Table({"s": pool[r.permutation(len(pool))], "w": r.integers(0, 100, len(pool))})
+    str_bytes = int(sum(len(v) for v in pool[r.integers(0, len(pool), 10_000)]) / 10_000 * n)
+
+    def group(engine, tbl):
+        return nv.collect(nv.group_agg_node(engine, tbl.scan(), ["s", "k"], [("n", "n", None, 0),
+                                                                             ("total", "sum", "x", 0)], None))
+
+    def join(engine, tbl):
+        j = nv.join_node(engine, tbl.scan(), dim.scan(), "inner", [(0, 0)])
+        return nv.collect(nv.group_agg_node(engine, j, ["w"], [("n", "n", None, 0)], "/tmp"))
+
+    lib = nv.vgpu()
+    for _ in range(args.warmup):
+        g, jn = group("gpu", fact), join("gpu", fact)
+    assert int(g["n"][0].sum()) == n and int(jn["n"][0].sum()) == n
+    res = {}
+    for name, fn in (("group", group), ("join", join)):
+        lib.vgpu_synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn("gpu", fact)
+        lib.vgpu_synchronize()
+        res[name] = (time.perf_counter() - t0) * 1e3 / args.steps
+        lib.vgpu_kernel_timing(1)
+        lib.vgpu_kernel_time_ms(None, 1)
+        fn("gpu", fact)
+        nl = C.c_int64()
+        res[name + "_k"] = (lib.vgpu_kernel_time_ms(C.byref(nl), 1), nl.value)
+        lib.vgpu_kernel_timing(0)
+    log(f"[bench] strings: group_by(s, k) {res['group']:.1f} ms ({n / res['group'] / 1e3:.0f} M rows/s, kernels "
+        f"{res['group_k'][0]:.2f} ms in {res['group_k'][1]} launches); join on s + group_by(w) {res['join']:.1f} ms "
+        f"({n / res['join'] / 1e3:.0f} M rows/s, kernels {res['join_k'][0]:.2f} ms in {res['join_k'][1]} launches); "
+        f"about {str_bytes / 1e6:.0f} MB of key strings")
+    cpu = None
+    if not args.no_cpu:
+        nv.load_oracle(ROOT / "oracle" / "_build" / "liboracle.so")
+        m = min(n, 2_000_000)
+        sample = nv.Table({k: v[:m] for k, v in cols.items()}, rg_rows=1_000_000)
+        t0 = time.perf_counter()
+        group("oracle", sample)
+        join("oracle", sample)
+        secs = time.perf_counter() - t0
+        cpu = {"value": m / secs * 2, "unit": "rows/s", "cores": 1, "kind": "port",
+               "sample": f"the first {m:,} fact rows through both queries (oracle restatement of the reference "
+                         f"engine, {secs:.2f} s); rows/s counts each query's rows"}
+    ms = res["group"] + res["join"]
+    kms = res["group_k"][0] + res["join_k"][0]
+    # algorithmic bytes of both queries: the key strings and their offsets
+    # read once per query, k and x (group) and the dim table (join)
+    alg = 2 * (str_bytes + 8 * n) + 16 * n + 20_000 * 32
+    achieved = alg / (kms / 1e3) / 1e9
+    return {
+        "metric": METRIC, "value": 2 * n / (ms / 1e3), "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded numpy generators)",
+        "config": {"workload": "string keys: group_by(s, k) |> summarise(n, sum(x)) and inner_join(dim, by = s) |> "
+                               "group_by(w) |> summarise(n); 10M fact rows in host row groups of 1M, 20,000 distinct "
+                               "strings; drop-in nodes, key strings staged over PCIe",
+                   "rows": n, "distinct_strings": 20_000},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_string_keys + k_key_codes + group-by / join kernels, all launches of both queries",
+                     "bytes_per_launch": alg / max(res["group_k"][1] + res["join_k"][1], 1),
+                     "avg_launch_ms": kms / max(res["group_k"][1] + res["join_k"][1], 1)},
+        "cpu_baseline": cpu,
+    }
+
+
 def run_c1(args, nv):
     """filter(x > 0) |> group_by(key) |> summarise(total = sum(value), n = n(),
     avg = mean(value)) over 1M rows in 10 row groups of 100,000, through the
@@ -749,7 +831,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c1-vtr", "c2", "c3-high", "c3-low", "c4", "c5"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c1-vtr", "c2", "c3-high", "c3-low", "c4", "c5",
+                                                             "strings"])
     ap.add_argument("--c4-queries", type=int, default=4096)
     ap.add_argument("--c4-card", type=int, default=0, help="fix every query's key cardinality (0: log-uniform)")
     ap.add_argument("--c5-rows", type=int, default=1_000_000_000)
@@ -761,6 +844,7 @@ def main():
     ap.add_argument("--c5-batched-rates", type=float, nargs="*", default=[50, 100, 200, 350, 500, 650, 800, 1000])
     ap.add_argument("--c3-rows", type=int, default=200_000_000)
     ap.add_argument("--vtr-rows", type=int, default=50_000_000)
+    ap.add_argument("--str-rows", type=int, default=10_000_000)
     ap.add_argument("--vtr-rg", type=int, default=1_000_000)
     ap.add_argument("--vtr-path", default="/tmp/vectra_bench_c1.vtr")
     ap.add_argument("--fact-rows", type=int, default=50_000_000)
@@ -797,6 +881,8 @@ def main():
         log(f"[bench] knob {name} = {val}")
     if args.workload == "c1":
         out = run_c1(args, nv)
+    elif args.workload == "strings":
+        out = run_strings(args, nv)
     elif args.workload == "c1-vtr":
         out = run_c1_vtr(args, nv)
     elif args.workload == "c3-high":
