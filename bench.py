@@ -204,7 +204,7 @@ def run_strings(args, nv):
     """group_by(s, k) |> summarise(n = n(), total = sum(x)) and
     inner_join(dim, by = "s") |> group_by(w) |> summarise(n = n()) over
     string keys: 10M fact rows in host row groups of 1M (s from 20,000
-    strings of 6-24 bytes, k U{0..9}, x N(0,1)), a 20,000-row dim table with
+    strings of 15-23 bytes, k U{0..9}, x N(0,1)), a 20,000-row dim table with
     an int64 w; through the drop-in GroupAggNode and JoinNode (key strings
     staged over PCIe and hashed on the device)."""
     import ctypes as C
