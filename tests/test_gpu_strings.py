@@ -139,3 +139,36 @@ def test_string_key_against_number_is_an_error(gpu):
     R = gpu.Table({"name": np.array([1.]), "y": np.array([2.])})
     with pytest.raises(gpu.VectraError, match="key type mismatch"):
         gpu.join_node("gpu", L.scan(), R.scan(), "inner", [(0, 0)])
+
+
+def test_string_keys_long_and_shared_prefix(gpu, oracle):
+    """strings up to 6,000 bytes that differ only in their last byte or
+    their length, in both orders"""
+    r = rng(171)
+    base = "p" * 5999
+    pool = np.array([base + c for c in "abc"] + [base, base[:3000], "", "q"], dtype=object)
+    n = 20_000
+    cols = {"g": (pool[r.integers(0, len(pool), n)], with_na(r, n, 0.05)), "x": r.normal(size=n)}
+    cols["ax"] = np.abs(cols["x"])
+    for td in (None, "/tmp"):
+        got = group(gpu, "gpu", cols, ["g"], td, 3_000, None, [("n", "n", None, 0), ("s", "sum", "x", 0)])
+        exp = group(gpu, "oracle", cols, ["g"], td, 3_000, None, [("n", "n", None, 0), ("s", "sum", "x", 0),
+                                                                  ("__sa", "sum", "ax", 0)])
+        sa = exp.pop("__sa")[0]
+        assert_tables_equal(got, exp, {"s": sa})
+
+
+@pytest.mark.parametrize("kind", ["inner", "left", "full", "semi", "anti"])
+def test_string_join_empty_sides(gpu, oracle, kind):
+    """an empty build side and an empty probe side"""
+    full = {"name": np.array(["a", "b", "a"]), "x": np.array([1., 2., 3.])}
+    empty = {"name": np.array([], dtype=object), "y": np.array([], dtype=np.float64)}
+    for lcols, rcols in ((full, empty), ({"name": np.array([], dtype=object), "x": np.array([], np.float64)},
+                                         {"name": np.array(["a"]), "y": np.array([5.])})):
+        out = {}
+        for engine in ("gpu", "oracle"):
+            node = gpu.join_node(engine, gpu.Table(lcols).scan(), gpu.Table(rcols).scan(), kind, [(0, 0)])
+            out[engine] = gpu.next_batches(node)
+        assert len(out["gpu"]) == len(out["oracle"])
+        for g, e in zip(out["gpu"], out["oracle"]):
+            assert_tables_equal(g["cols"], e["cols"])
