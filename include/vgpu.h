@@ -124,6 +124,23 @@ int vgpu_filter(const vgpu_prog *prog, const vgpu_col *cols, int n_cols,
                 int64_t n_rows, const int32_t *sel_in, int64_t n_sel_in,
                 int32_t *sel_out, int64_t *n_out);
 
+/* The same filter without any allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Its
+ * scratch (ballot mask, tile counts and offsets, scan partials; about
+ * 8.5 MB) is a fixed device array that each GPU holds from vgpu_init until
+ * the process ends; vgpu_init must have run on the device (it looks the
+ * array up), and the call allocates nothing -- no hipMalloc / hipFree, no
+ * pool growth.  At most VGPU_FILTER_NOALLOC_MAX_ROWS logical rows per call
+ * (more is an error).  Calls share the array: the library serializes its
+ * own callers, but work that overlaps a call on the same GPU from outside
+ * the library (another process's stream excepted, which has its own copy)
+ * must not use it.  Results equal vgpu_filter's. */
+#define VGPU_FILTER_NOALLOC_MAX_ROWS (1 << 26)
+int vgpu_filter_noalloc(const vgpu_prog *prog, const vgpu_col *cols, int n_cols,
+                        int64_t n_rows, const int32_t *sel_in, int64_t n_sel_in,
+                        int32_t *sel_out, int64_t *n_out);
+
 /* Host execution of the same program interpreter and fast path, over HOST
  * columns: a test hook that lets the expression compiler and the device
  * evaluator's semantics be checked without a GPU.  No operator node calls

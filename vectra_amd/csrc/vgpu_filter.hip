@@ -156,56 +156,167 @@ extern "C" int vgpu_eval_host(const vgpu_prog *prog, const vgpu_col *cols, int n
 static std::mutex g_filter_mu;
 static DevBuf g_mask, g_counts, g_offs, g_scan, g_total;
 
-extern "C" int vgpu_filter(const vgpu_prog *prog, const vgpu_col *cols, int n_cols, int64_t n_rows,
-                           const int32_t *sel_in, int64_t n_sel_in, int32_t *sel_out, int64_t *n_out) {
-    if (vg_ensure_init()) return -1;
+// argument checks shared by both filter entry points
+static int filter_check(const char *who, const vgpu_prog *prog, int n_cols) {
     if (!prog || prog->n_insn <= 0 || prog->n_insn > VGPU_MAX_INSN) {
-        vg_set_error("vgpu_filter: invalid program");
+        vg_set_error("%s: invalid program", who);
         return -1;
     }
     if (n_cols < 0 || n_cols > VGPU_MAX_COLS) {
-        vg_set_error("vgpu_filter: %d columns exceeds %d", n_cols, VGPU_MAX_COLS);
+        vg_set_error("%s: %d columns exceeds %d", who, n_cols, VGPU_MAX_COLS);
         return -1;
     }
     for (int i = 0; i < prog->n_insn; i++) {
         const vgpu_insn &in = prog->insn[i];
         if (in.op == VGPU_OP_COL && (in.col < 0 || in.col >= n_cols)) {
-            vg_set_error("vgpu_filter: instruction %d reads column slot %d of %d", i, in.col, n_cols);
+            vg_set_error("%s: instruction %d reads column slot %d of %d", who, i, in.col, n_cols);
             return -1;
         }
         if (in.op >= VGPU_OP_CAST && (in.a >= i || (in.op >= VGPU_OP_ADD && in.op <= VGPU_OP_OR && in.b >= i))) {
-            vg_set_error("vgpu_filter: instruction %d reads a later register", i);
+            vg_set_error("%s: instruction %d reads a later register", who, i);
             return -1;
         }
     }
-    std::lock_guard<std::mutex> lk(g_filter_mu);
-    int64_t n_logical = sel_in ? n_sel_in : n_rows;
-    *n_out = 0;
-    if (n_logical <= 0) return 0;
+    return 0;
+}
+
+// mask -> tile counts -> scan -> ordered write, all into the given buffers
+// (mask: n_words words, counts: n_tiles, offs: n_tiles, total: 1 word);
+// scan_tmp / scan_buf is the scan's scratch (one of the two)
+static int filter_enqueue(const vgpu_prog *prog, const vgpu_col *cols, int n_cols, int64_t n_logical,
+                          const int32_t *sel_in, int32_t *sel_out, uint64_t *mask, uint32_t *counts,
+                          uint64_t *offs, uint64_t *total, void *scan_tmp, size_t scan_cap, DevBuf *scan_buf,
+                          bool timed) {
     ColSet cs;
     memset(&cs, 0, sizeof cs);
     for (int i = 0; i < n_cols; i++) cs.c[i] = cols[i];
     PredInfo pi = vg_pred_info(prog);
     int64_t n_tiles = (n_logical + FT_TILE - 1) / FT_TILE;
+    hipStream_t s = vg_stream();
+    size_t lds = pi.fast ? 0 : vg_vm_lds_bytes(prog, FT_THREADS);
+    if (timed) vg_kt_begin();
+    k_filter_mask<<<(unsigned)n_tiles, FT_THREADS, lds, s>>>(pi, *prog, cs, n_logical, sel_in, mask, counts);
+    VG_LAUNCH_CHECK();
+    if (timed) vg_kt_end();
+    int rc = scan_buf ? vg_scan_u32_to_u64(counts, offs, n_tiles, total, *scan_buf)
+                      : vg_scan_u32_to_u64_in(counts, offs, n_tiles, total, scan_tmp, scan_cap);
+    if (rc) return -1;
+    k_filter_write<<<(unsigned)n_tiles, FT_THREADS, 0, s>>>(n_logical, sel_in, mask, offs, sel_out);
+    VG_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int vgpu_filter(const vgpu_prog *prog, const vgpu_col *cols, int n_cols, int64_t n_rows,
+                           const int32_t *sel_in, int64_t n_sel_in, int32_t *sel_out, int64_t *n_out) {
+    if (vg_ensure_init()) return -1;
+    if (filter_check("vgpu_filter", prog, n_cols)) return -1;
+    std::lock_guard<std::mutex> lk(g_filter_mu);
+    int64_t n_logical = sel_in ? n_sel_in : n_rows;
+    *n_out = 0;
+    if (n_logical <= 0) return 0;
+    int64_t n_tiles = (n_logical + FT_TILE - 1) / FT_TILE;
     int64_t n_words = (n_logical + 63) / 64;
     if (g_mask.reserve((size_t)n_words * 8) || g_counts.reserve((size_t)n_tiles * 4) ||
         g_offs.reserve((size_t)n_tiles * 8) || g_total.reserve(8))
         return -1;
-    hipStream_t s = vg_stream();
-    size_t lds = pi.fast ? 0 : vg_vm_lds_bytes(prog, FT_THREADS);
-    vg_kt_begin();
-    k_filter_mask<<<(unsigned)n_tiles, FT_THREADS, lds, s>>>(pi, *prog, cs, n_logical, sel_in,
-                                                              g_mask.as<uint64_t>(), g_counts.as<uint32_t>());
-    VG_LAUNCH_CHECK();
-    vg_kt_end();
-    if (vg_scan_u32_to_u64(g_counts.as<uint32_t>(), g_offs.as<uint64_t>(), n_tiles, g_total.as<uint64_t>(), g_scan))
+    if (filter_enqueue(prog, cols, n_cols, n_logical, sel_in, sel_out, g_mask.as<uint64_t>(),
+                       g_counts.as<uint32_t>(), g_offs.as<uint64_t>(), g_total.as<uint64_t>(), nullptr, 0,
+                       &g_scan, true))
         return -1;
-    k_filter_write<<<(unsigned)n_tiles, FT_THREADS, 0, s>>>(n_logical, sel_in, g_mask.as<uint64_t>(),
-                                                             g_offs.as<uint64_t>(), sel_out);
-    VG_LAUNCH_CHECK();
+    hipStream_t s = vg_stream();
     uint64_t total = 0;
     VG_TRY(hipMemcpyAsync(&total, g_total.ptr, 8, hipMemcpyDeviceToHost, s));
     VG_TRY(hipStreamSynchronize(s));
     *n_out = (int64_t)total;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// The filter without allocation.  Its scratch -- the ballot mask, the tile
+// counts, their offsets, the total and the scan's partials -- lives in one
+// file-scope device array, `scratch`, sized for VGPU_FILTER_NOALLOC_MAX_ROWS
+// logical rows.  Each device holds its own copy of the array; its address is
+// looked up once per device by vgpu_init (hipGetSymbolAddress) and only that
+// address is handed to kernels.  Every scratch byte a call reads is written
+// by the same call first (mask words of the logical rows, one count and one
+// offset per tile, the total), so nothing has to start at zero.
+// ---------------------------------------------------------------------------
+
+#define NA_MAX_ROWS ((int64_t)VGPU_FILTER_NOALLOC_MAX_ROWS)
+#define NA_MAX_TILES ((NA_MAX_ROWS + FT_TILE - 1) / FT_TILE)
+#define NA_MASK_OFF ((size_t)0)
+#define NA_COUNTS_OFF (NA_MASK_OFF + (size_t)(NA_MAX_ROWS / 64) * 8)
+#define NA_OFFS_OFF (NA_COUNTS_OFF + (((size_t)NA_MAX_TILES * 4 + 255) & ~(size_t)255))
+#define NA_TOTAL_OFF (NA_OFFS_OFF + (size_t)NA_MAX_TILES * 8)
+#define NA_SCAN_OFF (NA_TOTAL_OFF + 256)
+#define NA_SCAN_BYTES ((size_t)65536)
+#define NA_SCRATCH_BYTES (NA_SCAN_OFF + NA_SCAN_BYTES)
+
+static __device__ __align__(256) uint8_t scratch[NA_SCRATCH_BYTES];
+
+#define NA_MAX_DEVICES 64
+static void *g_na_scratch[NA_MAX_DEVICES];   // per device: the address of `scratch`
+static uint64_t *g_na_total[NA_MAX_DEVICES];  // per device: pinned host word for the count
+static int g_na_device = -1;                  // device of the last set-up (the library's device)
+static std::mutex g_na_mu;
+
+int vg_filter_noalloc_setup(int device) {
+    if (device < 0 || device >= NA_MAX_DEVICES) {
+        vg_set_error("vgpu_init: device %d beyond the %d devices the no-allocation filter supports", device,
+                     NA_MAX_DEVICES);
+        return -1;
+    }
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (!g_na_scratch[device]) {
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            vg_set_error("vgpu_init: address of the filter scratch array on device %d: %s", device,
+                         hipGetErrorString(e));
+            return -1;
+        }
+        void *host = nullptr;
+        e = hipHostMalloc(&host, 64, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            vg_set_error("vgpu_init: pinned word for the filter count: %s", hipGetErrorString(e));
+            return -1;
+        }
+        g_na_total[device] = static_cast<uint64_t *>(host);
+        g_na_scratch[device] = ptr;
+    }
+    g_na_device = device;
+    return 0;
+}
+
+extern "C" int vgpu_filter_noalloc(const vgpu_prog *prog, const vgpu_col *cols, int n_cols, int64_t n_rows,
+                                   const int32_t *sel_in, int64_t n_sel_in, int32_t *sel_out, int64_t *n_out) {
+    int dev = g_na_device;
+    if (dev < 0 || !g_na_scratch[dev]) {
+        vg_set_error("vgpu_filter_noalloc: call vgpu_init first (set-up allocates, this entry point does not)");
+        return -1;
+    }
+    if (filter_check("vgpu_filter_noalloc", prog, n_cols)) return -1;
+    int64_t n_logical = sel_in ? n_sel_in : n_rows;
+    if (n_logical > NA_MAX_ROWS) {
+        vg_set_error("vgpu_filter_noalloc: %lld logical rows exceed the maximum of %lld (VGPU_FILTER_NOALLOC_MAX_ROWS)",
+                     (long long)n_logical, (long long)NA_MAX_ROWS);
+        return -1;
+    }
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    *n_out = 0;
+    if (n_logical <= 0) return 0;
+    uint8_t *base = static_cast<uint8_t *>(g_na_scratch[dev]);
+    if (filter_enqueue(prog, cols, n_cols, n_logical, sel_in, sel_out,
+                       reinterpret_cast<uint64_t *>(base + NA_MASK_OFF),
+                       reinterpret_cast<uint32_t *>(base + NA_COUNTS_OFF),
+                       reinterpret_cast<uint64_t *>(base + NA_OFFS_OFF),
+                       reinterpret_cast<uint64_t *>(base + NA_TOTAL_OFF), base + NA_SCAN_OFF, NA_SCAN_BYTES,
+                       nullptr, false))
+        return -1;
+    hipStream_t s = vg_stream();
+    uint64_t *total = g_na_total[dev];
+    VG_TRY(hipMemcpyAsync(total, base + NA_TOTAL_OFF, 8, hipMemcpyDeviceToHost, s));
+    VG_TRY(hipStreamSynchronize(s));
+    *n_out = (int64_t)*total;
     return 0;
 }

@@ -120,6 +120,14 @@ struct DevBuf {
 // *total (device, may be null) gets the sum.  tmp is scratch.
 int vg_scan_u32_to_u64(const uint32_t *in, uint64_t *out, int64_t n,
                        uint64_t *total, DevBuf &tmp);
+// the same over caller-provided scratch of tmp_bytes bytes (device): no
+// allocation; fails when vg_scan_tmp_bytes(n) exceeds tmp_bytes
+int vg_scan_u32_to_u64_in(const uint32_t *in, uint64_t *out, int64_t n,
+                          uint64_t *total, void *tmp, size_t tmp_bytes);
+size_t vg_scan_tmp_bytes(int64_t n);
+// per-device set-up of the no-allocation filter (vgpu_filter.hip), called by
+// vgpu_init with the device current
+int vg_filter_noalloc_setup(int device);
 // device-wide exclusive scan of n uint64 values (in may equal out)
 int vg_scan_u64(const uint64_t *in, uint64_t *out, int64_t n, uint64_t *total,
                 DevBuf &tmp);

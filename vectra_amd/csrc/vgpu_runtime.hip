@@ -52,6 +52,7 @@ extern "C" int vgpu_init(int device) {
         return -1;
     }
     if (!g_stream) VG_TRY(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    if (vg_filter_noalloc_setup(device)) return -1;
     // an autotuner table named by the environment applies from the start
     const char *tuning = getenv("VECTRA_GPU_TUNING");
     if (tuning && *tuning && vgpu_tune_load(tuning) < 0) return -1;
@@ -578,6 +579,20 @@ static size_t scan_tmp_bytes(int64_t n) {
         b += (size_t)n * 8 + 64;
     }
     return b * 2 + 4096;
+}
+
+size_t vg_scan_tmp_bytes(int64_t n) { return scan_tmp_bytes(n); }
+
+int vg_scan_u32_to_u64_in(const uint32_t *in, uint64_t *out, int64_t n, uint64_t *total, void *tmp,
+                          size_t tmp_bytes) {
+    if (scan_tmp_bytes(n) > tmp_bytes) {
+        vg_set_error("scan: scratch of %zu bytes too small for %lld values", tmp_bytes, (long long)n);
+        return -1;
+    }
+    DevBuf view;   // borrowed: never reserved or released
+    view.ptr = tmp;
+    view.cap = tmp_bytes;
+    return scan_impl<uint32_t>(in, out, n, total, view, 0);
 }
 
 int vg_scan_u32_to_u64(const uint32_t *in, uint64_t *out, int64_t n, uint64_t *total, DevBuf &tmp) {

@@ -235,6 +235,8 @@ def vgpu() -> C.CDLL:
         _proto(lib, "vgpu_synchronize", C.c_int, [])
         _proto(lib, "vgpu_filter", C.c_int, [C.POINTER(VgpuProg), C.POINTER(VgpuCol), C.c_int, C.c_int64, C.c_void_p,
                                              C.c_int64, C.c_void_p, C.POINTER(C.c_int64)])
+        _proto(lib, "vgpu_filter_noalloc", C.c_int, [C.POINTER(VgpuProg), C.POINTER(VgpuCol), C.c_int, C.c_int64,
+                                                     C.c_void_p, C.c_int64, C.c_void_p, C.POINTER(C.c_int64)])
         _proto(lib, "vgpu_eval_host", C.c_int, [C.POINTER(VgpuProg), C.POINTER(VgpuCol), C.c_int, C.c_int64,
                                                 C.c_void_p, C.c_int64, C.c_void_p, C.POINTER(C.c_int64)])
         _proto(lib, "vgpu_groupby_create", C.c_void_p, [C.c_int, C.c_int, C.POINTER(C.c_int32), C.c_int,
