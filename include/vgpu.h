@@ -263,13 +263,18 @@ vgpu_jtable *vgpu_join_build_payload(const vgpu_col *key, int64_t n_build,
 /* Probes the logical rows of a batch.  For each probe row in logical order
  * the output holds its matches in descending build row order; LEFT/FULL
  * append the batch's unmatched probe rows after all matches; SEMI/ANTI
- * emit matching / non-matching probe rows once.  first_match_only limits
- * LEFT/FULL matches to the first one (the reference's behaviour when the
- * probe key type differs from the build key type).  Results stay on the
- * device: vgpu_join_pairs() returns them. */
+ * emit matching / non-matching probe rows once.  uncoerced_key (device,
+ * optional) is the probe key column before its coercion to the build key
+ * type (int64 or bool; probe_key holds the coerced values): LEFT/FULL then
+ * keep, after each row's first match, only the build rows whose key bytes
+ * equal the uncoerced key's -- an int64 probe against the build double's
+ * 64 bits, a bool probe against byte `build_row` of the build key array --
+ * the reference's chain walk over uncoerced probe columns
+ * (src/join.c:398-411).  Results stay on the device: vgpu_join_pairs()
+ * returns them. */
 int vgpu_join_probe(vgpu_jtable *t, int kind, const vgpu_col *probe_key,
                     int64_t n_rows, const int32_t *sel, int64_t n_sel,
-                    int first_match_only, int64_t *n_out);
+                    const vgpu_col *uncoerced_key, int64_t *n_out);
 
 /* Device pointers to the last probe's output: probe physical rows and
  * build rows (-1 = no build row). Valid until the next probe/destroy. */
@@ -331,6 +336,11 @@ typedef struct {
     const vgpu_input *inputs;      /* host array */
 } vgpu_feed_spec;
 
+/* Order of the pairs (first-seen groups, NaN / zero-sign min and max): a
+ * pair's position is pos_base + (unmatched << 43 | probe logical row << 12
+ * | match index), so one feed takes fewer than 2^31 probe rows and a build
+ * key of at most 4,096 rows (more is an error), and a caller that feeds
+ * several batches advances pos_base by 2^44 per feed. */
 int vgpu_groupby_feed_join(vgpu_groupby *g, const vgpu_feed_spec *spec);
 
 /* Of n join pairs (probe_rows[i], build_rows[i]; device; build row < 0 =

@@ -13,6 +13,7 @@
 #include "vx_core.h"
 
 #include <math.h>
+#include <setjmp.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -105,13 +106,15 @@ static VecArray *arith(const VecArray *l, const VecArray *r, char op) {
             case '+': o->buf.i64[i] = (int64_t)(a + b); break;
             case '-': o->buf.i64[i] = (int64_t)(a - b); break;
             case '*': o->buf.i64[i] = (int64_t)(a * b); break;
+            /* INT64_MIN / -1 overflows: undefined in the reference (x86
+             * traps); it wraps here, and INT64_MIN % -1 is 0 */
             case '/':
                 if (sb == 0) vec_array_set_null(o, i);
-                else o->buf.i64[i] = sa / sb;
+                else o->buf.i64[i] = sb == -1 ? (int64_t)(0 - a) : sa / sb;
                 break;
             case '%':
                 if (sb == 0) vec_array_set_null(o, i);
-                else o->buf.i64[i] = sa % sb;
+                else o->buf.i64[i] = sb == -1 ? 0 : sa % sb;
                 break;
             default: vectra_error("unknown arith op: %c", op);
             }
@@ -327,6 +330,20 @@ VecArray *oracle_expr_eval(const VecExpr *e, const VecBatch *b) {
         vectra_error("oracle: expression kind %d not on the hot path", (int)e->kind);
     }
 }
+
+int oracle_try_expr_eval(const VecExpr *e, const VecBatch *b, VecArray **out) {
+    vx_frame f;
+    *out = NULL;
+    vx_frame_push(&f);
+    if (setjmp(f.jb) == 0) {
+        *out = oracle_expr_eval(e, b);
+        vx_frame_pop(&f);
+        return 0;
+    }
+    return -1;
+}
+
+void oracle_array_free(VecArray *a) { drop_array(a); }
 
 /* ==================================================================== */
 /* filter (src/filter.c:13-84)                                           */

@@ -25,6 +25,10 @@ extern "C" {
 /* Expression evaluation over all physical rows of a batch
  * (src/expr.c:91-146, src/scalar_ops.c, src/coerce.c). */
 VecArray *oracle_expr_eval(const VecExpr *expr, const VecBatch *batch);
+/* The same, returning -1 (message in vx_last_error) where the evaluation
+ * raises vectra_error; *out is released with oracle_array_free. */
+int oracle_try_expr_eval(const VecExpr *expr, const VecBatch *batch, VecArray **out);
+void oracle_array_free(VecArray *arr);
 
 VecNode *oracle_filter_node_create(VecNode *child, VecExpr *predicate);
 VecNode *oracle_group_agg_node_create(VecNode *child, int n_keys,

@@ -175,3 +175,24 @@ def test_radix_sort_pairs(gpu, n, bits):
     order = np.argsort(keys, kind="stable")
     assert np.array_equal(ok, keys[order])
     assert np.array_equal(ov, vals[order])
+
+
+def test_fused_too_many_matches_is_an_error(gpu):
+    """a build key with more than 4,096 rows: the fused feed cannot keep the
+    pairs' output order and says so instead of merging positions"""
+    nv = gpu
+    n_dim = 5000
+    dim_key = np.zeros(n_dim, np.int64)          # one key, 5,000 rows
+    dim_key[-1] = 1
+    fkey = np.array([0, 1, 2], np.int64)
+    value = np.ones(3)
+    cols = [nv.DeviceColumn(a) for a in (fkey, value, dim_key, np.arange(n_dim))]
+    fk, val, dk, at = cols
+    jt = nv.JoinTable(dk)
+    gb = nv.GroupBy(nv.VEC_INT64, [nv.VEC_DOUBLE], [("sum", 0, 0)], nv.VGPU_ORDER_FIRST_SEEN)
+    with pytest.raises(nv.VectraError, match="at most 4096 matches per probe row"):
+        gb.feed_join(3, [fk, val], [dk, at], [(0, 1)], key=(1, 1), join=jt, join_kind=nv.JOIN_INNER)
+    gb.free()
+    jt.free()
+    for c in cols:
+        c.free()

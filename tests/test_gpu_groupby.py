@@ -293,3 +293,35 @@ def test_groupby_partitioned_after_filter(gpu, oracle, part_threshold):
     """a selection vector from the filter feeds the partitioned path"""
     mk = lambda: table(gpu, 2_400_000, 31, "int", 50_000, 0, p_na=0.01, p_nan=0.01)
     check(gpu, mk, ["g"], ALL_AGGS, "/tmp", pred=("cmp", ">", ("col", "ax"), ("dbl", 0.3)))
+
+
+@pytest.mark.parametrize("shape", ["general", "narrow", "multi", "partitioned"])
+def test_nan_keys_in_key_order_are_an_error(gpu, oracle, shape):
+    """NaN group keys with key order: the reference's merge sort ranks NaN
+    equal to every key (src/sort.c:32-47, 97-116), so NaN groups land where
+    its merge pattern puts them; the GPU node raises an error instead of
+    returning another order, on every feed path.  The hash path (temp_dir
+    None) groups NaN keys one row each, as the other tests check."""
+    nv = gpu
+    n = 1_200_000 if shape == "partitioned" else 40_000
+    mk = lambda: table(nv, n, 41, "dbl", 4000 if shape == "partitioned" else 8, 0, p_na=0.02, p_nan=0.01)
+    keys = ["g", "k2"] if shape == "multi" else ["g"]
+    aggs = NARROW_AGGS if shape in ("narrow", "multi") else ALL_AGGS
+
+    def tbl():
+        t = mk()
+        if shape == "multi":
+            t.cols["k2"] = nv.Column(np.arange(n) % 3)
+        return t
+    old = nv.vgpu().vgpu_tune_get(b"gb.part.min_groups")
+    if shape == "partitioned":
+        assert nv.vgpu().vgpu_tune_set(b"gb.part.min_groups", 1000) == 0
+    try:
+        with pytest.raises(nv.VectraError, match="NaN values in a double group key with key order"):
+            run(nv, "gpu", tbl(), keys, aggs, "/tmp")
+        # the same rows on the hash path are computed and match the oracle
+        got = run(nv, "gpu", tbl(), keys, aggs, None)
+        exp = run(nv, "oracle", tbl(), keys, aggs, None)
+        assert np.array_equal(got["n"][0], exp["n"][0])
+    finally:
+        nv.vgpu().vgpu_tune_set(b"gb.part.min_groups", old)

@@ -68,6 +68,32 @@ def test_join_key_types(gpu, oracle, kind, lt, rt):
     same(run(gpu, "gpu", mk, kind), run(gpu, "oracle", mk, kind))
 
 
+@pytest.mark.parametrize("kind", ["inner", "left", "full", "semi", "anti"])
+@pytest.mark.parametrize("types", [("bool", "int"), ("bool", "dbl"), ("int", "dbl")])
+def test_join_coerced_chain_rule(gpu, oracle, kind, types):
+    """a probe key coerced to the build key's type: left / full joins keep
+    a later duplicate build row only when the uncoerced probe key equals the
+    build key array's bytes -- a bool probe against byte `row` of the build
+    array, an int64 probe against the build double's bits (src/join.c:398-411)"""
+    lt, rt = types
+
+    def mk():
+        r = rng({("bool", "int"): 1, ("bool", "dbl"): 2, ("int", "dbl"): 3}[types] + 17)
+        nl, nr = 5000, 700
+        lk = (r.random(nl) < 0.5) if lt == "bool" else r.integers(-2, 4, nl)
+        if rt == "dbl":
+            rk = r.choice(np.array([0.0, -0.0, 1.0, 2.0, -1.0, 0.5, np.nan]), nr)
+        else:
+            rk = r.integers(-1, 3, nr)
+        left = gpu.Table({"id": (lk, with_na(r, nl, 0.05)), "x": r.normal(size=nl)}, rg_rows=1024)
+        # build keys all valid: the value bytes of NA build rows are not defined by the reference
+        right = gpu.Table({"y": r.integers(0, 10**6, nr), "id": rk, "w": r.random(nr)})
+        return left, right
+    pred = ("cmp", ">", ("col", "x"), ("dbl", -0.5))
+    same(run(gpu, "gpu", mk, kind), run(gpu, "oracle", mk, kind))
+    same(run(gpu, "gpu", mk, kind, pred), run(gpu, "oracle", mk, kind, pred))
+
+
 def test_join_unique_dimension(gpu, oracle):
     """fact -> unique-key dimension (the join-then-aggregate shape)."""
     def mk():

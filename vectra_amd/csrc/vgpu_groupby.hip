@@ -422,9 +422,12 @@ __device__ __forceinline__ void jf_emit(const GbLayout &L, const JoinFeedArgs &A
 // Output position of a join pair, order-preserving within one feed call:
 // matched pairs (probe row li, j-th match) precede the batch's unmatched
 // tail, as in the reference's left join output (src/join.c:439-448).
+// A feed holds fewer than 2^31 probe rows and build keys of at most
+// JF_MAX_MATCHES rows (both checked on the host), so positions of one feed
+// are distinct and below pos_base + 2^44.
+#define JF_MAX_MATCHES 4096
 __device__ __forceinline__ uint64_t jf_pos(uint64_t base, int64_t li, uint32_t j, bool tail) {
-    uint32_t jj = j < 0xFFFu ? j : 0xFFFu;
-    return base + ((tail ? (1ull << 43) : 0ull) | ((uint64_t)li << 12) | jj);
+    return base + ((tail ? (1ull << 43) : 0ull) | ((uint64_t)li << 12) | (j & 0xFFFu));
 }
 
 __global__ void __launch_bounds__(GB_THREADS)
@@ -2219,6 +2222,18 @@ static int groupby_feed_join_body(vgpu_groupby *g, const vgpu_feed_spec *sp) {
     if (sp->join) {
         if (sp->join_kind != VGPU_JOIN_INNER && sp->join_kind != VGPU_JOIN_LEFT) {
             vg_set_error("groupby_feed_join: join kind %d not fused (inner and left only)", sp->join_kind);
+            return -1;
+        }
+        // a pair's position keeps 12 bits for its match index (jf_pos): more
+        // matches per probe row would tie positions, which decide first-seen
+        // order and the NaN / zero-sign min / max rules
+        if (sp->join->max_dup > JF_MAX_MATCHES) {
+            vg_set_error("groupby_feed_join: a build key has %d rows; the fused join -> aggregate orders at most "
+                         "%d matches per probe row", (int)sp->join->max_dup, JF_MAX_MATCHES);
+            return -1;
+        }
+        if (n_logical > ((int64_t)1 << 31)) {
+            vg_set_error("groupby_feed_join: %lld probe rows in one feed exceed 2^31", (long long)n_logical);
             return -1;
         }
         if (!check_ref(0, sp->probe_key_col)) {

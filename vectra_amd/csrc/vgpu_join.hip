@@ -310,8 +310,23 @@ extern "C" int64_t vgpu_join_build_rows(const vgpu_jtable *t) { return t ? t->n_
 // probe
 // ---------------------------------------------------------------------------
 
+// A left / full probe whose key had to be coerced to the build key's type
+// continues the reference's chain walk after the first match comparing the
+// ORIGINAL probe key with the build key array's bytes (src/join.c:398-411
+// pass the uncoerced probe columns to jht_chain_next, whose join_keys_equal
+// switches on the probe column's type): an int64 probe against the coerced
+// double build key read as int64, a bool probe's byte against byte `br` of
+// the build key array.  Every later row of the segment holds the same key
+// value, so this comparison alone decides whether it is still emitted.
+__device__ __forceinline__ bool jp_chain_keep(const vgpu_col &orig, int64_t pr, const uint64_t *bkey, uint32_t br) {
+    if (!bkey) return false;
+    if (orig.type == VGPU_BOOL)
+        return static_cast<const uint8_t *>(orig.data)[pr] == reinterpret_cast<const uint8_t *>(bkey)[br];
+    return static_cast<const uint64_t *>(orig.data)[pr] == bkey[br];
+}
+
 __global__ void __launch_bounds__(JT_THREADS)
-k_jp_count(JView v, vgpu_col key, int64_t n_logical, const int32_t *sel, int kind, int first_only,
+k_jp_count(JView v, vgpu_col key, int64_t n_logical, const int32_t *sel, int kind, int coerced, vgpu_col orig,
            uint32_t *cnt_main, uint32_t *un, uint32_t *segoff, uint32_t *segcnt) {
     int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (li >= n_logical) return;
@@ -324,12 +339,9 @@ k_jp_count(JView v, vgpu_col key, int64_t n_logical, const int32_t *sel, int kin
     case VGPU_JOIN_ANTI: m = c == 0; break;
     default:
         m = c;
-        if (first_only && c > 1) {
-            // the continued chain walk compares the int64 probe value with
-            // the build double's raw bits: only 0 against +0.0 still matches
+        if (coerced && c > 1) {
             m = 1;
-            if (vg_load_bits(key, pr) == 0 && v.bkey)
-                for (uint32_t j = 1; j < c; j++) m += v.bkey[v.rows[off + j]] == 0;
+            for (uint32_t j = 1; j < c; j++) m += jp_chain_keep(orig, pr, v.bkey, v.rows[off + j]);
         }
         break;
     }
@@ -340,10 +352,10 @@ k_jp_count(JView v, vgpu_col key, int64_t n_logical, const int32_t *sel, int kin
 }
 
 __global__ void __launch_bounds__(JT_THREADS)
-k_jp_write(JView v, int64_t n_logical, const int32_t *sel, int kind, const uint32_t *cnt_main,
-           const uint64_t *off_main, const uint32_t *un, const uint64_t *off_un, const uint64_t *totals,
-           const uint32_t *segoff, const uint32_t *segcnt, int32_t *out_probe, int64_t *out_build,
-           uint8_t *bmatched) {
+k_jp_write(JView v, int64_t n_logical, const int32_t *sel, int kind, int coerced, vgpu_col orig,
+           const uint32_t *cnt_main, const uint64_t *off_main, const uint32_t *un, const uint64_t *off_un,
+           const uint64_t *totals, const uint32_t *segoff, const uint32_t *segcnt, int32_t *out_probe,
+           int64_t *out_build, uint8_t *bmatched) {
     int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (li >= n_logical) return;
     int32_t pr = sel ? sel[li] : (int32_t)li;
@@ -357,9 +369,9 @@ k_jp_write(JView v, int64_t n_logical, const int32_t *sel, int kind, const uint3
     uint32_t c = segcnt[li];
     for (uint32_t j = 0, k = 0; j < c && k < m; j++) {
         uint32_t br = v.rows[so + j];
-        // rows past the first are skipped exactly when the count pass
-        // did not count them (first-match mode, see k_jp_count)
-        if (m < c && j > 0 && !(v.bkey && v.bkey[br] == 0)) continue;
+        // rows past the first are emitted exactly when the count pass
+        // counted them (k_jp_count, same rule)
+        if (coerced && j > 0 && !jp_chain_keep(orig, pr, v.bkey, br)) continue;
         out_probe[o + k] = pr;
         out_build[o + k] = (int64_t)br;
         k++;
@@ -373,7 +385,7 @@ k_jp_write(JView v, int64_t n_logical, const int32_t *sel, int kind, const uint3
 }
 
 static int join_probe_body(vgpu_jtable *t, int kind, const vgpu_col *probe_key, int64_t n_rows, const int32_t *sel,
-                           int64_t n_sel, int first_match_only, int64_t *n_out) {
+                           int64_t n_sel, const vgpu_col *uncoerced_key, int64_t *n_out) {
     if (kind < VGPU_JOIN_INNER || kind > VGPU_JOIN_ANTI) {
         vg_set_error("join_probe: unknown join kind %d", kind);
         return -1;
@@ -381,6 +393,22 @@ static int join_probe_body(vgpu_jtable *t, int kind, const vgpu_col *probe_key, 
     if (probe_key->type != t->key_type) {
         vg_set_error("join_probe: probe key type %d differs from build key type %d", probe_key->type, t->key_type);
         return -1;
+    }
+    // the chain rule of a coerced probe applies to left / full joins only
+    // (inner joins walk the chain on the coerced key, src/join.c:388-395)
+    int coerced = uncoerced_key && (kind == VGPU_JOIN_LEFT || kind == VGPU_JOIN_FULL);
+    vgpu_col orig;
+    memset(&orig, 0, sizeof orig);
+    if (coerced) {
+        orig = *uncoerced_key;
+        if (orig.type != VGPU_INT64 && orig.type != VGPU_BOOL) {
+            vg_set_error("join_probe: an uncoerced probe key is int64 or bool (got type %d)", orig.type);
+            return -1;
+        }
+        if (!t->bkey) {
+            vg_set_error("join_probe: a coerced probe needs an int64 or double build key");
+            return -1;
+        }
     }
     hipStream_t s = vg_stream();
     int64_t n = sel ? n_sel : n_rows;
@@ -400,7 +428,7 @@ static int join_probe_body(vgpu_jtable *t, int kind, const vgpu_col *probe_key, 
     JView v = t->view();
     vgpu_col k = *probe_key;
     unsigned grid = vg_blocks(n, JT_THREADS, 1 << 30);
-    k_jp_count<<<grid, JT_THREADS, 0, s>>>(v, k, n, sel, kind, first_match_only, t->cnt.as<uint32_t>(),
+    k_jp_count<<<grid, JT_THREADS, 0, s>>>(v, k, n, sel, kind, coerced, orig, t->cnt.as<uint32_t>(),
                                            tail ? t->un.as<uint32_t>() : nullptr, segoff, segcnt);
     VG_LAUNCH_CHECK();
     uint64_t *tot = t->totals.as<uint64_t>();
@@ -417,7 +445,8 @@ static int join_probe_body(vgpu_jtable *t, int kind, const vgpu_col *probe_key, 
     if (t->out_probe.reserve((size_t)std::max<int64_t>(total, 1) * 4) ||
         t->out_build.reserve((size_t)std::max<int64_t>(total, 1) * 8))
         return -1;
-    k_jp_write<<<grid, JT_THREADS, 0, s>>>(v, n, sel, kind, t->cnt.as<uint32_t>(), t->off_main.as<uint64_t>(),
+    k_jp_write<<<grid, JT_THREADS, 0, s>>>(v, n, sel, kind, coerced, orig, t->cnt.as<uint32_t>(),
+                                           t->off_main.as<uint64_t>(),
                                            tail ? t->un.as<uint32_t>() : nullptr, t->off_un.as<uint64_t>(), tot, segoff,
                                            segcnt, t->out_probe.as<int32_t>(), t->out_build.as<int64_t>(),
                                            kind == VGPU_JOIN_FULL ? t->bmatched.as<uint8_t>() : nullptr);
@@ -429,9 +458,9 @@ static int join_probe_body(vgpu_jtable *t, int kind, const vgpu_col *probe_key, 
 }
 
 extern "C" int vgpu_join_probe(vgpu_jtable *t, int kind, const vgpu_col *probe_key, int64_t n_rows, const int32_t *sel,
-                               int64_t n_sel, int first_match_only, int64_t *n_out) {
+                               int64_t n_sel, const vgpu_col *uncoerced_key, int64_t *n_out) {
     return vg_guard<int>(-1, "vgpu_join_probe", [&] { return join_probe_body(t, kind, probe_key, n_rows, sel, n_sel,
-        first_match_only, n_out); });
+        uncoerced_key, n_out); });
 }
 
 extern "C" int vgpu_join_pairs(vgpu_jtable *t, const int32_t **probe_rows, const int64_t **build_rows) {

@@ -15,8 +15,19 @@
  */
 #include "devscan.h"
 
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
+
+/* The reference sorts the rows of a key-ordered group-by with a top-down
+ * merge sort whose comparator finds NaN equal to every number
+ * (src/sort.c:32-47, 97-116), so where NaN-key groups land -- and whether
+ * the groups of one number are split around them -- follows the merge
+ * pattern over all input rows.  The GPU does not reproduce that order and
+ * says so instead of returning another one. */
+#define NAN_KEY_ORDER_ERROR \
+    "GPU group_by: NaN values in a double group key with key order (the reference's merge sort places " \
+    "NaN keys by its merge pattern over all rows; not reproduced on the GPU)"
 
 typedef struct {
     DevCol key;
@@ -559,12 +570,17 @@ static VecBatch *gpu_group_run_multi(GroupAggNode *ga) {
         t.sp[j] = ksp[j];
         t.sl[j] = ksl[j];
     }
-    if (collided) {
+    int nan_sorted = 0;
+    for (int j = 0; j < m && ga->use_sorted; j++)
+        for (int64_t r = 0; r < ng && ktype[j] == VEC_DOUBLE && !nan_sorted; r++)
+            nan_sorted = kok[j][r] && isnan(kv[j][r]);
+    if (collided || nan_sorted) {
         for (int a = 0; a < na; a++) { free(vals[a]); free(valid[a]); }
         for (int j = 0; j < m; j++) { free(kv[j]); free(kok[j]); free(ksp[j]); free(ksl[j]); }
         free(vals);
         free(valid);
         kept_free(kept, m, n_kept);
+        if (nan_sorted) vectra_error("%s", NAN_KEY_ORDER_ERROR);
         vectra_error("GPU group_by: two different key tuples share a 64-bit code (probability about 2^-64 "
                      "per pair of groups)");
     }
@@ -750,6 +766,15 @@ static VecBatch *gpu_group_run(GroupAggNode *ga) {
     if (ga->n_keys == 1 && key_type == VEC_BOOL) {
         for (int64_t i = 0; i < ng; i++) res->columns[0].buf.bln[i] = (uint8_t)kbuf[i];
         free(kbuf);
+    }
+    if (ga->use_sorted && ga->n_keys == 1 && key_type == VEC_DOUBLE) {
+        for (int64_t i = 0; i < ng; i++)
+            if (vec_array_is_valid(&res->columns[0], i) && isnan(res->columns[0].buf.dbl[i])) {
+                free(vals);
+                free(valid);
+                vec_batch_free(res);
+                vectra_error("%s", NAN_KEY_ORDER_ERROR);
+            }
     }
     free(vals);
     free(valid);

@@ -162,9 +162,9 @@ static void join_build_side(JoinNode *jn) {
     VecType common = vx_common_type(lt, rt);
     g->build_key_type = common;
     /* Left/full joins continue a chain walk by comparing the probe key in
-     * its ORIGINAL type against the build key (src/join.c:398-400,
-     * 413-415), so when the probe key had to be coerced only the first
-     * (highest build row) match survives. */
+     * its ORIGINAL type against the build key's bytes (src/join.c:398-400,
+     * 413-415); when the probe key had to be coerced the probe passes the
+     * uncoerced column so that the device applies the same rule. */
     g->first_only = (jn->kind == JOIN_LEFT || jn->kind == JOIN_FULL) && lt != common;
     g->r_dev = (DevCol *)calloc((size_t)(g->r_ncols > 0 ? g->r_ncols : 1), sizeof(DevCol));
     for (int c = 0; c < g->r_ncols; c++) {
@@ -324,7 +324,7 @@ static VecBatch *probe_batch(JoinNode *jn, VecBatch *pb) {
     }
     int64_t n_out = 0;
     VX_GPU(vgpu_join_probe(g->jt, (int)jn->kind, &pkey, pb->n_rows, d_sel, pb->sel ? pb->sel_n : 0,
-                           g->first_only, &n_out));
+                           g->first_only ? &g->l_dev[lk].col : NULL, &n_out));
     if (n_out == 0) return NULL;
     const int32_t *d_prow;
     const int64_t *d_brow;
@@ -451,7 +451,7 @@ static VecBatch *probe_batch_multi(JoinNode *jn, VecBatch *pb) {
     const int semi_anti = jn->kind == JOIN_SEMI || jn->kind == JOIN_ANTI;
     int64_t n_out = 0;
     VX_GPU(vgpu_join_probe(g->jt, semi_anti ? (int)JOIN_INNER : (int)jn->kind, &pkey, pb->n_rows, d_sel,
-                           pb->sel ? pb->sel_n : 0, g->first_only, &n_out));
+                           pb->sel ? pb->sel_n : 0, NULL, &n_out));
     const int32_t *d_prow = NULL;
     const int64_t *d_brow = NULL;
     if (n_out > 0) {

@@ -253,7 +253,7 @@ def vgpu() -> C.CDLL:
         _proto(lib, "vgpu_join_build_payload", C.c_void_p, [C.POINTER(VgpuCol), C.c_int64, C.POINTER(VgpuCol),
                                                             C.c_int])
         _proto(lib, "vgpu_join_probe", C.c_int, [C.c_void_p, C.c_int, C.POINTER(VgpuCol), C.c_int64, C.c_void_p,
-                                                 C.c_int64, C.c_int, C.POINTER(C.c_int64)])
+                                                 C.c_int64, C.POINTER(VgpuCol), C.POINTER(C.c_int64)])
         _proto(lib, "vgpu_join_pairs", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)])
         _proto(lib, "vgpu_join_unmatched_build", C.c_int, [C.c_void_p, C.POINTER(C.c_int64)])
         _proto(lib, "vgpu_join_build_rows", C.c_int64, [C.c_void_p])
@@ -387,6 +387,8 @@ def load_oracle(path: Path) -> C.CDLL:
         _proto(lib, "oracle_exact_sum", C.c_double, [C.POINTER(C.c_double), C.c_int64])
         _proto(lib, "oracle_hash_value", C.c_uint64, [C.POINTER(VecArray), C.c_int64])
         _proto(lib, "oracle_project_mul_node_create", NODE_P, [NODE_P, C.c_char_p, C.c_char_p, C.c_char_p])
+        _proto(lib, "oracle_try_expr_eval", C.c_int, [EXPR_P, BATCH_P, C.POINTER(C.POINTER(VecArray))])
+        _proto(lib, "oracle_array_free", None, [C.POINTER(VecArray)])
         _libs["oracle"] = lib
     return _libs["oracle"]
 
