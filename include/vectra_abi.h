@@ -16,6 +16,7 @@
  *   AggSpec, GroupAggNode ............................. src/group_agg.h:7-23
  *   JoinKind, JoinKey, JoinNode prefix ................ src/join.h:6-61
  *   FilterNode ........................................ src/filter.h:7-11
+ *   ProjEntry, ProjectNode ............................ src/project.h:7-18
  *   Vtr1Header, Vtr1ColStat, Vtr1RowGroup, Vtr1File ... src/vtr1.h:7-37
  *   ScanNode .......................................... src/scan.h:9-16
  *
@@ -219,6 +220,23 @@ typedef struct {
     char     *suffix_y;
     void     *gpu;           /* GPU-private state */
 } JoinNode;
+
+/* src/project.h:7-18 -- the reference's ProjectNode, which summarise()
+ * inserts under a GroupAggNode for an aggregate of an expression such as
+ * sum(value * weight) (R/verbs.R:266-284).  The GPU GroupAggNode reads
+ * these fields, as the optimizer does (src/optimize.c:128-150), to fuse
+ * GroupAgg <- Project(tmp = a * b) <- Join into one device pass. */
+typedef struct {
+    char    *output_name;
+    VecExpr *expr;        /* NULL: pass the child column output_name through */
+} ProjEntry;
+
+typedef struct {
+    VecNode    base;
+    VecNode   *child;
+    int        n_entries;
+    ProjEntry *entries;
+} ProjectNode;
 
 /* ---- drop-in constructors (same signatures and ownership) ----------- */
 

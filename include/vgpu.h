@@ -286,6 +286,8 @@ int vgpu_join_pairs(vgpu_jtable *t, const int32_t **probe_rows,
 int vgpu_join_unmatched_build(vgpu_jtable *t, int64_t *n_out);
 
 int64_t vgpu_join_build_rows(const vgpu_jtable *t);
+/* the largest number of build rows sharing one (non-NA) key */
+int64_t vgpu_join_max_dup(const vgpu_jtable *t);
 void    vgpu_join_destroy(vgpu_jtable *t);
 
 /* ---- cast --------------------------------------------------------------

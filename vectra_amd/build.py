@@ -72,7 +72,7 @@ def build(verbose: bool = False) -> dict[str, Path]:
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *map(str, objs), "-o", str(libvgpu)], verbose)
 
     libcore = LIB / "libvx_core.so"
-    _run([CC, *C_FLAGS, "-shared", "-Wl,-soname,libvx_core.so", str(HOST / "vx_core.c"), str(HOST / "vx_vtr.c"), "-o", str(libcore)],
+    _run([CC, *C_FLAGS, "-shared", "-Wl,-soname,libvx_core.so", str(HOST / "vx_core.c"), str(HOST / "vx_vtr.c"), "-lm", "-o", str(libcore)],
          verbose)
 
     libnodes = LIB / "libvectra_gpu.so"

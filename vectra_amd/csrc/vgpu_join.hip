@@ -305,6 +305,7 @@ extern "C" vgpu_jtable *vgpu_join_build_payload(const vgpu_col *key, int64_t n_b
 }
 
 extern "C" int64_t vgpu_join_build_rows(const vgpu_jtable *t) { return t ? t->n_build : -1; }
+extern "C" int64_t vgpu_join_max_dup(const vgpu_jtable *t) { return t ? (int64_t)t->max_dup : -1; }
 
 // ---------------------------------------------------------------------------
 // probe

@@ -44,6 +44,11 @@ static void annotation(const VecNode *node, char *buf, size_t size) {
         put(buf, size, &pos, "in memory, %d cols", node->output_schema.n_cols);
     } else if (strcmp(kind, "FilterNode") == 0) {
         put(buf, size, &pos, "streaming");
+    } else if (strcmp(kind, "ProjectNode") == 0) {
+        const ProjectNode *pn = (const ProjectNode *)node;
+        int tmp = 0;
+        for (int i = 0; i < pn->n_entries; i++) tmp |= strncmp(pn->entries[i].output_name, ".vectra_tmp_", 12) == 0;
+        put(buf, size, &pos, tmp ? "streaming, hidden mutate" : "streaming");
     } else if (strcmp(kind, "GroupAggNode") == 0) {
         put(buf, size, &pos, "materializes, %d keys", ((const GroupAggNode *)node)->n_keys);
     } else if (strcmp(kind, "JoinNode") == 0) {
@@ -52,7 +57,7 @@ static void annotation(const VecNode *node, char *buf, size_t size) {
         const char *jk = (int)jn->kind >= 0 && (int)jn->kind < 5 ? names[jn->kind] : "unknown";
         put(buf, size, &pos, "build right + stream left, %s, %d keys", jk, jn->n_keys);
     }
-    char note[192];
+    char note[320];
     if (vx_gpu_filter_note(node, note, (int)sizeof note) >= 0 || vx_gpu_group_note(node, note, (int)sizeof note) >= 0 ||
         vx_gpu_join_note(node, note, (int)sizeof note) >= 0)
         put(buf, size, &pos, "%s%s", pos ? "; " : "", note);
@@ -68,6 +73,10 @@ static int children(const VecNode *n, const VecNode **out) {
         out[0] = ((const GroupAggNode *)n)->child;
         return 1;
     }
+    if (strcmp(kind, "ProjectNode") == 0) {
+        out[0] = ((const ProjectNode *)n)->child;
+        return 1;
+    }
     if (strcmp(kind, "JoinNode") == 0) {
         out[0] = ((const JoinNode *)n)->left;
         out[1] = ((const JoinNode *)n)->right;
@@ -78,7 +87,7 @@ static int children(const VecNode *n, const VecNode **out) {
 
 static void lines(const VecNode *n, int depth, char *buf, size_t size, size_t *pos, int *count) {
     if (*count >= EX_MAX_LINES) return;
-    char ann[320];
+    char ann[512];
     annotation(n, ann, sizeof ann);
     put(buf, size, pos, "%*s%s [%s]\n", 2 * depth, "", n->kind ? n->kind : "Unknown", ann);
     (*count)++;

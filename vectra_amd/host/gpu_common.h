@@ -66,6 +66,36 @@ int vx_try_compile_expr(const VecExpr *e, const VecBatch *batch, CompiledExpr *o
  * of the given schema; returns the row count */
 int64_t vx_concat_batches(VecNode *child, VecArray **cols_out);
 
+/* ---- a GPU JoinNode read by the GPU GroupAggNode above it ------------
+ * GroupAgg <- Project(tmp = a * b) <- Join (what summarise(sum(a * b))
+ * builds over a join, R/verbs.R:266-284) runs as one fused probe ->
+ * aggregate pass: the GroupAgg node has the JoinNode build its table (with
+ * the build columns it aggregates embedded) and then feeds the join's left
+ * child itself.  If it finds the built table unsuited (see group_agg_node.c)
+ * it pulls batches from the Project as usual and the JoinNode probes its
+ * already built table. */
+typedef struct {
+    vgpu_jtable  *jt;
+    VecNode      *left;            /* the join's probe child, not yet pulled */
+    int           kind;            /* JOIN_INNER or JOIN_LEFT */
+    int           left_key, right_key;
+    VecType       build_key_type;
+    const DevCol *r_dev;           /* right columns on the device, key coerced */
+    int           r_ncols;
+    int64_t       max_dup;         /* most build rows of one key */
+    int           coerced;         /* the probe key is cast to build_key_type */
+} VxJoinFusion;
+
+/* explain() notes of the GPU nodes: -1 when n is not such a node */
+int vx_gpu_join_note(const VecNode *n, char *buf, int size);
+
+/* 1 for a GPU JoinNode that has not started, one non-string key, inner or left */
+int vx_gpu_join_fusable(const VecNode *n);
+/* join output column -> (0, left column) or (1, right column); -1 if none */
+int vx_gpu_join_output_source(const VecNode *n, int out_col, int *side, int *col);
+/* builds the table (right columns payload_cols embedded, at most four) */
+void vx_gpu_join_prepare_fusion(VecNode *n, const int *payload_cols, int n_payload, VxJoinFusion *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,6 +30,8 @@
     "NaN keys by its merge pattern over all rows; not reproduced on the GPU)"
 
 typedef struct {
+    DevCol fj_probe[VGPU_MAX_COLS];   /* fused join: probe columns of one batch */
+    DevCol fj_cast;                   /* fused join: probe key cast to the build key type */
     DevCol key;
     DevCol in[VGPU_MAX_INPUTS];
     DevMem sel;
@@ -40,6 +42,7 @@ typedef struct {
      * two hashes and positions computed from them */
     DevMem s_off[VGPU_MAX_COLS], s_chr[VGPU_MAX_COLS], s_val[VGPU_MAX_COLS];
     DevMem s_h1[VGPU_MAX_COLS], s_h2[VGPU_MAX_COLS], s_pos[VGPU_MAX_COLS];
+    int fused;         /* the last run took the fused Project <- Join path */
 } GroupGpu;
 
 /* file column of a scan's k-th output column (its columns in file order) */
@@ -643,9 +646,275 @@ static VecBatch *gpu_group_run_multi(GroupAggNode *ga) {
     return res;
 }
 
+/* finishes gb and builds the result batch: the key column (n_keys == 1),
+ * then one double column per aggregate; destroys gb */
+static VecBatch *group_result(GroupAggNode *ga, vgpu_groupby *gb, VecType key_type) {
+    int64_t ng = vgpu_groupby_finish(gb);
+    if (ng < 0) {
+        vgpu_groupby_destroy(gb);
+        vectra_error("GPU: %s", vgpu_last_error());
+    }
+    int n_out = ga->n_keys + ga->n_aggs;
+    VecBatch *res = vec_batch_alloc(n_out, ng);
+    double **vals = (double **)calloc((size_t)(ga->n_aggs + 1), sizeof(double *));
+    uint8_t **valid = (uint8_t **)calloc((size_t)(ga->n_aggs + 1), sizeof(uint8_t *));
+    for (int a = 0; a < ga->n_aggs; a++) {
+        res->columns[ga->n_keys + a] = vec_array_alloc(VEC_DOUBLE, ng);
+        vals[a] = res->columns[ga->n_keys + a].buf.dbl;
+        valid[a] = res->columns[ga->n_keys + a].validity;
+        res->col_names[ga->n_keys + a] = vx_strdup(ga->agg_specs[a].output_name);
+    }
+    uint64_t *kbuf = NULL;
+    if (ga->n_keys == 1) {
+        res->columns[0] = vec_array_alloc(key_type, ng);
+        res->col_names[0] = vx_strdup(ga->key_names[0]);
+        kbuf = key_type == VEC_BOOL ? (uint64_t *)malloc((size_t)(ng > 0 ? ng : 1) * 8)
+                                    : (uint64_t *)res->columns[0].buf.i64;
+    }
+    int64_t warns = vgpu_groupby_fetch(gb, kbuf, ga->n_keys ? res->columns[0].validity : NULL, vals, valid);
+    vgpu_groupby_destroy(gb);
+    if (warns < 0) {
+        free(vals); free(valid);
+        vec_batch_free(res);
+        vectra_error("GPU: %s", vgpu_last_error());
+    }
+    if (ga->n_keys == 1 && key_type == VEC_BOOL) {
+        for (int64_t i = 0; i < ng; i++) res->columns[0].buf.bln[i] = (uint8_t)kbuf[i];
+        free(kbuf);
+    }
+    if (ga->use_sorted && ga->n_keys == 1 && key_type == VEC_DOUBLE) {
+        for (int64_t i = 0; i < ng; i++)
+            if (vec_array_is_valid(&res->columns[0], i) && isnan(res->columns[0].buf.dbl[i])) {
+                free(vals);
+                free(valid);
+                vec_batch_free(res);
+                vectra_error("%s", NAN_KEY_ORDER_ERROR);
+            }
+    }
+    free(vals);
+    free(valid);
+    /* one warning per min/max group without a value (src/agg_ops.c:218-239) */
+    for (int64_t w = 0; w < warns; w++)
+        vectra_warning("no non-missing arguments to min/max; returning Inf/-Inf");
+    return res;
+}
+
+/* ---- GroupAgg <- Project(tmp = a * b) <- Join, fused -----------------
+ * summarise(s = sum(value * weight)) over a join is, in R, a GroupAggNode
+ * over the hidden ProjectNode that computes .vectra_tmp_s = value * weight
+ * over the JoinNode (R/verbs.R:266-284, src/project.c:51-252,
+ * src/join.c:262-480).  When the Project's entries the aggregation reads
+ * are join columns or products of two, and the join is a GPU join of one
+ * key, this node has the join build its table and then streams the join's
+ * left child through vgpu_groupby_feed_join: probe, product and
+ * aggregation in one device pass, no join rows materialized.  Group order
+ * is the join output's: feed k's pairs take positions k * 2^44 + (probe
+ * row, match) with each batch's unmatched left-join rows after its matches
+ * (src/join.c:439-448). */
+
+typedef struct {
+    int op;              /* 0 = one join column, 1 = product of two */
+    int side_a, col_a;   /* side 0 = left (probe), 1 = right (build) */
+    int side_b, col_b;
+} FjSrc;
+
+static int fj_col(const VecNode *join, const char *name, int *side, int *col) {
+    int oc = vec_schema_find_col(&join->output_schema, name);
+    return oc < 0 ? -1 : vx_gpu_join_output_source(join, oc, side, col);
+}
+
+static VecType fj_type(const JoinNode *jn, int side, int col) {
+    const VecSchema *sc = side == 0 ? &jn->left->output_schema : &jn->right->output_schema;
+    return sc->col_types[col];
+}
+
+/* the source of Project output column `name`, or -1 when it is not a join
+ * column (pass-through or reference) or, with allow_product, a product of
+ * two numeric join columns */
+static int fj_resolve(const ProjectNode *pn, const char *name, int allow_product, FjSrc *src) {
+    const JoinNode *jn = (const JoinNode *)pn->child;
+    memset(src, 0, sizeof *src);
+    for (int i = 0; i < pn->n_entries; i++) {
+        const ProjEntry *pe = &pn->entries[i];
+        if (strcmp(pe->output_name, name) != 0) continue;
+        const VecExpr *e = pe->expr;
+        if (!e) return fj_col(pn->child, name, &src->side_a, &src->col_a);
+        if (e->kind == EXPR_COL_REF) return fj_col(pn->child, e->col_name, &src->side_a, &src->col_a);
+        if (!allow_product || e->kind != EXPR_ARITH || e->op != '*' || !e->left || !e->right ||
+            e->left->kind != EXPR_COL_REF || e->right->kind != EXPR_COL_REF)
+            return -1;
+        if (fj_col(pn->child, e->left->col_name, &src->side_a, &src->col_a) ||
+            fj_col(pn->child, e->right->col_name, &src->side_b, &src->col_b))
+            return -1;
+        VecType ta = fj_type(jn, src->side_a, src->col_a), tb = fj_type(jn, src->side_b, src->col_b);
+        if ((ta != VEC_INT64 && ta != VEC_DOUBLE) || (tb != VEC_INT64 && tb != VEC_DOUBLE)) return -1;
+        src->op = 1;
+        return 0;
+    }
+    return -1;
+}
+
+/* slot of col in a side's compact column list, appended if absent; the
+ * probe side's slot 0 holds the (possibly cast) probe key alone, so probe
+ * columns search from slot 1 */
+static int fj_slot(int *cols, int *n, int col, int first) {
+    for (int i = first; i < *n; i++)
+        if (cols[i] == col) return i;
+    if (*n >= VGPU_MAX_COLS) return -1;
+    cols[*n] = col;
+    return (*n)++;
+}
+
+/* Returns the result batch, or NULL when the tree below is not the fused
+ * shape (then nothing has been pulled; the join may have built its table). */
+static VecBatch *gpu_group_run_fused(GroupAggNode *ga) {
+    GroupGpu *g = (GroupGpu *)ga->gpu;
+    if (!ga->child->kind || strcmp(ga->child->kind, "ProjectNode") != 0) return NULL;
+    const ProjectNode *pn = (const ProjectNode *)ga->child;
+    if (!vx_gpu_join_fusable(pn->child) || ga->n_keys > 1 || ga->n_aggs > VGPU_MAX_AGGS) return NULL;
+    const JoinNode *jn = (const JoinNode *)pn->child;
+    /* probe columns: slot 0 is the probe key; build columns by reference */
+    int pcols[VGPU_MAX_COLS], npc = 1, bcols[VGPU_MAX_COLS], nbc = 0;
+    pcols[0] = jn->keys[0].left_col;
+    int key_side = 0, key_slot = 0;
+    VecType key_type = VEC_INT64;
+    if (ga->n_keys == 1) {
+        FjSrc k;
+        if (fj_resolve(pn, ga->key_names[0], 0, &k)) return NULL;
+        key_type = fj_type(jn, k.side_a, k.col_a);
+        if (key_type == VEC_STRING) return NULL;
+        key_side = k.side_a;
+        key_slot = k.side_a == 0 ? fj_slot(pcols, &npc, k.col_a, 1) : fj_slot(bcols, &nbc, k.col_a, 0);
+        if (key_slot < 0) return NULL;
+    }
+    vgpu_agg aggs[VGPU_MAX_AGGS];
+    vgpu_input ins[VGPU_MAX_INPUTS];
+    int32_t in_type[VGPU_MAX_INPUTS];
+    FjSrc in_src[VGPU_MAX_INPUTS];
+    int n_in = 0;
+    for (int a = 0; a < ga->n_aggs; a++) {
+        const AggSpec *sp = &ga->agg_specs[a];
+        aggs[a].kind = (int32_t)sp->kind;
+        aggs[a].na_rm = sp->na_rm;
+        aggs[a].reserved = 0;
+        aggs[a].input = -1;
+        if (sp->kind == AGG_COUNT_STAR) continue;
+        FjSrc src;
+        if (!sp->input_col || fj_resolve(pn, sp->input_col, 1, &src)) return NULL;
+        if (src.op == 0 && fj_type(jn, src.side_a, src.col_a) == VEC_STRING) return NULL;
+        int slot = -1;
+        for (int i = 0; i < n_in; i++)
+            if (!memcmp(&in_src[i], &src, sizeof src)) slot = i;
+        if (slot < 0) {
+            if (n_in >= VGPU_MAX_INPUTS) return NULL;
+            slot = n_in++;
+            in_src[slot] = src;
+            VecType ta = fj_type(jn, src.side_a, src.col_a);
+            if (src.op == 1) {
+                VecType tb = fj_type(jn, src.side_b, src.col_b);
+                in_type[slot] = (ta == VEC_DOUBLE || tb == VEC_DOUBLE) ? VGPU_DOUBLE : VGPU_INT64;
+            } else {
+                in_type[slot] = (int32_t)ta;
+            }
+            vgpu_input *in = &ins[slot];
+            memset(in, 0, sizeof *in);
+            in->op = src.op;
+            in->side_a = src.side_a;
+            in->col_a = src.side_a == 0 ? fj_slot(pcols, &npc, src.col_a, 1) : fj_slot(bcols, &nbc, src.col_a, 0);
+            if (src.op == 1) {
+                in->side_b = src.side_b;
+                in->col_b = src.side_b == 0 ? fj_slot(pcols, &npc, src.col_b, 1) : fj_slot(bcols, &nbc, src.col_b, 0);
+            }
+            if (in->col_a < 0 || (src.op == 1 && in->col_b < 0)) return NULL;
+        }
+        aggs[a].input = slot;
+    }
+    for (int i = 1; i < npc; i++)
+        if (jn->left->output_schema.col_types[pcols[i]] == VEC_STRING) return NULL;
+
+    /* the shape fits: build the join table with the build columns embedded */
+    VxJoinFusion fu;
+    vx_gpu_join_prepare_fusion(pn->child, bcols, nbc, &fu);
+    /* The fused pass orders at most 4,096 matches per probe row, and a
+     * coerced left join continues its chains on uncoerced key bytes
+     * (src/join.c:398-411): those joins run unfused, through the Project. */
+    if (fu.max_dup > 4096 || (fu.coerced && fu.kind == JOIN_LEFT && fu.max_dup > 1)) return NULL;
+
+    int order = ga->use_sorted ? VGPU_ORDER_KEY : VGPU_ORDER_FIRST_SEEN;
+    vgpu_groupby *gb = vgpu_groupby_create(ga->n_keys ? (int)key_type : VGPU_KEY_NONE, n_in, in_type, ga->n_aggs,
+                                           aggs, order, 0);
+    if (!gb) vectra_error("GPU: %s", vgpu_last_error());
+    vgpu_col bdev[VGPU_MAX_COLS];
+    for (int i = 0; i < nbc; i++) bdev[i] = fu.r_dev[bcols[i]].col;
+    VecNode *left = fu.left;
+    const VecSchema *ls = &left->output_schema;
+    int64_t feed = 0;
+    VecBatch *b;
+    while ((b = left->next_batch(left)) != NULL) {
+        int64_t nl = vec_batch_logical_rows(b);
+        if (nl > 0) {
+            vgpu_col pdev[VGPU_MAX_COLS];
+            for (int i = 0; i < npc; i++) {
+                if (b->columns[pcols[i]].type != ls->col_types[pcols[i]]) {
+                    vgpu_groupby_destroy(gb);
+                    vec_batch_free(b);
+                    vectra_error("join: column type changed between batches");
+                }
+                devcol_upload(&g->fj_probe[i], &b->columns[pcols[i]], b->n_rows);
+                pdev[i] = g->fj_probe[i].col;
+            }
+            if (fu.coerced) {
+                devcol_reserve(&g->fj_cast, (size_t)(b->n_rows > 0 ? b->n_rows : 1) * 8, 0);
+                if (vgpu_cast(&pdev[0], b->n_rows, (int)fu.build_key_type, g->fj_cast.d_data) < 0) {
+                    vgpu_groupby_destroy(gb);
+                    vec_batch_free(b);
+                    vectra_error("GPU: %s", vgpu_last_error());
+                }
+                pdev[0].data = g->fj_cast.d_data;
+                pdev[0].type = (int32_t)fu.build_key_type;
+            }
+            const int32_t *d_sel = NULL;
+            if (b->sel) {
+                d_sel = (const int32_t *)devmem_reserve(&g->sel, (size_t)(b->sel_n > 0 ? b->sel_n : 1) * 4);
+                VX_GPU(vgpu_memcpy_h2d((void *)d_sel, b->sel, (size_t)b->sel_n * 4));
+            }
+            vgpu_feed_spec sp;
+            memset(&sp, 0, sizeof sp);
+            sp.n_rows = b->n_rows;
+            sp.sel = d_sel;
+            sp.n_sel = b->sel ? b->sel_n : 0;
+            sp.pos_base = feed << 44;
+            sp.join = fu.jt;
+            sp.join_kind = fu.kind == JOIN_LEFT ? VGPU_JOIN_LEFT : VGPU_JOIN_INNER;
+            sp.key_side = key_side;
+            sp.key_col = key_slot;
+            sp.probe_key_col = 0;
+            sp.n_probe_cols = npc;
+            sp.n_build_cols = nbc;
+            sp.probe_cols = pdev;
+            sp.build_cols = bdev;
+            sp.n_inputs = n_in;
+            sp.inputs = ins;
+            if (vgpu_groupby_feed_join(gb, &sp) < 0) {
+                vgpu_groupby_destroy(gb);
+                vec_batch_free(b);
+                vectra_error("GPU: %s", vgpu_last_error());
+            }
+            /* the next batch's uploads reuse the probe buffers */
+            VX_GPU(vgpu_synchronize());
+        }
+        feed++;
+        vec_batch_free(b);
+    }
+    g->fused = 1;
+    return group_result(ga, gb, key_type);
+}
+
 static VecBatch *gpu_group_run(GroupAggNode *ga) {
     GroupGpu *g = (GroupGpu *)ga->gpu;
     const VecSchema *cs = &ga->child->output_schema;
+    VecBatch *fused = gpu_group_run_fused(ga);
+    if (fused) return fused;
     if (ga->n_keys > 1) return gpu_group_run_multi(ga);
     if (ga->n_keys == 1) {
         const int ki = vec_schema_find_col(cs, ga->key_names[0]);
@@ -734,54 +1003,7 @@ static VecBatch *gpu_group_run(GroupAggNode *ga) {
         vec_batch_free(b);
     }
 
-    int64_t ng = vgpu_groupby_finish(gb);
-    if (ng < 0) {
-        vgpu_groupby_destroy(gb);
-        vectra_error("GPU: %s", vgpu_last_error());
-    }
-    int n_out = ga->n_keys + ga->n_aggs;
-    VecBatch *res = vec_batch_alloc(n_out, ng);
-    double **vals = (double **)calloc((size_t)(ga->n_aggs + 1), sizeof(double *));
-    uint8_t **valid = (uint8_t **)calloc((size_t)(ga->n_aggs + 1), sizeof(uint8_t *));
-    for (int a = 0; a < ga->n_aggs; a++) {
-        res->columns[ga->n_keys + a] = vec_array_alloc(VEC_DOUBLE, ng);
-        vals[a] = res->columns[ga->n_keys + a].buf.dbl;
-        valid[a] = res->columns[ga->n_keys + a].validity;
-        res->col_names[ga->n_keys + a] = vx_strdup(ga->agg_specs[a].output_name);
-    }
-    uint64_t *kbuf = NULL;
-    if (ga->n_keys == 1) {
-        res->columns[0] = vec_array_alloc(key_type, ng);
-        res->col_names[0] = vx_strdup(ga->key_names[0]);
-        kbuf = key_type == VEC_BOOL ? (uint64_t *)malloc((size_t)(ng > 0 ? ng : 1) * 8)
-                                    : (uint64_t *)res->columns[0].buf.i64;
-    }
-    int64_t warns = vgpu_groupby_fetch(gb, kbuf, ga->n_keys ? res->columns[0].validity : NULL, vals, valid);
-    vgpu_groupby_destroy(gb);
-    if (warns < 0) {
-        free(vals); free(valid);
-        vec_batch_free(res);
-        vectra_error("GPU: %s", vgpu_last_error());
-    }
-    if (ga->n_keys == 1 && key_type == VEC_BOOL) {
-        for (int64_t i = 0; i < ng; i++) res->columns[0].buf.bln[i] = (uint8_t)kbuf[i];
-        free(kbuf);
-    }
-    if (ga->use_sorted && ga->n_keys == 1 && key_type == VEC_DOUBLE) {
-        for (int64_t i = 0; i < ng; i++)
-            if (vec_array_is_valid(&res->columns[0], i) && isnan(res->columns[0].buf.dbl[i])) {
-                free(vals);
-                free(valid);
-                vec_batch_free(res);
-                vectra_error("%s", NAN_KEY_ORDER_ERROR);
-            }
-    }
-    free(vals);
-    free(valid);
-    /* one warning per min/max group without a value (src/agg_ops.c:218-239) */
-    for (int64_t w = 0; w < warns; w++)
-        vectra_warning("no non-missing arguments to min/max; returning Inf/-Inf");
-    return res;
+    return group_result(ga, gb, key_type);
 }
 
 static VecBatch *gpu_group_next(VecNode *self) {
@@ -805,6 +1027,8 @@ static void gpu_group_free(VecNode *self) {
     if (g) {
         devcol_free(&g->key);
         for (int i = 0; i < VGPU_MAX_INPUTS; i++) devcol_free(&g->in[i]);
+        for (int i = 0; i < VGPU_MAX_COLS; i++) devcol_free(&g->fj_probe[i]);
+        devcol_free(&g->fj_cast);
         devmem_free(&g->sel);
         devmem_free(&g->codes);
         for (int j = 0; j < VGPU_MAX_COLS; j++) {
@@ -878,7 +1102,15 @@ int vx_gpu_group_note(const VecNode *n, char *buf, int size) {
     const int filtered = c->kind && strcmp(c->kind, "FilterNode") == 0;
     if (filtered) c = ((const FilterNode *)c)->child;
     const int feed = !tuple && c->kind && strcmp(c->kind, "ScanNode") == 0 && vgpu_tune_get("scan.device_feed") > 1;
-    return snprintf(buf, (size_t)size, "gpu: %s group-by%s, %s%s", tuple ? "tuple-code" : "hash",
+    /* over Project <- GPU Join: fused at run time when the shape holds */
+    const GroupGpu *g = (const GroupGpu *)ga->gpu;
+    const int over_join = ga->child->kind && strcmp(ga->child->kind, "ProjectNode") == 0 &&
+                          vx_gpu_join_note(((const ProjectNode *)ga->child)->child, NULL, 0) >= 0;
+    const char *fuse = g && g->fused ? ", fused with the Project and Join below (probe, product and aggregate in one "
+                                       "device pass)"
+                       : over_join ? ", fuses the Project and Join below when their shape allows" : "";
+    return snprintf(buf, (size_t)size, "gpu: %s group-by%s, %s%s%s", tuple ? "tuple-code" : "hash",
                     strs ? " (string keys hashed on the device)" : "", ga->use_sorted ? "key order" : "first-seen order",
-                    feed ? (filtered ? ", device scan feed with the filter in the feed" : ", device scan feed") : "");
+                    feed ? (filtered ? ", device scan feed with the filter in the feed" : ", device scan feed") : "",
+                    fuse);
 }

@@ -34,6 +34,8 @@ typedef struct {
     int      *r_nonkey;
     int       n_r_nonkey;
     int       first_only;     /* see join_build_side */
+    int       pay_cols[4];    /* right columns embedded in the table entries */
+    int       n_pay;          /* (set by a fused GroupAgg before the build) */
     DevCol    l_dev[64];
     DevCol    pkey_cast;
     DevMem    sel;
@@ -185,7 +187,13 @@ static void join_build_side(JoinNode *jn) {
         dc->col.type = (int32_t)common;
         bkey = dc->col;
     }
-    g->jt = vgpu_join_build(&bkey, g->n_build);
+    if (g->n_pay > 0) {
+        vgpu_col pay[4];
+        for (int i = 0; i < g->n_pay; i++) pay[i] = g->r_dev[g->pay_cols[i]].col;
+        g->jt = vgpu_join_build_payload(&bkey, g->n_build, pay, g->n_pay);
+    } else {
+        g->jt = vgpu_join_build(&bkey, g->n_build);
+    }
     if (!g->jt) vectra_error("GPU: %s", vgpu_last_error());
 }
 
@@ -684,4 +692,57 @@ int vx_gpu_join_note(const VecNode *n, char *buf, int size) {
     const int tuple = jn->n_keys > 1 || strs > 0;
     return snprintf(buf, (size_t)size, "gpu: %s table built on the device%s, probe per batch",
                     tuple ? "tuple-code" : "hash", strs ? " (string keys hashed on the device)" : "");
+}
+
+/* ---- fusion with a GroupAggNode above a ProjectNode ------------------ */
+
+int vx_gpu_join_fusable(const VecNode *n) {
+    if (!n || n->next_batch != gpu_join_next) return 0;
+    const JoinNode *jn = (const JoinNode *)n;
+    const JoinGpu *g = (const JoinGpu *)jn->gpu;
+    if (g->built || g->state != 0) return 0;
+    if (jn->kind != JOIN_INNER && jn->kind != JOIN_LEFT) return 0;
+    if (jn->n_keys != 1) return 0;
+    VecType lt = jn->left->output_schema.col_types[jn->keys[0].left_col];
+    VecType rt = jn->right->output_schema.col_types[jn->keys[0].right_col];
+    return lt != VEC_STRING && rt != VEC_STRING;
+}
+
+int vx_gpu_join_output_source(const VecNode *n, int out_col, int *side, int *col) {
+    const JoinNode *jn = (const JoinNode *)n;
+    const JoinGpu *g = (const JoinGpu *)jn->gpu;
+    const int nl = jn->left->output_schema.n_cols;
+    if (out_col < 0) return -1;
+    if (out_col < nl) {
+        *side = 0;
+        *col = out_col;
+        return 0;
+    }
+    if (out_col - nl >= g->n_r_nonkey || jn->kind == JOIN_SEMI || jn->kind == JOIN_ANTI) return -1;
+    *side = 1;
+    *col = g->r_nonkey[out_col - nl];
+    return 0;
+}
+
+void vx_gpu_join_prepare_fusion(VecNode *n, const int *payload_cols, int n_payload, VxJoinFusion *out) {
+    JoinNode *jn = (JoinNode *)n;
+    JoinGpu *g = (JoinGpu *)jn->gpu;
+    g->n_pay = 0;
+    for (int i = 0; i < n_payload && g->n_pay < 4; i++) {
+        VecType t = jn->right->output_schema.col_types[payload_cols[i]];
+        if (t != VEC_STRING && payload_cols[i] != jn->keys[0].right_col) g->pay_cols[g->n_pay++] = payload_cols[i];
+    }
+    join_build_side(jn);
+    g->built = 1;
+    memset(out, 0, sizeof *out);
+    out->jt = g->jt;
+    out->left = jn->left;
+    out->kind = jn->kind;
+    out->left_key = jn->keys[0].left_col;
+    out->right_key = jn->keys[0].right_col;
+    out->build_key_type = g->build_key_type;
+    out->r_dev = g->r_dev;
+    out->r_ncols = g->r_ncols;
+    out->max_dup = vgpu_join_max_dup(g->jt);
+    out->coerced = jn->left->output_schema.col_types[jn->keys[0].left_col] != g->build_key_type;
 }

@@ -12,6 +12,7 @@
 #define _GNU_SOURCE
 #include "vx_core.h"
 
+#include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -440,6 +441,251 @@ int vx_try_join(vx_join_ctor fn, VecNode *left, VecNode *right, JoinKind kind, i
     vx_frame_push(&f);
     if (setjmp(f.jb) == 0) {
         *out = fn(left, right, kind, n_keys, keys, sx, sy);
+        vx_frame_pop(&f);
+        return 0;
+    }
+    return -1;
+}
+
+/* ==================================================================== */
+/* ProjectNode (the reference's src/project.c, for the harness)          */
+/* ==================================================================== */
+
+/* Stands in for the reference's own ProjectNode where R is absent, with the
+ * reference's struct layout (src/project.h:7-18) and output schema (entry
+ * names; an expression entry takes its expression's type, which the
+ * reference's parser stores in expr->result_type, a pass-through entry its
+ * child column's type; src/project.c:287-320).  Each batch comes out
+ * dense (the child's selection gathered), pass-through columns copied and
+ * expression entries evaluated over the gathered rows.  The expressions
+ * evaluated are the ones summarise() inserts for nested aggregate inputs:
+ * column references, and + - * / % over columns and literals with vectra's
+ * arithmetic (src/scalar_ops.c:11-66: NA if either side is NA, bool
+ * promoted to int64, int64 wrapping, integer / and % by zero NA, double
+ * IEEE and fmod); anything else is an error. */
+
+typedef struct {
+    VecType  type;
+    int64_t  n;
+    const VecArray *col;    /* column operand (physical rows), or NULL */
+    const int32_t  *sel;    /* gather map for col, or NULL */
+    int64_t  li;            /* literal (int64 / bool) */
+    double   ld;            /* literal (double) */
+    int      lit_na;
+} PjOperand;
+
+static const VecArray *pj_find(const VecBatch *b, const char *name) {
+    for (int c = 0; c < b->n_cols; c++)
+        if (strcmp(b->col_names[c], name) == 0) return &b->columns[c];
+    vectra_error("column not found: %s", name);
+}
+
+static VecArray pj_gather(const VecArray *a, const VecBatch *b) {
+    int64_t n = vec_batch_logical_rows(b);
+    VecArray o = vec_array_alloc(a->type, n);
+    for (int64_t i = 0; i < n; i++) {
+        int64_t r = vec_batch_physical_row(b, i);
+        if (!vec_array_is_valid(a, r)) continue;
+        vec_array_set_valid(&o, i);
+        switch (a->type) {
+        case VEC_INT64: o.buf.i64[i] = a->buf.i64[r]; break;
+        case VEC_DOUBLE: o.buf.dbl[i] = a->buf.dbl[r]; break;
+        case VEC_BOOL: o.buf.bln[i] = a->buf.bln[r]; break;
+        case VEC_STRING: break;
+        }
+    }
+    if (a->type == VEC_STRING) {
+        int64_t total = 0;
+        for (int64_t i = 0; i < n; i++) {
+            int64_t r = vec_batch_physical_row(b, i);
+            total += a->buf.str.offsets[r + 1] - a->buf.str.offsets[r];
+        }
+        free(o.buf.str.data);
+        o.buf.str.data = (char *)malloc((size_t)(total > 0 ? total : 1));
+        o.buf.str.data_len = total;
+        int64_t at = 0;
+        for (int64_t i = 0; i < n; i++) {
+            int64_t r = vec_batch_physical_row(b, i);
+            int64_t len = a->buf.str.offsets[r + 1] - a->buf.str.offsets[r];
+            o.buf.str.offsets[i] = at;
+            memcpy(o.buf.str.data + at, a->buf.str.data + a->buf.str.offsets[r], (size_t)len);
+            at += len;
+        }
+        o.buf.str.offsets[n] = at;
+    }
+    return o;
+}
+
+static PjOperand pj_operand(const VecExpr *e, const VecBatch *b) {
+    PjOperand o;
+    memset(&o, 0, sizeof o);
+    o.n = vec_batch_logical_rows(b);
+    switch (e->kind) {
+    case EXPR_COL_REF:
+        o.col = pj_find(b, e->col_name);
+        o.type = o.col->type;
+        if (o.type == VEC_STRING) vectra_error("cannot coerce string in arithmetic/comparison");
+        break;
+    case EXPR_LIT_INT64: o.type = VEC_INT64; o.li = e->lit_i64; break;
+    case EXPR_LIT_DOUBLE: o.type = VEC_DOUBLE; o.ld = e->lit_dbl; break;
+    case EXPR_LIT_BOOL: o.type = VEC_BOOL; o.li = e->lit_bln; break;
+    case EXPR_LIT_NA: o.type = VEC_DOUBLE; o.lit_na = 1; break;
+    default:
+        vectra_error("project: the harness evaluates column references and arithmetic over columns and literals "
+                     "(expression kind %d)", (int)e->kind);
+    }
+    return o;
+}
+
+static int pj_valid(const PjOperand *o, const VecBatch *b, int64_t i) {
+    if (o->col) return vec_array_is_valid(o->col, vec_batch_physical_row(b, i));
+    return !o->lit_na;
+}
+
+static int64_t pj_i64(const PjOperand *o, const VecBatch *b, int64_t i) {
+    if (!o->col) return o->li;
+    int64_t r = vec_batch_physical_row(b, i);
+    return o->type == VEC_BOOL ? (int64_t)o->col->buf.bln[r] : o->col->buf.i64[r];
+}
+
+static double pj_dbl(const PjOperand *o, const VecBatch *b, int64_t i) {
+    if (!o->col) return o->type == VEC_DOUBLE ? o->ld : (double)o->li;
+    int64_t r = vec_batch_physical_row(b, i);
+    switch (o->type) {
+    case VEC_DOUBLE: return o->col->buf.dbl[r];
+    case VEC_INT64: return (double)o->col->buf.i64[r];
+    default: return (double)o->col->buf.bln[r];
+    }
+}
+
+static VecArray pj_eval(const VecExpr *e, const VecBatch *b) {
+    if (e->kind == EXPR_COL_REF) return pj_gather(pj_find(b, e->col_name), b);
+    if (e->kind != EXPR_ARITH)
+        vectra_error("project: the harness evaluates column references and arithmetic over columns and literals "
+                     "(expression kind %d)", (int)e->kind);
+    PjOperand l = pj_operand(e->left, b), r = pj_operand(e->right, b);
+    VecType t = (l.type == VEC_DOUBLE || r.type == VEC_DOUBLE) ? VEC_DOUBLE : VEC_INT64;
+    int64_t n = vec_batch_logical_rows(b);
+    VecArray o = vec_array_alloc(t, n);
+    for (int64_t i = 0; i < n; i++) {
+        if (!pj_valid(&l, b, i) || !pj_valid(&r, b, i)) continue;
+        vec_array_set_valid(&o, i);
+        if (t == VEC_INT64) {
+            uint64_t x = (uint64_t)pj_i64(&l, b, i), y = (uint64_t)pj_i64(&r, b, i);
+            int64_t sx = (int64_t)x, sy = (int64_t)y;
+            switch (e->op) {
+            case '+': o.buf.i64[i] = (int64_t)(x + y); break;
+            case '-': o.buf.i64[i] = (int64_t)(x - y); break;
+            case '*': o.buf.i64[i] = (int64_t)(x * y); break;
+            case '/':
+                if (sy == 0) vec_array_set_null(&o, i);
+                else o.buf.i64[i] = sy == -1 ? (int64_t)(0 - x) : sx / sy;
+                break;
+            case '%':
+                if (sy == 0) vec_array_set_null(&o, i);
+                else o.buf.i64[i] = sy == -1 ? 0 : sx % sy;
+                break;
+            default: vec_array_free(&o); vectra_error("unknown arith op: %c", e->op);
+            }
+        } else {
+            double x = pj_dbl(&l, b, i), y = pj_dbl(&r, b, i);
+            switch (e->op) {
+            case '+': o.buf.dbl[i] = x + y; break;
+            case '-': o.buf.dbl[i] = x - y; break;
+            case '*': o.buf.dbl[i] = x * y; break;
+            case '/': o.buf.dbl[i] = x / y; break;
+            case '%': o.buf.dbl[i] = fmod(x, y); break;
+            default: vec_array_free(&o); vectra_error("unknown arith op: %c", e->op);
+            }
+        }
+    }
+    return o;
+}
+
+static VecBatch *pj_next(VecNode *self) {
+    ProjectNode *pn = (ProjectNode *)self;
+    VecBatch *in = pn->child->next_batch(pn->child);
+    if (!in) return NULL;
+    int64_t n = vec_batch_logical_rows(in);
+    VecBatch *out = vec_batch_alloc(pn->n_entries, n);
+    for (int i = 0; i < pn->n_entries; i++) {
+        const ProjEntry *pe = &pn->entries[i];
+        out->columns[i] = pe->expr ? pj_eval(pe->expr, in) : pj_gather(pj_find(in, pe->output_name), in);
+        out->col_names[i] = vx_strdup(pe->output_name);
+    }
+    vec_batch_free(in);
+    return out;
+}
+
+static void pj_free(VecNode *self) {
+    ProjectNode *pn = (ProjectNode *)self;
+    pn->child->free_node(pn->child);
+    for (int i = 0; i < pn->n_entries; i++) {
+        free(pn->entries[i].output_name);
+        if (pn->entries[i].expr) vec_expr_free(pn->entries[i].expr);
+    }
+    free(pn->entries);
+    vec_schema_free(&pn->base.output_schema);
+    free(pn);
+}
+
+/* the type an entry's expression evaluates to over the child schema (the
+ * reference's parser records it in expr->result_type, src/r_bridge.c:918-935) */
+static VecType pj_type(const VecExpr *e, const VecSchema *cs) {
+    switch (e->kind) {
+    case EXPR_COL_REF: {
+        int i = vec_schema_find_col(cs, e->col_name);
+        if (i < 0) vectra_error("column not found: %s", e->col_name);
+        return cs->col_types[i];
+    }
+    case EXPR_LIT_INT64: return VEC_INT64;
+    case EXPR_LIT_BOOL: return VEC_BOOL;
+    case EXPR_ARITH: {
+        VecType l = pj_type(e->left, cs), r = pj_type(e->right, cs);
+        return (l == VEC_DOUBLE || r == VEC_DOUBLE) ? VEC_DOUBLE : VEC_INT64;
+    }
+    default: return VEC_DOUBLE;
+    }
+}
+
+ProjectNode *project_node_create(VecNode *child, int n_entries, ProjEntry *entries) {
+    ProjectNode *pn = (ProjectNode *)calloc(1, sizeof *pn);
+    if (!pn) vectra_error("alloc failed");
+    pn->child = child;
+    pn->n_entries = n_entries;
+    pn->entries = entries;
+    char **names = (char **)malloc(sizeof(char *) * (size_t)(n_entries > 0 ? n_entries : 1));
+    VecType *types = (VecType *)malloc(sizeof(VecType) * (size_t)(n_entries > 0 ? n_entries : 1));
+    for (int i = 0; i < n_entries; i++) {
+        names[i] = entries[i].output_name;
+        if (entries[i].expr) {
+            types[i] = pj_type(entries[i].expr, &child->output_schema);
+            entries[i].expr->result_type = types[i];
+        } else {
+            int idx = vec_schema_find_col(&child->output_schema, entries[i].output_name);
+            if (idx < 0) {
+                free(names);
+                free(types);
+                vectra_error("select: column not found: %s", entries[i].output_name);
+            }
+            types[i] = child->output_schema.col_types[idx];
+        }
+    }
+    pn->base.output_schema = vec_schema_create(n_entries, names, types);
+    free(names);
+    free(types);
+    pn->base.next_batch = pj_next;
+    pn->base.free_node = pj_free;
+    pn->base.kind = "ProjectNode";
+    return pn;
+}
+
+int vx_try_project(VecNode *child, int n_entries, ProjEntry *entries, VecNode **out) {
+    vx_frame f;
+    *out = NULL;
+    vx_frame_push(&f);
+    if (setjmp(f.jb) == 0) {
+        *out = (VecNode *)project_node_create(child, n_entries, entries);
         vx_frame_pop(&f);
         return 0;
     }

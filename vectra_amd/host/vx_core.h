@@ -93,6 +93,16 @@ int vx_try_group(vx_group_ctor fn, VecNode *child, int n_keys, char **key_names,
 int vx_try_join(vx_join_ctor fn, VecNode *left, VecNode *right, JoinKind kind, int n_keys,
                 JoinKey *keys, const char *sx, const char *sy, VecNode **out);
 
+/* ---- ProjectNode ----------------------------------------------------
+ * The reference's project_node_create (src/project.h:20-23), for trees the
+ * harness builds where R is absent: summarise(s = sum(a * b)) inserts
+ * Project(child columns..., .vectra_tmp_s = a * b) under the GroupAggNode
+ * (R/verbs.R:266-284).  Takes ownership of entries, their names and
+ * expressions; evaluates column references and + - * / % over columns and
+ * literals (other expressions raise). */
+ProjectNode *project_node_create(VecNode *child, int n_entries, ProjEntry *entries);
+int vx_try_project(VecNode *child, int n_entries, ProjEntry *entries, VecNode **out);
+
 /* ---- .vtr harness entry points -------------------------------------- */
 /* the optimizer's pushdown of a filter's predicate onto the .vtr scan
  * below it (src/optimize.c:353-369), along a chain of filters */

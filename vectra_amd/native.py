@@ -86,6 +86,10 @@ class JoinKey(C.Structure):
     _fields_ = [("left_col", C.c_int), ("right_col", C.c_int)]
 
 
+class ProjEntry(C.Structure):
+    _fields_ = [("output_name", C.c_void_p), ("expr", C.c_void_p)]
+
+
 # --------------------------------------------------------------------------
 # kernel ABI structs (include/vgpu.h)
 # --------------------------------------------------------------------------
@@ -217,6 +221,7 @@ def core() -> C.CDLL:
         _proto(lib, "vx_try_vtr_write", C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_char_p), C.POINTER(VecArray),
                                                   C.c_int64, C.c_int64])
         _proto(lib, "vx_pushdown_predicates", None, [NODE_P])
+        _proto(lib, "vx_try_project", C.c_int, [NODE_P, C.c_int, C.c_void_p, C.POINTER(NODE_P)])
         _libs["core"] = lib
     return _libs["core"]
 
@@ -257,6 +262,7 @@ def vgpu() -> C.CDLL:
         _proto(lib, "vgpu_join_pairs", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)])
         _proto(lib, "vgpu_join_unmatched_build", C.c_int, [C.c_void_p, C.POINTER(C.c_int64)])
         _proto(lib, "vgpu_join_build_rows", C.c_int64, [C.c_void_p])
+        _proto(lib, "vgpu_join_max_dup", C.c_int64, [C.c_void_p])
         _proto(lib, "vgpu_join_destroy", None, [C.c_void_p])
         _proto(lib, "vgpu_gather", C.c_int, [C.POINTER(VgpuCol), C.c_void_p, C.c_int, C.c_int64, C.c_void_p,
                                              C.c_void_p])
@@ -655,6 +661,32 @@ def join_node(engine: str, left: Any, right: Any, kind: str, keys: Sequence[tupl
                               suffix_x.encode(), suffix_y.encode(), C.byref(out)))
     out._keep = (left, right)
     return out
+
+
+def project_node(child: Any, entries: Sequence[tuple[str, Any]]) -> Any:
+    """The reference's ProjectNode (src/project.h, the harness's stand-in):
+    entries are (output_name, None) to pass a child column through or
+    (output_name, expression tuple) for a computed column -- the hidden
+    mutate summarise() inserts is every child column passed through plus
+    (".vectra_tmp_<name>", ("arith", "*", ("col", a), ("col", b)))."""
+    n = len(entries)
+    arr = _libc.calloc(max(1, n), C.sizeof(ProjEntry))
+    ep = C.cast(arr, C.POINTER(ProjEntry))
+    for i, (name, tree) in enumerate(entries):
+        ep[i].output_name = _malloc_str(name)
+        ep[i].expr = C.cast(expr(tree), C.c_void_p).value if tree is not None else None
+    out = NODE_P()
+    _check(core().vx_try_project(child, n, arr, C.byref(out)))
+    out._keep = child
+    return out
+
+
+def hidden_mutate(child: Any, tmp_name: str, a: str, b: str) -> Any:
+    """Project(child columns..., tmp_name = a * b), as summarise(sum(a * b))
+    builds it (R/verbs.R:266-284)."""
+    names = [child.contents.output_schema.col_names[i].decode()
+             for i in range(child.contents.output_schema.n_cols)]
+    return project_node(child, [(nm, None) for nm in names] + [(tmp_name, ("arith", "*", ("col", a), ("col", b)))])
 
 
 def batch_to_numpy(b: Any) -> dict[str, tuple[np.ndarray, np.ndarray]]:
