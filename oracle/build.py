@@ -21,14 +21,19 @@ OUT = HERE / "_build"
 CC = os.environ.get("CC", "gcc")
 
 
-def build(verbose: bool = False) -> Path:
-    OUT.mkdir(exist_ok=True)
-    lib = OUT / "liboracle.so"
-    core_dir = ROOT / "vectra_amd" / "lib"
-    cmd = [CC, "-std=c11", "-O2", "-fPIC", "-shared", "-Wall", "-Wextra", "-Wno-unused-parameter",
+def build(verbose: bool = False, out: Path = OUT, core_dir: Path | None = None, sanitize: bool = False) -> Path:
+    """liboracle.so into out, linked to the libvx_core.so in core_dir
+    (default vectra_amd/lib); sanitize adds AddressSanitizer and
+    UndefinedBehaviorSanitizer at -O1 (tests/run_sanitized.py)."""
+    out.mkdir(parents=True, exist_ok=True)
+    lib = out / "liboracle.so"
+    core_dir = core_dir or ROOT / "vectra_amd" / "lib"
+    opt = ["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
+           "-fno-sanitize-recover=undefined"] if sanitize else ["-O2"]
+    rpath = "-Wl,-rpath," + (str(core_dir) if sanitize else "$ORIGIN/../../vectra_amd/lib")
+    cmd = [CC, "-std=c11", *opt, "-fPIC", "-shared", "-Wall", "-Wextra", "-Wno-unused-parameter",
            f"-I{ROOT / 'include'}", f"-I{ROOT / 'vectra_amd' / 'host'}",
-           str(HERE / "oracle.c"), f"-L{core_dir}", "-lvx_core", "-lm",
-           "-Wl,-rpath,$ORIGIN/../../vectra_amd/lib", "-o", str(lib)]
+           str(HERE / "oracle.c"), f"-L{core_dir}", "-lvx_core", "-lm", rpath, "-o", str(lib)]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)

@@ -7,7 +7,7 @@ import pytest
 ROOT = Path(__file__).resolve().parent.parent
 sys.path.insert(0, str(ROOT))
 
-ORACLE_LIB = ROOT / "oracle" / "_build" / "liboracle.so"
+ORACLE_LIB = Path(os.environ.get("VECTRA_ORACLE_LIB", ROOT / "oracle" / "_build" / "liboracle.so"))
 NATIVE = [ROOT / "vectra_amd" / "lib" / n for n in ("libvgpu.so", "libvx_core.so", "libvectra_gpu.so")]
 
 

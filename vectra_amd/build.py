@@ -71,14 +71,25 @@ def build(verbose: bool = False) -> dict[str, Path]:
     libvgpu = LIB / "libvgpu.so"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *map(str, objs), "-o", str(libvgpu)], verbose)
 
-    libcore = LIB / "libvx_core.so"
-    _run([CC, *C_FLAGS, "-shared", "-Wl,-soname,libvx_core.so", str(HOST / "vx_core.c"), str(HOST / "vx_vtr.c"), "-lm", "-o", str(libcore)],
-         verbose)
-
-    libnodes = LIB / "libvectra_gpu.so"
-    _run([CC, *C_FLAGS, "-shared", *[str(HOST / s) for s in NODE_SOURCES],
-          f"-L{LIB}", "-lvgpu", "-lvx_core", "-lpthread", "-Wl,-rpath,$ORIGIN", "-o", str(libnodes)], verbose)
+    libcore, libnodes = build_host(LIB, verbose=verbose)
     return {"vgpu": libvgpu, "core": libcore, "nodes": libnodes}
+
+
+def build_host(out: Path, sanitize: bool = False, verbose: bool = False) -> tuple[Path, Path]:
+    """The C11 libraries (libvx_core, libvectra_gpu) into out, which must
+    hold libvgpu.so (or a link to it).  sanitize: -O1 with AddressSanitizer
+    and UndefinedBehaviorSanitizer, for CPU test runs (tests/run_sanitized.py)."""
+    flags = list(C_FLAGS)
+    if sanitize:
+        flags = [f for f in flags if f != "-O2"] + ["-O1", "-g", "-fno-omit-frame-pointer",
+                                                   "-fsanitize=address,undefined", "-fno-sanitize-recover=undefined"]
+    libcore = out / "libvx_core.so"
+    _run([CC, *flags, "-shared", "-Wl,-soname,libvx_core.so", str(HOST / "vx_core.c"), str(HOST / "vx_vtr.c"), "-lm",
+          "-o", str(libcore)], verbose)
+    libnodes = out / "libvectra_gpu.so"
+    _run([CC, *flags, "-shared", *[str(HOST / s) for s in NODE_SOURCES],
+          f"-L{out}", "-lvgpu", "-lvx_core", "-lpthread", "-Wl,-rpath,$ORIGIN", "-o", str(libnodes)], verbose)
+    return libcore, libnodes
 
 
 if __name__ == "__main__":

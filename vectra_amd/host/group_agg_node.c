@@ -29,7 +29,19 @@
     "GPU group_by: NaN values in a double group key with key order (the reference's merge sort places " \
     "NaN keys by its merge pattern over all rows; not reproduced on the GPU)"
 
+#define MK_MAX_KEYS 8
+
+/* a batch's string key column, kept on the host until the groups' strings
+ * are read from it; base = physical position of its first row */
 typedef struct {
+    VecArray a;
+    int64_t  base;
+} KeptStr;
+
+typedef struct {
+    vgpu_groupby *live_gb;            /* the group-by being fed (released on errors by free) */
+    KeptStr *kept[MK_MAX_KEYS];       /* string keys: batches' string columns kept */
+    int64_t n_kept;
     DevCol fj_probe[VGPU_MAX_COLS];   /* fused join: probe columns of one batch */
     DevCol fj_cast;                   /* fused join: probe key cast to the build key type */
     DevCol key;
@@ -44,6 +56,19 @@ typedef struct {
     DevMem s_h1[VGPU_MAX_COLS], s_h2[VGPU_MAX_COLS], s_pos[VGPU_MAX_COLS];
     int fused;         /* the last run took the fused Project <- Join path */
 } GroupGpu;
+
+/* the group-by a run holds: kept in the node state so that free_node
+ * releases it when an error unwinds the run */
+static vgpu_groupby *gb_hold(GroupAggNode *ga, vgpu_groupby *gb) {
+    ((GroupGpu *)ga->gpu)->live_gb = gb;
+    return gb;
+}
+
+static void gb_drop(GroupAggNode *ga, vgpu_groupby *gb) {
+    GroupGpu *g = (GroupGpu *)ga->gpu;
+    if (g->live_gb == gb) g->live_gb = NULL;
+    gb_drop(ga, gb);
+}
 
 /* file column of a scan's k-th output column (its columns in file order) */
 static int scan_file_col(const ScanNode *sn, int k) {
@@ -144,7 +169,7 @@ static int feed_from_scan(GroupAggNode *ga, vgpu_groupby *gb, int key_idx, int n
         for (int p = 0; p < cx.n_cols; p++) pcols[p] = cols[pslot[p]];
         if (vgpu_groupby_feed(gb, kslot >= 0 ? &cols[kslot] : NULL, ins, n, NULL, 0, pos0, fn ? &cx.prog : NULL,
                               fn ? pcols : NULL, fn ? cx.n_cols : 0) < 0) {
-            vgpu_groupby_destroy(gb);
+            gb_drop(ga, gb);
             vectra_error("GPU: %s", vgpu_last_error());
         }
     }
@@ -170,7 +195,6 @@ static int feed_from_scan(GroupAggNode *ga, vgpu_groupby *gb, int key_idx, int n
  * (temp_dir set) the groups are then sorted stably by the tuple, each
  * column ascending with NA last (NaN after the numbers, in
  * first-occurrence order; strings by bytes, a prefix first, src/sort.c:58-66). */
-#define MK_MAX_KEYS 8
 
 typedef struct {
     int       m;
@@ -295,12 +319,6 @@ static int slot_for(const VecSchema *cs, int ci, int *in_col, int32_t *in_type, 
     return (*n_in)++;
 }
 
-/* a batch's string key column, kept on the host until the groups' strings
- * are read from it; base = physical position of its first row */
-typedef struct {
-    VecArray a;
-    int64_t  base;
-} KeptStr;
 
 static int synth_slot(int *in_col, int32_t *in_type, int *n_in, int code) {
     if (*n_in >= VGPU_MAX_INPUTS) vectra_error("GPU summarise: more than %d input columns", VGPU_MAX_INPUTS);
@@ -388,11 +406,11 @@ static VecBatch *gpu_group_run_multi(GroupAggNode *ga) {
         a_cnt[j] = na;
         aggs[na++] = (vgpu_agg){VGPU_AGG_COUNT, kslot[j], 0, 0};
     }
-    vgpu_groupby *gb = vgpu_groupby_create(VGPU_INT64, n_in, in_type, na, aggs, VGPU_ORDER_FIRST_SEEN, 0);
+    vgpu_groupby *gb = gb_hold(ga, vgpu_groupby_create(VGPU_INT64, n_in, in_type, na, aggs, VGPU_ORDER_FIRST_SEEN, 0));
     if (!gb) vectra_error("GPU: %s", vgpu_last_error());
 
     int64_t pos = 0, phys = 0;
-    KeptStr *kept[MK_MAX_KEYS] = {0};
+    KeptStr **kept = g->kept;
     int64_t n_kept = 0, cap_kept = 0;
     VecBatch *b;
     while ((b = ga->child->next_batch(ga->child)) != NULL) {
@@ -402,7 +420,7 @@ static VecBatch *gpu_group_run_multi(GroupAggNode *ga) {
             for (int s2 = 0; s2 < n_in; s2++) {
                 if (in_col[s2] < 0) continue;   /* a string key's hash or positions: below */
                 if (b->columns[in_col[s2]].type != (VecType)in_type[s2]) {
-                    vgpu_groupby_destroy(gb);
+                    gb_drop(ga, gb);
                     vec_batch_free(b);
                     vectra_error("summarise: column type changed between batches");
                 }
@@ -424,7 +442,7 @@ static VecBatch *gpu_group_run_multi(GroupAggNode *ga) {
                 }
                 VecArray *sa = &b->columns[kcol[j]];
                 if (sa->type != VEC_STRING) {
-                    vgpu_groupby_destroy(gb);
+                    gb_drop(ga, gb);
                     vec_batch_free(b);
                     kept_free(kept, m, n_kept);
                     vectra_error("group_by: column type changed between batches");
@@ -461,7 +479,7 @@ static VecBatch *gpu_group_run_multi(GroupAggNode *ga) {
                 }
                 memset(sa, 0, sizeof *sa);
             }
-            if (n_str > 0) n_kept++;
+            if (n_str > 0) g->n_kept = ++n_kept;
             int64_t *codes = (int64_t *)devmem_reserve(&g->codes, (size_t)b->n_rows * 8);
             vgpu_col key = {VGPU_INT64, 0, codes, NULL};
             const int32_t *d_sel = NULL;
@@ -472,7 +490,7 @@ static VecBatch *gpu_group_run_multi(GroupAggNode *ga) {
             /* codes of NaN tuples come from physical positions, unique over batches */
             if (vgpu_key_codes(keys, m, b->n_rows, phys, codes, NULL) < 0 ||
                 vgpu_groupby_feed(gb, &key, ins, b->n_rows, d_sel, b->sel ? b->sel_n : 0, pos, NULL, NULL, 0) < 0) {
-                vgpu_groupby_destroy(gb);
+                gb_drop(ga, gb);
                 vec_batch_free(b);
                 vectra_error("GPU: %s", vgpu_last_error());
             }
@@ -484,7 +502,7 @@ static VecBatch *gpu_group_run_multi(GroupAggNode *ga) {
     }
     const int64_t ng = vgpu_groupby_finish(gb);
     if (ng < 0) {
-        vgpu_groupby_destroy(gb);
+        gb_drop(ga, gb);
         kept_free(kept, m, n_kept);
         vectra_error("GPU: %s", vgpu_last_error());
     }
@@ -499,7 +517,7 @@ static VecBatch *gpu_group_run_multi(GroupAggNode *ga) {
         if (!vals[a] || !valid[a]) vectra_error("alloc failed");
     }
     const int64_t warns = vgpu_groupby_fetch(gb, codes_h, NULL, vals, valid);
-    vgpu_groupby_destroy(gb);
+    gb_drop(ga, gb);
     free(codes_h);
     if (warns < 0) {
         kept_free(kept, m, n_kept);
@@ -651,7 +669,7 @@ static VecBatch *gpu_group_run_multi(GroupAggNode *ga) {
 static VecBatch *group_result(GroupAggNode *ga, vgpu_groupby *gb, VecType key_type) {
     int64_t ng = vgpu_groupby_finish(gb);
     if (ng < 0) {
-        vgpu_groupby_destroy(gb);
+        gb_drop(ga, gb);
         vectra_error("GPU: %s", vgpu_last_error());
     }
     int n_out = ga->n_keys + ga->n_aggs;
@@ -672,7 +690,7 @@ static VecBatch *group_result(GroupAggNode *ga, vgpu_groupby *gb, VecType key_ty
                                     : (uint64_t *)res->columns[0].buf.i64;
     }
     int64_t warns = vgpu_groupby_fetch(gb, kbuf, ga->n_keys ? res->columns[0].validity : NULL, vals, valid);
-    vgpu_groupby_destroy(gb);
+    gb_drop(ga, gb);
     if (warns < 0) {
         free(vals); free(valid);
         vec_batch_free(res);
@@ -841,8 +859,8 @@ static VecBatch *gpu_group_run_fused(GroupAggNode *ga) {
     if (fu.max_dup > 4096 || (fu.coerced && fu.kind == JOIN_LEFT && fu.max_dup > 1)) return NULL;
 
     int order = ga->use_sorted ? VGPU_ORDER_KEY : VGPU_ORDER_FIRST_SEEN;
-    vgpu_groupby *gb = vgpu_groupby_create(ga->n_keys ? (int)key_type : VGPU_KEY_NONE, n_in, in_type, ga->n_aggs,
-                                           aggs, order, 0);
+    vgpu_groupby *gb = gb_hold(ga, vgpu_groupby_create(ga->n_keys ? (int)key_type : VGPU_KEY_NONE, n_in, in_type, ga->n_aggs,
+                                           aggs, order, 0));
     if (!gb) vectra_error("GPU: %s", vgpu_last_error());
     vgpu_col bdev[VGPU_MAX_COLS];
     for (int i = 0; i < nbc; i++) bdev[i] = fu.r_dev[bcols[i]].col;
@@ -856,7 +874,7 @@ static VecBatch *gpu_group_run_fused(GroupAggNode *ga) {
             vgpu_col pdev[VGPU_MAX_COLS];
             for (int i = 0; i < npc; i++) {
                 if (b->columns[pcols[i]].type != ls->col_types[pcols[i]]) {
-                    vgpu_groupby_destroy(gb);
+                    gb_drop(ga, gb);
                     vec_batch_free(b);
                     vectra_error("join: column type changed between batches");
                 }
@@ -866,7 +884,7 @@ static VecBatch *gpu_group_run_fused(GroupAggNode *ga) {
             if (fu.coerced) {
                 devcol_reserve(&g->fj_cast, (size_t)(b->n_rows > 0 ? b->n_rows : 1) * 8, 0);
                 if (vgpu_cast(&pdev[0], b->n_rows, (int)fu.build_key_type, g->fj_cast.d_data) < 0) {
-                    vgpu_groupby_destroy(gb);
+                    gb_drop(ga, gb);
                     vec_batch_free(b);
                     vectra_error("GPU: %s", vgpu_last_error());
                 }
@@ -896,7 +914,7 @@ static VecBatch *gpu_group_run_fused(GroupAggNode *ga) {
             sp.n_inputs = n_in;
             sp.inputs = ins;
             if (vgpu_groupby_feed_join(gb, &sp) < 0) {
-                vgpu_groupby_destroy(gb);
+                gb_drop(ga, gb);
                 vec_batch_free(b);
                 vectra_error("GPU: %s", vgpu_last_error());
             }
@@ -961,8 +979,8 @@ static VecBatch *gpu_group_run(GroupAggNode *ga) {
     int order = ga->use_sorted ? VGPU_ORDER_KEY : VGPU_ORDER_FIRST_SEEN;
     const ScanNode *below = scan_below(ga->child);
     const int64_t hint = below && key_idx >= 0 ? scan_group_bound(below, key_idx) : 0;
-    vgpu_groupby *gb = vgpu_groupby_create(ga->n_keys ? (int)key_type : VGPU_KEY_NONE, n_in, in_type,
-                                           ga->n_aggs, aggs, order, hint);
+    vgpu_groupby *gb = gb_hold(ga, vgpu_groupby_create(ga->n_keys ? (int)key_type : VGPU_KEY_NONE, n_in, in_type,
+                                           ga->n_aggs, aggs, order, hint));
     if (!gb) vectra_error("GPU: %s", vgpu_last_error());
 
     int64_t pos = 0;
@@ -978,7 +996,7 @@ static VecBatch *gpu_group_run(GroupAggNode *ga) {
             }
             for (int s = 0; s < n_in; s++) {
                 if (b->columns[in_col[s]].type != (VecType)in_type[s]) {
-                    vgpu_groupby_destroy(gb);
+                    gb_drop(ga, gb);
                     vectra_error("summarise: column type changed between batches");
                 }
                 devcol_upload(&g->in[s], &b->columns[in_col[s]], b->n_rows);
@@ -991,7 +1009,7 @@ static VecBatch *gpu_group_run(GroupAggNode *ga) {
             }
             if (vgpu_groupby_feed(gb, key_idx >= 0 ? &key : NULL, ins, b->n_rows, d_sel, b->sel ? b->sel_n : 0,
                                   pos, NULL, NULL, 0) < 0) {
-                vgpu_groupby_destroy(gb);
+                gb_drop(ga, gb);
                 vec_batch_free(b);
                 vectra_error("GPU: %s", vgpu_last_error());
             }
@@ -1025,6 +1043,8 @@ static void gpu_group_free(VecNode *self) {
     }
     free(ga->agg_specs);
     if (g) {
+        if (g->live_gb) vgpu_groupby_destroy(g->live_gb);
+        kept_free(g->kept, MK_MAX_KEYS, g->n_kept);
         devcol_free(&g->key);
         for (int i = 0; i < VGPU_MAX_INPUTS; i++) devcol_free(&g->in[i]);
         for (int i = 0; i < VGPU_MAX_COLS; i++) devcol_free(&g->fj_probe[i]);
