@@ -115,10 +115,6 @@ def test_c2_full_kernel_api(gpu, oracle, c2_full, kind):
     check_c2(got["key"][0], got["key"][1], got["a0"][0], got["a0"][1], got["a1"][0], exp, kind)
 
 
-@pytest.mark.skip(reason="its first run (inner, 50M fact rows in 48 row groups) wrote nothing for 180 s and was "
-                         "killed for silence; no unbounded loop was found by reading the fused path, so it stays off "
-                         "the GPU until a cause is shown (the fused node tree is tested at 400k rows in "
-                         "test_gpu_c2_nodes.py, the same kernel at 50M rows through the kernel API above)")
 @pytest.mark.parametrize("kind", ["inner", "left"])
 def test_c2_full_node_tree(gpu, oracle, c2_full, kind):
     """the tree R builds: GroupAgg <- Project(.vectra_tmp_s = value * weight)

@@ -67,7 +67,7 @@ static vgpu_groupby *gb_hold(GroupAggNode *ga, vgpu_groupby *gb) {
 static void gb_drop(GroupAggNode *ga, vgpu_groupby *gb) {
     GroupGpu *g = (GroupGpu *)ga->gpu;
     if (g->live_gb == gb) g->live_gb = NULL;
-    gb_drop(ga, gb);
+    vgpu_groupby_destroy(gb);
 }
 
 /* file column of a scan's k-th output column (its columns in file order) */
@@ -787,6 +787,8 @@ static int fj_slot(int *cols, int *n, int col, int first) {
  * shape (then nothing has been pulled; the join may have built its table). */
 static VecBatch *gpu_group_run_fused(GroupAggNode *ga) {
     GroupGpu *g = (GroupGpu *)ga->gpu;
+    /* knob "gb.fuse_join" = 1 runs the tree node by node */
+    if (vgpu_tune_get("gb.fuse_join") < 2) return NULL;
     if (!ga->child->kind || strcmp(ga->child->kind, "ProjectNode") != 0) return NULL;
     const ProjectNode *pn = (const ProjectNode *)ga->child;
     if (!vx_gpu_join_fusable(pn->child) || ga->n_keys > 1 || ga->n_aggs > VGPU_MAX_AGGS) return NULL;
@@ -1128,7 +1130,8 @@ int vx_gpu_group_note(const VecNode *n, char *buf, int size) {
                           vx_gpu_join_note(((const ProjectNode *)ga->child)->child, NULL, 0) >= 0;
     const char *fuse = g && g->fused ? ", fused with the Project and Join below (probe, product and aggregate in one "
                                        "device pass)"
-                       : over_join ? ", fuses the Project and Join below when their shape allows" : "";
+                       : over_join && vgpu_tune_get("gb.fuse_join") >= 2
+                           ? ", fuses the Project and Join below when their shape allows" : "";
     return snprintf(buf, (size_t)size, "gpu: %s group-by%s, %s%s%s", tuple ? "tuple-code" : "hash",
                     strs ? " (string keys hashed on the device)" : "", ga->use_sorted ? "key order" : "first-seen order",
                     feed ? (filtered ? ", device scan feed with the filter in the feed" : ", device scan feed") : "",
