@@ -56,6 +56,20 @@ int         vgpu_memcpy_h2d(void *dst_device, const void *src_host, size_t bytes
 int         vgpu_memcpy_d2h(void *dst_host, const void *src_device, size_t bytes);
 int         vgpu_memset(void *dst_device, int value, size_t bytes);
 int         vgpu_synchronize(void);
+/* Streams (no counterpart in the reference, which runs one query at a time):
+ * a thread's library calls go to the library's own stream unless the thread
+ * selected another with vgpu_stream_use (NULL selects the library stream
+ * again).  Streams are created at the default priority.  Device buffers
+ * the library caches are reused only by later work on the stream that
+ * released them.  vgpu_stream_destroy waits for the stream's work. */
+void       *vgpu_stream_create(void);
+int         vgpu_stream_use(void *stream);
+int         vgpu_stream_destroy(void *stream);
+/* free and total device memory in bytes */
+int         vgpu_mem_info(int64_t *free_bytes, int64_t *total_bytes);
+/* sets this thread's vgpu_last_error() text (for callers that report their
+ * own failures through it); returns -1 */
+int         vgpu_error(const char *msg);
 /* pinned host staging (hipHostMalloc); never handed to vectra consumers */
 void       *vgpu_host_alloc(size_t bytes);
 int         vgpu_host_free(void *ptr);
@@ -288,6 +302,8 @@ int vgpu_join_unmatched_build(vgpu_jtable *t, int64_t *n_out);
 int64_t vgpu_join_build_rows(const vgpu_jtable *t);
 /* the largest number of build rows sharing one (non-NA) key */
 int64_t vgpu_join_max_dup(const vgpu_jtable *t);
+/* device bytes the table holds (entries, segments, filter, payload) */
+int64_t vgpu_join_device_bytes(const vgpu_jtable *t);
 void    vgpu_join_destroy(vgpu_jtable *t);
 
 /* ---- cast --------------------------------------------------------------
@@ -467,6 +483,9 @@ void    vgpu_batch_destroy(vgpu_batch *b);
  *                         filter (3 bits a key), read when a table is built
  *   "join.load_pct"      (50) highest load of a join table's slots, in
  *                         percent (10..95), read when a table is built
+ *   "gb.fuse_join"       (2) a GPU GroupAggNode over Project(a * b) over a
+ *                         GPU JoinNode runs the three as one fused device
+ *                         pass (2), or node by node (1)
  * Returns -1 for an unknown name or a value <= 0. */
 int     vgpu_tune_set(const char *name, int64_t value);
 int64_t vgpu_tune_get(const char *name);

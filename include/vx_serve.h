@@ -17,12 +17,17 @@
  * Two modes over the same requests:
  *   plain     requests run one at a time in arrival order, each as its own
  *             group-by (class 1 probes the resident join table);
- *   batched   continuous batching: whenever the device finishes a pass,
- *             the oldest waiting request's class forms the next pass from
- *             every waiting request of that class (up to max_batch, and as
- *             many as the query-state budget admits); a pass is ONE shared
- *             scan of the fact columns for all its requests, so requests
- *             that arrive while a pass runs join the next one.
+ *   batched   continuous batching: up to in_flight passes run on the device
+ *             at once, each on a stream of its own; whenever a pass slot is
+ *             free, the oldest waiting request's class forms the next pass
+ *             from every waiting request of that class (up to max_batch, and
+ *             as many as the per-pass state budget admits).  A pass is ONE
+ *             shared scan of the fact columns for all its requests; requests
+ *             that arrive while passes run join the next pass as soon as a
+ *             slot frees, of either class, while the others still run.
+ * Admission by device memory: the resident columns and the join table are
+ * counted against hbm_budget, and pass slots are opened only while their
+ * query state (pass templates planned for max_batch requests) still fits.
  * The dimension's join table is built once and stays resident.
  */
 #ifndef VX_SERVE_H
@@ -68,9 +73,11 @@ typedef struct {
     int64_t  n_dim;
     int32_t  batched;        /* 0 plain, 1 continuous batching */
     int32_t  max_batch;      /* requests per pass (batched) */
-    int64_t  state_budget;   /* device bytes for per-request query state */
+    int64_t  state_budget;   /* device bytes of query state one pass may hold (0: no cap) */
     int32_t  keep_results;
-    int32_t  pad;
+    int32_t  in_flight;      /* batched: passes on the device at once (1..8; 0 = 2) */
+    int64_t  hbm_budget;     /* device bytes the run may hold: resident columns, join
+                                table, pass state (0: the device's memory) */
 } vx_serve_cfg;
 
 typedef struct {
@@ -80,7 +87,10 @@ typedef struct {
     double  wall_s;            /* first arrival to last result */
     double  setup_s;           /* resident join table, pass templates */
     int64_t state_bytes;       /* device bytes one admitted request holds */
-    int64_t resident_bytes;    /* join table + pass templates */
+    int64_t resident_bytes;    /* columns + join table + pass templates */
+    int32_t max_in_flight;     /* most passes on the device at once */
+    int32_t slots;             /* pass slots the budget admitted */
+    int64_t hbm_budget;        /* the budget applied */
 } vx_serve_stats;
 
 /* Serves reqs[0..n) (arrival_s ascending) in real time; out[i] is

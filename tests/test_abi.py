@@ -63,6 +63,8 @@ int main(void) {
   printf("%zu %zu %zu %zu\n", offsetof(vx_result, n_groups), offsetof(vx_serve_cfg, n_dim),
          offsetof(vx_serve_cfg, state_budget), offsetof(vx_serve_stats, resident_bytes));
   printf("%zu %zu\n", sizeof(vgpu_query), offsetof(vgpu_query, n_rows));
+  printf("%zu %zu %zu %zu\n", offsetof(vx_serve_cfg, in_flight), offsetof(vx_serve_cfg, hbm_budget),
+         offsetof(vx_serve_stats, max_in_flight), offsetof(vx_serve_stats, hbm_budget));
   return 0;
 }
 """)
@@ -75,6 +77,8 @@ int main(void) {
     assert got[1] == [nv.VxResult.n_groups.offset, nv.VxServeCfg.n_dim.offset, nv.VxServeCfg.state_budget.offset,
                       nv.VxServeStats.resident_bytes.offset]
     assert got[2] == [C.sizeof(nv.VgpuQuery), nv.VgpuQuery.n_rows.offset]
+    assert got[3] == [nv.VxServeCfg.in_flight.offset, nv.VxServeCfg.hbm_budget.offset,
+                      nv.VxServeStats.max_in_flight.offset, nv.VxServeStats.hbm_budget.offset]
 
 
 def test_struct_layouts(native, tmp_path):

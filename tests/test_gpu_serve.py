@@ -64,11 +64,15 @@ def oracle_request(nv, host, cls, t):
 def test_serve_same_alone_and_in_the_mix(gpu, oracle):
     host, fact, dim = make_tables(gpu, 41, 2_000_000, 20_000)
     reqs = requests(43, 40, burst=24)
-    mixed, st = gpu.serve(fact, dim, reqs, batched=True, max_batch=8)
+    mixed, st = gpu.serve(fact, dim, reqs, batched=True, max_batch=8, in_flight=2)
     alone, st_plain = gpu.serve(fact, dim, reqs, batched=False)
     assert st["n_done"] == len(reqs) and st_plain["n_done"] == len(reqs)
     assert st["max_pass"] > 1, "the burst should have been served in shared passes"
     assert max(r["batch"] for r in mixed) > 1
+    assert st["slots"] == 2 and st["max_in_flight"] == 2, "two passes should have been on the device at once"
+    # passes overlap in time: some pass started before another one ended
+    spans = sorted({(r["pass"], r["start_s"], r["end_s"]) for r in mixed}, key=lambda x: x[1])
+    assert any(b[1] < a[2] for a, b in zip(spans, spans[1:])), "no two passes overlapped"
     for m, a in zip(mixed, alone):
         same(m, a)
     for i in (0, 1, 5, 23, 39):
@@ -87,5 +91,41 @@ def test_serve_budget_limits_the_pass(gpu, oracle):
     assert st["max_pass"] <= 2 and st["n_done"] == len(reqs)
     for i, (cls, t, _) in enumerate(reqs):
         same(res[i], oracle_request(gpu, host, cls, t))
+    for c in list(fact.values()) + list(dim.values()):
+        c.free()
+
+
+def test_serve_many_in_flight_same_results(gpu, oracle):
+    """four pass slots over a dense arrival stream of both classes: every
+    result equals the plain one-at-a-time result"""
+    host, fact, dim = make_tables(gpu, 59, 1_000_000, 10_000)
+    reqs = requests(61, 64, burst=40)
+    mixed, st = gpu.serve(fact, dim, reqs, batched=True, max_batch=6, in_flight=4)
+    alone, _ = gpu.serve(fact, dim, reqs, batched=False)
+    assert st["n_done"] == len(reqs) and st["slots"] == 4 and st["max_in_flight"] > 1
+    for m, a in zip(mixed, alone):
+        same(m, a)
+    for i in (2, 17, 63):
+        cls, t, _ = reqs[i]
+        same(mixed[i], oracle_request(gpu, host, cls, t))
+    for c in list(fact.values()) + list(dim.values()):
+        c.free()
+
+
+def test_serve_hbm_budget_admission(gpu, oracle):
+    """the budget counts resident columns, the join table and every pass
+    slot's state: a budget that holds one slot runs one pass at a time, and
+    one below the resident columns is refused"""
+    host, fact, dim = make_tables(gpu, 67, 300_000, 5_000)
+    reqs = requests(71, 16, burst=16)
+    _, st1 = gpu.serve(fact, dim, reqs[:1], batched=True, max_batch=4, in_flight=1, keep_results=False)
+    one_slot = st1["resident_bytes"]
+    res, st = gpu.serve(fact, dim, reqs, batched=True, max_batch=4, in_flight=4, hbm_budget=one_slot)
+    assert st["slots"] == 1 and st["max_in_flight"] == 1 and st["hbm_budget"] == one_slot
+    assert st["resident_bytes"] <= one_slot
+    for i, (cls, t, _) in enumerate(reqs):
+        same(res[i], oracle_request(gpu, host, cls, t))
+    with pytest.raises(gpu.VectraError, match="exceed the budget"):
+        gpu.serve(fact, dim, reqs[:1], batched=True, hbm_budget=1 << 20)
     for c in list(fact.values()) + list(dim.values()):
         c.free()

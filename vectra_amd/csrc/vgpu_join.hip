@@ -307,6 +307,13 @@ extern "C" vgpu_jtable *vgpu_join_build_payload(const vgpu_col *key, int64_t n_b
 extern "C" int64_t vgpu_join_build_rows(const vgpu_jtable *t) { return t ? t->n_build : -1; }
 extern "C" int64_t vgpu_join_max_dup(const vgpu_jtable *t) { return t ? (int64_t)t->max_dup : -1; }
 
+extern "C" int64_t vgpu_join_device_bytes(const vgpu_jtable *t) {
+    if (!t) return -1;
+    int64_t b = (int64_t)(t->ent.cap + t->rows.cap + t->bloom.cap);
+    for (int p = 0; p < JT_MAX_PAYLOAD; p++) b += (int64_t)(t->cpay[p].cap + t->cval[p].cap);
+    return b;
+}
+
 // ---------------------------------------------------------------------------
 // probe
 // ---------------------------------------------------------------------------

@@ -30,7 +30,52 @@ void vg_set_error(const char *fmt, ...) {
 
 extern "C" const char *vgpu_last_error(void) { return g_err; }
 
-hipStream_t vg_stream() { return g_stream; }
+// a thread's library calls run on the stream it selected (vgpu_stream_use),
+// else on the library stream
+static thread_local hipStream_t t_stream = nullptr;
+
+hipStream_t vg_stream() { return t_stream ? t_stream : g_stream; }
+
+extern "C" void *vgpu_stream_create(void) {
+    if (vg_ensure_init()) return nullptr;
+    hipStream_t s = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);   // default priority
+    if (e != hipSuccess) {
+        vg_set_error("vgpu_stream_create: %s", hipGetErrorString(e));
+        return nullptr;
+    }
+    return s;
+}
+
+extern "C" int vgpu_stream_use(void *stream) {
+    if (vg_ensure_init()) return -1;
+    t_stream = static_cast<hipStream_t>(stream);
+    return 0;
+}
+
+extern "C" int vgpu_stream_destroy(void *stream) {
+    if (!stream) return 0;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (t_stream == s) t_stream = nullptr;
+    VG_TRY(hipStreamSynchronize(s));
+    vg_pool_forget_stream(s);
+    VG_TRY(hipStreamDestroy(s));
+    return 0;
+}
+
+extern "C" int vgpu_error(const char *msg) {
+    vg_set_error("%s", msg ? msg : "");
+    return -1;
+}
+
+extern "C" int vgpu_mem_info(int64_t *free_bytes, int64_t *total_bytes) {
+    if (vg_ensure_init()) return -1;
+    size_t f = 0, t = 0;
+    VG_TRY(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = (int64_t)f;
+    if (total_bytes) *total_bytes = (int64_t)t;
+    return 0;
+}
 
 extern "C" int vgpu_tune_load(const char *path);
 
@@ -92,7 +137,25 @@ int64_t vg_resident_blocks(size_t lds) {
 // ---------------------------------------------------------------------------
 
 static std::mutex g_pool_mu;
-static std::vector<void *> g_pool[64];   // free blocks of 2^k bytes
+
+// Free blocks by size class (2^k bytes), kept per stream: a block released
+// while work enqueued on stream S may still read it is handed out again
+// only to later work on S, which runs after that work.  (With one stream
+// this is the plain caching allocator.)
+struct PoolSet {
+    hipStream_t stream;
+    std::vector<void *> cls[64];
+};
+static std::vector<PoolSet *> g_pools;
+
+static PoolSet *pool_for(hipStream_t s) {   // g_pool_mu held
+    for (PoolSet *p : g_pools)
+        if (p->stream == s) return p;
+    PoolSet *p = new PoolSet();
+    p->stream = s;
+    g_pools.push_back(p);
+    return p;
+}
 
 static int pool_class(size_t bytes, size_t *rounded) {
     int k = 12;
@@ -106,9 +169,10 @@ void *vg_pool_alloc(size_t bytes, size_t *granted) {
     int k = pool_class(bytes ? bytes : 1, &r);
     {
         std::lock_guard<std::mutex> lk(g_pool_mu);
-        if (!g_pool[k].empty()) {
-            void *p = g_pool[k].back();
-            g_pool[k].pop_back();
+        PoolSet *ps = pool_for(vg_stream());
+        if (!ps->cls[k].empty()) {
+            void *p = ps->cls[k].back();
+            ps->cls[k].pop_back();
             *granted = r;
             return p;
         }
@@ -129,7 +193,7 @@ void vg_pool_free(void *ptr, size_t bytes) {
     int k = pool_class(bytes ? bytes : 1, &r);
     std::lock_guard<std::mutex> lk(g_pool_mu);
     try {
-        g_pool[k].push_back(ptr);
+        pool_for(vg_stream())->cls[k].push_back(ptr);
     } catch (const std::bad_alloc &) {
         // no room to cache it: give the block back (hipFree waits for the
         // device work that may still read it)
@@ -138,15 +202,31 @@ void vg_pool_free(void *ptr, size_t bytes) {
     }
 }
 
+// a stream about to be destroyed (and synchronized): its free blocks are
+// safe for any stream now and move to the library stream's lists
+void vg_pool_forget_stream(hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    for (size_t i = 0; i < g_pools.size(); i++) {
+        PoolSet *p = g_pools[i];
+        if (p->stream != s) continue;
+        PoolSet *dst = pool_for(g_stream);
+        for (int k = 0; k < 64; k++) dst->cls[k].insert(dst->cls[k].end(), p->cls[k].begin(), p->cls[k].end());
+        g_pools.erase(g_pools.begin() + (long)i);
+        delete p;
+        return;
+    }
+}
+
 void vg_pool_trim() {
     std::lock_guard<std::mutex> lk(g_pool_mu);
-    for (auto &v : g_pool) {
-        for (void *p : v) {
-            hipError_t e = hipFree(p);
-            (void)e;
+    for (PoolSet *ps : g_pools)
+        for (auto &v : ps->cls) {
+            for (void *p : v) {
+                hipError_t e = hipFree(p);
+                (void)e;
+            }
+            v.clear();
         }
-        v.clear();
-    }
 }
 
 extern "C" void *vgpu_malloc(size_t bytes) {
@@ -165,30 +245,30 @@ extern "C" int vgpu_free(void *p) {
 extern "C" int vgpu_memcpy_h2d(void *dst, const void *src, size_t bytes) {
     if (vg_ensure_init()) return -1;
     if (!bytes) return 0;
-    VG_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream));
-    VG_TRY(hipStreamSynchronize(g_stream));
+    VG_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, vg_stream()));
+    VG_TRY(hipStreamSynchronize(vg_stream()));
     return 0;
 }
 
 extern "C" int vgpu_memcpy_d2h(void *dst, const void *src, size_t bytes) {
     if (vg_ensure_init()) return -1;
     if (!bytes) return 0;
-    VG_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream));
-    VG_TRY(hipStreamSynchronize(g_stream));
+    VG_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, vg_stream()));
+    VG_TRY(hipStreamSynchronize(vg_stream()));
     return 0;
 }
 
 extern "C" int vgpu_memset(void *dst, int value, size_t bytes) {
     if (vg_ensure_init()) return -1;
     if (!bytes) return 0;
-    VG_TRY(hipMemsetAsync(dst, value, bytes, g_stream));
-    VG_TRY(hipStreamSynchronize(g_stream));
+    VG_TRY(hipMemsetAsync(dst, value, bytes, vg_stream()));
+    VG_TRY(hipStreamSynchronize(vg_stream()));
     return 0;
 }
 
 extern "C" int vgpu_synchronize(void) {
     if (vg_ensure_init()) return -1;
-    VG_TRY(hipStreamSynchronize(g_stream));
+    VG_TRY(hipStreamSynchronize(vg_stream()));
     return 0;
 }
 
@@ -257,13 +337,13 @@ extern "C" int vgpu_upload_async(int slot, void *dst, const void *src, size_t by
 
 extern "C" int vgpu_upload_acquire(int slot) {
     if (vg_upload_slot(slot)) return -1;
-    VG_TRY(hipStreamWaitEvent(g_stream, g_up_done[slot], 0));
+    VG_TRY(hipStreamWaitEvent(vg_stream(), g_up_done[slot], 0));
     return 0;
 }
 
 extern "C" int vgpu_upload_release(int slot) {
     if (vg_upload_slot(slot)) return -1;
-    VG_TRY(hipEventRecord(g_up_read[slot], g_stream));
+    VG_TRY(hipEventRecord(g_up_read[slot], vg_stream()));
     g_up_read_set[slot] = true;
     return 0;
 }
@@ -276,12 +356,12 @@ extern "C" int vgpu_upload_host_wait(int slot) {
 
 extern "C" int vgpu_timer_start(void) {
     if (vg_ensure_init()) return -1;
-    VG_TRY(hipEventRecord(g_t0, g_stream));
+    VG_TRY(hipEventRecord(g_t0, vg_stream()));
     return 0;
 }
 
 extern "C" double vgpu_timer_stop(void) {
-    if (hipEventRecord(g_t1, g_stream) != hipSuccess) return -1.0;
+    if (hipEventRecord(g_t1, vg_stream()) != hipSuccess) return -1.0;
     if (hipEventSynchronize(g_t1) != hipSuccess) return -1.0;
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, g_t0, g_t1) != hipSuccess) return -1.0;
@@ -296,8 +376,8 @@ static const char *g_tune_names[VG_TUNE_COUNT] = {"gb.lds.max_blocks", "gb.lds.r
                                                   "gb.star.items", "gb.part.min_groups", "batch.shared_scan",
                                                   "gb.lds.slots_per_group", "batch.bucketed",
                                                   "gb.star.dense", "join.bloom",
-                                                  "gb.lds.copies", "gb.narrow", "batch.dense", "scan.device_feed", "join.bloom_keys", "join.load_pct"};
-#define VG_TUNE_DEFAULTS {1024, 8192, 16384, 1024, 4096, 65536, 4, 65536, 1, 2, 2, 2, 2, 2, 2, 2, 2, 4, 50}
+                                                  "gb.lds.copies", "gb.narrow", "batch.dense", "scan.device_feed", "join.bloom_keys", "join.load_pct", "gb.fuse_join"};
+#define VG_TUNE_DEFAULTS {1024, 8192, 16384, 1024, 4096, 65536, 4, 65536, 1, 2, 2, 2, 2, 2, 2, 2, 2, 4, 50, 2}
 static int64_t g_tune[VG_TUNE_COUNT] = VG_TUNE_DEFAULTS;
 
 struct TuneBucket {
@@ -448,12 +528,12 @@ extern "C" double vgpu_kernel_time_ms(int64_t *launches, int reset) {
 }
 
 void vg_kt_begin() {
-    if (g_kt_on) (void)hipEventRecord(g_k0, g_stream);
+    if (g_kt_on) (void)hipEventRecord(g_k0, vg_stream());
 }
 
 void vg_kt_end() {
     if (!g_kt_on) return;
-    if (hipEventRecord(g_k1, g_stream) != hipSuccess || hipEventSynchronize(g_k1) != hipSuccess) return;
+    if (hipEventRecord(g_k1, vg_stream()) != hipSuccess || hipEventSynchronize(g_k1) != hipSuccess) return;
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, g_k0, g_k1) == hipSuccess) {
         g_kt_ms += ms;

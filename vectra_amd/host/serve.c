@@ -1,10 +1,13 @@
 /*
  * serve.c -- the request loop of include/vx_serve.h: plain (one query at a
- * time) and continuous batching (shared scans of the fact columns), over
- * device-resident columns and a resident join table.
+ * time) and continuous batching (shared scans of the fact columns, several
+ * passes in flight on streams of their own), over device-resident columns
+ * and a resident join table.
  */
-#define _POSIX_C_SOURCE 199309L
+#define _POSIX_C_SOURCE 200809L
 #include "vx_serve.h"
+
+#include <pthread.h>
 
 #include <math.h>
 #include <stdarg.h>
@@ -178,86 +181,283 @@ static int sv_pass(const vx_serve_cfg *c, vgpu_batch *b, const vx_request *reqs,
     return 0;
 }
 
+/* ---- batched: pass slots ---------------------------------------------- */
+
+#define SV_MAX_SLOTS 8
+
+struct SvSched;
+
+/* one pass slot: a host thread driving its own stream, with a pass template
+ * of each class planned on that stream */
+typedef struct {
+    struct SvSched *S;
+    pthread_t th;
+    int       started;
+    void     *stream;
+    vgpu_batch *tpl[2];
+    int64_t   tpl_bytes;
+    int       ready;       /* templates planned (1) or failed (-1) */
+    int       state;       /* 0 idle, 1 pass assigned, 2 stop */
+    int       cls, k, pass;
+    int       take[SV_MAX_BATCH];
+    char      err[512];
+} SvSlot;
+
+typedef struct SvSched {
+    pthread_mutex_t mu;
+    pthread_cond_t  cv_slot;   /* slots wait for work */
+    pthread_cond_t  cv_main;   /* the scheduler waits for a slot or an arrival */
+    const vx_serve_cfg *c;
+    vgpu_jtable *jt;
+    const vx_request *reqs;
+    vx_result *out;
+    int       kmax;
+    double    t0;
+    int       done, in_flight, failed;
+    char      err[512];
+} SvSched;
+
+static void *sv_slot_main(void *arg) {
+    SvSlot *w = (SvSlot *)arg;
+    SvSched *S = w->S;
+    int ok = vgpu_stream_use(w->stream) == 0;
+    for (int cls = 0; cls < 2 && ok; cls++) {
+        w->tpl[cls] = sv_template(S->c, S->jt, cls, S->kmax);
+        ok = w->tpl[cls] != NULL;
+    }
+    if (ok) {
+        w->tpl_bytes = vgpu_batch_device_bytes(w->tpl[0]) + vgpu_batch_device_bytes(w->tpl[1]);
+        ok = vgpu_synchronize() == 0;
+    }
+    pthread_mutex_lock(&S->mu);
+    if (!ok) snprintf(w->err, sizeof w->err, "%s", vgpu_last_error());
+    w->ready = ok ? 1 : -1;
+    pthread_cond_broadcast(&S->cv_main);
+    for (;;) {
+        while (w->state == 0) pthread_cond_wait(&S->cv_slot, &S->mu);
+        if (w->state == 2) break;
+        int take[SV_MAX_BATCH];
+        const int cls = w->cls, k = w->k, pass = w->pass;
+        memcpy(take, w->take, (size_t)k * sizeof(int));
+        pthread_mutex_unlock(&S->mu);
+        /* the pass runs on this slot's stream while other slots' passes run */
+        int rc = sv_pass(S->c, w->tpl[cls], S->reqs, take, k, S->out, S->t0, pass);
+        pthread_mutex_lock(&S->mu);
+        if (rc < 0 && !S->failed) {
+            S->failed = 1;
+            snprintf(S->err, sizeof S->err, "%s", vgpu_last_error());
+        }
+        S->done += k;
+        S->in_flight--;
+        w->state = 0;
+        pthread_cond_broadcast(&S->cv_main);
+    }
+    pthread_mutex_unlock(&S->mu);
+    for (int cls = 0; cls < 2; cls++)
+        if (w->tpl[cls]) vgpu_batch_destroy(w->tpl[cls]);
+    vgpu_stream_use(NULL);
+    return NULL;
+}
+
+/* starts slot w and waits until its templates are planned; 0 or -1 */
+static int sv_slot_open(SvSched *S, SvSlot *w) {
+    memset(w, 0, sizeof *w);
+    w->S = S;
+    w->stream = vgpu_stream_create();
+    if (!w->stream) return -1;
+    if (pthread_create(&w->th, NULL, sv_slot_main, w) != 0) {
+        vgpu_stream_destroy(w->stream);
+        w->stream = NULL;
+        return vgpu_error("serve: cannot start a pass slot thread");
+    }
+    w->started = 1;
+    pthread_mutex_lock(&S->mu);
+    while (w->ready == 0) pthread_cond_wait(&S->cv_main, &S->mu);
+    const int ok = w->ready > 0;
+    pthread_mutex_unlock(&S->mu);
+    return ok ? 0 : -1;
+}
+
+static void sv_slot_close(SvSched *S, SvSlot *w) {
+    if (!w->started) return;
+    pthread_mutex_lock(&S->mu);
+    w->state = 2;
+    pthread_cond_broadcast(&S->cv_slot);
+    pthread_mutex_unlock(&S->mu);
+    pthread_join(w->th, NULL);
+    vgpu_stream_destroy(w->stream);
+    w->started = 0;
+}
+
+static void sv_abs_deadline(double t0, double at, struct timespec *ts) {
+    /* cv_main runs on CLOCK_MONOTONIC, like sv_now */
+    double t = t0 + at;
+    ts->tv_sec = (time_t)t;
+    ts->tv_nsec = (long)((t - (double)ts->tv_sec) * 1e9);
+    if (ts->tv_nsec >= 1000000000L) {
+        ts->tv_sec++;
+        ts->tv_nsec -= 1000000000L;
+    }
+}
+
+static int sv_run_batched(const vx_serve_cfg *c, vgpu_jtable *jt, const vx_request *reqs, int n, vx_result *out,
+                          vx_serve_stats *st, int64_t resident) {
+    int kmax = c->max_batch < 1 ? 1 : (c->max_batch > SV_MAX_BATCH ? SV_MAX_BATCH : c->max_batch);
+    int want = c->in_flight <= 0 ? 2 : (c->in_flight > SV_MAX_SLOTS ? SV_MAX_SLOTS : c->in_flight);
+    SvSched S;
+    memset(&S, 0, sizeof S);
+    pthread_condattr_t ca;
+    pthread_condattr_init(&ca);
+    pthread_condattr_setclock(&ca, CLOCK_MONOTONIC);
+    pthread_mutex_init(&S.mu, NULL);
+    pthread_cond_init(&S.cv_slot, NULL);
+    pthread_cond_init(&S.cv_main, &ca);
+    pthread_condattr_destroy(&ca);
+    S.c = c;
+    S.jt = jt;
+    S.reqs = reqs;
+    S.out = out;
+    S.kmax = kmax;
+    SvSlot slots[SV_MAX_SLOTS];
+    memset(slots, 0, sizeof slots);
+    int n_slots = 0, rc = 0;
+    /* Slots open while their pass state fits the budget (at least one):
+     * each holds a template of each class planned for kmax requests. */
+    while (n_slots < want && rc == 0) {
+        if (sv_slot_open(&S, &slots[n_slots]) < 0) {
+            if (slots[n_slots].err[0]) vgpu_error(slots[n_slots].err);
+            rc = -1;
+            n_slots++;
+            break;
+        }
+        const int64_t tb = slots[n_slots].tpl_bytes;
+        if (n_slots > 0 && resident + tb > st->hbm_budget) {
+            sv_slot_close(&S, &slots[n_slots]);
+            break;
+        }
+        resident += tb;
+        n_slots++;
+    }
+    st->slots = n_slots;
+    st->resident_bytes = resident;
+    if (rc == 0 && n_slots > 0) {
+        /* a request holds its share of the larger template */
+        int64_t per = 0;
+        for (int cls = 0; cls < 2; cls++) {
+            int64_t b = vgpu_batch_device_bytes(slots[0].tpl[cls]) / kmax;
+            if (b > per) per = b;
+        }
+        st->state_bytes = per;
+        if (c->state_budget > 0 && per > 0 && c->state_budget / per < kmax)
+            kmax = (int)(c->state_budget / per) > 0 ? (int)(c->state_budget / per) : 1;
+    }
+    int *pending = (int *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int));
+    if (!pending && rc == 0) rc = vgpu_error("serve: alloc failed");
+    S.t0 = sv_now();
+    int next = 0, n_pend = 0;
+    pthread_mutex_lock(&S.mu);
+    while (rc == 0 && !S.failed && S.done < n) {
+        double now = sv_now() - S.t0;
+        while (next < n && reqs[next].arrival_s <= now) pending[n_pend++] = next++;
+        /* fill every free slot: the oldest waiting request's class, every
+         * waiting request of that class in arrival order, up to kmax */
+        for (int s = 0; s < n_slots && n_pend > 0; s++) {
+            SvSlot *w = &slots[s];
+            if (w->state != 0) continue;
+            int cls = reqs[pending[0]].cls, k = 0, wr = 0;
+            for (int j = 0; j < n_pend; j++) {
+                int i = pending[j];
+                if (reqs[i].cls == cls && k < kmax) w->take[k++] = i;
+                else pending[wr++] = i;
+            }
+            n_pend = wr;
+            w->cls = cls;
+            w->k = k;
+            w->pass = st->n_passes++;
+            w->state = 1;
+            S.in_flight++;
+            if (S.in_flight > st->max_in_flight) st->max_in_flight = S.in_flight;
+            if (k > st->max_pass) st->max_pass = k;
+            pthread_cond_broadcast(&S.cv_slot);
+        }
+        if (S.done >= n || S.failed) break;
+        /* wait for a slot to free or for the next arrival */
+        if (next < n && (n_pend == 0 || S.in_flight < n_slots)) {
+            struct timespec ts;
+            sv_abs_deadline(S.t0, reqs[next].arrival_s, &ts);
+            pthread_cond_timedwait(&S.cv_main, &S.mu, &ts);
+        } else {
+            pthread_cond_wait(&S.cv_main, &S.mu);
+        }
+    }
+    /* let passes in flight finish */
+    while (S.in_flight > 0) pthread_cond_wait(&S.cv_main, &S.mu);
+    pthread_mutex_unlock(&S.mu);
+    st->n_done = S.done;
+    st->wall_s = sv_now() - S.t0 - (n > 0 ? reqs[0].arrival_s : 0);
+    for (int s = 0; s < n_slots; s++) sv_slot_close(&S, &slots[s]);
+    free(pending);
+    if (rc == 0 && S.failed) rc = vgpu_error(S.err);
+    pthread_cond_destroy(&S.cv_slot);
+    pthread_cond_destroy(&S.cv_main);
+    pthread_mutex_destroy(&S.mu);
+    return rc;
+}
+
+static int64_t sv_col_bytes(const vgpu_col *col, int64_t n) {
+    if (!col->data) return 0;
+    return n * (col->type == VGPU_BOOL ? 1 : 8) + (col->validity ? (n + 7) / 8 : 0);
+}
+
 int vx_serve_run(const vx_serve_cfg *c, const vx_request *reqs, int n, vx_result *out, vx_serve_stats *st) {
     memset(st, 0, sizeof *st);
     memset(out, 0, (size_t)n * sizeof *out);
-    int kmax = c->max_batch < 1 ? 1 : (c->max_batch > SV_MAX_BATCH ? SV_MAX_BATCH : c->max_batch);
     double s0 = sv_now();
+    int64_t total = 0;
+    if (vgpu_mem_info(NULL, &total) < 0) return -1;
+    st->hbm_budget = c->hbm_budget > 0 ? c->hbm_budget : total;
     /* the dimension's join table stays resident for every class-1 request */
     vgpu_jtable *jt = vgpu_join_build_payload(&c->dim_key, c->n_dim, &c->dim_attr, 1);
     if (!jt) return -1;
-    vgpu_batch *tpl[2] = {NULL, NULL};
+    int64_t resident = sv_col_bytes(&c->fact_key, c->n_rows) + sv_col_bytes(&c->fact_value, c->n_rows) +
+                       sv_col_bytes(&c->fact_group, c->n_rows) + sv_col_bytes(&c->dim_key, c->n_dim) +
+                       sv_col_bytes(&c->dim_attr, c->n_dim) + vgpu_join_device_bytes(jt);
     int rc = 0;
-    if (c->batched) {
-        for (int cls = 0; cls < 2 && rc == 0; cls++) {
-            tpl[cls] = sv_template(c, jt, cls, kmax);
-            if (!tpl[cls]) rc = -1;
-        }
-        if (rc == 0) {
-            /* admission by the query-state budget: a request holds its share
-             * of the larger pass template */
-            int64_t per = 0;
-            for (int cls = 0; cls < 2; cls++) {
-                int64_t bytes = vgpu_batch_device_bytes(tpl[cls]);
-                st->resident_bytes += bytes;
-                if (bytes / kmax > per) per = bytes / kmax;
-            }
-            st->state_bytes = per;
-            if (c->state_budget > 0 && per > 0 && c->state_budget / per < kmax)
-                kmax = (int)(c->state_budget / per) > 0 ? (int)(c->state_budget / per) : 1;
-        }
+    if (resident > st->hbm_budget) {
+        vgpu_join_destroy(jt);
+        char msg[160];
+        snprintf(msg, sizeof msg, "serve: resident columns and join table (%lld bytes) exceed the budget of %lld",
+                 (long long)resident, (long long)st->hbm_budget);
+        return vgpu_error(msg);
     }
+    /* the pass slots plan on their own streams: the table must be built first */
+    if (vgpu_synchronize() < 0) rc = -1;
+    if (rc == 0 && c->batched) {
+        st->setup_s = sv_now() - s0;
+        rc = sv_run_batched(c, jt, reqs, n, out, st, resident);
+        vgpu_join_destroy(jt);
+        return rc;
+    }
+    st->resident_bytes = resident;
     st->setup_s = sv_now() - s0;
-    int *pending = (int *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int));
-    int *take = (int *)malloc((size_t)kmax * sizeof(int));
-    if (!pending || !take) rc = -1;
     double t0 = sv_now();
-    int next = 0, n_pend = 0, done = 0;
-    while (rc == 0 && done < n) {
-        double now = sv_now() - t0;
-        while (next < n && reqs[next].arrival_s <= now) pending[n_pend++] = next++;
-        if (n_pend == 0) {
-            sv_sleep_until(t0, reqs[next].arrival_s);
-            continue;
-        }
-        if (!c->batched) {
-            int i = pending[0];
-            memmove(pending, pending + 1, (size_t)(n_pend - 1) * sizeof(int));
-            n_pend--;
-            out[i].start_s = sv_now() - t0;
-            if (sv_plain_one(c, jt, &reqs[i], &out[i]) < 0) {
-                rc = -1;
-                break;
-            }
-            out[i].end_s = sv_now() - t0;
-            out[i].pass = -1;
-            out[i].batch = 1;
-            done++;
-            continue;
-        }
-        /* the next pass: the oldest request's class, every waiting request
-         * of that class in arrival order, up to kmax */
-        int cls = reqs[pending[0]].cls, k = 0, w = 0;
-        for (int j = 0; j < n_pend; j++) {
-            int i = pending[j];
-            if (reqs[i].cls == cls && k < kmax) take[k++] = i;
-            else pending[w++] = i;
-        }
-        n_pend = w;
-        if (sv_pass(c, tpl[cls], reqs, take, k, out, t0, st->n_passes) < 0) {
+    int done = 0;
+    for (int i = 0; rc == 0 && i < n; i++) {
+        sv_sleep_until(t0, reqs[i].arrival_s);
+        out[i].start_s = sv_now() - t0;
+        if (sv_plain_one(c, jt, &reqs[i], &out[i]) < 0) {
             rc = -1;
             break;
         }
-        st->n_passes++;
-        if (k > st->max_pass) st->max_pass = k;
-        done += k;
+        out[i].end_s = sv_now() - t0;
+        out[i].pass = -1;
+        out[i].batch = 1;
+        done++;
     }
     st->n_done = done;
+    st->max_in_flight = done > 0 ? 1 : 0;
+    st->slots = 1;
     st->wall_s = sv_now() - t0 - (n > 0 ? reqs[0].arrival_s : 0);
-    free(pending);
-    free(take);
-    for (int cls = 0; cls < 2; cls++)
-        if (tpl[cls]) vgpu_batch_destroy(tpl[cls]);
     vgpu_join_destroy(jt);
     return rc;
 }

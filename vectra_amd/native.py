@@ -143,12 +143,13 @@ class VxServeCfg(C.Structure):
     _fields_ = [("fact_key", VgpuCol), ("fact_value", VgpuCol), ("fact_group", VgpuCol), ("n_rows", C.c_int64),
                 ("dim_key", VgpuCol), ("dim_attr", VgpuCol), ("n_dim", C.c_int64), ("batched", C.c_int32),
                 ("max_batch", C.c_int32), ("state_budget", C.c_int64), ("keep_results", C.c_int32),
-                ("pad", C.c_int32)]
+                ("in_flight", C.c_int32), ("hbm_budget", C.c_int64)]
 
 
 class VxServeStats(C.Structure):
     _fields_ = [("n_done", C.c_int64), ("n_passes", C.c_int32), ("max_pass", C.c_int32), ("wall_s", C.c_double),
-                ("setup_s", C.c_double), ("state_bytes", C.c_int64), ("resident_bytes", C.c_int64)]
+                ("setup_s", C.c_double), ("state_bytes", C.c_int64), ("resident_bytes", C.c_int64),
+                ("max_in_flight", C.c_int32), ("slots", C.c_int32), ("hbm_budget", C.c_int64)]
 
 
 class VgpuFeedSpec(C.Structure):
@@ -1066,7 +1067,8 @@ class Batch:
 
 
 def serve(fact: dict, dim: dict, requests: Sequence[tuple], batched: bool, max_batch: int = 32,
-          state_budget: int = 0, keep_results: bool = True) -> tuple[list[dict], dict]:
+          state_budget: int = 0, keep_results: bool = True, in_flight: int = 2,
+          hbm_budget: int = 0) -> tuple[list[dict], dict]:
     """Runs the C request loop (include/vx_serve.h) over device columns.
     fact: {"key", "value", "group"} DeviceColumns; dim: {"key", "attr"};
     requests: (cls, threshold, arrival_s) in arrival order."""
@@ -1080,6 +1082,8 @@ def serve(fact: dict, dim: dict, requests: Sequence[tuple], batched: bool, max_b
     cfg.max_batch = max_batch
     cfg.state_budget = state_budget
     cfg.keep_results = 1 if keep_results else 0
+    cfg.in_flight = in_flight
+    cfg.hbm_budget = hbm_budget
     reqs = (VxRequest * max(n, 1))()
     for i, (cls, t, at) in enumerate(requests):
         reqs[i].cls, reqs[i].threshold, reqs[i].arrival_s = cls, t, at
