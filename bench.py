@@ -1,29 +1,34 @@
 #!/usr/bin/env python3
-"""Benchmark of the join-then-aggregate hot path on one MI355X.
+"""Benchmark of vectra's relational hot path on one MI355X.
 
-Default workload (the headline configuration): a 50,000,000-row fact table
-(fkey int64 Zipf(s=1.1) over the 1,000,000 keys of a dimension table, 10% of
-the rows drawn from keys absent from it; value double lognormal(0,1)) joined
-to that dimension (unique shuffled int64 keys; attr uniform in [0,100);
-weight U(0,1)).  One step runs both queries of the configuration,
+Default workload (c5, the largest configuration that fits one GPU): the
+serving loop over device-resident tables.  Fact 1,000,000,000 rows (key
+int64 Zipf(1.0) over the 10,000,000 keys of a dimension table, value
+U(0,1000), category U{0..999}; about 24 GB), dimension 10,000,000 rows
+(unique shuffled int64 key, attr U{0..255}).  Poisson arrivals at a swept
+offered load: 70% of the requests are
 
-    inner_join(fact, dim, by = c(fkey = "key")) |> group_by(attr) |>
-        summarise(s = sum(value * weight), n = n())
-    left_join(...)  |> the same aggregation
+    tbl |> filter(value > t) |> group_by(category) |> summarise(s = sum(value), n = n())
 
-on device-resident columns: build of the dimension's hash table, one fused
-probe -> aggregate pass per query, ordering of the groups by key (NA last)
-and the copy of the 101-row results to the host.  `value` is fact rows
-processed per second (2 x 50M per step).
+and 30% filter(value > t) |> inner_join(dim, by = c(key = "key")) |>
+group_by(attr) |> summarise(...), with t drawn for 1-50% selectivity.  Each
+load point runs twice over the same trace shape: one query at a time
+(plain) and continuously batched (several requests of a class share one
+scan; two passes on the device at once; slots admitted by device memory).
+`value` is the batched mode's highest achieved queries/s with p99 latency
+within the SLO (100 ms); the plain mode's figure at the same SLO and their
+ratio are in the line.  The roofline is that of the serving pass's scan
+kernel (k_bt_chain_feed, 8 class-0 queries over the 1B rows), timed with
+HIP events over --steps passes after --warmup.
 
-Other workloads (--workload): c1 (filter -> group-by through the drop-in
-nodes, 1M rows in 10 row groups), c3-high / c3-low (group-by over 200M rows
+Other workloads (--workload): c2 (join-then-aggregate, 50M x 1M rows,
+inner and left, through the kernel API), c1 (filter -> group-by through
+the drop-in nodes, 1M rows in 10 row groups), c1-vtr (the same from a .vtr
+file fed straight to the device), c3-high / c3-low (group-by over 200M rows
 with 50M / 16 keys, 9 aggregates), c4 (4096 uneven filter -> group-by
-queries in one persistent launch, replayed as a hipGraph), c5 (the serving
-loop over a 1B-row fact table: p50 / p99 latency against offered load,
-continuous batching vs one query at a time).
+queries in one persistent launch, replayed as a hipGraph), strings.
 
-Prints ONE JSON line.  Data is synthetic (seeded numpy generators).
+Prints ONE JSON line.  Data is synthetic (seeded generators).
 """
 from __future__ import annotations
 
@@ -45,6 +50,36 @@ HBM_PEAK_GBS = 8000.0
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def pinned(fn):
+    """Runs fn() on one core (the calling thread's affinity set to the last
+    CPU it may use, as `taskset -c N` does), then restores the affinity.
+    Returns (fn's result, a description of the host)."""
+    allowed = sorted(os.sched_getaffinity(0))
+    core = allowed[-1]
+    os.sched_setaffinity(0, {core})
+    try:
+        out = fn()
+    finally:
+        os.sched_setaffinity(0, allowed)
+    return out, {"pinned_core": core, "cpu_model": cpu_model(), "nproc": len(allowed),
+                 "host_cpus": os.cpu_count()}
+
+
+ORACLE_NOTE = ("the reference's C engine needs R's headers (R.h, Rinternals.h) to build, which this image lacks, "
+               "so the baseline runs the oracle: a single-threaded C restatement of the reference's filter, "
+               "hash join and SortNode-fed group-by (oracle/oracle.c)")
 
 
 # ---------------------------------------------------------------- data
@@ -169,10 +204,12 @@ def run_c2(args, nv):
     if not args.no_cpu:
         sample = min(args.cpu_sample, n_fact)
         log(f"[bench] cpu baseline on {sample:,} fact rows")
-        secs = cpu_baseline_c2(nv, fact, dim, sample)
+        secs, hostinfo = pinned(lambda: cpu_baseline_c2(nv, fact, dim, sample))
         cpu = {"value": 2 * sample / secs, "unit": "rows/s", "cores": 1, "kind": "port",
                "sample": f"first {sample:,} fact rows of the same tables, inner + left join -> group_by(attr) "
-                         f"(oracle restatement of the reference engine, {secs:.1f} s)"}
+                         f"({secs:.1f} s); one thread pinned to core {hostinfo['pinned_core']} of "
+                         f"{hostinfo['nproc']} ({hostinfo['cpu_model']}); {ORACLE_NOTE}",
+               **hostinfo}
     gbs = (2 * per_launch) / (ms / 1e3) / 1e9
     traffic = None
     tfile = ROOT / "profiles" / "k_gb_star_join_aggregate_50M.json"
@@ -735,15 +772,17 @@ def run_c5(args, nv):
             n = int(min(max(60, rate * args.c5_seconds), 2000))
             reqs = c5_requests(rate, n, seed=int(rate * 1000) + (7 if mode == "batched" else 3))
             res, st = nv.serve(fact, dim, reqs, batched=(mode == "batched"), max_batch=args.c5_max_batch,
-                               keep_results=False)
+                               keep_results=False, in_flight=args.c5_in_flight, hbm_budget=args.c5_hbm_budget)
             lat = np.array([r["end_s"] - a for r, (_, _, a) in zip(res, reqs)]) * 1e3
             qps = n / max(st["wall_s"], 1e-9)
             pt = {"mode": mode, "offered_qps": rate, "requests": n, "achieved_qps": qps,
                   "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-                  "passes": st["n_passes"], "max_pass": st["max_pass"]}
+                  "passes": st["n_passes"], "max_pass": st["max_pass"], "max_in_flight": st["max_in_flight"],
+                  "slots": st["slots"], "hbm_budget": st["hbm_budget"], "resident_bytes": st["resident_bytes"]}
             sweep.append(pt)
             log(f"[bench] c5 {mode:7s} offered {rate:6.0f}/s: achieved {qps:7.1f}/s p50 {pt['p50_ms']:8.2f} ms "
-                f"p99 {pt['p99_ms']:8.2f} ms (passes {st['n_passes']}, largest {st['max_pass']})")
+                f"p99 {pt['p99_ms']:8.2f} ms (passes {st['n_passes']}, largest {st['max_pass']}, "
+                f"in flight {st['max_in_flight']} of {st['slots']})")
 
     def best(mode):
         ok = [p for p in sweep if p["mode"] == mode and p["p99_ms"] <= slo]
@@ -767,15 +806,18 @@ def run_c5(args, nv):
         side = {"key": fact["group"]} if cls == 0 else {"probe_key": fact["key"]}
         b.plan([dict(side, inputs=[fact["value"]], pred_cols=[fact["value"]], n_rows=n_fact, param=t) for t in thr])
         bucketed = b.bucketed()
+        for _ in range(args.warmup):
+            b.rerun(thr, K)
         lib.vgpu_kernel_timing(1)
         lib.vgpu_kernel_time_ms(None, 1)
-        for _ in range(3):
+        for _ in range(args.steps):
             b.rerun(thr, K)
         nl = C.c_int64()
-        pass_ms[cls] = lib.vgpu_kernel_time_ms(C.byref(nl), 1) / 3
+        pass_ms[cls] = lib.vgpu_kernel_time_ms(C.byref(nl), 1) / max(nl.value, 1)
         lib.vgpu_kernel_timing(0)
         b.free()
-        log(f"[bench] c5 class-{cls} pass of {K} queries (bucketed {bucketed}): {pass_ms[cls]:.3f} ms")
+        log(f"[bench] c5 class-{cls} pass of {K} queries (bucketed {bucketed}): k_bt_chain_feed "
+            f"{pass_ms[cls]:.3f} ms avg over {nl.value} launches")
     jt.free()
     kms = pass_ms[0]
     alg = n_fact * 16   # value + group, read once per pass
@@ -785,28 +827,41 @@ def run_c5(args, nv):
     cpu = None
     if not args.no_cpu:
         nv.load_oracle(ROOT / "oracle" / "_build" / "liboracle.so")
-        m = min(args.cpu_sample, n_fact)
+        m = min(args.c5_cpu_rows, n_fact)
         host = {k: v[:m].cpu().numpy() for k, v in fact_t.items()}
         dh = {k: v.cpu().numpy() for k, v in dim_t.items()}
-        secs = {}
-        for cls in (0, 1):
-            t1 = time.perf_counter()
-            f = nv.filter_node("oracle", nv.Table(host, rg_rows=1 << 20).scan(),
-                               ("cmp", ">", ("col", "value"), ("dbl", 750.0)))
-            if cls == 1:
-                f = nv.join_node("oracle", f, nv.Table({"dk": dh["key"], "attr": dh["attr"]}, rg_rows=1 << 20).scan(),
-                                 "inner", [(0, 0)])
-            nv.collect(nv.group_agg_node("oracle", f, ["group" if cls == 0 else "attr"],
-                                         [("s", "sum", "value", 0), ("n", "n", None, 0)], "/tmp"))
-            secs[cls] = time.perf_counter() - t1
+        fact_tbl = nv.Table(host, rg_rows=1 << 20)
+        dim_tbl = nv.Table({"dk": dh["key"], "attr": dh["attr"]}, rg_rows=1 << 20)
+
+        def both():
+            secs = {}
+            for cls in (0, 1):
+                t1 = time.perf_counter()
+                f = nv.filter_node("oracle", fact_tbl.scan(), ("cmp", ">", ("col", "value"), ("dbl", 750.0)))
+                if cls == 1:
+                    f = nv.join_node("oracle", f, dim_tbl.scan(), "inner", [(0, 0)])
+                nv.collect(nv.group_agg_node("oracle", f, ["group" if cls == 0 else "attr"],
+                                             [("s", "sum", "value", 0), ("n", "n", None, 0)], "/tmp"))
+                secs[cls] = time.perf_counter() - t1
+            return secs
+
+        secs, hostinfo = pinned(both)
         per_query = 0.7 * secs[0] * n_fact / m + 0.3 * secs[1] * n_fact / m
         cpu = {"value": 1.0 / per_query, "unit": "queries/s", "cores": 1, "kind": "port",
                "sample": f"one class-0 and one class-1 query (25% selectivity) on the first {m:,} fact rows "
-                         f"({secs[0]:.1f} s / {secs[1]:.1f} s, oracle restatement of the reference engine), "
-                         f"extrapolated to {n_fact:,} rows and the 70/30 mix"}
+                         f"({secs[0]:.1f} s / {secs[1]:.1f} s), extrapolated to {n_fact:,} rows and the 70/30 "
+                         f"mix; one thread pinned to core {hostinfo['pinned_core']} of {hostinfo['nproc']} "
+                         f"({hostinfo['cpu_model']}); {ORACLE_NOTE}",
+               **hostinfo}
+    traffic = None
+    tfile = ROOT / "profiles" / "k_bt_chain_feed_serving_1B_rows.json"
+    if tfile.exists() and n_fact == 1_000_000_000 and K == 8:
+        traffic = json.loads(tfile.read_text())["traffic_bytes_per_launch"]
+    # a step is one request: steps = the requests served at the reported load
     return {
         "metric": METRIC, "value": bb["achieved_qps"] if bb else 0.0, "unit": "queries/s", "n_gpus": 1,
-        "steps": len(sweep), "warmup": 0, "ms_per_step": (1e3 / bb["achieved_qps"]) if bb else None,
+        "steps": bb["requests"] if bb else 0, "warmup": args.warmup,
+        "ms_per_step": (1e3 / bb["achieved_qps"]) if bb else None,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (torch generators on the device)",
         "config": {"workload": f"serving loop over device-resident tables: fact {n_fact:,} rows (key Zipf(1.0) over "
@@ -818,9 +873,11 @@ def run_c5(args, nv):
                    "fact_rows": n_fact, "dim_rows": n_dim, "max_batch": args.c5_max_batch, "slo_p99_ms": slo},
         "batched_at_slo": bb, "plain_at_slo": bp, "batched_vs_plain_at_slo": ratio, "sweep": sweep,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": f"k_bt_chain_feed -> reduce -> expand (bucketed pass of {K} group-by queries, "
-                               "one scan)", "bytes_per_launch": alg, "avg_launch_ms": kms},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/k_bt_chain_feed_serving_1B_rows.md"
+                     if traffic else None,
+                     "kernel": f"k_bt_chain_feed<true> (the scan of a bucketed pass of {K} class-0 queries)",
+                     "bytes_per_launch": alg, "avg_launch_ms": kms, "launches_timed": args.steps},
         "pass_ms": {"class0_group_by": pass_ms[0], "class1_join_group_by": pass_ms[1], "queries_per_pass": K},
         "cpu_baseline": cpu,
     }
@@ -831,13 +888,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c1-vtr", "c2", "c3-high", "c3-low", "c4", "c5",
+    ap.add_argument("--workload", default="c5", choices=["c1", "c1-vtr", "c2", "c3-high", "c3-low", "c4", "c5",
                                                              "strings"])
     ap.add_argument("--c4-queries", type=int, default=4096)
     ap.add_argument("--c4-card", type=int, default=0, help="fix every query's key cardinality (0: log-uniform)")
     ap.add_argument("--c5-rows", type=int, default=1_000_000_000)
     ap.add_argument("--c5-dim", type=int, default=10_000_000)
     ap.add_argument("--c5-max-batch", type=int, default=8)
+    ap.add_argument("--c5-cpu-rows", type=int, default=100_000_000, help="fact rows of the CPU baseline's sample")
+    ap.add_argument("--c5-in-flight", type=int, default=2, help="passes on the device at once (batched mode)")
+    ap.add_argument("--c5-hbm-budget", type=int, default=0, help="device bytes the serving run may hold (0: all)")
     ap.add_argument("--c5-seconds", type=float, default=3.0)
     ap.add_argument("--c5-slo-ms", type=float, default=100.0)
     ap.add_argument("--c5-plain-rates", type=float, nargs="*", default=[10, 25, 40, 55, 70, 85, 100])
@@ -891,10 +951,10 @@ def main():
         out = run_c3(args, nv, 16)
     elif args.workload == "c4":
         out = run_c4(args, nv)
-    elif args.workload == "c5":
-        out = run_c5(args, nv)
-    else:
+    elif args.workload == "c2":
         out = run_c2(args, nv)
+    else:
+        out = run_c5(args, nv)
     print(json.dumps(out), flush=True)
 
 

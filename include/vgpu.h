@@ -509,10 +509,11 @@ int    vgpu_timer_start(void);
 double vgpu_timer_stop(void);
 
 /* Per-launch timing of the main kernel of vgpu_filter /
- * vgpu_groupby_feed / vgpu_groupby_feed_join: when enabled, each of those
- * calls brackets its kernel with events on the library stream and adds the
- * elapsed time here.  Returns the accumulated milliseconds and launch count
- * since the last reset. */
+ * vgpu_groupby_feed / vgpu_groupby_feed_join / vgpu_batch_run (the scan
+ * kernel of a bucketed pass, the feed kernels of the others): when
+ * enabled, each of those calls brackets its kernel with events on the
+ * stream it runs on and adds the elapsed time here.  Returns the
+ * accumulated milliseconds and launch count since the last reset. */
 int    vgpu_kernel_timing(int enable);
 double vgpu_kernel_time_ms(int64_t *launches, int reset);
 

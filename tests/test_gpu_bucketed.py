@@ -244,20 +244,30 @@ def test_bucketed_join(gpu, oracle, dup):
         c.free()
 
 
+@pytest.mark.parametrize("n", [260_000, 0, 1, 1_000, 8_193, 1_048_577])
+@pytest.mark.parametrize("plain", [False, True])
 @pytest.mark.parametrize("join", [False, True])
-def test_bucketed_sums_the_compared_column(gpu, oracle, join):
+def test_bucketed_sums_the_compared_column(gpu, oracle, join, plain, n):
     """filter(p > t_q) |> [inner_join(dim) |>] group_by(...) |> summarise(
     n = n(), c = count(p), s = sum(p), m = mean(p)): the summed input is the
     compared column itself (the scan keeps one column per row, eight rows
-    per thread and step)"""
-    n = 260_000
+    per thread and step).  plain: the group / probe key has no NAs either,
+    so the scan loads both columns without validity checks (rows past the
+    end of a block's range masked out): row counts below one step, one past
+    a step and one past a power of two leave partial last steps"""
+    if n != 260_000 and not plain:
+        pytest.skip("the row-count sweep is for the plain scan")
     if join:
-        tbl, dim = join_tables(101, n, 20_000, False)
+        tbl, dim = join_tables(101 + n, n, 20_000, False)
         key = "attr"
+        if plain:
+            tbl["fk"] = (tbl["fk"][0], None)
     else:
-        tbl, dim = make_table(103, n), None
+        tbl, dim = make_table(103 + n, n), None
         key = "g"
-    r = rng(107)
+        if plain:
+            tbl["g"] = (tbl["g"][0], None)
+    r = rng(107 + n)
     p = r.normal(0.0, 1.0, n)
     p[::1013] = 0.5                          # rows equal to a threshold
     tbl["p"] = p
@@ -279,7 +289,7 @@ def test_bucketed_sums_the_compared_column(gpu, oracle, join):
         owned.append(gc)
         qs = [{"key": gc, "inputs": [pc], "pred_cols": [pc], "n_rows": n, "param": t} for t in ts]
     b.plan(qs)
-    assert b.bucketed()
+    assert b.bucketed() or n == 0
     cols = {"p": p, "ap": np.abs(p)}
     cols.update({"fk": tbl["fk"]} if join else {"g": tbl["g"]})
     for q, t in enumerate(ts):
@@ -299,3 +309,60 @@ def test_bucketed_sums_the_compared_column(gpu, oracle, join):
         jt.free()
     for c in owned:
         c.free()
+
+
+I64_T = [0, 5, 5, -300, 999, 2**63 - 1, -(2**63) + 1, 42, -7]
+DBL_T9 = [0.0, -0.0, 0.5, 0.5, float("nan"), float("inf"), -float("inf"), -1.2, 1.7]
+
+
+@pytest.mark.parametrize("k", [8, 9])
+@pytest.mark.parametrize("dtype", ["double", "int64"])
+@pytest.mark.parametrize("op", list(OPS))
+def test_bucketed_plain_scan_every_operator(gpu, oracle, op, dtype, k):
+    """the plain scan (no NAs in the compared column or the key, one sum of
+    the compared column) with up to eight queries counts each row's passes
+    against the constants in registers; nine take the search: every
+    operator on double and int64 columns, NaN / +-0 / +-inf and extreme
+    int64 constants (INT64_MAX, INT64_MIN + 1), duplicate constants,
+    values equal to a constant"""
+    n = 200_003
+    r = rng(211 + 7 * k + len(op) + (dtype == "int64"))
+    g = r.integers(-40, 250, n)
+    if dtype == "double":
+        p = dbl_with_specials(r, n, 0.02, 0.03)
+        u = r.random(n)
+        p[u < 0.01] = np.inf
+        p[(u >= 0.01) & (u < 0.02)] = -np.inf
+        p[(u >= 0.02) & (u < 0.04)] = 0.5
+        ts = DBL_T9[:k]
+        ptype, lit = gpu.VEC_DOUBLE, "dbl"
+    else:
+        p = r.integers(-1000, 1000, n)
+        u = r.random(n)
+        p[u < 0.01] = 10**12                  # large values, no int64 sum overflows
+        p[(u >= 0.01) & (u < 0.02)] = -10**12
+        p[(u >= 0.02) & (u < 0.05)] = 5
+        ts = I64_T[:k]
+        ptype, lit = gpu.VEC_INT64, "int"
+    pc, gc = gpu.DeviceColumn(p), gpu.DeviceColumn(g)
+    pred = gpu.cmp_const_prog(0, ptype, OPS[op], 0.0 if dtype == "double" else 0)
+    lib_aggs = [("n", -1, 0), ("count", 0, 0), ("sum", 0, 0), ("mean", 0, 0)]
+    b = gpu.Batch(gpu.VEC_INT64, [ptype], lib_aggs, gpu.VGPU_ORDER_KEY, pred, 1, 1)
+    b.plan([{"key": gc, "inputs": [pc], "pred_cols": [pc], "n_rows": n, "param": t} for t in ts])
+    assert b.bucketed()
+    ap = np.abs(np.nan_to_num(p.astype(np.float64), nan=0.0, posinf=0.0, neginf=0.0))
+    cols = {"p": p, "ap": ap, "g": g}
+    for q, t in enumerate(ts):
+        node = gpu.filter_node("oracle", gpu.Table(cols, rg_rows=0).scan(), ("cmp", op, ("col", "p"), (lit, t)))
+        out = gpu.collect(gpu.group_agg_node(
+            "oracle", node, ["g"], [("n", "n", None, 0), ("c", "count", "p", 0), ("s", "sum", "p", 0),
+                                    ("m", "mean", "p", 0), ("__sa", "sum", "ap", 1)], "/tmp"))
+        sa = out.pop("__sa")[0]
+        got = b.fetch(q)
+        got = {"g": got["key"], "n": got["a0"], "c": got["a1"], "s": got["a2"], "m": got["a3"]}
+        # int64 sums are exact, and so are their means (sum / n)
+        close = {"s": sa, "m": sa / np.maximum(out["c"][0], 1)} if dtype == "double" else {}
+        assert_tables_equal(got, out, close)
+    b.free()
+    pc.free()
+    gc.free()

@@ -691,6 +691,50 @@ __device__ __forceinline__ int bt_chain_T(const BtChainArgs &A, const uint64_t *
     return lo;
 }
 
+// Up to BT_CH_LIN active queries over a plain scan whose comparison and
+// column share one type: T counted branch-free against the constants held
+// in registers, the operator fixed at compile time.  With cmp = (a>b)-(a<b)
+// (a NaN operand gives 0), `>` passes iff v > t, `<` iff v < t, `>=` iff
+// !(v < t) and `<=` iff !(v > t): GREATER counts v > t, else v < t, and NEG
+// (>= / <=) takes K minus the count.  Slots past K hold a constant that no
+// value passes the counted comparison with (+inf / INT64_MAX for v > t,
+// -inf / INT64_MIN for v < t), so the count runs over all BT_CH_LIN slots.
+#define BT_CH_LIN 8
+// TM (T mode): 0 = the search above; 1 + 2*int64 + less: the count
+#define BT_TM_SEARCH 0
+__host__ __device__ constexpr int bt_tm(bool f64, bool greater) { return 1 + (f64 ? 0 : 2) + (greater ? 0 : 1); }
+
+template <int TM>
+__device__ __forceinline__ int bt_chain_T_lin(const uint64_t (&kr)[BT_CH_LIN], uint64_t bits, int K, bool neg) {
+    constexpr bool F64 = TM == 1 || TM == 2;
+    constexpr bool GREATER = TM == 1 || TM == 3;
+    int c = 0;
+#pragma unroll
+    for (int i = 0; i < BT_CH_LIN; i++) {
+        bool pass;
+        if (F64) {
+            const double v = vg_bits_to_f64(bits), t = vg_bits_to_f64(kr[i]);
+            pass = GREATER ? v > t : v < t;
+        } else {
+            const int64_t v = (int64_t)bits, t = (int64_t)kr[i];
+            pass = GREATER ? v > t : v < t;
+        }
+        c += pass ? 1 : 0;
+    }
+    return neg ? K - c : c;
+}
+
+// the constants of a linear count in registers (uniform loads), padded
+template <int TM>
+__device__ __forceinline__ void bt_chain_load_lin(const BtChainArgs &A, uint64_t (&kr)[BT_CH_LIN]) {
+    constexpr bool F64 = TM == 1 || TM == 2;
+    constexpr bool GREATER = TM == 1 || TM == 3;
+    const uint64_t pad = F64 ? (GREATER ? 0x7FF0000000000000ull : 0xFFF0000000000000ull)
+                             : (GREATER ? 0x7FFFFFFFFFFFFFFFull : 0x8000000000000000ull);
+#pragma unroll
+    for (int i = 0; i < BT_CH_LIN; i++) kr[i] = i < A.K ? A.ks[i] : pad;
+}
+
 // dense entry of a group key, the NA group last; -1 outside the plan's range
 __device__ __forceinline__ int64_t bt_chain_entry(const BtChainArgs &A, bool valid, uint64_t bits) {
     if (!valid) return A.E - 1;
@@ -711,6 +755,16 @@ __device__ __forceinline__ void bt_chain_add(const BtChainArgs &A, uint32_t *cnt
     }
 }
 
+// bt_chain_add for one sum (n_sum == 1) of a known type
+template <bool F64>
+__device__ __forceinline__ void bt_chain_add_one(const BtChainArgs &A, uint32_t *cnt, uint64_t *sums, int64_t e,
+                                                 int T, uint64_t v) {
+    const int64_t idx = e * A.m + (T - 1);
+    atomicAdd(&cnt[idx], 1u);
+    if (F64) atomicAdd(reinterpret_cast<double *>(&sums[idx]), vg_bits_to_f64(v));
+    else atomicAdd((unsigned long long *)&sums[idx], (unsigned long long)v);
+}
+
 // one thread's rows of one step: compared value and group / probe key of
 // each, the summed inputs (SP: every sum reads the compared column, so none
 // is kept), bit u of ok / kv: row u is in range with a valid compared
@@ -722,9 +776,31 @@ struct BtRows {
     uint32_t ok, kv;
 };
 
-template <int U, bool SP>
+// PLAIN: the compared column and the key are 8-byte columns without
+// validity and every sum reads the compared column; rows past r1 read row
+// r1 - 1 (the block's range is not empty) and are masked out, so every load
+// of the step is issued without a branch and none is waited on here.
+template <int U, bool SP, bool PLAIN>
 __device__ __forceinline__ void bt_chain_rows(const BtChainArgs &A, const vgpu_col &kc, int64_t base, int64_t r1,
                                               BtRows<U, SP> &R) {
+    if (PLAIN) {
+        static_assert(!PLAIN || SP, "the plain scan keeps no separate sum inputs");
+        const uint64_t *pd = static_cast<const uint64_t *>(A.pcol.data);
+        const uint64_t *kd = static_cast<const uint64_t *>(kc.data);
+        uint32_t in_mask = 0;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int64_t row = base + (int64_t)u * BT_CH_THREADS;
+            const bool in = row < r1;
+            const int64_t rc = in ? row : r1 - 1;
+            R.pv[u] = __builtin_nontemporal_load(pd + rc);
+            R.kb[u] = __builtin_nontemporal_load(kd + rc);
+            in_mask |= (uint32_t)in << u;
+        }
+        R.ok = in_mask;
+        R.kv = in_mask;
+        return;
+    }
     R.ok = 0;
     R.kv = 0;
 #pragma unroll
@@ -750,14 +826,16 @@ __device__ __forceinline__ void bt_chain_rows(const BtChainArgs &A, const vgpu_c
 
 // the bucket of every row of a step that passes at least one query, and
 // its aggregation into the block's table
-template <int U, bool SP>
+template <int TM, int U, bool SP>
 __device__ __forceinline__ void bt_chain_rows_add(const BtChainArgs &A, const vgpu_col &kc, const uint64_t *s_k,
-                                                  uint32_t *cnt, uint64_t *sums, const BtRows<U, SP> &R, bool &err) {
+                                                  const uint64_t (&kr)[BT_CH_LIN], bool neg, uint32_t *cnt,
+                                                  uint64_t *sums, const BtRows<U, SP> &R, bool &err) {
     int T[U];
     uint64_t sv[U][BT_CH_MAX_SUMS];
 #pragma unroll
     for (int u = 0; u < U; u++) {
-        T[u] = (R.ok >> u) & 1 ? bt_chain_T(A, s_k, R.pv[u]) : 0;
+        if (TM == BT_TM_SEARCH) T[u] = (R.ok >> u) & 1 ? bt_chain_T(A, s_k, R.pv[u]) : 0;
+        else T[u] = (R.ok >> u) & 1 ? bt_chain_T_lin<TM>(kr, R.pv[u], A.K, neg) : 0;
 #pragma unroll
         for (int s = 0; s < BT_CH_MAX_SUMS; s++) sv[u][s] = SP ? R.pv[u] : R.sv[SP ? 0 : u][s];
     }
@@ -790,7 +868,8 @@ __device__ __forceinline__ void bt_chain_rows_add(const BtChainArgs &A, const vg
                 }
                 const int64_t e = bt_chain_entry(A, v, bits);
                 if (e < 0) err = true;
-                else bt_chain_add(A, cnt, sums, e, T[u], sv[u]);
+                else if (TM == BT_TM_SEARCH) bt_chain_add(A, cnt, sums, e, T[u], sv[u]);
+                else bt_chain_add_one<TM == 1 || TM == 2>(A, cnt, sums, e, T[u], sv[u][0]);
             }
         }
     } else {
@@ -799,7 +878,8 @@ __device__ __forceinline__ void bt_chain_rows_add(const BtChainArgs &A, const vg
             if (!T[u]) continue;
             const int64_t e = bt_chain_entry(A, (R.kv >> u) & 1, R.kb[u]);
             if (e < 0) err = true;
-            else bt_chain_add(A, cnt, sums, e, T[u], sv[u]);
+            else if (TM == BT_TM_SEARCH) bt_chain_add(A, cnt, sums, e, T[u], sv[u]);
+            else bt_chain_add_one<TM == 1 || TM == 2>(A, cnt, sums, e, T[u], sv[u][0]);
         }
     }
 }
@@ -809,9 +889,14 @@ __device__ __forceinline__ void bt_chain_rows_add(const BtChainArgs &A, const vg
 // before the current step is aggregated, so a wave keeps its loads in
 // flight while it works on LDS: with one 1,024-thread block per CU the
 // scan is otherwise bound by the loads the 16 waves have outstanding.
-template <bool SP>
-__global__ void __launch_bounds__(BT_CH_THREADS) k_bt_chain_feed(BtChainArgs A) {
+// TM != 0 (a linear count, PLAIN only): at most BT_CH_LIN active queries,
+// one sum; neg: the operator is >= or <=.
+template <bool SP, bool PLAIN, int TM>
+__global__ void __launch_bounds__(BT_CH_THREADS) k_bt_chain_feed(BtChainArgs A, int neg) {
+    static_assert(TM == BT_TM_SEARCH || PLAIN, "the linear count runs on the plain scan");
     constexpr int U = SP ? BT_CH_U : BT_CH_U / 2;
+    uint64_t kr[BT_CH_LIN];
+    if (TM != BT_TM_SEARCH) bt_chain_load_lin<TM>(A, kr);
     extern __shared__ uint64_t dynw[];
     uint64_t *s_k = dynw;                                   // m constants
     uint64_t *sums = dynw + A.m;                            // E*m*n_sum
@@ -827,11 +912,13 @@ __global__ void __launch_bounds__(BT_CH_THREADS) k_bt_chain_feed(BtChainArgs A) 
     const int64_t step = (int64_t)BT_CH_THREADS * U;
     bool err = false;
     BtRows<U, SP> cur, nxt;
-    bt_chain_rows(A, kc, r0 + threadIdx.x, r1, cur);
-    for (int64_t base = r0 + threadIdx.x; base < r1; base += step) {
-        bt_chain_rows(A, kc, base + step, r1, nxt);
-        bt_chain_rows_add(A, kc, s_k, cnt, sums, cur, err);
-        cur = nxt;
+    if (r0 < r1) {
+        bt_chain_rows<U, SP, PLAIN>(A, kc, r0 + threadIdx.x, r1, cur);
+        for (int64_t base = r0 + threadIdx.x; base < r1; base += step) {
+            bt_chain_rows<U, SP, PLAIN>(A, kc, base + step, r1, nxt);
+            bt_chain_rows_add<TM>(A, kc, s_k, kr, neg != 0, cnt, sums, cur, err);
+            cur = nxt;
+        }
     }
     if (err) atomicOr((unsigned long long *)&A.bctl[BT_CTL_ERR], 1ull);
     __syncthreads();
@@ -1083,10 +1170,15 @@ static int bt_chain_plan(vgpu_batch *b, const BtQuery &Q0) {
         b->ch_aux.reserve((size_t)m * 12 + 16))
         return -1;
     if (lds > 65536) {
-        VG_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(k_bt_chain_feed<true>),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        VG_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(k_bt_chain_feed<false>),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        const void *fns[] = {reinterpret_cast<const void *>(k_bt_chain_feed<true, true, 1>),
+                             reinterpret_cast<const void *>(k_bt_chain_feed<true, true, 2>),
+                             reinterpret_cast<const void *>(k_bt_chain_feed<true, true, 3>),
+                             reinterpret_cast<const void *>(k_bt_chain_feed<true, true, 4>),
+                             reinterpret_cast<const void *>(k_bt_chain_feed<true, true, BT_TM_SEARCH>),
+                             reinterpret_cast<const void *>(k_bt_chain_feed<true, false, BT_TM_SEARCH>),
+                             reinterpret_cast<const void *>(k_bt_chain_feed<false, false, BT_TM_SEARCH>)};
+        for (const void *fn : fns)
+            VG_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     }
     b->ch_kmin = h[0] <= h[1] ? (int64_t)(h[0] ^ 0x8000000000000000ull) : 0;
     b->ch_E = (int32_t)E;
@@ -1525,14 +1617,33 @@ static int bt_enqueue(vgpu_batch *b, bool first) {
         vg_kt_begin();
         bool sp = true;
         for (int s2 = 0; s2 < C.n_sum; s2++) sp = sp && C.sum_pred[s2];
-        if (sp) k_bt_chain_feed<true><<<(unsigned)C.G, BT_CH_THREADS, b->ch_lds, st>>>(C);
-        else k_bt_chain_feed<false><<<(unsigned)C.G, BT_CH_THREADS, b->ch_lds, st>>>(C);
+        const vgpu_col &kc = C.jn.on ? C.probe_key : C.key;
+        const bool plain = sp && !C.pcol.validity && C.pcol.type != VGPU_BOOL && !kc.validity &&
+                           kc.type != VGPU_BOOL;
+        const int op = C.pi.cmp_op;
+        const bool same_type = (C.pi.cmp_type == VGPU_DOUBLE) == (C.pcol.type == VGPU_DOUBLE);
+        const int tm = plain && same_type && C.n_sum == 1 && C.K <= BT_CH_LIN
+                           ? bt_tm(C.pi.cmp_type == VGPU_DOUBLE, op == VGPU_OP_GT || op == VGPU_OP_LE)
+                           : BT_TM_SEARCH;
+        const int neg = op == VGPU_OP_GE || op == VGPU_OP_LE;
+        const dim3 g((unsigned)C.G), t(BT_CH_THREADS);
+        const size_t lds = b->ch_lds;
+        switch (plain ? tm : -1) {
+        case 1: k_bt_chain_feed<true, true, 1><<<g, t, lds, st>>>(C, neg); break;
+        case 2: k_bt_chain_feed<true, true, 2><<<g, t, lds, st>>>(C, neg); break;
+        case 3: k_bt_chain_feed<true, true, 3><<<g, t, lds, st>>>(C, neg); break;
+        case 4: k_bt_chain_feed<true, true, 4><<<g, t, lds, st>>>(C, neg); break;
+        case BT_TM_SEARCH: k_bt_chain_feed<true, true, BT_TM_SEARCH><<<g, t, lds, st>>>(C, neg); break;
+        default:
+            if (sp) k_bt_chain_feed<true, false, BT_TM_SEARCH><<<g, t, lds, st>>>(C, neg);
+            else k_bt_chain_feed<false, false, BT_TM_SEARCH><<<g, t, lds, st>>>(C, neg);
+        }
         VG_LAUNCH_CHECK();
+        vg_kt_end();   // the scan kernel alone; the slab reduce and expand below are small
         k_bt_chain_reduce<<<vg_blocks(W, 256), 256, 0, st>>>(C);
         VG_LAUNCH_CHECK();
         k_bt_chain_expand<<<vg_blocks((int64_t)C.K * C.E, 256), 256, 0, st>>>(L, A, C);
         VG_LAUNCH_CHECK();
-        vg_kt_end();
     } else if (b->n_items > 0) {
         const size_t rwb = (size_t)(L.RH + (b->lds_cold ? L.RC : 0)) * 8;
         vg_kt_begin();
