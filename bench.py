@@ -901,7 +901,8 @@ def main():
     ap.add_argument("--c5-seconds", type=float, default=3.0)
     ap.add_argument("--c5-slo-ms", type=float, default=100.0)
     ap.add_argument("--c5-plain-rates", type=float, nargs="*", default=[10, 25, 40, 55, 70, 85, 100])
-    ap.add_argument("--c5-batched-rates", type=float, nargs="*", default=[50, 100, 200, 350, 500, 650, 800, 1000])
+    ap.add_argument("--c5-batched-rates", type=float, nargs="*",
+                    default=[50, 100, 200, 400, 600, 800, 1000, 1200, 1400, 1600, 1800])
     ap.add_argument("--c3-rows", type=int, default=200_000_000)
     ap.add_argument("--vtr-rows", type=int, default=50_000_000)
     ap.add_argument("--str-rows", type=int, default=10_000_000)
