@@ -237,6 +237,84 @@ def run_c2(args, nv):
 
 # ---------------------------------------------------------------- c1: drop-in nodes
 
+def run_c2_nodes(args, nv):
+    """The join-then-aggregate query as R builds it and collect() runs it:
+    GroupAgg([attr], s = sum(.vectra_tmp_s), n = n()) <- Project(...,
+    .vectra_tmp_s = value * weight) <- Join(fact, dim, fkey = key), inner and
+    left, through the drop-in nodes (the GPU GroupAggNode fuses the three
+    into one device pass).  The fact table is host-resident and pulled in
+    row groups of --vtr-rg rows, as a scan hands them over, so every step
+    includes the copies of the fact columns to the device."""
+    import ctypes as C
+    n_fact, n_dim = args.fact_rows, args.dim_rows
+    log(f"[bench] c2-nodes: generating fact {n_fact:,} rows, dim {n_dim:,} rows")
+    fact, dim = c2_data(n_fact, n_dim)
+    ft = nv.Table({"fkey": fact["fkey"], "value": fact["value"]}, rg_rows=args.vtr_rg)
+    dt = nv.Table(dim, rg_rows=0)
+    aggs = [("s", "sum", ".vectra_tmp_s", 0), ("n", "n", None, 0)]
+
+    def tree(kind):
+        j = nv.join_node("gpu", ft.scan(), dt.scan(), kind, [(0, 0)])
+        return nv.group_agg_node("gpu", nv.hidden_mutate(j, ".vectra_tmp_s", "value", "weight"), ["attr"], aggs,
+                                 "/tmp")
+
+    def step():
+        return {kind: nv.collect(tree(kind)) for kind in ("inner", "left")}
+
+    node = tree("inner")
+    nv.collect(node, free=False)
+    note = "\n".join(nv.explain(node))
+    nv.core().vx_node_free(node)
+    fused = "fused with the Project and Join below" in note
+    log(f"[bench] c2-nodes: fused pass {'taken' if fused else 'NOT taken'}")
+    for _ in range(args.warmup):
+        res = step()
+    assert int(res["left"]["n"][0].sum()) == n_fact, "left join keeps every fact row"
+    lib = nv.vgpu()
+    lib.vgpu_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    lib.vgpu_synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    rows_s = 2 * n_fact / (ms / 1e3)
+    lib.vgpu_kernel_timing(1)
+    lib.vgpu_kernel_time_ms(None, 1)
+    step()
+    nl = C.c_int64()
+    kms = lib.vgpu_kernel_time_ms(C.byref(nl), 1)
+    lib.vgpu_kernel_timing(0)
+    avg_ms = kms / max(nl.value, 1)
+    rows_per_launch = 2 * n_fact / max(nl.value, 1)
+    per_launch = rows_per_launch * 16      # fact fkey + value read once per launch
+    achieved = per_launch / (avg_ms / 1e3) / 1e9
+    log(f"[bench] c2-nodes: {ms:.2f} ms/step ({rows_s:.3e} fact rows/s); fused kernel {nl.value} launches, "
+        f"avg {avg_ms:.3f} ms, {achieved:.0f} GB/s; kernels {kms:.2f} ms of the step")
+    cpu = None
+    if not args.no_cpu:
+        sample = min(args.cpu_sample, n_fact)
+        secs, hostinfo = pinned(lambda: cpu_baseline_c2(nv, fact, dim, sample))
+        cpu = {"value": 2 * sample / secs, "unit": "rows/s", "cores": 1, "kind": "port",
+               "sample": f"first {sample:,} fact rows, inner + left join -> group_by(attr) ({secs:.1f} s); one "
+                         f"thread pinned to core {hostinfo['pinned_core']} of {hostinfo['nproc']} "
+                         f"({hostinfo['cpu_model']}); {ORACLE_NOTE}", **hostinfo}
+    return {
+        "metric": METRIC, "value": rows_s, "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded numpy generators)",
+        "config": {"workload": "join-then-aggregate through the drop-in node tree R builds (GroupAgg <- "
+                               "Project(value * weight) <- Join), inner + left; fact 50M rows host-resident in "
+                               f"row groups of {args.vtr_rg:,} (host-to-device copies inside every step), dim 1M",
+                   "fact_rows": n_fact, "dim_rows": n_dim, "row_group": args.vtr_rg, "fused": fused},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_gb_star (fused probe -> product -> aggregate, one launch per row group)",
+                     "bytes_per_launch": per_launch, "avg_launch_ms": avg_ms, "launches": nl.value,
+                     "kernel_ms_per_step": kms},
+        "cpu_baseline": cpu,
+    }
+
+
 def run_strings(args, nv):
     """group_by(s, k) |> summarise(n = n(), total = sum(x)) and
     inner_join(dim, by = "s") |> group_by(w) |> summarise(n = n()) over
@@ -888,8 +966,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c5", choices=["c1", "c1-vtr", "c2", "c3-high", "c3-low", "c4", "c5",
-                                                             "strings"])
+    ap.add_argument("--workload", default="c5", choices=["c1", "c1-vtr", "c2", "c2-nodes", "c3-high", "c3-low",
+                                                             "c4", "c5", "strings"])
     ap.add_argument("--c4-queries", type=int, default=4096)
     ap.add_argument("--c4-card", type=int, default=0, help="fix every query's key cardinality (0: log-uniform)")
     ap.add_argument("--c5-rows", type=int, default=1_000_000_000)
@@ -954,6 +1032,8 @@ def main():
         out = run_c4(args, nv)
     elif args.workload == "c2":
         out = run_c2(args, nv)
+    elif args.workload == "c2-nodes":
+        out = run_c2_nodes(args, nv)
     else:
         out = run_c5(args, nv)
     print(json.dumps(out), flush=True)
