@@ -233,6 +233,32 @@ int vgpu_key_codes(const vgpu_col *keys, int n_keys, int64_t n_rows, int64_t pos
 int vgpu_string_keys(const int64_t *offsets, const uint8_t *chars, const uint8_t *validity, int64_t n_rows,
                      int64_t pos_base, int64_t *h1, int64_t *h2, int64_t *pos);
 
+/* ---- exact string dictionary -----------------------------------------
+ * Replaces the byte comparisons of the reference's string keys
+ * (src/hash.c:58-66 in group-by tables, src/join.c:96-108 in join chains):
+ * every distinct string gets one int64 code, the arena row index of a row
+ * that holds it, and two rows get the same code iff their bytes are equal
+ * (the table compares bytes; hashes only choose slots), so grouping or
+ * joining on the codes is exact.  The dictionary keeps every appended
+ * row's bytes on the device (offsets 8 bytes a row plus the bytes).
+ *   vgpu_strdict_append  appends rows [0, n) of a host string column
+ *       (offsets[0..n], chars indexed by them, host memory) as arena rows
+ *       rows..rows+n-1, inserting the strings; codes (device, n) gets each
+ *       row's code, 0 for NA rows (validity: device bitmap or NULL); pos
+ *       (device, may be NULL) gets pos_base + row;
+ *   vgpu_strdict_lookup  codes of a host string column without inserting:
+ *       -1 for a string that was never appended, 0 for NA rows;
+ *   vgpu_strdict_size    distinct strings (and arena rows into *rows). */
+typedef struct vgpu_strdict vgpu_strdict;
+vgpu_strdict *vgpu_strdict_create(void);
+void          vgpu_strdict_free(vgpu_strdict *d);
+int           vgpu_strdict_append(vgpu_strdict *d, const int64_t *offsets, const uint8_t *chars,
+                                  const uint8_t *validity, int64_t n_rows, int64_t pos_base, int64_t *codes,
+                                  int64_t *pos);
+int           vgpu_strdict_lookup(vgpu_strdict *d, const int64_t *offsets, const uint8_t *chars,
+                                  const uint8_t *validity, int64_t n_rows, int64_t *codes);
+int64_t       vgpu_strdict_size(const vgpu_strdict *d, int64_t *rows);
+
 /* Finishes on the device and orders the groups; returns the group count. */
 int64_t vgpu_groupby_finish(vgpu_groupby *g);
 
@@ -486,6 +512,11 @@ void    vgpu_batch_destroy(vgpu_batch *b);
  *   "gb.fuse_join"       (2) a GPU GroupAggNode over Project(a * b) over a
  *                         GPU JoinNode runs the three as one fused device
  *                         pass (2), or node by node (1)
+ *   "strdict.hash_bits"  (64) bits of the string dictionary's hash that are
+ *                         kept (1..64), read when a dictionary is created;
+ *                         fewer bits make different strings share slots and
+ *                         tags, which the byte comparisons resolve (a test
+ *                         of exactness under collisions)
  * Returns -1 for an unknown name or a value <= 0. */
 int     vgpu_tune_set(const char *name, int64_t value);
 int64_t vgpu_tune_get(const char *name);

@@ -22,8 +22,8 @@ def test_gpu_plan_notes(gpu, tmp_path):
     gpu.core().vx_node_free(g)
     t = gpu.Table({"s": np.array(["a", "b"]), "x": np.ones(2)})
     g = gpu.group_agg_node("gpu", t.scan(), ["s"], [("n", "n", None, 0)], "/tmp")
-    assert gpu.explain(g)[0] == ("GroupAggNode [materializes, 1 keys; gpu: tuple-code group-by (string keys hashed "
-                                 "on the device), key order]")
+    assert gpu.explain(g)[0] == ("GroupAggNode [materializes, 1 keys; gpu: tuple-code group-by (string keys by "
+                                 "exact device dictionary codes), key order]")
     gpu.core().vx_node_free(g)
     L, R = gpu.Table({"k": np.arange(3)}), gpu.Table({"k": np.arange(2)})
     j = gpu.join_node("gpu", L.scan(), R.scan(), "left", [(0, 0)])

@@ -82,6 +82,71 @@ def test_string_groupby_empty_input(gpu, oracle):
         assert_tables_equal(got, exp)
 
 
+def dict_pool(r, n_distinct):
+    """distinct strings with empty strings, shared prefixes, multi-byte
+    UTF-8 and strings that differ only in their last byte or length"""
+    base = ["", "a", "ab", "abc", "b", "ba", "é", "ü1", "x" * 40, "x" * 41, "x" * 40 + "y"]
+    return base + [f"s{i:07d}" + "z" * int(r.integers(0, 5)) for i in range(n_distinct - len(base))]
+
+
+@pytest.mark.parametrize("bits", [64, 1])
+def test_strdict_codes_are_exact(gpu, bits):
+    """codes equal iff bytes equal, over three appended batches and the
+    table's growth (6,000 distinct strings, first table 1,024 slots); with
+    the hash cut to one bit every string shares two slots and one tag, so
+    only the byte comparisons tell strings apart; lookups return the
+    appended codes and -1 for strings never appended"""
+    lib = gpu.vgpu()
+    old = lib.vgpu_tune_get(b"strdict.hash_bits")
+    lib.vgpu_tune_set(b"strdict.hash_bits", bits)
+    try:
+        d = gpu.StrDict()
+    finally:
+        lib.vgpu_tune_set(b"strdict.hash_bits", old)
+    r = rng(601 + bits)
+    pool = dict_pool(r, 6_000)
+    seen = {}
+    batches = []
+    for size in (5_000, 1, 20_000):
+        vals = [None if r.random() < 0.03 else pool[int(r.integers(0, len(pool)))] for _ in range(size)]
+        codes = d.append(vals)
+        batches.append((vals, codes))
+        for v, c in zip(vals, codes):
+            if v is None:
+                continue
+            assert seen.setdefault(v, int(c)) == int(c), f"{v!r} got two codes"
+    assert len(set(seen.values())) == len(seen), "two strings share a code"
+    distinct, rows = d.size()
+    assert distinct == len(seen) and rows == 25_001
+    for vals, codes in batches:
+        got = d.lookup(vals)
+        ok = np.array([v is not None for v in vals])
+        assert np.array_equal(got[ok], codes[ok])
+    unseen = ["never", "x" * 42, "s9999999", "abcd", "é" * 2]
+    assert list(d.lookup(unseen)) == [-1] * len(unseen)
+    d.free()
+
+
+@pytest.mark.parametrize("temp_dir", [None, "/tmp"])
+def test_string_groupby_exact_under_forced_collisions(gpu, oracle, temp_dir):
+    """the group-by node with the dictionary's hash cut to one bit: every
+    string collides with every other in slot and tag, groups are still the
+    oracle's (one per distinct byte string), in first-seen and key order"""
+    lib = gpu.vgpu()
+    old = lib.vgpu_tune_get(b"strdict.hash_bits")
+    lib.vgpu_tune_set(b"strdict.hash_bits", 1)
+    try:
+        cols = group_table(701, 30_000, 400)
+        got = group(gpu, "gpu", cols, ["g", "k"], temp_dir, 4_000, None, AGGS)
+    finally:
+        lib.vgpu_tune_set(b"strdict.hash_bits", old)
+    exp = group(gpu, "oracle", cols, ["g", "k"], temp_dir, 4_000, None,
+                AGGS + [("__sa", "sum", "ax", 1), ("__cx", "count", "x", 0)])
+    sa = exp.pop("__sa")[0]
+    cnt = exp.pop("__cx")[0]
+    assert_tables_equal(got, exp, {"sx": sa, "mx": sa / np.maximum(cnt, 1)})
+
+
 def join_tables(nv, seed, n_left, n_right, n_distinct, composite):
     def mk():
         r = rng(seed)

@@ -37,7 +37,8 @@ HIP_FLAGS = [
 C_FLAGS = ["-std=c11", "-O2", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter",
            f"-I{INCLUDE}", f"-I{HOST}"]
 
-HIP_SOURCES = ["vgpu_runtime.hip", "vgpu_filter.hip", "vgpu_groupby.hip", "vgpu_join.hip", "vgpu_batch.hip"]
+HIP_SOURCES = ["vgpu_runtime.hip", "vgpu_filter.hip", "vgpu_groupby.hip", "vgpu_join.hip", "vgpu_batch.hip",
+               "vgpu_strdict.hip"]
 NODE_SOURCES = ["gpu_common.c", "devscan.c", "filter_node.c", "group_agg_node.c", "join_node.c", "serve.c",
                 "explain.c"]
 

@@ -52,8 +52,8 @@ typedef struct {
     int ds_open;
     /* string key columns: the batch's offsets, bytes and validity, and the
      * two hashes and positions computed from them */
-    DevMem s_off[VGPU_MAX_COLS], s_chr[VGPU_MAX_COLS], s_val[VGPU_MAX_COLS];
-    DevMem s_h1[VGPU_MAX_COLS], s_h2[VGPU_MAX_COLS], s_pos[VGPU_MAX_COLS];
+    DevMem s_val[VGPU_MAX_COLS], s_code[VGPU_MAX_COLS], s_pos[VGPU_MAX_COLS];
+    vgpu_strdict *sdict[VGPU_MAX_COLS];   /* per string key: its exact dictionary (this run) */
     int fused;         /* the last run took the fused Project <- Join path */
 } GroupGpu;
 
@@ -187,11 +187,12 @@ static int feed_from_scan(GroupAggNode *ga, vgpu_groupby *gb, int key_idx, int n
  * int64 bits) or, for bool, the sum, and the valid count -- which give the
  * group's key values back and prove that no two different tuples shared a
  * code: within a group each column must be all NA or all valid with
- * min == max (sum 0 or all TRUE). A string column enters the code as a
- * 64-bit hash of its bytes (vgpu_string_keys) and feeds a second,
- * independent hash to min / max (one string per group) and the row
- * position to min: the batches' string columns stay on the host until the
- * end, and each group's string is read from that row. In key order
+ * min == max (sum 0 or all TRUE). A string column enters the code as its
+ * exact dictionary code (vgpu_strdict_append: equal bytes, equal code;
+ * src/hash.c:58-66 compares bytes), which also feeds min / max (one string
+ * per group, as exact as for int64), and the row position to min: the
+ * batches' string columns stay on the host until the end, and each group's
+ * string is read from that row. In key order
  * (temp_dir set) the groups are then sorted stably by the tuple, each
  * column ascending with NA last (NaN after the numbers, in
  * first-occurrence order; strings by bytes, a prefix first, src/sort.c:58-66). */
@@ -370,7 +371,7 @@ static VecBatch *gpu_group_run_multi(GroupAggNode *ga) {
             vectra_error("GPU summarise: string column '%s' is not aggregated on the GPU", sp->input_col);
         aggs[a].input = slot_for(cs, ci, in_col, in_type, &n_in);
     }
-    /* a string key's slots are its second hash and the row position */
+    /* a string key's slots are its dictionary code (min / max) and the row position */
     int pslot[MK_MAX_KEYS];
     for (int j = 0; j < m; j++) {
         if (ktype[j] == VEC_STRING) {
@@ -418,7 +419,7 @@ static VecBatch *gpu_group_run_multi(GroupAggNode *ga) {
         if (nl > 0) {
             vgpu_col ins[VGPU_MAX_INPUTS], keys[MK_MAX_KEYS];
             for (int s2 = 0; s2 < n_in; s2++) {
-                if (in_col[s2] < 0) continue;   /* a string key's hash or positions: below */
+                if (in_col[s2] < 0) continue;   /* a string key's codes or positions: below */
                 if (b->columns[in_col[s2]].type != (VecType)in_type[s2]) {
                     gb_drop(ga, gb);
                     vec_batch_free(b);
@@ -448,22 +449,23 @@ static VecBatch *gpu_group_run_multi(GroupAggNode *ga) {
                     vectra_error("group_by: column type changed between batches");
                 }
                 const int64_t nb = b->n_rows, bytes = sa->buf.str.offsets[nb];
-                int64_t *d_off = (int64_t *)devmem_reserve(&g->s_off[j], (size_t)(nb + 1) * 8);
-                uint8_t *d_chr = (uint8_t *)devmem_reserve(&g->s_chr[j], (size_t)(bytes > 0 ? bytes : 1));
-                VX_GPU(vgpu_memcpy_h2d(d_off, sa->buf.str.offsets, (size_t)(nb + 1) * 8));
-                if (bytes > 0) VX_GPU(vgpu_memcpy_h2d(d_chr, sa->buf.str.data, (size_t)bytes));
+                (void)bytes;
                 uint8_t *d_val = NULL;
                 if (!vx_all_valid(sa->validity, nb)) {
                     const size_t vb = (size_t)vec_validity_bytes(nb);
                     d_val = (uint8_t *)devmem_reserve(&g->s_val[j], vb + 8);
                     VX_GPU(vgpu_memcpy_h2d(d_val, sa->validity, vb));
                 }
-                int64_t *h1 = (int64_t *)devmem_reserve(&g->s_h1[j], (size_t)nb * 8);
-                int64_t *h2 = (int64_t *)devmem_reserve(&g->s_h2[j], (size_t)nb * 8);
-                int64_t *ps = (int64_t *)devmem_reserve(&g->s_pos[j], (size_t)nb * 8);
-                VX_GPU(vgpu_string_keys(d_off, d_chr, d_val, nb, phys, h1, h2, ps));
-                keys[j] = (vgpu_col){VGPU_INT64, 0, h1, d_val};
-                ins[kslot[j]] = (vgpu_col){VGPU_INT64, 0, h2, d_val};
+                /* the string's exact dictionary code stands for it in the
+                 * tuple code and in the hidden min / max (equal bytes, equal
+                 * code: one string per group is then exact, as for int64) */
+                if (!g->sdict[j] && !(g->sdict[j] = vgpu_strdict_create())) vectra_error("GPU: %s", vgpu_last_error());
+                int64_t *code = (int64_t *)devmem_reserve(&g->s_code[j], (size_t)(nb > 0 ? nb : 1) * 8);
+                int64_t *ps = (int64_t *)devmem_reserve(&g->s_pos[j], (size_t)(nb > 0 ? nb : 1) * 8);
+                VX_GPU(vgpu_strdict_append(g->sdict[j], sa->buf.str.offsets, (const uint8_t *)sa->buf.str.data, d_val,
+                                           nb, phys, code, ps));
+                keys[j] = (vgpu_col){VGPU_INT64, 0, code, d_val};
+                ins[kslot[j]] = (vgpu_col){VGPU_INT64, 0, code, d_val};
                 ins[pslot[j]] = (vgpu_col){VGPU_INT64, 0, ps, NULL};
                 /* the column moves out of the batch (its bytes are copied
                  * when the batch only borrows them) */
@@ -505,6 +507,10 @@ static VecBatch *gpu_group_run_multi(GroupAggNode *ga) {
         gb_drop(ga, gb);
         kept_free(kept, m, n_kept);
         vectra_error("GPU: %s", vgpu_last_error());
+    }
+    for (int j = 0; j < m; j++) {   /* fed: the groups' strings come from the kept batches */
+        vgpu_strdict_free(g->sdict[j]);
+        g->sdict[j] = NULL;
     }
     const size_t cap = (size_t)(ng > 0 ? ng : 1);
     double **vals = (double **)calloc((size_t)na, sizeof(double *));
@@ -1054,12 +1060,11 @@ static void gpu_group_free(VecNode *self) {
         devmem_free(&g->sel);
         devmem_free(&g->codes);
         for (int j = 0; j < VGPU_MAX_COLS; j++) {
-            devmem_free(&g->s_off[j]);
-            devmem_free(&g->s_chr[j]);
             devmem_free(&g->s_val[j]);
-            devmem_free(&g->s_h1[j]);
-            devmem_free(&g->s_h2[j]);
+            devmem_free(&g->s_code[j]);
             devmem_free(&g->s_pos[j]);
+            vgpu_strdict_free(g->sdict[j]);
+            g->sdict[j] = NULL;
         }
         if (g->ds_open) vx_devscan_close(&g->ds);
         free(g);
@@ -1133,7 +1138,7 @@ int vx_gpu_group_note(const VecNode *n, char *buf, int size) {
                        : over_join && vgpu_tune_get("gb.fuse_join") >= 2
                            ? ", fuses the Project and Join below when their shape allows" : "";
     return snprintf(buf, (size_t)size, "gpu: %s group-by%s, %s%s%s", tuple ? "tuple-code" : "hash",
-                    strs ? " (string keys hashed on the device)" : "", ga->use_sorted ? "key order" : "first-seen order",
+                    strs ? " (string keys by exact device dictionary codes)" : "", ga->use_sorted ? "key order" : "first-seen order",
                     feed ? (filtered ? ", device scan feed with the filter in the feed" : ", device scan feed") : "",
                     fuse);
 }

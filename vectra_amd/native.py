@@ -295,6 +295,13 @@ def vgpu() -> C.CDLL:
         _proto(lib, "vgpu_batch_destroy", None, [C.c_void_p])
         _proto(lib, "vgpu_batch_device_bytes", C.c_int64, [C.c_void_p])
         _proto(lib, "vgpu_batch_bucketed", C.c_int, [C.c_void_p])
+        _proto(lib, "vgpu_strdict_create", C.c_void_p, [])
+        _proto(lib, "vgpu_strdict_free", None, [C.c_void_p])
+        _proto(lib, "vgpu_strdict_append", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                                     C.c_int64, C.c_void_p, C.c_void_p])
+        _proto(lib, "vgpu_strdict_lookup", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                                     C.c_void_p])
+        _proto(lib, "vgpu_strdict_size", C.c_int64, [C.c_void_p, C.POINTER(C.c_int64)])
         _libs["vgpu"] = lib
         # the autotuner's table for this build, unless the environment names one
         table = PKG / "tuning.txt"
@@ -821,6 +828,64 @@ class DeviceColumn:
         if self.valid:
             lib.vgpu_free(self.valid)
         self.data = self.valid = None
+
+
+class StrDict:
+    """The exact device string dictionary (include/vgpu.h, vgpu_strdict_*):
+    host string arrays (None = NA) in, int64 codes out."""
+
+    def __init__(self):
+        lib = vgpu()
+        self.h = lib.vgpu_strdict_create()
+        if not self.h:
+            raise VectraError(lib.vgpu_last_error().decode())
+
+    @staticmethod
+    def _pack(strings: Sequence[str | None]):
+        raw = [b"" if x is None else x.encode() for x in strings]
+        off = np.zeros(len(raw) + 1, np.int64)
+        off[1:] = np.cumsum([len(b) for b in raw])
+        chars = np.frombuffer(b"".join(raw) or b"\0", np.uint8).copy()
+        valid = np.array([x is not None for x in strings], bool)
+        return off, chars, valid
+
+    def _call(self, run, strings: Sequence[str | None]) -> np.ndarray:
+        lib = vgpu()
+        off, chars, valid = self._pack(strings)
+        n = len(strings)
+        dcodes = lib.vgpu_malloc(max(n, 1) * 8)
+        dval = None
+        if not valid.all():
+            bm = np.packbits(valid, bitorder="little")
+            dval = lib.vgpu_malloc(max(bm.nbytes, 8))
+            _vcheck(lib.vgpu_memcpy_h2d(dval, bm.ctypes.data, bm.nbytes))
+        try:
+            _vcheck(run(off.ctypes.data, chars.ctypes.data, dval, n, dcodes))
+            codes = np.empty(n, np.int64)
+            _vcheck(lib.vgpu_memcpy_d2h(codes.ctypes.data, dcodes, n * 8))
+        finally:
+            lib.vgpu_free(dcodes)
+            if dval:
+                lib.vgpu_free(dval)
+        return codes
+
+    def append(self, strings: Sequence[str | None]) -> np.ndarray:
+        lib = vgpu()
+        return self._call(lambda o, c, v, n, d: lib.vgpu_strdict_append(self.h, o, c, v, n, 0, d, None), strings)
+
+    def lookup(self, strings: Sequence[str | None]) -> np.ndarray:
+        lib = vgpu()
+        return self._call(lambda o, c, v, n, d: lib.vgpu_strdict_lookup(self.h, o, c, v, n, d), strings)
+
+    def size(self) -> tuple[int, int]:
+        rows = C.c_int64()
+        d = vgpu().vgpu_strdict_size(self.h, C.byref(rows))
+        return int(d), int(rows.value)
+
+    def free(self) -> None:
+        if self.h:
+            vgpu().vgpu_strdict_free(self.h)
+            self.h = None
 
 
 def col_array(cols: Sequence[DeviceColumn | None]) -> Any:
