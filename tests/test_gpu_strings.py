@@ -187,6 +187,32 @@ def test_string_join_matches_oracle(gpu, oracle, kind, composite):
         assert_tables_equal(g["cols"], e["cols"])
 
 
+@pytest.mark.parametrize("kind,composite", [("inner", False), ("left", False), ("semi", False), ("anti", False),
+                                            ("full", False), ("inner", True), ("semi", True), ("anti", True)])
+def test_string_join_exact_under_forced_collisions(gpu, oracle, kind, composite):
+    """the join's string dictionary with its hash cut to one bit: every
+    build string shares two slots and one tag, probe strings absent from the
+    build side included; rows and order are still the oracle's"""
+    mk = join_tables(gpu, 157, 4000, 1500, 600, composite)
+    keys = [(0, 1)] + ([(1, 2)] if composite else [])
+    lib = gpu.vgpu()
+    old = lib.vgpu_tune_get(b"strdict.hash_bits")
+    out = {}
+    for engine in ("gpu", "oracle"):
+        left, right = mk()
+        lnode = gpu.filter_node(engine, left.scan(), ("cmp", ">", ("col", "x"), ("dbl", -0.8)))
+        node = gpu.join_node(engine, lnode, right.scan(), kind, keys)
+        lib.vgpu_tune_set(b"strdict.hash_bits", 1)
+        try:
+            out[engine] = gpu.next_batches(node)
+        finally:
+            lib.vgpu_tune_set(b"strdict.hash_bits", old)
+    got, exp = out["gpu"], out["oracle"]
+    assert len(got) == len(exp), f"{len(got)} batches vs {len(exp)}"
+    for g, e in zip(got, exp):
+        assert_tables_equal(g["cols"], e["cols"])
+
+
 def test_string_join_known_answers(gpu):
     """test-joins.R:135-163 through the GPU node"""
     L = gpu.Table({"name": np.array(["alice", "bob"]), "x": np.array([1., 2.])})

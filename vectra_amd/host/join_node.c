@@ -47,31 +47,36 @@ typedef struct {
     VecType   common[JN_MAX_KEYS];
     DevCol    pcast[JN_MAX_KEYS];
     DevMem    bcode, bvalid, pcode, pvalid;
-    /* string key columns: offsets, bytes, validity and hash (build side
-     * first, then each probe batch) */
-    DevMem    s_off[JN_MAX_KEYS], s_chr[JN_MAX_KEYS], s_val[JN_MAX_KEYS], s_h1[JN_MAX_KEYS];
+    /* string key columns: the build side's strings in an exact dictionary
+     * (vgpu_strdict), the build rows' codes kept for the pair check, and
+     * each probe batch's codes (looked up, -1 for strings not in the build) */
+    vgpu_strdict *sdict[JN_MAX_KEYS];
+    DevMem    s_bcode[JN_MAX_KEYS], s_bval[JN_MAX_KEYS], s_pcode[JN_MAX_KEYS], s_pval[JN_MAX_KEYS];
+    vgpu_col  s_bcol[JN_MAX_KEYS];
 } JoinGpu;
 
 /* the rows [0, n) of host string column a as a device int64 column of
- * their hashes (vgpu_string_keys) with the column's validity: the key that
- * stands for the string in tuple codes */
-static vgpu_col string_key_col(JoinGpu *g, int k, const VecArray *a, int64_t n) {
-    const int64_t bytes = n > 0 ? a->buf.str.offsets[n] : 0;
-    int64_t *d_off = (int64_t *)devmem_reserve(&g->s_off[k], (size_t)(n + 1) * 8);
-    uint8_t *d_chr = (uint8_t *)devmem_reserve(&g->s_chr[k], (size_t)(bytes > 0 ? bytes : 1));
-    int64_t *h1 = (int64_t *)devmem_reserve(&g->s_h1[k], (size_t)(n > 0 ? n : 1) * 8);
+ * exact dictionary codes with the column's validity (equal bytes, equal
+ * code; src/join.c:96-108 compares bytes): the build side appends its
+ * strings, a probe batch looks them up */
+static vgpu_col string_key_col(JoinGpu *g, int k, const VecArray *a, int64_t n, int build) {
+    DevMem *cm = build ? &g->s_bcode[k] : &g->s_pcode[k], *vm = build ? &g->s_bval[k] : &g->s_pval[k];
+    int64_t *codes = (int64_t *)devmem_reserve(cm, (size_t)(n > 0 ? n : 1) * 8);
     uint8_t *d_val = NULL;
+    if (build && !g->sdict[k] && !(g->sdict[k] = vgpu_strdict_create())) vectra_error("GPU: %s", vgpu_last_error());
     if (n > 0) {
-        VX_GPU(vgpu_memcpy_h2d(d_off, a->buf.str.offsets, (size_t)(n + 1) * 8));
-        if (bytes > 0) VX_GPU(vgpu_memcpy_h2d(d_chr, a->buf.str.data, (size_t)bytes));
         if (!vx_all_valid(a->validity, n)) {
             const size_t vb = (size_t)vec_validity_bytes(n);
-            d_val = (uint8_t *)devmem_reserve(&g->s_val[k], vb + 8);
+            d_val = (uint8_t *)devmem_reserve(vm, vb + 8);
             VX_GPU(vgpu_memcpy_h2d(d_val, a->validity, vb));
         }
-        VX_GPU(vgpu_string_keys(d_off, d_chr, d_val, n, 0, h1, NULL, NULL));
+        const uint8_t *chars = (const uint8_t *)a->buf.str.data;
+        if (build) VX_GPU(vgpu_strdict_append(g->sdict[k], a->buf.str.offsets, chars, d_val, n, 0, codes, NULL));
+        else VX_GPU(vgpu_strdict_lookup(g->sdict[k], a->buf.str.offsets, chars, d_val, n, codes));
     }
-    return (vgpu_col){VGPU_INT64, 0, h1, d_val};
+    const vgpu_col c = {VGPU_INT64, 0, codes, d_val};
+    if (build) g->s_bcol[k] = c;
+    return c;
 }
 
 /* Several key columns (src/join.c:67-113 hashes and compares the tuple):
@@ -81,8 +86,10 @@ static vgpu_col string_key_col(JoinGpu *g, int k, const VecArray *a, int64_t n) 
  * codes, and every matched pair's tuples are compared afterwards, so a code
  * shared by two different tuples (about 2^-64 per pair) is an error, never
  * a wrong match.  A string column (one key or several) enters the codes as
- * a 64-bit hash of its bytes; the pairs' strings are compared byte for
- * byte (src/join.c:96-108). */
+ * its exact dictionary code, so equal codes are equal bytes
+ * (src/join.c:96-108): one string key cannot collide at all (a tuple code
+ * of one column is a bijection of its value), and the pair check compares
+ * the codes on the device like numbers. */
 static void multi_build_side(JoinNode *jn) {
     JoinGpu *g = (JoinGpu *)jn->gpu;
     const int m = jn->n_keys;
@@ -121,7 +128,7 @@ static void multi_build_side(JoinNode *jn) {
     for (int k = 0; k < m; k++) {
         const int rk = jn->keys[k].right_col;
         if (g->common[k] == VEC_STRING) {
-            keys[k] = string_key_col(g, k, &g->r_host[rk], g->n_build);
+            keys[k] = string_key_col(g, k, &g->r_host[rk], g->n_build, 1);
             continue;
         }
         DevCol *dc = &g->r_dev[rk];
@@ -246,65 +253,19 @@ static VecBatch *make_output(JoinNode *jn, int64_t n) {
     return res;
 }
 
-/* one key value of a host column in the pair's common type, as a double
- * and as an int64 (a double's bits for double columns) */
-static void key_value(const VecArray *a, int64_t r, VecType common, double *d, int64_t *i) {
-    if (common == VEC_DOUBLE) {
-        *d = a->type == VEC_DOUBLE ? a->buf.dbl[r] : a->type == VEC_INT64 ? (double)a->buf.i64[r] : (double)a->buf.bln[r];
-        *i = 0;
-    } else {
-        *i = a->type == VEC_INT64 ? a->buf.i64[r] : (int64_t)a->buf.bln[r];
-        *d = 0;
-    }
-}
-
 static VecBatch *probe_batch_multi(JoinNode *jn, VecBatch *pb);
 
-/* every matched pair (build row >= 0) joins equal tuples: on the device
- * when every key column is a number (pkeys: the probe's key columns in the
- * common types), else on the host */
-static void verify_pairs(JoinNode *jn, const VecBatch *pb, const vgpu_col *pkeys, const int32_t *d_prow,
-                         const int64_t *d_brow, int64_t n) {
+/* every matched pair (build row >= 0) joins equal tuples, compared on the
+ * device (pkeys: the probe's key columns in the common types, string keys
+ * as dictionary codes) */
+static void verify_pairs(JoinNode *jn, const vgpu_col *pkeys, const int32_t *d_prow, const int64_t *d_brow,
+                         int64_t n) {
     JoinGpu *g = (JoinGpu *)jn->gpu;
-    int strings = 0;
-    for (int k = 0; k < jn->n_keys; k++) strings |= g->common[k] == VEC_STRING;
-    if (!strings) {
-        vgpu_col bkeys[JN_MAX_KEYS];
-        for (int k = 0; k < jn->n_keys; k++) bkeys[k] = g->r_dev[jn->keys[k].right_col].col;
-        int64_t bad = 0;
-        VX_GPU(vgpu_pairs_unequal(pkeys, bkeys, jn->n_keys, d_prow, d_brow, n, &bad));
-        if (bad) vectra_error("GPU join: two different key tuples share a 64-bit code (probability about 2^-64 per pair)");
-        return;
-    }
-    int32_t *pr = (int32_t *)malloc((size_t)(n > 0 ? n : 1) * 4);
-    int64_t *br = (int64_t *)malloc((size_t)(n > 0 ? n : 1) * 8);
-    if (!pr || !br) vectra_error("alloc failed");
-    VX_GPU(vgpu_memcpy_d2h(pr, d_prow, (size_t)n * 4));
-    VX_GPU(vgpu_memcpy_d2h(br, d_brow, (size_t)n * 8));
-    int bad = 0;
-    for (int64_t i = 0; i < n && !bad; i++) {
-        if (br[i] < 0) continue;
-        for (int k = 0; k < jn->n_keys; k++) {
-            if (g->common[k] == VEC_STRING) {
-                const VecArray *pa = &pb->columns[jn->keys[k].left_col], *ba = &g->r_host[jn->keys[k].right_col];
-                if (!vec_array_is_valid(pa, pr[i]) || !vec_array_is_valid(ba, br[i])) {
-                    bad = 1;
-                    continue;
-                }
-                const int64_t ps = pa->buf.str.offsets[pr[i]], pl = pa->buf.str.offsets[pr[i] + 1] - ps;
-                const int64_t bs = ba->buf.str.offsets[br[i]], bl = ba->buf.str.offsets[br[i] + 1] - bs;
-                if (pl != bl || memcmp(pa->buf.str.data + ps, ba->buf.str.data + bs, (size_t)pl) != 0) bad = 1;
-                continue;
-            }
-            double pd, bd;
-            int64_t pi, bi;
-            key_value(&pb->columns[jn->keys[k].left_col], pr[i], g->common[k], &pd, &pi);
-            key_value(&g->r_host[jn->keys[k].right_col], br[i], g->common[k], &bd, &bi);
-            if (g->common[k] == VEC_DOUBLE ? !(pd == bd) : pi != bi) bad = 1;
-        }
-    }
-    free(pr);
-    free(br);
+    vgpu_col bkeys[JN_MAX_KEYS];
+    for (int k = 0; k < jn->n_keys; k++)
+        bkeys[k] = g->common[k] == VEC_STRING ? g->s_bcol[k] : g->r_dev[jn->keys[k].right_col].col;
+    int64_t bad = 0;
+    VX_GPU(vgpu_pairs_unequal(pkeys, bkeys, jn->n_keys, d_prow, d_brow, n, &bad));
     if (bad) vectra_error("GPU join: two different key tuples share a 64-bit code (probability about 2^-64 per pair)");
 }
 
@@ -434,7 +395,7 @@ static VecBatch *probe_batch_multi(JoinNode *jn, VecBatch *pb) {
     for (int k = 0; k < m; k++) {
         const int lk = jn->keys[k].left_col;
         if (g->common[k] == VEC_STRING) {
-            keys[k] = string_key_col(g, k, &pb->columns[lk], pb->n_rows);
+            keys[k] = string_key_col(g, k, &pb->columns[lk], pb->n_rows, 0);
             continue;
         }
         devcol_upload(&g->l_dev[lk], &pb->columns[lk], pb->n_rows);
@@ -464,7 +425,7 @@ static VecBatch *probe_batch_multi(JoinNode *jn, VecBatch *pb) {
     const int64_t *d_brow = NULL;
     if (n_out > 0) {
         vgpu_join_pairs(g->jt, &d_prow, &d_brow);
-        verify_pairs(jn, pb, keys, d_prow, d_brow, n_out);
+        verify_pairs(jn, keys, d_prow, d_brow, n_out);
     }
     if (!semi_anti) return n_out > 0 ? gather_output(jn, pb, d_prow, d_brow, n_out) : NULL;
     /* probe rows with a pair, in probe order */
@@ -591,10 +552,12 @@ static void gpu_join_free(VecNode *self) {
         devcol_free(&g->pkey_cast);
         for (int k = 0; k < JN_MAX_KEYS; k++) {
             devcol_free(&g->pcast[k]);
-            devmem_free(&g->s_off[k]);
-            devmem_free(&g->s_chr[k]);
-            devmem_free(&g->s_val[k]);
-            devmem_free(&g->s_h1[k]);
+            devmem_free(&g->s_bcode[k]);
+            devmem_free(&g->s_bval[k]);
+            devmem_free(&g->s_pcode[k]);
+            devmem_free(&g->s_pval[k]);
+            vgpu_strdict_free(g->sdict[k]);
+            g->sdict[k] = NULL;
         }
         devmem_free(&g->bcode);
         devmem_free(&g->bvalid);
@@ -691,7 +654,7 @@ int vx_gpu_join_note(const VecNode *n, char *buf, int size) {
         strs += jn->left->output_schema.col_types[jn->keys[k].left_col] == VEC_STRING;
     const int tuple = jn->n_keys > 1 || strs > 0;
     return snprintf(buf, (size_t)size, "gpu: %s table built on the device%s, probe per batch",
-                    tuple ? "tuple-code" : "hash", strs ? " (string keys hashed on the device)" : "");
+                    tuple ? "tuple-code" : "hash", strs ? " (string keys by exact device dictionary codes)" : "");
 }
 
 /* ---- fusion with a GroupAggNode above a ProjectNode ------------------ */
