@@ -82,6 +82,23 @@ def test_string_groupby_empty_input(gpu, oracle):
         assert_tables_equal(got, exp)
 
 
+def test_string_key_order_on_the_device(gpu, oracle):
+    """key order of string groups sorted on the device (length, then 8-byte
+    chunks, then NA last): prefixes at and across chunk boundaries,
+    embedded NUL bytes, bytes above 0x7F (unsigned), the empty string and
+    NA, alone and with a number key"""
+    pool = ["", "\x00", "a", "a\x00", "a\x00b", "ab", "b", "é", "z", "zz", "a" * 8, "a" * 8 + "\x00", "a" * 9,
+            "a" * 16, "a" * 17, "a" * 7 + "b", "a" * 8 + "b", "ÿ", "y"]
+    r = rng(811)
+    n = 6_000
+    g = np.array([pool[i] for i in r.integers(0, len(pool), n)], dtype=object)
+    cols = {"g": (g, with_na(r, n, 0.05)), "k": r.integers(0, 3, n), "x": r.normal(size=n)}
+    for keys in (["g"], ["k", "g"], ["g", "k"]):
+        got = group(gpu, "gpu", cols, keys, "/tmp", 1_000, None, [("n", "n", None, 0)])
+        exp = group(gpu, "oracle", cols, keys, "/tmp", 1_000, None, [("n", "n", None, 0)])
+        assert_tables_equal(got, exp)
+
+
 def dict_pool(r, n_distinct):
     """distinct strings with empty strings, shared prefixes, multi-byte
     UTF-8 and strings that differ only in their last byte or length"""

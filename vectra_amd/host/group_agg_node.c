@@ -248,15 +248,18 @@ static void tuple_sort(const TupleCmp *t, int64_t *idx, int64_t *tmp, int64_t n)
     }
 }
 
-/* The same order on the device for numeric key columns: a stable LSD sort
- * of the group order (first-seen to begin with), column by column from the
- * last, each by its value -- int64 with the sign bit flipped, double by its
- * IEEE bits mapped to an unsigned order (-0 as +0), bool 0 / 1 -- and then,
- * where the column has NaN or NA, by its class (number, NaN, NA).  Groups
- * equal on a column keep their relative order, as in tuple_sort.  Returns 0
- * (nothing done) for string keys and for fewer than DEV_SORT_MIN groups. */
-#define DEV_SORT_MIN 1024
-
+/* The same order on the device: a stable LSD radix sort of the group order
+ * (first-seen to begin with), column by column from the last.  A number
+ * column takes one pass by its value -- int64 with the sign bit flipped,
+ * double by its IEEE bits mapped to an unsigned order (-0 as +0), bool 0 /
+ * 1; a string column takes passes from its least significant key up: the
+ * length, then its bytes in 8-byte big-endian chunks, zero-padded, from the
+ * last chunk to the first (zero-padded order with the shorter string first
+ * on a tie is memcmp order with a prefix first, src/sort.c:58-66); then,
+ * where the column has NaN or NA, a pass by its class (number, NaN, NA).
+ * A pass whose keys are all equal is skipped.  Groups equal on a column
+ * keep their relative order, as in tuple_sort.  Returns 0 (nothing done)
+ * for fewer than two groups or when device memory runs out. */
 static uint64_t order_bits(const TupleCmp *t, int j, int64_t g) {
     if (t->type[j] == VEC_BOOL) return t->v[j][g] > 0;
     uint64_t u;
@@ -269,10 +272,38 @@ static uint64_t order_bits(const TupleCmp *t, int j, int64_t g) {
     return (u >> 63) ? ~u : u | 0x8000000000000000ull;
 }
 
+/* bytes [8c, 8c + 8) of group g's string, zero-padded, big-endian */
+static uint64_t chunk_bits(const TupleCmp *t, int j, int64_t g, int64_t c) {
+    const int64_t len = t->sl[j][g];
+    const unsigned char *p = (const unsigned char *)t->sp[j][g];
+    uint64_t u = 0;
+    for (int b = 0; b < 8; b++) {
+        const int64_t k = 8 * c + b;
+        u = (u << 8) | (k < len ? p[k] : 0u);
+    }
+    return u;
+}
+
+/* the keys of one pass: pass -1 = the class; for strings 0 = the length
+ * and c + 1 = chunk c; for numbers 0 = the value */
+static int pass_keys(const TupleCmp *t, int j, int64_t pass, const uint32_t *hp, uint64_t *hk, int64_t n) {
+    int differ = 0;
+    for (int64_t i = 0; i < n; i++) {
+        const int64_t g = hp[i];
+        const int r = tuple_rank(t, j, g);
+        uint64_t k;
+        if (pass < 0) k = (uint64_t)r;
+        else if (r) k = 0;
+        else if (t->type[j] != VEC_STRING) k = order_bits(t, j, g);
+        else k = pass == 0 ? (uint64_t)t->sl[j][g] : chunk_bits(t, j, g, pass - 1);
+        hk[i] = k;
+        differ |= i > 0 && k != hk[0];
+    }
+    return differ;
+}
+
 static int device_tuple_order(const TupleCmp *t, int64_t *order, int64_t n) {
-    if (n < DEV_SORT_MIN || n >= ((int64_t)1 << 32)) return 0;
-    for (int j = 0; j < t->m; j++)
-        if (t->type[j] == VEC_STRING) return 0;
+    if (n < 2 || n >= ((int64_t)1 << 32)) return 0;
     uint64_t *hk = (uint64_t *)malloc((size_t)n * 8);
     uint32_t *hp = (uint32_t *)malloc((size_t)n * 4);
     uint64_t *dk = (uint64_t *)vgpu_malloc((size_t)n * 8);
@@ -287,15 +318,16 @@ static int device_tuple_order(const TupleCmp *t, int64_t *order, int64_t n) {
     for (int64_t i = 0; i < n; i++) hp[i] = (uint32_t)order[i];
     int rc = 0;
     for (int j = t->m - 1; j >= 0 && rc == 0; j--) {
-        for (int pass = 0; pass < 2 && rc == 0; pass++) {
-            int special = 0;
-            for (int64_t i = 0; i < n; i++) {
-                const int r = tuple_rank(t, j, hp[i]);
-                special |= r != 0;
-                hk[i] = pass == 0 ? (r == 0 ? order_bits(t, j, hp[i]) : 0) : (uint64_t)r;
-            }
-            if (pass == 1 && !special) continue;
-            const int end_bit = pass == 0 ? (t->type[j] == VEC_BOOL ? 1 : 64) : 2;
+        int64_t chunks = 0;
+        if (t->type[j] == VEC_STRING)
+            for (int64_t g = 0; g < n; g++)
+                if (t->ok[j][g] && (t->sl[j][g] + 7) / 8 > chunks) chunks = (t->sl[j][g] + 7) / 8;
+        /* value passes, least significant first, then the class */
+        const int64_t n_pass = t->type[j] == VEC_STRING ? 1 + chunks : 1;
+        for (int64_t q = 0; q <= n_pass && rc == 0; q++) {
+            const int64_t pass = q == n_pass ? -1 : t->type[j] == VEC_STRING ? (q == 0 ? 0 : n_pass - q) : 0;
+            if (!pass_keys(t, j, pass, hp, hk, n)) continue;
+            const int end_bit = pass < 0 ? 2 : t->type[j] == VEC_BOOL ? 1 : 64;
             if (vgpu_memcpy_h2d(dk, hk, (size_t)n * 8) < 0 || vgpu_memcpy_h2d(dp, hp, (size_t)n * 4) < 0 ||
                 vgpu_sort_pairs_u64(dk, dp, n, 0, end_bit) < 0 || vgpu_memcpy_d2h(hp, dp, (size_t)n * 4) < 0)
                 rc = -1;
