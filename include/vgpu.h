@@ -233,6 +233,26 @@ int vgpu_key_codes(const vgpu_col *keys, int n_keys, int64_t n_rows, int64_t pos
 int vgpu_string_keys(const int64_t *offsets, const uint8_t *chars, const uint8_t *validity, int64_t n_rows,
                      int64_t pos_base, int64_t *h1, int64_t *h2, int64_t *pos);
 
+/* ---- string comparison ------------------------------------------------
+ * vec_cmp on two string operands (src/scalar_ops.c:77-106): per row
+ * cmp = memcmp over the shorter length, then the shorter string first, as
+ * -1 / 0 / 1 into out (int64), NA where either side is NA.  A side is a
+ * device string column (offsets n+1 entries, the bytes they index,
+ * validity bitmap or NULL) or a constant (constant != 0: offsets[0..1]
+ * bound the one string for every row; null_const: the NA string).  out
+ * has n_rows entries; out_valid (8-byte aligned, (n_rows + 63) / 64 words)
+ * gets a bit per row.  A filter compiles `a op b` on strings to
+ * `cmp op 0` over this column. */
+typedef struct {
+    const int64_t *offsets;
+    const uint8_t *chars;
+    const uint8_t *validity;
+    int32_t constant;
+    int32_t null_const;
+} vgpu_strcol;
+int vgpu_string_compare(const vgpu_strcol *a, const vgpu_strcol *b, int64_t n_rows, int64_t *out,
+                        uint8_t *out_valid);
+
 /* ---- exact string dictionary -----------------------------------------
  * Replaces the byte comparisons of the reference's string keys
  * (src/hash.c:58-66 in group-by tables, src/join.c:96-108 in join chains):

@@ -4,7 +4,7 @@ filter parity test, selection vectors bit-exact.  No GPU needed."""
 import numpy as np
 import pytest
 
-from test_gpu_filter import PREDS, make_table
+from test_gpu_filter import PREDS, STR_PREDS, make_str_table, make_table
 
 
 @pytest.mark.parametrize("pi", range(len(PREDS)))
@@ -12,6 +12,18 @@ def test_compiled_predicate_matches_oracle(native, oracle, pi):
     mk = make_table(native, 5003, 100 + pi, 997)
     got = native.host_eval_batches(mk(), PREDS[pi])
     exp = [b["sel"] for b in native.next_batches(native.filter_node("oracle", mk().scan(), PREDS[pi]))]
+    assert len(got) == len(exp)
+    for g, e in zip(got, exp):
+        assert np.array_equal(g, e)
+
+
+@pytest.mark.parametrize("pi", range(len(STR_PREDS)))
+def test_compiled_string_predicate_matches_oracle(native, oracle, pi):
+    """string comparisons compiled to a -1 / 0 / 1 column (computed on the
+    host here) compared with 0 by the program"""
+    mk = make_str_table(native, 3001, 500 + pi, 700)
+    got = native.host_eval_batches(mk(), STR_PREDS[pi])
+    exp = [b["sel"] for b in native.next_batches(native.filter_node("oracle", mk().scan(), STR_PREDS[pi]))]
     assert len(got) == len(exp)
     for g, e in zip(got, exp):
         assert np.array_equal(g, e)

@@ -30,6 +30,31 @@ PREDS = [
 ]
 
 
+STR_POOL = ["", "a", "a\x00", "ab", "abc", "abd", "b", "é", "ÿ", "z", "key-001", "key-010", "key-1", "x" * 40]
+STR_PREDS = [
+    ("cmp", ">", ("col", "s"), ("str", "ab")),
+    ("cmp", "<=", ("str", "key-010"), ("col", "s")),
+    ("cmp", "==", ("col", "s"), ("col", "t")),
+    ("cmp", "<", ("col", "s"), ("col", "t")),
+    ("cmp", "!=", ("col", "s"), ("str", "")),
+    ("cmp", ">=", ("col", "t"), ("str", "é")),
+    ("cmp", "==", ("col", "s"), ("na_str",)),
+    ("and", ("cmp", ">", ("col", "s"), ("str", "a")), ("cmp", "<", ("col", "k"), ("int", 500))),
+    ("or", ("cmp", "==", ("col", "s"), ("str", "z")), ("cmp", "<", ("col", "t"), ("col", "s"))),
+    ("not", ("cmp", "<", ("col", "s"), ("str", "key-1"))),
+]
+
+
+def make_str_table(nv, n, seed, rg):
+    def mk():
+        r = rng(seed)
+        s = np.array([STR_POOL[i] for i in r.integers(0, len(STR_POOL), n)], dtype=object)
+        t = np.array([STR_POOL[i] for i in r.integers(0, len(STR_POOL), n)], dtype=object)
+        return nv.Table({"s": (s, with_na(r, n, 0.05)), "t": (t, with_na(r, n, 0.03)),
+                         "k": r.integers(0, 1000, n), "id": np.arange(n)}, rg_rows=rg)
+    return mk
+
+
 def make_table(nv, n, seed, rg):
     r = rng(seed)
     return lambda: nv.Table({
@@ -139,3 +164,45 @@ def test_filter_type_errors(gpu):
     t = nv.Table({"x": np.array([1., 2.])})
     with pytest.raises(nv.VectraError, match="bool: operands must be boolean"):
         nv.collect(nv.filter_node("gpu", t.scan(), ("and", ("col", "x"), ("lgl", True))))
+
+
+@pytest.mark.parametrize("pi", range(len(STR_PREDS)))
+def test_string_predicates_match_oracle(gpu, oracle, pi):
+    """string comparisons (src/scalar_ops.c:77-106) against literals, the
+    NA string and other string columns, inside & / | / !: a -1 / 0 / 1
+    column per comparison computed on the device, the program compares it
+    with 0; selection vectors bit-exact over row groups"""
+    mk = make_str_table(gpu, 9001, 300 + pi, 1500)
+    same_batches(run(gpu, "gpu", mk(), STR_PREDS[pi]), run(gpu, "oracle", mk(), STR_PREDS[pi]))
+
+
+def test_string_predicate_below_group_by_and_join(gpu, oracle):
+    """a string filter under a GPU group-by (the device scan feed leaves it
+    to the FilterNode) and under a GPU join's probe side"""
+    mk = make_str_table(gpu, 20_000, 911, 3000)
+    pred = ("cmp", ">=", ("col", "s"), ("str", "b"))
+    out = {}
+    for engine in ("gpu", "oracle"):
+        node = gpu.filter_node(engine, mk().scan(), pred)
+        out[engine] = gpu.collect(gpu.group_agg_node(engine, node, ["t"], [("n", "n", None, 0),
+                                                                         ("sk", "sum", "k", 0)], "/tmp"))
+    assert_col_exact("t", out["gpu"]["t"], out["oracle"]["t"])
+    assert_col_exact("n", out["gpu"]["n"], out["oracle"]["n"])
+    assert_col_exact("sk", out["gpu"]["sk"], out["oracle"]["sk"])
+    dim = {"dk": np.arange(0, 1000, 3), "w": np.arange(0, 1000, 3) * 0.5}
+    for kind in ("inner", "left"):
+        got = gpu.next_batches(gpu.join_node("gpu", gpu.filter_node("gpu", mk().scan(), pred),
+                                             gpu.Table(dim).scan(), kind, [(2, 0)]))
+        exp = gpu.next_batches(gpu.join_node("oracle", gpu.filter_node("oracle", mk().scan(), pred),
+                                             gpu.Table(dim).scan(), kind, [(2, 0)]))
+        assert len(got) == len(exp)
+        for g, e in zip(got, exp):
+            for c in e["cols"]:
+                assert_col_exact(c, g["cols"][c], e["cols"][c])
+
+
+def test_string_against_number_is_an_error(gpu):
+    mk = make_str_table(gpu, 100, 3, 0)
+    node = gpu.filter_node("gpu", mk().scan(), ("cmp", ">", ("col", "s"), ("int", 3)))
+    with pytest.raises(gpu.VectraError, match="cannot coerce string in arithmetic/comparison"):
+        gpu.next_batches(node)

@@ -234,6 +234,42 @@ def test_compare_brute_force(native, oracle, op, types):
         assert valid[i] and bool(vals[i]) == OPS[op](sign3(x, y)), f"row {i}: {a[i]} {op} {b[i]}"
 
 
+STR_POOL = ["", "\x00", "a", "a\x00", "ab", "abc", "b", "é", "ÿ", "z", "zz", "a" * 9, "Z"]
+
+
+@pytest.mark.parametrize("op", list(OPS))
+def test_compare_strings_against_bytes_order(native, oracle, op):
+    """two string columns, and a column against a literal on either side:
+    Python orders bytes as the reference does (memcmp over the shorter
+    length, then the shorter first, src/scalar_ops.c:77-106), so each row
+    is checked against bytes comparison; NA on either side is NA"""
+    nv = native
+    r = rng(seed_of(op, "str", 3))
+    n = 400
+    a = np.array([STR_POOL[i] for i in r.integers(0, len(STR_POOL), n)], dtype=object)
+    b = np.array([STR_POOL[i] for i in r.integers(0, len(STR_POOL), n)], dtype=object)
+    av, bv = r.random(n) > 0.1, r.random(n) > 0.1
+    cols = {"a": (a, av), "b": (b, bv)}
+    key = lambda x: x.encode()
+    for tree, rhs in ((("cmp", op, ("col", "a"), ("col", "b")), None),
+                      (("cmp", op, ("col", "a"), ("str", "ab")), "ab"),
+                      (("cmp", op, ("str", "a\x00"), ("col", "a")), "a\x00")):
+        if rhs == "a\x00":   # literals end at their first NUL byte, as C strings do
+            rhs = "a"
+        vals, valid, rtype = eval_oracle(nv, oracle, cols, tree)
+        assert rtype == nv.VEC_BOOL
+        for i in range(n):
+            if tree[2][0] == "str":
+                ok, c = av[i], sign3(key(rhs), key(a[i]))
+            elif rhs is None:
+                ok, c = av[i] and bv[i], sign3(key(a[i]), key(b[i]))
+            else:
+                ok, c = av[i], sign3(key(a[i]), key(rhs))
+            assert bool(valid[i]) == bool(ok), f"row {i}"
+            if ok:
+                assert bool(vals[i]) == OPS[op](c), f"row {i}: {tree}"
+
+
 def test_compare_int64_double_rounding(native, oracle):
     """an int64 operand is converted to double first: 2^53 + 1 == 2^53.0"""
     nv = native

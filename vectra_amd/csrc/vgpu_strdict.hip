@@ -316,3 +316,82 @@ extern "C" int64_t vgpu_strdict_size(const vgpu_strdict *d, int64_t *rows) {
     if (rows) *rows = d->rows;
     return d->distinct;
 }
+
+// ---------------------------------------------------------------------------
+// string comparison (vec_cmp on strings, src/scalar_ops.c:77-106)
+// ---------------------------------------------------------------------------
+
+struct ScSide {
+    const int64_t *off;
+    const uint8_t *chr;
+    const uint8_t *valid;
+    int32_t constant, null_const;
+};
+
+__device__ __forceinline__ bool sc_get(const ScSide &S, int64_t r, const uint8_t *&p, int64_t &len) {
+    if (S.null_const) return false;
+    const int64_t i = S.constant ? 0 : r;
+    if (!S.constant && S.valid && !((S.valid[r >> 3] >> (r & 7)) & 1)) return false;
+    p = S.chr + S.off[i];
+    len = S.off[i + 1] - S.off[i];
+    return true;
+}
+
+// rows are taken 64 to a wave in order (block size and grid stride are
+// multiples of 64), so a wave's ballot is one whole word of the bitmap
+__global__ void __launch_bounds__(256) k_string_compare(ScSide A, ScSide B, int64_t n, int64_t *out,
+                                                        uint64_t *out_valid) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
+        const uint8_t *pa = nullptr, *pb = nullptr;
+        int64_t la = 0, lb = 0;
+        const bool ok = sc_get(A, r, pa, la) & sc_get(B, r, pb, lb);
+        int64_t c = 0;
+        if (ok) {
+            const int64_t m = la < lb ? la : lb;
+            for (int64_t k = 0; k < m && c == 0; k++) c = (int64_t)(pa[k] > pb[k]) - (int64_t)(pa[k] < pb[k]);
+            if (c == 0) c = (int64_t)(la > lb) - (int64_t)(la < lb);
+        }
+        out[r] = c;
+        const uint64_t w = vg_ballot(ok);
+        if (vg_lane() == 0) out_valid[r >> 6] = w;
+    }
+}
+
+static int sc_side(const vgpu_strcol *s, ScSide &S, const char *name) {
+    if (!s) {
+        vg_set_error("string compare: %s side missing", name);
+        return -1;
+    }
+    S.off = s->offsets;
+    S.chr = s->chars;
+    S.valid = s->validity;
+    S.constant = s->constant;
+    S.null_const = s->null_const;
+    if (!S.null_const && !S.off) {
+        vg_set_error("string compare: %s side has no offsets", name);
+        return -1;
+    }
+    return 0;
+}
+
+extern "C" int vgpu_string_compare(const vgpu_strcol *a, const vgpu_strcol *b, int64_t n_rows, int64_t *out,
+                                   uint8_t *out_valid) {
+    if (vg_ensure_init()) return -1;
+    if (n_rows < 0 || (n_rows > 0 && (!out || !out_valid))) {
+        vg_set_error("string compare: invalid arguments");
+        return -1;
+    }
+    if (reinterpret_cast<uintptr_t>(out_valid) & 7) {
+        vg_set_error("string compare: the validity bitmap must be 8-byte aligned");
+        return -1;
+    }
+    ScSide A, B;
+    if (sc_side(a, A, "left") || sc_side(b, B, "right")) return -1;
+    if (n_rows == 0) return 0;
+    // one wave's rows share a bitmap word: the grid covers whole words
+    k_string_compare<<<vg_blocks(n_rows, 256, 8192), 256, 0, vg_stream()>>>(A, B, n_rows, out,
+                                                                            reinterpret_cast<uint64_t *>(out_valid));
+    VG_LAUNCH_CHECK();
+    return 0;
+}

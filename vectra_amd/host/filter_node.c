@@ -18,6 +18,7 @@ typedef struct {
     DevCol cols[VGPU_MAX_COLS];
     DevMem sel_in;
     DevMem sel_out;
+    DevMem sstage[6];   /* string comparisons: operands staged on the device */
 } FilterGpu;
 
 static VecBatch *gpu_filter_next(VecNode *self) {
@@ -38,6 +39,10 @@ static VecBatch *gpu_filter_next(VecNode *self) {
         if (n_logical > 0) {
             vgpu_col cols[VGPU_MAX_COLS];
             for (int s = 0; s < cx.n_cols; s++) {
+                if (cx.col_idx[s] < 0) {   /* a string comparison's -1 / 0 / 1 column */
+                    cols[s] = vx_string_cmp_col(&cx, -1 - cx.col_idx[s], b, &g->cols[s], g->sstage);
+                    continue;
+                }
                 devcol_upload(&g->cols[s], &b->columns[cx.col_idx[s]], b->n_rows);
                 cols[s] = g->cols[s].col;
             }
@@ -68,6 +73,7 @@ static void gpu_filter_free(VecNode *self) {
         for (int i = 0; i < VGPU_MAX_COLS; i++) devcol_free(&g->cols[i]);
         devmem_free(&g->sel_in);
         devmem_free(&g->sel_out);
+        for (int i = 0; i < 6; i++) devmem_free(&g->sstage[i]);
         free(g);
     }
     vec_schema_free(&fn->base.output_schema);

@@ -174,6 +174,54 @@ static int cx_col(Cx *c, const char *name) {
     return r;
 }
 
+/* the batch column a string operand names, -1 for a literal / NA (then
+ * *lit / *na are set), -2 when e is no string operand */
+static int str_operand(Cx *c, const VecExpr *e, const char **lit, int *na) {
+    *lit = NULL;
+    *na = 0;
+    if (e->kind == EXPR_LIT_STRING) {
+        *lit = e->lit_str ? e->lit_str : "";
+        return -1;
+    }
+    if (e->kind == EXPR_LIT_NA && e->result_type == VEC_STRING) {
+        *na = 1;
+        return -1;
+    }
+    if (e->kind != EXPR_COL_REF) return -2;
+    const VecBatch *b = c->batch;
+    for (int i = 0; i < b->n_cols; i++)
+        if (strcmp(b->col_names[i], e->col_name) == 0) return b->columns[i].type == VEC_STRING ? i : -2;
+    return -2;
+}
+
+static uint8_t cmp_opcode(char op, char op2);
+
+/* `l op r` on two strings: a synthetic int64 column cmp(l, r) in a new
+ * column slot, compared against 0; -1 when an operand is no string */
+static int cx_string_cmp(Cx *c, const VecExpr *e) {
+    const char *ll, *rl;
+    int lna, rna;
+    const int lc = str_operand(c, e->left, &ll, &lna), rc = str_operand(c, e->right, &rl, &rna);
+    const int ls = lc != -2, rs = rc != -2;
+    if (!ls && !rs) return -1;
+    if (ls != rs) vectra_error("cannot coerce string in arithmetic/comparison");
+    CompiledExpr *o = c->out;
+    if (o->n_scmp >= VX_MAX_SCMP) vectra_error("GPU: more than %d string comparisons in an expression", VX_MAX_SCMP);
+    if (o->n_cols >= VGPU_MAX_COLS) vectra_error("GPU: expression reads more than %d columns", VGPU_MAX_COLS);
+    const int k = o->n_scmp++;
+    o->scmp[k] = (VxStrCmp){{lc, rc}, {ll, rl}, {lna, rna}};
+    const int slot = o->n_cols++;
+    o->col_idx[slot] = -1 - k;
+    const int col = cx_push(c, VGPU_OP_COL, VEC_INT64, -1, -1);
+    o->prog.insn[col].col = slot;
+    const int zero = cx_push(c, VGPU_OP_CONST, VEC_INT64, -1, -1);
+    o->prog.insn[zero].imm.i = 0;
+    const uint8_t op = cmp_opcode(e->op, e->op2);
+    const int r = op ? cx_push(c, op, VEC_INT64, col, zero) : cx_push(c, VGPU_OP_NE, VEC_INT64, col, col);
+    c->rtype[r] = VEC_BOOL;
+    return r;
+}
+
 static uint8_t cmp_opcode(char op, char op2) {
     if (op == '=' && op2 == '=') return VGPU_OP_EQ;
     if (op == '!' && op2 == '=') return VGPU_OP_NE;
@@ -229,6 +277,8 @@ static int cx_emit(Cx *c, const VecExpr *e) {
         return cx_push(c, op, t, l, r);
     }
     case EXPR_CMP: {
+        const int sc = cx_string_cmp(c, e);
+        if (sc >= 0) return sc;
         int l = cx_emit(c, e->left), r = cx_emit(c, e->right);
         VecType t = vx_common_type(c->rtype[l], c->rtype[r]);
         if (t == VEC_BOOL) t = VEC_INT64;
@@ -297,6 +347,81 @@ int vx_try_compile_expr(const VecExpr *e, const VecBatch *batch, CompiledExpr *o
         return 0;
     }
     return -1;
+}
+
+/* one side of a string comparison on the device: the batch column's
+ * offsets / bytes / validity uploaded, or the literal as a one-row column */
+static vgpu_strcol scmp_side(const VxStrCmp *s, int side, const VecBatch *b, DevMem *off, DevMem *chr, DevMem *val) {
+    vgpu_strcol sc = {NULL, NULL, NULL, 0, 0};
+    if (s->na[side]) {
+        sc.null_const = 1;
+        return sc;
+    }
+    if (s->col[side] < 0) {
+        const int64_t len = (int64_t)strlen(s->lit[side]);
+        const int64_t o[2] = {0, len};
+        sc.offsets = (const int64_t *)devmem_reserve(off, 16);
+        sc.chars = (const uint8_t *)devmem_reserve(chr, (size_t)(len > 0 ? len : 1));
+        VX_GPU(vgpu_memcpy_h2d((void *)sc.offsets, o, 16));
+        if (len > 0) VX_GPU(vgpu_memcpy_h2d((void *)sc.chars, s->lit[side], (size_t)len));
+        sc.constant = 1;
+        return sc;
+    }
+    const VecArray *a = &b->columns[s->col[side]];
+    const int64_t n = b->n_rows, bytes = a->buf.str.offsets[n];   /* the bytes the offsets index */
+    int64_t *d_off = (int64_t *)devmem_reserve(off, (size_t)(n + 1) * 8);
+    uint8_t *d_chr = (uint8_t *)devmem_reserve(chr, (size_t)(bytes > 0 ? bytes : 1));
+    VX_GPU(vgpu_memcpy_h2d(d_off, a->buf.str.offsets, (size_t)(n + 1) * 8));
+    if (bytes > 0) VX_GPU(vgpu_memcpy_h2d(d_chr, a->buf.str.data, (size_t)bytes));
+    sc.offsets = d_off;
+    sc.chars = d_chr;
+    if (!vx_all_valid(a->validity, n)) {
+        const size_t vb = (size_t)vec_validity_bytes(n);
+        sc.validity = (const uint8_t *)devmem_reserve(val, vb + 8);
+        VX_GPU(vgpu_memcpy_h2d((void *)sc.validity, a->validity, vb));
+    }
+    return sc;
+}
+
+vgpu_col vx_string_cmp_col(const CompiledExpr *cx, int k, const VecBatch *b, DevCol *dc, DevMem stage[6]) {
+    const int64_t n = b->n_rows;
+    devcol_reserve(dc, (size_t)(n > 0 ? n : 1) * 8, (size_t)((n + 63) / 64 + 1) * 8);
+    const VxStrCmp *s = &cx->scmp[k];
+    const vgpu_strcol l = scmp_side(s, 0, b, &stage[0], &stage[1], &stage[2]);
+    const vgpu_strcol r = scmp_side(s, 1, b, &stage[3], &stage[4], &stage[5]);
+    VX_GPU(vgpu_string_compare(&l, &r, n, (int64_t *)dc->d_data, dc->d_valid));
+    dc->col = (vgpu_col){VGPU_INT64, 0, dc->d_data, dc->d_valid};
+    return dc->col;
+}
+
+static int scmp_host_get(const VxStrCmp *s, int side, const VecBatch *b, int64_t r, const char **p, int64_t *len) {
+    if (s->na[side]) return 0;
+    if (s->col[side] < 0) {
+        *p = s->lit[side];
+        *len = (int64_t)strlen(s->lit[side]);
+        return 1;
+    }
+    const VecArray *a = &b->columns[s->col[side]];
+    if (!vec_array_is_valid(a, r)) return 0;
+    *p = a->buf.str.data + a->buf.str.offsets[r];
+    *len = a->buf.str.offsets[r + 1] - a->buf.str.offsets[r];
+    return 1;
+}
+
+void vx_string_cmp_host(const CompiledExpr *cx, int k, const VecBatch *b, int64_t *out, uint8_t *valid) {
+    const VxStrCmp *s = &cx->scmp[k];
+    memset(valid, 0, (size_t)vec_validity_bytes(b->n_rows));
+    for (int64_t r = 0; r < b->n_rows; r++) {
+        const char *pa = NULL, *pb = NULL;
+        int64_t la = 0, lb = 0;
+        out[r] = 0;
+        if (!scmp_host_get(s, 0, b, r, &pa, &la) || !scmp_host_get(s, 1, b, r, &pb, &lb)) continue;
+        const int64_t m = la < lb ? la : lb;
+        int c = m > 0 ? memcmp(pa, pb, (size_t)m) : 0;
+        c = c < 0 ? -1 : c > 0 ? 1 : (la > lb) - (la < lb);
+        out[r] = c;
+        valid[r >> 3] |= (uint8_t)(1u << (r & 7));
+    }
 }
 
 /* ------------------------------------------------------------------ */

@@ -51,14 +51,35 @@ const char *vx_type_name(VecType t);            /* src/types.h:14-22 */
  * expression's type.  Raises vectra_error with the reference's messages
  * for type errors and with an explicit message for shapes the GPU path
  * does not evaluate (string operands). */
+/* a comparison of two strings (src/scalar_ops.c:77-106), evaluated before
+ * the program into an int64 column of -1 / 0 / 1 that the program compares
+ * against 0; each side is a batch column (col >= 0), a literal (lit) or
+ * the NA string (na) */
+#define VX_MAX_SCMP 4
+typedef struct {
+    int         col[2];
+    const char *lit[2];
+    int         na[2];
+} VxStrCmp;
+
 typedef struct {
     vgpu_prog prog;
     int       n_cols;
-    int       col_idx[VGPU_MAX_COLS];
+    int       col_idx[VGPU_MAX_COLS];   /* batch column, or -1 - k: string comparison k */
     VecType   result_type;
+    int       n_scmp;
+    VxStrCmp  scmp[VX_MAX_SCMP];
 } CompiledExpr;
 
 void vx_compile_expr(const VecExpr *e, const VecBatch *batch, CompiledExpr *out);
+
+/* the column of string comparison k of cx over batch b on the device
+ * (n_rows entries; buffers in dc, each operand's offsets, bytes and
+ * validity staged in stage[0..2] and stage[3..5]); ordered on the library
+ * stream before the work that reads it */
+vgpu_col vx_string_cmp_col(const CompiledExpr *cx, int k, const VecBatch *b, DevCol *dc, DevMem stage[6]);
+/* the same on the host: out (b->n_rows) and a validity bitmap */
+void vx_string_cmp_host(const CompiledExpr *cx, int k, const VecBatch *b, int64_t *out, uint8_t *valid);
 /* the same, returning -1 (message in vx_last_error) instead of raising */
 int vx_try_compile_expr(const VecExpr *e, const VecBatch *batch, CompiledExpr *out);
 

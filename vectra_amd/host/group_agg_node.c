@@ -158,6 +158,7 @@ static int feed_from_scan(GroupAggNode *ga, vgpu_groupby *gb, int key_idx, int n
         vec_batch_free(shape);
         if (rc) vectra_error("%s", vx_last_error());
         if (cx.result_type != VEC_BOOL) vectra_error("filter predicate must evaluate to boolean");
+        if (cx.n_scmp > 0) return 0;   /* string comparisons: the FilterNode's batches take them */
         for (int p = 0; p < cx.n_cols; p++) pslot[p] = fed_slot(fcols, &nf, scan_file_col(sn, cx.col_idx[p]));
     }
     vx_devscan_open(&g->ds, sn, fcols, nf);

@@ -319,8 +319,13 @@ def _node_protos(lib: C.CDLL, prefix: str) -> None:
                                                       C.c_char_p])
 
 
+class VxStrCmp(C.Structure):
+    _fields_ = [("col", C.c_int * 2), ("lit", C.c_char_p * 2), ("na", C.c_int * 2)]
+
+
 class CompiledExpr(C.Structure):
-    _fields_ = [("prog", VgpuProg), ("n_cols", C.c_int), ("col_idx", C.c_int * 8), ("result_type", C.c_int)]
+    _fields_ = [("prog", VgpuProg), ("n_cols", C.c_int), ("col_idx", C.c_int * 8), ("result_type", C.c_int),
+                ("n_scmp", C.c_int), ("scmp", VxStrCmp * 4)]
 
 
 def nodes() -> C.CDLL:
@@ -329,6 +334,7 @@ def nodes() -> C.CDLL:
         lib = C.CDLL(str(_need(LIB / "libvectra_gpu.so")))
         _node_protos(lib, "")
         _proto(lib, "vx_try_compile_expr", C.c_int, [EXPR_P, BATCH_P, C.POINTER(CompiledExpr)])
+        _proto(lib, "vx_string_cmp_host", None, [C.POINTER(CompiledExpr), C.c_int, BATCH_P, C.c_void_p, C.c_void_p])
         _proto(lib, "vx_serve_run", C.c_int, [C.POINTER(VxServeCfg), C.POINTER(VxRequest), C.c_int,
                                               C.POINTER(VxResult), C.POINTER(VxServeStats)])
         _proto(lib, "vx_result_free", None, [C.POINTER(VxResult)])
@@ -361,19 +367,27 @@ def host_eval_batches(table: "Table", pred: tuple) -> list[np.ndarray]:
             raise VectraError(core().vx_last_error().decode())
         if not b:
             break
-        e = expr(pred)
+        e = expr(pred)   # freed after the evaluation: string comparisons point at its literals
         cx = CompiledExpr()
         rc = lib.vx_try_compile_expr(e, b, C.byref(cx))
-        core().vec_expr_free(e)
-        if rc != 0:
+        if rc != 0 or cx.result_type != VEC_BOOL:
+            core().vec_expr_free(e)
             core().vec_batch_free(b)
-            raise VectraError(core().vx_last_error().decode())
-        if cx.result_type != VEC_BOOL:
-            core().vec_batch_free(b)
-            raise VectraError("filter predicate must evaluate to boolean")
+            raise VectraError(core().vx_last_error().decode() if rc != 0 else
+                              "filter predicate must evaluate to boolean")
         bt = b.contents
         cols = (VgpuCol * 8)()
+        keep = []
         for s in range(cx.n_cols):
+            if cx.col_idx[s] < 0:   # a string comparison's -1 / 0 / 1 column, computed on the host
+                vals = np.zeros(max(bt.n_rows, 1), np.int64)
+                bm = np.zeros(max((bt.n_rows + 7) // 8, 1), np.uint8)
+                lib.vx_string_cmp_host(C.byref(cx), -1 - cx.col_idx[s], b, vals.ctypes.data, bm.ctypes.data)
+                keep += [vals, bm]
+                cols[s].type = VEC_INT64
+                cols[s].data = vals.ctypes.data
+                cols[s].validity = bm.ctypes.data
+                continue
             a = bt.columns[cx.col_idx[s]]
             cols[s].type = a.type
             cols[s].data = C.cast(a.buf.i64, C.c_void_p)
@@ -387,6 +401,7 @@ def host_eval_batches(table: "Table", pred: tuple) -> list[np.ndarray]:
                                       C.byref(k)))
         if k.value:
             out.append(sel[:k.value].copy())
+        core().vec_expr_free(e)
         core().vec_batch_free(b)
     core().vx_node_free(scan)
     return out
@@ -536,7 +551,7 @@ def _malloc_str(s: str) -> int:
 
 def expr(tree: Any) -> Any:
     """Builds a malloc'd VecExpr from a nested tuple:
-    ("col", name) ("dbl", v) ("int", v) ("lgl", v) ("na",) ("cmp", op, l, r)
+    ("col", name) ("dbl", v) ("int", v) ("lgl", v) ("str", s) ("na",) ("na_str",) ("cmp", op, l, r)
     ("arith", op, l, r) ("and", l, r) ("or", l, r) ("not", x) ("is_na", x) ("neg", x)."""
     lib = core()
     kind = tree[0]
@@ -562,6 +577,15 @@ def expr(tree: Any) -> Any:
     if kind == "na":
         e = lib.vec_expr_alloc(EXPR_LIT_NA)
         e.contents.result_type = VEC_DOUBLE
+        return e
+    if kind == "str":
+        e = lib.vec_expr_alloc(EXPR_LIT_STRING)
+        e.contents.lit_str = _malloc_str(tree[1])
+        e.contents.result_type = VEC_STRING
+        return e
+    if kind == "na_str":
+        e = lib.vec_expr_alloc(EXPR_LIT_NA)
+        e.contents.result_type = VEC_STRING
         return e
     if kind in ("cmp", "arith"):
         e = lib.vec_expr_alloc(EXPR_CMP if kind == "cmp" else EXPR_ARITH)
