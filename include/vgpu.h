@@ -300,6 +300,9 @@ int vgpu_groupby_result_device(vgpu_groupby *g, const void **keys,
                                const uint8_t **validity);
 
 void vgpu_groupby_destroy(vgpu_groupby *g);
+/* 8-byte words of one group's record (hot + cold): the width axis of the
+ * tuning table's buckets */
+int  vgpu_groupby_record_words(const vgpu_groupby *g);
 
 /* ---- hash join -------------------------------------------------------- */
 enum { VGPU_JOIN_INNER = 0, VGPU_JOIN_LEFT, VGPU_JOIN_FULL, VGPU_JOIN_SEMI,
@@ -529,6 +532,9 @@ void    vgpu_batch_destroy(vgpu_batch *b);
  *                         filter (3 bits a key), read when a table is built
  *   "join.load_pct"      (50) highest load of a join table's slots, in
  *                         percent (10..95), read when a table is built
+ *                         (bucket: build rows, entry words)
+ *   "gb.glb.load_pct"    (50) highest load of a group-by's HBM table, in
+ *                         percent (25..90), read when the table is sized
  *   "gb.fuse_join"       (2) a GPU GroupAggNode over Project(a * b) over a
  *                         GPU JoinNode runs the three as one fused device
  *                         pass (2), or node by node (1)
@@ -537,18 +543,25 @@ void    vgpu_batch_destroy(vgpu_batch *b);
  *                         fewer bits make different strings share slots and
  *                         tags, which the byte comparisons resolve (a test
  *                         of exactness under collisions)
- * Returns -1 for an unknown name or a value <= 0. */
+ * Returns -1 for an unknown name or a value <= 0 (or out of its range). */
 int     vgpu_tune_set(const char *name, int64_t value);
 int64_t vgpu_tune_get(const char *name);
-/* A knob value for group-bys expecting at most max_groups groups (a
- * cardinality bucket of the autotuner's table; the smallest covering
- * bucket that sets the knob wins, else the global value). */
+/* A knob value for one bucket of the autotuner's table: work expecting at
+ * most max_groups groups, whose records are at most max_width 8-byte words
+ * (a group-by's hot + cold record, a join table's entry) and whose launch
+ * covers at most max_rows rows.  A lookup takes the first bucket that
+ * covers the work and sets the knob, in ascending (max_width, max_rows,
+ * max_groups) order, else the global value. */
+int     vgpu_tune_set_at(int64_t max_groups, int64_t max_width, int64_t max_rows, const char *name, int64_t value);
+/* the same bucket covering any width and row count */
 int     vgpu_tune_set_for(int64_t max_groups, const char *name, int64_t value);
-/* the value a group-by expecting `groups` groups uses (-1: unknown name) */
+/* the value work of `groups` groups, `width` words and `rows` rows uses
+ * (0 for width / rows: any); -1 for an unknown name */
+int64_t vgpu_tune_get_at(int64_t groups, int64_t width, int64_t rows, const char *name);
 int64_t vgpu_tune_get_for(int64_t groups, const char *name);
 /* Loads a table written by the autotuner (vectra_amd/autotune.py): lines
- * "max_groups name value", max_groups 0 for global values.  Returns the
- * number of settings or -1. */
+ * "max_groups name value [w<=W] [rows<=R]", max_groups 0 for global
+ * values.  Returns the number of settings or -1. */
 int     vgpu_tune_load(const char *path);
 /* Drops every bucket and restores the built-in global values. */
 void    vgpu_tune_reset(void);

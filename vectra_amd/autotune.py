@@ -1,23 +1,28 @@
-"""Autotuner of the group-by launch shapes.
+"""Autotuner of the launch shapes and table loads.
 
-For each cardinality bucket it runs the group-by the bucket stands for on
-the GPU (50M rows, int64 keys uniform over the bucket's cardinality, key
-order) with two record widths -- sum and n of one double input, and n with
-sum / min / max / mean of a double and an int64 input -- under every
-setting of the knobs that decide its shape -- LDS table slots per group
-(the table's inverse load factor), copies of a small LDS table, blocks and
-rows per block of the LDS-privatized feed, and the group count from which
-a large feed is hash-partitioned -- times the feed kernels with HIP
-events, and keeps per bucket the setting whose worse relative slowdown
-over the two widths is smallest.  The
-fused join -> aggregate shape (a unique-key dimension of 1M rows, 20M
-probe rows) picks the rows in flight per thread of its probe.
+A tuning-table bucket covers work by three axes: the groups it expects,
+the width of its records (8-byte words of a group-by's hot + cold record,
+of a join table's entry) and the rows of the launch.  For each cardinality
+bucket the tuner runs the group-by the bucket stands for on the GPU (int64
+keys uniform over the bucket's cardinality, key order) in two record
+widths -- sum and n of one double input (narrow), and n with sum / min /
+max / mean of a double and an int64 input (wide) -- at two row counts -- a
+row group of 4M rows and a 50M-row column -- under every setting of the
+knobs that decide its shape: LDS table slots per group (the LDS table's
+inverse load factor), copies of a small LDS table, blocks and rows per
+block of the LDS-privatized feed, the group count from which a large feed
+is hash-partitioned, rows per block of the HBM-table feed and the HBM
+table's load factor.  It times the feed kernels with HIP events and keeps
+the fastest setting per (width, rows, cardinality) bucket.  Join tables:
+the load factor of a 1M-key and a 10M-key build, timed over build and a
+20M / 50M-row fused probe -> aggregate; and the rows in flight per thread
+of that probe.
 
     python -m vectra_amd.autotune --out vectra_amd/tuning.txt
 
-writes "max_groups name value" lines that libvgpu loads at start-up
-(vgpu_tune_load; vectra_amd.native loads vectra_amd/tuning.txt, libvgpu
-the file named by VECTRA_GPU_TUNING).
+writes "max_groups name value [w<=W] [rows<=R]" lines that libvgpu loads
+at start-up (vgpu_tune_load; vectra_amd.native loads
+vectra_amd/tuning.txt, libvgpu the file named by VECTRA_GPU_TUNING).
 """
 from __future__ import annotations
 
@@ -33,6 +38,8 @@ import numpy as np
 from . import native as nv
 
 BUCKETS = [(64, 16), (1024, 256), (16384, 4096), (1 << 20, 65536), (1 << 40, 4_000_000)]
+ROWS = [(8_000_000, 4_000_000), (None, 50_000_000)]   # (rows bound of the bucket, rows swept)
+ANY = 1 << 40
 
 
 def _feed_ms(run, reps: int = 3) -> float:
@@ -50,15 +57,26 @@ def _feed_ms(run, reps: int = 3) -> float:
     return float(np.median(times))
 
 
-# two record widths: one sum and a count (3 words), and n with sum / min /
-# max / mean of a double and an int64 input (about 14 words); a setting is
-# scored by its worse relative slowdown over the two
+# two record widths: one sum and a count, and n with sum / min / max / mean
+# of a double and an int64 input; each gets its own best setting
 SHAPES = {"narrow": [("sum", 0, 0), ("n", -1, 0)],
           "wide": [("n", -1, 0), ("sum", 0, 0), ("min", 0, 0), ("max", 0, 0), ("mean", 0, 0),
                    ("sum", 1, 0), ("min", 1, 0), ("max", 1, 0), ("mean", 1, 0)]}
 
 
+def shape_types(shape):
+    return [nv.VEC_DOUBLE] if shape == "narrow" else [nv.VEC_DOUBLE, nv.VEC_INT64]
+
+
+def record_words(shape) -> int:
+    gb = nv.GroupBy(nv.VEC_INT64, shape_types(shape), SHAPES[shape], nv.VGPU_ORDER_KEY, 0)
+    w = nv.vgpu().vgpu_groupby_record_words(gb.h)
+    gb.free()
+    return int(w)
+
+
 def tune_groupby(card: int, rows: int, grid: dict, log) -> dict:
+    """{shape: (best knobs, ms)} for one cardinality and row count"""
     import torch
     lib = nv.vgpu()
     g = torch.Generator(device="cuda")
@@ -72,11 +90,10 @@ def tune_groupby(card: int, rows: int, grid: dict, log) -> dict:
     ic = nv.DeviceColumn.wrap(ival.data_ptr(), rows, nv.VEC_INT64, keep=ival)
 
     def runner(shape):
-        aggs = SHAPES[shape]
-        types, cols = ([nv.VEC_DOUBLE], [vc]) if shape == "narrow" else ([nv.VEC_DOUBLE, nv.VEC_INT64], [vc, ic])
+        cols = [vc] if shape == "narrow" else [vc, ic]
 
         def run():
-            gb = nv.GroupBy(nv.VEC_INT64, types, aggs, nv.VGPU_ORDER_KEY, 0)
+            gb = nv.GroupBy(nv.VEC_INT64, shape_types(shape), SHAPES[shape], nv.VGPU_ORDER_KEY, 0)
             gb.feed(kc, cols, rows)
             gb.finish()
             gb.free()
@@ -90,14 +107,53 @@ def tune_groupby(card: int, rows: int, grid: dict, log) -> dict:
             lib.vgpu_tune_set(n.encode(), v)
         ms = {s: _feed_ms(r) for s, r in runs.items()}
         results.append((dict(zip(names, values)), ms))
-        log(f"  card {card:>9,}: {dict(zip(names, values))} -> "
+        log(f"  card {card:>9,} rows {rows:>11,}: {dict(zip(names, values))} -> "
             + ", ".join(f"{s} {t:.3f} ms" for s, t in ms.items()))
     lib.vgpu_tune_reset()
-    best_of = {s: min(r[1][s] for r in results) for s in SHAPES}
-    knobs, ms = min(results, key=lambda r: max(r[1][s] / best_of[s] for s in SHAPES))
     del key, val, ival
     torch.cuda.empty_cache()
-    return {"ms": ms, "knobs": knobs}
+    return {s: min(((k, ms[s]) for k, ms in results), key=lambda r: r[1]) for s in SHAPES}
+
+
+def tune_join_load(log) -> list[tuple[int | None, int, float]]:
+    """(rows bound, best join.load_pct, ms) for a 1M-key and a 10M-key build"""
+    import torch
+    lib = nv.vgpu()
+    out = []
+    for bound, n_dim, n_fact in ((2_000_000, 1_000_000, 20_000_000), (None, 10_000_000, 50_000_000)):
+        g = torch.Generator(device="cuda")
+        g.manual_seed(n_dim)
+        dk = torch.randperm(n_dim, device="cuda", generator=g).to(torch.int64) * 7 + 3
+        attr = torch.randint(0, 256, (n_dim,), dtype=torch.int64, device="cuda", generator=g)
+        fk = dk[torch.randint(0, n_dim, (n_fact,), device="cuda", generator=g)]
+        v = torch.rand(n_fact, dtype=torch.float64, device="cuda", generator=g)
+        torch.cuda.synchronize()
+        wrap = lambda t, ty: nv.DeviceColumn.wrap(t.data_ptr(), t.numel(), ty, keep=t)
+        dkc, atc, fkc, vc = wrap(dk, nv.VEC_INT64), wrap(attr, nv.VEC_INT64), wrap(fk, nv.VEC_INT64), wrap(v, nv.VEC_DOUBLE)
+        best = None
+        for pct in (30, 50, 70):
+            lib.vgpu_tune_set(b"join.load_pct", pct)
+            times = []
+            for _ in range(3):
+                lib.vgpu_synchronize()
+                lib.vgpu_timer_start()
+                jt = nv.JoinTable(dkc, [atc])
+                gb = nv.GroupBy(nv.VEC_INT64, [nv.VEC_DOUBLE], [("sum", 0, 0), ("n", -1, 0)], nv.VGPU_ORDER_KEY, 257)
+                gb.feed_join(n_fact, [fkc, vc], [dkc, atc], [(0, 1)], key=(1, 1), join=jt,
+                             join_kind=nv.JOIN_INNER, probe_key_col=0)
+                gb.finish()
+                times.append(lib.vgpu_timer_stop())
+                gb.free()
+                jt.free()
+            ms = float(np.median(times))
+            log(f"  join build {n_dim:,} keys + probe {n_fact:,}: join.load_pct={pct} -> {ms:.3f} ms")
+            if best is None or ms < best[1]:
+                best = (pct, ms)
+        lib.vgpu_tune_reset()
+        out.append((bound, best[0], best[1]))
+        del dk, attr, fk, v
+        torch.cuda.empty_cache()
+    return out
 
 
 def tune_star(log) -> dict:
@@ -148,6 +204,8 @@ def main(argv=None) -> int:
         raise SystemExit("autotune: " + lib.vgpu_last_error().decode())
     lines = []
     t0 = time.time()
+    words = {sh: record_words(sh) for sh in SHAPES}
+    narrow_w = words["narrow"]
     for max_groups, card in BUCKETS:
         if card <= 256:
             grid = {"gb.lds.slots_per_group": [2, 4], "gb.lds.copies": [1, 2, 4],
@@ -157,17 +215,27 @@ def main(argv=None) -> int:
             grid = {"gb.part.min_groups": [card // 2, 1 << 40], "gb.lds.slots_per_group": [2, 4],
                     "gb.lds.max_blocks": [512, 1024, 2048], "gb.lds.rows_per_block": [8192, 32768]}
         else:
-            grid = {"gb.part.min_groups": [16384, 1 << 40], "gb.glb.rows_per_block": [512, 1024, 4096]}
-        r = tune_groupby(card, args.rows, grid, log)
-        log(f"bucket <= {max_groups:,} groups: best {r['knobs']} ({r['ms']})")
-        for name, value in r["knobs"].items():
-            t = ", ".join(f"{s} {v:.3f} ms" for s, v in r["ms"].items())
-            lines.append(f"{max_groups} {name} {value}   # {card:,} keys: {t}")
+            grid = {"gb.part.min_groups": [16384, 1 << 40], "gb.glb.rows_per_block": [512, 1024, 4096],
+                    "gb.glb.load_pct": [35, 50, 70]}
+        for bound, rows in ROWS:
+            if rows > args.rows:
+                rows = args.rows
+            best = tune_groupby(card, rows, grid, log)
+            for sh, (knobs, ms) in best.items():
+                tail = (f" w<={narrow_w}" if sh == "narrow" else "") + (f" rows<={bound}" if bound else "")
+                log(f"bucket <= {max_groups:,} groups, {sh} ({words[sh]} words), {rows:,} rows: {knobs} ({ms:.3f} ms)")
+                for name, value in knobs.items():
+                    lines.append(f"{max_groups} {name} {value}{tail}   # {card:,} keys, {sh}, {rows:,} rows: {ms:.3f} ms")
+    for bound, pct, ms in tune_join_load(log):
+        tail = f" rows<={bound}" if bound else ""
+        lines.append(f"{ANY} join.load_pct {pct}{tail}   # build + probe: {ms:.3f} ms")
     r = tune_star(log)
     lines.append(f"128 gb.star.items {r['knobs']['gb.star.items']}   # join -> group_by(attr): {r['ms']:.3f} ms")
-    head = [f"# group-by launch shapes per cardinality bucket, measured by vectra_amd/autotune.py",
+    head = [f"# launch shapes and table loads per (width, rows, cardinality) bucket, measured by "
+            f"vectra_amd/autotune.py",
             f"# command: python -m vectra_amd.autotune --out {Path(args.out).name} --rows {args.rows}",
-            f"# {time.strftime('%Y-%m-%d')}, {time.time() - t0:.0f} s of sweeps", "# max_groups name value"]
+            f"# {time.strftime('%Y-%m-%d')}, {time.time() - t0:.0f} s of sweeps; record words: narrow {narrow_w}, "
+            f"wide {words['wide']}", "# max_groups name value [w<=W] [rows<=R]"]
     Path(args.out).write_text("\n".join(head + lines) + "\n")
     log(f"wrote {args.out}")
     return 0

@@ -1237,6 +1237,7 @@ static int batch_plan_body(vgpu_batch *b, const vgpu_query *queries, int n_queri
         return -1;
     }
     const GbLayout &L = b->L;
+    VgTuneScope tune_scope(L.RH + L.RC, 0);
     hipStream_t st = vg_stream();
     std::vector<BtQuery> qs((size_t)n_queries);
     std::vector<vgpu_prog> progs((size_t)n_queries);

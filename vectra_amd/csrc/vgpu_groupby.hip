@@ -962,17 +962,24 @@ static int gb_sketch(vgpu_groupby *g, const vgpu_col *key, const int32_t *sel, i
     return 0;
 }
 
-// grows the table so that `incoming` more groups keep the load <= 1/2
+// hashed slots for `groups` groups at the tuned load of the HBM table
+// (knob gb.glb.load_pct, 25..90 percent; default 50)
+static int64_t gb_slots_for(int64_t groups) {
+    const int64_t pct = vg_tune_for(VG_TUNE_GB_GLB_LOAD_PCT, groups);
+    return pow2_at_least((groups * 100 + pct - 1) / pct);
+}
+
+// grows the table so that `incoming` more groups keep the tuned load
 static int gb_ensure_capacity(vgpu_groupby *g, int64_t incoming) {
     uint64_t ctl[GB_CTL_WORDS];
     if (g->cap == 0) {
-        int64_t want = pow2_at_least(2 * std::max<int64_t>(incoming, 32));
-        if (g->hint > 0) want = std::max(want, pow2_at_least(2 * g->hint));
+        int64_t want = gb_slots_for(std::max<int64_t>(incoming, 32));
+        if (g->hint > 0) want = std::max(want, gb_slots_for(g->hint));
         return gb_alloc_table(g, want);
     }
     // a table sized for the caller's group count is trusted until finish,
     // which reports an overflow; no per-feed synchronization
-    if (g->hint > 0 && g->cap >= pow2_at_least(2 * g->hint)) return 0;
+    if (g->hint > 0 && g->cap >= gb_slots_for(g->hint)) return 0;
     if (gb_read_ctl(g, ctl)) return -1;
     if (ctl[1]) {
         vg_set_error("groupby: hash table overflow");
@@ -981,7 +988,7 @@ static int gb_ensure_capacity(vgpu_groupby *g, int64_t incoming) {
     // new groups of this feed are bounded by its rows (or its sketch)
     int64_t occupied = (int64_t)ctl[0];
     if (g->hint <= 0) g->est_groups = std::max(g->est_groups, occupied);
-    int64_t need = pow2_at_least(2 * (occupied + incoming));
+    int64_t need = gb_slots_for(occupied + incoming);
     if (need <= g->cap) return 0;
     DevBuf nhot, ncold;
     int64_t slots = need + 2;
@@ -1982,6 +1989,7 @@ static int groupby_feed_body(vgpu_groupby *g, const vgpu_col *key, const vgpu_co
     }
     int64_t n_logical = sel ? n_sel : n_rows;
     if (n_logical <= 0) return 0;
+    VgTuneScope tune_scope(g->L.RH + g->L.RC, n_logical);
     if (!g->L.key_none && !key) {
         vg_set_error("groupby: key column missing");
         return -1;
@@ -2206,6 +2214,7 @@ static int groupby_feed_join_body(vgpu_groupby *g, const vgpu_feed_spec *sp) {
     if (g->run && gbp_flush(g)) return -1;
     int64_t n_logical = sp->sel ? sp->n_sel : sp->n_rows;
     if (n_logical <= 0) return 0;
+    VgTuneScope tune_scope(g->L.RH + g->L.RC, n_logical);
     if (sp->n_probe_cols < 0 || sp->n_probe_cols > VGPU_MAX_COLS || sp->n_build_cols < 0 ||
         sp->n_build_cols > VGPU_MAX_COLS || sp->n_inputs != g->L.n_inputs) {
         vg_set_error("groupby_feed_join: column / input counts out of range");
@@ -2601,6 +2610,8 @@ extern "C" int vgpu_groupby_result_device(vgpu_groupby *g, const void **keys, co
     }
     return 0;
 }
+
+extern "C" int vgpu_groupby_record_words(const vgpu_groupby *g) { return g ? g->L.RH + g->L.RC : -1; }
 
 extern "C" void vgpu_groupby_destroy(vgpu_groupby *g) {
     if (!g) return;

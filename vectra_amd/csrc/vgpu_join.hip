@@ -158,7 +158,9 @@ static vgpu_jtable *join_build_impl(const vgpu_col *key, int64_t n_build, const 
     t->E = n_payload == 0 ? 2 : (n_payload <= 2 ? 4 : 6);
     if (key->type != VGPU_BOOL) t->bkey = static_cast<const uint64_t *>(key->data);
     // slots: a power of two keeping the load at most join.load_pct percent
-    const int64_t load = std::min<int64_t>(95, std::max<int64_t>(10, vg_tune(VG_TUNE_JOIN_LOAD_PCT)));
+    // (tuned per build size and entry width)
+    VgTuneScope tune_scope(t->E, n_build);
+    const int64_t load = std::min<int64_t>(95, std::max<int64_t>(10, vg_tune_for(VG_TUNE_JOIN_LOAD_PCT, n_build)));
     int64_t cap = 64;
     while (cap * load < 100 * n_build) cap <<= 1;
     t->cap = cap;

@@ -377,29 +377,52 @@ static const char *g_tune_names[VG_TUNE_COUNT] = {"gb.lds.max_blocks", "gb.lds.r
                                                   "gb.lds.slots_per_group", "batch.bucketed",
                                                   "gb.star.dense", "join.bloom",
                                                   "gb.lds.copies", "gb.narrow", "batch.dense", "scan.device_feed", "join.bloom_keys", "join.load_pct", "gb.fuse_join",
-                                                  "strdict.hash_bits"};
-#define VG_TUNE_DEFAULTS {1024, 8192, 16384, 1024, 4096, 65536, 4, 65536, 1, 2, 2, 2, 2, 2, 2, 2, 2, 4, 50, 2, 64}
+                                                  "strdict.hash_bits", "gb.glb.load_pct"};
+#define VG_TUNE_DEFAULTS {1024, 8192, 16384, 1024, 4096, 65536, 4, 65536, 1, 2, 2, 2, 2, 2, 2, 2, 2, 4, 50, 2, 64, 50}
 static int64_t g_tune[VG_TUNE_COUNT] = VG_TUNE_DEFAULTS;
 
+// A bucket covers groups <= max_groups, record widths <= max_width (8-byte
+// words) and launches of rows <= max_rows; lookups take the first bucket
+// that covers the work and sets the knob, in ascending (max_width,
+// max_rows, max_groups) order: the narrowest width class first, then the
+// smallest row class, then the smallest cardinality class.
+#define TUNE_ANY INT64_MAX
 struct TuneBucket {
-    int64_t max_groups;
+    int64_t max_groups, max_width, max_rows;
     int64_t val[VG_TUNE_COUNT];
     uint32_t set;   // bit i: knob i overridden
 };
-static std::vector<TuneBucket> g_buckets;   // ascending max_groups
+static std::vector<TuneBucket> g_buckets;
 static std::mutex g_tune_mu;
+
+static bool bucket_before(const TuneBucket &a, const TuneBucket &b) {
+    if (a.max_width != b.max_width) return a.max_width < b.max_width;
+    if (a.max_rows != b.max_rows) return a.max_rows < b.max_rows;
+    return a.max_groups < b.max_groups;
+}
+
+static thread_local int64_t t_tune_width = 0, t_tune_rows = 0;
+
+VgTuneScope::VgTuneScope(int64_t width, int64_t rows) : saved_width(t_tune_width), saved_rows(t_tune_rows) {
+    t_tune_width = width;
+    t_tune_rows = rows;
+}
+VgTuneScope::~VgTuneScope() {
+    t_tune_width = saved_width;
+    t_tune_rows = saved_rows;
+}
 
 int64_t vg_tune(int knob) { return g_tune[knob]; }
 
-int64_t vg_tune_for(int knob, int64_t groups) {
+static int64_t tune_at(int knob, int64_t groups, int64_t width, int64_t rows) {
     if (groups > 0)
         for (const TuneBucket &b : g_buckets)
-            if (groups <= b.max_groups) {
-                if (b.set & (1u << knob)) return b.val[knob];
-                break;
-            }
+            if (groups <= b.max_groups && width <= b.max_width && rows <= b.max_rows && (b.set & (1u << knob)))
+                return b.val[knob];
     return g_tune[knob];
 }
+
+int64_t vg_tune_for(int knob, int64_t groups) { return tune_at(knob, groups, t_tune_width, t_tune_rows); }
 
 static int tune_index(const char *name) {
     for (int i = 0; i < VG_TUNE_COUNT; i++)
@@ -424,6 +447,11 @@ static int tune_check(int i, const char *name, int64_t value) {
         vg_set_error("vgpu_tune_set: %s is between 2 and 8", name);
         return -1;
     }
+    if ((i == VG_TUNE_GB_GLB_LOAD_PCT && (value < 25 || value > 90)) ||
+        (i == VG_TUNE_JOIN_LOAD_PCT && (value < 10 || value > 95))) {
+        vg_set_error("vgpu_tune_set: %s is out of range", name);
+        return -1;
+    }
     return 0;
 }
 
@@ -434,24 +462,27 @@ extern "C" int vgpu_tune_set(const char *name, int64_t value) {
     return 0;
 }
 
-extern "C" int vgpu_tune_set_for(int64_t max_groups, const char *name, int64_t value) {
+extern "C" int vgpu_tune_set_at(int64_t max_groups, int64_t max_width, int64_t max_rows, const char *name,
+                                int64_t value) {
     int i = tune_index(name);
     if (tune_check(i, name, value)) return -1;
-    if (max_groups <= 0) {
-        vg_set_error("vgpu_tune_set_for: max_groups must be positive");
+    if (max_groups <= 0 || max_width <= 0 || max_rows <= 0) {
+        vg_set_error("vgpu_tune_set_at: bucket bounds must be positive");
         return -1;
     }
     std::lock_guard<std::mutex> lk(g_tune_mu);
+    TuneBucket key;
+    memset(&key, 0, sizeof key);
+    key.max_groups = max_groups;
+    key.max_width = max_width;
+    key.max_rows = max_rows;
     auto it = g_buckets.begin();
-    while (it != g_buckets.end() && it->max_groups < max_groups) ++it;
-    if (it == g_buckets.end() || it->max_groups != max_groups) {
-        TuneBucket b;
-        memset(&b, 0, sizeof b);
-        b.max_groups = max_groups;
+    while (it != g_buckets.end() && bucket_before(*it, key)) ++it;
+    if (it == g_buckets.end() || bucket_before(key, *it)) {
         try {
-            it = g_buckets.insert(it, b);
+            it = g_buckets.insert(it, key);
         } catch (const std::bad_alloc &) {
-            vg_set_error("vgpu_tune_set_for: host memory exhausted");
+            vg_set_error("vgpu_tune_set_at: host memory exhausted");
             return -1;
         }
     }
@@ -460,9 +491,19 @@ extern "C" int vgpu_tune_set_for(int64_t max_groups, const char *name, int64_t v
     return 0;
 }
 
-extern "C" int64_t vgpu_tune_get_for(int64_t groups, const char *name) {
+extern "C" int vgpu_tune_set_for(int64_t max_groups, const char *name, int64_t value) {
+    return vgpu_tune_set_at(max_groups, TUNE_ANY, TUNE_ANY, name, value);
+}
+
+extern "C" int64_t vgpu_tune_get_at(int64_t groups, int64_t width, int64_t rows, const char *name) {
     int i = tune_index(name);
-    return i < 0 ? -1 : vg_tune_for(i, groups);
+    if (i < 0) return -1;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    return tune_at(i, groups, width, rows);
+}
+
+extern "C" int64_t vgpu_tune_get_for(int64_t groups, const char *name) {
+    return vgpu_tune_get_at(groups, 0, 0, name);
 }
 
 extern "C" void vgpu_tune_reset(void) {
@@ -472,7 +513,8 @@ extern "C" void vgpu_tune_reset(void) {
     memcpy(g_tune, defaults, sizeof g_tune);
 }
 
-// "max_groups name value" per line (max_groups 0 = the global value);
+// "max_groups name value [w<=W] [rows<=R]" per line (max_groups 0 = the
+// global value; a bucket without w<= / rows<= covers any width / rows);
 // '#' starts a comment
 extern "C" int vgpu_tune_load(const char *path) {
     FILE *f = fopen(path, "r");
@@ -487,14 +529,26 @@ extern "C" int vgpu_tune_load(const char *path) {
         if (h) *h = 0;
         long long mg, v;
         char name[64];
-        int k = sscanf(line, "%lld %63s %lld", &mg, name, &v);
+        int used = 0;
+        int k = sscanf(line, "%lld %63s %lld%n", &mg, name, &v, &used);
         if (k <= 0) continue;
         if (k != 3) {
             vg_set_error("vgpu_tune_load: %s: malformed line", path);
             rc = -1;
             break;
         }
-        rc = mg > 0 ? vgpu_tune_set_for(mg, name, v) : vgpu_tune_set(name, v);
+        long long mw = TUNE_ANY, mr = TUNE_ANY;
+        const char *rest = line + used;
+        char tok[64];
+        int adv = 0;
+        while (rc == 0 && sscanf(rest, "%63s%n", tok, &adv) == 1) {
+            rest += adv;
+            if (sscanf(tok, "w<=%lld", &mw) == 1 || sscanf(tok, "rows<=%lld", &mr) == 1) continue;
+            vg_set_error("vgpu_tune_load: %s: unknown bucket bound %s", path, tok);
+            rc = -1;
+        }
+        if (rc) break;
+        rc = mg > 0 ? vgpu_tune_set_at(mg, mw, mr, name, v) : vgpu_tune_set(name, v);
         n++;
     }
     fclose(f);

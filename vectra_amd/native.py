@@ -255,6 +255,7 @@ def vgpu() -> C.CDLL:
         _proto(lib, "vgpu_groupby_fetch", C.c_int64, [C.c_void_p, C.c_void_p, C.c_void_p,
                                                       C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)])
         _proto(lib, "vgpu_groupby_destroy", None, [C.c_void_p])
+        _proto(lib, "vgpu_groupby_record_words", C.c_int, [C.c_void_p])
         _proto(lib, "vgpu_join_build", C.c_void_p, [C.POINTER(VgpuCol), C.c_int64])
         _proto(lib, "vgpu_join_build_payload", C.c_void_p, [C.POINTER(VgpuCol), C.c_int64, C.POINTER(VgpuCol),
                                                             C.c_int])
@@ -279,6 +280,8 @@ def vgpu() -> C.CDLL:
         _proto(lib, "vgpu_tune_load", C.c_int, [C.c_char_p])
         _proto(lib, "vgpu_tune_reset", None, [])
         _proto(lib, "vgpu_tune_get_for", C.c_int64, [C.c_int64, C.c_char_p])
+        _proto(lib, "vgpu_tune_set_at", C.c_int, [C.c_int64, C.c_int64, C.c_int64, C.c_char_p, C.c_int64])
+        _proto(lib, "vgpu_tune_get_at", C.c_int64, [C.c_int64, C.c_int64, C.c_int64, C.c_char_p])
         _proto(lib, "vgpu_kernel_timing", C.c_int, [C.c_int])
         _proto(lib, "vgpu_kernel_time_ms", C.c_double, [C.POINTER(C.c_int64), C.c_int])
         _proto(lib, "vgpu_batch_create", C.c_void_p, [C.c_int, C.c_int, C.POINTER(C.c_int32), C.c_int,
