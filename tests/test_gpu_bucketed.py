@@ -219,12 +219,20 @@ def join_tables(seed, n_fact, n_dim, dup):
     return tbl, {"dk": dk, "attr": (attr, with_na(r, n_dim, 0.02))}
 
 
+@pytest.mark.parametrize("dense", [True, False])
 @pytest.mark.parametrize("dup", [False, True])
-def test_bucketed_join(gpu, oracle, dup):
+def test_bucketed_join(gpu, oracle, dup, dense):
     """filter(p > t_q) |> inner_join(dim, fk = dk) |> group_by(attr): one
-    bucketed pass probes the resident table once per selected row"""
+    bucketed pass probes the resident table once per selected row, through
+    the direct-addressed map of unique keys (dense; keys 3 apart) or the
+    hash table (duplicate keys, or the map switched off); probe keys at the
+    range's ends, just past it, far past it and below it"""
     tbl, dim = join_tables(97, 300_000, 30_000, dup)
-    dkc = gpu.DeviceColumn(dim["dk"])
+    fk = tbl["fk"][0].copy()
+    dk = dim["dk"]
+    fk[::997], fk[1::997], fk[2::997], fk[3::997] = dk.max() + 1, dk.min(), dk.max(), 2**62
+    tbl["fk"] = (fk, tbl["fk"][1])
+    dkc = gpu.DeviceColumn(dk)
     atc = gpu.DeviceColumn(*dim["attr"])
     jt = gpu.JoinTable(dkc, [atc])
     fkc = gpu.DeviceColumn(*tbl["fk"])
@@ -233,7 +241,14 @@ def test_bucketed_join(gpu, oracle, dup):
     pred = gpu.cmp_const_prog(0, gpu.VEC_DOUBLE, "GT", 0.0)
     b = gpu.Batch(gpu.VEC_INT64, [gpu.VEC_DOUBLE, gpu.VEC_INT64], BATCH_AGGS, gpu.VGPU_ORDER_KEY, pred, 1, 1)
     b.join(jt, 0)
-    b.plan([{"probe_key": fkc, "inputs": [xc, vc], "pred_cols": [pc], "n_rows": 300_000, "param": t} for t in ts])
+    lib = gpu.vgpu()
+    old = lib.vgpu_tune_get(b"join.dense_map")
+    lib.vgpu_tune_set(b"join.dense_map", 2 if dense else 1)   # read when the first plan builds the map
+    try:
+        b.plan([{"probe_key": fkc, "inputs": [xc, vc], "pred_cols": [pc], "n_rows": 300_000, "param": t}
+                for t in ts])
+    finally:
+        lib.vgpu_tune_set(b"join.dense_map", old)
     assert b.bucketed()
     check_all(gpu, b, tbl, ">", ts, key="attr", join_dim=dim)
     b.rerun([0.2, -0.4], n_active=2)

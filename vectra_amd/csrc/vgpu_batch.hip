@@ -143,6 +143,8 @@ struct vgpu_batch {
     int32_t join_key_type = 0;
     vgpu_col join_payload;
     int64_t join_rows = 0;
+    vgpu_jtable *jtab = nullptr;      // its direct-addressed map is built on the first bucketed plan
+    JDense jd = {nullptr, 0, 0, 0};   // the map a bucketed pass probes instead of the hash table
     // shared scans
     int32_t shared = 0;
     int32_t n_active = 0;            // queries the shared feed runs (rerun may lower it)
@@ -678,6 +680,7 @@ struct BtChainArgs {
     int32_t f64_mask;      // bit s: sum s adds doubles (else int64 / bool bits, wrapping)
     int32_t sum_in[BT_CH_MAX_SUMS];
     int32_t sum_pred[BT_CH_MAX_SUMS];   // sum s reads the compared column itself
+    JDense jd;             // join batches: direct-addressed map of the group key payload (map null: hash probe)
 };
 
 // the count of the active queries a valid compared value passes
@@ -839,7 +842,22 @@ __device__ __forceinline__ void bt_chain_rows_add(const BtChainArgs &A, const vg
 #pragma unroll
         for (int s = 0; s < BT_CH_MAX_SUMS; s++) sv[u][s] = SP ? R.pv[u] : R.sv[SP ? 0 : u][s];
     }
-    if (A.jn.on) {
+    if (A.jn.on && A.jd.map) {
+        // unique int64 build keys over a short range: one 2-byte load per
+        // row; NA probe keys never match (src/join.c:85)
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (!T[u] || !((R.kv >> u) & 1)) continue;
+            const uint64_t off = R.kb[u] - (uint64_t)A.jd.kmin;
+            if (off >= A.jd.range) continue;
+            const uint32_t c = A.jd.map[off];
+            if (c == JD_ABSENT) continue;
+            const int64_t e = c == JD_NA_PAYLOAD ? A.E - 1 : bt_chain_entry(A, true, (uint64_t)(A.jd.pmin + (int64_t)c));
+            if (e < 0) err = true;
+            else if (TM == BT_TM_SEARCH) bt_chain_add(A, cnt, sums, e, T[u], sv[u]);
+            else bt_chain_add_one<TM == 1 || TM == 2>(A, cnt, sums, e, T[u], sv[u][0]);
+        }
+    } else if (A.jn.on) {
         const uint64_t *ent[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
@@ -1080,6 +1098,8 @@ static int batch_join_body(vgpu_batch *b, const vgpu_jtable *t, int key_payload)
     b->join_key_type = t->key_type;
     b->join_payload = t->payload[key_payload];
     b->join_rows = t->n_build;
+    // the table's lazily built direct-addressed map is internal state of it
+    b->jtab = const_cast<vgpu_jtable *>(t);
     return 0;
 }
 
@@ -1187,6 +1207,10 @@ static int bt_chain_plan(vgpu_batch *b, const BtQuery &Q0) {
     b->ch_rpb = rpb;
     b->ch_lds = lds;
     b->chain = 1;
+    b->jd = {nullptr, 0, 0, 0};
+    if (b->join.on && b->jtab && Q0.probe_key.type == VGPU_INT64 &&
+        vg_jtable_dense(b->jtab, b->join.key_payload, &b->jd))
+        return -1;
     return 0;
 }
 
@@ -1610,6 +1634,7 @@ static int bt_enqueue(vgpu_batch *b, bool first) {
         C.key_type = b->ch_key_type;
         C.G = b->ch_G;
         C.f64_mask = b->ch_f64;
+        C.jd = b->jd;
         for (int s2 = 0; s2 < BT_CH_MAX_SUMS; s2++) {
             C.sum_in[s2] = b->ch_sum_in[s2];
             C.sum_pred[s2] = b->ch_sum_pred[s2];

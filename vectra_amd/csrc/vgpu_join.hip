@@ -523,10 +523,89 @@ extern "C" int vgpu_join_unmatched_build(vgpu_jtable *t, int64_t *n_out) {
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// direct-addressed map of a unique-key table
+// ---------------------------------------------------------------------------
+
+// acc[0] = min, acc[1] = max of the regular keys (sign-flipped order);
+// acc[2] != 0 when a key repeats or the sentinel slot holds rows
+__global__ void k_jd_range(const uint64_t *ent, int64_t cap, int E, unsigned long long *acc) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s <= cap; s += stride) {
+        const uint64_t *e = ent + (size_t)s * E;
+        const uint32_t n = (uint32_t)(e[1] & 0xFFFFFFu);
+        if (s == cap) {
+            if (n) atomicOr(&acc[2], 1ull);
+            continue;
+        }
+        if (e[0] == VG_EMPTY_I64) continue;
+        if (n != 1) atomicOr(&acc[2], 1ull);
+        const unsigned long long o = e[0] ^ 0x8000000000000000ull;
+        atomicMin(&acc[0], o);
+        atomicMax(&acc[1], o);
+    }
+}
+
+__global__ void k_jd_fill(const uint64_t *ent, int64_t cap, int E, int p, int64_t kmin, int64_t pmin, uint16_t *map) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < cap; s += stride) {
+        const uint64_t *e = ent + (size_t)s * E;
+        if (e[0] == VG_EMPTY_I64) continue;
+        const bool valid = (e[1] >> (24 + p)) & 1;
+        map[e[0] - (uint64_t)kmin] = valid ? (uint16_t)((int64_t)e[2 + p] - pmin) : (uint16_t)JD_NA_PAYLOAD;
+    }
+}
+
+static int jtable_dense_build(vgpu_jtable *t, int p) {
+    if (vg_tune(VG_TUNE_JOIN_DENSE) < 2 || t->key_type != VGPU_INT64 || t->max_dup > 1 || p < 0 ||
+        p >= t->n_payload || (t->payload[p].type != VGPU_INT64 && t->payload[p].type != VGPU_BOOL) ||
+        !t->pay_ranged[p] || (uint64_t)t->pay_hi[p] - (uint64_t)t->pay_lo[p] >= 0xFFFDull || t->n_valid <= 0)
+        return 0;
+    hipStream_t s = vg_stream();
+    DevBuf acc;
+    if (acc.reserve(24)) return -1;
+    const unsigned long long init[3] = {~0ull, 0ull, 0ull};
+    VG_TRY(hipMemcpyAsync(acc.ptr, init, 24, hipMemcpyHostToDevice, s));
+    k_jd_range<<<vg_blocks(t->cap + 1, 256, 8192), 256, 0, s>>>(t->ent.as<uint64_t>(), t->cap, t->E,
+                                                               acc.as<unsigned long long>());
+    VG_LAUNCH_CHECK();
+    unsigned long long h[3];
+    VG_TRY(hipMemcpyAsync(h, acc.ptr, 24, hipMemcpyDeviceToHost, s));
+    VG_TRY(hipStreamSynchronize(s));
+    acc.release();
+    if (h[2] || h[0] > h[1]) return 0;
+    const uint64_t range = h[1] - h[0] + 1;
+    if (range > (uint64_t)1 << 30 || range > (uint64_t)16 * (uint64_t)t->n_valid) return 0;
+    if (t->dmap.reserve((size_t)range * 2)) return -1;
+    const int64_t kmin = (int64_t)(h[0] ^ 0x8000000000000000ull);
+    VG_TRY(hipMemsetAsync(t->dmap.ptr, 0xFF, (size_t)range * 2, s));
+    k_jd_fill<<<vg_blocks(t->cap, 256, 8192), 256, 0, s>>>(t->ent.as<uint64_t>(), t->cap, t->E, p, kmin,
+                                                          t->pay_lo[p], t->dmap.as<uint16_t>());
+    VG_LAUNCH_CHECK();
+    VG_TRY(hipStreamSynchronize(s));
+    t->dm = {t->dmap.as<uint16_t>(), kmin, range, t->pay_lo[p]};
+    return 1;
+}
+
+int vg_jtable_dense(vgpu_jtable *t, int p, JDense *out) {
+    *out = {nullptr, 0, 0, 0};
+    std::lock_guard<std::mutex> lk(t->dm_mu);
+    // one map per table, for the first payload column asked for: a map in
+    // use by other passes is never replaced
+    if (t->dm_state == 0) {
+        const int rc = jtable_dense_build(t, p);
+        if (rc < 0) return -1;
+        t->dm_state = rc ? 1 : -1;
+        t->dm_payload = p;
+    }
+    if (t->dm_state == 1 && t->dm_payload == p) *out = t->dm;
+    return 0;
+}
+
 extern "C" void vgpu_join_destroy(vgpu_jtable *t) {
     if (!t) return;
-    DevBuf *bufs[] = {&t->ent, &t->rows, &t->cnt, &t->un, &t->off_main, &t->off_un,
-                      &t->out_probe, &t->out_build, &t->totals, &t->scan_tmp, &t->sort_tmp, &t->bmatched};
+    DevBuf *bufs[] = {&t->ent, &t->rows, &t->cnt, &t->un, &t->off_main, &t->off_un, &t->out_probe,
+                      &t->out_build, &t->totals, &t->scan_tmp, &t->sort_tmp, &t->bmatched, &t->bloom, &t->dmap};
     for (DevBuf *b : bufs) b->release();
     for (int p = 0; p < JT_MAX_PAYLOAD; p++) { t->cpay[p].release(); t->cval[p].release(); }
     delete t;

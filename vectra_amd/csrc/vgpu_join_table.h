@@ -16,9 +16,26 @@
 //                  cval[p][i] its validity byte.
 #pragma once
 
+#include <mutex>
+
 #include "vgpu_keys.h"
 
 #define JT_MAX_PAYLOAD 4
+
+// Direct-addressed view of a table whose int64 keys are unique and span a
+// short range: map[key - kmin] (key - kmin < range) holds the key's
+// payload column value minus pmin, JD_NA_PAYLOAD for an NA payload and
+// JD_ABSENT where no build key lies.  A probe reads two bytes instead of a
+// hash slot's line; for a range of up to 16 keys per slot the map takes at
+// most 32 bytes a key (2 a slot).
+#define JD_ABSENT 0xFFFFu
+#define JD_NA_PAYLOAD 0xFFFEu
+struct JDense {
+    const uint16_t *map;   // nullptr: not built / not applicable
+    int64_t kmin;
+    uint64_t range;
+    int64_t pmin;
+};
 
 struct JView {
     const uint64_t *ent;
@@ -68,6 +85,12 @@ struct vgpu_jtable {
     DevBuf cnt, un, off_main, off_un, out_probe, out_build, totals, scan_tmp, sort_tmp;
     DevBuf bmatched;       // FULL join: one byte per build row
     int64_t last_n_out = 0;
+    // direct-addressed map of one payload column (built on first request)
+    DevBuf dmap;
+    int32_t dm_state = 0;       // 0 not tried, 1 built, -1 not applicable
+    int32_t dm_payload = -1;
+    JDense dm = {nullptr, 0, 0, 0};
+    std::mutex dm_mu;
     JView view() const {
         JView v;
         memset(&v, 0, sizeof v);
@@ -125,3 +148,12 @@ __device__ __forceinline__ uint32_t jt_lookup(const JView &v, const vgpu_col &kc
     off = (uint32_t)(sg >> 32);
     return jt_count(v, sg);
 }
+
+// The direct-addressed map of payload column p (built on the first request,
+// under the table's lock, on the caller's stream, which is synchronized
+// before the map is shared; one map per table, for the first payload column
+// asked for); out->map == nullptr when the table does not take one: keys not int64, a repeated key, an int64 key equal to the
+// empty sentinel, a key range above 16 per key or 2^30, a payload not
+// int64 / bool or spanning more than 65,533 values, or knob join.dense_map
+// off.  Returns -1 with the error set on a device failure.
+int vg_jtable_dense(vgpu_jtable *t, int p, JDense *out);
