@@ -980,7 +980,7 @@ def main():
     ap.add_argument("--c5-slo-ms", type=float, default=100.0)
     ap.add_argument("--c5-plain-rates", type=float, nargs="*", default=[10, 25, 40, 55, 70, 85, 100])
     ap.add_argument("--c5-batched-rates", type=float, nargs="*",
-                    default=[50, 100, 200, 400, 600, 800, 1000, 1200, 1400, 1600, 1800])
+                    default=[50, 100, 200, 400, 800, 1200, 1600, 1800, 2000, 2200, 2400, 2600])
     ap.add_argument("--c3-rows", type=int, default=200_000_000)
     ap.add_argument("--vtr-rows", type=int, default=50_000_000)
     ap.add_argument("--str-rows", type=int, default=10_000_000)
@@ -992,6 +992,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tune", action="append", default=[], metavar="NAME=VALUE",
                     help="set a launch knob (vgpu_tune_set) before the run, e.g. batch.dense=1")
+    ap.add_argument("--tune-reset", action="store_true",
+                    help="drop the tuning table's buckets first (so --tune values apply everywhere)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="copy 1 GiB three times (reference kernel for rocprofv3 FETCH_SIZE / WRITE_SIZE scaling)")
     args = ap.parse_args()
@@ -1013,6 +1015,9 @@ def main():
     lib = nv.vgpu()
     if lib.vgpu_init(0) != 0:
         raise SystemExit("bench: " + lib.vgpu_last_error().decode())
+    if args.tune_reset:
+        lib.vgpu_tune_reset()
+        log("[bench] tuning table dropped: built-in defaults")
     for kv in args.tune:
         name, _, val = kv.partition("=")
         if lib.vgpu_tune_set(name.encode(), int(val)) != 0:

@@ -535,6 +535,10 @@ void    vgpu_batch_destroy(vgpu_batch *b);
  *                         (bucket: build rows, entry words)
  *   "gb.glb.load_pct"    (50) highest load of a group-by's HBM table, in
  *                         percent (25..90), read when the table is sized
+ *   "join.dense_map"     (2: on; 1: off) a bucketed join pass over unique
+ *                         int64 build keys of a short range probes a
+ *                         direct-addressed map (built once per table)
+ *   "gb.star.blocks_per_cu" (4) blocks per CU of the dense star kernel
  *   "gb.fuse_join"       (2) a GPU GroupAggNode over Project(a * b) over a
  *                         GPU JoinNode runs the three as one fused device
  *                         pass (2), or node by node (1)

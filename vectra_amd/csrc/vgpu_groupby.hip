@@ -705,112 +705,170 @@ __device__ __forceinline__ void star_dense_update(const StarDenseArgs &A, uint64
     }
 }
 
-// IN_MODE: 0 = probe column, 1 = payload column, 2 = their product
+// One step's probes and updates: every row's filter word and first entry
+// load, then the chain walks (rare at load <= 1/2) and the record updates.
+// Every row's entry load is issued -- a row whose key is absent or filtered
+// out reads slot 0, a line already in cache -- so no load sits in a branch
+// whose result the compiler must wait for before issuing the next.
 template <int ITEMS, int IN_MODE, bool LEFT>
+__device__ __forceinline__ void star_dense_step(const GbLayout &L, const StarDenseArgs &A, uint64_t *lhot,
+                                                uint64_t *lcold, int64_t base, int tid, const bool (&live)[ITEMS],
+                                                const int (&cls)[ITEMS], const uint64_t (&kn)[ITEMS],
+                                                const bool (&pv_ok)[ITEMS], const uint64_t (&pv)[ITEMS]) {
+    const uint64_t empty_j = A.jkey_double ? VG_EMPTY_F64 : VG_EMPTY_I64;
+    bool found[ITEMS];
+    uint64_t slot[ITEMS], bw[ITEMS], bb[ITEMS];
+    uint64_t e0[ITEMS], e1[ITEMS], e2[ITEMS], e3[ITEMS];
+#pragma unroll
+    for (int i = 0; i < ITEMS; i++) {
+        found[i] = cls[i] == VG_KEY_REGULAR || cls[i] == VG_KEY_SPECIAL;
+        const uint64_t kh = vg_key_hash(kn[i]);
+        slot[i] = cls[i] == VG_KEY_SPECIAL ? A.jmask + 1 : (kh & A.jmask);
+        // the filter word now; a regular key whose bits are missing is
+        // absent from the table and reads no entry line of its own
+        bw[i] = ~0ull;
+        bb[i] = 0;
+        if (A.bloom) {
+            const uint64_t w = A.bloom[jt_bloom_word(kh, A.bloom_mask)];
+            if (cls[i] == VG_KEY_REGULAR) {
+                bw[i] = w;
+                bb[i] = jt_bloom_bits(kh);
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < ITEMS; i++) {
+        found[i] = found[i] && (bw[i] & bb[i]) == bb[i];
+        const uint64_t *e = A.ent + (found[i] ? slot[i] : 0) * 4;
+        e0[i] = e[0];
+        e1[i] = e[1];
+        e2[i] = e[2];
+        e3[i] = e[3];
+    }
+#pragma unroll
+    for (int i = 0; i < ITEMS; i++) {
+        if (!live[i]) continue;
+        bool matched = false;
+        if (found[i]) {
+            if (slot[i] <= A.jmask) {
+                uint64_t probes = 0;
+                while (e0[i] != kn[i] && e0[i] != empty_j && probes <= A.jmask) {
+                    slot[i] = (slot[i] + 1) & A.jmask;
+                    const uint64_t *e = A.ent + slot[i] * 4;
+                    e0[i] = e[0];
+                    e1[i] = e[1];
+                    e2[i] = e[2];
+                    e3[i] = e[3];
+                    probes++;
+                }
+                matched = e0[i] == kn[i] && (e1[i] & 0xFFFFFFu) != 0;
+            } else {
+                matched = (e1[i] & 0xFFFFFFu) != 0;
+            }
+        }
+        if (!LEFT && !matched) continue;
+        const int64_t li = base + (int64_t)i * GB_THREADS + tid;
+        // group record
+        const bool kvalid = matched && ((e1[i] >> (24 + A.kslot)) & 1);
+        uint64_t gk;
+        const int gcls = vg_key_class_bits(A.ktype, kvalid, A.kslot ? e3[i] : e2[i], gk);
+        const uint64_t pos = jf_pos((uint64_t)A.pos_base, li, 0, !matched);
+        // input
+        InVal v;
+        if (IN_MODE == 0) {
+            v.valid = pv_ok[i];
+            v.bits = pv[i];
+        } else {
+            InVal b;
+            b.valid = matched && ((e1[i] >> (24 + A.wslot)) & 1);
+            b.bits = A.wslot ? e3[i] : e2[i];
+            if (IN_MODE == 1) {
+                v = b;
+            } else {
+                InVal a;
+                a.valid = pv_ok[i];
+                a.bits = pv[i];
+                v = jf_mul(a, A.ta, b, A.tb);
+            }
+        }
+        const int64_t ls = gb_lds_dense_slot(A.S, gcls, gk, A.kmin);
+        // one expansion per address space, so the LDS one compiles to
+        // ds_* atomics rather than flat ones
+        if (ls >= 0) {
+            star_dense_update(A, lhot + (size_t)ls * A.RH, lcold + (size_t)ls * A.RC, pos, v);
+        } else {
+            const int64_t gs = gb_global_slot(L, A.hot, A.cap, gcls, gk, pos, A.ctl);
+            if (gs >= 0) star_dense_update(A, A.hot + (size_t)gs * A.RH, A.cold + (size_t)gs * A.RC, pos, v);
+        }
+    }
+}
+
+// IN_MODE: 0 = probe column, 1 = payload column, 2 = their product.
+// PLAIN: the probe key (and the probe operand) are 8-byte columns without
+// validity: rows past the end read the last row and are masked out, and
+// the next step's keys and operands are loaded while this step probes the
+// table (a step's probe chain -- key, hash, filter word, entry -- is
+// dependent; the streamed loads of the next step are not).
+template <int ITEMS, int IN_MODE, bool LEFT, bool PLAIN>
 __global__ void __launch_bounds__(GB_THREADS) k_gb_star_dense(GbLayout L, StarDenseArgs A) {
     extern __shared__ uint64_t dyn64[];
     const int tid = threadIdx.x;
     uint64_t *lhot = dyn64;
     uint64_t *lcold = lhot + (size_t)(A.S + 2) * A.RH;
-    const uint64_t empty_j = A.jkey_double ? VG_EMPTY_F64 : VG_EMPTY_I64;
     gb_lds_init(L, lhot, lcold, A.S, VG_EMPTY_I64, true, A.kmin);
     const int64_t step = (int64_t)gridDim.x * GB_THREADS * ITEMS;
-    for (int64_t base = (int64_t)blockIdx.x * GB_THREADS * ITEMS; base < A.n_logical; base += step) {
-        bool live[ITEMS], found[ITEMS], pv_ok[ITEMS];
-        uint64_t kn[ITEMS], pv[ITEMS], slot[ITEMS], bw[ITEMS], bb[ITEMS];
-        uint64_t e0[ITEMS], e1[ITEMS], e2[ITEMS], e3[ITEMS];
-        // every row's probe-side loads, then every first entry load
+    const int64_t base0 = (int64_t)blockIdx.x * GB_THREADS * ITEMS;
+    if (PLAIN) {
+        const uint64_t *kd = static_cast<const uint64_t *>(A.pkey.data);
+        const uint64_t *vd = static_cast<const uint64_t *>(A.pval.data);
+        uint64_t ck[ITEMS], cv[ITEMS], nk[ITEMS], nv[ITEMS];
+        auto load_step = [&](int64_t b, uint64_t (&k)[ITEMS], uint64_t (&v)[ITEMS]) {
 #pragma unroll
-        for (int i = 0; i < ITEMS; i++) {
-            const int64_t li = base + (int64_t)i * GB_THREADS + tid;
-            live[i] = li < A.n_logical;
-            const bool kv = live[i] && vg_valid(A.pkey.validity, li);
-            const int cls = vg_key_class_bits(A.pkey.type, kv, kv ? vg_load_bits_stream(A.pkey, li) : 0, kn[i]);
-            found[i] = cls == VG_KEY_REGULAR || cls == VG_KEY_SPECIAL;
-            const uint64_t kh = vg_key_hash(kn[i]);
-            slot[i] = cls == VG_KEY_SPECIAL ? A.jmask + 1 : (kh & A.jmask);
-            e0[i] = e1[i] = e2[i] = e3[i] = 0;
-            // the filter word now; a regular key whose bits are missing is
-            // absent from the table and loads no entry
-            bw[i] = ~0ull;
-            bb[i] = 0;
-            if (A.bloom && cls == VG_KEY_REGULAR) {
-                bw[i] = A.bloom[jt_bloom_word(kh, A.bloom_mask)];
-                bb[i] = jt_bloom_bits(kh);
+            for (int i = 0; i < ITEMS; i++) {
+                const int64_t li = b + (int64_t)i * GB_THREADS + tid;
+                const int64_t lc = li < A.n_logical ? li : A.n_logical - 1;
+                k[i] = __builtin_nontemporal_load(kd + lc);
+                v[i] = IN_MODE != 1 ? __builtin_nontemporal_load(vd + lc) : 0;
             }
-            pv_ok[i] = false;
-            pv[i] = 0;
-            if (IN_MODE != 1) {
-                pv_ok[i] = live[i] && vg_valid(A.pval.validity, li);
-                pv[i] = pv_ok[i] ? vg_load_bits_stream(A.pval, li) : 0;
+        };
+        if (base0 < A.n_logical) load_step(base0, ck, cv);
+        for (int64_t base = base0; base < A.n_logical; base += step) {
+            if (base + step < A.n_logical) load_step(base + step, nk, nv);
+            bool live[ITEMS], pv_ok[ITEMS];
+            int cls[ITEMS];
+            uint64_t kn[ITEMS];
+#pragma unroll
+            for (int i = 0; i < ITEMS; i++) {
+                live[i] = base + (int64_t)i * GB_THREADS + tid < A.n_logical;
+                cls[i] = vg_key_class_bits(A.pkey.type, live[i], ck[i], kn[i]);
+                pv_ok[i] = live[i];
+            }
+            star_dense_step<ITEMS, IN_MODE, LEFT>(L, A, lhot, lcold, base, tid, live, cls, kn, pv_ok, cv);
+#pragma unroll
+            for (int i = 0; i < ITEMS; i++) {
+                ck[i] = nk[i];
+                cv[i] = nv[i];
             }
         }
+    } else {
+        for (int64_t base = base0; base < A.n_logical; base += step) {
+            bool live[ITEMS], pv_ok[ITEMS];
+            int cls[ITEMS];
+            uint64_t kn[ITEMS], pv[ITEMS];
 #pragma unroll
-        for (int i = 0; i < ITEMS; i++) {
-            found[i] = found[i] && (bw[i] & bb[i]) == bb[i];
-            if (found[i]) {
-                const uint64_t *e = A.ent + slot[i] * 4;
-                e0[i] = e[0];
-                e1[i] = e[1];
-                e2[i] = e[2];
-                e3[i] = e[3];
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < ITEMS; i++) {
-            if (!live[i]) continue;
-            bool matched = false;
-            if (found[i]) {
-                if (slot[i] <= A.jmask) {
-                    uint64_t probes = 0;
-                    while (e0[i] != kn[i] && e0[i] != empty_j && probes <= A.jmask) {
-                        slot[i] = (slot[i] + 1) & A.jmask;
-                        const uint64_t *e = A.ent + slot[i] * 4;
-                        e0[i] = e[0];
-                        e1[i] = e[1];
-                        e2[i] = e[2];
-                        e3[i] = e[3];
-                        probes++;
-                    }
-                    matched = e0[i] == kn[i] && (e1[i] & 0xFFFFFFu) != 0;
-                } else {
-                    matched = (e1[i] & 0xFFFFFFu) != 0;
+            for (int i = 0; i < ITEMS; i++) {
+                const int64_t li = base + (int64_t)i * GB_THREADS + tid;
+                live[i] = li < A.n_logical;
+                const bool kv = live[i] && vg_valid(A.pkey.validity, li);
+                cls[i] = vg_key_class_bits(A.pkey.type, kv, kv ? vg_load_bits_stream(A.pkey, li) : 0, kn[i]);
+                pv_ok[i] = false;
+                pv[i] = 0;
+                if (IN_MODE != 1) {
+                    pv_ok[i] = live[i] && vg_valid(A.pval.validity, li);
+                    pv[i] = pv_ok[i] ? vg_load_bits_stream(A.pval, li) : 0;
                 }
             }
-            if (!LEFT && !matched) continue;
-            const int64_t li = base + (int64_t)i * GB_THREADS + tid;
-            // group record
-            const bool kvalid = matched && ((e1[i] >> (24 + A.kslot)) & 1);
-            uint64_t gk;
-            const int gcls = vg_key_class_bits(A.ktype, kvalid, A.kslot ? e3[i] : e2[i], gk);
-            const uint64_t pos = jf_pos((uint64_t)A.pos_base, li, 0, !matched);
-            // input
-            InVal v;
-            if (IN_MODE == 0) {
-                v.valid = pv_ok[i];
-                v.bits = pv[i];
-            } else {
-                InVal b;
-                b.valid = matched && ((e1[i] >> (24 + A.wslot)) & 1);
-                b.bits = A.wslot ? e3[i] : e2[i];
-                if (IN_MODE == 1) {
-                    v = b;
-                } else {
-                    InVal a;
-                    a.valid = pv_ok[i];
-                    a.bits = pv[i];
-                    v = jf_mul(a, A.ta, b, A.tb);
-                }
-            }
-            const int64_t ls = gb_lds_dense_slot(A.S, gcls, gk, A.kmin);
-            // one expansion per address space, so the LDS one compiles to
-            // ds_* atomics rather than flat ones
-            if (ls >= 0) {
-                star_dense_update(A, lhot + (size_t)ls * A.RH, lcold + (size_t)ls * A.RC, pos, v);
-            } else {
-                const int64_t gs = gb_global_slot(L, A.hot, A.cap, gcls, gk, pos, A.ctl);
-                if (gs >= 0) star_dense_update(A, A.hot + (size_t)gs * A.RH, A.cold + (size_t)gs * A.RC, pos, v);
-            }
+            star_dense_step<ITEMS, IN_MODE, LEFT>(L, A, lhot, lcold, base, tid, live, cls, kn, pv_ok, pv);
         }
     }
     gb_lds_merge(L, lhot, lcold, A.S, VG_EMPTY_I64, A.hot, A.cold, A.cap, A.ctl);
@@ -2363,15 +2421,25 @@ static int groupby_feed_join_body(vgpu_groupby *g, const vgpu_feed_spec *sp) {
         const size_t dl = (size_t)(sd.S + 2) * (sd.RH + sd.RC) * 8;
         int items = (int)vg_tune_for(VG_TUNE_GB_STAR_ITEMS, groups);
         items = items <= 2 ? 2 : (items >= 8 ? 8 : 4);
-        // four blocks per CU: more blocks only add table inits and merges
-        // (measured on 50M probe rows: 1,024 blocks 0.97 ms, 2,048 1.22 ms)
+        // blocks per CU (knob gb.star.blocks_per_cu, default 4): more blocks
+        // add table inits and merges (measured on 50M probe rows with the
+        // general feed: 1,024 blocks 0.97 ms, 2,048 1.22 ms)
         const unsigned grid = vg_blocks(n_logical, vg_tune_for(VG_TUNE_GB_LDS_ROWS, groups),
-                                        std::min<int64_t>((int64_t)vg_cus() * 4, vg_resident_blocks(dl)));
+                                        std::min<int64_t>((int64_t)vg_cus() * vg_tune_for(VG_TUNE_GB_STAR_BPC, groups),
+                                                          vg_resident_blocks(dl)));
         const int mode = st.pa[0] >= 0 && st.pb[0] >= 0 ? 2 : 0;
         const bool left = sp->join_kind == VGPU_JOIN_LEFT;
+        // both modes read the probe operand: plain when it and the probe key
+        // are 8-byte columns without validity
+        const bool plain = !sd.pkey.validity && sd.pkey.type != VGPU_BOOL && sd.pval.data && !sd.pval.validity &&
+                           sd.pval.type != VGPU_BOOL;
         hipStream_t ks = vg_stream();
         vg_kt_begin();
-#define STAR_DENSE(I, M, LF) k_gb_star_dense<I, M, LF><<<grid, GB_THREADS, dl, ks>>>(g->L, sd)
+#define STAR_DENSE(I, M, LF)                                                                \
+    do {                                                                                    \
+        if (plain) k_gb_star_dense<I, M, LF, true><<<grid, GB_THREADS, dl, ks>>>(g->L, sd);  \
+        else k_gb_star_dense<I, M, LF, false><<<grid, GB_THREADS, dl, ks>>>(g->L, sd);       \
+    } while (0)
         if (mode == 2) {
             if (left) { if (items == 2) STAR_DENSE(2, 2, true); else if (items == 4) STAR_DENSE(4, 2, true); else STAR_DENSE(8, 2, true); }
             else { if (items == 2) STAR_DENSE(2, 2, false); else if (items == 4) STAR_DENSE(4, 2, false); else STAR_DENSE(8, 2, false); }
