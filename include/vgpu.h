@@ -303,6 +303,10 @@ void vgpu_groupby_destroy(vgpu_groupby *g);
 /* 8-byte words of one group's record (hot + cold): the width axis of the
  * tuning table's buckets */
 int  vgpu_groupby_record_words(const vgpu_groupby *g);
+/* How the last feed aggregated: 0 into the hash table, 1 through hash
+ * partitions, 2 through key-range partitions (records already in key
+ * order at finish).  A diagnostic with no counterpart in vectra. */
+int  vgpu_groupby_last_path(const vgpu_groupby *g);
 
 /* ---- hash join -------------------------------------------------------- */
 enum { VGPU_JOIN_INNER = 0, VGPU_JOIN_LEFT, VGPU_JOIN_FULL, VGPU_JOIN_SEMI,
@@ -539,6 +543,11 @@ void    vgpu_batch_destroy(vgpu_batch *b);
  *                         int64 build keys of a short range probes a
  *                         direct-addressed map (built once per table)
  *   "gb.star.blocks_per_cu" (4) blocks per CU of the dense star kernel
+ *   "gb.part.range"      (2: on; 1: off) a partitioned group-by in key order
+ *                         over int64 keys splits rows by key range instead
+ *                         of by hash, so its records come out in key order
+ *                         and finish does not sort them (hash partitions
+ *                         are kept when the ranges are badly balanced)
  *   "gb.fuse_join"       (2) a GPU GroupAggNode over Project(a * b) over a
  *                         GPU JoinNode runs the three as one fused device
  *                         pass (2), or node by node (1)

@@ -289,6 +289,103 @@ def test_groupby_partitioned_then_more_batches(gpu, oracle, part_threshold, temp
     check(gpu, mk, ["g"], ALL_AGGS, temp_dir)
 
 
+RANGE_AGGS = [("n", -1, 0), ("count", 0, 0), ("sum", 0, 1), ("min", 0, 1), ("max", 0, 0), ("sum", 1, 1),
+              ("min", 1, 1)]
+
+
+def range_keys(dist, n, r):
+    """key values of a range-partition case and the path its balance picks
+    (2 key ranges, 1 back to hash partitions)"""
+    k = r.integers(-2**62, 2**62, n)
+    if dist == "dense":
+        return r.integers(0, 300_000, n), 2
+    if dist == "cluster":
+        # 120K rows over 50 keys: within one first-pass bucket's slack, but
+        # one partition of 30 items -> hash partitions
+        k[:120_000] = r.integers(0, 50, 120_000)
+        return r.permutation(k), 1
+    if dist == "hot":
+        k[r.random(n) < 0.25] = 7       # a quarter of the rows in one bucket
+        return k, 1
+    if dist == "empty_key":
+        k[r.random(n) < 0.01] = np.iinfo(np.int64).min   # its record joins last: finish sorts
+        return k, 2
+    return k, 2
+
+
+@pytest.mark.parametrize("dist", ["uniform", "dense", "cluster", "hot", "empty_key"])
+def test_groupby_range_partitions(gpu, part_threshold, dist):
+    """key order over int64 keys: records of key-range partitions come out of
+    the partitioned feed in key order (finish sorts nothing unless an
+    INT64_MIN key is present); keys, counts, int64 sums, min and max are
+    bit-exact against numpy and against hash partitions (gb.part.range 1),
+    double sums within 1e-12 * sum |x|; badly balanced ranges fall back"""
+    n = 1_500_000
+    r = rng(1000 + len(dist))
+    k, want_path = range_keys(dist, n, r)
+    kval = r.random(n) >= 0.02
+    iv = r.integers(-10**9, 10**9, n)
+    ival = r.random(n) >= 0.03
+    dv = r.normal(size=n) * 1e6
+    cols = [gpu.DeviceColumn(k, kval), gpu.DeviceColumn(iv, ival), gpu.DeviceColumn(dv)]
+    lib = gpu.vgpu()
+    old = lib.vgpu_tune_get(b"gb.part.range")
+    res = {}
+    try:
+        for mode in (2, 1):
+            assert lib.vgpu_tune_set(b"gb.part.range", mode) == 0
+            gb = gpu.GroupBy(gpu.VEC_INT64, [gpu.VEC_INT64, gpu.VEC_DOUBLE], RANGE_AGGS, gpu.VGPU_ORDER_KEY, 0)
+            gb.feed(cols[0], cols[1:], n)
+            path = gb.last_path()
+            gb.finish()
+            res[mode] = gb.fetch()
+            gb.free()
+            assert path == (want_path if mode == 2 else 1), (mode, path)
+    finally:
+        lib.vgpu_tune_set(b"gb.part.range", old)
+        for c in cols:
+            c.free()
+    # numpy: regular keys ascending, then the NA group
+    kk = k[kval]
+    uk, inv = np.unique(kk, return_inverse=True)
+    ng = len(uk)
+    got = res[2]
+    assert np.array_equal(got["key"][0][:ng], uk) and got["key"][1][:ng].all()
+    assert len(got["key"][0]) == ng + 1 and not got["key"][1][ng]
+    g = np.concatenate([inv, np.full((~kval).sum(), ng)])
+    order = np.concatenate([np.flatnonzero(kval), np.flatnonzero(~kval)])
+    ivo, ivalo, dvo = iv[order], ival[order], dv[order]
+    cnt = np.bincount(g, minlength=ng + 1)
+    assert np.array_equal(got["a0"][0], cnt)
+    assert np.array_equal(got["a1"][0], np.bincount(g, weights=ivalo, minlength=ng + 1))
+    # every result is a double: int64 sums (small enough to be exact), min
+    # with na_rm (+Inf for a group with no value), max without (NA when any
+    # input is NA)
+    s_int = np.zeros(ng + 1, np.int64)
+    np.add.at(s_int, g[ivalo], ivo[ivalo])
+    assert got["a2"][1].all() and np.array_equal(got["a2"][0], s_int.astype(np.float64))
+    lo = np.full(ng + 1, np.inf)
+    np.minimum.at(lo, g[ivalo], ivo[ivalo].astype(np.float64))
+    assert got["a3"][1].all() and np.array_equal(got["a3"][0], lo)
+    hi = np.full(ng + 1, -np.inf)
+    np.maximum.at(hi, g[ivalo], ivo[ivalo].astype(np.float64))
+    all_valid = np.bincount(g[~ivalo], minlength=ng + 1) == 0
+    assert np.array_equal(got["a4"][1], all_valid)
+    assert np.array_equal(got["a4"][0][all_valid], hi[all_valid])
+    sd = np.bincount(g, weights=dvo, minlength=ng + 1)
+    sa = np.bincount(g, weights=np.abs(dvo), minlength=ng + 1)
+    assert got["a5"][1].all() and np.all(np.abs(got["a5"][0] - sd) <= 1e-12 * sa + 1e-300)
+    mn = np.full(ng + 1, np.inf)
+    np.minimum.at(mn, g, dvo)
+    assert got["a6"][1].all() and np.array_equal(got["a6"][0], mn)
+    for col in got:
+        assert np.array_equal(got[col][1], res[1][col][1]), col
+        if col == "a5":
+            assert np.all(np.abs(got[col][0] - res[1][col][0]) <= 2e-12 * sa + 1e-300)
+        else:
+            assert np.array_equal(got[col][0], res[1][col][0]), col
+
+
 def test_groupby_partitioned_after_filter(gpu, oracle, part_threshold):
     """a selection vector from the filter feeds the partitioned path"""
     mk = lambda: table(gpu, 2_400_000, 31, "int", 50_000, 0, p_na=0.01, p_nan=0.01)

@@ -56,6 +56,12 @@
 
 struct GbpArgs {
     int32_t W, RW, B1, B2, S, G1;
+    // range partitioning (key order over int64 keys of a known range): the
+    // partition of a regular key grows with (ordered key - omin) >> rshift,
+    // so the partitions, and the items in them, follow key order; every
+    // item writes its records sorted by key, and finish needs no sort
+    int32_t range, rshift;         // f = min((ordered key - omin) >> rshift, fmax)
+    uint64_t omin, fmax, fmul;     // partition (f * fmul) >> 32, fmul = (P - 1) * 2^32 / (fmax + 1)
     int64_t n_rows;                // logical rows of the feed (bounds every buffer)
     int64_t T1;                    // first-pass rows per block
     int64_t P;                     // partitions
@@ -75,6 +81,8 @@ struct GbpArgs {
     uint64_t *irec;                // first record index of an item
     uint32_t *rflags;              // reserved partials of an item: 1 NA, 2 EMPTY-valued key
     uint32_t *mlist;               // partitions with more than one item
+    uint64_t *chk;                 // balance of range partitions: [0] rows of the largest
+                                   // first-pass bucket, [1] items of the largest partition
     uint64_t *rec;                 // records: (n_rows + 2) x RW words; n_rows / n_rows+1 = NA / EMPTY key
     uint64_t *hot, *cold, *ctl;    // hash table (fallback for records that do not fit an LDS table)
     int64_t cap;
@@ -89,6 +97,7 @@ struct vgpu_groupby {
     int64_t hint = 0;
     int64_t est_groups = 0;   // groups expected when no hint was given (sketch or groups seen); 0 = unknown
     int32_t sk_ranged = 0;    // the last sketch saw regular int64 / bool keys in [sk_lo, sk_hi]
+    int32_t path = 0;         // the last feed: 0 hash table, 1 hash partitions, 2 range partitions
     int64_t sk_lo = 0, sk_hi = 0;
     int32_t finished = 0;
     int32_t any_fed = 0;
@@ -1129,6 +1138,23 @@ __device__ __forceinline__ uint64_t gbp_hash(int cls, uint64_t kn) {
     return vg_key_hash(kn);
 }
 
+// the word whose top B = B1 + B2 bits are a row's partition: its key's
+// hash, or (range partitioning) its key's place in the range scaled onto
+// partitions 0 .. P - 2; NA and EMPTY-valued keys alone in the last one
+__device__ __forceinline__ uint64_t gbp_pkey(const GbpArgs &G, int cls, uint64_t kn) {
+    if (!G.range) return gbp_hash(cls, kn);
+    if (cls != VG_KEY_REGULAR) return ~0ull;
+    const int B = G.B1 + G.B2;
+    // clamped, so that a key outside the sketched range still lands in a
+    // partition that keeps the order (first or last)
+    const uint64_t o = kn ^ 0x8000000000000000ull;
+    uint64_t f = (o > G.omin ? o - G.omin : 0) >> G.rshift;
+    f = f < G.fmax ? f : G.fmax;
+    // f * (P - 1) / (fmax + 1), below P - 1 and monotone; f * fmul < (P - 1) * 2^32 <= 2^52
+    const uint64_t b = (f * G.fmul) >> 32;
+    return b << (64 - B);
+}
+
 // largest i in [0, n) with a[i] <= x (a ascending, a[0] <= x)
 __device__ __forceinline__ int64_t gbp_search(const uint64_t *a, int64_t n, uint64_t x) {
     int64_t lo = 0, hi = n - 1;
@@ -1172,7 +1198,7 @@ k_gbp_hist1(FeedArgs A, GbpArgs G, PredInfo pi, vgpu_prog prog, ColSet pcs) {
         int cls;
         uint64_t kn, raw;
         if (!gbp_source_row(A, pi, prog, pcs, regs, regv, li, row, cls, kn, raw)) continue;
-        atomicAdd(&hist[gbp_hash(cls, kn) >> (64 - G.B1)], 1u);
+        atomicAdd(&hist[gbp_pkey(G, cls, kn) >> (64 - G.B1)], 1u);
     }
     __syncthreads();
     for (int b = threadIdx.x; b < NB; b += GB_THREADS) G.H1[(int64_t)b * G.G1 + blockIdx.x] = hist[b];
@@ -1206,7 +1232,7 @@ k_gbp_scatter1(GbLayout L, FeedArgs A, GbpArgs G, InSet ins, PredInfo pi, vgpu_p
                 gb_atomic_min(&A.ctl[3], (uint64_t)(A.pos_base + li));
         }
         if (!ok) continue;
-        uint64_t dst = atomicAdd(&cur[gbp_hash(cls, kn) >> (64 - G.B1)], 1ull);
+        uint64_t dst = atomicAdd(&cur[gbp_pkey(G, cls, kn) >> (64 - G.B1)], 1ull);
         uint64_t w[VGPU_MAX_INPUTS + 2];
         uint64_t meta = (uint64_t)(uint32_t)li | ((uint64_t)cls << 32);
         w[0] = kn;
@@ -1288,7 +1314,7 @@ __device__ __forceinline__ int64_t gbp_h2_index(const GbpArgs &G, int64_t b, int
 
 __device__ __forceinline__ uint32_t gbp_sub(const GbpArgs &G, const uint64_t *w) {
     int cls = (int)((w[G.W - 1] >> 32) & 3);
-    return (uint32_t)((gbp_hash(cls, w[0]) >> (64 - G.B1 - G.B2)) & ((1u << G.B2) - 1));
+    return (uint32_t)((gbp_pkey(G, cls, w[0]) >> (64 - G.B1 - G.B2)) & ((1u << G.B2) - 1));
 }
 
 __global__ void __launch_bounds__(GB_THREADS) k_gbp_hist2(GbpArgs G) {
@@ -1357,13 +1383,34 @@ __global__ void k_gbp_pstart(GbpArgs G) {
         start = nblk ? G.O2[gbp_h2_index(G, b, sub, 0, nblk)] : G.O1[b * G.G1];
     }
     G.pstart[p] = start;
+    if (p == 0) G.chk[1] = 0;
 }
 
 __global__ void k_gbp_nitems(GbpArgs G) {
     int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= G.P) return;
     uint64_t cnt = G.pstart[p + 1] - G.pstart[p];
-    G.nitems[p] = (uint32_t)((cnt + GBP_CH - 1) / GBP_CH);
+    const uint32_t items = (uint32_t)((cnt + GBP_CH - 1) / GBP_CH);
+    G.nitems[p] = items;
+    if (G.range && items > 1 && p < G.P - 1)   // not the NA / EMPTY-valued key partition
+        atomicMax((unsigned long long *)&G.chk[1], (unsigned long long)items);
+}
+
+// rows of the largest first-pass bucket but the last (range partitioning;
+// the last holds the NA and EMPTY-valued key rows)
+__global__ void __launch_bounds__(1024) k_gbp_bmax(GbpArgs G) {
+    __shared__ unsigned long long m;
+    if (threadIdx.x == 0) m = 0;
+    __syncthreads();
+    const int64_t NB = (int64_t)1 << G.B1;
+    unsigned long long mine = 0;
+    for (int64_t b = threadIdx.x; b < NB - 1; b += 1024) {
+        const unsigned long long size = G.O1[(b + 1) * G.G1] - G.O1[b * G.G1];
+        mine = size > mine ? size : mine;
+    }
+    atomicMax(&m, mine);
+    __syncthreads();
+    if (threadIdx.x == 0) G.chk[0] = m;
 }
 
 // initial (empty) NA and EMPTY-key records, multi-item list counter
@@ -1378,11 +1425,15 @@ __global__ void k_gbp_init_reserved(GbLayout L, GbpArgs G) {
     if (w == 0) G.ctl[10] = 0;
 }
 
-// copies the occupied records of an LDS table (slots [0, S) regular) to
-// dst, compacted, with coalesced stores; returns the count in every thread
+// copies the occupied records of an LDS table (slots [0, S) regular, S at
+// most 512) to dst, compacted, with coalesced stores; sorted: in ascending
+// order of their int64 keys, each record placed at its rank (the keys of a
+// table are distinct), counted against a compact copy of the keys in kc
+// whose reads are the same address across a wave.  Returns the count in
+// every thread.
 __device__ __forceinline__ uint32_t gbp_write_records(const GbLayout &L, const uint64_t *lhot, const uint64_t *lcold,
                                                       int S, uint64_t empty, uint64_t *dst, int RW, uint16_t *slist,
-                                                      uint32_t *scan_lds) {
+                                                      uint32_t *scan_lds, bool sorted, int64_t *kc) {
     const int per = (S + GB_THREADS - 1) / GB_THREADS;
     uint32_t mine = 0;
     for (int j = 0; j < per; j++) {
@@ -1396,9 +1447,25 @@ __device__ __forceinline__ uint32_t gbp_write_records(const GbLayout &L, const u
         if (s < S && lhot[(size_t)s * L.RH] != empty) slist[off++] = (uint16_t)s;
     }
     __syncthreads();
-    for (int64_t q = threadIdx.x; q < (int64_t)total * RW; q += GB_THREADS) {
-        int k = (int)(q / RW), w = (int)(q % RW);
-        int s = slist[k];
+    const uint16_t *order = slist;
+    if (sorted && total > 1) {
+        const int n = (int)total;
+        for (int i = threadIdx.x; i < n; i += GB_THREADS) kc[i] = (int64_t)lhot[(size_t)slist[i] * L.RH];
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += GB_THREADS) {
+            const int64_t key = kc[i];
+            int rank = 0;
+#pragma unroll 8
+            for (int j = 0; j < n; j++) rank += kc[j] < key;
+            slist[512 + rank] = slist[i];
+        }
+        __syncthreads();
+        order = slist + 512;
+    }
+    const int words = (int)total * RW;
+    for (int q = threadIdx.x; q < words; q += GB_THREADS) {
+        const int k = q / RW, w = q - k * RW;
+        const int s = order[k];
         dst[q] = w < L.RH ? lhot[(size_t)s * L.RH + w] : lcold[(size_t)s * L.RC + (w - L.RH)];
     }
     return total;
@@ -1450,7 +1517,8 @@ __device__ __forceinline__ void gbp_agg_row(const GbLayout &L, const GbpArgs &G,
 
 __global__ void __launch_bounds__(GB_THREADS) k_gbp_agg(GbLayout L, GbpArgs G) {
     extern __shared__ uint64_t dynw[];   // 8-byte aligned after the static arrays
-    __shared__ uint16_t slist[1024];
+    __shared__ uint16_t slist[1024];   // [0, 512) occupied slots, [512, 1024) in key order
+    __shared__ int64_t kc[512];
     __shared__ uint32_t scan_lds[GB_THREADS / VG_WAVE];
     const int64_t item = blockIdx.x;
     const int64_t NI = (int64_t)G.itembase[G.P];
@@ -1496,7 +1564,8 @@ __global__ void __launch_bounds__(GB_THREADS) k_gbp_agg(GbLayout L, GbpArgs G) {
     }
     if (spilled) G.ctl[9] = 1;
     __syncthreads();
-    uint32_t total = gbp_write_records(L, lhot, lcold, S, empty, G.rec + r0 * G.RW, G.RW, slist, scan_lds);
+    uint32_t total =
+        gbp_write_records(L, lhot, lcold, S, empty, G.rec + r0 * G.RW, G.RW, slist, scan_lds, G.range != 0, kc);
     if (threadIdx.x == 0) {
         G.icount[item] = total;
         G.irec[item] = (uint64_t)r0;
@@ -1522,7 +1591,8 @@ __global__ void __launch_bounds__(GB_THREADS) k_gbp_agg(GbLayout L, GbpArgs G) {
 // one block per partition with several items: merge their partial records
 __global__ void __launch_bounds__(GB_THREADS) k_gbp_combine(GbLayout L, GbpArgs G) {
     extern __shared__ uint64_t dynw[];   // 8-byte aligned after the static arrays
-    __shared__ uint16_t slist[1024];
+    __shared__ uint16_t slist[1024];   // [0, 512) occupied slots, [512, 1024) in key order
+    __shared__ int64_t kc[512];
     __shared__ uint32_t scan_lds[GB_THREADS / VG_WAVE];
     if ((uint64_t)blockIdx.x >= G.ctl[10]) return;
     const int S = G.S;
@@ -1555,7 +1625,7 @@ __global__ void __launch_bounds__(GB_THREADS) k_gbp_combine(GbLayout L, GbpArgs 
     if (spilled) G.ctl[9] = 1;
     __syncthreads();
     uint32_t total = gbp_write_records(L, lhot, lcold, S, empty, G.rec + (int64_t)G.irec[i0] * G.RW, G.RW, slist,
-                                       scan_lds);
+                                       scan_lds, G.range != 0, kc);
     for (int64_t it = i0 + 1 + threadIdx.x; it < i1; it += GB_THREADS) G.icount[it] = 0;
     if (threadIdx.x == 0) {
         G.icount[i0] = total;
@@ -1595,10 +1665,12 @@ k_gbp_collect(GbLayout L, RecSrc R, int order, const uint32_t *icount, const uin
 __global__ void k_gbp_reserved(GbLayout L, RecSrc R, int order, uint64_t *skeys, uint32_t *svals, uint64_t *ctl) {
     if (threadIdx.x != 0) return;
     ctl[8] = R.hot[R.na_slot * R.hstride + 1];
+    ctl[11] = 0;
     const int64_t slots[2] = {R.sp_slot, R.na_slot};
     for (int j = 0; j < 2; j++) {
         uint64_t sk;
         if (!gb_sort_entry(L, R, order, slots[j], sk)) continue;
+        ctl[11] |= 1ull << j;   // entries appended after the run's key-ordered records
         uint64_t at = ctl[5]++;
         skeys[at] = sk;
         svals[at] = (uint32_t)slots[j];
@@ -1674,11 +1746,15 @@ static void gbp_layout(GbpArgs &G, uint8_t *base, size_t *bytes) {
     G.irec = c.take<uint64_t>(G.NImax);
     G.rflags = c.take<uint32_t>(G.NImax);
     G.mlist = c.take<uint32_t>(ML);
+    G.chk = c.take<uint64_t>(2);
     *bytes = c.off;
 }
 
+// bits of a range of ordered keys' differences (0 for a range of one key)
+static int span_bits(uint64_t span) { return span ? 64 - __builtin_clzll(span) : 0; }
+
 static int gbp_feed(vgpu_groupby *g, FeedArgs A, const InSet &ins, const PredInfo &pi, const vgpu_prog &prog,
-                    const ColSet &pcs, int64_t groups) {
+                    const ColSet &pcs, int64_t groups, bool sketched) {
     hipStream_t st = vg_stream();
     const GbLayout &L = g->L;
     const int64_t n = A.n_logical;
@@ -1699,6 +1775,19 @@ static int gbp_feed(vgpu_groupby *g, FeedArgs A, const InSet &ins, const PredInf
     G.pos_base = A.pos_base;
     G.NB2max = ((int64_t)1 << G.B1) + n / GBP_T2 + 1;
     G.NImax = G.P + n / GBP_CH + 1;
+    // key order over int64 keys whose range this feed's sketch saw: range
+    // partitions (knob gb.part.range)
+    if (sketched && g->sk_ranged && g->order == VGPU_ORDER_KEY && !L.key_is_double &&
+        vg_tune_for(VG_TUNE_GB_PART_RANGE, groups) >= 2) {
+        G.range = 1;
+        G.omin = (uint64_t)g->sk_lo ^ 0x8000000000000000ull;
+        // f keeps up to B + 8 bits of a key's place in the range: every
+        // partition within about one part in 2^7 of its even share
+        const uint64_t span = ((uint64_t)g->sk_hi ^ 0x8000000000000000ull) - G.omin;
+        G.rshift = std::max(0, span_bits(span) - (B + 8));
+        G.fmax = span >> G.rshift;
+        G.fmul = (uint64_t)(((unsigned __int128)(G.P - 1) << 32) / ((unsigned __int128)G.fmax + 1));
+    }
     size_t aux = 0;
     gbp_layout(G, nullptr, &aux);
     size_t stage = (size_t)n * G.W * 8;
@@ -1719,23 +1808,53 @@ static int gbp_feed(vgpu_groupby *g, FeedArgs A, const InSet &ins, const PredInf
 
     const int64_t NB = (int64_t)1 << G.B1, NS = (int64_t)1 << G.B2;
     size_t lds1 = (size_t)A.vm_lds + NB * 8;
+    // Range partitions are only as balanced as the keys are spread over
+    // their range.  Two checks, each one small read: the largest first-pass
+    // bucket (before any row is moved) and the largest partition in items
+    // (after both passes; a partition of many items is merged by one
+    // block).  Either beyond 4x its even share plus slack -> hash partitions.
+    const uint64_t bucket_limit = 4 * (uint64_t)(n / NB) + 4 * GBP_T2;
+    const uint64_t item_limit = 4 * (uint64_t)((n / G.P + GBP_CH - 1) / GBP_CH) + 4;
     vg_kt_begin();
-    k_gbp_hist1<<<G.G1, GB_THREADS, lds1, st>>>(A, G, pi, prog, pcs);
-    VG_LAUNCH_CHECK();
-    if (vg_scan_u32_to_u64(G.H1, G.O1, NB * G.G1, G.O1 + NB * G.G1, g->sort_tmp)) return -1;
-    k_gbp_scatter1<<<G.G1, GB_THREADS, lds1, st>>>(L, A, G, ins, pi, prog, pcs);
-    VG_LAUNCH_CHECK();
-    k_gbp_plan2<<<1, 1024, 0, st>>>(G);
-    VG_LAUNCH_CHECK();
-    k_gbp_hist2<<<(unsigned)G.NB2max, GB_THREADS, 0, st>>>(G);
-    VG_LAUNCH_CHECK();
-    if (vg_scan_u32_to_u64(G.H2, G.O2, G.NB2max * NS, nullptr, g->sort_tmp)) return -1;
-    k_gbp_scatter2<<<(unsigned)G.NB2max, GB_THREADS, 0, st>>>(G);
-    VG_LAUNCH_CHECK();
-    k_gbp_pstart<<<vg_blocks(G.P + 1, 256), 256, 0, st>>>(G);
-    VG_LAUNCH_CHECK();
-    k_gbp_nitems<<<vg_blocks(G.P, 256), 256, 0, st>>>(G);
-    VG_LAUNCH_CHECK();
+    for (;;) {
+        k_gbp_hist1<<<G.G1, GB_THREADS, lds1, st>>>(A, G, pi, prog, pcs);
+        VG_LAUNCH_CHECK();
+        if (vg_scan_u32_to_u64(G.H1, G.O1, NB * G.G1, G.O1 + NB * G.G1, g->sort_tmp)) return -1;
+        if (G.range) {
+            uint64_t big = 0;
+            k_gbp_bmax<<<1, 1024, 0, st>>>(G);
+            VG_LAUNCH_CHECK();
+            VG_TRY(hipMemcpyAsync(&big, G.chk, 8, hipMemcpyDeviceToHost, st));
+            VG_TRY(hipStreamSynchronize(st));
+            if (big > bucket_limit) {
+                G.range = 0;
+                continue;
+            }
+        }
+        k_gbp_scatter1<<<G.G1, GB_THREADS, lds1, st>>>(L, A, G, ins, pi, prog, pcs);
+        VG_LAUNCH_CHECK();
+        k_gbp_plan2<<<1, 1024, 0, st>>>(G);
+        VG_LAUNCH_CHECK();
+        k_gbp_hist2<<<(unsigned)G.NB2max, GB_THREADS, 0, st>>>(G);
+        VG_LAUNCH_CHECK();
+        if (vg_scan_u32_to_u64(G.H2, G.O2, G.NB2max * NS, nullptr, g->sort_tmp)) return -1;
+        k_gbp_scatter2<<<(unsigned)G.NB2max, GB_THREADS, 0, st>>>(G);
+        VG_LAUNCH_CHECK();
+        k_gbp_pstart<<<vg_blocks(G.P + 1, 256), 256, 0, st>>>(G);
+        VG_LAUNCH_CHECK();
+        k_gbp_nitems<<<vg_blocks(G.P, 256), 256, 0, st>>>(G);
+        VG_LAUNCH_CHECK();
+        if (G.range) {
+            uint64_t big = 0;
+            VG_TRY(hipMemcpyAsync(&big, G.chk + 1, 8, hipMemcpyDeviceToHost, st));
+            VG_TRY(hipStreamSynchronize(st));
+            if (big > item_limit) {
+                G.range = 0;
+                continue;
+            }
+        }
+        break;
+    }
     if (vg_scan_u32_to_u64(G.nitems, G.itembase, G.P, G.itembase + G.P, g->sort_tmp)) return -1;
     k_gbp_init_reserved<<<1, 128, 0, st>>>(L, G);
     VG_LAUNCH_CHECK();
@@ -1748,6 +1867,7 @@ static int gbp_feed(vgpu_groupby *g, FeedArgs A, const InSet &ins, const PredInf
     g->gbp = G;
     g->run = 1;
     g->any_fed = 1;
+    g->path = G.range ? 2 : 1;
     return 0;
 }
 
@@ -2069,6 +2189,7 @@ static int groupby_feed_body(vgpu_groupby *g, const vgpu_col *key, const vgpu_co
             return -1;
         }
     if (g->run && gbp_flush(g)) return -1;
+    g->path = 0;
     // groups created by this feed are bounded by its rows, by the hint, or
     // (a large feed without a hint) by its sketched key count
     int64_t incoming = n_logical;
@@ -2113,7 +2234,7 @@ static int groupby_feed_body(vgpu_groupby *g, const vgpu_col *key, const vgpu_co
     memset(&pcs, 0, sizeof pcs);
     for (int i = 0; i < n_pred_cols && pred_cols; i++) pcs.c[i] = pred_cols[i];
     A.vm_lds = (int32_t)vm;
-    if (partitioned) return gbp_feed(g, A, ins, pi, prog, pcs, groups);
+    if (partitioned) return gbp_feed(g, A, ins, pi, prog, pcs, groups, sketched);
     A.lds_cold = lds_cold;
     if (gb_narrow_shape(g->L, pi) && vg_tune(VG_TUNE_GB_NARROW) >= 2)
         return gb_feed_narrow(g, A, inputs, pi, pred_cols, groups, sketched);
@@ -2270,6 +2391,7 @@ static int groupby_feed_join_body(vgpu_groupby *g, const vgpu_feed_spec *sp) {
         return -1;
     }
     if (g->run && gbp_flush(g)) return -1;
+    g->path = 0;
     int64_t n_logical = sp->sel ? sp->n_sel : sp->n_rows;
     if (n_logical <= 0) return 0;
     VgTuneScope tune_scope(g->L.RH + g->L.RC, n_logical);
@@ -2557,7 +2679,10 @@ static int64_t groupby_finish_body(vgpu_groupby *g) {
     // NA group of the key-sorted order: appended after the sort
     int na_last = g->order == VGPU_ORDER_KEY && ctl[8] != 0;
     uint64_t diff = ctl[6] ^ ctl[7];
-    if (n_sorted > 1 && diff) {
+    // a run of range partitions collected its records in key order; only
+    // an EMPTY-valued key's record, appended last, would need the sort
+    const bool in_order = !from_table && g->gbp.range && (ctl[11] & 1) == 0;
+    if (n_sorted > 1 && diff && !in_order) {
         int lo = lowest_bit(diff), hi = highest_bit(diff) + 1;
         if (vg_sort_pairs(g->sort_keys.as<uint64_t>(), g->sort_vals.as<uint32_t>(), n_sorted, lo, hi, g->sort_tmp))
             return -1;
@@ -2680,6 +2805,8 @@ extern "C" int vgpu_groupby_result_device(vgpu_groupby *g, const void **keys, co
 }
 
 extern "C" int vgpu_groupby_record_words(const vgpu_groupby *g) { return g ? g->L.RH + g->L.RC : -1; }
+
+extern "C" int vgpu_groupby_last_path(const vgpu_groupby *g) { return g ? g->path : -1; }
 
 extern "C" void vgpu_groupby_destroy(vgpu_groupby *g) {
     if (!g) return;

@@ -256,6 +256,7 @@ def vgpu() -> C.CDLL:
                                                       C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)])
         _proto(lib, "vgpu_groupby_destroy", None, [C.c_void_p])
         _proto(lib, "vgpu_groupby_record_words", C.c_int, [C.c_void_p])
+        _proto(lib, "vgpu_groupby_last_path", C.c_int, [C.c_void_p])
         _proto(lib, "vgpu_join_build", C.c_void_p, [C.POINTER(VgpuCol), C.c_int64])
         _proto(lib, "vgpu_join_build_payload", C.c_void_p, [C.POINTER(VgpuCol), C.c_int64, C.POINTER(VgpuCol),
                                                             C.c_int])
@@ -1028,6 +1029,10 @@ class GroupBy:
         _vcheck(n)
         self.n_groups = n
         return n
+
+    def last_path(self) -> int:
+        """0 hash table, 1 hash partitions, 2 key-range partitions"""
+        return vgpu().vgpu_groupby_last_path(self.h)
 
     def fetch(self) -> dict[str, tuple[np.ndarray, np.ndarray]]:
         n = self.n_groups
