@@ -313,21 +313,25 @@ def range_keys(dist, n, r):
     return k, 2
 
 
-@pytest.mark.parametrize("dist", ["uniform", "dense", "cluster", "hot", "empty_key"])
-def test_groupby_range_partitions(gpu, part_threshold, dist):
+@pytest.mark.parametrize("dist,nulls", [("uniform", True), ("dense", True), ("cluster", True), ("hot", True),
+                                        ("empty_key", True), ("uniform", False), ("empty_key", False),
+                                        ("hot", False)])
+def test_groupby_range_partitions(gpu, part_threshold, dist, nulls):
     """key order over int64 keys: records of key-range partitions come out of
     the partitioned feed in key order (finish sorts nothing unless an
     INT64_MIN key is present); keys, counts, int64 sums, min and max are
     bit-exact against numpy and against hash partitions (gb.part.range 1),
-    double sums within 1e-12 * sum |x|; badly balanced ranges fall back"""
+    double sums within 1e-12 * sum |x|; badly balanced ranges fall back.
+    Without NA keys or inputs the first pass takes its plain form."""
     n = 1_500_000
     r = rng(1000 + len(dist))
     k, want_path = range_keys(dist, n, r)
-    kval = r.random(n) >= 0.02
+    kval = r.random(n) >= 0.02 if nulls else np.ones(n, bool)
     iv = r.integers(-10**9, 10**9, n)
-    ival = r.random(n) >= 0.03
+    ival = r.random(n) >= 0.03 if nulls else np.ones(n, bool)
     dv = r.normal(size=n) * 1e6
-    cols = [gpu.DeviceColumn(k, kval), gpu.DeviceColumn(iv, ival), gpu.DeviceColumn(dv)]
+    cols = [gpu.DeviceColumn(k, kval if nulls else None), gpu.DeviceColumn(iv, ival if nulls else None),
+            gpu.DeviceColumn(dv)]
     lib = gpu.vgpu()
     old = lib.vgpu_tune_get(b"gb.part.range")
     res = {}
@@ -345,37 +349,40 @@ def test_groupby_range_partitions(gpu, part_threshold, dist):
         lib.vgpu_tune_set(b"gb.part.range", old)
         for c in cols:
             c.free()
-    # numpy: regular keys ascending, then the NA group
+    # numpy: regular keys ascending, then the NA group (if any)
     kk = k[kval]
     uk, inv = np.unique(kk, return_inverse=True)
     ng = len(uk)
     got = res[2]
     assert np.array_equal(got["key"][0][:ng], uk) and got["key"][1][:ng].all()
-    assert len(got["key"][0]) == ng + 1 and not got["key"][1][ng]
+    assert len(got["key"][0]) == ng + int(nulls)
+    if nulls:
+        assert not got["key"][1][ng]
+    m = ng + int(nulls)
     g = np.concatenate([inv, np.full((~kval).sum(), ng)])
     order = np.concatenate([np.flatnonzero(kval), np.flatnonzero(~kval)])
     ivo, ivalo, dvo = iv[order], ival[order], dv[order]
-    cnt = np.bincount(g, minlength=ng + 1)
+    cnt = np.bincount(g, minlength=m)
     assert np.array_equal(got["a0"][0], cnt)
-    assert np.array_equal(got["a1"][0], np.bincount(g, weights=ivalo, minlength=ng + 1))
+    assert np.array_equal(got["a1"][0], np.bincount(g, weights=ivalo, minlength=m))
     # every result is a double: int64 sums (small enough to be exact), min
     # with na_rm (+Inf for a group with no value), max without (NA when any
     # input is NA)
-    s_int = np.zeros(ng + 1, np.int64)
+    s_int = np.zeros(m, np.int64)
     np.add.at(s_int, g[ivalo], ivo[ivalo])
     assert got["a2"][1].all() and np.array_equal(got["a2"][0], s_int.astype(np.float64))
-    lo = np.full(ng + 1, np.inf)
+    lo = np.full(m, np.inf)
     np.minimum.at(lo, g[ivalo], ivo[ivalo].astype(np.float64))
     assert got["a3"][1].all() and np.array_equal(got["a3"][0], lo)
-    hi = np.full(ng + 1, -np.inf)
+    hi = np.full(m, -np.inf)
     np.maximum.at(hi, g[ivalo], ivo[ivalo].astype(np.float64))
-    all_valid = np.bincount(g[~ivalo], minlength=ng + 1) == 0
+    all_valid = np.bincount(g[~ivalo], minlength=m) == 0
     assert np.array_equal(got["a4"][1], all_valid)
     assert np.array_equal(got["a4"][0][all_valid], hi[all_valid])
-    sd = np.bincount(g, weights=dvo, minlength=ng + 1)
-    sa = np.bincount(g, weights=np.abs(dvo), minlength=ng + 1)
+    sd = np.bincount(g, weights=dvo, minlength=m)
+    sa = np.bincount(g, weights=np.abs(dvo), minlength=m)
     assert got["a5"][1].all() and np.all(np.abs(got["a5"][0] - sd) <= 1e-12 * sa + 1e-300)
-    mn = np.full(ng + 1, np.inf)
+    mn = np.full(m, np.inf)
     np.minimum.at(mn, g, dvo)
     assert got["a6"][1].all() and np.array_equal(got["a6"][0], mn)
     for col in got:

@@ -1182,6 +1182,18 @@ __device__ __forceinline__ bool gbp_source_row(const FeedArgs &A, const PredInfo
     return true;
 }
 
+// Plain first pass: no predicate, no selection vector, no validity bitmap,
+// an int64 key and 8-byte inputs.  Every row is regular or EMPTY-valued, so
+// the loads of four rows per thread go out together (row indexes clamped,
+// results of rows past the end unused) instead of one at a time behind the
+// validity and type branches.
+#define GBP_PLAIN_U 4
+
+__device__ __forceinline__ int gbp_plain_class(uint64_t k) {
+    return k == VG_EMPTY_I64 ? VG_KEY_SPECIAL : VG_KEY_REGULAR;
+}
+
+template <bool PLAIN>
 __global__ void __launch_bounds__(GB_THREADS)
 k_gbp_hist1(FeedArgs A, GbpArgs G, PredInfo pi, vgpu_prog prog, ColSet pcs) {
     extern __shared__ uint8_t dyn[];
@@ -1193,17 +1205,47 @@ k_gbp_hist1(FeedArgs A, GbpArgs G, PredInfo pi, vgpu_prog prog, ColSet pcs) {
     __syncthreads();
     const int64_t r0 = (int64_t)blockIdx.x * G.T1;
     const int64_t r1 = r0 + G.T1 < A.n_logical ? r0 + G.T1 : A.n_logical;
-    for (int64_t li = r0 + threadIdx.x; li < r1; li += GB_THREADS) {
-        int64_t row;
-        int cls;
-        uint64_t kn, raw;
-        if (!gbp_source_row(A, pi, prog, pcs, regs, regv, li, row, cls, kn, raw)) continue;
-        atomicAdd(&hist[gbp_pkey(G, cls, kn) >> (64 - G.B1)], 1u);
+    if (PLAIN) {
+        const uint64_t *kp = static_cast<const uint64_t *>(A.key.data);
+        for (int64_t li = r0 + threadIdx.x; li < r1; li += (int64_t)GB_THREADS * GBP_PLAIN_U) {
+            uint64_t k[GBP_PLAIN_U];
+#pragma unroll
+            for (int u = 0; u < GBP_PLAIN_U; u++) {
+                const int64_t rr = li + (int64_t)u * GB_THREADS;
+                k[u] = kp[rr < r1 ? rr : r1 - 1];
+            }
+#pragma unroll
+            for (int u = 0; u < GBP_PLAIN_U; u++)
+                if (li + (int64_t)u * GB_THREADS < r1)
+                    atomicAdd(&hist[gbp_pkey(G, gbp_plain_class(k[u]), k[u]) >> (64 - G.B1)], 1u);
+        }
+    } else {
+        for (int64_t li = r0 + threadIdx.x; li < r1; li += GB_THREADS) {
+            int64_t row;
+            int cls;
+            uint64_t kn, raw;
+            if (!gbp_source_row(A, pi, prog, pcs, regs, regv, li, row, cls, kn, raw)) continue;
+            atomicAdd(&hist[gbp_pkey(G, cls, kn) >> (64 - G.B1)], 1u);
+        }
     }
     __syncthreads();
     for (int b = threadIdx.x; b < NB; b += GB_THREADS) G.H1[(int64_t)b * G.G1 + blockIdx.x] = hist[b];
 }
 
+// a staged row: key, inputs, meta (G.W words; 16-byte stores for four)
+__device__ __forceinline__ void gbp_store_row(const GbpArgs &G, uint64_t *d, const uint64_t *w, uint64_t meta) {
+    if (G.W == 4) {
+        reinterpret_cast<ulonglong2 *>(d)[0] = make_ulonglong2(w[0], w[1]);
+        reinterpret_cast<ulonglong2 *>(d)[1] = make_ulonglong2(w[2], meta);
+    } else {
+#pragma unroll
+        for (int k = 0; k < VGPU_MAX_INPUTS + 1; k++)
+            if (k < G.W - 1) d[k] = w[k];
+        d[G.W - 1] = meta;
+    }
+}
+
+template <bool PLAIN>
 __global__ void __launch_bounds__(GB_THREADS)
 k_gbp_scatter1(GbLayout L, FeedArgs A, GbpArgs G, InSet ins, PredInfo pi, vgpu_prog prog, ColSet pcs) {
     extern __shared__ uint8_t dyn[];
@@ -1215,6 +1257,32 @@ k_gbp_scatter1(GbLayout L, FeedArgs A, GbpArgs G, InSet ins, PredInfo pi, vgpu_p
     __syncthreads();
     const int64_t r0 = (int64_t)blockIdx.x * G.T1;
     const int64_t r1 = r0 + G.T1 < A.n_logical ? r0 + G.T1 : A.n_logical;
+    if (PLAIN) {
+        const uint64_t *kp = static_cast<const uint64_t *>(A.key.data);
+        uint64_t valid_bits = 0;
+        for (int i = 0; i < L.n_inputs; i++) valid_bits |= 1ull << (34 + i);
+        for (int64_t li = r0 + threadIdx.x; li < r1; li += (int64_t)GB_THREADS * GBP_PLAIN_U) {
+            uint64_t w[GBP_PLAIN_U][VGPU_MAX_INPUTS + 1];
+#pragma unroll
+            for (int u = 0; u < GBP_PLAIN_U; u++) {
+                int64_t rr = li + (int64_t)u * GB_THREADS;
+                rr = rr < r1 ? rr : r1 - 1;
+                w[u][0] = kp[rr];
+#pragma unroll
+                for (int i = 0; i < VGPU_MAX_INPUTS; i++)
+                    w[u][1 + i] = i < L.n_inputs ? static_cast<const uint64_t *>(ins.c[i].data)[rr] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < GBP_PLAIN_U; u++) {
+                const int64_t rr = li + (int64_t)u * GB_THREADS;
+                if (rr >= r1) continue;
+                const int cls = gbp_plain_class(w[u][0]);
+                const uint64_t dst = atomicAdd(&cur[gbp_pkey(G, cls, w[u][0]) >> (64 - G.B1)], 1ull);
+                gbp_store_row(G, G.S1 + dst * G.W, w[u], (uint64_t)(uint32_t)rr | ((uint64_t)cls << 32) | valid_bits);
+            }
+        }
+        return;
+    }
     const int lane = vg_lane();
     for (int64_t li = r0 + threadIdx.x; li < r1; li += GB_THREADS) {
         int64_t row = 0;
@@ -1244,17 +1312,7 @@ k_gbp_scatter1(GbLayout L, FeedArgs A, GbpArgs G, InSet ins, PredInfo pi, vgpu_p
                 meta |= (uint64_t)v << (34 + i);
             }
         }
-        uint64_t *d = G.S1 + dst * G.W;
-        if (G.W == 4) {
-            // the common two-input row: two 16-byte stores
-            reinterpret_cast<ulonglong2 *>(d)[0] = make_ulonglong2(w[0], w[1]);
-            reinterpret_cast<ulonglong2 *>(d)[1] = make_ulonglong2(w[2], meta);
-        } else {
-#pragma unroll
-            for (int k = 0; k < VGPU_MAX_INPUTS + 1; k++)
-                if (k < G.W - 1) d[k] = w[k];
-            d[G.W - 1] = meta;
-        }
+        gbp_store_row(G, G.S1 + dst * G.W, w, meta);
     }
 }
 
@@ -1636,15 +1694,21 @@ __global__ void __launch_bounds__(GB_THREADS) k_gbp_combine(GbLayout L, GbpArgs 
     }
 }
 
-// finish of a run: (sort key, record) pairs per item at scanned offsets
+// finish of a run: (sort key, record) pairs per item at scanned offsets;
+// a run of range partitions (records in key order) writes only the records
+// from entry 1 on (entry 0 is for an EMPTY-valued key's record)
 __global__ void __launch_bounds__(GB_THREADS)
 k_gbp_collect(GbLayout L, RecSrc R, int order, const uint32_t *icount, const uint64_t *irec, const uint64_t *ioff,
-              uint64_t *skeys, uint32_t *svals, uint64_t *por, uint64_t *pand) {
+              uint64_t *skeys, uint32_t *svals, uint64_t *por, uint64_t *pand, int range) {
     __shared__ uint64_t lor[2 * GB_THREADS / VG_WAVE];
     const int64_t item = blockIdx.x;
     const int64_t cnt = icount[item];
     const int64_t base = cnt ? (int64_t)irec[item] : 0;
     const uint64_t off = ioff[item];
+    if (range) {
+        for (int64_t k = threadIdx.x; k < cnt; k += GB_THREADS) svals[1 + off + k] = (uint32_t)(base + k);
+        return;
+    }
     uint64_t o = 0, a = ~0ull;
     for (int64_t k = threadIdx.x; k < cnt; k += GB_THREADS) {
         uint64_t sk = 0;
@@ -1661,16 +1725,27 @@ k_gbp_collect(GbLayout L, RecSrc R, int order, const uint32_t *icount, const uin
     }
 }
 
-// the reserved records of a run join the sort after the regular ones
-__global__ void k_gbp_reserved(GbLayout L, RecSrc R, int order, uint64_t *skeys, uint32_t *svals, uint64_t *ctl) {
+// the reserved records of a run join the sort after the regular ones; in a
+// run of range partitions (key order: the NA group is placed by the emit)
+// an EMPTY-valued key's record takes entry 0, before the regular ones
+// (ctl[11] = 1)
+__global__ void k_gbp_reserved(GbLayout L, RecSrc R, int order, uint64_t *skeys, uint32_t *svals, uint64_t *ctl,
+                               int range) {
     if (threadIdx.x != 0) return;
     ctl[8] = R.hot[R.na_slot * R.hstride + 1];
     ctl[11] = 0;
+    uint64_t sk;
+    if (range) {
+        if (gb_sort_entry(L, R, order, R.sp_slot, sk)) {
+            svals[0] = (uint32_t)R.sp_slot;
+            ctl[11] = 1;
+            ctl[5]++;
+        }
+        return;
+    }
     const int64_t slots[2] = {R.sp_slot, R.na_slot};
     for (int j = 0; j < 2; j++) {
-        uint64_t sk;
         if (!gb_sort_entry(L, R, order, slots[j], sk)) continue;
-        ctl[11] |= 1ull << j;   // entries appended after the run's key-ordered records
         uint64_t at = ctl[5]++;
         skeys[at] = sk;
         svals[at] = (uint32_t)slots[j];
@@ -1815,9 +1890,15 @@ static int gbp_feed(vgpu_groupby *g, FeedArgs A, const InSet &ins, const PredInf
     // block).  Either beyond 4x its even share plus slack -> hash partitions.
     const uint64_t bucket_limit = 4 * (uint64_t)(n / NB) + 4 * GBP_T2;
     const uint64_t item_limit = 4 * (uint64_t)((n / G.P + GBP_CH - 1) / GBP_CH) + 4;
+    // the plain first pass (every row of the feed, no NA, 8-byte columns)
+    bool plain = !A.sel && !pi.present && A.key.type == VGPU_INT64 && !A.key.validity;
+    for (int i = 0; i < L.n_inputs; i++)
+        plain = plain && !ins.c[i].validity && (ins.c[i].type == VGPU_INT64 || ins.c[i].type == VGPU_DOUBLE);
+    auto hist1 = plain ? k_gbp_hist1<true> : k_gbp_hist1<false>;
+    auto scatter1 = plain ? k_gbp_scatter1<true> : k_gbp_scatter1<false>;
     vg_kt_begin();
     for (;;) {
-        k_gbp_hist1<<<G.G1, GB_THREADS, lds1, st>>>(A, G, pi, prog, pcs);
+        hist1<<<G.G1, GB_THREADS, lds1, st>>>(A, G, pi, prog, pcs);
         VG_LAUNCH_CHECK();
         if (vg_scan_u32_to_u64(G.H1, G.O1, NB * G.G1, G.O1 + NB * G.G1, g->sort_tmp)) return -1;
         if (G.range) {
@@ -1831,7 +1912,7 @@ static int gbp_feed(vgpu_groupby *g, FeedArgs A, const InSet &ins, const PredInf
                 continue;
             }
         }
-        k_gbp_scatter1<<<G.G1, GB_THREADS, lds1, st>>>(L, A, G, ins, pi, prog, pcs);
+        scatter1<<<G.G1, GB_THREADS, lds1, st>>>(L, A, G, ins, pi, prog, pcs);
         VG_LAUNCH_CHECK();
         k_gbp_plan2<<<1, 1024, 0, st>>>(G);
         VG_LAUNCH_CHECK();
@@ -1916,12 +1997,14 @@ static int gbp_collect(vgpu_groupby *g, const RecSrc &R) {
     if (vg_scan_u32_to_u64(G.icount, ioff, NI, ctl + 5, g->sort_tmp)) return -1;
     k_gbp_collect<<<(unsigned)NI, GB_THREADS, 0, st>>>(g->L, R, g->order, G.icount, G.irec, ioff,
                                                        g->sort_keys.as<uint64_t>(), g->sort_vals.as<uint32_t>(), por,
-                                                       pand);
+                                                       pand, G.range);
     VG_LAUNCH_CHECK();
-    k_gb_or_and_final<<<vg_blocks(NI, 4 * GB_THREADS, 256), GB_THREADS, 0, st>>>(por, pand, NI, ctl);
-    VG_LAUNCH_CHECK();
+    if (!G.range) {
+        k_gb_or_and_final<<<vg_blocks(NI, 4 * GB_THREADS, 256), GB_THREADS, 0, st>>>(por, pand, NI, ctl);
+        VG_LAUNCH_CHECK();
+    }
     k_gbp_reserved<<<1, 64, 0, st>>>(g->L, R, g->order, g->sort_keys.as<uint64_t>(), g->sort_vals.as<uint32_t>(),
-                                     ctl);
+                                     ctl, G.range);
     VG_LAUNCH_CHECK();
     return 0;
 }
@@ -2679,9 +2762,10 @@ static int64_t groupby_finish_body(vgpu_groupby *g) {
     // NA group of the key-sorted order: appended after the sort
     int na_last = g->order == VGPU_ORDER_KEY && ctl[8] != 0;
     uint64_t diff = ctl[6] ^ ctl[7];
-    // a run of range partitions collected its records in key order; only
-    // an EMPTY-valued key's record, appended last, would need the sort
-    const bool in_order = !from_table && g->gbp.range && (ctl[11] & 1) == 0;
+    // a run of range partitions collected its records in key order from
+    // entry 1, entry 0 holding an EMPTY-valued key's record if there is one
+    const bool in_order = !from_table && g->gbp.range;
+    const uint32_t *order_vals = g->sort_vals.as<uint32_t>() + (in_order && !(ctl[11] & 1) ? 1 : 0);
     if (n_sorted > 1 && diff && !in_order) {
         int lo = lowest_bit(diff), hi = highest_bit(diff) + 1;
         if (vg_sort_pairs(g->sort_keys.as<uint64_t>(), g->sort_vals.as<uint32_t>(), n_sorted, lo, hi, g->sort_tmp))
@@ -2714,10 +2798,10 @@ static int64_t groupby_finish_body(vgpu_groupby *g) {
         const int RW = g->L.RH + g->L.RC;
         if (RW <= GB_EMIT_STAGE_WORDS)
             k_gb_emit<true><<<vg_blocks(n_out, GB_THREADS), GB_THREADS, (size_t)GB_THREADS * RW * 8, s>>>(
-                g->L, g->aggs, R, g->sort_vals.as<uint32_t>(), n_sorted, na_last, n_out, 8, O, g->ctl.as<uint64_t>());
+                g->L, g->aggs, R, order_vals, n_sorted, na_last, n_out, 8, O, g->ctl.as<uint64_t>());
         else
             k_gb_emit<false><<<vg_blocks(n_out, GB_THREADS), GB_THREADS, 0, s>>>(
-                g->L, g->aggs, R, g->sort_vals.as<uint32_t>(), n_sorted, na_last, n_out, 8, O, g->ctl.as<uint64_t>());
+                g->L, g->aggs, R, order_vals, n_sorted, na_last, n_out, 8, O, g->ctl.as<uint64_t>());
         VG_LAUNCH_CHECK();
     }
     g->n_groups = n_out;
