@@ -548,6 +548,12 @@ void    vgpu_batch_destroy(vgpu_batch *b);
  *                         of by hash, so its records come out in key order
  *                         and finish does not sort them (hash partitions
  *                         are kept when the ranges are badly balanced)
+ *   "gb.part.stage"      (2) the partition passes of a partitioned
+ *                         group-by stage each tile of rows in LDS in bucket
+ *                         order and write a bucket's rows of the tile as one
+ *                         run, and its aggregation runs on resident blocks:
+ *                         2 = tiles of 1,024 rows, 3 = of 2,048 rows, 1 =
+ *                         row-wise scatters and one block per item
  *   "gb.fuse_join"       (2) a GPU GroupAggNode over Project(a * b) over a
  *                         GPU JoinNode runs the three as one fused device
  *                         pass (2), or node by node (1)

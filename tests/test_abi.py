@@ -136,3 +136,77 @@ int main(void) {
     assert lines[0].split() == ["24", "8", "24", "16", "48"]
     assert lines[1].split() == ["8", "40", "8", "56"]
     assert lines[2].split() == ["56", "64", "72", "80", "88", "96"]
+
+
+REF_SRC = Path("/root/reference/src")
+REF_HEADERS = ("types.h", "expr.h", "agg_ops.h", "group_agg.h", "join.h", "filter.h", "project.h")
+
+# sizes, offsets and enum values of everything the drop-in boundary shares
+# with the reference; the node structs are compared on the reference prefix
+# (the GPU build appends one opaque pointer)
+LAYOUT_PROBE = r'''
+#include <stdio.h>
+#include <stddef.h>
+#ifdef USE_REF
+#include "types.h"
+#include "expr.h"
+#include "agg_ops.h"
+#include "group_agg.h"
+#include "join.h"
+#include "filter.h"
+#include "project.h"
+#else
+#include "vectra_abi.h"
+#endif
+#define S(t) printf(#t " %zu\n", sizeof(t));
+#define O(t, f) printf(#t "." #f " %zu\n", offsetof(t, f));
+#define E(v) printf(#v " %d\n", (int)(v));
+int main(void) {
+  S(VecArray) O(VecArray, type) O(VecArray, length) O(VecArray, validity) O(VecArray, owns_data)
+  O(VecArray, buf.i64) O(VecArray, buf.str.offsets) O(VecArray, buf.str.data) O(VecArray, buf.str.data_len)
+  S(VecBatch) O(VecBatch, n_rows) O(VecBatch, n_cols) O(VecBatch, col_names) O(VecBatch, columns)
+  O(VecBatch, sel) O(VecBatch, sel_n)
+  S(VecSchema) O(VecSchema, n_cols) O(VecSchema, col_names) O(VecSchema, col_types) O(VecSchema, col_annotations)
+  S(VecNode) O(VecNode, next_batch) O(VecNode, free_node) O(VecNode, output_schema) O(VecNode, kind)
+  S(VecExpr) O(VecExpr, kind) O(VecExpr, result_type) O(VecExpr, col_name) O(VecExpr, lit_i64)
+  O(VecExpr, lit_dbl) O(VecExpr, lit_bln) O(VecExpr, lit_str) O(VecExpr, op) O(VecExpr, op2) O(VecExpr, left)
+  O(VecExpr, right) O(VecExpr, operand) O(VecExpr, cond) O(VecExpr, then_expr) O(VecExpr, else_expr)
+  O(VecExpr, cast_to)
+  S(AggSpec) O(AggSpec, output_name) O(AggSpec, kind) O(AggSpec, input_col) O(AggSpec, na_rm)
+  S(JoinKey) O(JoinKey, left_col) O(JoinKey, right_col)
+  S(ProjEntry) O(ProjEntry, output_name) O(ProjEntry, expr)
+  O(FilterNode, base) O(FilterNode, child) O(FilterNode, predicate)
+  O(GroupAggNode, base) O(GroupAggNode, child) O(GroupAggNode, n_keys) O(GroupAggNode, key_names)
+  O(GroupAggNode, n_aggs) O(GroupAggNode, agg_specs) O(GroupAggNode, done) O(GroupAggNode, use_sorted)
+  O(JoinNode, base) O(JoinNode, left) O(JoinNode, right) O(JoinNode, kind) O(JoinNode, n_keys)
+  O(JoinNode, keys) O(JoinNode, suffix_x) O(JoinNode, suffix_y)
+  S(ProjectNode) O(ProjectNode, base) O(ProjectNode, child) O(ProjectNode, n_entries) O(ProjectNode, entries)
+  E(VEC_INT64) E(VEC_DOUBLE) E(VEC_BOOL) E(VEC_STRING)
+  E(EXPR_COL_REF) E(EXPR_LIT_INT64) E(EXPR_LIT_DOUBLE) E(EXPR_LIT_BOOL) E(EXPR_LIT_STRING) E(EXPR_LIT_NA)
+  E(EXPR_ARITH) E(EXPR_CMP) E(EXPR_BOOL) E(EXPR_IS_NA) E(EXPR_IF_ELSE) E(EXPR_CAST) E(EXPR_NEGATE)
+  E(EXPR_NCHAR) E(EXPR_SUBSTR) E(EXPR_GREPL)
+  E(AGG_COUNT) E(AGG_COUNT_STAR) E(AGG_SUM) E(AGG_MEAN) E(AGG_MIN) E(AGG_MAX)
+  E(JOIN_INNER) E(JOIN_LEFT) E(JOIN_FULL) E(JOIN_SEMI) E(JOIN_ANTI)
+  return 0;
+}
+'''
+
+
+@pytest.mark.skipif(not all((REF_SRC / h).exists() for h in REF_HEADERS),
+                    reason="the reference sources are not on this machine")
+def test_abi_matches_reference_headers(tmp_path):
+    """include/vectra_abi.h against the reference's own headers (src/types.h,
+    expr.h, agg_ops.h, group_agg.h, join.h, filter.h, project.h), both
+    compiled by gcc from the same probe: every size, offset and enum value
+    the drop-in boundary shares must agree"""
+    src = tmp_path / "probe.c"
+    src.write_text(LAYOUT_PROBE)
+    out = {}
+    for tag, flags in (("ref", ["-DUSE_REF", f"-I{REF_SRC}"]), ("abi", [f"-I{ROOT / 'include'}"])):
+        exe = tmp_path / f"probe_{tag}"
+        subprocess.run(["gcc", "-std=c11", *flags, str(src), "-o", str(exe)], check=True)
+        out[tag] = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines()
+    assert len(out["ref"]) > 90
+    diff = [(a, b) for a, b in zip(out["ref"], out["abi"]) if a != b]
+    assert not diff, diff
+    assert out["ref"] == out["abi"]

@@ -429,3 +429,69 @@ def test_nan_keys_in_key_order_are_an_error(gpu, oracle, shape):
         assert np.array_equal(got["n"][0], exp["n"][0])
     finally:
         nv.vgpu().vgpu_tune_set(b"gb.part.min_groups", old)
+
+
+STAGE_AGGS = {0: [("n", -1, 0)],
+              1: [("n", -1, 0), ("sum", 0, 0), ("min", 0, 0), ("max", 0, 0)],
+              2: [("n", -1, 0), ("sum", 0, 0), ("min", 0, 0), ("max", 0, 0), ("sum", 1, 0), ("max", 1, 0)],
+              3: [("n", -1, 0), ("sum", 0, 0), ("min", 1, 0), ("sum", 2, 0), ("max", 2, 0)],
+              4: [("n", -1, 0), ("max", 0, 0), ("sum", 1, 0), ("min", 2, 0), ("sum", 3, 0)]}
+
+
+@pytest.mark.parametrize("stage", [2, 3])
+@pytest.mark.parametrize("range_mode", [2, 1])
+@pytest.mark.parametrize("n_inputs", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("valid", [False, True])
+def test_groupby_partition_staging(gpu, part_threshold, stage, range_mode, n_inputs, valid):
+    """the LDS-staged partition passes and resident aggregation blocks
+    (gb.part.stage 2 and 3: tiles of 1,024 and 2,048 rows) against the
+    row-wise ones (1): staged rows of 2 to 6 words, the plain first pass (no
+    validity) and the general one, key ranges and hashes; keys, order,
+    counts, min and max bit-exact, sums equal or within 1e-9 of
+    max(1, |sum|) (summation order), counts and keys against numpy"""
+    n = 1_300_000
+    r = rng(4000 + 10 * n_inputs + range_mode)
+    k = r.integers(-10**12, 10**12, 300_000)[r.integers(0, 300_000, n)]
+    kval = r.random(n) >= 0.01 if valid else None
+    types, datas = [], []
+    for i in range(n_inputs):
+        if i % 2 == 0:
+            types.append(gpu.VEC_DOUBLE)
+            datas.append(r.normal(size=n) * 1e3)
+        else:
+            types.append(gpu.VEC_INT64)
+            datas.append(r.integers(-10**9, 10**9, n))
+    cols = [gpu.DeviceColumn(k, kval)] + [gpu.DeviceColumn(d) for d in datas]
+    lib = gpu.vgpu()
+    old = (lib.vgpu_tune_get(b"gb.part.stage"), lib.vgpu_tune_get(b"gb.part.range"))
+    res = {}
+    try:
+        assert lib.vgpu_tune_set(b"gb.part.range", range_mode) == 0
+        for mode in (stage, 1):
+            assert lib.vgpu_tune_set(b"gb.part.stage", mode) == 0
+            gb = gpu.GroupBy(gpu.VEC_INT64, types, STAGE_AGGS[n_inputs], gpu.VGPU_ORDER_KEY, 0)
+            gb.feed(cols[0], cols[1:], n)
+            path = gb.last_path()
+            gb.finish()
+            res[mode] = gb.fetch()
+            gb.free()
+            assert path == (2 if range_mode == 2 else 1), (mode, path)
+    finally:
+        lib.vgpu_tune_set(b"gb.part.stage", old[0])
+        lib.vgpu_tune_set(b"gb.part.range", old[1])
+        for c in cols:
+            c.free()
+    kk = k if kval is None else k[kval]
+    uk, cnt = np.unique(kk, return_counts=True)
+    got, ref = res[stage], res[1]
+    ng = len(uk)
+    assert np.array_equal(got["key"][0][:ng], uk)
+    assert np.array_equal(got["a0"][0][:ng], cnt)
+    for col in got:
+        assert np.array_equal(got[col][1], ref[col][1]), col
+        a, b = got[col][0], ref[col][0]
+        if a.dtype == np.float64 and col != "key":
+            same = np.array_equal(a, b) or np.all(np.abs(a - b) <= 1e-9 * np.maximum(1.0, np.abs(b)))
+            assert same, col
+        else:
+            assert np.array_equal(a, b), col

@@ -76,6 +76,7 @@ struct GbpArgs {
     uint64_t *pstart;              // partition row starts (P + 1)
     uint32_t *nitems;              // items per partition (P)
     uint64_t *itembase;            // scanned items (P + 1)
+    uint32_t *ipart;               // partition of every item (NImax)
     int64_t NImax;
     uint32_t *icount;              // regular records per item
     uint64_t *irec;                // first record index of an item
@@ -1427,6 +1428,300 @@ __global__ void __launch_bounds__(GB_THREADS) k_gbp_scatter2(GbpArgs G) {
     }
 }
 
+// ---- LDS-staged scatter ---------------------------------------------------
+// A block moves its rows in tiles of SU * GB_THREADS: every row of a tile is
+// counted by destination bucket (an LDS counter gives its rank), the tile's
+// counts are scanned, the rows are placed in LDS in bucket order, and the
+// block then writes the tile out in that order with consecutive lanes on
+// consecutive 16-byte chunks (8-byte words for rows of an odd width), so
+// that one bucket's rows of the tile land on consecutive addresses and a
+// store instruction covers a few runs instead of one scattered row per
+// lane.  Rows keep their meta word (feed row index), so the order inside a
+// bucket is free.  Four-word rows are also read from staged rows chunk by
+// chunk (a lane pair holds one row) in the second pass.
+struct GbpStage {
+    uint64_t *rows;                 // tile x W words, row-major
+    unsigned long long *cur;        // next global row of each bucket
+    uint32_t *cnt, *tst;            // the tile's rows per bucket, their scan
+    uint16_t *bkt;                  // bucket of each placed row
+    uint32_t *scan;                 // block prefix scratch
+};
+
+__host__ __device__ __forceinline__ size_t gbp_stage_bytes(int W, int NBk, int tile) {
+    return (size_t)tile * W * 8 + (size_t)NBk * 16 + (size_t)tile * 2 + 4 * (GB_THREADS / VG_WAVE);
+}
+
+__device__ __forceinline__ GbpStage gbp_stage_carve(uint64_t *base, int W, int NBk, int tile) {
+    GbpStage S;
+    S.rows = base;
+    S.cur = reinterpret_cast<unsigned long long *>(base + (size_t)tile * W);
+    S.cnt = reinterpret_cast<uint32_t *>(S.cur + NBk);
+    S.tst = S.cnt + NBk;
+    S.bkt = reinterpret_cast<uint16_t *>(S.tst + NBk);
+    S.scan = reinterpret_cast<uint32_t *>(S.bkt + tile);
+    return S;
+}
+
+// exclusive scan of the tile's bucket counts into S.tst
+__device__ __forceinline__ void gbp_stage_scan(GbpStage &S, int NBk) {
+    const int per = (NBk + GB_THREADS - 1) / GB_THREADS;
+    uint32_t loc = 0;
+    for (int q = 0; q < per; q++) {
+        const int s = (int)threadIdx.x * per + q;
+        if (s < NBk) loc += S.cnt[s];
+    }
+    uint32_t total;
+    uint32_t run = gb_block_prefix(loc, S.scan, total);
+    for (int q = 0; q < per; q++) {
+        const int s = (int)threadIdx.x * per + q;
+        if (s < NBk) {
+            S.tst[s] = run;
+            run += S.cnt[s];
+        }
+    }
+    __syncthreads();
+}
+
+// A tile in two halves, so that a kernel can issue the next tile's loads
+// between them.  gbp_stage_place ranks rows w[u] (u < SU: row
+// u * GB_THREADS + threadIdx.x of the tile, live below n) by bucket bk[u]
+// and places them in LDS in bucket order (w is free afterwards);
+// gbp_stage_write writes the placed rows to out (rows of W words) and
+// advances the cursors.  S.cnt is zero on entry and left zero.
+template <int WM, int SU>
+__device__ __forceinline__ void gbp_stage_place(GbpStage &S, int W, int NBk, int n, const uint64_t (&w)[SU][WM],
+                                                const uint32_t (&bk)[SU]) {
+    uint32_t rank[SU];
+#pragma unroll
+    for (int u = 0; u < SU; u++)
+        if (u * GB_THREADS + (int)threadIdx.x < n) rank[u] = atomicAdd(&S.cnt[bk[u]], 1u);
+    __syncthreads();
+    gbp_stage_scan(S, NBk);
+#pragma unroll
+    for (int u = 0; u < SU; u++) {
+        if (u * GB_THREADS + (int)threadIdx.x >= n) continue;
+        const uint32_t p = S.tst[bk[u]] + rank[u];
+        S.bkt[p] = (uint16_t)bk[u];
+        uint64_t *d = S.rows + (size_t)p * W;
+        if constexpr (WM == 4) {
+            reinterpret_cast<ulonglong2 *>(d)[0] = make_ulonglong2(w[u][0], w[u][1]);
+            reinterpret_cast<ulonglong2 *>(d)[1] = make_ulonglong2(w[u][2], w[u][3]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < WM; k++)
+                if (k < W) d[k] = w[u][k];
+        }
+    }
+    __syncthreads();
+}
+
+// the same for four-word rows held as chunks: chunk c = u * GB_THREADS +
+// threadIdx.x of the tile (u < 2 SU) is half c & 1 of row c >> 1, and the
+// two lanes of a row hold its bucket
+template <int SU>
+__device__ __forceinline__ void gbp_stage_place4c(GbpStage &S, int NBk, int n, const ulonglong2 (&ch)[2 * SU],
+                                                  const uint32_t (&bk)[2 * SU]) {
+    uint32_t rank[2 * SU];
+    const bool half = threadIdx.x & 1;
+#pragma unroll
+    for (int u = 0; u < 2 * SU; u++) {
+        const int row = (u * GB_THREADS + (int)threadIdx.x) >> 1;
+        uint32_t r = 0;
+        if (row < n && !half) r = atomicAdd(&S.cnt[bk[u]], 1u);
+        const uint32_t o = __shfl_xor(r, 1, VG_WAVE);
+        rank[u] = half ? o : r;
+    }
+    __syncthreads();
+    gbp_stage_scan(S, NBk);
+#pragma unroll
+    for (int u = 0; u < 2 * SU; u++) {
+        const int row = (u * GB_THREADS + (int)threadIdx.x) >> 1;
+        if (row >= n) continue;
+        const uint32_t p = S.tst[bk[u]] + rank[u];
+        if (!half) S.bkt[p] = (uint16_t)bk[u];
+        reinterpret_cast<ulonglong2 *>(S.rows + (size_t)p * 4)[half] = ch[u];
+    }
+    __syncthreads();
+}
+
+template <int WM>
+__device__ __forceinline__ void gbp_stage_write(GbpStage &S, int W, int NBk, int n, uint64_t *out) {
+    if constexpr (WM == 4) {
+        for (int c = threadIdx.x; c < 2 * n; c += GB_THREADS) {
+            const int p = c >> 1, s = S.bkt[p];
+            const uint64_t dst = S.cur[s] + (uint64_t)(p - (int)S.tst[s]);
+            reinterpret_cast<ulonglong2 *>(out + dst * 4)[c & 1] =
+                reinterpret_cast<const ulonglong2 *>(S.rows + (size_t)p * 4)[c & 1];
+        }
+    } else if (W % 2 == 0) {
+        const int half = W >> 1;
+        for (int c = threadIdx.x; c < n * half; c += GB_THREADS) {
+            const int p = c / half, k = c - p * half, s = S.bkt[p];
+            const uint64_t dst = S.cur[s] + (uint64_t)(p - (int)S.tst[s]);
+            reinterpret_cast<ulonglong2 *>(out + dst * W)[k] =
+                reinterpret_cast<const ulonglong2 *>(S.rows + (size_t)p * W)[k];
+        }
+    } else {
+        for (int c = threadIdx.x; c < n * W; c += GB_THREADS) {
+            const int p = c / W, k = c - p * W, s = S.bkt[p];
+            const uint64_t dst = S.cur[s] + (uint64_t)(p - (int)S.tst[s]);
+            out[dst * W + k] = S.rows[(size_t)p * W + k];
+        }
+    }
+    __syncthreads();
+    for (int s = threadIdx.x; s < NBk; s += GB_THREADS) {
+        S.cur[s] += S.cnt[s];
+        S.cnt[s] = 0;
+    }
+    __syncthreads();
+}
+
+// rows of the plain first pass's tile at t0 (n rows) with their buckets
+template <int WM, int SU>
+__device__ __forceinline__ void gbp_load1(const GbLayout &L, const FeedArgs &A, const GbpArgs &G, const InSet &ins,
+                                          int64_t t0, int n, uint64_t valid_bits, uint64_t (&w)[SU][WM],
+                                          uint32_t (&bk)[SU]) {
+    const uint64_t *kp = static_cast<const uint64_t *>(A.key.data);
+    const int W = G.W;
+#pragma unroll
+    for (int u = 0; u < SU; u++) {
+        const int j = u * GB_THREADS + (int)threadIdx.x;
+        const int64_t rr = t0 + (j < n ? j : n - 1);
+        w[u][0] = kp[rr];
+#pragma unroll
+        for (int q = 1; q < WM; q++) {
+            const int i = q - 1;
+            w[u][q] = i < L.n_inputs ? static_cast<const uint64_t *>(ins.c[i].data)[rr] : 0;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < SU; u++) {
+        const int j = u * GB_THREADS + (int)threadIdx.x;
+        const int64_t rr = t0 + (j < n ? j : n - 1);
+        const uint64_t k = w[u][0];
+        const int cls = gbp_plain_class(k);
+        const uint64_t meta = (uint64_t)(uint32_t)rr | ((uint64_t)cls << 32) | valid_bits;
+#pragma unroll
+        for (int q = 1; q < WM; q++)
+            if (q == W - 1) w[u][q] = meta;
+        bk[u] = (uint32_t)(gbp_pkey(G, cls, k) >> (64 - G.B1));
+    }
+}
+
+// the staged form of the plain first pass (see k_gbp_scatter1<true>)
+template <int WM, int SU>
+__global__ void __launch_bounds__(GB_THREADS) k_gbp_scatter1s(GbLayout L, FeedArgs A, GbpArgs G, InSet ins) {
+    extern __shared__ uint64_t dyn_st[];
+    constexpr int TILE = SU * GB_THREADS;
+    const int NB = 1 << G.B1, W = G.W;
+    GbpStage S = gbp_stage_carve(dyn_st, W, NB, TILE);
+    for (int b = threadIdx.x; b < NB; b += GB_THREADS) {
+        S.cur[b] = G.O1[(int64_t)b * G.G1 + blockIdx.x];
+        S.cnt[b] = 0;
+    }
+    __syncthreads();
+    const int64_t r0 = (int64_t)blockIdx.x * G.T1;
+    const int64_t r1 = r0 + G.T1 < A.n_logical ? r0 + G.T1 : A.n_logical;
+    uint64_t valid_bits = 0;
+    for (int i = 0; i < L.n_inputs; i++) valid_bits |= 1ull << (34 + i);
+    uint64_t w[SU][WM];
+    uint32_t bk[SU];
+    auto rows_at = [&](int64_t t) { return r1 - t < TILE ? (int)(r1 - t) : TILE; };
+    if (r0 < r1) gbp_load1<WM, SU>(L, A, G, ins, r0, rows_at(r0), valid_bits, w, bk);
+    for (int64_t t0 = r0; t0 < r1; t0 += TILE) {
+        const int n = rows_at(t0);
+        gbp_stage_place<WM, SU>(S, W, NB, n, w, bk);
+        if (t0 + TILE < r1) gbp_load1<WM, SU>(L, A, G, ins, t0 + TILE, rows_at(t0 + TILE), valid_bits, w, bk);
+        gbp_stage_write<WM>(S, W, NB, n, G.S1);
+    }
+}
+
+// staged rows of the second pass's tile at t0 (n rows) with their sub-buckets
+template <int WM, int SU>
+__device__ __forceinline__ void gbp_load2(const GbpArgs &G, int64_t t0, int n, uint64_t (&w)[SU][WM],
+                                          uint32_t (&bk)[SU]) {
+    const int W = G.W;
+#pragma unroll
+    for (int u = 0; u < SU; u++) {
+        const int j = u * GB_THREADS + (int)threadIdx.x;
+        const int64_t rr = t0 + (j < n ? j : n - 1);
+        const uint64_t *src = G.S1 + rr * W;
+#pragma unroll
+        for (int q = 0; q < WM; q++) w[u][q] = q < W ? src[q] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < SU; u++) {
+        uint64_t meta = 0;
+#pragma unroll
+        for (int q = 0; q < WM; q++)
+            if (q == W - 1) meta = w[u][q];
+        const int cls = (int)((meta >> 32) & 3);
+        bk[u] = (uint32_t)((gbp_pkey(G, cls, w[u][0]) >> (64 - G.B1 - G.B2)) & ((1u << G.B2) - 1));
+    }
+}
+
+// four-word staged rows of the second pass's tile at t0 as chunks
+template <int SU>
+__device__ __forceinline__ void gbp_load2c(const GbpArgs &G, int64_t t0, int n, ulonglong2 (&ch)[2 * SU],
+                                           uint32_t (&bk)[2 * SU]) {
+    const bool half = threadIdx.x & 1;
+#pragma unroll
+    for (int u = 0; u < 2 * SU; u++) {
+        const int row = (u * GB_THREADS + (int)threadIdx.x) >> 1;
+        const int64_t rr = t0 + (row < n ? row : n - 1);
+        ch[u] = reinterpret_cast<const ulonglong2 *>(G.S1 + rr * 4)[half];
+    }
+#pragma unroll
+    for (int u = 0; u < 2 * SU; u++) {
+        // half 0 holds (key, input 0), half 1 (input 1, meta)
+        const uint64_t mine = half ? ch[u].y : ch[u].x;
+        const uint32_t lo = __shfl_xor((uint32_t)mine, 1, VG_WAVE);
+        const uint32_t hi = __shfl_xor((uint32_t)(mine >> 32), 1, VG_WAVE);
+        const uint64_t other = ((uint64_t)hi << 32) | lo;
+        const uint64_t key = half ? other : mine, meta = half ? mine : other;
+        const int cls = (int)((meta >> 32) & 3);
+        bk[u] = (uint32_t)((gbp_pkey(G, cls, key) >> (64 - G.B1 - G.B2)) & ((1u << G.B2) - 1));
+    }
+}
+
+// the staged form of the second pass (see k_gbp_scatter2)
+template <int WM, int SU>
+__global__ void __launch_bounds__(GB_THREADS) k_gbp_scatter2s(GbpArgs G) {
+    extern __shared__ uint64_t dyn_st[];
+    constexpr int TILE = SU * GB_THREADS;
+    int64_t b, chunk, nblk, r0, r1;
+    if (!gbp_block2(G, blockIdx.x, b, chunk, nblk, r0, r1)) return;
+    const int NS = 1 << G.B2, W = G.W;
+    GbpStage S = gbp_stage_carve(dyn_st, W, NS, TILE);
+    for (int i = threadIdx.x; i < NS; i += GB_THREADS) {
+        S.cur[i] = G.O2[gbp_h2_index(G, b, i, chunk, nblk)];
+        S.cnt[i] = 0;
+    }
+    __syncthreads();
+    auto rows_at = [&](int64_t t) { return r1 - t < TILE ? (int)(r1 - t) : TILE; };
+    if constexpr (WM == 4) {
+        ulonglong2 ch[2 * SU];
+        uint32_t bk[2 * SU];
+        if (r0 < r1) gbp_load2c<SU>(G, r0, rows_at(r0), ch, bk);
+        for (int64_t t0 = r0; t0 < r1; t0 += TILE) {
+            const int n = rows_at(t0);
+            gbp_stage_place4c<SU>(S, NS, n, ch, bk);
+            if (t0 + TILE < r1) gbp_load2c<SU>(G, t0 + TILE, rows_at(t0 + TILE), ch, bk);
+            gbp_stage_write<4>(S, 4, NS, n, G.S2);
+        }
+    } else {
+        uint64_t w[SU][WM];
+        uint32_t bk[SU];
+        if (r0 < r1) gbp_load2<WM, SU>(G, r0, rows_at(r0), w, bk);
+        for (int64_t t0 = r0; t0 < r1; t0 += TILE) {
+            const int n = rows_at(t0);
+            gbp_stage_place<WM, SU>(S, W, NS, n, w, bk);
+            if (t0 + TILE < r1) gbp_load2<WM, SU>(G, t0 + TILE, rows_at(t0 + TILE), w, bk);
+            gbp_stage_write<WM>(S, W, NS, n, G.S2);
+        }
+    }
+}
+
 // row start of every partition (P + 1 entries) and its item count
 __global__ void k_gbp_pstart(GbpArgs G) {
     int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1480,7 +1775,10 @@ __global__ void k_gbp_init_reserved(GbLayout L, GbpArgs G) {
         if (w < L.RH) r[w] = w == 0 ? 0 : (((L.hot_init_ones >> w) & 1) ? ~0ull : 0ull);
         else if (w < L.RH + L.RC) r[w] = ((L.cold_init_ones >> (w - L.RH)) & 1) ? ~0ull : 0ull;
     }
-    if (w == 0) G.ctl[10] = 0;
+    if (w == 0) {
+        G.ctl[10] = 0;
+        G.ctl[12] = 0;   // k_gbp_agg_loop's item ticket
+    }
 }
 
 // copies the occupied records of an LDS table (slots [0, S) regular, S at
@@ -1489,9 +1787,10 @@ __global__ void k_gbp_init_reserved(GbLayout L, GbpArgs G) {
 // table are distinct), counted against a compact copy of the keys in kc
 // whose reads are the same address across a wave.  Returns the count in
 // every thread.
-__device__ __forceinline__ uint32_t gbp_write_records(const GbLayout &L, const uint64_t *lhot, const uint64_t *lcold,
+__device__ __forceinline__ uint32_t gbp_write_records(const GbLayout &L, uint64_t *lhot, uint64_t *lcold,
                                                       int S, uint64_t empty, uint64_t *dst, int RW, uint16_t *slist,
-                                                      uint32_t *scan_lds, bool sorted, int64_t *kc) {
+                                                      uint32_t *scan_lds, bool sorted, int64_t *kc,
+                                                      bool reset = false) {
     const int per = (S + GB_THREADS - 1) / GB_THREADS;
     uint32_t mine = 0;
     for (int j = 0; j < per; j++) {
@@ -1524,9 +1823,22 @@ __device__ __forceinline__ uint32_t gbp_write_records(const GbLayout &L, const u
     for (int q = threadIdx.x; q < words; q += GB_THREADS) {
         const int k = q / RW, w = q - k * RW;
         const int s = order[k];
-        dst[q] = w < L.RH ? lhot[(size_t)s * L.RH + w] : lcold[(size_t)s * L.RC + (w - L.RH)];
+        uint64_t *src = w < L.RH ? lhot + (size_t)s * L.RH + w : lcold + (size_t)s * L.RC + (w - L.RH);
+        dst[q] = *src;
+        // reset: the word back to its initial value (one thread per word)
+        if (reset)
+            *src = w == 0 ? empty
+                   : w < L.RH ? (((L.hot_init_ones >> w) & 1) ? ~0ull : 0ull)
+                              : (((L.cold_init_ones >> (w - L.RH)) & 1) ? ~0ull : 0ull);
     }
     return total;
+}
+
+// a record of an LDS table back to its initial (empty) value
+__device__ __forceinline__ void gbp_reset_record(const GbLayout &L, uint64_t *h, uint64_t *c, uint64_t empty) {
+    h[0] = empty;
+    for (int w = 1; w < L.RH; w++) h[w] = ((L.hot_init_ones >> w) & 1) ? ~0ull : 0ull;
+    for (int w = 0; w < L.RC; w++) c[w] = ((L.cold_init_ones >> w) & 1) ? ~0ull : 0ull;
 }
 
 __device__ __forceinline__ void gbp_copy_record(const GbLayout &L, const uint64_t *h, const uint64_t *c, uint64_t *dst) {
@@ -1644,6 +1956,97 @@ __global__ void __launch_bounds__(GB_THREADS) k_gbp_agg(GbLayout L, GbpArgs G) {
         G.rflags[item] = fl;
         if (nit > 1 && chunk == 0) G.mlist[atomicAdd((unsigned long long *)&G.ctl[10], 1ull)] = (uint32_t)p;
     }
+}
+
+// The aggregation with resident blocks: a block takes items in turn (an
+// atomic ticket), reads its partition from ipart instead of searching for
+// it, and sets only the LDS words it copied out back to their initial
+// values, so the table is cleared once per block rather than once per item.
+// Same records, flags and lists as k_gbp_agg.
+__global__ void __launch_bounds__(GB_THREADS) k_gbp_agg_loop(GbLayout L, GbpArgs G) {
+    extern __shared__ uint64_t dynw[];   // 8-byte aligned after the static arrays
+    __shared__ uint16_t slist[1024];   // [0, 512) occupied slots, [512, 1024) in key order
+    __shared__ int64_t kc[512];
+    __shared__ uint32_t scan_lds[GB_THREADS / VG_WAVE];
+    __shared__ int64_t ticket;
+    const int S = G.S;
+    uint64_t *lhot = dynw;
+    uint64_t *lcold = lhot + (size_t)(S + 2) * L.RH;
+    const uint64_t empty = L.key_is_double ? VG_EMPTY_F64 : VG_EMPTY_I64;
+    gb_lds_init(L, lhot, lcold, S, empty);
+    const int64_t NI = (int64_t)G.itembase[G.P];
+    bool spilled = false;
+    for (;;) {
+        if (threadIdx.x == 0) ticket = (int64_t)atomicAdd((unsigned long long *)&G.ctl[12], 1ull);
+        __syncthreads();
+        const int64_t item = ticket;
+        if (item >= NI) break;
+        const int64_t p = G.ipart[item];
+        const int64_t i0 = (int64_t)G.itembase[p], nit = (int64_t)G.itembase[p + 1] - i0, chunk = item - i0;
+        const int64_t pe = (int64_t)G.pstart[p + 1];
+        const int64_t r0 = (int64_t)G.pstart[p] + chunk * GBP_CH;
+        const int64_t r1 = r0 + GBP_CH < pe ? r0 + GBP_CH : pe;
+        if (G.W == 4) {
+            for (int64_t r = r0 + threadIdx.x; r < r1; r += (int64_t)GB_THREADS * 4) {
+                ulonglong2 a[4], b[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int64_t rr = r + (int64_t)u * GB_THREADS;
+                    a[u] = b[u] = make_ulonglong2(0, 0);
+                    if (rr < r1) {
+                        const ulonglong2 *src = reinterpret_cast<const ulonglong2 *>(G.S2 + rr * 4);
+                        a[u] = src[0];
+                        b[u] = src[1];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    if (r + (int64_t)u * GB_THREADS >= r1) continue;
+                    const uint64_t w[4] = {a[u].x, a[u].y, b[u].x, b[u].y};
+                    gbp_agg_row(L, G, lhot, lcold, S, empty, w, spilled);
+                }
+            }
+        } else {
+            for (int64_t r = r0 + threadIdx.x; r < r1; r += GB_THREADS)
+                gbp_agg_row(L, G, lhot, lcold, S, empty, G.S2 + r * G.W, spilled);
+        }
+        __syncthreads();
+        uint32_t total = gbp_write_records(L, lhot, lcold, S, empty, G.rec + r0 * G.RW, G.RW, slist, scan_lds,
+                                           G.range != 0, kc, true);
+        if (threadIdx.x == 0) {
+            G.icount[item] = total;
+            G.irec[item] = (uint64_t)r0;
+            uint32_t fl = 0;
+            int64_t at = r0 + total;
+            for (int j = 0; j < 2; j++) {
+                uint64_t *h = lhot + (size_t)(S + j) * L.RH;
+                if (h[1] == 0) continue;
+                uint64_t *c = lcold + (size_t)(S + j) * L.RC;
+                if (nit == 1) {
+                    gbp_copy_record(L, h, c, G.rec + (G.n_rows + j) * G.RW);
+                } else {
+                    gbp_copy_record(L, h, c, G.rec + at * G.RW);
+                    at++;
+                    fl |= 1u << j;
+                }
+                gbp_reset_record(L, h, c, empty);
+            }
+            G.rflags[item] = fl;
+            if (nit > 1 && chunk == 0) G.mlist[atomicAdd((unsigned long long *)&G.ctl[10], 1ull)] = (uint32_t)p;
+        }
+        __syncthreads();
+    }
+    if (spilled) G.ctl[9] = 1;
+    for (int64_t it = NI + (int64_t)blockIdx.x * GB_THREADS + threadIdx.x; it < G.NImax;
+         it += (int64_t)gridDim.x * GB_THREADS)
+        G.icount[it] = 0;
+}
+
+// partition of every item
+__global__ void k_gbp_ipart(GbpArgs G) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= G.P) return;
+    for (uint64_t i = G.itembase[p]; i < G.itembase[p + 1]; i++) G.ipart[i] = (uint32_t)p;
 }
 
 // one block per partition with several items: merge their partial records
@@ -1817,6 +2220,7 @@ static void gbp_layout(GbpArgs &G, uint8_t *base, size_t *bytes) {
     G.pstart = c.take<uint64_t>(G.P + 1);
     G.nitems = c.take<uint32_t>(G.P);
     G.itembase = c.take<uint64_t>(G.P + 1);
+    G.ipart = c.take<uint32_t>(G.NImax);
     G.icount = c.take<uint32_t>(G.NImax);
     G.irec = c.take<uint64_t>(G.NImax);
     G.rflags = c.take<uint32_t>(G.NImax);
@@ -1896,6 +2300,21 @@ static int gbp_feed(vgpu_groupby *g, FeedArgs A, const InSet &ins, const PredInf
         plain = plain && !ins.c[i].validity && (ins.c[i].type == VGPU_INT64 || ins.c[i].type == VGPU_DOUBLE);
     auto hist1 = plain ? k_gbp_hist1<true> : k_gbp_hist1<false>;
     auto scatter1 = plain ? k_gbp_scatter1<true> : k_gbp_scatter1<false>;
+    // staged scatters (knob gb.part.stage): rows of 3, 4 or up to
+    // VGPU_MAX_INPUTS + 2 words
+    int64_t stage_mode = vg_tune_for(VG_TUNE_GB_PART_STAGE, groups);
+    // 2,048-row tiles of wide rows would not fit a CU's LDS: 1,024-row ones
+    if (stage_mode >= 3 && gbp_stage_bytes(G.W, 1 << std::max(G.B1, G.B2), 8 * GB_THREADS) > 96 * 1024) stage_mode = 2;
+    const bool staged = stage_mode >= 2;
+    const int tile = stage_mode >= 3 ? 8 * GB_THREADS : 4 * GB_THREADS;
+    auto scatter1s = stage_mode >= 3 ? (G.W == 4 ? k_gbp_scatter1s<4, 8> : G.W <= 3 ? k_gbp_scatter1s<3, 8>
+                                                                                   : k_gbp_scatter1s<VGPU_MAX_INPUTS + 2, 8>)
+                                     : (G.W == 4 ? k_gbp_scatter1s<4, 4> : G.W <= 3 ? k_gbp_scatter1s<3, 4>
+                                                                                   : k_gbp_scatter1s<VGPU_MAX_INPUTS + 2, 4>);
+    auto scatter2s = stage_mode >= 3 ? (G.W == 4 ? k_gbp_scatter2s<4, 8> : G.W <= 3 ? k_gbp_scatter2s<3, 8>
+                                                                                   : k_gbp_scatter2s<VGPU_MAX_INPUTS + 2, 8>)
+                                     : (G.W == 4 ? k_gbp_scatter2s<4, 4> : G.W <= 3 ? k_gbp_scatter2s<3, 4>
+                                                                                   : k_gbp_scatter2s<VGPU_MAX_INPUTS + 2, 4>);
     vg_kt_begin();
     for (;;) {
         hist1<<<G.G1, GB_THREADS, lds1, st>>>(A, G, pi, prog, pcs);
@@ -1912,14 +2331,20 @@ static int gbp_feed(vgpu_groupby *g, FeedArgs A, const InSet &ins, const PredInf
                 continue;
             }
         }
-        scatter1<<<G.G1, GB_THREADS, lds1, st>>>(L, A, G, ins, pi, prog, pcs);
+        if (plain && staged)
+            scatter1s<<<G.G1, GB_THREADS, gbp_stage_bytes(G.W, (int)NB, tile), st>>>(L, A, G, ins);
+        else
+            scatter1<<<G.G1, GB_THREADS, lds1, st>>>(L, A, G, ins, pi, prog, pcs);
         VG_LAUNCH_CHECK();
         k_gbp_plan2<<<1, 1024, 0, st>>>(G);
         VG_LAUNCH_CHECK();
         k_gbp_hist2<<<(unsigned)G.NB2max, GB_THREADS, 0, st>>>(G);
         VG_LAUNCH_CHECK();
         if (vg_scan_u32_to_u64(G.H2, G.O2, G.NB2max * NS, nullptr, g->sort_tmp)) return -1;
-        k_gbp_scatter2<<<(unsigned)G.NB2max, GB_THREADS, 0, st>>>(G);
+        if (staged)
+            scatter2s<<<(unsigned)G.NB2max, GB_THREADS, gbp_stage_bytes(G.W, (int)NS, tile), st>>>(G);
+        else
+            k_gbp_scatter2<<<(unsigned)G.NB2max, GB_THREADS, 0, st>>>(G);
         VG_LAUNCH_CHECK();
         k_gbp_pstart<<<vg_blocks(G.P + 1, 256), 256, 0, st>>>(G);
         VG_LAUNCH_CHECK();
@@ -1940,7 +2365,18 @@ static int gbp_feed(vgpu_groupby *g, FeedArgs A, const InSet &ins, const PredInf
     k_gbp_init_reserved<<<1, 128, 0, st>>>(L, G);
     VG_LAUNCH_CHECK();
     size_t lds = (size_t)(G.S + 2) * G.RW * 8;
-    k_gbp_agg<<<(unsigned)G.NImax, GB_THREADS, lds, st>>>(L, G);
+    if (staged) {
+        k_gbp_ipart<<<vg_blocks(G.P, 256), 256, 0, st>>>(G);
+        VG_LAUNCH_CHECK();
+        // resident blocks: as many as the LDS of a CU holds (table + the
+        // kernel's static arrays), at most 8 per CU
+        const size_t per_block = lds + 1024 * 2 + 512 * 8 + 64;
+        const int64_t bpc = std::max<int64_t>(1, std::min<int64_t>(8, (int64_t)(160 * 1024 / per_block)));
+        const int64_t grid = std::min<int64_t>(G.NImax, (int64_t)vg_cus() * bpc);
+        k_gbp_agg_loop<<<(unsigned)grid, GB_THREADS, lds, st>>>(L, G);
+    } else {
+        k_gbp_agg<<<(unsigned)G.NImax, GB_THREADS, lds, st>>>(L, G);
+    }
     VG_LAUNCH_CHECK();
     k_gbp_combine<<<(unsigned)std::min<int64_t>(G.P, n / GBP_CH + 1), GB_THREADS, lds, st>>>(L, G);
     VG_LAUNCH_CHECK();
