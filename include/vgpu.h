@@ -548,7 +548,7 @@ void    vgpu_batch_destroy(vgpu_batch *b);
  *                         of by hash, so its records come out in key order
  *                         and finish does not sort them (hash partitions
  *                         are kept when the ranges are badly balanced)
- *   "gb.part.stage"      (2) the partition passes of a partitioned
+ *   "gb.part.stage"      (3) the partition passes of a partitioned
  *                         group-by stage each tile of rows in LDS in bucket
  *                         order and write a bucket's rows of the tile as one
  *                         run, and its aggregation runs on resident blocks:
