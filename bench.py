@@ -119,6 +119,10 @@ class C2:
         self.at = nv.DeviceColumn(dim["attr"])
         self.wt = nv.DeviceColumn(dim["weight"])
 
+    def free(self):
+        for c in (self.fk, self.val, self.dk, self.at, self.wt):
+            c.free()
+
     def query(self, jt, kind):
         nv = self.nv
         gb = nv.GroupBy(nv.VEC_INT64, [nv.VEC_DOUBLE], [("sum", 0, 0), ("n", -1, 0)], nv.VGPU_ORDER_KEY, 101)
@@ -210,6 +214,7 @@ def run_c2(args, nv):
                          f"({secs:.1f} s); one thread pinned to core {hostinfo['pinned_core']} of "
                          f"{hostinfo['nproc']} ({hostinfo['cpu_model']}); {ORACLE_NOTE}",
                **hostinfo}
+    w.free()
     gbs = (2 * per_launch) / (ms / 1e3) / 1e9
     traffic = None
     tfile = ROOT / "profiles" / "k_gb_star_join_aggregate_50M.json"
@@ -645,6 +650,8 @@ def run_c3(args, nv, card):
         cpu = {"value": sample / secs, "unit": "rows/s", "cores": 1, "kind": "port",
                "sample": f"first {sample:,} rows, sort-based group-by (oracle restatement of the reference "
                          f"engine's GroupAgg over SortNode, {secs:.1f} s)"}
+    for c in (kc, c1, c2):
+        c.free()
     label = "high" if card > 16 else "low"
     return {
         "metric": METRIC, "value": n / (ms / 1e3), "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
@@ -961,6 +968,41 @@ def run_c5(args, nv):
     }
 
 
+def run_extras(args, nv):
+    """Short runs of the other device-resident workloads after the serving
+    sweep (their own figures; the serving loop's tables are released
+    first): the join-then-aggregate at 50M x 1M rows and the 200M-row
+    group-by at 50M and 16 key values, five timed steps each, no CPU
+    baseline (each has its own in its full run)."""
+    import copy
+    import gc
+    try:
+        import torch
+        gc.collect()
+        torch.cuda.empty_cache()
+    except Exception:   # noqa: BLE001 -- torch only holds the serving tables
+        pass
+    sub = copy.copy(args)
+    sub.steps, sub.warmup, sub.no_cpu = 5, 1, True
+    sub.fact_rows, sub.dim_rows, sub.c3_rows = 50_000_000, 1_000_000, 200_000_000
+    extras = {}
+    for name, fn in (("c2_join_aggregate", lambda: run_c2(sub, nv)),
+                     ("c3_high_groupby", lambda: run_c3(sub, nv, 50_000_000)),
+                     ("c3_low_groupby", lambda: run_c3(sub, nv, 16))):
+        t = time.perf_counter()
+        o = fn()
+        rf = o.get("roofline") or {}
+        extras[name] = {"value": o["value"], "unit": o["unit"], "ms_per_step": o["ms_per_step"],
+                        "steps": o["steps"], "workload": o["config"]["workload"],
+                        "roofline": {k: rf.get(k) for k in ("kernel", "achieved", "peak", "unit", "frac",
+                                                            "bytes_per_launch", "avg_launch_ms", "traffic")},
+                        "result_check": o.get("result_check"), "wall_s": time.perf_counter() - t}
+        log(f"[bench] extra {name}: {o['value']:.4g} {o['unit']}, {o['ms_per_step']:.3f} ms/step, "
+            f"roofline {rf.get('frac', 0):.3f}")
+        gc.collect()
+    return extras
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -990,6 +1032,8 @@ def main():
     ap.add_argument("--dim-rows", type=int, default=1_000_000)
     ap.add_argument("--cpu-sample", type=int, default=10_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="c5: skip the short join-aggregate and 200M-row group-by runs after the serving sweep")
     ap.add_argument("--tune", action="append", default=[], metavar="NAME=VALUE",
                     help="set a launch knob (vgpu_tune_set) before the run, e.g. batch.dense=1")
     ap.add_argument("--tune-reset", action="store_true",
@@ -1041,6 +1085,8 @@ def main():
         out = run_c2_nodes(args, nv)
     else:
         out = run_c5(args, nv)
+        if not args.no_extras:
+            out["other_workloads"] = run_extras(args, nv)
     print(json.dumps(out), flush=True)
 
 

@@ -554,6 +554,9 @@ void    vgpu_batch_destroy(vgpu_batch *b);
  *                         run, and its aggregation runs on resident blocks:
  *                         2 = tiles of 1,024 rows, 3 = of 2,048 rows, 1 =
  *                         row-wise scatters and one block per item
+ *   "gb.part.slots"      (512) LDS table slots of a partition's block in a
+ *                         partitioned group-by (64..512, a power of two);
+ *                         partitions are sized to half of it
  *   "gb.fuse_join"       (2) a GPU GroupAggNode over Project(a * b) over a
  *                         GPU JoinNode runs the three as one fused device
  *                         pass (2), or node by node (1)

@@ -1781,6 +1781,98 @@ __global__ void k_gbp_init_reserved(GbLayout L, GbpArgs G) {
     }
 }
 
+// LDS scratch of the binned key sort of an item's records
+#define GBP_SORT_BINS 1024
+struct GbpSortLds {
+    uint32_t bin[GBP_SORT_BINS + 1];    // counts, then their exclusive scan
+    uint16_t tmp[512];                  // record indexes grouped by bin
+    uint64_t red[2 * GB_THREADS / VG_WAVE];
+};
+
+// Order of n (<= 512) distinct int64 keys kc: slots src[i] into dst at the
+// rank of kc[i].  Keys are binned by a monotone function of their place in
+// [min, max] (GBP_SORT_BINS bins), the bins counted and scanned, and a key
+// ranked inside its bin by comparing it with the bin's other keys only: for
+// keys spread over their range (the records of one key-range partition)
+// a bin holds one or two, so the sort costs a few LDS operations per key
+// instead of n comparisons; any spread gives the exact order.
+__device__ __forceinline__ void gbp_sort_binned(GbpSortLds &B, const int64_t *kc, int n, const uint16_t *src,
+                                                uint16_t *dst) {
+    const uint64_t SIGN = 0x8000000000000000ull;
+    uint64_t mn = ~0ull, mx = 0;
+    for (int i = threadIdx.x; i < n; i += GB_THREADS) {
+        const uint64_t o = (uint64_t)kc[i] ^ SIGN;
+        mn = o < mn ? o : mn;
+        mx = o > mx ? o : mx;
+    }
+    for (int d = VG_WAVE / 2; d >= 1; d >>= 1) {
+        const uint64_t a = __shfl_xor(mn, d, VG_WAVE), b = __shfl_xor(mx, d, VG_WAVE);
+        mn = a < mn ? a : mn;
+        mx = b > mx ? b : mx;
+    }
+    const int wid = threadIdx.x >> 6;
+    if (vg_lane() == 0) {
+        B.red[2 * wid] = mn;
+        B.red[2 * wid + 1] = mx;
+    }
+    for (int b = threadIdx.x; b <= GBP_SORT_BINS; b += GB_THREADS) B.bin[b] = 0;
+    __syncthreads();
+    mn = B.red[0];
+    mx = B.red[1];
+    for (int w = 1; w < GB_THREADS / VG_WAVE; w++) {
+        mn = B.red[2 * w] < mn ? B.red[2 * w] : mn;
+        mx = B.red[2 * w + 1] > mx ? B.red[2 * w + 1] : mx;
+    }
+    const uint64_t span = mx - mn;
+    const int bits = span ? 64 - __builtin_clzll(span) : 0;
+    const int sh = bits > 10 ? bits - 10 : 0;   // (o - mn) >> sh < 1024 = GBP_SORT_BINS
+    constexpr int PER = (512 + GB_THREADS - 1) / GB_THREADS;
+    uint32_t bn[PER], rk[PER];
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const int i = (int)threadIdx.x + j * GB_THREADS;
+        if (i < n) {
+            bn[j] = (uint32_t)((((uint64_t)kc[i] ^ SIGN) - mn) >> sh);
+            rk[j] = atomicAdd(&B.bin[bn[j]], 1u);
+        }
+    }
+    __syncthreads();
+    // exclusive scan of the bin counts (bin[GBP_SORT_BINS] = n afterwards)
+    constexpr int BPER = GBP_SORT_BINS / GB_THREADS;
+    uint32_t c[BPER], loc = 0;
+#pragma unroll
+    for (int q = 0; q < BPER; q++) {
+        c[q] = B.bin[threadIdx.x * BPER + q];
+        loc += c[q];
+    }
+    uint32_t total;
+    uint32_t run = gb_block_prefix(loc, reinterpret_cast<uint32_t *>(B.red), total);
+#pragma unroll
+    for (int q = 0; q < BPER; q++) {
+        B.bin[threadIdx.x * BPER + q] = run;
+        run += c[q];
+    }
+    if (threadIdx.x == 0) B.bin[GBP_SORT_BINS] = total;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const int i = (int)threadIdx.x + j * GB_THREADS;
+        if (i < n) B.tmp[B.bin[bn[j]] + rk[j]] = (uint16_t)i;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const int i = (int)threadIdx.x + j * GB_THREADS;
+        if (i >= n) continue;
+        const int64_t key = kc[i];
+        const uint32_t b0 = B.bin[bn[j]], b1 = B.bin[bn[j] + 1];
+        uint32_t r = b0;
+        for (uint32_t q = b0; q < b1; q++) r += kc[B.tmp[q]] < key;
+        dst[r] = src[i];
+    }
+    __syncthreads();
+}
+
 // copies the occupied records of an LDS table (slots [0, S) regular, S at
 // most 512) to dst, compacted, with coalesced stores; sorted: in ascending
 // order of their int64 keys, each record placed at its rank (the keys of a
@@ -1790,7 +1882,7 @@ __global__ void k_gbp_init_reserved(GbLayout L, GbpArgs G) {
 __device__ __forceinline__ uint32_t gbp_write_records(const GbLayout &L, uint64_t *lhot, uint64_t *lcold,
                                                       int S, uint64_t empty, uint64_t *dst, int RW, uint16_t *slist,
                                                       uint32_t *scan_lds, bool sorted, int64_t *kc,
-                                                      bool reset = false) {
+                                                      bool reset = false, GbpSortLds *bs = nullptr) {
     const int per = (S + GB_THREADS - 1) / GB_THREADS;
     uint32_t mine = 0;
     for (int j = 0; j < per; j++) {
@@ -1809,19 +1901,26 @@ __device__ __forceinline__ uint32_t gbp_write_records(const GbLayout &L, uint64_
         const int n = (int)total;
         for (int i = threadIdx.x; i < n; i += GB_THREADS) kc[i] = (int64_t)lhot[(size_t)slist[i] * L.RH];
         __syncthreads();
-        for (int i = threadIdx.x; i < n; i += GB_THREADS) {
-            const int64_t key = kc[i];
-            int rank = 0;
+        if (bs) {
+            gbp_sort_binned(*bs, kc, n, slist, slist + 512);
+        } else {
+            for (int i = threadIdx.x; i < n; i += GB_THREADS) {
+                const int64_t key = kc[i];
+                int rank = 0;
 #pragma unroll 8
-            for (int j = 0; j < n; j++) rank += kc[j] < key;
-            slist[512 + rank] = slist[i];
+                for (int j = 0; j < n; j++) rank += kc[j] < key;
+                slist[512 + rank] = slist[i];
+            }
+            __syncthreads();
         }
-        __syncthreads();
         order = slist + 512;
     }
+    // word q of the output is word q % RW of record q / RW (the quotient by
+    // a multiply: exact for q < 2^32 / RW)
+    const uint32_t mag = (uint32_t)((0xFFFFFFFFull + (uint64_t)RW) / (uint64_t)RW);
     const int words = (int)total * RW;
     for (int q = threadIdx.x; q < words; q += GB_THREADS) {
-        const int k = q / RW, w = q - k * RW;
+        const int k = (int)__umulhi((uint32_t)q, mag), w = q - k * RW;
         const int s = order[k];
         uint64_t *src = w < L.RH ? lhot + (size_t)s * L.RH + w : lcold + (size_t)s * L.RC + (w - L.RH);
         dst[q] = *src;
@@ -1969,6 +2068,7 @@ __global__ void __launch_bounds__(GB_THREADS) k_gbp_agg_loop(GbLayout L, GbpArgs
     __shared__ int64_t kc[512];
     __shared__ uint32_t scan_lds[GB_THREADS / VG_WAVE];
     __shared__ int64_t ticket;
+    __shared__ GbpSortLds bsort;
     const int S = G.S;
     uint64_t *lhot = dynw;
     uint64_t *lcold = lhot + (size_t)(S + 2) * L.RH;
@@ -2012,7 +2112,7 @@ __global__ void __launch_bounds__(GB_THREADS) k_gbp_agg_loop(GbLayout L, GbpArgs
         }
         __syncthreads();
         uint32_t total = gbp_write_records(L, lhot, lcold, S, empty, G.rec + r0 * G.RW, G.RW, slist, scan_lds,
-                                           G.range != 0, kc, true);
+                                           G.range != 0, kc, true, &bsort);
         if (threadIdx.x == 0) {
             G.icount[item] = total;
             G.irec[item] = (uint64_t)r0;
@@ -2181,7 +2281,7 @@ k_gbp_flush(GbLayout L, const uint64_t *rec, int RW, const uint32_t *icount, con
 
 static int gbp_lds_slots(const vgpu_groupby *g) {
     size_t rw = (size_t)(g->L.RH + g->L.RC) * 8;
-    int S = 512;
+    int S = (int)vg_tune(VG_TUNE_GB_PART_SLOTS);
     while (S >= 64 && (size_t)(S + 2) * rw > 61440) S >>= 1;
     return S >= 64 ? S : 0;
 }
@@ -2370,7 +2470,7 @@ static int gbp_feed(vgpu_groupby *g, FeedArgs A, const InSet &ins, const PredInf
         VG_LAUNCH_CHECK();
         // resident blocks: as many as the LDS of a CU holds (table + the
         // kernel's static arrays), at most 8 per CU
-        const size_t per_block = lds + 1024 * 2 + 512 * 8 + 64;
+        const size_t per_block = lds + 1024 * 2 + 512 * 8 + 64 + sizeof(GbpSortLds);
         const int64_t bpc = std::max<int64_t>(1, std::min<int64_t>(8, (int64_t)(160 * 1024 / per_block)));
         const int64_t grid = std::min<int64_t>(G.NImax, (int64_t)vg_cus() * bpc);
         k_gbp_agg_loop<<<(unsigned)grid, GB_THREADS, lds, st>>>(L, G);
