@@ -515,8 +515,10 @@ void    vgpu_batch_destroy(vgpu_batch *b);
  *                         the common star shape (unique build keys, an
  *                         int64 / bool payload group key of a short range)
  *                         aggregate into a dense LDS table
- *   "join.bloom"         (2: on; 1: off) those feeds test the join table's
- *                         filter of build keys before loading an entry
+ *   "join.bloom"         (1: off; 2: on) those feeds test the join table's
+ *                         filter of build keys before loading an entry (one
+ *                         more dependent random load per probe row: it pays
+ *                         only when most probe keys are absent)
  *   "gb.lds.copies"      (2: at most this many copies of an LDS table of up
  *                         to 256 slots, lane l updating copy l % copies, as
  *                         the LDS budget allows; 1 = one table)
@@ -557,6 +559,9 @@ void    vgpu_batch_destroy(vgpu_batch *b);
  *   "gb.part.slots"      (512) LDS table slots of a partition's block in a
  *                         partitioned group-by (64..512, a power of two);
  *                         partitions are sized to half of it
+ *   "gb.feed.prefetch"   (2: on; 1: off) the general LDS-table feed loads
+ *                         a tile's key and input words (up to four inputs)
+ *                         before it applies the first row
  *   "gb.fuse_join"       (2) a GPU GroupAggNode over Project(a * b) over a
  *                         GPU JoinNode runs the three as one fused device
  *                         pass (2), or node by node (1)

@@ -84,6 +84,22 @@ def test_groupby_matches_oracle(gpu, oracle, temp_dir, key_kind, card, rg):
     check(gpu, mk, ["g"], ALL_AGGS, temp_dir)
 
 
+@pytest.mark.parametrize("prefetch", [1, 2])
+@pytest.mark.parametrize("key_kind,card", [("int", 40), ("dbl", 8)])
+def test_groupby_feed_prefetch(gpu, oracle, prefetch, key_kind, card):
+    """the general feed with a tile's words loaded before its rows are
+    applied (gb.feed.prefetch 2) and row by row (1), against the oracle,
+    after a filter (NA keys and inputs, NaN and signed zeros)"""
+    lib = gpu.vgpu()
+    old = lib.vgpu_tune_get(b"gb.feed.prefetch")
+    assert lib.vgpu_tune_set(b"gb.feed.prefetch", prefetch) == 0
+    try:
+        mk = lambda: table(gpu, 60000, 17, key_kind, card, 7001)
+        check(gpu, mk, ["g"], ALL_AGGS, None, pred=("cmp", ">", ("col", "ax"), ("dbl", 0.2)))
+    finally:
+        lib.vgpu_tune_set(b"gb.feed.prefetch", old)
+
+
 # n / count / sum / mean of at most two inputs: the narrow feed kernel
 NARROW_AGGS = [("n", "n", None, 0), ("cx", "count", "x", 0), ("sx", "sum", "x", 0), ("sx_rm", "sum", "x", 1),
                ("mx_rm", "mean", "x", 1), ("sv", "sum", "v", 1), ("mv", "mean", "v", 0)]

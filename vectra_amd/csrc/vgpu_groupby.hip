@@ -124,6 +124,11 @@ struct vgpu_groupby {
 
 #define GB_U 4
 
+// PF > 0: feeds of at most PF inputs load every row's key and input words
+// (and validity) of a tile before the first row's slot is looked up, so a
+// wave has 4 x (1 + inputs) loads in flight instead of one row's at a time
+// behind its LDS probe
+template <int PF>
 __global__ void __launch_bounds__(GB_THREADS)
 k_gb_feed(GbLayout L, FeedArgs A, InSet ins, PredInfo pi, vgpu_prog prog, ColSet pcs) {
     extern __shared__ uint8_t dyn[];
@@ -162,11 +167,36 @@ k_gb_feed(GbLayout L, FeedArgs A, InSet ins, PredInfo pi, vgpu_prog prog, ColSet
         }
 #pragma unroll
         for (int u = 0; u < GB_U; u++) ok[u] = ok[u] && vg_pred_row(pi, prog, pcs, row[u], regs, regv, GB_THREADS, tid);
+        // PF: the words of every row first (rows past the end or dropped by
+        // the predicate read row 0 and are not applied), then the classes
+        uint64_t kraw[GB_U];
+        bool kval[GB_U];
+        uint64_t ib[GB_U][PF > 0 ? PF : 1];
+        bool iv[GB_U][PF > 0 ? PF : 1];
+        if (PF > 0) {
+#pragma unroll
+            for (int u = 0; u < GB_U; u++) {
+                const int64_t r = ok[u] ? row[u] : 0;
+                kval[u] = !L.key_none && vg_valid(A.key.validity, r);
+                kraw[u] = L.key_none ? 0 : vg_load_bits(A.key, r);
+#pragma unroll
+                for (int i = 0; i < (PF > 0 ? PF : 1); i++) {
+                    if (i < L.n_inputs) {
+                        iv[u][i] = vg_valid(ins.c[i].validity, r);
+                        ib[u][i] = vg_load_bits(ins.c[i], r);
+                    }
+                }
+            }
+        }
 #pragma unroll
         for (int u = 0; u < GB_U; u++) {
             kn[u] = 0;
             cls[u] = VG_KEY_REGULAR;
-            if (ok[u] && !L.key_none) cls[u] = vg_key_class(A.key, row[u], kn[u]);
+            if (PF > 0) {
+                if (ok[u] && !L.key_none) cls[u] = vg_key_class_bits(A.key.type, kval[u], kval[u] ? kraw[u] : 0, kn[u]);
+            } else {
+                if (ok[u] && !L.key_none) cls[u] = vg_key_class(A.key, row[u], kn[u]);
+            }
         }
 #pragma unroll
         for (int u = 0; u < GB_U; u++) {
@@ -175,12 +205,29 @@ k_gb_feed(GbLayout L, FeedArgs A, InSet ins, PredInfo pi, vgpu_prog prog, ColSet
             if (L.key_is_double && cls[u] == VG_KEY_REGULAR && kn[u] == 0)
                 gb_note_zero_key(A.ctl, vg_load_bits(A.key, row[u]), pos);
             int64_t ls = S ? gb_lds_slot(lhot, S, L.RH, cls[u], kn[u], empty) : -1;
+            uint64_t *h, *c;
             if (ls >= 0) {
-                gb_apply(L, ins, row[u], pos, lhot + (size_t)ls * L.RH, lcold ? lcold + (size_t)ls * L.RC : nullptr);
+                h = lhot + (size_t)ls * L.RH;
+                c = lcold ? lcold + (size_t)ls * L.RC : nullptr;
             } else {
                 int64_t gs = gb_global_slot(L, A.hot, A.cap, cls[u], kn[u], pos, A.ctl);
-                if (gs >= 0)
-                    gb_apply(L, ins, row[u], pos, A.hot + (size_t)gs * L.RH, A.cold + (size_t)gs * L.RC);
+                if (gs < 0) continue;
+                h = A.hot + (size_t)gs * L.RH;
+                c = A.cold + (size_t)gs * L.RC;
+            }
+            if (PF > 0) {
+                gb_update_row(L, h, pos);
+#pragma unroll
+                for (int i = 0; i < (PF > 0 ? PF : 1); i++) {
+                    if (i < L.n_inputs) {
+                        InVal v;
+                        v.valid = iv[u][i];
+                        v.bits = iv[u][i] ? ib[u][i] : 0;
+                        gb_update_input(L, h, c, i, pos, v);
+                    }
+                }
+            } else {
+                gb_apply(L, ins, row[u], pos, h, c);
             }
         }
     }
@@ -2868,7 +2915,8 @@ static int groupby_feed_body(vgpu_groupby *g, const vgpu_col *key, const vgpu_co
     }
     size_t lds = vm + table * A.lds_copies;
     vg_kt_begin();
-    k_gb_feed<<<gb_grid(n_logical, A.lds_slots, lds, gb_groups(g)), GB_THREADS, lds, vg_stream()>>>(g->L, A, ins, pi, prog, pcs);
+    auto feed = g->L.n_inputs <= 4 && vg_tune(VG_TUNE_GB_FEED_PF) >= 2 ? k_gb_feed<4> : k_gb_feed<0>;
+    feed<<<gb_grid(n_logical, A.lds_slots, lds, gb_groups(g)), GB_THREADS, lds, vg_stream()>>>(g->L, A, ins, pi, prog, pcs);
     VG_LAUNCH_CHECK();
     vg_kt_end();
     g->any_fed = 1;
