@@ -562,6 +562,10 @@ void    vgpu_batch_destroy(vgpu_batch *b);
  *   "gb.feed.prefetch"   (2: on; 1: off) the general LDS-table feed loads
  *                         a tile's key and input words (up to four inputs)
  *                         before it applies the first row
+ *   "gb.merge.partials"  (2: on; 1: off) blocks of a dense-table feed store
+ *                         their tables as partials that one block per record
+ *                         folds, instead of merging into the HBM table with
+ *                         device-scope atomics on the same few records
  *   "gb.fuse_join"       (2) a GPU GroupAggNode over Project(a * b) over a
  *                         GPU JoinNode runs the three as one fused device
  *                         pass (2), or node by node (1)

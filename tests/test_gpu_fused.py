@@ -74,7 +74,8 @@ def test_fused_dense_group_table(gpu, oracle, kind, temp_dir, n_fact):
     """the dense-LDS star kernel (unique build keys, int64 payload group key
     of a short range) against the oracle and against the general kernel
     (knob gb.star.dense = 1), with and without the join table's key filter
-    (knob join.bloom): NA group keys and NA weights in the dimension,
+    (knob join.bloom), its blocks merged through partials and by HBM
+    atomics (knob gb.merge.partials): NA group keys and NA weights in the dimension,
     NA and INT64_MIN probe keys, first-seen and key order, a feed of more
     than 2^20 rows"""
     nv = gpu
@@ -101,10 +102,11 @@ def test_fused_dense_group_table(gpu, oracle, kind, temp_dir, n_fact):
     order = nv.VGPU_ORDER_KEY if temp_dir else nv.VGPU_ORDER_FIRST_SEEN
     lib = nv.vgpu()
     outs = []
-    for dense, bloom in ((2, 2), (2, 1), (1, 2)):
-        old = [lib.vgpu_tune_get(k) for k in (b"gb.star.dense", b"join.bloom")]
-        lib.vgpu_tune_set(b"gb.star.dense", dense)
-        lib.vgpu_tune_set(b"join.bloom", bloom)
+    knobs = (b"gb.star.dense", b"join.bloom", b"gb.merge.partials")
+    for setting in ((2, 2, 2), (2, 1, 2), (2, 1, 1), (1, 2, 2)):
+        old = [lib.vgpu_tune_get(k) for k in knobs]
+        for k, v in zip(knobs, setting):
+            assert lib.vgpu_tune_set(k, v) == 0
         try:
             gb = nv.GroupBy(nv.VEC_INT64, [nv.VEC_DOUBLE], [("sum", 0, 0), ("n", -1, 0), ("count", 0, 0),
                                                               ("mean", 0, 1)], order, 101)
@@ -114,7 +116,7 @@ def test_fused_dense_group_table(gpu, oracle, kind, temp_dir, n_fact):
             got = gb.fetch()
             gb.free()
         finally:
-            for k, v in zip((b"gb.star.dense", b"join.bloom"), old):
+            for k, v in zip(knobs, old):
                 lib.vgpu_tune_set(k, v)
         got = {"attr": got["key"], "s": got["a0"], "n": got["a1"], "c": got["a2"], "m": got["a3"]}
         cnt = np.maximum(exp["c"][0], 1)

@@ -111,6 +111,8 @@ struct vgpu_groupby {
     DevBuf sort_keys, sort_vals, sort_tmp, cmp;
     // partitioned feed: staged rows, bookkeeping and the run's records
     DevBuf gbp_s1, gbp_s2, gbp_aux, run_rec;
+    // per-block partials of dense LDS tables
+    DevBuf part;
     GbpArgs gbp;
     int32_t run = 0;          // the records of a partitioned feed await finish
     // the finished result in one buffer: keys, key validity, then per
@@ -722,6 +724,48 @@ k_gb_star(GbLayout L, StarArgs A, JView jv, PredInfo pi, ColSet pcs) {
     if (S) gb_lds_merge(L, lhot, lcold, S, empty_g, A.hot, A.cold, A.cap, A.ctl);
 }
 
+// Dense LDS tables (record s of every block holds the same key) merge
+// through per-block partials instead of HBM atomics: every block stores its
+// S + 2 records (RH + RC words each) at part + blockIdx.x * (S + 2) * (RH +
+// RC), and k_gb_part_reduce folds record s of all blocks in one block per
+// record, which then merges it into the HBM table once.  Without this every
+// block's merge is a device-scope atomic on the same few records, which
+// the memory side serializes.
+__device__ __forceinline__ void gb_lds_store_partial(const uint64_t *lhot, const uint64_t *lcold, int S, int RH, int RC,
+                                                     uint64_t *part) {
+    const int RW = RH + RC;
+    const int words = (S + 2) * RW;
+    uint64_t *dst = part + (size_t)blockIdx.x * words;
+    for (int q = threadIdx.x; q < words; q += GB_THREADS) {
+        const int s = q / RW, w = q - s * RW;
+        dst[q] = w < RH ? lhot[(size_t)s * RH + w] : lcold[(size_t)s * RC + (w - RH)];
+    }
+}
+
+__global__ void __launch_bounds__(GB_THREADS) k_gb_part_reduce(GbLayout L, const uint64_t *part, int64_t nparts,
+                                                               int S, uint64_t *hot, uint64_t *cold, int64_t cap,
+                                                               uint64_t *ctl) {
+    __shared__ uint64_t rec[2 * GB_MAXW];
+    const int s = blockIdx.x, RW = L.RH + L.RC;
+    const uint64_t *first = part + (size_t)s * RW;
+    if ((int)threadIdx.x < RW) {
+        const int w = threadIdx.x;
+        rec[w] = w == 0 ? first[0]
+                 : w < L.RH ? (((L.hot_init_ones >> w) & 1) ? ~0ull : 0ull)
+                            : (((L.cold_init_ones >> (w - L.RH)) & 1) ? ~0ull : 0ull);
+    }
+    __syncthreads();
+    for (int64_t b = threadIdx.x; b < nparts; b += GB_THREADS) {
+        const uint64_t *p = part + ((size_t)b * (S + 2) + s) * RW;
+        if (p[1] == 0) continue;
+        gb_merge(L, rec, rec + L.RH, p, p + L.RH);
+    }
+    __syncthreads();
+    if (threadIdx.x != 0 || rec[1] == 0) return;
+    const int64_t gs = s < S ? gb_global_slot(L, hot, cap, VG_KEY_REGULAR, rec[0], 0, ctl) : cap + (s - S);
+    if (gs >= 0) gb_merge(L, hot + (size_t)gs * L.RH, cold + (size_t)gs * L.RC, rec, rec + L.RH);
+}
+
 // The star shape of most join -> aggregate queries, with as little as
 // possible kept per row: a unique-key join table whose 32-byte entries hold
 // the payload, the group key an int64 / bool payload column over a short
@@ -739,6 +783,7 @@ struct StarDenseArgs {
     int64_t n_logical, pos_base;
     int64_t kmin;           // group key of dense record 0
     uint64_t *hot, *cold, *ctl;
+    uint64_t *part;         // per-block copies of the dense table (nullptr: merge into HBM directly)
     int64_t cap;
     int32_t S;              // dense records; the NA group at S, the EMPTY-valued key at S + 1
     int32_t RH, RC;
@@ -928,7 +973,13 @@ __global__ void __launch_bounds__(GB_THREADS) k_gb_star_dense(GbLayout L, StarDe
             star_dense_step<ITEMS, IN_MODE, LEFT>(L, A, lhot, lcold, base, tid, live, cls, kn, pv_ok, pv);
         }
     }
-    gb_lds_merge(L, lhot, lcold, A.S, VG_EMPTY_I64, A.hot, A.cold, A.cap, A.ctl);
+    if (A.part) {
+        // the block's table as one partial, reduced by k_gb_part_reduce
+        __syncthreads();
+        gb_lds_store_partial(lhot, lcold, A.S, A.RH, A.RC, A.part);
+    } else {
+        gb_lds_merge(L, lhot, lcold, A.S, VG_EMPTY_I64, A.hot, A.cold, A.cap, A.ctl);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -3223,6 +3274,12 @@ static int groupby_feed_join_body(vgpu_groupby *g, const vgpu_feed_spec *sp) {
         const bool plain = !sd.pkey.validity && sd.pkey.type != VGPU_BOOL && sd.pval.data && !sd.pval.validity &&
                            sd.pval.type != VGPU_BOOL;
         hipStream_t ks = vg_stream();
+        // block partials reduced per record (knob gb.merge.partials)
+        sd.part = nullptr;
+        if (vg_tune(VG_TUNE_GB_MERGE_PARTIALS) >= 2) {
+            if (g->part.reserve((size_t)grid * (sd.S + 2) * (sd.RH + sd.RC) * 8)) return -1;
+            sd.part = g->part.as<uint64_t>();
+        }
         vg_kt_begin();
 #define STAR_DENSE(I, M, LF)                                                                \
     do {                                                                                    \
@@ -3238,6 +3295,11 @@ static int groupby_feed_join_body(vgpu_groupby *g, const vgpu_feed_spec *sp) {
         }
 #undef STAR_DENSE
         VG_LAUNCH_CHECK();
+        if (sd.part) {
+            k_gb_part_reduce<<<(unsigned)(sd.S + 2), GB_THREADS, 0, ks>>>(g->L, sd.part, (int64_t)grid, sd.S, sd.hot,
+                                                                          sd.cold, sd.cap, sd.ctl);
+            VG_LAUNCH_CHECK();
+        }
         vg_kt_end();
         g->any_fed = 1;
         return 0;
@@ -3491,6 +3553,7 @@ extern "C" void vgpu_groupby_destroy(vgpu_groupby *g) {
     g->gbp_s2.release();
     g->gbp_aux.release();
     g->run_rec.release();
+    g->part.release();
     g->out.release();
     delete g;
 }
