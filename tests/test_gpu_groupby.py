@@ -84,20 +84,41 @@ def test_groupby_matches_oracle(gpu, oracle, temp_dir, key_kind, card, rg):
     check(gpu, mk, ["g"], ALL_AGGS, temp_dir)
 
 
-@pytest.mark.parametrize("prefetch", [1, 2])
-@pytest.mark.parametrize("key_kind,card", [("int", 40), ("dbl", 8)])
-def test_groupby_feed_prefetch(gpu, oracle, prefetch, key_kind, card):
+@pytest.mark.parametrize("prefetch,partials", [(1, 2), (2, 2), (2, 1)])
+@pytest.mark.parametrize("key_kind,card,n", [("int", 40, 60000), ("dbl", 8, 60000), ("int", 300, 2_000_000)])
+def test_groupby_feed_prefetch(gpu, oracle, prefetch, partials, key_kind, card, n):
     """the general feed with a tile's words loaded before its rows are
-    applied (gb.feed.prefetch 2) and row by row (1), against the oracle,
-    after a filter (NA keys and inputs, NaN and signed zeros)"""
+    applied (gb.feed.prefetch 2) and row by row (1), its blocks merged
+    through folded partials (gb.merge.partials 2) and by HBM atomics (1),
+    against the oracle, after a filter (NA keys and inputs, NaN and signed
+    zeros)"""
     lib = gpu.vgpu()
-    old = lib.vgpu_tune_get(b"gb.feed.prefetch")
-    assert lib.vgpu_tune_set(b"gb.feed.prefetch", prefetch) == 0
+    knobs = (b"gb.feed.prefetch", b"gb.merge.partials")
+    old = [lib.vgpu_tune_get(k) for k in knobs]
+    for k, v in zip(knobs, (prefetch, partials)):
+        assert lib.vgpu_tune_set(k, v) == 0
     try:
-        mk = lambda: table(gpu, 60000, 17, key_kind, card, 7001)
+        mk = lambda: table(gpu, n, 17, key_kind, card, 0 if n > 100000 else 7001)
         check(gpu, mk, ["g"], ALL_AGGS, None, pred=("cmp", ">", ("col", "ax"), ("dbl", 0.2)))
     finally:
-        lib.vgpu_tune_set(b"gb.feed.prefetch", old)
+        for k, v in zip(knobs, old):
+            lib.vgpu_tune_set(k, v)
+
+
+@pytest.mark.parametrize("partials", [1, 2])
+@pytest.mark.parametrize("key_kind,card", [("int", 40), ("int", 3000), ("dbl", 16)])
+def test_groupby_narrow_partials(gpu, oracle, partials, key_kind, card):
+    """the narrow feed (dense and hashed LDS tables) with its blocks merged
+    through folded partials and by HBM atomics, against the oracle, one
+    feed of 2M rows (enough blocks for the partials)"""
+    lib = gpu.vgpu()
+    old = lib.vgpu_tune_get(b"gb.merge.partials")
+    assert lib.vgpu_tune_set(b"gb.merge.partials", partials) == 0
+    try:
+        mk = lambda: table(gpu, 2_000_000, 23, key_kind, card, 0)
+        check(gpu, mk, ["g"], NARROW_AGGS, None)
+    finally:
+        lib.vgpu_tune_set(b"gb.merge.partials", old)
 
 
 # n / count / sum / mean of at most two inputs: the narrow feed kernel

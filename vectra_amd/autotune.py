@@ -15,8 +15,9 @@ is hash-partitioned, rows per block of the HBM-table feed and the HBM
 table's load factor.  It times the feed kernels with HIP events and keeps
 the fastest setting per (width, rows, cardinality) bucket.  Join tables:
 the load factor of a 1M-key and a 10M-key build, timed over build and a
-20M / 50M-row fused probe -> aggregate; and the rows in flight per thread
-of that probe.
+20M / 50M-row fused probe -> aggregate whose keys follow a star schema's
+skewed foreign keys (Zipf(1.1) over the dimension, 10% absent); and the
+rows in flight per thread and blocks per CU of that probe.
 
     python -m vectra_amd.autotune --out vectra_amd/tuning.txt
 
@@ -115,6 +116,26 @@ def tune_groupby(card: int, rows: int, grid: dict, log) -> dict:
     return {s: min(((k, ms[s]) for k, ms in results), key=lambda r: r[1]) for s in SHAPES}
 
 
+def star_probe_keys(dk, n_fact: int, g, s: float = 1.1, absent: float = 0.1):
+    """probe keys of a star schema's fact: Zipf(s) over the dimension's keys
+    in a random rank order (inverse CDF by search), a fraction `absent`
+    replaced by keys the dimension does not hold (dk * 2 + 1 never equals an
+    even dimension key: the callers' keys are all even)"""
+    import torch
+    n_dim = dk.numel()
+    w = torch.arange(1, n_dim + 1, dtype=torch.float64, device="cuda").pow(-s)
+    cdf = torch.cumsum(w, 0)
+    cdf /= cdf[-1].clone()
+    ranks = torch.searchsorted(cdf, torch.rand(n_fact, dtype=torch.float64, device="cuda", generator=g))
+    ranks.clamp_(max=n_dim - 1)
+    order = torch.randperm(n_dim, device="cuda", generator=g)
+    fk = dk[order[ranks]]
+    miss = torch.rand(n_fact, device="cuda", generator=g) < absent
+    fk[miss] = fk[miss] * 2 + 1
+    del w, cdf, ranks, order, miss
+    return fk
+
+
 def tune_join_load(log) -> list[tuple[int | None, int, float]]:
     """(rows bound, best join.load_pct, ms) for a 1M-key and a 10M-key build"""
     import torch
@@ -123,15 +144,15 @@ def tune_join_load(log) -> list[tuple[int | None, int, float]]:
     for bound, n_dim, n_fact in ((2_000_000, 1_000_000, 20_000_000), (None, 10_000_000, 50_000_000)):
         g = torch.Generator(device="cuda")
         g.manual_seed(n_dim)
-        dk = torch.randperm(n_dim, device="cuda", generator=g).to(torch.int64) * 7 + 3
+        dk = torch.randperm(n_dim, device="cuda", generator=g).to(torch.int64) * 14 + 6
         attr = torch.randint(0, 256, (n_dim,), dtype=torch.int64, device="cuda", generator=g)
-        fk = dk[torch.randint(0, n_dim, (n_fact,), device="cuda", generator=g)]
+        fk = star_probe_keys(dk, n_fact, g)
         v = torch.rand(n_fact, dtype=torch.float64, device="cuda", generator=g)
         torch.cuda.synchronize()
         wrap = lambda t, ty: nv.DeviceColumn.wrap(t.data_ptr(), t.numel(), ty, keep=t)
         dkc, atc, fkc, vc = wrap(dk, nv.VEC_INT64), wrap(attr, nv.VEC_INT64), wrap(fk, nv.VEC_INT64), wrap(v, nv.VEC_DOUBLE)
         best = None
-        for pct in (30, 50, 70):
+        for pct in (25, 30, 50, 70):
             lib.vgpu_tune_set(b"join.load_pct", pct)
             times = []
             for _ in range(3):
@@ -162,10 +183,10 @@ def tune_star(log) -> dict:
     n_dim, n_fact = 1_000_000, 20_000_000
     g = torch.Generator(device="cuda")
     g.manual_seed(7)
-    dk = torch.randperm(n_dim, device="cuda", generator=g).to(torch.int64) * 3 + 1
+    dk = torch.randperm(n_dim, device="cuda", generator=g).to(torch.int64) * 6 + 2
     attr = torch.randint(0, 100, (n_dim,), dtype=torch.int64, device="cuda", generator=g)
     w = torch.rand(n_dim, dtype=torch.float64, device="cuda", generator=g)
-    fk = dk[torch.randint(0, n_dim, (n_fact,), device="cuda", generator=g)]
+    fk = star_probe_keys(dk, n_fact, g)
     v = torch.rand(n_fact, dtype=torch.float64, device="cuda", generator=g)
     torch.cuda.synchronize()
     wrap = lambda t, ty: nv.DeviceColumn.wrap(t.data_ptr(), t.numel(), ty, keep=t)
@@ -181,15 +202,16 @@ def tune_star(log) -> dict:
         gb.free()
 
     best = None
-    for items in (1, 2, 4, 8):
+    for items, bpc in itertools.product((1, 2, 4, 8), (2, 4, 8)):
         lib.vgpu_tune_set(b"gb.star.items", items)
+        lib.vgpu_tune_set(b"gb.star.blocks_per_cu", bpc)
         ms = _feed_ms(run)
-        log(f"  star: gb.star.items={items} -> {ms:.3f} ms")
+        log(f"  star: gb.star.items={items} gb.star.blocks_per_cu={bpc} -> {ms:.3f} ms")
         if best is None or ms < best[0]:
-            best = (ms, items)
+            best = (ms, items, bpc)
     lib.vgpu_tune_reset()
     jt.free()
-    return {"ms": best[0], "knobs": {"gb.star.items": best[1]}}
+    return {"ms": best[0], "knobs": {"gb.star.items": best[1], "gb.star.blocks_per_cu": best[2]}}
 
 
 def main(argv=None) -> int:
@@ -230,7 +252,8 @@ def main(argv=None) -> int:
         tail = f" rows<={bound}" if bound else ""
         lines.append(f"{ANY} join.load_pct {pct}{tail}   # build + probe: {ms:.3f} ms")
     r = tune_star(log)
-    lines.append(f"128 gb.star.items {r['knobs']['gb.star.items']}   # join -> group_by(attr): {r['ms']:.3f} ms")
+    for name, value in r["knobs"].items():
+        lines.append(f"128 {name} {value}   # join -> group_by(attr): {r['ms']:.3f} ms")
     head = [f"# launch shapes and table loads per (width, rows, cardinality) bucket, measured by "
             f"vectra_amd/autotune.py",
             f"# command: python -m vectra_amd.autotune --out {Path(args.out).name} --rows {args.rows}",

@@ -250,7 +250,7 @@ k_gb_feed(GbLayout L, FeedArgs A, InSet ins, PredInfo pi, vgpu_prog prog, ColSet
                     gb_merge_global(L, h, cc ? cc + (size_t)s * L.RC : nullptr, A.hot, A.cold, A.cap, A.ctl);
             }
         }
-        gb_lds_merge(L, lhot0, lcold0, S, empty, A.hot, A.cold, A.cap, A.ctl);
+        gb_lds_finish(L, lhot0, lcold0, S, empty, A.hot, A.cold, A.cap, A.ctl, A.part);
     }
 }
 
@@ -260,6 +260,7 @@ k_gb_feed(GbLayout L, FeedArgs A, InSet ins, PredInfo pi, vgpu_prog prog, ColSet
 // DENSE: int64 / bool keys own record k - kmin of the LDS table (the range
 // the feed's sketch saw), found without hashing.
 struct NarrowArgs {
+    uint64_t *part;           // block partials of the LDS table (nullptr: merge into HBM directly)
     int64_t n_logical, pos_base;
     const int32_t *sel;
     vgpu_col key;
@@ -350,7 +351,7 @@ __global__ void __launch_bounds__(GB_THREADS) k_gb_feed_narrow(GbLayout L, Narro
             }
         }
     }
-    if (S) gb_lds_merge(L, lhot, lcold, S, empty, A.hot, A.cold, A.cap, A.ctl);
+    if (S) gb_lds_finish(L, lhot, lcold, S, empty, A.hot, A.cold, A.cap, A.ctl, A.part);
 }
 
 // fused probe -> aggregate ----------------------------------------------------
@@ -724,27 +725,15 @@ k_gb_star(GbLayout L, StarArgs A, JView jv, PredInfo pi, ColSet pcs) {
     if (S) gb_lds_merge(L, lhot, lcold, S, empty_g, A.hot, A.cold, A.cap, A.ctl);
 }
 
-// Dense LDS tables (record s of every block holds the same key) merge
-// through per-block partials instead of HBM atomics: every block stores its
-// S + 2 records (RH + RC words each) at part + blockIdx.x * (S + 2) * (RH +
-// RC), and k_gb_part_reduce folds record s of all blocks in one block per
-// record, which then merges it into the HBM table once.  Without this every
-// block's merge is a device-scope atomic on the same few records, which
-// the memory side serializes.
-__device__ __forceinline__ void gb_lds_store_partial(const uint64_t *lhot, const uint64_t *lcold, int S, int RH, int RC,
-                                                     uint64_t *part) {
-    const int RW = RH + RC;
-    const int words = (S + 2) * RW;
-    uint64_t *dst = part + (size_t)blockIdx.x * words;
-    for (int q = threadIdx.x; q < words; q += GB_THREADS) {
-        const int s = q / RW, w = q - s * RW;
-        dst[q] = w < RH ? lhot[(size_t)s * RH + w] : lcold[(size_t)s * RC + (w - RH)];
-    }
-}
-
-__global__ void __launch_bounds__(GB_THREADS) k_gb_part_reduce(GbLayout L, const uint64_t *part, int64_t nparts,
-                                                               int S, uint64_t *hot, uint64_t *cold, int64_t cap,
-                                                               uint64_t *ctl) {
+// Folds of block partials (see gb_lds_store_partial).  Dense tables (record
+// s of every block holds the same key): one block per record folds record s
+// of every partial in LDS and merges it into the HBM table once.  Hashed
+// tables: block j folds partials [j * per, (j + 1) * per) into an LDS table
+// of S slots (a key that finds no slot goes to the HBM table directly) and
+// merges that, so a key's HBM record takes one merge per fold block.
+__global__ void __launch_bounds__(GB_THREADS) k_gb_part_fold_dense(GbLayout L, const uint64_t *part, int64_t nparts,
+                                                                   int S, uint64_t *hot, uint64_t *cold, int64_t cap,
+                                                                   uint64_t *ctl) {
     __shared__ uint64_t rec[2 * GB_MAXW];
     const int s = blockIdx.x, RW = L.RH + L.RC;
     const uint64_t *first = part + (size_t)s * RW;
@@ -764,6 +753,30 @@ __global__ void __launch_bounds__(GB_THREADS) k_gb_part_reduce(GbLayout L, const
     if (threadIdx.x != 0 || rec[1] == 0) return;
     const int64_t gs = s < S ? gb_global_slot(L, hot, cap, VG_KEY_REGULAR, rec[0], 0, ctl) : cap + (s - S);
     if (gs >= 0) gb_merge(L, hot + (size_t)gs * L.RH, cold + (size_t)gs * L.RC, rec, rec + L.RH);
+}
+
+__global__ void __launch_bounds__(GB_THREADS) k_gb_part_fold(GbLayout L, const uint64_t *part, int64_t nparts, int S,
+                                                             int per, uint64_t *hot, uint64_t *cold, int64_t cap,
+                                                             uint64_t *ctl) {
+    extern __shared__ uint64_t dyn_fold[];
+    const int RW = L.RH + L.RC;
+    uint64_t *lhot = dyn_fold;
+    uint64_t *lcold = lhot + (size_t)(S + 2) * L.RH;
+    const uint64_t empty = L.key_is_double ? VG_EMPTY_F64 : VG_EMPTY_I64;
+    gb_lds_init(L, lhot, lcold, S, empty);
+    const int64_t b0 = (int64_t)blockIdx.x * per;
+    const int64_t nb = nparts - b0 < per ? nparts - b0 : per;
+    const int64_t entries = nb * (S + 2);
+    for (int64_t q = threadIdx.x; q < entries; q += GB_THREADS) {
+        const int64_t b = b0 + q / (S + 2);
+        const int s = (int)(q % (S + 2));
+        const uint64_t *r = part + ((size_t)b * (S + 2) + s) * RW;
+        if (r[1] == 0) continue;
+        const int64_t ls = s < S ? gb_lds_slot(lhot, S, L.RH, VG_KEY_REGULAR, r[0], empty) : s;
+        if (ls >= 0) gb_merge(L, lhot + (size_t)ls * L.RH, lcold + (size_t)ls * L.RC, r, r + L.RH);
+        else gb_merge_global(L, r, r + L.RH, hot, cold, cap, ctl);
+    }
+    gb_lds_merge(L, lhot, lcold, S, empty, hot, cold, cap, ctl);
 }
 
 // The star shape of most join -> aggregate queries, with as little as
@@ -973,13 +986,7 @@ __global__ void __launch_bounds__(GB_THREADS) k_gb_star_dense(GbLayout L, StarDe
             star_dense_step<ITEMS, IN_MODE, LEFT>(L, A, lhot, lcold, base, tid, live, cls, kn, pv_ok, pv);
         }
     }
-    if (A.part) {
-        // the block's table as one partial, reduced by k_gb_part_reduce
-        __syncthreads();
-        gb_lds_store_partial(lhot, lcold, A.S, A.RH, A.RC, A.part);
-    } else {
-        gb_lds_merge(L, lhot, lcold, A.S, VG_EMPTY_I64, A.hot, A.cold, A.cap, A.ctl);
-    }
+    gb_lds_finish(L, lhot, lcold, A.S, VG_EMPTY_I64, A.hot, A.cold, A.cap, A.ctl, A.part);
 }
 
 // ---------------------------------------------------------------------------
@@ -2693,6 +2700,40 @@ static bool gb_narrow_shape(const GbLayout &L, const PredInfo &pi) {
     return true;
 }
 
+// block partials of an LDS-table feed of `grid` blocks with S hashed or
+// dense slots (knob gb.merge.partials): room for every block's S + 2 full
+// records, when the feed has enough blocks for HBM merges to contend, the
+// partials stay within 64 MB and a fold block's table fits in 64 KB of LDS;
+// *part = nullptr otherwise (the blocks merge into HBM directly)
+static int gb_part_setup(vgpu_groupby *g, int64_t grid, int S, uint64_t **part) {
+    *part = nullptr;
+    const size_t rw = (size_t)(g->L.RH + g->L.RC) * 8;
+    const size_t bytes = (size_t)grid * (S + 2) * rw;
+    if (S <= 0 || grid < 64 || vg_tune(VG_TUNE_GB_MERGE_PARTIALS) < 2 || bytes > ((size_t)64 << 20) ||
+        (size_t)(S + 2) * rw > 65536)
+        return 0;
+    if (g->part.reserve(bytes)) return -1;
+    *part = g->part.as<uint64_t>();
+    return 0;
+}
+
+#define GB_FOLD_PER 32   // partials per fold block of a hashed table
+
+// folds the partials of `grid` blocks into the HBM table
+static int gb_part_fold(vgpu_groupby *g, const uint64_t *part, int64_t grid, int S, bool dense) {
+    hipStream_t st = vg_stream();
+    uint64_t *hot = g->hot.as<uint64_t>(), *cold = g->cold.as<uint64_t>(), *ctl = g->ctl.as<uint64_t>();
+    if (dense) {
+        k_gb_part_fold_dense<<<(unsigned)(S + 2), GB_THREADS, 0, st>>>(g->L, part, grid, S, hot, cold, g->cap, ctl);
+    } else {
+        const size_t lds = (size_t)(S + 2) * (g->L.RH + g->L.RC) * 8;
+        k_gb_part_fold<<<(unsigned)((grid + GB_FOLD_PER - 1) / GB_FOLD_PER), GB_THREADS, lds, st>>>(
+            g->L, part, grid, S, GB_FOLD_PER, hot, cold, g->cap, ctl);
+    }
+    VG_LAUNCH_CHECK();
+    return 0;
+}
+
 static int gb_feed_narrow(vgpu_groupby *g, const FeedArgs &F, const vgpu_col *inputs, const PredInfo &pi,
                           const vgpu_col *pred_cols, int64_t groups, bool sketched) {
     const GbLayout &L = g->L;
@@ -2737,6 +2778,7 @@ static int gb_feed_narrow(vgpu_groupby *g, const FeedArgs &F, const vgpu_col *in
     const size_t lds = A.S ? (size_t)(A.S + 2) * rec : 0;
     const unsigned grid = gb_grid(F.n_logical, A.S, lds, gb_groups(g));
     hipStream_t st = vg_stream();
+    if (gb_part_setup(g, grid, A.S, &A.part)) return -1;
     vg_kt_begin();
     // NI is the exact input count: an instantiation reads inputs 0..NI-1
     if (L.n_inputs == 0) {
@@ -2750,6 +2792,7 @@ static int gb_feed_narrow(vgpu_groupby *g, const FeedArgs &F, const vgpu_col *in
         else k_gb_feed_narrow<2, false><<<grid, GB_THREADS, lds, st>>>(L, A);
     }
     VG_LAUNCH_CHECK();
+    if (A.part && gb_part_fold(g, A.part, grid, A.S, dense)) return -1;
     vg_kt_end();
     g->any_fed = 1;
     return 0;
@@ -2965,10 +3008,14 @@ static int groupby_feed_body(vgpu_groupby *g, const vgpu_col *key, const vgpu_co
         while (A.lds_copies * 2 <= want && vm + table * A.lds_copies * 2 <= budget) A.lds_copies *= 2;
     }
     size_t lds = vm + table * A.lds_copies;
+    const unsigned grid = gb_grid(n_logical, A.lds_slots, lds, gb_groups(g));
+    A.part = nullptr;
+    if (gb_part_setup(g, grid, A.lds_slots, &A.part)) return -1;
     vg_kt_begin();
     auto feed = g->L.n_inputs <= 4 && vg_tune(VG_TUNE_GB_FEED_PF) >= 2 ? k_gb_feed<4> : k_gb_feed<0>;
-    feed<<<gb_grid(n_logical, A.lds_slots, lds, gb_groups(g)), GB_THREADS, lds, vg_stream()>>>(g->L, A, ins, pi, prog, pcs);
+    feed<<<grid, GB_THREADS, lds, vg_stream()>>>(g->L, A, ins, pi, prog, pcs);
     VG_LAUNCH_CHECK();
+    if (A.part && gb_part_fold(g, A.part, grid, A.lds_slots, false)) return -1;
     vg_kt_end();
     g->any_fed = 1;
     return 0;
@@ -3275,11 +3322,7 @@ static int groupby_feed_join_body(vgpu_groupby *g, const vgpu_feed_spec *sp) {
                            sd.pval.type != VGPU_BOOL;
         hipStream_t ks = vg_stream();
         // block partials reduced per record (knob gb.merge.partials)
-        sd.part = nullptr;
-        if (vg_tune(VG_TUNE_GB_MERGE_PARTIALS) >= 2) {
-            if (g->part.reserve((size_t)grid * (sd.S + 2) * (sd.RH + sd.RC) * 8)) return -1;
-            sd.part = g->part.as<uint64_t>();
-        }
+        if (gb_part_setup(g, grid, sd.S, &sd.part)) return -1;
         vg_kt_begin();
 #define STAR_DENSE(I, M, LF)                                                                \
     do {                                                                                    \
@@ -3295,11 +3338,7 @@ static int groupby_feed_join_body(vgpu_groupby *g, const vgpu_feed_spec *sp) {
         }
 #undef STAR_DENSE
         VG_LAUNCH_CHECK();
-        if (sd.part) {
-            k_gb_part_reduce<<<(unsigned)(sd.S + 2), GB_THREADS, 0, ks>>>(g->L, sd.part, (int64_t)grid, sd.S, sd.hot,
-                                                                          sd.cold, sd.cap, sd.ctl);
-            VG_LAUNCH_CHECK();
-        }
+        if (sd.part && gb_part_fold(g, sd.part, grid, sd.S, true)) return -1;
         vg_kt_end();
         g->any_fed = 1;
         return 0;

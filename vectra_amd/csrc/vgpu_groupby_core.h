@@ -208,6 +208,7 @@ struct FeedArgs {
     int32_t vm_lds;        // bytes of VM registers at the start of dyn LDS
     int32_t lds_cold;      // LDS records carry cold words (an input has NAs or a double min / max)
     int32_t lds_copies;    // copies of the LDS table, lane l updating copy l % lds_copies (1, 2, 4, 8)
+    uint64_t *part;        // block partials of the LDS table (nullptr: merge into HBM directly)
 };
 
 // cold words can be touched only by NA inputs and by double min / max
@@ -280,6 +281,37 @@ __device__ __forceinline__ void gb_lds_merge(const GbLayout &L, const uint64_t *
         }
         if (gs >= 0)
             gb_merge(L, hot + (size_t)gs * L.RH, cold + (size_t)gs * L.RC, h, lcold ? lcold + (size_t)s * L.RC : nullptr);
+    }
+}
+
+// LDS tables merge through per-block partials instead of HBM atomics when
+// the host gives them room: every block stores its S + 2 records (RH + RC
+// words each; a table without cold words stores their initial values) at
+// part + blockIdx.x * (S + 2) * (RH + RC), and k_gb_part_fold folds them,
+// so that a record's HBM words take a few merges instead of one
+// device-scope atomic per block, which the memory side serializes.
+__device__ __forceinline__ void gb_lds_store_partial(const GbLayout &L, const uint64_t *lhot, const uint64_t *lcold,
+                                                     int S, uint64_t *part) {
+    const int RW = L.RH + L.RC;
+    const int words = (S + 2) * RW;
+    uint64_t *dst = part + (size_t)blockIdx.x * words;
+    for (int q = threadIdx.x; q < words; q += GB_THREADS) {
+        const int s = q / RW, w = q - s * RW;
+        dst[q] = w < L.RH ? lhot[(size_t)s * L.RH + w]
+                 : lcold  ? lcold[(size_t)s * L.RC + (w - L.RH)]
+                          : (((L.cold_init_ones >> (w - L.RH)) & 1) ? ~0ull : 0ull);
+    }
+}
+
+// the end of an LDS-table feed block
+__device__ __forceinline__ void gb_lds_finish(const GbLayout &L, const uint64_t *lhot, const uint64_t *lcold, int S,
+                                              uint64_t empty, uint64_t *hot, uint64_t *cold, int64_t cap,
+                                              uint64_t *ctl, uint64_t *part) {
+    if (part) {
+        __syncthreads();
+        gb_lds_store_partial(L, lhot, lcold, S, part);
+    } else {
+        gb_lds_merge(L, lhot, lcold, S, empty, hot, cold, cap, ctl);
     }
 }
 
