@@ -475,17 +475,18 @@ STAGE_AGGS = {0: [("n", -1, 0)],
               4: [("n", -1, 0), ("max", 0, 0), ("sum", 1, 0), ("min", 2, 0), ("sum", 3, 0)]}
 
 
-@pytest.mark.parametrize("stage", [2, 3])
+@pytest.mark.parametrize("stage,agg", [(2, 3), (3, 3), (3, 2)])
 @pytest.mark.parametrize("range_mode", [2, 1])
 @pytest.mark.parametrize("n_inputs", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("valid", [False, True])
-def test_groupby_partition_staging(gpu, part_threshold, stage, range_mode, n_inputs, valid):
+def test_groupby_partition_staging(gpu, part_threshold, stage, agg, range_mode, n_inputs, valid):
     """the LDS-staged partition passes and resident aggregation blocks
-    (gb.part.stage 2 and 3: tiles of 1,024 and 2,048 rows) against the
-    row-wise ones (1): staged rows of 2 to 6 words, the plain first pass (no
-    validity) and the general one, key ranges and hashes; keys, order,
-    counts, min and max bit-exact, sums equal or within 1e-9 of
-    max(1, |sum|) (summation order), counts and keys against numpy"""
+    (gb.part.stage 2 and 3: tiles of 1,024 and 2,048 rows; gb.part.agg 3:
+    items at a fixed stride with the next item's rows loaded ahead, 2: items
+    by ticket) against the row-wise ones (1): staged rows of 2 to 6 words,
+    the plain first pass (no validity) and the general one, key ranges and
+    hashes; keys, order, counts, min and max bit-exact, sums equal or within
+    1e-9 of max(1, |sum|) (summation order), counts and keys against numpy"""
     n = 1_300_000
     r = rng(4000 + 10 * n_inputs + range_mode)
     k = r.integers(-10**12, 10**12, 300_000)[r.integers(0, 300_000, n)]
@@ -500,10 +501,11 @@ def test_groupby_partition_staging(gpu, part_threshold, stage, range_mode, n_inp
             datas.append(r.integers(-10**9, 10**9, n))
     cols = [gpu.DeviceColumn(k, kval)] + [gpu.DeviceColumn(d) for d in datas]
     lib = gpu.vgpu()
-    old = (lib.vgpu_tune_get(b"gb.part.stage"), lib.vgpu_tune_get(b"gb.part.range"))
+    old = (lib.vgpu_tune_get(b"gb.part.stage"), lib.vgpu_tune_get(b"gb.part.range"), lib.vgpu_tune_get(b"gb.part.agg"))
     res = {}
     try:
         assert lib.vgpu_tune_set(b"gb.part.range", range_mode) == 0
+        assert lib.vgpu_tune_set(b"gb.part.agg", agg) == 0
         for mode in (stage, 1):
             assert lib.vgpu_tune_set(b"gb.part.stage", mode) == 0
             gb = gpu.GroupBy(gpu.VEC_INT64, types, STAGE_AGGS[n_inputs], gpu.VGPU_ORDER_KEY, 0)
@@ -516,6 +518,7 @@ def test_groupby_partition_staging(gpu, part_threshold, stage, range_mode, n_inp
     finally:
         lib.vgpu_tune_set(b"gb.part.stage", old[0])
         lib.vgpu_tune_set(b"gb.part.range", old[1])
+        lib.vgpu_tune_set(b"gb.part.agg", old[2])
         for c in cols:
             c.free()
     kk = k if kval is None else k[kval]

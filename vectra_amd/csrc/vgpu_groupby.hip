@@ -77,6 +77,8 @@ struct GbpArgs {
     uint32_t *nitems;              // items per partition (P)
     uint64_t *itembase;            // scanned items (P + 1)
     uint32_t *ipart;               // partition of every item (NImax)
+    uint4 *idesc;                  // every item: first row, end row, partition, flags (1: the
+                                   // partition has several items, 2: the item is its first)
     int64_t NImax;
     uint32_t *icount;              // regular records per item
     uint64_t *irec;                // first record index of an item
@@ -2089,6 +2091,39 @@ __device__ __forceinline__ void gbp_agg_row(const GbLayout &L, const GbpArgs &G,
     }
 }
 
+// gbp_apply_staged / gbp_agg_row for rows of a width known at compile time
+// (W = inputs + 2): the row's words are indexed by constants only, so they
+// stay in registers
+template <int W>
+__device__ __forceinline__ void gbp_apply_w(const GbLayout &L, uint64_t *h, uint64_t *c, const uint64_t (&w)[W],
+                                            uint64_t meta, uint64_t pos) {
+    gb_update_row(L, h, pos);
+#pragma unroll
+    for (int i = 0; i < W - 2; i++) {
+        InVal v;
+        v.valid = (meta >> (34 + i)) & 1;
+        v.bits = w[1 + i];
+        gb_update_input(L, h, c, i, pos, v);
+    }
+}
+
+template <int W>
+__device__ __forceinline__ void gbp_agg_row_w(const GbLayout &L, const GbpArgs &G, uint64_t *lhot, uint64_t *lcold,
+                                              int S, uint64_t empty, const uint64_t (&w)[W], bool &spilled) {
+    const uint64_t kn = w[0], meta = w[W - 1];
+    const int cls = (int)((meta >> 32) & 3);
+    const uint64_t pos = (uint64_t)G.pos_base + (uint32_t)meta;
+    const int64_t ls = gbp_lds_slot(lhot, S, L.RH, cls, kn, empty);
+    // one expansion per address space: ds_* atomics for the LDS record
+    if (ls >= 0) {
+        gbp_apply_w<W>(L, lhot + (size_t)ls * L.RH, lcold + (size_t)ls * L.RC, w, meta, pos);
+    } else {
+        const int64_t gs = gb_global_slot(L, G.hot, G.cap, cls, kn, pos, G.ctl);
+        spilled = true;
+        if (gs >= 0) gbp_apply_w<W>(L, G.hot + (size_t)gs * L.RH, G.cold + (size_t)gs * L.RC, w, meta, pos);
+    }
+}
+
 __global__ void __launch_bounds__(GB_THREADS) k_gbp_agg(GbLayout L, GbpArgs G) {
     extern __shared__ uint64_t dynw[];   // 8-byte aligned after the static arrays
     __shared__ uint16_t slist[1024];   // [0, 512) occupied slots, [512, 1024) in key order
@@ -2162,6 +2197,35 @@ __global__ void __launch_bounds__(GB_THREADS) k_gbp_agg(GbLayout L, GbpArgs G) {
     }
 }
 
+// after an item's regular records are written (thread 0): its counts,
+// its NA / EMPTY-valued key records (final when the partition has one
+// item, partials behind the regular records otherwise), the LDS words of
+// those two records back to their initial values, and the partition on
+// the merge list when this is its first item of several
+__device__ __forceinline__ void gbp_item_tail(const GbLayout &L, const GbpArgs &G, uint64_t *lhot, uint64_t *lcold,
+                                              int S, uint64_t empty, int64_t item, int64_t p, int64_t nit,
+                                              int64_t chunk, int64_t r0, uint32_t total) {
+    G.icount[item] = total;
+    G.irec[item] = (uint64_t)r0;
+    uint32_t fl = 0;
+    int64_t at = r0 + total;
+    for (int j = 0; j < 2; j++) {
+        uint64_t *h = lhot + (size_t)(S + j) * L.RH;
+        if (h[1] == 0) continue;
+        uint64_t *c = lcold + (size_t)(S + j) * L.RC;
+        if (nit == 1) {
+            gbp_copy_record(L, h, c, G.rec + (G.n_rows + j) * G.RW);
+        } else {
+            gbp_copy_record(L, h, c, G.rec + at * G.RW);
+            at++;
+            fl |= 1u << j;
+        }
+        gbp_reset_record(L, h, c, empty);
+    }
+    G.rflags[item] = fl;
+    if (nit > 1 && chunk == 0) G.mlist[atomicAdd((unsigned long long *)&G.ctl[10], 1ull)] = (uint32_t)p;
+}
+
 // The aggregation with resident blocks: a block takes items in turn (an
 // atomic ticket), reads its partition from ipart instead of searching for
 // it, and sets only the LDS words it copied out back to their initial
@@ -2218,27 +2282,7 @@ __global__ void __launch_bounds__(GB_THREADS) k_gbp_agg_loop(GbLayout L, GbpArgs
         __syncthreads();
         uint32_t total = gbp_write_records(L, lhot, lcold, S, empty, G.rec + r0 * G.RW, G.RW, slist, scan_lds,
                                            G.range != 0, kc, true, &bsort);
-        if (threadIdx.x == 0) {
-            G.icount[item] = total;
-            G.irec[item] = (uint64_t)r0;
-            uint32_t fl = 0;
-            int64_t at = r0 + total;
-            for (int j = 0; j < 2; j++) {
-                uint64_t *h = lhot + (size_t)(S + j) * L.RH;
-                if (h[1] == 0) continue;
-                uint64_t *c = lcold + (size_t)(S + j) * L.RC;
-                if (nit == 1) {
-                    gbp_copy_record(L, h, c, G.rec + (G.n_rows + j) * G.RW);
-                } else {
-                    gbp_copy_record(L, h, c, G.rec + at * G.RW);
-                    at++;
-                    fl |= 1u << j;
-                }
-                gbp_reset_record(L, h, c, empty);
-            }
-            G.rflags[item] = fl;
-            if (nit > 1 && chunk == 0) G.mlist[atomicAdd((unsigned long long *)&G.ctl[10], 1ull)] = (uint32_t)p;
-        }
+        if (threadIdx.x == 0) gbp_item_tail(L, G, lhot, lcold, S, empty, item, p, nit, chunk, r0, total);
         __syncthreads();
     }
     if (spilled) G.ctl[9] = 1;
@@ -2247,11 +2291,105 @@ __global__ void __launch_bounds__(GB_THREADS) k_gbp_agg_loop(GbLayout L, GbpArgs
         G.icount[it] = 0;
 }
 
-// partition of every item
+// U staged rows of W words per thread, rows r, r + GB_THREADS, ...; rows
+// at or past r1 read as zeros (never used)
+template <int W, int U>
+__device__ __forceinline__ void gbp_load_rows(const uint64_t *S2, int64_t r, int64_t r1, uint64_t (&v)[U][W]) {
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int64_t rr = r + (int64_t)u * GB_THREADS;
+        if (rr < r1) {
+            const uint64_t *src = S2 + rr * W;
+            if constexpr ((W & 1) == 0) {
+#pragma unroll
+                for (int k = 0; k < W; k += 2) {
+                    const ulonglong2 q = *reinterpret_cast<const ulonglong2 *>(src + k);
+                    v[u][k] = q.x;
+                    v[u][k + 1] = q.y;
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < W; k++) v[u][k] = src[k];
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < W; k++) v[u][k] = 0;
+        }
+    }
+}
+
+// The aggregation with resident blocks taking items in a fixed stride
+// (block b: items b, b + grid, ...), so a block knows its next item while
+// it works on the current one: the next item's bounds are loaded while the
+// current rows are applied, and its first U rows per thread while the
+// current records are sorted and written.  The barriers of that phase
+// wait for LDS only, so those loads stay in flight across them.  Same
+// records, flags and lists as k_gbp_agg.
+template <int W, int U>
+__global__ void __launch_bounds__(GB_THREADS) k_gbp_agg_pf(GbLayout L, GbpArgs G, const uint4 *__restrict__ idesc) {
+    extern __shared__ uint64_t dynw[];   // 8-byte aligned after the static arrays
+    __shared__ uint16_t slist[1024];   // [0, 512) occupied slots, [512, 1024) in key order
+    __shared__ int64_t kc[512];
+    __shared__ uint32_t scan_lds[GB_THREADS / VG_WAVE];
+    __shared__ GbpSortLds bsort;
+    const int S = G.S;
+    uint64_t *lhot = dynw;
+    uint64_t *lcold = lhot + (size_t)(S + 2) * L.RH;
+    const uint64_t empty = L.key_is_double ? VG_EMPTY_F64 : VG_EMPTY_I64;
+    gb_lds_init(L, lhot, lcold, S, empty);
+    const int64_t NI = (int64_t)G.itembase[G.P];
+    const int64_t step = (int64_t)gridDim.x;
+    bool spilled = false;
+    uint64_t v[U][W];
+    int64_t item = blockIdx.x;
+    uint4 cur = make_uint4(0, 0, 0, 0);
+    if (item < NI) {
+        cur = idesc[item];
+        gbp_load_rows<W, U>(G.S2, (int64_t)cur.x + threadIdx.x, cur.y, v);
+    }
+    for (; item < NI; item += step) {
+        const bool more = item + step < NI;
+        // (a scalar load: idesc is read-only here)
+        const uint4 nxt = more ? idesc[item + step] : make_uint4(0, 0, 0, 0);
+        const int64_t r0 = cur.x, r1 = cur.y;
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (r0 + threadIdx.x + (int64_t)u * GB_THREADS < r1)
+                gbp_agg_row_w<W>(L, G, lhot, lcold, S, empty, v[u], spilled);
+        for (int64_t r = r0 + threadIdx.x + (int64_t)U * GB_THREADS; r < r1; r += (int64_t)U * GB_THREADS) {
+            uint64_t w[U][W];
+            gbp_load_rows<W, U>(G.S2, r, r1, w);
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (r + (int64_t)u * GB_THREADS < r1) gbp_agg_row_w<W>(L, G, lhot, lcold, S, empty, w[u], spilled);
+        }
+        if (more) gbp_load_rows<W, U>(G.S2, (int64_t)nxt.x + threadIdx.x, nxt.y, v);
+        __syncthreads();
+        const uint32_t total = gbp_write_records(L, lhot, lcold, S, empty, G.rec + r0 * G.RW, G.RW, slist,
+                                                 scan_lds, G.range != 0, kc, true, &bsort);
+        if (threadIdx.x == 0)
+            gbp_item_tail(L, G, lhot, lcold, S, empty, item, cur.z, (cur.w & 1) ? 2 : 1, (cur.w & 2) ? 0 : 1, r0,
+                          total);
+        __syncthreads();
+        cur = nxt;
+    }
+    if (spilled) G.ctl[9] = 1;
+    for (int64_t it = NI + (int64_t)blockIdx.x * GB_THREADS + threadIdx.x; it < G.NImax;
+         it += (int64_t)gridDim.x * GB_THREADS)
+        G.icount[it] = 0;
+}
+
+// partition and descriptor of every item (staged rows fit 32 bits: the
+// partitioned path takes feeds of fewer than 2^32 - 2 rows)
 __global__ void k_gbp_ipart(GbpArgs G) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= G.P) return;
-    for (uint64_t i = G.itembase[p]; i < G.itembase[p + 1]; i++) G.ipart[i] = (uint32_t)p;
+    const uint64_t i0 = G.itembase[p], i1 = G.itembase[p + 1], pe = G.pstart[p + 1];
+    for (uint64_t i = i0; i < i1; i++) {
+        G.ipart[i] = (uint32_t)p;
+        const uint64_t r0 = G.pstart[p] + (i - i0) * GBP_CH, r1 = r0 + GBP_CH < pe ? r0 + GBP_CH : pe;
+        G.idesc[i] = make_uint4((uint32_t)r0, (uint32_t)r1, (uint32_t)p, (i1 - i0 > 1 ? 1u : 0u) | (i == i0 ? 2u : 0u));
+    }
 }
 
 // one block per partition with several items: merge their partial records
@@ -2426,6 +2564,7 @@ static void gbp_layout(GbpArgs &G, uint8_t *base, size_t *bytes) {
     G.nitems = c.take<uint32_t>(G.P);
     G.itembase = c.take<uint64_t>(G.P + 1);
     G.ipart = c.take<uint32_t>(G.NImax);
+    G.idesc = c.take<uint4>(G.NImax);
     G.icount = c.take<uint32_t>(G.NImax);
     G.irec = c.take<uint64_t>(G.NImax);
     G.rflags = c.take<uint32_t>(G.NImax);
@@ -2578,7 +2717,13 @@ static int gbp_feed(vgpu_groupby *g, FeedArgs A, const InSet &ins, const PredInf
         const size_t per_block = lds + 1024 * 2 + 512 * 8 + 64 + sizeof(GbpSortLds);
         const int64_t bpc = std::max<int64_t>(1, std::min<int64_t>(8, (int64_t)(160 * 1024 / per_block)));
         const int64_t grid = std::min<int64_t>(G.NImax, (int64_t)vg_cus() * bpc);
-        k_gbp_agg_loop<<<(unsigned)grid, GB_THREADS, lds, st>>>(L, G);
+        // knob gb.part.agg: 3 = fixed-stride items with the next item's rows
+        // loaded ahead (rows of 3 to 6 words), 2 = items taken by ticket
+        const int64_t agg = vg_tune_for(VG_TUNE_GB_PART_AGG, groups);
+        auto pf = G.W == 3 ? k_gbp_agg_pf<3, 4> : G.W == 4 ? k_gbp_agg_pf<4, 4>
+                : G.W == 5 ? k_gbp_agg_pf<5, 2> : G.W == 6 ? k_gbp_agg_pf<6, 2> : nullptr;
+        if (agg >= 3 && pf) pf<<<(unsigned)grid, GB_THREADS, lds, st>>>(L, G, G.idesc);
+        else k_gbp_agg_loop<<<(unsigned)grid, GB_THREADS, lds, st>>>(L, G);
     } else {
         k_gbp_agg<<<(unsigned)G.NImax, GB_THREADS, lds, st>>>(L, G);
     }
