@@ -592,8 +592,10 @@ template <class SlotOf>
 __device__ __forceinline__ void gb_stage_records(const GbLayout &L, const RecSrc &R, int cnt, SlotOf rec_of,
                                                  uint64_t *dst) {
     const int RW = L.RH + L.RC;
+    // q / RW by a multiply (exact for q < 2^32 / RW)
+    const uint32_t mag = (uint32_t)((0xFFFFFFFFull + (uint64_t)RW) / (uint64_t)RW);
     for (int q = threadIdx.x; q < cnt * RW; q += GB_THREADS) {
-        int k = q / RW, w = q - k * RW;
+        int k = (int)__umulhi((uint32_t)q, mag), w = q - k * RW;
         int64_t sk = rec_of(k);
         dst[q] = w < L.RH ? R.hot[sk * R.hstride + w] : R.cold[sk * R.cstride + (w - L.RH)];
     }
