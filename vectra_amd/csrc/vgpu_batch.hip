@@ -871,7 +871,9 @@ __device__ __forceinline__ void bt_chain_rows_add(const BtChainArgs &A, const vg
 #pragma unroll
         for (int u = 0; u < U; u++) {
             if (!ent[u]) continue;
-            const uint64_t segw = ent[u][1];
+            // table entries are device memory: global loads (the pointer
+            // array alone would be read with flat ones)
+            const uint64_t segw = vg_gload<uint64_t>(ent[u], 1);
             const uint32_t n = jt_count(A.jn.jv, segw), off = (uint32_t)(segw >> 32);
             const int p = A.jn.key_payload;
             for (uint32_t j = 0; j < n; j++) {
@@ -879,7 +881,7 @@ __device__ __forceinline__ void bt_chain_rows_add(const BtChainArgs &A, const vg
                 uint64_t bits;
                 if (j == 0) {
                     v = (segw >> (24 + p)) & 1;
-                    bits = ent[u][2 + p];
+                    bits = vg_gload<uint64_t>(ent[u], 2 + p);
                 } else {
                     v = A.jn.jv.cval[p][off + j];
                     bits = A.jn.jv.cpay[p][off + j];
@@ -926,7 +928,7 @@ __global__ void __launch_bounds__(BT_CH_THREADS) k_bt_chain_feed(BtChainArgs A, 
     __syncthreads();
     const int64_t r0 = (int64_t)blockIdx.x * A.rows_per_block;
     const int64_t r1 = r0 + A.rows_per_block < A.n_rows ? r0 + A.rows_per_block : A.n_rows;
-    const vgpu_col &kc = A.jn.on ? A.probe_key : A.key;
+    const vgpu_col kc = A.jn.on ? A.probe_key : A.key;   // a copy: a reference to either kernel argument would be read with flat loads
     const int64_t step = (int64_t)BT_CH_THREADS * U;
     bool err = false;
     BtRows<U, SP> cur, nxt;

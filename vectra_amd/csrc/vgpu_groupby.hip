@@ -209,29 +209,31 @@ k_gb_feed(GbLayout L, FeedArgs A, InSet ins, PredInfo pi, vgpu_prog prog, ColSet
             if (L.key_is_double && cls[u] == VG_KEY_REGULAR && kn[u] == 0)
                 gb_note_zero_key(A.ctl, vg_load_bits(A.key, row[u]), pos);
             int64_t ls = S ? gb_lds_slot(lhot, S, L.RH, cls[u], kn[u], empty) : -1;
-            uint64_t *h, *c;
+            auto apply = [&](uint64_t *h, uint64_t *c) {
+                if (PF > 0) {
+                    gb_update_row(L, h, pos);
+#pragma unroll
+                    for (int i = 0; i < (PF > 0 ? PF : 1); i++) {
+                        if (i < L.n_inputs) {
+                            InVal v;
+                            v.valid = iv[u][i];
+                            v.bits = iv[u][i] ? ib[u][i] : 0;
+                            gb_update_input(L, h, c, i, pos, v);
+                        }
+                    }
+                } else {
+                    gb_apply(L, ins, row[u], pos, h, c);
+                }
+            };
+            // one expansion per address space: ds_* atomics on the LDS
+            // record, global atomics on the HBM one (a pointer that may be
+            // either would make every update a flat atomic)
             if (ls >= 0) {
-                h = lhot + (size_t)ls * L.RH;
-                c = lcold ? lcold + (size_t)ls * L.RC : nullptr;
+                apply(lhot + (size_t)ls * L.RH, lcold ? lcold + (size_t)ls * L.RC : nullptr);
             } else {
                 int64_t gs = gb_global_slot(L, A.hot, A.cap, cls[u], kn[u], pos, A.ctl);
                 if (gs < 0) continue;
-                h = A.hot + (size_t)gs * L.RH;
-                c = A.cold + (size_t)gs * L.RC;
-            }
-            if (PF > 0) {
-                gb_update_row(L, h, pos);
-#pragma unroll
-                for (int i = 0; i < (PF > 0 ? PF : 1); i++) {
-                    if (i < L.n_inputs) {
-                        InVal v;
-                        v.valid = iv[u][i];
-                        v.bits = iv[u][i] ? ib[u][i] : 0;
-                        gb_update_input(L, h, c, i, pos, v);
-                    }
-                }
-            } else {
-                gb_apply(L, ins, row[u], pos, h, c);
+                apply(A.hot + (size_t)gs * L.RH, A.cold + (size_t)gs * L.RC);
             }
         }
     }

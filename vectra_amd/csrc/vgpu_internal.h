@@ -173,13 +173,34 @@ __device__ __forceinline__ int vg_lane_prefix(uint64_t mask) {
 
 __device__ __forceinline__ uint64_t vg_ballot(bool p) { return __ballot(p); }
 
-__host__ __device__ __forceinline__ bool vg_valid(const uint8_t *validity, int64_t row) {
-    return validity == nullptr || ((validity[row >> 3] >> (row & 7)) & 1);
+// Column data and validity bitmaps always live in device (global) memory.
+// They are read through a global address-space pointer, so a column whose
+// descriptor was itself loaded from memory (a batch's query table) still
+// gets global loads: through a plain pointer the compiler cannot rule out
+// LDS and emits flat loads, which also count against the LDS wait counter.
+template <class T>
+__device__ __forceinline__ T vg_gload(const void *p, int64_t i) {
+    return ((const __attribute__((address_space(1))) T *)p)[i];
+}
+
+__device__ __forceinline__ bool vg_valid(const uint8_t *validity, int64_t row) {
+    return validity == nullptr || ((vg_gload<uint8_t>(validity, row >> 3) >> (row & 7)) & 1);
 }
 
 // raw 8-byte view of a row of an int64/double column, or the 0/1 byte of a
 // bool column widened to 8 bytes
-__host__ __device__ __forceinline__ uint64_t vg_load_bits(const vgpu_col &c, int64_t row) {
+__device__ __forceinline__ uint64_t vg_load_bits(const vgpu_col &c, int64_t row) {
+    if (c.type == VGPU_BOOL) return (uint64_t)vg_gload<uint8_t>(c.data, row);
+    return vg_gload<uint64_t>(c.data, row);
+}
+
+// host overloads of the two (the expression VM is also built for the host,
+// where columns are host arrays)
+__host__ inline bool vg_valid(const uint8_t *validity, int64_t row) {
+    return validity == nullptr || ((validity[row >> 3] >> (row & 7)) & 1);
+}
+
+__host__ inline uint64_t vg_load_bits(const vgpu_col &c, int64_t row) {
     if (c.type == VGPU_BOOL) return (uint64_t)static_cast<const uint8_t *>(c.data)[row];
     return static_cast<const uint64_t *>(c.data)[row];
 }
