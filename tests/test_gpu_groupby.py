@@ -105,6 +105,27 @@ def test_groupby_feed_prefetch(gpu, oracle, prefetch, partials, key_kind, card, 
             lib.vgpu_tune_set(k, v)
 
 
+@pytest.mark.parametrize("dense", [2, 1])
+@pytest.mark.parametrize("key_kind,card", [("int", 8), ("int", 40), ("bool", 0)])
+@pytest.mark.parametrize("filtered", [False, True])
+def test_groupby_feed_dense(gpu, oracle, dense, key_kind, card, filtered):
+    """the general feed with a dense LDS table (gb.feed.dense 2: the keys of
+    a short sketched range own record key - min, no hashing) and with a
+    hashed one (1), against the oracle: one feed of 2M rows (sketched), NA
+    keys and the int64 key equal to the hash tables' empty marker (reserved
+    records), NA / NaN / signed-zero inputs, every aggregate, with and
+    without a filter"""
+    lib = gpu.vgpu()
+    old = lib.vgpu_tune_get(b"gb.feed.dense")
+    assert lib.vgpu_tune_set(b"gb.feed.dense", dense) == 0
+    try:
+        mk = lambda: table(gpu, 2_000_000, 41 + card, key_kind, card, 0)
+        pred = ("cmp", ">", ("col", "ax"), ("dbl", 0.2)) if filtered else None
+        check(gpu, mk, ["g"], ALL_AGGS, None, pred=pred)
+    finally:
+        lib.vgpu_tune_set(b"gb.feed.dense", old)
+
+
 @pytest.mark.parametrize("partials", [1, 2])
 @pytest.mark.parametrize("key_kind,card", [("int", 40), ("int", 3000), ("dbl", 16)])
 def test_groupby_narrow_partials(gpu, oracle, partials, key_kind, card):

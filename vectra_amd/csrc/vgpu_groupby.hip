@@ -132,7 +132,9 @@ struct vgpu_groupby {
 // (and validity) of a tile before the first row's slot is looked up, so a
 // wave has 4 x (1 + inputs) loads in flight instead of one row's at a time
 // behind its LDS probe
-template <int PF>
+// DENSE: int64 / bool keys own record k - A.kmin of the LDS table (the
+// range the feed's sketch saw), found without hashing or probing.
+template <int PF, bool DENSE>
 __global__ void __launch_bounds__(GB_THREADS)
 k_gb_feed(GbLayout L, FeedArgs A, InSet ins, PredInfo pi, vgpu_prog prog, ColSet pcs) {
     extern __shared__ uint8_t dyn[];
@@ -150,7 +152,17 @@ k_gb_feed(GbLayout L, FeedArgs A, InSet ins, PredInfo pi, vgpu_prog prog, ColSet
     uint64_t *lhot = lhot0 + (size_t)copy * (S + 2) * L.RH;
     uint64_t *lcold = lcold0 ? lcold0 + (size_t)copy * (S + 2) * L.RC : nullptr;
     const uint64_t empty = L.key_is_double ? VG_EMPTY_F64 : VG_EMPTY_I64;
-    if (S) gb_lds_init(L, lhot0, lcold0, NC * (S + 2) - 2, empty);
+    if (S) {
+        if (DENSE) {
+            // every copy's records keyed kmin, kmin + 1, ... (the copies'
+            // reserved records as well: never keyed, never merged by key)
+            for (int c = 0; c < NC; c++)
+                gb_lds_init(L, lhot0 + (size_t)c * (S + 2) * L.RH, lcold0 ? lcold0 + (size_t)c * (S + 2) * L.RC : nullptr,
+                            S, empty, true, A.kmin);
+        } else {
+            gb_lds_init(L, lhot0, lcold0, NC * (S + 2) - 2, empty);
+        }
+    }
     // Blocks walk tiles of GB_U * GB_THREADS contiguous rows; each thread's
     // GB_U rows of a tile have their selection, predicate and key loads
     // issued together, so each wave keeps several loads in flight within
@@ -208,7 +220,8 @@ k_gb_feed(GbLayout L, FeedArgs A, InSet ins, PredInfo pi, vgpu_prog prog, ColSet
             uint64_t pos = (uint64_t)(A.pos_base + base + u * stride);
             if (L.key_is_double && cls[u] == VG_KEY_REGULAR && kn[u] == 0)
                 gb_note_zero_key(A.ctl, vg_load_bits(A.key, row[u]), pos);
-            int64_t ls = S ? gb_lds_slot(lhot, S, L.RH, cls[u], kn[u], empty) : -1;
+            int64_t ls = !S ? -1 : DENSE ? gb_lds_dense_slot(S, cls[u], kn[u], A.kmin)
+                                         : gb_lds_slot(lhot, S, L.RH, cls[u], kn[u], empty);
             auto apply = [&](uint64_t *h, uint64_t *c) {
                 if (PF > 0) {
                     gb_update_row(L, h, pos);
@@ -246,7 +259,7 @@ k_gb_feed(GbLayout L, FeedArgs A, InSet ins, PredInfo pi, vgpu_prog prog, ColSet
             for (int s = tid; s < S + 2; s += GB_THREADS) {
                 const uint64_t *h = ch + (size_t)s * L.RH;
                 if (h[1] == 0) continue;
-                const int64_t d = s < S ? gb_lds_slot(lhot0, S, L.RH, VG_KEY_REGULAR, h[0], empty) : s;
+                const int64_t d = s < S && !DENSE ? gb_lds_slot(lhot0, S, L.RH, VG_KEY_REGULAR, h[0], empty) : s;
                 if (d >= 0)
                     gb_merge(L, lhot0 + (size_t)d * L.RH, lcold0 ? lcold0 + (size_t)d * L.RC : nullptr, h,
                              cc ? cc + (size_t)s * L.RC : nullptr);
@@ -3146,7 +3159,20 @@ static int groupby_feed_body(vgpu_groupby *g, const vgpu_col *key, const vgpu_co
     if (gb_narrow_shape(g->L, pi) && vg_tune(VG_TUNE_GB_NARROW) >= 2)
         return gb_feed_narrow(g, A, inputs, pi, pred_cols, groups, sketched);
     A.lds_slots = gb_lds_slots(g, vm, A.lds_cold);
-    const size_t table = A.lds_slots ? (size_t)(A.lds_slots + 2) * (g->L.RH + (A.lds_cold ? g->L.RC : 0)) * 8 : 0;
+    const size_t rec = (size_t)(g->L.RH + (A.lds_cold ? g->L.RC : 0)) * 8;
+    // a dense table when this feed's sketch saw a short range of int64 /
+    // bool keys (knob gb.feed.dense; not below the hashed table's slots)
+    bool dense = false;
+    if (sketched && g->sk_ranged && !g->L.key_is_double && !g->L.key_none && A.lds_slots &&
+        vg_tune_for(VG_TUNE_GB_FEED_DENSE, groups) >= 2) {
+        const uint64_t range = (uint64_t)g->sk_hi - (uint64_t)g->sk_lo + 1;
+        if (range != 0 && range <= (uint64_t)A.lds_slots && vm + (range + 2) * rec <= (size_t)vg_tune(VG_TUNE_GB_LDS_BUDGET)) {
+            dense = true;
+            A.lds_slots = (int32_t)range;
+            A.kmin = g->sk_lo;
+        }
+    }
+    const size_t table = A.lds_slots ? (size_t)(A.lds_slots + 2) * rec : 0;
     // small tables are replicated while the copies stay within the budget
     A.lds_copies = 1;
     if (A.lds_slots && A.lds_slots <= 256) {
@@ -3159,10 +3185,12 @@ static int groupby_feed_body(vgpu_groupby *g, const vgpu_col *key, const vgpu_co
     A.part = nullptr;
     if (gb_part_setup(g, grid, A.lds_slots, &A.part)) return -1;
     vg_kt_begin();
-    auto feed = g->L.n_inputs <= 4 && vg_tune(VG_TUNE_GB_FEED_PF) >= 2 ? k_gb_feed<4> : k_gb_feed<0>;
+    const bool pf = g->L.n_inputs <= 4 && vg_tune(VG_TUNE_GB_FEED_PF) >= 2;
+    auto feed = pf ? (dense ? k_gb_feed<4, true> : k_gb_feed<4, false>)
+                   : (dense ? k_gb_feed<0, true> : k_gb_feed<0, false>);
     feed<<<grid, GB_THREADS, lds, vg_stream()>>>(g->L, A, ins, pi, prog, pcs);
     VG_LAUNCH_CHECK();
-    if (A.part && gb_part_fold(g, A.part, grid, A.lds_slots, false)) return -1;
+    if (A.part && gb_part_fold(g, A.part, grid, A.lds_slots, dense)) return -1;
     vg_kt_end();
     g->any_fed = 1;
     return 0;

@@ -209,6 +209,7 @@ struct FeedArgs {
     int32_t lds_cold;      // LDS records carry cold words (an input has NAs or a double min / max)
     int32_t lds_copies;    // copies of the LDS table, lane l updating copy l % lds_copies (1, 2, 4, 8)
     uint64_t *part;        // block partials of the LDS table (nullptr: merge into HBM directly)
+    int64_t kmin;          // a dense LDS table's record 0 key (DENSE feeds)
 };
 
 // cold words can be touched only by NA inputs and by double min / max
