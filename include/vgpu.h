@@ -52,6 +52,10 @@ int         vgpu_init(int device);
 const char *vgpu_last_error(void);
 void       *vgpu_malloc(size_t bytes);
 int         vgpu_free(void *ptr);
+/* vgpu_memcpy_h2d returns once src has been read (it may be reused at
+ * once); the copy itself completes in order on the calling thread's library
+ * stream, before any later work there (pinned staging, no synchronization).
+ * vgpu_memcpy_d2h returns with the data in dst. */
 int         vgpu_memcpy_h2d(void *dst_device, const void *src_host, size_t bytes);
 int         vgpu_memcpy_d2h(void *dst_host, const void *src_device, size_t bytes);
 int         vgpu_memset(void *dst_device, int value, size_t bytes);

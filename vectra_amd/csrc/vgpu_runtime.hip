@@ -8,6 +8,7 @@
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 #include <mutex>
 #include <vector>
 
@@ -47,10 +48,14 @@ extern "C" void *vgpu_stream_create(void) {
     return s;
 }
 
+static int vg_h2d_order(hipStream_t st);
+
 extern "C" int vgpu_stream_use(void *stream) {
     if (vg_ensure_init()) return -1;
     t_stream = static_cast<hipStream_t>(stream);
-    return 0;
+    // work on the selected stream comes after the staged host -> device
+    // copies this process has issued (vgpu_memcpy_h2d does not wait for them)
+    return vg_h2d_order(vg_stream());
 }
 
 extern "C" int vgpu_stream_destroy(void *stream) {
@@ -242,12 +247,60 @@ extern "C" int vgpu_free(void *p) {
     return 0;
 }
 
+// Host -> device copies of pageable memory go through a ring of pinned
+// slots on the caller's library stream: the host copies a piece of the
+// source into a slot (first waiting for the device copy that last read
+// that slot) and the stream copies the slot to the device.  The call
+// returns once the whole source has been read, without waiting for the
+// device: later work on the same stream sees the data, the source may be
+// reused at once, and the host's copy of the next piece overlaps the
+// device copy of this one.  (A pageable hipMemcpyAsync followed by a
+// stream synchronization cost a round trip per call and ran at a fraction
+// of the link's rate.)
+#define VG_H2D_SLOTS 4
+#define VG_H2D_SLOT_BYTES ((size_t)4 << 20)
+static std::mutex g_h2d_mu;
+static uint8_t *g_h2d_buf[VG_H2D_SLOTS];
+static hipEvent_t g_h2d_ev[VG_H2D_SLOTS];
+static bool g_h2d_busy[VG_H2D_SLOTS];
+static int g_h2d_next = 0;
+
+static int vg_h2d_staged(void *dst, const void *src, size_t bytes) {
+    std::lock_guard<std::mutex> lk(g_h2d_mu);
+    if (!g_h2d_buf[0]) {
+        for (int k = 0; k < VG_H2D_SLOTS; k++) {
+            void *p = nullptr;
+            VG_TRY(hipHostMalloc(&p, VG_H2D_SLOT_BYTES, hipHostMallocDefault));
+            g_h2d_buf[k] = static_cast<uint8_t *>(p);
+            VG_TRY(hipEventCreateWithFlags(&g_h2d_ev[k], hipEventDisableTiming));
+        }
+    }
+    hipStream_t st = vg_stream();
+    for (size_t off = 0; off < bytes; off += VG_H2D_SLOT_BYTES) {
+        const size_t len = std::min(VG_H2D_SLOT_BYTES, bytes - off);
+        const int k = g_h2d_next;
+        g_h2d_next = (k + 1) % VG_H2D_SLOTS;
+        if (g_h2d_busy[k]) VG_TRY(hipEventSynchronize(g_h2d_ev[k]));
+        memcpy(g_h2d_buf[k], static_cast<const uint8_t *>(src) + off, len);
+        VG_TRY(hipMemcpyAsync(static_cast<uint8_t *>(dst) + off, g_h2d_buf[k], len, hipMemcpyHostToDevice, st));
+        VG_TRY(hipEventRecord(g_h2d_ev[k], st));
+        g_h2d_busy[k] = true;
+    }
+    return 0;
+}
+
+// st waits (on the device) for every staged copy still in flight
+static int vg_h2d_order(hipStream_t st) {
+    std::lock_guard<std::mutex> lk(g_h2d_mu);
+    for (int k = 0; k < VG_H2D_SLOTS; k++)
+        if (g_h2d_busy[k]) VG_TRY(hipStreamWaitEvent(st, g_h2d_ev[k], 0));
+    return 0;
+}
+
 extern "C" int vgpu_memcpy_h2d(void *dst, const void *src, size_t bytes) {
     if (vg_ensure_init()) return -1;
     if (!bytes) return 0;
-    VG_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, vg_stream()));
-    VG_TRY(hipStreamSynchronize(vg_stream()));
-    return 0;
+    return vg_h2d_staged(dst, src, bytes);
 }
 
 extern "C" int vgpu_memcpy_d2h(void *dst, const void *src, size_t bytes) {
