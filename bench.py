@@ -876,20 +876,43 @@ def run_c5(args, nv):
     bb, bp = best("batched"), best("plain")
     ratio = bb["achieved_qps"] / bp["achieved_qps"] if bb and bp else None
 
-    # dominant kernel: one bucketed pass of max_batch class-0 queries (and,
-    # for the record, of class-1 queries), timed with events
+    # dominant kernel: one bucketed pass of class-0 queries (and, for the
+    # record, of class-1 queries), timed with events; a pass takes the
+    # serving loop's size for its class: max_batch, or the largest size
+    # whose shared scan is still bucketed (its groups x queries table fits
+    # a CU's LDS), found by the same binary search
     lib = nv.vgpu()
-    K = args.c5_max_batch
-    thr = [1000.0 * (1.0 - s) for s in np.linspace(0.01, 0.5, K)]
     jt = nv.JoinTable(dim["key"], [dim["attr"]])
-    pass_ms = {}
-    for cls in (0, 1):
+    pass_ms, pass_k = {}, {}
+
+    def plan(cls, k):
+        thr = [1000.0 * (1.0 - s) for s in np.linspace(0.01, 0.5, k)]
         pred = nv.cmp_const_prog(0, nv.VEC_DOUBLE, "GT", 0.0)
         b = nv.Batch(nv.VEC_INT64, [nv.VEC_DOUBLE], [("sum", 0, 0), ("n", -1, 0)], nv.VGPU_ORDER_KEY, pred, 1, 1)
         if cls == 1:
             b.join(jt, 0)
         side = {"key": fact["group"]} if cls == 0 else {"probe_key": fact["key"]}
         b.plan([dict(side, inputs=[fact["value"]], pred_cols=[fact["value"]], n_rows=n_fact, param=t) for t in thr])
+        return b, thr
+
+    for cls in (0, 1):
+        K = args.c5_max_batch
+        b, thr = plan(cls, K)
+        if not b.bucketed() and K > 1:
+            lo, hi, best = 0, K - 1, None
+            while lo < hi:
+                mid = (lo + hi + 1) // 2
+                t, tt = plan(cls, mid)
+                if t.bucketed():
+                    if best:
+                        best[0].free()
+                    best, lo = (t, tt), mid
+                else:
+                    t.free()
+                    hi = mid - 1
+            if best:
+                b.free()
+                (b, thr), K = best, lo
         bucketed = b.bucketed()
         for _ in range(args.warmup):
             b.rerun(thr, K)
@@ -899,11 +922,13 @@ def run_c5(args, nv):
             b.rerun(thr, K)
         nl = C.c_int64()
         pass_ms[cls] = lib.vgpu_kernel_time_ms(C.byref(nl), 1) / max(nl.value, 1)
+        pass_k[cls] = K
         lib.vgpu_kernel_timing(0)
         b.free()
         log(f"[bench] c5 class-{cls} pass of {K} queries (bucketed {bucketed}): k_bt_chain_feed "
             f"{pass_ms[cls]:.3f} ms avg over {nl.value} launches")
     jt.free()
+    K = pass_k[0]
     kms = pass_ms[0]
     alg = n_fact * 16   # value + group, read once per pass
     achieved = alg / (kms / 1e3) / 1e9
@@ -940,8 +965,10 @@ def run_c5(args, nv):
                **hostinfo}
     traffic = None
     tfile = ROOT / "profiles" / "k_bt_chain_feed_serving_1B_rows.json"
-    if tfile.exists() and n_fact == 1_000_000_000 and K == 8:
-        traffic = json.loads(tfile.read_text())["traffic_bytes_per_launch"]
+    if tfile.exists() and n_fact == 1_000_000_000:
+        prof = json.loads(tfile.read_text())
+        if prof.get("queries_per_pass") == K:
+            traffic = prof["traffic_bytes_per_launch"]
     # a step is one request: steps = the requests served at the reported load
     return {
         "metric": METRIC, "value": bb["achieved_qps"] if bb else 0.0, "unit": "queries/s", "n_gpus": 1,
@@ -963,7 +990,8 @@ def run_c5(args, nv):
                      if traffic else None,
                      "kernel": f"k_bt_chain_feed<true> (the scan of a bucketed pass of {K} class-0 queries)",
                      "bytes_per_launch": alg, "avg_launch_ms": kms, "launches_timed": args.steps},
-        "pass_ms": {"class0_group_by": pass_ms[0], "class1_join_group_by": pass_ms[1], "queries_per_pass": K},
+        "pass_ms": {"class0_group_by": pass_ms[0], "class1_join_group_by": pass_ms[1],
+                    "queries_per_pass": {"class0": pass_k[0], "class1": pass_k[1]}},
         "cpu_baseline": cpu,
     }
 
@@ -1014,7 +1042,7 @@ def main():
     ap.add_argument("--c4-card", type=int, default=0, help="fix every query's key cardinality (0: log-uniform)")
     ap.add_argument("--c5-rows", type=int, default=1_000_000_000)
     ap.add_argument("--c5-dim", type=int, default=10_000_000)
-    ap.add_argument("--c5-max-batch", type=int, default=8)
+    ap.add_argument("--c5-max-batch", type=int, default=32)
     ap.add_argument("--c5-cpu-rows", type=int, default=100_000_000, help="fact rows of the CPU baseline's sample")
     ap.add_argument("--c5-in-flight", type=int, default=2, help="passes on the device at once (batched mode)")
     ap.add_argument("--c5-hbm-budget", type=int, default=0, help="device bytes the serving run may hold (0: all)")
@@ -1022,7 +1050,7 @@ def main():
     ap.add_argument("--c5-slo-ms", type=float, default=100.0)
     ap.add_argument("--c5-plain-rates", type=float, nargs="*", default=[10, 25, 40, 55, 70, 85, 100])
     ap.add_argument("--c5-batched-rates", type=float, nargs="*",
-                    default=[50, 100, 200, 400, 800, 1200, 1600, 1800, 2000, 2200, 2400, 2600])
+                    default=[50, 100, 200, 400, 800, 1600, 2400, 3200, 3600, 4000, 4400, 4800])
     ap.add_argument("--c3-rows", type=int, default=200_000_000)
     ap.add_argument("--vtr-rows", type=int, default=50_000_000)
     ap.add_argument("--str-rows", type=int, default=10_000_000)

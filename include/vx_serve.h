@@ -20,8 +20,10 @@
  *   batched   continuous batching: up to in_flight passes run on the device
  *             at once, each on a stream of its own; whenever a pass slot is
  *             free, the oldest waiting request's class forms the next pass
- *             from every waiting request of that class (up to max_batch, and
- *             as many as the per-pass state budget admits).  A pass is ONE
+ *             from every waiting request of that class (up to max_batch, or
+ *             fewer for a class whose shared scan keeps a table of groups x
+ *             requests in on-chip memory: the largest pass whose table still
+ *             fits; and as many as the per-pass state budget admits).  A pass is ONE
  *             shared scan of the fact columns for all its requests; requests
  *             that arrive while passes run join the next pass as soon as a
  *             slot frees, of either class, while the others still run.

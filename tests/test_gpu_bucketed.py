@@ -259,8 +259,7 @@ def test_bucketed_join(gpu, oracle, dup, dense):
         c.free()
 
 
-@pytest.mark.parametrize("n", [260_000, 0, 1, 1_000, 8_193, 1_048_577])
-@pytest.mark.parametrize("plain", [False, True])
+@pytest.mark.parametrize("plain,n", [(False, 260_000)] + [(True, n) for n in (260_000, 0, 1, 1_000, 8_193, 1_048_577)])
 @pytest.mark.parametrize("join", [False, True])
 def test_bucketed_sums_the_compared_column(gpu, oracle, join, plain, n):
     """filter(p > t_q) |> [inner_join(dim) |>] group_by(...) |> summarise(
@@ -270,8 +269,6 @@ def test_bucketed_sums_the_compared_column(gpu, oracle, join, plain, n):
     so the scan loads both columns without validity checks (rows past the
     end of a block's range masked out): row counts below one step, one past
     a step and one past a power of two leave partial last steps"""
-    if n != 260_000 and not plain:
-        pytest.skip("the row-count sweep is for the plain scan")
     if join:
         tbl, dim = join_tables(101 + n, n, 20_000, False)
         key = "attr"
@@ -326,17 +323,19 @@ def test_bucketed_sums_the_compared_column(gpu, oracle, join, plain, n):
         c.free()
 
 
-I64_T = [0, 5, 5, -300, 999, 2**63 - 1, -(2**63) + 1, 42, -7]
-DBL_T9 = [0.0, -0.0, 0.5, 0.5, float("nan"), float("inf"), -float("inf"), -1.2, 1.7]
+I64_T = [0, 5, 5, -300, 999, 2**63 - 1, -(2**63) + 1, 42, -7, 1, -1, 500, -500, 6, 4, 1000, -999]
+DBL_T = [0.0, -0.0, 0.5, 0.5, float("nan"), float("inf"), -float("inf"), -1.2, 1.7,
+         -0.5, 0.25, float("nan"), 3.0, -3.0, 0.0, 1e-300, -2.2]
 
 
-@pytest.mark.parametrize("k", [8, 9])
+@pytest.mark.parametrize("k", [8, 9, 13, 16, 17])
 @pytest.mark.parametrize("dtype", ["double", "int64"])
 @pytest.mark.parametrize("op", list(OPS))
 def test_bucketed_plain_scan_every_operator(gpu, oracle, op, dtype, k):
     """the plain scan (no NAs in the compared column or the key, one sum of
-    the compared column) with up to eight queries counts each row's passes
-    against the constants in registers; nine take the search: every
+    the compared column) with up to sixteen queries counts each row's passes
+    against the constants in registers (8 compare slots up to eight queries,
+    16 up to sixteen); seventeen take the search: every
     operator on double and int64 columns, NaN / +-0 / +-inf and extreme
     int64 constants (INT64_MAX, INT64_MIN + 1), duplicate constants,
     values equal to a constant"""
@@ -349,7 +348,7 @@ def test_bucketed_plain_scan_every_operator(gpu, oracle, op, dtype, k):
         p[u < 0.01] = np.inf
         p[(u >= 0.01) & (u < 0.02)] = -np.inf
         p[(u >= 0.02) & (u < 0.04)] = 0.5
-        ts = DBL_T9[:k]
+        ts = DBL_T[:k]
         ptype, lit = gpu.VEC_DOUBLE, "dbl"
     else:
         p = r.integers(-1000, 1000, n)

@@ -701,19 +701,21 @@ __device__ __forceinline__ int bt_chain_T(const BtChainArgs &A, const uint64_t *
 // !(v < t) and `<=` iff !(v > t): GREATER counts v > t, else v < t, and NEG
 // (>= / <=) takes K minus the count.  Slots past K hold a constant that no
 // value passes the counted comparison with (+inf / INT64_MAX for v > t,
-// -inf / INT64_MIN for v < t), so the count runs over all BT_CH_LIN slots.
-#define BT_CH_LIN 8
+// -inf / INT64_MIN for v < t), so the count runs over all NL slots (8 for up
+// to 8 queries, 16 for up to 16).
+#define BT_CH_LIN 16
 // TM (T mode): 0 = the search above; 1 + 2*int64 + less: the count
 #define BT_TM_SEARCH 0
 __host__ __device__ constexpr int bt_tm(bool f64, bool greater) { return 1 + (f64 ? 0 : 2) + (greater ? 0 : 1); }
 
-template <int TM>
+template <int TM, int NL>
 __device__ __forceinline__ int bt_chain_T_lin(const uint64_t (&kr)[BT_CH_LIN], uint64_t bits, int K, bool neg) {
+    static_assert(NL == 8 || NL == 16, "8 or 16 compare slots");
     constexpr bool F64 = TM == 1 || TM == 2;
     constexpr bool GREATER = TM == 1 || TM == 3;
     int c = 0;
 #pragma unroll
-    for (int i = 0; i < BT_CH_LIN; i++) {
+    for (int i = 0; i < NL; i++) {
         bool pass;
         if (F64) {
             const double v = vg_bits_to_f64(bits), t = vg_bits_to_f64(kr[i]);
@@ -728,14 +730,14 @@ __device__ __forceinline__ int bt_chain_T_lin(const uint64_t (&kr)[BT_CH_LIN], u
 }
 
 // the constants of a linear count in registers (uniform loads), padded
-template <int TM>
+template <int TM, int NL>
 __device__ __forceinline__ void bt_chain_load_lin(const BtChainArgs &A, uint64_t (&kr)[BT_CH_LIN]) {
     constexpr bool F64 = TM == 1 || TM == 2;
     constexpr bool GREATER = TM == 1 || TM == 3;
     const uint64_t pad = F64 ? (GREATER ? 0x7FF0000000000000ull : 0xFFF0000000000000ull)
                              : (GREATER ? 0x7FFFFFFFFFFFFFFFull : 0x8000000000000000ull);
 #pragma unroll
-    for (int i = 0; i < BT_CH_LIN; i++) kr[i] = i < A.K ? A.ks[i] : pad;
+    for (int i = 0; i < BT_CH_LIN; i++) kr[i] = i < NL && i < A.K ? A.ks[i] : pad;
 }
 
 // dense entry of a group key, the NA group last; -1 outside the plan's range
@@ -829,7 +831,7 @@ __device__ __forceinline__ void bt_chain_rows(const BtChainArgs &A, const vgpu_c
 
 // the bucket of every row of a step that passes at least one query, and
 // its aggregation into the block's table
-template <int TM, int U, bool SP>
+template <int TM, int NL, int U, bool SP>
 __device__ __forceinline__ void bt_chain_rows_add(const BtChainArgs &A, const vgpu_col &kc, const uint64_t *s_k,
                                                   const uint64_t (&kr)[BT_CH_LIN], bool neg, uint32_t *cnt,
                                                   uint64_t *sums, const BtRows<U, SP> &R, bool &err) {
@@ -838,7 +840,7 @@ __device__ __forceinline__ void bt_chain_rows_add(const BtChainArgs &A, const vg
 #pragma unroll
     for (int u = 0; u < U; u++) {
         if (TM == BT_TM_SEARCH) T[u] = (R.ok >> u) & 1 ? bt_chain_T(A, s_k, R.pv[u]) : 0;
-        else T[u] = (R.ok >> u) & 1 ? bt_chain_T_lin<TM>(kr, R.pv[u], A.K, neg) : 0;
+        else T[u] = (R.ok >> u) & 1 ? bt_chain_T_lin<TM, NL>(kr, R.pv[u], A.K, neg) : 0;
 #pragma unroll
         for (int s = 0; s < BT_CH_MAX_SUMS; s++) sv[u][s] = SP ? R.pv[u] : R.sv[SP ? 0 : u][s];
     }
@@ -909,14 +911,14 @@ __device__ __forceinline__ void bt_chain_rows_add(const BtChainArgs &A, const vg
 // before the current step is aggregated, so a wave keeps its loads in
 // flight while it works on LDS: with one 1,024-thread block per CU the
 // scan is otherwise bound by the loads the 16 waves have outstanding.
-// TM != 0 (a linear count, PLAIN only): at most BT_CH_LIN active queries,
+// TM != 0 (a linear count, PLAIN only): at most NL active queries,
 // one sum; neg: the operator is >= or <=.
-template <bool SP, bool PLAIN, int TM>
+template <bool SP, bool PLAIN, int TM, int NL = 8>
 __global__ void __launch_bounds__(BT_CH_THREADS) k_bt_chain_feed(BtChainArgs A, int neg) {
     static_assert(TM == BT_TM_SEARCH || PLAIN, "the linear count runs on the plain scan");
     constexpr int U = SP ? BT_CH_U : BT_CH_U / 2;
     uint64_t kr[BT_CH_LIN];
-    if (TM != BT_TM_SEARCH) bt_chain_load_lin<TM>(A, kr);
+    if (TM != BT_TM_SEARCH) bt_chain_load_lin<TM, NL>(A, kr);
     extern __shared__ uint64_t dynw[];
     uint64_t *s_k = dynw;                                   // m constants
     uint64_t *sums = dynw + A.m;                            // E*m*n_sum
@@ -936,7 +938,7 @@ __global__ void __launch_bounds__(BT_CH_THREADS) k_bt_chain_feed(BtChainArgs A, 
         bt_chain_rows<U, SP, PLAIN>(A, kc, r0 + threadIdx.x, r1, cur);
         for (int64_t base = r0 + threadIdx.x; base < r1; base += step) {
             bt_chain_rows<U, SP, PLAIN>(A, kc, base + step, r1, nxt);
-            bt_chain_rows_add<TM>(A, kc, s_k, kr, neg != 0, cnt, sums, cur, err);
+            bt_chain_rows_add<TM, NL>(A, kc, s_k, kr, neg != 0, cnt, sums, cur, err);
             cur = nxt;
         }
     }
@@ -1196,6 +1198,10 @@ static int bt_chain_plan(vgpu_batch *b, const BtQuery &Q0) {
                              reinterpret_cast<const void *>(k_bt_chain_feed<true, true, 2>),
                              reinterpret_cast<const void *>(k_bt_chain_feed<true, true, 3>),
                              reinterpret_cast<const void *>(k_bt_chain_feed<true, true, 4>),
+                             reinterpret_cast<const void *>(k_bt_chain_feed<true, true, 1, 16>),
+                             reinterpret_cast<const void *>(k_bt_chain_feed<true, true, 2, 16>),
+                             reinterpret_cast<const void *>(k_bt_chain_feed<true, true, 3, 16>),
+                             reinterpret_cast<const void *>(k_bt_chain_feed<true, true, 4, 16>),
                              reinterpret_cast<const void *>(k_bt_chain_feed<true, true, BT_TM_SEARCH>),
                              reinterpret_cast<const void *>(k_bt_chain_feed<true, false, BT_TM_SEARCH>),
                              reinterpret_cast<const void *>(k_bt_chain_feed<false, false, BT_TM_SEARCH>)};
@@ -1656,11 +1662,16 @@ static int bt_enqueue(vgpu_batch *b, bool first) {
         const int neg = op == VGPU_OP_GE || op == VGPU_OP_LE;
         const dim3 g((unsigned)C.G), t(BT_CH_THREADS);
         const size_t lds = b->ch_lds;
-        switch (plain ? tm : -1) {
+        // the count over 8 compare slots for up to 8 queries, 16 for up to 16
+        switch (plain ? (tm != BT_TM_SEARCH && C.K > 8 ? tm + 16 : tm) : -1) {
         case 1: k_bt_chain_feed<true, true, 1><<<g, t, lds, st>>>(C, neg); break;
         case 2: k_bt_chain_feed<true, true, 2><<<g, t, lds, st>>>(C, neg); break;
         case 3: k_bt_chain_feed<true, true, 3><<<g, t, lds, st>>>(C, neg); break;
         case 4: k_bt_chain_feed<true, true, 4><<<g, t, lds, st>>>(C, neg); break;
+        case 17: k_bt_chain_feed<true, true, 1, 16><<<g, t, lds, st>>>(C, neg); break;
+        case 18: k_bt_chain_feed<true, true, 2, 16><<<g, t, lds, st>>>(C, neg); break;
+        case 19: k_bt_chain_feed<true, true, 3, 16><<<g, t, lds, st>>>(C, neg); break;
+        case 20: k_bt_chain_feed<true, true, 4, 16><<<g, t, lds, st>>>(C, neg); break;
         case BT_TM_SEARCH: k_bt_chain_feed<true, true, BT_TM_SEARCH><<<g, t, lds, st>>>(C, neg); break;
         default:
             if (sp) k_bt_chain_feed<true, false, BT_TM_SEARCH><<<g, t, lds, st>>>(C, neg);

@@ -195,6 +195,7 @@ typedef struct {
     int       started;
     void     *stream;
     vgpu_batch *tpl[2];
+    int       kcap[2];     /* requests a pass of each class takes (its template's size) */
     int64_t   tpl_bytes;
     int       ready;       /* templates planned (1) or failed (-1) */
     int       state;       /* 0 idle, 1 pass assigned, 2 stop */
@@ -212,6 +213,7 @@ typedef struct SvSched {
     const vx_request *reqs;
     vx_result *out;
     int       kmax;
+    int       kcap[2];     /* requests per pass of each class (at most kmax) */
     double    t0;
     int       done, in_flight, failed;
     char      err[512];
@@ -224,6 +226,37 @@ static void *sv_slot_main(void *arg) {
     for (int cls = 0; cls < 2 && ok; cls++) {
         w->tpl[cls] = sv_template(S->c, S->jt, cls, S->kmax);
         ok = w->tpl[cls] != NULL;
+        w->kcap[cls] = S->kmax;
+        /* A template too large for the bucketed scan (its per-block table
+         * of groups x requests outgrows LDS) would run every request of a
+         * pass as a scan of its own: take the largest pass that is still
+         * bucketed instead (the table grows with the requests, so a binary
+         * search finds it); none: keep kmax. */
+        if (ok && S->kmax > 1 && !vgpu_batch_bucketed(w->tpl[cls])) {
+            int lo = 0, hi = S->kmax - 1;   /* largest bucketed size in [1, kmax) is in (lo, hi] or none */
+            vgpu_batch *best = NULL;
+            while (lo < hi && ok) {
+                const int mid = (lo + hi + 1) / 2;
+                vgpu_batch *t = sv_template(S->c, S->jt, cls, mid);
+                ok = t != NULL;
+                if (!ok) break;
+                if (vgpu_batch_bucketed(t)) {
+                    if (best) vgpu_batch_destroy(best);
+                    best = t;
+                    lo = mid;
+                } else {
+                    vgpu_batch_destroy(t);
+                    hi = mid - 1;
+                }
+            }
+            if (!ok) {
+                if (best) vgpu_batch_destroy(best);
+            } else if (best) {
+                vgpu_batch_destroy(w->tpl[cls]);
+                w->tpl[cls] = best;
+                w->kcap[cls] = lo;
+            }
+        }
     }
     if (ok) {
         w->tpl_bytes = vgpu_batch_device_bytes(w->tpl[0]) + vgpu_batch_device_bytes(w->tpl[1]);
@@ -340,16 +373,22 @@ static int sv_run_batched(const vx_serve_cfg *c, vgpu_jtable *jt, const vx_reque
     }
     st->slots = n_slots;
     st->resident_bytes = resident;
+    S.kcap[0] = S.kcap[1] = kmax;
     if (rc == 0 && n_slots > 0) {
+        /* every slot planned the same templates */
+        for (int cls = 0; cls < 2; cls++) S.kcap[cls] = slots[0].kcap[cls];
         /* a request holds its share of the larger template */
         int64_t per = 0;
         for (int cls = 0; cls < 2; cls++) {
-            int64_t b = vgpu_batch_device_bytes(slots[0].tpl[cls]) / kmax;
+            int64_t b = vgpu_batch_device_bytes(slots[0].tpl[cls]) / S.kcap[cls];
             if (b > per) per = b;
         }
         st->state_bytes = per;
-        if (c->state_budget > 0 && per > 0 && c->state_budget / per < kmax)
+        if (c->state_budget > 0 && per > 0 && c->state_budget / per < kmax) {
             kmax = (int)(c->state_budget / per) > 0 ? (int)(c->state_budget / per) : 1;
+            for (int cls = 0; cls < 2; cls++)
+                if (S.kcap[cls] > kmax) S.kcap[cls] = kmax;
+        }
     }
     int *pending = (int *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int));
     if (!pending && rc == 0) rc = vgpu_error("serve: alloc failed");
@@ -367,7 +406,7 @@ static int sv_run_batched(const vx_serve_cfg *c, vgpu_jtable *jt, const vx_reque
             int cls = reqs[pending[0]].cls, k = 0, wr = 0;
             for (int j = 0; j < n_pend; j++) {
                 int i = pending[j];
-                if (reqs[i].cls == cls && k < kmax) w->take[k++] = i;
+                if (reqs[i].cls == cls && k < S.kcap[cls]) w->take[k++] = i;
                 else pending[wr++] = i;
             }
             n_pend = wr;
