@@ -772,15 +772,18 @@ __global__ void __launch_bounds__(GB_THREADS) k_gb_part_fold_dense(GbLayout L, c
     if (gs >= 0) gb_merge(L, hot + (size_t)gs * L.RH, cold + (size_t)gs * L.RC, rec, rec + L.RH);
 }
 
+// DENSE: the partials are dense tables (record s keyed kmin + s), folded
+// record s into record s of a dense LDS table
+template <bool DENSE>
 __global__ void __launch_bounds__(GB_THREADS) k_gb_part_fold(GbLayout L, const uint64_t *part, int64_t nparts, int S,
                                                              int per, uint64_t *hot, uint64_t *cold, int64_t cap,
-                                                             uint64_t *ctl) {
+                                                             uint64_t *ctl, int64_t kmin) {
     extern __shared__ uint64_t dyn_fold[];
     const int RW = L.RH + L.RC;
     uint64_t *lhot = dyn_fold;
     uint64_t *lcold = lhot + (size_t)(S + 2) * L.RH;
     const uint64_t empty = L.key_is_double ? VG_EMPTY_F64 : VG_EMPTY_I64;
-    gb_lds_init(L, lhot, lcold, S, empty);
+    gb_lds_init(L, lhot, lcold, S, empty, DENSE, kmin);
     const int64_t b0 = (int64_t)blockIdx.x * per;
     const int64_t nb = nparts - b0 < per ? nparts - b0 : per;
     const int64_t entries = nb * (S + 2);
@@ -789,7 +792,7 @@ __global__ void __launch_bounds__(GB_THREADS) k_gb_part_fold(GbLayout L, const u
         const int s = (int)(q % (S + 2));
         const uint64_t *r = part + ((size_t)b * (S + 2) + s) * RW;
         if (r[1] == 0) continue;
-        const int64_t ls = s < S ? gb_lds_slot(lhot, S, L.RH, VG_KEY_REGULAR, r[0], empty) : s;
+        const int64_t ls = s < S && !DENSE ? gb_lds_slot(lhot, S, L.RH, VG_KEY_REGULAR, r[0], empty) : s;
         if (ls >= 0) gb_merge(L, lhot + (size_t)ls * L.RH, lcold + (size_t)ls * L.RC, r, r + L.RH);
         else gb_merge_global(L, r, r + L.RH, hot, cold, cap, ctl);
     }
@@ -2880,15 +2883,19 @@ static int gb_part_setup(vgpu_groupby *g, int64_t grid, int S, uint64_t **part) 
 #define GB_FOLD_PER 32   // partials per fold block of a hashed table
 
 // folds the partials of `grid` blocks into the HBM table
-static int gb_part_fold(vgpu_groupby *g, const uint64_t *part, int64_t grid, int S, bool dense) {
+static int gb_part_fold(vgpu_groupby *g, const uint64_t *part, int64_t grid, int S, bool dense, int64_t kmin) {
     hipStream_t st = vg_stream();
     uint64_t *hot = g->hot.as<uint64_t>(), *cold = g->cold.as<uint64_t>(), *ctl = g->ctl.as<uint64_t>();
-    if (dense) {
+    const size_t lds = (size_t)(S + 2) * (g->L.RH + g->L.RC) * 8;
+    const unsigned nb = (unsigned)((grid + GB_FOLD_PER - 1) / GB_FOLD_PER);
+    // a dense table of many records: one block per record; of few records
+    // (too few blocks to fill the device): one block per GB_FOLD_PER partials
+    if (dense && S + 2 >= 256) {
         k_gb_part_fold_dense<<<(unsigned)(S + 2), GB_THREADS, 0, st>>>(g->L, part, grid, S, hot, cold, g->cap, ctl);
+    } else if (dense) {
+        k_gb_part_fold<true><<<nb, GB_THREADS, lds, st>>>(g->L, part, grid, S, GB_FOLD_PER, hot, cold, g->cap, ctl, kmin);
     } else {
-        const size_t lds = (size_t)(S + 2) * (g->L.RH + g->L.RC) * 8;
-        k_gb_part_fold<<<(unsigned)((grid + GB_FOLD_PER - 1) / GB_FOLD_PER), GB_THREADS, lds, st>>>(
-            g->L, part, grid, S, GB_FOLD_PER, hot, cold, g->cap, ctl);
+        k_gb_part_fold<false><<<nb, GB_THREADS, lds, st>>>(g->L, part, grid, S, GB_FOLD_PER, hot, cold, g->cap, ctl, 0);
     }
     VG_LAUNCH_CHECK();
     return 0;
@@ -2952,7 +2959,7 @@ static int gb_feed_narrow(vgpu_groupby *g, const FeedArgs &F, const vgpu_col *in
         else k_gb_feed_narrow<2, false><<<grid, GB_THREADS, lds, st>>>(L, A);
     }
     VG_LAUNCH_CHECK();
-    if (A.part && gb_part_fold(g, A.part, grid, A.S, dense)) return -1;
+    if (A.part && gb_part_fold(g, A.part, grid, A.S, dense, A.kmin)) return -1;
     vg_kt_end();
     g->any_fed = 1;
     return 0;
@@ -3190,7 +3197,7 @@ static int groupby_feed_body(vgpu_groupby *g, const vgpu_col *key, const vgpu_co
                    : (dense ? k_gb_feed<0, true> : k_gb_feed<0, false>);
     feed<<<grid, GB_THREADS, lds, vg_stream()>>>(g->L, A, ins, pi, prog, pcs);
     VG_LAUNCH_CHECK();
-    if (A.part && gb_part_fold(g, A.part, grid, A.lds_slots, dense)) return -1;
+    if (A.part && gb_part_fold(g, A.part, grid, A.lds_slots, dense, A.kmin)) return -1;
     vg_kt_end();
     g->any_fed = 1;
     return 0;
@@ -3513,7 +3520,7 @@ static int groupby_feed_join_body(vgpu_groupby *g, const vgpu_feed_spec *sp) {
         }
 #undef STAR_DENSE
         VG_LAUNCH_CHECK();
-        if (sd.part && gb_part_fold(g, sd.part, grid, sd.S, true)) return -1;
+        if (sd.part && gb_part_fold(g, sd.part, grid, sd.S, true, sd.kmin)) return -1;
         vg_kt_end();
         g->any_fed = 1;
         return 0;
