@@ -1042,7 +1042,7 @@ def main():
     ap.add_argument("--c4-card", type=int, default=0, help="fix every query's key cardinality (0: log-uniform)")
     ap.add_argument("--c5-rows", type=int, default=1_000_000_000)
     ap.add_argument("--c5-dim", type=int, default=10_000_000)
-    ap.add_argument("--c5-max-batch", type=int, default=32)
+    ap.add_argument("--c5-max-batch", type=int, default=64)
     ap.add_argument("--c5-cpu-rows", type=int, default=100_000_000, help="fact rows of the CPU baseline's sample")
     ap.add_argument("--c5-in-flight", type=int, default=2, help="passes on the device at once (batched mode)")
     ap.add_argument("--c5-hbm-budget", type=int, default=0, help="device bytes the serving run may hold (0: all)")
@@ -1050,7 +1050,7 @@ def main():
     ap.add_argument("--c5-slo-ms", type=float, default=100.0)
     ap.add_argument("--c5-plain-rates", type=float, nargs="*", default=[10, 25, 40, 55, 70, 85, 100])
     ap.add_argument("--c5-batched-rates", type=float, nargs="*",
-                    default=[50, 100, 200, 400, 800, 1600, 2400, 3200, 3600, 4000, 4400, 4800])
+                    default=[50, 100, 200, 400, 800, 1600, 2400, 3200, 3600, 4000, 4400, 4800, 5200])
     ap.add_argument("--c3-rows", type=int, default=200_000_000)
     ap.add_argument("--vtr-rows", type=int, default=50_000_000)
     ap.add_argument("--str-rows", type=int, default=10_000_000)
