@@ -556,3 +556,70 @@ def test_groupby_partition_staging(gpu, part_threshold, stage, agg, range_mode, 
             assert same, col
         else:
             assert np.array_equal(a, b), col
+
+
+@pytest.mark.parametrize("special", ["none", "zero", "negzero", "nan"])
+@pytest.mark.parametrize("range_mode", [2, 1])
+@pytest.mark.parametrize("n_inputs", [1, 2])
+def test_groupby_partition_rows_without_meta(gpu, part_threshold, special, range_mode, n_inputs):
+    """plain key-ordered feeds stage rows without their meta word (key and
+    inputs only) unless a double min / max input holds a NaN or a zero,
+    which needs positions: then the feed restarts with meta words.  Both
+    against the row-wise passes (gb.part.stage 1) and the oracle-free
+    closed forms: keys, order, counts, min and max bit-exact (the NaN-first
+    and signed-zero rules included), sums within 1e-9 of max(1, |sum|)"""
+    n = 1_300_000
+    r = rng(4100 + 10 * n_inputs + range_mode)
+    k = r.integers(-10**12, 10**12, 300_000)[r.integers(0, 300_000, n)]
+    types, datas = [], []
+    for i in range(n_inputs):
+        if i % 2 == 0:
+            d = r.normal(size=n) * 1e3
+            if special == "zero":
+                d[r.integers(0, n, 50)] = 0.0
+            elif special == "negzero":
+                d[r.integers(0, n, 50)] = -0.0
+            elif special == "nan":
+                d[r.integers(0, n, 50)] = np.nan
+            types.append(gpu.VEC_DOUBLE)
+            datas.append(d)
+        else:
+            types.append(gpu.VEC_INT64)
+            datas.append(r.integers(-10**9, 10**9, n))
+    cols = [gpu.DeviceColumn(k)] + [gpu.DeviceColumn(d) for d in datas]
+    lib = gpu.vgpu()
+    knobs = (b"gb.part.stage", b"gb.part.range", b"gb.part.agg")
+    old = [lib.vgpu_tune_get(x) for x in knobs]
+    res = {}
+    try:
+        assert lib.vgpu_tune_set(b"gb.part.range", range_mode) == 0
+        assert lib.vgpu_tune_set(b"gb.part.agg", 3) == 0
+        for mode in (3, 1):
+            assert lib.vgpu_tune_set(b"gb.part.stage", mode) == 0
+            gb = gpu.GroupBy(gpu.VEC_INT64, types, STAGE_AGGS[n_inputs], gpu.VGPU_ORDER_KEY, 0)
+            gb.feed(cols[0], cols[1:], n)
+            gb.finish()
+            res[mode] = gb.fetch()
+            gb.free()
+    finally:
+        for x, v in zip(knobs, old):
+            lib.vgpu_tune_set(x, v)
+        for c in cols:
+            c.free()
+    uk, cnt = np.unique(k, return_counts=True)
+    got, ref = res[3], res[1]
+    ng = len(uk)
+    assert np.array_equal(got["key"][0][:ng], uk)
+    assert np.array_equal(got["a0"][0][:ng], cnt)
+    for col in got:
+        assert np.array_equal(got[col][1], ref[col][1]), col
+        a, b = got[col][0], ref[col][0]
+        if a.dtype == np.float64 and col != "key":
+            bits_equal = np.array_equal(a.view(np.uint64), b.view(np.uint64))
+            if col in ("a2", "a3"):   # min and max of the double input: bit for bit
+                assert bits_equal, col
+                continue
+            close = np.all((np.isnan(a) & np.isnan(b)) | (np.abs(a - b) <= 1e-9 * np.maximum(1.0, np.abs(b))))
+            assert bits_equal or close, col
+        else:
+            assert np.array_equal(a, b), col

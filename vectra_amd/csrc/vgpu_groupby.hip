@@ -61,6 +61,13 @@ struct GbpArgs {
     // so the partitions, and the items in them, follow key order; every
     // item writes its records sorted by key, and finish needs no sort
     int32_t range, rshift;         // f = min((ordered key - omin) >> rshift, fmax)
+    // staged rows without the meta word (W = inputs + 1): a plain feed (no
+    // predicate, selection or NAs; int64 key) in key order whose double
+    // min / max inputs hold no NaN or zero, so no position is ever read: a
+    // row's key class follows from its key, every input is valid, and its
+    // position reads as 0 (the first pass sets chk[2] when it meets a NaN
+    // or a zero, and the feed restarts with meta words)
+    int32_t nometa, pad_nm;
     uint64_t omin, fmax, fmul;     // partition (f * fmul) >> 32, fmul = (P - 1) * 2^32 / (fmax + 1)
     int64_t n_rows;                // logical rows of the feed (bounds every buffer)
     int64_t T1;                    // first-pass rows per block
@@ -1327,6 +1334,7 @@ k_gbp_hist1(FeedArgs A, GbpArgs G, PredInfo pi, vgpu_prog prog, ColSet pcs) {
     uint8_t *regv = dyn + (size_t)prog.n_insn * GB_THREADS * 8;
     uint32_t *hist = reinterpret_cast<uint32_t *>(dyn + A.vm_lds);
     const int NB = 1 << G.B1;
+    if (blockIdx.x == 0 && threadIdx.x == 0) G.chk[2] = 0;   // the first pass's NaN / zero flag
     for (int b = threadIdx.x; b < NB; b += GB_THREADS) hist[b] = 0;
     __syncthreads();
     const int64_t r0 = (int64_t)blockIdx.x * G.T1;
@@ -1497,7 +1505,7 @@ __device__ __forceinline__ int64_t gbp_h2_index(const GbpArgs &G, int64_t b, int
 }
 
 __device__ __forceinline__ uint32_t gbp_sub(const GbpArgs &G, const uint64_t *w) {
-    int cls = (int)((w[G.W - 1] >> 32) & 3);
+    int cls = G.nometa ? gbp_plain_class(w[0]) : (int)((w[G.W - 1] >> 32) & 3);
     return (uint32_t)((gbp_pkey(G, cls, w[0]) >> (64 - G.B1 - G.B2)) & ((1u << G.B2) - 1));
 }
 
@@ -1705,7 +1713,7 @@ __device__ __forceinline__ void gbp_stage_write(GbpStage &S, int W, int NBk, int
 template <int WM, int SU>
 __device__ __forceinline__ void gbp_load1(const GbLayout &L, const FeedArgs &A, const GbpArgs &G, const InSet &ins,
                                           int64_t t0, int n, uint64_t valid_bits, uint64_t (&w)[SU][WM],
-                                          uint32_t (&bk)[SU]) {
+                                          uint32_t (&bk)[SU], bool &nan_or_zero) {
     const uint64_t *kp = static_cast<const uint64_t *>(A.key.data);
     const int W = G.W;
 #pragma unroll
@@ -1725,10 +1733,22 @@ __device__ __forceinline__ void gbp_load1(const GbLayout &L, const FeedArgs &A, 
         const int64_t rr = t0 + (j < n ? j : n - 1);
         const uint64_t k = w[u][0];
         const int cls = gbp_plain_class(k);
-        const uint64_t meta = (uint64_t)(uint32_t)rr | ((uint64_t)cls << 32) | valid_bits;
+        if (G.nometa) {
+            // a NaN or a zero in a double min / max input needs positions
 #pragma unroll
-        for (int q = 1; q < WM; q++)
-            if (q == W - 1) w[u][q] = meta;
+            for (int q = 1; q < WM; q++) {
+                const int i = q - 1;
+                if (i < L.n_inputs && L.c_nan[i] >= 0) {
+                    const uint64_t a = w[u][q] & 0x7FFFFFFFFFFFFFFFull;
+                    nan_or_zero = nan_or_zero || a > 0x7FF0000000000000ull || a == 0;
+                }
+            }
+        } else {
+            const uint64_t meta = (uint64_t)(uint32_t)rr | ((uint64_t)cls << 32) | valid_bits;
+#pragma unroll
+            for (int q = 1; q < WM; q++)
+                if (q == W - 1) w[u][q] = meta;
+        }
         bk[u] = (uint32_t)(gbp_pkey(G, cls, k) >> (64 - G.B1));
     }
 }
@@ -1751,14 +1771,17 @@ __global__ void __launch_bounds__(GB_THREADS) k_gbp_scatter1s(GbLayout L, FeedAr
     for (int i = 0; i < L.n_inputs; i++) valid_bits |= 1ull << (34 + i);
     uint64_t w[SU][WM];
     uint32_t bk[SU];
+    bool nan_or_zero = false;
     auto rows_at = [&](int64_t t) { return r1 - t < TILE ? (int)(r1 - t) : TILE; };
-    if (r0 < r1) gbp_load1<WM, SU>(L, A, G, ins, r0, rows_at(r0), valid_bits, w, bk);
+    if (r0 < r1) gbp_load1<WM, SU>(L, A, G, ins, r0, rows_at(r0), valid_bits, w, bk, nan_or_zero);
     for (int64_t t0 = r0; t0 < r1; t0 += TILE) {
         const int n = rows_at(t0);
         gbp_stage_place<WM, SU>(S, W, NB, n, w, bk);
-        if (t0 + TILE < r1) gbp_load1<WM, SU>(L, A, G, ins, t0 + TILE, rows_at(t0 + TILE), valid_bits, w, bk);
+        if (t0 + TILE < r1)
+            gbp_load1<WM, SU>(L, A, G, ins, t0 + TILE, rows_at(t0 + TILE), valid_bits, w, bk, nan_or_zero);
         gbp_stage_write<WM>(S, W, NB, n, G.S1);
     }
+    if (nan_or_zero) atomicOr((unsigned long long *)&G.chk[2], 1ull);
 }
 
 // staged rows of the second pass's tile at t0 (n rows) with their sub-buckets
@@ -1780,7 +1803,7 @@ __device__ __forceinline__ void gbp_load2(const GbpArgs &G, int64_t t0, int n, u
 #pragma unroll
         for (int q = 0; q < WM; q++)
             if (q == W - 1) meta = w[u][q];
-        const int cls = (int)((meta >> 32) & 3);
+        const int cls = G.nometa ? gbp_plain_class(w[u][0]) : (int)((meta >> 32) & 3);
         bk[u] = (uint32_t)((gbp_pkey(G, cls, w[u][0]) >> (64 - G.B1 - G.B2)) & ((1u << G.B2) - 1));
     }
 }
@@ -2112,12 +2135,12 @@ __device__ __forceinline__ void gbp_agg_row(const GbLayout &L, const GbpArgs &G,
 // gbp_apply_staged / gbp_agg_row for rows of a width known at compile time
 // (W = inputs + 2): the row's words are indexed by constants only, so they
 // stay in registers
-template <int W>
+template <int W, bool META>
 __device__ __forceinline__ void gbp_apply_w(const GbLayout &L, uint64_t *h, uint64_t *c, const uint64_t (&w)[W],
                                             uint64_t meta, uint64_t pos) {
     gb_update_row(L, h, pos);
 #pragma unroll
-    for (int i = 0; i < W - 2; i++) {
+    for (int i = 0; i < (META ? W - 2 : W - 1); i++) {
         InVal v;
         v.valid = (meta >> (34 + i)) & 1;
         v.bits = w[1 + i];
@@ -2125,20 +2148,25 @@ __device__ __forceinline__ void gbp_apply_w(const GbLayout &L, uint64_t *h, uint
     }
 }
 
-template <int W>
+// META false: rows of key and inputs only (GbpArgs::nometa): the key class
+// from the key, every input valid, position 0
+template <int W, bool META>
 __device__ __forceinline__ void gbp_agg_row_w(const GbLayout &L, const GbpArgs &G, uint64_t *lhot, uint64_t *lcold,
                                               int S, uint64_t empty, const uint64_t (&w)[W], bool &spilled) {
-    const uint64_t kn = w[0], meta = w[W - 1];
+    constexpr int NI = META ? W - 2 : W - 1;
+    const uint64_t kn = w[0];
+    const uint64_t meta = META ? w[W - 1]
+                               : ((uint64_t)gbp_plain_class(kn) << 32) | ((((uint64_t)1 << NI) - 1) << 34);
     const int cls = (int)((meta >> 32) & 3);
     const uint64_t pos = (uint64_t)G.pos_base + (uint32_t)meta;
     const int64_t ls = gbp_lds_slot(lhot, S, L.RH, cls, kn, empty);
     // one expansion per address space: ds_* atomics for the LDS record
     if (ls >= 0) {
-        gbp_apply_w<W>(L, lhot + (size_t)ls * L.RH, lcold + (size_t)ls * L.RC, w, meta, pos);
+        gbp_apply_w<W, META>(L, lhot + (size_t)ls * L.RH, lcold + (size_t)ls * L.RC, w, meta, pos);
     } else {
         const int64_t gs = gb_global_slot(L, G.hot, G.cap, cls, kn, pos, G.ctl);
         spilled = true;
-        if (gs >= 0) gbp_apply_w<W>(L, G.hot + (size_t)gs * L.RH, G.cold + (size_t)gs * L.RC, w, meta, pos);
+        if (gs >= 0) gbp_apply_w<W, META>(L, G.hot + (size_t)gs * L.RH, G.cold + (size_t)gs * L.RC, w, meta, pos);
     }
 }
 
@@ -2343,7 +2371,7 @@ __device__ __forceinline__ void gbp_load_rows(const uint64_t *S2, int64_t r, int
 // current records are sorted and written.  The barriers of that phase
 // wait for LDS only, so those loads stay in flight across them.  Same
 // records, flags and lists as k_gbp_agg.
-template <int W, int U>
+template <int W, int U, bool META = true>
 __global__ void __launch_bounds__(GB_THREADS) k_gbp_agg_pf(GbLayout L, GbpArgs G, const uint4 *__restrict__ idesc) {
     extern __shared__ uint64_t dynw[];   // 8-byte aligned after the static arrays
     __shared__ uint16_t slist[1024];   // [0, 512) occupied slots, [512, 1024) in key order
@@ -2373,13 +2401,13 @@ __global__ void __launch_bounds__(GB_THREADS) k_gbp_agg_pf(GbLayout L, GbpArgs G
 #pragma unroll
         for (int u = 0; u < U; u++)
             if (r0 + threadIdx.x + (int64_t)u * GB_THREADS < r1)
-                gbp_agg_row_w<W>(L, G, lhot, lcold, S, empty, v[u], spilled);
+                gbp_agg_row_w<W, META>(L, G, lhot, lcold, S, empty, v[u], spilled);
         for (int64_t r = r0 + threadIdx.x + (int64_t)U * GB_THREADS; r < r1; r += (int64_t)U * GB_THREADS) {
             uint64_t w[U][W];
             gbp_load_rows<W, U>(G.S2, r, r1, w);
 #pragma unroll
             for (int u = 0; u < U; u++)
-                if (r + (int64_t)u * GB_THREADS < r1) gbp_agg_row_w<W>(L, G, lhot, lcold, S, empty, w[u], spilled);
+                if (r + (int64_t)u * GB_THREADS < r1) gbp_agg_row_w<W, META>(L, G, lhot, lcold, S, empty, w[u], spilled);
         }
         if (more) gbp_load_rows<W, U>(G.S2, (int64_t)nxt.x + threadIdx.x, nxt.y, v);
         __syncthreads();
@@ -2587,7 +2615,7 @@ static void gbp_layout(GbpArgs &G, uint8_t *base, size_t *bytes) {
     G.irec = c.take<uint64_t>(G.NImax);
     G.rflags = c.take<uint32_t>(G.NImax);
     G.mlist = c.take<uint32_t>(ML);
-    G.chk = c.take<uint64_t>(2);
+    G.chk = c.take<uint64_t>(3);
     *bytes = c.off;
 }
 
@@ -2662,23 +2690,31 @@ static int gbp_feed(vgpu_groupby *g, FeedArgs A, const InSet &ins, const PredInf
         plain = plain && !ins.c[i].validity && (ins.c[i].type == VGPU_INT64 || ins.c[i].type == VGPU_DOUBLE);
     auto hist1 = plain ? k_gbp_hist1<true> : k_gbp_hist1<false>;
     auto scatter1 = plain ? k_gbp_scatter1<true> : k_gbp_scatter1<false>;
-    // staged scatters (knob gb.part.stage): rows of 3, 4 or up to
+    // staged scatters (knob gb.part.stage): rows of 2 to 4 or up to
     // VGPU_MAX_INPUTS + 2 words
-    int64_t stage_mode = vg_tune_for(VG_TUNE_GB_PART_STAGE, groups);
-    // 2,048-row tiles of wide rows would not fit a CU's LDS: 1,024-row ones
-    if (stage_mode >= 3 && gbp_stage_bytes(G.W, 1 << std::max(G.B1, G.B2), 8 * GB_THREADS) > 96 * 1024) stage_mode = 2;
-    const bool staged = stage_mode >= 2;
-    const int tile = stage_mode >= 3 ? 8 * GB_THREADS : 4 * GB_THREADS;
-    auto scatter1s = stage_mode >= 3 ? (G.W == 4 ? k_gbp_scatter1s<4, 8> : G.W <= 3 ? k_gbp_scatter1s<3, 8>
-                                                                                   : k_gbp_scatter1s<VGPU_MAX_INPUTS + 2, 8>)
-                                     : (G.W == 4 ? k_gbp_scatter1s<4, 4> : G.W <= 3 ? k_gbp_scatter1s<3, 4>
-                                                                                   : k_gbp_scatter1s<VGPU_MAX_INPUTS + 2, 4>);
-    auto scatter2s = stage_mode >= 3 ? (G.W == 4 ? k_gbp_scatter2s<4, 8> : G.W <= 3 ? k_gbp_scatter2s<3, 8>
-                                                                                   : k_gbp_scatter2s<VGPU_MAX_INPUTS + 2, 8>)
-                                     : (G.W == 4 ? k_gbp_scatter2s<4, 4> : G.W <= 3 ? k_gbp_scatter2s<3, 4>
-                                                                                   : k_gbp_scatter2s<VGPU_MAX_INPUTS + 2, 4>);
+    const int64_t stage_knob = vg_tune_for(VG_TUNE_GB_PART_STAGE, groups);
+    // rows without the meta word (knob gb.part.nometa): a plain feed in key
+    // order of one or two inputs, staged, aggregated by k_gbp_agg_pf
+    G.nometa = plain && stage_knob >= 2 && g->order == VGPU_ORDER_KEY && L.n_inputs >= 1 && L.n_inputs <= 2 &&
+               vg_tune_for(VG_TUNE_GB_PART_AGG, groups) >= 3 && vg_tune_for(VG_TUNE_GB_PART_NOMETA, groups) >= 2;
+    bool staged = false;
     vg_kt_begin();
     for (;;) {
+        G.W = L.n_inputs + (G.nometa ? 1 : 2);
+        int64_t stage_mode = stage_knob;
+        // 2,048-row tiles of wide rows would not fit a CU's LDS: 1,024-row ones
+        if (stage_mode >= 3 && gbp_stage_bytes(G.W, 1 << std::max(G.B1, G.B2), 8 * GB_THREADS) > 96 * 1024)
+            stage_mode = 2;
+        staged = stage_mode >= 2;
+        const int tile = stage_mode >= 3 ? 8 * GB_THREADS : 4 * GB_THREADS;
+        auto scatter1s = stage_mode >= 3 ? (G.W == 4 ? k_gbp_scatter1s<4, 8> : G.W <= 3 ? k_gbp_scatter1s<3, 8>
+                                                                                       : k_gbp_scatter1s<VGPU_MAX_INPUTS + 2, 8>)
+                                         : (G.W == 4 ? k_gbp_scatter1s<4, 4> : G.W <= 3 ? k_gbp_scatter1s<3, 4>
+                                                                                       : k_gbp_scatter1s<VGPU_MAX_INPUTS + 2, 4>);
+        auto scatter2s = stage_mode >= 3 ? (G.W == 4 ? k_gbp_scatter2s<4, 8> : G.W <= 3 ? k_gbp_scatter2s<3, 8>
+                                                                                       : k_gbp_scatter2s<VGPU_MAX_INPUTS + 2, 8>)
+                                         : (G.W == 4 ? k_gbp_scatter2s<4, 4> : G.W <= 3 ? k_gbp_scatter2s<3, 4>
+                                                                                       : k_gbp_scatter2s<VGPU_MAX_INPUTS + 2, 4>);
         hist1<<<G.G1, GB_THREADS, lds1, st>>>(A, G, pi, prog, pcs);
         VG_LAUNCH_CHECK();
         if (vg_scan_u32_to_u64(G.H1, G.O1, NB * G.G1, G.O1 + NB * G.G1, g->sort_tmp)) return -1;
@@ -2712,11 +2748,15 @@ static int gbp_feed(vgpu_groupby *g, FeedArgs A, const InSet &ins, const PredInf
         VG_LAUNCH_CHECK();
         k_gbp_nitems<<<vg_blocks(G.P, 256), 256, 0, st>>>(G);
         VG_LAUNCH_CHECK();
-        if (G.range) {
-            uint64_t big = 0;
-            VG_TRY(hipMemcpyAsync(&big, G.chk + 1, 8, hipMemcpyDeviceToHost, st));
+        if (G.range || G.nometa) {
+            uint64_t chk[2] = {0, 0};
+            VG_TRY(hipMemcpyAsync(chk, G.chk + 1, 16, hipMemcpyDeviceToHost, st));
             VG_TRY(hipStreamSynchronize(st));
-            if (big > item_limit) {
+            if (G.nometa && chk[1]) {   // a NaN or a zero that needs positions: rows with meta words
+                G.nometa = 0;
+                continue;
+            }
+            if (G.range && chk[0] > item_limit) {
                 G.range = 0;
                 continue;
             }
@@ -2738,7 +2778,8 @@ static int gbp_feed(vgpu_groupby *g, FeedArgs A, const InSet &ins, const PredInf
         // knob gb.part.agg: 3 = fixed-stride items with the next item's rows
         // loaded ahead (rows of 3 to 6 words), 2 = items taken by ticket
         const int64_t agg = vg_tune_for(VG_TUNE_GB_PART_AGG, groups);
-        auto pf = G.W == 3 ? k_gbp_agg_pf<3, 4> : G.W == 4 ? k_gbp_agg_pf<4, 4>
+        auto pf = G.nometa ? (G.W == 2 ? k_gbp_agg_pf<2, 4, false> : G.W == 3 ? k_gbp_agg_pf<3, 4, false> : nullptr)
+                : G.W == 3 ? k_gbp_agg_pf<3, 4> : G.W == 4 ? k_gbp_agg_pf<4, 4>
                 : G.W == 5 ? k_gbp_agg_pf<5, 2> : G.W == 6 ? k_gbp_agg_pf<6, 2> : nullptr;
         if (agg >= 3 && pf) pf<<<(unsigned)grid, GB_THREADS, lds, st>>>(L, G, G.idesc);
         else k_gbp_agg_loop<<<(unsigned)grid, GB_THREADS, lds, st>>>(L, G);

@@ -41,7 +41,7 @@ void vg_kt_end();
 // launch-shape knobs (see vgpu_tune_set)
 enum { VG_TUNE_GB_LDS_MAX_BLOCKS, VG_TUNE_GB_LDS_ROWS, VG_TUNE_GB_GLB_MAX_BLOCKS, VG_TUNE_GB_GLB_ROWS,
        VG_TUNE_GB_LDS_MAX_SLOTS, VG_TUNE_GB_LDS_BUDGET, VG_TUNE_GB_STAR_ITEMS, VG_TUNE_GB_PART_MIN_GROUPS,
-       VG_TUNE_BT_SHARED, VG_TUNE_GB_LDS_SLOTS_PER_GROUP, VG_TUNE_BT_BUCKETED, VG_TUNE_GB_STAR_DENSE, VG_TUNE_JOIN_BLOOM, VG_TUNE_GB_LDS_COPIES, VG_TUNE_GB_NARROW, VG_TUNE_BT_DENSE, VG_TUNE_SCAN_FEED, VG_TUNE_JOIN_BLOOM_KPW, VG_TUNE_JOIN_LOAD_PCT, VG_TUNE_GB_FUSE_JOIN, VG_TUNE_SD_HASH_BITS, VG_TUNE_GB_GLB_LOAD_PCT, VG_TUNE_JOIN_DENSE, VG_TUNE_GB_STAR_BPC, VG_TUNE_GB_PART_RANGE, VG_TUNE_GB_PART_STAGE, VG_TUNE_GB_PART_SLOTS, VG_TUNE_GB_FEED_PF, VG_TUNE_GB_MERGE_PARTIALS, VG_TUNE_GB_PART_AGG, VG_TUNE_GB_PART_FILL, VG_TUNE_GB_FEED_DENSE, VG_TUNE_COUNT };
+       VG_TUNE_BT_SHARED, VG_TUNE_GB_LDS_SLOTS_PER_GROUP, VG_TUNE_BT_BUCKETED, VG_TUNE_GB_STAR_DENSE, VG_TUNE_JOIN_BLOOM, VG_TUNE_GB_LDS_COPIES, VG_TUNE_GB_NARROW, VG_TUNE_BT_DENSE, VG_TUNE_SCAN_FEED, VG_TUNE_JOIN_BLOOM_KPW, VG_TUNE_JOIN_LOAD_PCT, VG_TUNE_GB_FUSE_JOIN, VG_TUNE_SD_HASH_BITS, VG_TUNE_GB_GLB_LOAD_PCT, VG_TUNE_JOIN_DENSE, VG_TUNE_GB_STAR_BPC, VG_TUNE_GB_PART_RANGE, VG_TUNE_GB_PART_STAGE, VG_TUNE_GB_PART_SLOTS, VG_TUNE_GB_FEED_PF, VG_TUNE_GB_MERGE_PARTIALS, VG_TUNE_GB_PART_AGG, VG_TUNE_GB_PART_FILL, VG_TUNE_GB_FEED_DENSE, VG_TUNE_GB_PART_NOMETA, VG_TUNE_COUNT };
 int64_t vg_tune(int knob);
 // the knob for a group-by expecting `groups` groups (tuning table buckets;
 // the bucket's other two axes come from the innermost VgTuneScope)

@@ -433,8 +433,8 @@ static const char *g_tune_names[VG_TUNE_COUNT] = {"gb.lds.max_blocks", "gb.lds.r
                                                   "strdict.hash_bits", "gb.glb.load_pct",
                                                   "join.dense_map", "gb.star.blocks_per_cu", "gb.part.range",
                                                   "gb.part.stage", "gb.part.slots", "gb.feed.prefetch",
-                                                  "gb.merge.partials", "gb.part.agg", "gb.part.fill_pct", "gb.feed.dense"};
-#define VG_TUNE_DEFAULTS {1024, 8192, 16384, 1024, 4096, 65536, 4, 65536, 1, 2, 2, 2, 1, 2, 2, 2, 2, 4, 50, 2, 64, 50, 2, 4, 2, 3, 512, 2, 2, 3, 50, 2}
+                                                  "gb.merge.partials", "gb.part.agg", "gb.part.fill_pct", "gb.feed.dense", "gb.part.nometa"};
+#define VG_TUNE_DEFAULTS {1024, 8192, 16384, 1024, 4096, 65536, 4, 65536, 1, 2, 2, 2, 1, 2, 2, 2, 2, 4, 50, 2, 64, 50, 2, 4, 2, 3, 512, 2, 2, 3, 50, 2, 2}
 static int64_t g_tune[VG_TUNE_COUNT] = VG_TUNE_DEFAULTS;
 
 // A bucket covers groups <= max_groups, record widths <= max_width (8-byte
