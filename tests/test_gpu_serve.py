@@ -129,3 +129,26 @@ def test_serve_hbm_budget_admission(gpu, oracle):
         gpu.serve(fact, dim, reqs[:1], batched=True, hbm_budget=1 << 20)
     for c in list(fact.values()) + list(dim.values()):
         c.free()
+
+
+def test_serve_pass_size_per_class(gpu, oracle):
+    """a class whose bucketed scan keeps a table of groups x requests in LDS
+    takes passes only as large as that table allows (class 0 here: 301
+    groups, at most 44 requests), the other class the full max_batch; a
+    burst of 240 requests forms full passes of both, and every result equals
+    the plain one-at-a-time result and, for a sample, the oracle"""
+    host, fact, dim = make_tables(gpu, 67, 1_000_000, 10_000)
+    reqs = requests(71, 240, burst=240)
+    mixed, st = gpu.serve(fact, dim, reqs, batched=True, max_batch=64, in_flight=2)
+    alone, _ = gpu.serve(fact, dim, reqs, batched=False)
+    assert st["n_done"] == len(reqs)
+    b0 = max(m["batch"] for m, (c, _, _) in zip(mixed, reqs) if c == 0)
+    b1 = max(m["batch"] for m, (c, _, _) in zip(mixed, reqs) if c == 1)
+    assert 1 < b0 <= 44 < b1 <= 64, (b0, b1)
+    for m, a in zip(mixed, alone):
+        same(m, a)
+    for i in (0, 41, 239):
+        cls, t, _ = reqs[i]
+        same(mixed[i], oracle_request(gpu, host, cls, t))
+    for c in list(fact.values()) + list(dim.values()):
+        c.free()
