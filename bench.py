@@ -583,6 +583,17 @@ def run_c1_vtr(args, nv):
     }
 
 
+def c3_traffic(n, card):
+    """HBM bytes per feed from the committed rocprofv3 summary of the same
+    workload (FETCH_SIZE / WRITE_SIZE, scaled), or None off that workload"""
+    name = "k_gb_feed_groupby_200M_rows_16_keys.json" if card == 16 else \
+        "k_gbp_groupby_200M_rows_50M_keys.json" if card == 50_000_000 else None
+    tfile = ROOT / "profiles" / name if name else None
+    if n != 200_000_000 or tfile is None or not tfile.exists():
+        return None
+    return json.loads(tfile.read_text())["traffic_bytes_per_launch"]
+
+
 def run_c3(args, nv, card):
     n = args.c3_rows
     log(f"[bench] generating c3 data: {n:,} rows, {card:,} key values")
@@ -663,7 +674,7 @@ def run_c3(args, nv, card):
                    "rows": n, "key_values": card, "groups": present},
         "result_check": {"groups": present, "sum_v1_rel_err_vs_fsum": rel_err},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": c3_traffic(n, card),
                      "kernel": "k_gb_feed" if card <= 16 else
                      "partitioned feed k_gbp_hist1 -> k_gbp_agg (timed as one feed)",
                      "bytes_per_launch": per_launch, "avg_launch_ms": avg_ms},
