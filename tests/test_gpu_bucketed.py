@@ -323,19 +323,22 @@ def test_bucketed_sums_the_compared_column(gpu, oracle, join, plain, n):
         c.free()
 
 
-I64_T = [0, 5, 5, -300, 999, 2**63 - 1, -(2**63) + 1, 42, -7, 1, -1, 500, -500, 6, 4, 1000, -999]
-DBL_T = [0.0, -0.0, 0.5, 0.5, float("nan"), float("inf"), -float("inf"), -1.2, 1.7,
+I64_TK = [0, 5, 5, -300, 999, 2**63 - 1, -(2**63) + 1, 42, -7, 1, -1, 500, -500, 6, 4, 1000, -999]
+I64_TK += [(-1) ** i * 37 * i for i in range(23)]
+DBL_TK = [0.0, -0.0, 0.5, 0.5, float("nan"), float("inf"), -float("inf"), -1.2, 1.7,
          -0.5, 0.25, float("nan"), 3.0, -3.0, 0.0, 1e-300, -2.2]
+DBL_TK += [(-1) ** i * 0.13 * i for i in range(23)]
 
 
-@pytest.mark.parametrize("k", [8, 9, 13, 16, 17])
+@pytest.mark.parametrize("k", [8, 9, 13, 16, 17, 40])
 @pytest.mark.parametrize("dtype", ["double", "int64"])
 @pytest.mark.parametrize("op", list(OPS))
 def test_bucketed_plain_scan_every_operator(gpu, oracle, op, dtype, k):
     """the plain scan (no NAs in the compared column or the key, one sum of
     the compared column) with up to sixteen queries counts each row's passes
     against the constants in registers (8 compare slots up to eight queries,
-    16 up to sixteen); seventeen take the search: every
+    16 up to sixteen); seventeen and forty take a branch-free search of the
+    constants with the comparison fixed at compile time: every
     operator on double and int64 columns, NaN / +-0 / +-inf and extreme
     int64 constants (INT64_MAX, INT64_MIN + 1), duplicate constants,
     values equal to a constant"""
@@ -348,7 +351,7 @@ def test_bucketed_plain_scan_every_operator(gpu, oracle, op, dtype, k):
         p[u < 0.01] = np.inf
         p[(u >= 0.01) & (u < 0.02)] = -np.inf
         p[(u >= 0.02) & (u < 0.04)] = 0.5
-        ts = DBL_T[:k]
+        ts = DBL_TK[:k]
         ptype, lit = gpu.VEC_DOUBLE, "dbl"
     else:
         p = r.integers(-1000, 1000, n)
@@ -356,7 +359,7 @@ def test_bucketed_plain_scan_every_operator(gpu, oracle, op, dtype, k):
         p[u < 0.01] = 10**12                  # large values, no int64 sum overflows
         p[(u >= 0.01) & (u < 0.02)] = -10**12
         p[(u >= 0.02) & (u < 0.05)] = 5
-        ts = I64_T[:k]
+        ts = I64_TK[:k]
         ptype, lit = gpu.VEC_INT64, "int"
     pc, gc = gpu.DeviceColumn(p), gpu.DeviceColumn(g)
     pred = gpu.cmp_const_prog(0, ptype, OPS[op], 0.0 if dtype == "double" else 0)

@@ -707,6 +707,34 @@ __device__ __forceinline__ int bt_chain_T(const BtChainArgs &A, const uint64_t *
 // TM (T mode): 0 = the search above; 1 + 2*int64 + less: the count
 #define BT_TM_SEARCH 0
 __host__ __device__ constexpr int bt_tm(bool f64, bool greater) { return 1 + (f64 ? 0 : 2) + (greater ? 0 : 1); }
+// TM 5..8 (TM - 4 as above): more than BT_CH_LIN queries on the plain scan:
+// a branch-free binary search of the constants in LDS with the comparison
+// fixed at compile time (7 steps, up to 127 queries)
+#define BT_TM_CSEARCH 4
+
+template <int TM>
+__device__ __forceinline__ int bt_chain_T_csearch(const uint64_t *s_k, int K, uint64_t bits, bool neg) {
+    constexpr bool F64 = TM == 1 || TM == 2;
+    constexpr bool GREATER = TM == 1 || TM == 3;
+    // the queries passed form a prefix of the chain: its length, by steps of
+    // 64, 32, ..., 1 (a step past K never passes)
+    int lo = 0;
+#pragma unroll
+    for (int step = 64; step >= 1; step >>= 1) {
+        const int at = lo + step;
+        const int ix = (at <= K ? at : K) - 1;   // the constant the step tests (K >= 1 whenever rows pass)
+        const uint64_t t = s_k[ix > 0 ? ix : 0];
+        bool base;
+        if (F64) {
+            const double v = vg_bits_to_f64(bits), c = vg_bits_to_f64(t);
+            base = GREATER ? v > c : v < c;
+        } else {
+            base = GREATER ? (int64_t)bits > (int64_t)t : (int64_t)bits < (int64_t)t;
+        }
+        if (at <= K && base != neg) lo = at;
+    }
+    return lo;
+}
 
 template <int TM, int NL>
 __device__ __forceinline__ int bt_chain_T_lin(const uint64_t (&kr)[BT_CH_LIN], uint64_t bits, int K, bool neg) {
@@ -840,6 +868,8 @@ __device__ __forceinline__ void bt_chain_rows_add(const BtChainArgs &A, const vg
 #pragma unroll
     for (int u = 0; u < U; u++) {
         if (TM == BT_TM_SEARCH) T[u] = (R.ok >> u) & 1 ? bt_chain_T(A, s_k, R.pv[u]) : 0;
+        else if (TM > BT_TM_CSEARCH)
+            T[u] = (R.ok >> u) & 1 ? bt_chain_T_csearch<TM - BT_TM_CSEARCH>(s_k, A.K, R.pv[u], neg) : 0;
         else T[u] = (R.ok >> u) & 1 ? bt_chain_T_lin<TM, NL>(kr, R.pv[u], A.K, neg) : 0;
 #pragma unroll
         for (int s = 0; s < BT_CH_MAX_SUMS; s++) sv[u][s] = SP ? R.pv[u] : R.sv[SP ? 0 : u][s];
@@ -857,7 +887,7 @@ __device__ __forceinline__ void bt_chain_rows_add(const BtChainArgs &A, const vg
             const int64_t e = c == JD_NA_PAYLOAD ? A.E - 1 : bt_chain_entry(A, true, (uint64_t)(A.jd.pmin + (int64_t)c));
             if (e < 0) err = true;
             else if (TM == BT_TM_SEARCH) bt_chain_add(A, cnt, sums, e, T[u], sv[u]);
-            else bt_chain_add_one<TM == 1 || TM == 2>(A, cnt, sums, e, T[u], sv[u][0]);
+            else bt_chain_add_one<(TM > BT_TM_CSEARCH ? TM - BT_TM_CSEARCH : TM) == 1 || (TM > BT_TM_CSEARCH ? TM - BT_TM_CSEARCH : TM) == 2>(A, cnt, sums, e, T[u], sv[u][0]);
         }
     } else if (A.jn.on) {
         const uint64_t *ent[U];
@@ -891,7 +921,7 @@ __device__ __forceinline__ void bt_chain_rows_add(const BtChainArgs &A, const vg
                 const int64_t e = bt_chain_entry(A, v, bits);
                 if (e < 0) err = true;
                 else if (TM == BT_TM_SEARCH) bt_chain_add(A, cnt, sums, e, T[u], sv[u]);
-                else bt_chain_add_one<TM == 1 || TM == 2>(A, cnt, sums, e, T[u], sv[u][0]);
+                else bt_chain_add_one<(TM > BT_TM_CSEARCH ? TM - BT_TM_CSEARCH : TM) == 1 || (TM > BT_TM_CSEARCH ? TM - BT_TM_CSEARCH : TM) == 2>(A, cnt, sums, e, T[u], sv[u][0]);
             }
         }
     } else {
@@ -901,7 +931,7 @@ __device__ __forceinline__ void bt_chain_rows_add(const BtChainArgs &A, const vg
             const int64_t e = bt_chain_entry(A, (R.kv >> u) & 1, R.kb[u]);
             if (e < 0) err = true;
             else if (TM == BT_TM_SEARCH) bt_chain_add(A, cnt, sums, e, T[u], sv[u]);
-            else bt_chain_add_one<TM == 1 || TM == 2>(A, cnt, sums, e, T[u], sv[u][0]);
+            else bt_chain_add_one<(TM > BT_TM_CSEARCH ? TM - BT_TM_CSEARCH : TM) == 1 || (TM > BT_TM_CSEARCH ? TM - BT_TM_CSEARCH : TM) == 2>(A, cnt, sums, e, T[u], sv[u][0]);
         }
     }
 }
@@ -918,7 +948,7 @@ __global__ void __launch_bounds__(BT_CH_THREADS) k_bt_chain_feed(BtChainArgs A, 
     static_assert(TM == BT_TM_SEARCH || PLAIN, "the linear count runs on the plain scan");
     constexpr int U = SP ? BT_CH_U : BT_CH_U / 2;
     uint64_t kr[BT_CH_LIN];
-    if (TM != BT_TM_SEARCH) bt_chain_load_lin<TM, NL>(A, kr);
+    if (TM != BT_TM_SEARCH && TM <= BT_TM_CSEARCH) bt_chain_load_lin<TM, NL>(A, kr);
     extern __shared__ uint64_t dynw[];
     uint64_t *s_k = dynw;                                   // m constants
     uint64_t *sums = dynw + A.m;                            // E*m*n_sum
@@ -1202,6 +1232,10 @@ static int bt_chain_plan(vgpu_batch *b, const BtQuery &Q0) {
                              reinterpret_cast<const void *>(k_bt_chain_feed<true, true, 2, 16>),
                              reinterpret_cast<const void *>(k_bt_chain_feed<true, true, 3, 16>),
                              reinterpret_cast<const void *>(k_bt_chain_feed<true, true, 4, 16>),
+                             reinterpret_cast<const void *>(k_bt_chain_feed<true, true, 5>),
+                             reinterpret_cast<const void *>(k_bt_chain_feed<true, true, 6>),
+                             reinterpret_cast<const void *>(k_bt_chain_feed<true, true, 7>),
+                             reinterpret_cast<const void *>(k_bt_chain_feed<true, true, 8>),
                              reinterpret_cast<const void *>(k_bt_chain_feed<true, true, BT_TM_SEARCH>),
                              reinterpret_cast<const void *>(k_bt_chain_feed<true, false, BT_TM_SEARCH>),
                              reinterpret_cast<const void *>(k_bt_chain_feed<false, false, BT_TM_SEARCH>)};
@@ -1656,14 +1690,16 @@ static int bt_enqueue(vgpu_batch *b, bool first) {
                            kc.type != VGPU_BOOL;
         const int op = C.pi.cmp_op;
         const bool same_type = (C.pi.cmp_type == VGPU_DOUBLE) == (C.pcol.type == VGPU_DOUBLE);
-        const int tm = plain && same_type && C.n_sum == 1 && C.K <= BT_CH_LIN
-                           ? bt_tm(C.pi.cmp_type == VGPU_DOUBLE, op == VGPU_OP_GT || op == VGPU_OP_LE)
-                           : BT_TM_SEARCH;
+        const int tm0 = bt_tm(C.pi.cmp_type == VGPU_DOUBLE, op == VGPU_OP_GT || op == VGPU_OP_LE);
+        const int tm = !(plain && same_type && C.n_sum == 1) ? BT_TM_SEARCH
+                       : C.K <= BT_CH_LIN                    ? tm0
+                       : C.K <= 127                          ? tm0 + BT_TM_CSEARCH
+                                                             : BT_TM_SEARCH;
         const int neg = op == VGPU_OP_GE || op == VGPU_OP_LE;
         const dim3 g((unsigned)C.G), t(BT_CH_THREADS);
         const size_t lds = b->ch_lds;
         // the count over 8 compare slots for up to 8 queries, 16 for up to 16
-        switch (plain ? (tm != BT_TM_SEARCH && C.K > 8 ? tm + 16 : tm) : -1) {
+        switch (plain ? (tm != BT_TM_SEARCH && tm <= BT_TM_CSEARCH && C.K > 8 ? tm + 16 : tm) : -1) {
         case 1: k_bt_chain_feed<true, true, 1><<<g, t, lds, st>>>(C, neg); break;
         case 2: k_bt_chain_feed<true, true, 2><<<g, t, lds, st>>>(C, neg); break;
         case 3: k_bt_chain_feed<true, true, 3><<<g, t, lds, st>>>(C, neg); break;
@@ -1672,6 +1708,10 @@ static int bt_enqueue(vgpu_batch *b, bool first) {
         case 18: k_bt_chain_feed<true, true, 2, 16><<<g, t, lds, st>>>(C, neg); break;
         case 19: k_bt_chain_feed<true, true, 3, 16><<<g, t, lds, st>>>(C, neg); break;
         case 20: k_bt_chain_feed<true, true, 4, 16><<<g, t, lds, st>>>(C, neg); break;
+        case 5: k_bt_chain_feed<true, true, 5><<<g, t, lds, st>>>(C, neg); break;
+        case 6: k_bt_chain_feed<true, true, 6><<<g, t, lds, st>>>(C, neg); break;
+        case 7: k_bt_chain_feed<true, true, 7><<<g, t, lds, st>>>(C, neg); break;
+        case 8: k_bt_chain_feed<true, true, 8><<<g, t, lds, st>>>(C, neg); break;
         case BT_TM_SEARCH: k_bt_chain_feed<true, true, BT_TM_SEARCH><<<g, t, lds, st>>>(C, neg); break;
         default:
             if (sp) k_bt_chain_feed<true, false, BT_TM_SEARCH><<<g, t, lds, st>>>(C, neg);
